@@ -1,0 +1,11 @@
+"""gpumesh_amd -- MI355X-native vertex-relocation mesh smoothing.
+
+Hand-written HIP kernels for gfx950 behind a C ABI (include/gpumesh.h,
+libgpumesh_hip.so); this package is the Python binding plus the synthetic
+cube-mesh generator used by the tests and the benchmark.
+"""
+from ._lib import GpuMeshError, device_count, lib, math_probe  # noqa: F401
+from .mesh import GpuMesh  # noqa: F401
+from .meshgen import CubeMesh, cube_mesh  # noqa: F401
+
+__all__ = ["GpuMesh", "GpuMeshError", "CubeMesh", "cube_mesh", "device_count", "lib", "math_probe"]

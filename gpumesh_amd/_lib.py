@@ -1,0 +1,92 @@
+"""ctypes loader of libgpumesh_hip.so (the HIP smoothing library).
+
+The library is built in-tree (gpumesh_amd/libgpumesh_hip.so) by
+`__graft_entry__.build()` / `make -C gpumesh_amd/csrc`.  There is no fallback:
+if the library is missing, importing the binding raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libgpumesh_hip.so")
+_LIB = None
+
+GM_OK = 0
+STATUS = {0: "GM_OK", 1: "GM_ERR_ARG", 2: "GM_ERR_HIP", 3: "GM_ERR_UNSUPPORTED", 4: "GM_ERR_NOMEM"}
+
+# Every symbol the C ABI exports (include/gpumesh.h).
+EXPORTS = [
+    "gm_last_error", "gm_version", "gm_device_count", "gm_mesh_create", "gm_mesh_create_host",
+    "gm_mesh_destroy", "gm_mesh_info", "gm_mesh_permutation", "gm_mesh_groups", "gm_mesh_elements",
+    "gm_mesh_topology", "gm_mesh_get_positions", "gm_mesh_set_positions", "gm_mesh_set_crew",
+    "gm_mesh_evaluate", "gm_mesh_smooth", "gm_mesh_last_timing", "gm_mesh_kernel_times", "gm_math_probe",
+]
+
+
+class GpuMeshError(RuntimeError):
+    """A C-ABI call returned an error status."""
+
+    def __init__(self, status: int, where: str, message: str):
+        super().__init__(f"{where}: {STATUS.get(status, status)}: {message}")
+        self.status = status
+
+
+def lib():
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is not built; run __graft_entry__.build() or make -C gpumesh_amd/csrc")
+    L = C.CDLL(LIB_PATH)
+    vp, i32, i64, dbl = C.c_void_p, C.c_int32, C.c_int64, C.c_double
+    L.gm_last_error.restype = C.c_char_p
+    L.gm_version.restype = C.c_char_p
+    L.gm_device_count.argtypes = [C.POINTER(C.c_int)]
+    for name in ("gm_mesh_create", "gm_mesh_create_host"):
+        getattr(L, name).argtypes = [i64, vp, vp, i64, vp, i64, vp, C.POINTER(vp)]
+    L.gm_mesh_destroy.argtypes = [vp]
+    L.gm_mesh_info.argtypes = [vp, vp]
+    L.gm_mesh_permutation.argtypes = [vp, vp]
+    L.gm_mesh_groups.argtypes = [vp, vp, vp]
+    L.gm_mesh_elements.argtypes = [vp, vp, vp]
+    L.gm_mesh_topology.argtypes = [vp, vp, vp, vp, vp]
+    L.gm_mesh_get_positions.argtypes = [vp, vp]
+    L.gm_mesh_set_positions.argtypes = [vp, vp]
+    L.gm_mesh_set_crew.argtypes = [vp, C.c_int, C.c_int, dbl, vp, vp, vp, vp]
+    L.gm_mesh_evaluate.argtypes = [vp, vp, vp, vp]
+    L.gm_mesh_smooth.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, dbl, vp, vp, vp]
+    L.gm_mesh_last_timing.argtypes = [vp, vp]
+    L.gm_mesh_kernel_times.argtypes = [vp, vp, vp]
+    L.gm_math_probe.argtypes = [C.c_int, vp, vp, vp, i64]
+    for name in EXPORTS:
+        if name not in ("gm_last_error", "gm_version"):
+            getattr(L, name).restype = C.c_int
+    _LIB = L
+    return L
+
+
+def check(status: int, where: str) -> None:
+    if status != GM_OK:
+        raise GpuMeshError(status, where, lib().gm_last_error().decode())
+
+
+def ptr(a: np.ndarray | None):
+    return None if a is None else a.ctypes.data
+
+
+def device_count() -> int:
+    n = C.c_int(0)
+    check(lib().gm_device_count(C.byref(n)), "gm_device_count")
+    return int(n.value)
+
+
+def math_probe(op: int, a: np.ndarray, b: np.ndarray | None = None) -> np.ndarray:
+    a = np.ascontiguousarray(a, np.float64)
+    b = None if b is None else np.ascontiguousarray(b, np.float64)
+    out = np.empty_like(a)
+    check(lib().gm_math_probe(int(op), ptr(a), ptr(b), ptr(out), a.size), "gm_math_probe")
+    return out

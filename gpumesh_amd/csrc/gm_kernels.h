@@ -1,0 +1,622 @@
+// gm_kernels.h -- kernel bodies of the smoothing path, instantiated once per
+// precision by kernels_f64.hip (built without contraction: bit-comparable to
+// an FP64 CPU build of the same operation order) and kernels_f32.hip.
+#ifndef GM_KERNELS_H
+#define GM_KERNELS_H
+
+#include "gm_device.h"
+#include "gm_smoothers.h"
+
+namespace gmd
+{
+using gmq::v3;
+using gmq::mk;
+
+template<typename T> struct PosOf;
+template<> struct PosOf<double>
+{
+    typedef double4 type;
+    static __device__ __forceinline__ double4* get(const DevMesh& m) { return m.pos64; }
+    static double4* host(const DevMesh& m) { return m.pos64; }
+};
+template<> struct PosOf<float>
+{
+    typedef float4 type;
+    static __device__ __forceinline__ float4* get(const DevMesh& m) { return m.pos32; }
+    static float4* host(const DevMesh& m) { return m.pos32; }
+};
+
+template<typename T, typename P> __device__ __forceinline__ v3<T> ld(const P* pos, uint32_t v)
+{
+    P q = pos[v];
+    return mk<T>((T)q.x, (T)q.y, (T)q.z);
+}
+
+template<typename P, typename T> __device__ __forceinline__ void st(P* pos, uint32_t v, v3<T> p)
+{
+    P q;
+    q.x = p.x; q.y = p.y; q.z = p.z; q.w = 0;
+    pos[v] = q;
+}
+
+// AbstractMeasurer::computeLocalElementSize (Measurers/AbstractMeasurer.cpp:137-155).
+template<typename T, typename P> __device__ T localSize(const DevMesh& m, const P* pos, uint32_t v, v3<T> p)
+{
+    uint32_t b = m.noff[v], e = m.noff[v + 1];
+    T total = T(0);
+    for(uint32_t r = b; r < e; ++r)
+    {
+        v3<T> d = p - ld<T>(pos, m.nbr[r]);
+        total += gmq::tsqrt(gmq::dot(d, d));
+    }
+    return total / (T)(e - b);
+}
+
+// Quality of the patch element `ref` with the patch's centre vertex at p.
+template<typename T, int EV, int SA, typename P>
+__device__ __forceinline__ T elementQualityAt(const DevMesh& m, const gmq::Crew& c, const P* pos, uint32_t ref, v3<T> p)
+{
+    uint32_t id = ref >> 4;
+    int k = (int)(ref & 7);
+    if(!(ref & 8))
+    {
+        uint4 t = m.tets[id];
+        v3<T> vp[4];
+        vp[0] = k == 0 ? p : ld<T>(pos, t.x);
+        vp[1] = k == 1 ? p : ld<T>(pos, t.y);
+        vp[2] = k == 2 ? p : ld<T>(pos, t.z);
+        vp[3] = k == 3 ? p : ld<T>(pos, t.w);
+        return gmq::tetQuality<T, EV, SA>(c, vp);
+    }
+    uint4 a = m.hexs[2 * id], b = m.hexs[2 * id + 1];
+    v3<T> vp[8];
+    vp[0] = k == 0 ? p : ld<T>(pos, a.x);
+    vp[1] = k == 1 ? p : ld<T>(pos, a.y);
+    vp[2] = k == 2 ? p : ld<T>(pos, a.z);
+    vp[3] = k == 3 ? p : ld<T>(pos, a.w);
+    vp[4] = k == 4 ? p : ld<T>(pos, b.x);
+    vp[5] = k == 5 ? p : ld<T>(pos, b.y);
+    vp[6] = k == 6 ? p : ld<T>(pos, b.z);
+    vp[7] = k == 7 ? p : ld<T>(pos, b.w);
+    return gmq::hexQuality<T, EV, SA>(c, vp);
+}
+
+// AbstractEvaluator::patchQuality (Evaluators/AbstractEvaluator.cpp:236-279),
+// one lane walking the whole patch.
+template<typename T, int EV, int SA>
+struct PlainPatch
+{
+    const DevMesh& m;
+    const gmq::Crew& c;
+    const typename PosOf<T>::type* pos;
+    uint32_t v;
+    int count;
+
+    __device__ T operator()(v3<T> p)
+    {
+        ++count;
+        T pq = T(0), pw = T(0);
+        uint32_t b = m.eoff[v], e = m.eoff[v + 1];
+        for(uint32_t r = b; r < e; ++r)
+            gmq::accumulatePatch<T>(pq, pw, elementQualityAt<T, EV, SA>(m, c, pos, m.eref[r], p));
+        return gmq::finalizePatch<T>(pq, pw);
+    }
+};
+
+// Plain kernel: one vertex of the colour group per lane.
+template<typename T, int EV, int SA, int SM>
+__global__ void __launch_bounds__(256) smoothPlainKernel(DevMesh m, gmq::Crew c, const uint32_t* __restrict__ nodes,
+                                                         uint32_t count, T moveCoeff, int32_t* evalCounts)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if(i >= count)
+        return;
+    typedef typename PosOf<T>::type P;
+    P* pos = PosOf<T>::get(m);
+    uint32_t v = nodes[i];
+    v3<T> p0 = ld<T>(pos, v);
+    T ls = localSize<T>(m, pos, v, p0);
+    PlainPatch<T, EV, SA> f{m, c, pos, v, 0};
+    v3<T> np;
+    if(SM == SM_NELDER_MEAD) np = gms::nelderMead<T>(p0, ls, f);
+    else if(SM == SM_GRADIENT_DESCENT) np = gms::gradientDescent<T>(p0, ls, f);
+    else np = gms::spawnSearch<T>(p0, ls, moveCoeff, f);
+    st(pos, v, np);
+    if(evalCounts)
+        evalCounts[v] += f.count;
+}
+
+// ---------------------------------------------------------------------------
+// Wavefront kernel: one vertex per group of G lanes (G a power of two >= the
+// patch size), lane j of the group holding patch element j's other vertices
+// in registers.  A patch evaluation computes the G element qualities in
+// parallel and reduces them in patch order through LDS, so every lane of the
+// group obtains the bit-identical sequential result and runs the optimiser
+// in lock-step.  64 / G vertices share a wavefront; a block is one wavefront.
+// ---------------------------------------------------------------------------
+template<typename T, int EV, int SA, int G>
+struct WavePatch
+{
+    const gmq::Crew* c;
+    T* red;           // LDS: 2 values per lane of this group (quality, 1/quality)
+    int lane;         // lane within the group
+    int nElem;        // patch size (<= G)
+    int isHex;
+    int k;            // local index of the centre vertex in this lane's element
+    v3<T> vp[8];      // this lane's element (centre slot overwritten per call)
+    int count;
+
+    __device__ T operator()(v3<T> p)
+    {
+        ++count;
+        T q = T(1);
+        if(lane < nElem)
+        {
+            if(!isHex)
+            {
+                v3<T> e[4];
+#pragma unroll
+                for(int i = 0; i < 4; ++i) e[i] = (i == k) ? p : vp[i];
+                q = gmq::tetQuality<T, EV, SA>(*c, e);
+            }
+            else
+            {
+                v3<T> e[8];
+#pragma unroll
+                for(int i = 0; i < 8; ++i) e[i] = (i == k) ? p : vp[i];
+                q = gmq::hexQuality<T, EV, SA>(*c, e);
+            }
+        }
+        T inv = T(1) / q;
+        // LDS accesses of one wavefront complete in issue order, so the reads
+        // below observe this group's writes; the barriers keep the compiler
+        // from moving memory operations across the exchange.
+        red[2 * lane] = q;
+        red[2 * lane + 1] = inv;
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        __builtin_amdgcn_wave_barrier();
+        T pq = T(0), pw = T(0);
+        for(int j = 0; j < nElem; ++j)
+        {
+            T qj = red[2 * j];
+            if(pw != T(-1) && qj > T(0)) { pw += T(1); pq += red[2 * j + 1]; }
+            else { pw = T(-1); pq = gmq::tmin(pq, qj); }
+        }
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        __builtin_amdgcn_wave_barrier();
+        return gmq::finalizePatch<T>(pq, pw);
+    }
+};
+
+template<typename T, int EV, int SA, int SM, int G>
+__global__ void __launch_bounds__(64) smoothWaveKernel(DevMesh m, gmq::Crew c, const uint32_t* __restrict__ nodes,
+                                                       uint32_t count, T moveCoeff, int32_t* evalCounts)
+{
+    typedef typename PosOf<T>::type P;
+    __shared__ T red[2 * 64];
+    const int lane = threadIdx.x % G;
+    const int grp = threadIdx.x / G;
+    const uint32_t i = blockIdx.x * (64 / G) + grp;
+    // Groups past the end leave; groups never synchronise with each other.
+    if(i >= count)
+        return;
+    P* pos = PosOf<T>::get(m);
+    uint32_t v = nodes[i];
+    WavePatch<T, EV, SA, G> f;
+    f.c = &c;
+    f.red = red + 2 * G * grp;
+    f.lane = lane;
+    f.count = 0;
+    uint32_t eb = m.eoff[v], ee = m.eoff[v + 1];
+    f.nElem = (int)(ee - eb);
+    f.isHex = 0;
+    f.k = 0;
+    if(lane < f.nElem)
+    {
+        uint32_t ref = m.eref[eb + lane];
+        uint32_t id = ref >> 4;
+        f.k = (int)(ref & 7);
+        f.isHex = (ref >> 3) & 1;
+        if(!f.isHex)
+        {
+            uint4 t = m.tets[id];
+            f.vp[0] = ld<T>(pos, t.x); f.vp[1] = ld<T>(pos, t.y);
+            f.vp[2] = ld<T>(pos, t.z); f.vp[3] = ld<T>(pos, t.w);
+        }
+        else
+        {
+            uint4 a = m.hexs[2 * id], b = m.hexs[2 * id + 1];
+            f.vp[0] = ld<T>(pos, a.x); f.vp[1] = ld<T>(pos, a.y);
+            f.vp[2] = ld<T>(pos, a.z); f.vp[3] = ld<T>(pos, a.w);
+            f.vp[4] = ld<T>(pos, b.x); f.vp[5] = ld<T>(pos, b.y);
+            f.vp[6] = ld<T>(pos, b.z); f.vp[7] = ld<T>(pos, b.w);
+        }
+    }
+    v3<T> p0 = ld<T>(pos, v);
+    T ls = localSize<T>(m, pos, v, p0);
+    v3<T> np;
+    if(SM == SM_NELDER_MEAD) np = gms::nelderMead<T>(p0, ls, f);
+    else if(SM == SM_GRADIENT_DESCENT) np = gms::gradientDescent<T>(p0, ls, f);
+    else np = gms::spawnSearch<T>(p0, ls, moveCoeff, f);
+    if(lane == 0)
+    {
+        st(pos, v, np);
+        if(evalCounts)
+            evalCounts[v] += f.count;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// GETMe element step (Smoothers/ElementWise/GetmeSmoother.cpp:55-127, 245-374):
+// every element's transformed positions and weight against the pass-start
+// positions; out[n_vertices_per_element * id + k] = (vpp_k, weight).
+// ---------------------------------------------------------------------------
+template<typename T, int EV, int SA>
+__global__ void __launch_bounds__(256) getmeTetKernel(DevMesh m, gmq::Crew c, typename PosOf<T>::type* out)
+{
+    uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    if(e >= m.ntets)
+        return;
+    typedef typename PosOf<T>::type P;
+    const P* pos = PosOf<T>::get(m);
+    const T lambda = T(0.78);
+    uint4 t = m.tets[e];
+    uint32_t vi[4] = {t.x, t.y, t.z, t.w};
+    v3<T> vp[4];
+#pragma unroll
+    for(int k = 0; k < 4; ++k) vp[k] = ld<T>(pos, vi[k]);
+    v3<T> n[4] = {gmq::cross(vp[3] - vp[1], vp[2] - vp[1]), gmq::cross(vp[3] - vp[2], vp[0] - vp[2]),
+                  gmq::cross(vp[1] - vp[3], vp[0] - vp[3]), gmq::cross(vp[1] - vp[0], vp[2] - vp[0])};
+    v3<T> vpp[4];
+#pragma unroll
+    for(int k = 0; k < 4; ++k) vpp[k] = vp[k] + n[k] * (lambda / gmq::tsqrt(gmq::tsqrt(gmq::dot(n[k], n[k]))));
+    T volume = gmq::tetVolume<T, SA>(c, vp);
+    T volumePrime = gmq::tetVolume<T, SA>(c, vpp);
+    T volumeVar = gmq::tcbrt(gmq::tabs(volume / volumePrime));
+    v3<T> center = T(1.0 / 4.0) * (vp[0] + vp[1] + vp[2] + vp[3]);
+#pragma unroll
+    for(int k = 0; k < 4; ++k) vpp[k] = center + volumeVar * (vpp[k] - center);
+#pragma unroll
+    for(int k = 0; k < 4; ++k) if(m.fixed[vi[k]]) vpp[k] = vp[k];
+    T quality = gmq::tetQuality<T, EV, SA>(c, vp);
+    T qualityPrime = gmq::tetQuality<T, EV, SA>(c, vpp);
+    T weight = qualityPrime / (T(1) + quality);
+#pragma unroll
+    for(int k = 0; k < 4; ++k)
+    {
+        P o;
+        o.x = vpp[k].x; o.y = vpp[k].y; o.z = vpp[k].z; o.w = weight;
+        out[4 * (size_t)e + k] = o;
+    }
+}
+
+template<typename T, int EV, int SA>
+__global__ void __launch_bounds__(256) getmeHexKernel(DevMesh m, gmq::Crew c, typename PosOf<T>::type* out)
+{
+    uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    if(e >= m.nhexs)
+        return;
+    typedef typename PosOf<T>::type P;
+    const P* pos = PosOf<T>::get(m);
+    const T lambda = T(0.78);
+    uint4 ha = m.hexs[2 * e], hb = m.hexs[2 * e + 1];
+    uint32_t vi[8] = {ha.x, ha.y, ha.z, ha.w, hb.x, hb.y, hb.z, hb.w};
+    v3<T> vp[8];
+#pragma unroll
+    for(int k = 0; k < 8; ++k) vp[k] = ld<T>(pos, vi[k]);
+    const T q4 = T(4), t3 = T(3);
+    v3<T> aux[6] = {(vp[0] + vp[1] + vp[2] + vp[3]) / q4, (vp[0] + vp[4] + vp[5] + vp[1]) / q4,
+                    (vp[1] + vp[5] + vp[6] + vp[2]) / q4, (vp[2] + vp[6] + vp[7] + vp[3]) / q4,
+                    (vp[0] + vp[3] + vp[7] + vp[4]) / q4, (vp[4] + vp[7] + vp[6] + vp[5]) / q4};
+    v3<T> n[8] = {gmq::cross(aux[1] - aux[0], aux[4] - aux[0]), gmq::cross(aux[2] - aux[0], aux[1] - aux[0]),
+                  gmq::cross(aux[3] - aux[0], aux[2] - aux[0]), gmq::cross(aux[4] - aux[0], aux[3] - aux[0]),
+                  gmq::cross(aux[4] - aux[5], aux[1] - aux[5]), gmq::cross(aux[1] - aux[5], aux[2] - aux[5]),
+                  gmq::cross(aux[2] - aux[5], aux[3] - aux[5]), gmq::cross(aux[3] - aux[5], aux[4] - aux[5])};
+    v3<T> bases[8] = {(aux[0] + aux[1] + aux[4]) / t3, (aux[0] + aux[2] + aux[1]) / t3,
+                      (aux[0] + aux[3] + aux[2]) / t3, (aux[0] + aux[4] + aux[3]) / t3,
+                      (aux[5] + aux[4] + aux[1]) / t3, (aux[5] + aux[1] + aux[2]) / t3,
+                      (aux[5] + aux[2] + aux[3]) / t3, (aux[5] + aux[3] + aux[4]) / t3};
+    v3<T> vpp[8];
+#pragma unroll
+    for(int k = 0; k < 8; ++k) vpp[k] = bases[k] + n[k] * (lambda / gmq::tsqrt(gmq::tsqrt(gmq::dot(n[k], n[k]))));
+    T volume = gmq::hexVolume<T, SA>(c, vp);
+    T volumePrime = gmq::hexVolume<T, SA>(c, vpp);
+    T volumeVar = gmq::tcbrt(gmq::tabs(volume / volumePrime));
+    v3<T> center = T(1.0 / 8.0) * (vp[0] + vp[1] + vp[2] + vp[3] + vp[4] + vp[5] + vp[6] + vp[7]);
+#pragma unroll
+    for(int k = 0; k < 8; ++k) vpp[k] = center + volumeVar * (vpp[k] - center);
+#pragma unroll
+    for(int k = 0; k < 8; ++k) if(m.fixed[vi[k]]) vpp[k] = vp[k];
+    T quality = gmq::hexQuality<T, EV, SA>(c, vp);
+    T qualityPrime = gmq::hexQuality<T, EV, SA>(c, vpp);
+    T weight = qualityPrime / (T(1) + quality);
+#pragma unroll
+    for(int k = 0; k < 8; ++k)
+    {
+        P o;
+        o.x = vpp[k].x; o.y = vpp[k].y; o.z = vpp[k].z; o.w = weight;
+        out[8 * (size_t)e + k] = o;
+    }
+}
+
+// GETMe vertex step (AbstractElementWiseSmoother::updateVertexPositions,
+// AbstractElementWiseSmoother.cpp:406-444; accumulation VertexAccum.cpp:28-49)
+// for one colour group: pull the contributions in patch order (the push order
+// of the element loop), move to their weighted mean, keep the move unless the
+// patch quality drops.
+template<typename T, int EV, int SA>
+__global__ void __launch_bounds__(256) getmeVertexKernel(DevMesh m, gmq::Crew c, const uint32_t* __restrict__ nodes,
+                                                         uint32_t count, const typename PosOf<T>::type* tetOut,
+                                                         const typename PosOf<T>::type* hexOut, int32_t* evalCounts)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if(i >= count)
+        return;
+    typedef typename PosOf<T>::type P;
+    P* pos = PosOf<T>::get(m);
+    uint32_t v = nodes[i];
+    v3<T> acc = mk<T>(T(0), T(0), T(0));
+    T wacc = T(0);
+    uint32_t b = m.eoff[v], e = m.eoff[v + 1];
+    for(uint32_t r = b; r < e; ++r)
+    {
+        uint32_t ref = m.eref[r];
+        uint32_t id = ref >> 4;
+        uint32_t k = ref & 7;
+        P o = (ref & 8) ? hexOut[8 * (size_t)id + k] : tetOut[4 * (size_t)id + k];
+        T w = (T)o.w;
+        acc = acc + mk<T>((T)o.x, (T)o.y, (T)o.z) * w;
+        wacc += w;
+    }
+    if(wacc != T(0))
+    {
+        v3<T> p0 = ld<T>(pos, v);
+        v3<T> p1 = acc / wacc;
+        PlainPatch<T, EV, SA> f{m, c, pos, v, 0};
+        T q0 = f(p0);
+        T q1 = f(p1);
+        if(!(q1 < q0))
+            st(pos, v, p1);
+        if(evalCounts)
+            evalCounts[v] += f.count;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Mesh quality (AbstractEvaluator::evaluateMeshQuality*, AbstractEvaluator.cpp:384-453;
+// QualityHistogram::add, DataStructures/QualityHistogram.cpp:63-92) in FP64.
+// Each block owns a contiguous element range; per-lane compensated sums are
+// combined in a fixed tree, so the result is deterministic.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int histBucket(double value)
+{
+    // size_t(value * 20) clamped to [0, 19]; products that truncate to a
+    // negative integer and NaN land in the last bucket (x86-64 conversion).
+    double x = value * 20.0;
+    if(x != x || x <= -1.0 || x >= 9.2e18) return 19;
+    long long t = (long long)x;
+    return t < 0 ? 0 : (t > 19 ? 19 : (int)t);
+}
+
+__device__ __forceinline__ void twoSum(double a, double b, double& s, double& e)
+{
+    s = a + b;
+    double bb = s - a;
+    e = (a - (s - bb)) + (b - bb);
+}
+
+template<int EV, int SA, typename P>
+__global__ void __launch_bounds__(EVAL_THREADS) evalKernel(DevMesh m, gmq::Crew c, const P* pos, EvalScratch s)
+{
+    __shared__ double smin[EVAL_THREADS], ssum[EVAL_THREADS], scmp[EVAL_THREADS];
+    __shared__ int shist[20];
+    __shared__ int sbad;
+    if(threadIdx.x < 20) shist[threadIdx.x] = 0;
+    if(threadIdx.x == 0) sbad = 0;
+    __syncthreads();
+    const uint64_t E = (uint64_t)m.ntets + m.nhexs;
+    const uint64_t chunk = (E + gridDim.x - 1) / gridDim.x;
+    const uint64_t b0 = (uint64_t)blockIdx.x * chunk;
+    const uint64_t b1 = (b0 + chunk < E) ? b0 + chunk : E;
+    double mn = 1.0, sum = 0.0, cmp = 0.0;
+    int bad = 0;
+    for(uint64_t e = b0 + threadIdx.x; e < b1; e += blockDim.x)
+    {
+        double q;
+        if(e < m.ntets)
+        {
+            uint4 t = m.tets[e];
+            v3<double> vp[4] = {ld<double>(pos, t.x), ld<double>(pos, t.y), ld<double>(pos, t.z), ld<double>(pos, t.w)};
+            q = gmq::tetQuality<double, EV, SA>(c, vp);
+        }
+        else
+        {
+            uint64_t h = e - m.ntets;
+            uint4 a = m.hexs[2 * h], bq = m.hexs[2 * h + 1];
+            v3<double> vp[8] = {ld<double>(pos, a.x), ld<double>(pos, a.y), ld<double>(pos, a.z), ld<double>(pos, a.w),
+                                ld<double>(pos, bq.x), ld<double>(pos, bq.y), ld<double>(pos, bq.z), ld<double>(pos, bq.w)};
+            q = gmq::hexQuality<double, EV, SA>(c, vp);
+        }
+        if(s.elemQual) s.elemQual[e] = q;
+        mn = gmq::tmin(mn, q);
+        if(q > 0.0)
+        {
+            double t, err;
+            twoSum(sum, 1.0 / q, t, err);
+            sum = t;
+            cmp += err;
+        }
+        else
+            bad = 1;
+        atomicAdd(&shist[histBucket(q)], 1);
+    }
+    smin[threadIdx.x] = mn;
+    ssum[threadIdx.x] = sum;
+    scmp[threadIdx.x] = cmp;
+    if(bad) sbad = 1;
+    __syncthreads();
+    for(int w = EVAL_THREADS / 2; w > 0; w >>= 1)
+    {
+        if(threadIdx.x < (unsigned)w)
+        {
+            double a = smin[threadIdx.x], bb = smin[threadIdx.x + w];
+            smin[threadIdx.x] = gmq::tmin(a, bb);
+            double t, err;
+            twoSum(ssum[threadIdx.x], ssum[threadIdx.x + w], t, err);
+            ssum[threadIdx.x] = t;
+            scmp[threadIdx.x] = scmp[threadIdx.x] + scmp[threadIdx.x + w] + err;
+        }
+        __syncthreads();
+    }
+    if(threadIdx.x == 0)
+    {
+        s.partials[3 * blockIdx.x] = smin[0];
+        s.partials[3 * blockIdx.x + 1] = ssum[0];
+        s.partials[3 * blockIdx.x + 2] = scmp[0];
+        s.flags[blockIdx.x] = sbad;
+    }
+    if(threadIdx.x < 20 && shist[threadIdx.x])
+        atomicAdd(&s.hist[threadIdx.x], shist[threadIdx.x]);
+}
+
+static __global__ void __launch_bounds__(EVAL_THREADS) evalFinalKernel(EvalScratch s, int nblocks, double n, int pass)
+{
+    __shared__ double smin[EVAL_THREADS], ssum[EVAL_THREADS], scmp[EVAL_THREADS];
+    __shared__ int sbad;
+    if(threadIdx.x == 0) sbad = 0;
+    __syncthreads();
+    double mn = 1.0, sum = 0.0, cmp = 0.0;
+    for(int b = threadIdx.x; b < nblocks; b += blockDim.x)
+    {
+        mn = gmq::tmin(mn, s.partials[3 * b]);
+        double t, err;
+        twoSum(sum, s.partials[3 * b + 1], t, err);
+        sum = t;
+        cmp += err + s.partials[3 * b + 2];
+        if(s.flags[b]) sbad = 1;
+    }
+    smin[threadIdx.x] = mn;
+    ssum[threadIdx.x] = sum;
+    scmp[threadIdx.x] = cmp;
+    __syncthreads();
+    for(int w = EVAL_THREADS / 2; w > 0; w >>= 1)
+    {
+        if(threadIdx.x < (unsigned)w)
+        {
+            smin[threadIdx.x] = gmq::tmin(smin[threadIdx.x], smin[threadIdx.x + w]);
+            double t, err;
+            twoSum(ssum[threadIdx.x], ssum[threadIdx.x + w], t, err);
+            ssum[threadIdx.x] = t;
+            scmp[threadIdx.x] = scmp[threadIdx.x] + scmp[threadIdx.x + w] + err;
+        }
+        __syncthreads();
+    }
+    if(threadIdx.x == 0)
+    {
+        s.result[3 * pass] = smin[0];
+        s.result[3 * pass + 1] = sbad ? (double)__builtin_huge_val() : ssum[0] + scmp[0];
+        s.result[3 * pass + 2] = n;
+    }
+    if(threadIdx.x < 20)
+    {
+        s.histOut[20 * pass + threadIdx.x] = s.hist[threadIdx.x];
+        s.hist[threadIdx.x] = 0;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Launchers of one (precision, evaluator, sampler) combination; each
+// combination is instantiated in its own translation unit (parallel build)
+// and selected at run time by dispatch.hip.
+// ---------------------------------------------------------------------------
+template<typename T, int EV, int SA, int SM>
+static hipError_t launchSmoothTyped(const DevMesh& m, const gmq::Crew& c, int shape, const uint32_t* nodes,
+                                    uint32_t count, double moveCoeff, int32_t* evalCounts, int maxPatch,
+                                    hipStream_t stream)
+{
+    if(count == 0)
+        return hipSuccess;
+    if(shape == KS_WAVE && maxPatch <= 64)
+    {
+        if(maxPatch <= 8)
+            smoothWaveKernel<T, EV, SA, SM, 8><<<(count + 7) / 8, 64, 0, stream>>>(m, c, nodes, count, (T)moveCoeff, evalCounts);
+        else if(maxPatch <= 16)
+            smoothWaveKernel<T, EV, SA, SM, 16><<<(count + 3) / 4, 64, 0, stream>>>(m, c, nodes, count, (T)moveCoeff, evalCounts);
+        else if(maxPatch <= 32)
+            smoothWaveKernel<T, EV, SA, SM, 32><<<(count + 1) / 2, 64, 0, stream>>>(m, c, nodes, count, (T)moveCoeff, evalCounts);
+        else
+            smoothWaveKernel<T, EV, SA, SM, 64><<<count, 64, 0, stream>>>(m, c, nodes, count, (T)moveCoeff, evalCounts);
+    }
+    else
+    {
+        smoothPlainKernel<T, EV, SA, SM><<<(count + 255) / 256, 256, 0, stream>>>(m, c, nodes, count, (T)moveCoeff, evalCounts);
+    }
+    return hipGetLastError();
+}
+
+template<typename T, int EV, int SA>
+hipError_t Launchers<T, EV, SA>::smooth(const DevMesh& m, const gmq::Crew& c, int smoother, int shape,
+                                        const uint32_t* nodes, uint32_t count, double moveCoeff,
+                                        int32_t* evalCounts, int maxPatch, hipStream_t stream)
+{
+    if(smoother == SM_NELDER_MEAD)
+        return launchSmoothTyped<T, EV, SA, SM_NELDER_MEAD>(m, c, shape, nodes, count, moveCoeff, evalCounts, maxPatch, stream);
+    if(smoother == SM_GRADIENT_DESCENT)
+        return launchSmoothTyped<T, EV, SA, SM_GRADIENT_DESCENT>(m, c, shape, nodes, count, moveCoeff, evalCounts, maxPatch, stream);
+    if(smoother == SM_SPAWN_SEARCH)
+        return launchSmoothTyped<T, EV, SA, SM_SPAWN_SEARCH>(m, c, shape, nodes, count, moveCoeff, evalCounts, maxPatch, stream);
+    return hipErrorInvalidValue;
+}
+
+template<typename T, int EV, int SA>
+hipError_t Launchers<T, EV, SA>::getmeElements(const DevMesh& m, const gmq::Crew& c, void* tetOut, void* hexOut,
+                                               hipStream_t stream)
+{
+    typedef typename PosOf<T>::type P;
+    if(m.ntets)
+        getmeTetKernel<T, EV, SA><<<(m.ntets + 255) / 256, 256, 0, stream>>>(m, c, (P*)tetOut);
+    if(m.nhexs)
+        getmeHexKernel<T, EV, SA><<<(m.nhexs + 255) / 256, 256, 0, stream>>>(m, c, (P*)hexOut);
+    return hipGetLastError();
+}
+
+template<typename T, int EV, int SA>
+hipError_t Launchers<T, EV, SA>::getmeVertices(const DevMesh& m, const gmq::Crew& c, const uint32_t* nodes,
+                                               uint32_t count, const void* tetOut, const void* hexOut,
+                                               int32_t* evalCounts, hipStream_t stream)
+{
+    typedef typename PosOf<T>::type P;
+    if(count == 0)
+        return hipSuccess;
+    getmeVertexKernel<T, EV, SA><<<(count + 255) / 256, 256, 0, stream>>>(m, c, nodes, count, (const P*)tetOut,
+                                                                         (const P*)hexOut, evalCounts);
+    return hipGetLastError();
+}
+
+template<int EV, int SA, typename P>
+static hipError_t launchEvalTyped(const DevMesh& m, const gmq::Crew& c, const P* pos, const EvalScratch& s, int pass,
+                                  hipStream_t stream)
+{
+    const double n = (double)m.ntets + (double)m.nhexs;
+    evalKernel<EV, SA, P><<<EVAL_BLOCKS, EVAL_THREADS, 0, stream>>>(m, c, pos, s);
+    evalFinalKernel<<<1, EVAL_THREADS, 0, stream>>>(s, EVAL_BLOCKS, n, pass);
+    return hipGetLastError();
+}
+
+template<int EV, int SA>
+hipError_t Evaluators<EV, SA>::evaluateF64(const DevMesh& m, const gmq::Crew& c, const EvalScratch& s, int pass,
+                                           hipStream_t stream)
+{
+    return launchEvalTyped<EV, SA, double4>(m, c, m.pos64, s, pass, stream);
+}
+
+template<int EV, int SA>
+hipError_t Evaluators<EV, SA>::evaluateF32(const DevMesh& m, const gmq::Crew& c, const EvalScratch& s, int pass,
+                                           hipStream_t stream)
+{
+    return launchEvalTyped<EV, SA, float4>(m, c, m.pos32, s, pass, stream);
+}
+
+} // namespace gmd
+
+#endif // GM_KERNELS_H

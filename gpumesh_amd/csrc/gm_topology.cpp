@@ -1,0 +1,188 @@
+// gm_topology.cpp -- see gm_topology.h.
+#include "gm_topology.h"
+
+#include <algorithm>
+#include <numeric>
+
+namespace gmt
+{
+
+// Element edge tables: MeshTet::edges and MeshHex::edges (Mesh.cpp:21-28, 98-111).
+static const int TET_EDGES[6][2] = {{0,1},{0,2},{0,3},{1,2},{1,3},{2,3}};
+static const int HEX_EDGES[12][2] = {{0,1},{0,3},{1,2},{2,3},{0,4},{1,5},{2,6},{3,7},{4,5},{4,7},{5,6},{6,7}};
+
+static const int NO_GROUP = -2;
+static const int UNSET_GROUP = -1;
+static const int FIXED_TYPE = 0;
+static const int INTER_TYPE = 3;
+
+bool build(int64_t V, const double* pos, const int32_t* fixed,
+           int64_t T, const uint32_t* tets, int64_t H, const uint32_t* hexs,
+           Topology& out, std::string& err)
+{
+    if(V < 0 || T < 0 || H < 0) { err = "negative size"; return false; }
+    if(V > (int64_t)0xFFFFFFF0) { err = "too many vertices"; return false; }
+    if(T >= (int64_t)1 << 28 || H >= (int64_t)1 << 28) { err = "too many elements (limit 2^28 per type)"; return false; }
+    for(int64_t i = 0; i < 4 * T; ++i) if((int64_t)tets[i] >= V) { err = "tet vertex id out of range"; return false; }
+    for(int64_t i = 0; i < 8 * H; ++i) if((int64_t)hexs[i] >= V) { err = "hex vertex id out of range"; return false; }
+    if(4 * T + 8 * H >= (int64_t)1 << 32) { err = "too many element references"; return false; }
+
+    // ---- Patch element lists (compileNeighborhoods: tets in id order, then hexes).
+    std::vector<uint32_t> eoff(V + 1, 0);
+    for(int64_t i = 0; i < 4 * T; ++i) ++eoff[tets[i] + 1];
+    for(int64_t i = 0; i < 8 * H; ++i) ++eoff[hexs[i] + 1];
+    for(int64_t v = 0; v < V; ++v) eoff[v + 1] += eoff[v];
+    std::vector<uint32_t> eref(eoff[V]);
+    {
+        std::vector<uint32_t> cur(eoff.begin(), eoff.end() - 1);
+        for(int64_t e = 0; e < T; ++e)
+            for(int k = 0; k < 4; ++k)
+                eref[cur[tets[4 * e + k]]++] = ((uint32_t)e << 4) | (uint32_t)k;
+        for(int64_t e = 0; e < H; ++e)
+            for(int k = 0; k < 8; ++k)
+                eref[cur[hexs[8 * e + k]]++] = ((uint32_t)e << 4) | 8u | (uint32_t)k;
+    }
+
+    // ---- Neighbour vertices (addEdge: append when absent, both directions).
+    // Every element adds at most 3 edges at each of its vertices.
+    std::vector<uint32_t> ncnt(V, 0);
+    std::vector<uint32_t> ntmp((size_t)3 * eoff[V]);
+    auto addEdge = [&](uint32_t a, uint32_t b) {
+        uint32_t* la = &ntmp[(size_t)3 * eoff[a]];
+        for(uint32_t i = 0; i < ncnt[a]; ++i)
+            if(la[i] == b)
+                return;
+        la[ncnt[a]++] = b;
+        ntmp[(size_t)3 * eoff[b] + ncnt[b]++] = a;
+    };
+    for(int64_t e = 0; e < T; ++e)
+        for(int k = 0; k < 6; ++k)
+            addEdge(tets[4 * e + TET_EDGES[k][0]], tets[4 * e + TET_EDGES[k][1]]);
+    for(int64_t e = 0; e < H; ++e)
+        for(int k = 0; k < 12; ++k)
+            addEdge(hexs[8 * e + HEX_EDGES[k][0]], hexs[8 * e + HEX_EDGES[k][1]]);
+
+    // ---- Types: only fixed and free vertices exist, so every movable node is
+    // interior (no movable boundary, hence no subsurface nodes).
+    std::vector<int> types(V);
+    for(int64_t v = 0; v < V; ++v) types[v] = fixed[v] ? FIXED_TYPE : INTER_TYPE;
+
+    // ---- Greedy colouring (determineGroups, NodeGroups.cpp:243-363): the
+    // smallest existing colour unused by already-coloured element-mates,
+    // else a new colour; breadth-first from the lowest unvisited id.
+    std::vector<int> groups(V, NO_GROUP);
+    std::vector<uint32_t> nextNodes;
+    nextNodes.reserve(V);
+    std::vector<uint64_t> usedBy;   // colour -> token of the last vertex that saw it
+    int64_t seekStart = 0;
+    while((int64_t)nextNodes.size() < V)
+    {
+        size_t firstNode = nextNodes.size();
+        for(int64_t vId = seekStart; vId < V; ++vId)
+        {
+            ++seekStart;
+            if(groups[vId] == NO_GROUP)
+            {
+                groups[vId] = UNSET_GROUP;
+                nextNodes.push_back((uint32_t)vId);
+                break;
+            }
+        }
+        for(size_t i = firstNode; i < nextNodes.size(); ++i)
+        {
+            uint32_t vId = nextNodes[i];
+            uint64_t token = (uint64_t)vId + 1;
+            for(uint32_t r = eoff[vId]; r < eoff[vId + 1]; ++r)
+            {
+                uint32_t ref = eref[r];
+                uint32_t id = ref >> 4;
+                const uint32_t* ev = (ref & 8) ? &hexs[8 * (size_t)id] : &tets[4 * (size_t)id];
+                int nv = (ref & 8) ? 8 : 4;
+                for(int n = 0; n < nv; ++n)
+                {
+                    int& g = groups[ev[n]];
+                    if(g == NO_GROUP) { g = UNSET_GROUP; nextNodes.push_back(ev[n]); }
+                    else if(g != UNSET_GROUP) usedBy[g] = token;
+                }
+            }
+            int g = -1;
+            for(size_t c = 0; c < usedBy.size(); ++c)
+                if(usedBy[c] != token) { g = (int)c; break; }
+            if(g < 0) { g = (int)usedBy.size(); usedBy.push_back(0); }
+            groups[vId] = g;
+        }
+    }
+
+    // ---- Ordering (determinePositions, NodeGroups.cpp:365-450): the same
+    // comparator through std::sort, so ties resolve identically.
+    std::vector<int> indices(V);
+    std::iota(indices.begin(), indices.end(), 0);
+    std::sort(indices.begin(), indices.end(), [&](int a, int b) {
+        if(types[a] == types[b])
+        {
+            if(types[a] == FIXED_TYPE) return false;
+            if(groups[a] == groups[b])
+                return (eoff[a + 1] - eoff[a]) < (eoff[b + 1] - eoff[b]);
+            return groups[a] < groups[b];
+        }
+        return types[a] < types[b];
+    });
+    out.permutation.assign(V, 0);
+    for(int64_t p = 0; p < V; ++p) out.permutation[indices[p]] = p;
+    const std::vector<int64_t>& P = out.permutation;
+
+    // ---- Renumbering (clusterNodes, NodeGroups.cpp:452-619).
+    out.nverts = V; out.ntets = T; out.nhexs = H;
+    out.pos.resize(3 * V);
+    out.fixed.resize(V);
+    out.eoff.assign(V + 1, 0);
+    out.noff.assign(V + 1, 0);
+    for(int64_t p = 0; p < V; ++p)
+    {
+        int old = indices[p];
+        out.pos[3 * p] = pos[3 * old]; out.pos[3 * p + 1] = pos[3 * old + 1]; out.pos[3 * p + 2] = pos[3 * old + 2];
+        out.fixed[p] = fixed[old] ? 1 : 0;
+        out.eoff[p + 1] = out.eoff[p] + (eoff[old + 1] - eoff[old]);
+        out.noff[p + 1] = out.noff[p] + ncnt[old];
+    }
+    out.eref.resize(out.eoff[V]);
+    out.nbr.resize(out.noff[V]);
+    for(int64_t p = 0; p < V; ++p)
+    {
+        int old = indices[p];
+        std::copy(eref.begin() + eoff[old], eref.begin() + eoff[old + 1], out.eref.begin() + out.eoff[p]);
+        const uint32_t* ln = &ntmp[(size_t)3 * eoff[old]];
+        for(uint32_t i = 0; i < ncnt[old]; ++i)
+            out.nbr[out.noff[p] + i] = (uint32_t)P[ln[i]];
+    }
+    out.tets.resize(4 * T);
+    for(int64_t i = 0; i < 4 * T; ++i) out.tets[i] = (uint32_t)P[tets[i]];
+    out.hexs.resize(8 * H);
+    for(int64_t i = 0; i < 8 * H; ++i) out.hexs[i] = (uint32_t)P[hexs[i]];
+
+    // ---- Groups: movable nodes are sorted by colour, so each colour's nodes
+    // are one contiguous run of new ids; colours that only fixed vertices use
+    // below the largest movable colour stay as empty groups.
+    out.nfixed = 0;
+    for(int64_t v = 0; v < V; ++v) if(fixed[v]) ++out.nfixed;
+    int maxGroup = -1;
+    for(int64_t p = out.nfixed; p < V; ++p) maxGroup = std::max(maxGroup, groups[indices[p]]);
+    int G = maxGroup + 1;
+    out.groupOff.assign(G + 1, 0);
+    for(int64_t p = out.nfixed; p < V; ++p) ++out.groupOff[groups[indices[p]] + 1];
+    for(int g = 0; g < G; ++g) out.groupOff[g + 1] += out.groupOff[g];
+    out.groupNodes.resize(V - out.nfixed);
+    out.serialGroup.resize(V - out.nfixed);
+    out.groupMaxPatch.assign(G, 0);
+    for(int64_t p = out.nfixed; p < V; ++p)
+    {
+        int g = groups[indices[p]];
+        // Sorted order: position within the group is p - (first new id of g).
+        out.serialGroup[p - out.nfixed] = (uint32_t)p;
+        out.groupNodes[p - out.nfixed] = (uint32_t)p;
+        out.groupMaxPatch[g] = std::max<int32_t>(out.groupMaxPatch[g], (int32_t)(out.eoff[p + 1] - out.eoff[p]));
+    }
+    return true;
+}
+
+} // namespace gmt

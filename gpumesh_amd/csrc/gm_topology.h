@@ -1,0 +1,53 @@
+// gm_topology.h -- host-side mesh topology and independent node groups.
+//
+// Produces exactly what the reference computes in Mesh::compileNeighborhoods
+// (DataStructures/Mesh.cpp:365-460) and NodeGroups::build
+// (DataStructures/NodeGroups.cpp:104-619): patch element lists, neighbour
+// vertex lists, the greedy colouring, the (type, colour, patch size) sort and
+// the vertex renumbering.  The data structures are flat arrays (CSR) instead
+// of per-vertex std::vector/std::set, so a million-vertex mesh builds in a
+// fraction of a second, with bit-identical integer output.
+#ifndef GM_TOPOLOGY_H
+#define GM_TOPOLOGY_H
+
+#include <stdint.h>
+#include <string>
+#include <vector>
+
+namespace gmt
+{
+
+struct Topology
+{
+    // Inputs after renumbering (new vertex ids).
+    int64_t nverts = 0, ntets = 0, nhexs = 0;
+    std::vector<double> pos;          // 3 * nverts
+    std::vector<uint8_t> fixed;       // nverts
+    std::vector<uint32_t> tets;       // 4 * ntets
+    std::vector<uint32_t> hexs;       // 8 * nhexs
+
+    // Patch element lists: eref packs (element id << 4) | (is_hex << 3) | local index.
+    std::vector<uint32_t> eoff, eref;
+    // Neighbour vertex lists (edge neighbours, in discovery order).
+    std::vector<uint32_t> noff, nbr;
+
+    // Node groups.
+    std::vector<int64_t> permutation;      // old id -> new id
+    int64_t nfixed = 0;                    // fixed vertices occupy [0, nfixed)
+    std::vector<int64_t> groupOff;         // ngroups + 1 offsets into groupNodes
+    std::vector<uint32_t> groupNodes;      // movable vertices, colour by colour
+    std::vector<int32_t> groupMaxPatch;    // largest patch per group
+    std::vector<uint32_t> serialGroup;     // movable vertices in serial order
+
+    int64_t ngroups() const { return (int64_t)groupOff.size() - 1; }
+};
+
+// Builds the topology.  `fixed[v]` != 0 marks a fixed vertex; every other
+// vertex is free.  Returns false and sets `err` on invalid input.
+bool build(int64_t nverts, const double* pos, const int32_t* fixed,
+           int64_t ntets, const uint32_t* tets, int64_t nhexs, const uint32_t* hexs,
+           Topology& out, std::string& err);
+
+} // namespace gmt
+
+#endif // GM_TOPOLOGY_H

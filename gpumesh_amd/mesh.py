@@ -1,0 +1,143 @@
+"""Python handle of a device-resident mesh (a thin layer over the C ABI).
+
+Names follow the reference's smoothing vocabulary: smoothers
+(Nelder-Mead, gradient descent, spawn search, GETMe), evaluators (mean ratio,
+metric conformity), samplers (uniform, analytic, texture), colour groups
+(NodeGroups' independent sets) and passes.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, lib, ptr
+
+SMOOTHERS = {"nm": 0, "nelder_mead": 0, "gd": 1, "gradient_descent": 1, "spawn": 2, "spawn_search": 2,
+             "getme": 3}
+EVALUATORS = {"mean_ratio": 0, "metric_conformity": 1}
+SAMPLERS = {"uniform": 0, "analytic": 1, "texture": 2}
+PRECISIONS = {"fp64": 0, "fp32": 1, "fp32_polish": 2}
+KERNELS = {"plain": 0, "wave": 1}
+# gm_kernel_class order (include/gpumesh.h).
+KERNEL_CLASSES = ["smooth_f32", "smooth_f64", "getme_elem_f32", "getme_elem_f64", "getme_vert_f32",
+                  "getme_vert_f64", "evaluate"]
+KERNEL_CLASSES_N = len(KERNEL_CLASSES)
+
+
+class GpuMesh:
+    """A mesh uploaded to the current HIP device (or, with device=False, its
+    host-side topology only).  Vertex ids are the renumbered ones."""
+
+    def __init__(self, pos, fixed, tets, hexs, device: bool = True):
+        L = lib()
+        pos = np.ascontiguousarray(pos, np.float64).reshape(-1, 3)
+        fixed = np.ascontiguousarray(fixed, np.int32).reshape(-1)
+        tets = np.ascontiguousarray(tets, np.uint32).reshape(-1, 4)
+        hexs = np.ascontiguousarray(hexs, np.uint32).reshape(-1, 8)
+        h = C.c_void_p()
+        fn = L.gm_mesh_create if device else L.gm_mesh_create_host
+        check(fn(pos.shape[0], ptr(pos), ptr(fixed), tets.shape[0], ptr(tets), hexs.shape[0], ptr(hexs),
+                 C.byref(h)), "gm_mesh_create" if device else "gm_mesh_create_host")
+        self._h = h
+        self.device = device
+        info = np.zeros(8, np.int64)
+        check(L.gm_mesh_info(self._h, ptr(info)), "gm_mesh_info")
+        (self.nverts, self.ntets, self.nhexs, self.nmovable, self.ngroups, self.sum_patch,
+         self.sum_nbr, self.max_patch) = (int(x) for x in info)
+
+    @classmethod
+    def from_cube(cls, m, device: bool = True):
+        return cls(m.pos, m.fixed, m.tets, m.hexs, device=device)
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib().gm_mesh_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ----- topology
+    def permutation(self) -> np.ndarray:
+        out = np.empty(self.nverts, np.int64)
+        check(lib().gm_mesh_permutation(self._h, ptr(out)), "gm_mesh_permutation")
+        return out
+
+    def groups(self):
+        off = np.empty(self.ngroups + 1, np.int64)
+        nodes = np.empty(max(self.nmovable, 1), np.uint32)
+        check(lib().gm_mesh_groups(self._h, ptr(off), ptr(nodes)), "gm_mesh_groups")
+        return off, nodes[: self.nmovable]
+
+    def elements(self):
+        t = np.empty(max(4 * self.ntets, 1), np.uint32)
+        h = np.empty(max(8 * self.nhexs, 1), np.uint32)
+        check(lib().gm_mesh_elements(self._h, ptr(t), ptr(h)), "gm_mesh_elements")
+        return t[: 4 * self.ntets].reshape(-1, 4), h[: 8 * self.nhexs].reshape(-1, 8)
+
+    def topology(self):
+        eo = np.empty(self.nverts + 1, np.uint32)
+        er = np.empty(max(self.sum_patch, 1), np.uint32)
+        no = np.empty(self.nverts + 1, np.uint32)
+        nb = np.empty(max(self.sum_nbr, 1), np.uint32)
+        check(lib().gm_mesh_topology(self._h, ptr(eo), ptr(er), ptr(no), ptr(nb)), "gm_mesh_topology")
+        return eo, er[: self.sum_patch], no, nb[: self.sum_nbr]
+
+    # ----- positions
+    def positions(self) -> np.ndarray:
+        out = np.empty(3 * self.nverts, np.float64)
+        check(lib().gm_mesh_get_positions(self._h, ptr(out)), "gm_mesh_get_positions")
+        return out.reshape(-1, 3)
+
+    def set_positions(self, pos) -> None:
+        pos = np.ascontiguousarray(pos, np.float64).reshape(-1)
+        check(lib().gm_mesh_set_positions(self._h, ptr(pos)), "gm_mesh_set_positions")
+
+    # ----- crew
+    def set_crew(self, evaluator="mean_ratio", sampler="uniform", scaling=1.0, texture=None) -> None:
+        args = [None, None, None, None]
+        if sampler == "texture":
+            size, tmin, text, texels = texture
+            keep = [np.ascontiguousarray(size, np.int32), np.ascontiguousarray(tmin, np.float64),
+                    np.ascontiguousarray(text, np.float64), np.ascontiguousarray(texels, np.float32)]
+            args = [ptr(a) for a in keep]
+        check(lib().gm_mesh_set_crew(self._h, EVALUATORS[evaluator], SAMPLERS[sampler], float(scaling), *args),
+              "gm_mesh_set_crew")
+
+    # ----- evaluation and smoothing
+    def evaluate(self, element_qualities: bool = False):
+        out = np.zeros(3, np.float64)
+        hist = np.zeros(20, np.int32)
+        q = np.zeros(max(self.ntets + self.nhexs, 1), np.float64) if element_qualities else None
+        check(lib().gm_mesh_evaluate(self._h, ptr(out), ptr(hist), ptr(q)), "gm_mesh_evaluate")
+        r = {"min": float(out[0]), "inv_sum": float(out[1]), "n": int(out[2]),
+             "mean": float(out[2] / out[1]) if out[1] != 0 else float("nan"), "hist": hist}
+        if element_qualities:
+            r["elem_qual"] = q[: self.ntets + self.nhexs]
+        return r
+
+    def smooth(self, smoother: str, passes: int, precision: str = "fp64", kernel: str = "plain",
+               spawn_coeff: float = 1.0 / 20.0, counts: bool = True):
+        stats = np.zeros((passes + 1, 3), np.float64)
+        hist = np.zeros((passes + 1, 20), np.int32)
+        cnt = np.zeros(self.nverts, np.int32) if counts else None
+        check(lib().gm_mesh_smooth(self._h, SMOOTHERS[smoother], int(passes), PRECISIONS[precision],
+                                   KERNELS[kernel], float(spawn_coeff), ptr(stats), ptr(hist), ptr(cnt)),
+              "gm_mesh_smooth")
+        t = np.zeros(2, np.float64)
+        check(lib().gm_mesh_last_timing(self._h, ptr(t)), "gm_mesh_last_timing")
+        kms = np.zeros(KERNEL_CLASSES_N, np.float64)
+        kn = np.zeros(KERNEL_CLASSES_N, np.int64)
+        check(lib().gm_mesh_kernel_times(self._h, ptr(kms), ptr(kn)), "gm_mesh_kernel_times")
+        r = {"min": stats[:, 0].copy(), "mean": stats[:, 1].copy(), "inv_sum": stats[:, 2].copy(),
+             "hist": hist, "total_ms": float(t[0]), "sweep_ms": float(t[1]),
+             "kernel_ms": dict(zip(KERNEL_CLASSES, kms.tolist())),
+             "kernel_launches": dict(zip(KERNEL_CLASSES, kn.tolist()))}
+        if counts:
+            r["eval_counts"] = cnt
+        return r

@@ -1,0 +1,154 @@
+/* gpumesh.h -- C ABI of libgpumesh_hip.so, the MI355X vertex-relocation
+ * smoothing library.
+ *
+ * Plain C types only (pointers, sizes, status codes).  A mesh handle owns a
+ * device-resident copy of the mesh (positions in FP64 and FP32, connectivity,
+ * patch lists, colour groups) and one HIP stream; calls on one handle are
+ * serialised by the caller (a handle is not reentrant), calls on different
+ * handles are independent.
+ *
+ * Numbering: gm_mesh_create renumbers vertices exactly as the reference's
+ * NodeGroups::clusterNodes does (fixed vertices first, then colour group by
+ * colour group).  Every array exchanged after creation (positions, per-vertex
+ * counts, connectivity) uses the NEW numbering; gm_mesh_permutation returns
+ * old id -> new id.
+ *
+ * Every function returns GM_OK (0) or a GM_ERR_* code; gm_last_error() gives
+ * a message for the calling thread.
+ */
+#ifndef GPUMESH_H
+#define GPUMESH_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct gm_mesh gm_mesh;
+
+enum gm_status
+{
+    GM_OK = 0,
+    GM_ERR_ARG = 1,          /* invalid argument */
+    GM_ERR_HIP = 2,          /* HIP runtime error (no device, launch failure...) */
+    GM_ERR_UNSUPPORTED = 3,  /* valid request this build does not implement */
+    GM_ERR_NOMEM = 4         /* device allocation failed */
+};
+
+enum gm_smoother { GM_NELDER_MEAD = 0, GM_GRADIENT_DESCENT = 1, GM_SPAWN_SEARCH = 2, GM_GETME = 3 };
+enum gm_evaluator { GM_MEAN_RATIO = 0, GM_METRIC_CONFORMITY = 1 };
+enum gm_sampler { GM_UNIFORM = 0, GM_ANALYTIC_SHELL = 1, GM_TEXTURE = 2 };
+/* GM_FP32_POLISH: every pass but the last in FP32, the last pass in FP64;
+ * mesh-quality reductions are FP64 in every mode. */
+enum gm_precision { GM_FP64 = 0, GM_FP32 = 1, GM_FP32_POLISH = 2 };
+/* Kernel shape: one vertex per lane (plain), or one vertex per group of
+ * lanes holding its patch elements (wave). */
+enum gm_kernel { GM_KERNEL_PLAIN = 0, GM_KERNEL_WAVE = 1 };
+
+const char* gm_last_error(void);
+const char* gm_version(void);
+
+/* Number of visible HIP devices (0 without a GPU). */
+int gm_device_count(int* count);
+
+/* Builds topology and colour groups on the host (bit-exact with the
+ * reference's Mesh::compileTopology, DataStructures/Mesh.cpp:213-284, and
+ * NodeGroups::build, DataStructures/NodeGroups.cpp:104-122) and uploads the
+ * mesh to the current device.  pos: 3*nverts doubles; fixed: nverts flags
+ * (non-zero = vertex fixed in place, the reference's MeshTopo(fixedPosition),
+ * Mesh.cpp:144-147); tets: 4*ntets ids; hexs: 8*nhexs ids (v0..v3 bottom,
+ * v4..v7 top).  Replaces Mesh::updateCudaTopology/updateCudaVertices
+ * (GpuMesh.cpp:227-276 -> resources/cuda/Mesh.cu:74-279). */
+int gm_mesh_create(int64_t nverts, const double* pos, const int32_t* fixed,
+                   int64_t ntets, const uint32_t* tets, int64_t nhexs, const uint32_t* hexs,
+                   gm_mesh** out);
+/* Same as gm_mesh_create but builds the topology and colour groups on the
+ * host only (no device copy, no GPU needed): the topology queries below work,
+ * calls that need the device return GM_ERR_ARG. */
+int gm_mesh_create_host(int64_t nverts, const double* pos, const int32_t* fixed,
+                        int64_t ntets, const uint32_t* tets, int64_t nhexs, const uint32_t* hexs,
+                        gm_mesh** out);
+int gm_mesh_destroy(gm_mesh* mesh);
+
+/* info[8] = {nverts, ntets, nhexs, nmovable, ngroups, sum of patch sizes,
+ *            sum of neighbour counts, largest patch}. */
+int gm_mesh_info(const gm_mesh* mesh, int64_t* info);
+/* perm[nverts]: old id -> new id (NodeGroups::determinePositions, NodeGroups.cpp:447-449). */
+int gm_mesh_permutation(const gm_mesh* mesh, int64_t* perm);
+/* group_off[ngroups+1], group_nodes[nmovable]: colour groups (NodeGroups::ParallelGroup). */
+int gm_mesh_groups(const gm_mesh* mesh, int64_t* group_off, uint32_t* group_nodes);
+/* Renumbered connectivity: tets[4*ntets], hexs[8*nhexs] (either may be NULL). */
+int gm_mesh_elements(const gm_mesh* mesh, uint32_t* tets, uint32_t* hexs);
+/* Patch lists and neighbour lists (MeshTopo::neighborElems / neighborVerts):
+ * eoff[nverts+1], eref[sum patch] = (elem id << 4) | (is_hex << 3) | local index,
+ * noff[nverts+1], nbr[sum neighbours]. */
+int gm_mesh_topology(const gm_mesh* mesh, uint32_t* eoff, uint32_t* eref, uint32_t* noff, uint32_t* nbr);
+
+/* FP64 positions, new numbering (replaces fetchCudaVerts, resources/cuda/Mesh.cu:56-72). */
+int gm_mesh_get_positions(gm_mesh* mesh, double* pos);
+/* Replaces updateCudaVerts (resources/cuda/Mesh.cu:74-90). */
+int gm_mesh_set_positions(gm_mesh* mesh, const double* pos);
+
+/* Crew: evaluator, sampler and its scaling (the measurer follows the sampler
+ * as in MeshCrew::setSampler, DataStructures/MeshCrew.cpp:93-103: uniform ->
+ * metric-free, analytic/texture -> metric-wise).  Texture arguments are used
+ * only for GM_TEXTURE: tsize[3] cells, tmin[3]/text[3] grid box,
+ * texels[6*cells] FP32 (m00 m01 m02 m11 m12 m22, cell (i,j,k) at
+ * (k*ny + j)*nx + i).  Replaces installCuda*Evaluator/Measurer/Sampler,
+ * setCudaMetricScaling* (Samplers/AbstractSampler.cpp:11-15, 103-111) and
+ * updateCudaSamplerTextures (Samplers/TextureSampler.cpp:75-80). */
+int gm_mesh_set_crew(gm_mesh* mesh, int evaluator, int sampler, double scaling,
+                     const int32_t* tsize, const double* tmin, const double* text, const float* texels);
+
+/* Mesh quality in FP64 (AbstractEvaluator::evaluateMeshQuality*,
+ * Evaluators/AbstractEvaluator.cpp:384-453, 574-610; replaces
+ * evaluateCudaMeshQuality).  out3 = {min, sum of 1/q (+inf if any q <= 0),
+ * element count}; hist20 = 20-bucket histogram; elem_qual (may be NULL) =
+ * per-element qualities, tets then hexes.  Positions are read from the FP64
+ * copy. */
+int gm_mesh_evaluate(gm_mesh* mesh, double* out3, int32_t* hist20, double* elem_qual);
+
+/* Runs `passes` smoothing sweeps (AbstractSmoother::smoothMesh with
+ * topology operations and autopilot off, Smoothers/AbstractSmoother.cpp:57-211;
+ * replaces smoothCudaVertices / smoothCudaSpawnVertices / smoothCudaElements +
+ * updateCudaSmoothedElementsVertices).  Each sweep visits the colour groups in
+ * order.  stats[3*(passes+1)] = per record {min, harmonic mean, sum 1/q},
+ * record 0 before the first sweep; hist[20*(passes+1)]; eval_counts[nverts]
+ * (may be NULL) = patch evaluations per vertex, accumulated over the run.
+ * spawn_coeff: spawn-search lattice scale relative to the local element size
+ * (the reference uses 1/20, SpawnSearchSmoother.cpp:14). */
+int gm_mesh_smooth(gm_mesh* mesh, int smoother, int passes, int precision, int kernel,
+                   double spawn_coeff, double* stats, int32_t* hist, int32_t* eval_counts);
+
+/* Wall time of the last gm_mesh_smooth on the device (HIP events around the
+ * sweeps and evaluations), in milliseconds; out[0] = total, out[1] = sweeps only. */
+int gm_mesh_last_timing(const gm_mesh* mesh, double* out2);
+
+/* Kernel classes of gm_mesh_kernel_times. */
+enum gm_kernel_class
+{
+    GM_KT_SMOOTH_F32 = 0,      /* vertex-wise optimiser launches (one per colour group), FP32 */
+    GM_KT_SMOOTH_F64 = 1,      /* same, FP64 */
+    GM_KT_GETME_ELEM_F32 = 2,  /* GETMe element transforms, FP32 */
+    GM_KT_GETME_ELEM_F64 = 3,
+    GM_KT_GETME_VERT_F32 = 4,  /* GETMe vertex updates (one per colour group), FP32 */
+    GM_KT_GETME_VERT_F64 = 5,
+    GM_KT_EVALUATE = 6,        /* FP64 mesh-quality reduction (one per pass record) */
+    GM_KT_COUNT = 7
+};
+
+/* Per-class device time (HIP events around each launch on the mesh's
+ * stream) and launch counts of the last gm_mesh_smooth: ms[GM_KT_COUNT],
+ * launches[GM_KT_COUNT] (either may be NULL). */
+int gm_mesh_kernel_times(const gm_mesh* mesh, double* ms, int64_t* launches);
+
+/* Device math probe (for tests): op 0 sqrt(a), 1 a/b, 2 a^(2/3), 3 a^(1/3),
+ * 4 exp(a), all in FP64 on the device; n values. */
+int gm_math_probe(int op, const double* a, const double* b, double* out, int64_t n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GPUMESH_H */
