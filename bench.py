@@ -89,7 +89,8 @@ def main():
     ap.add_argument("--precision", default="fp32_polish", choices=["fp64", "fp32", "fp32_polish"])
     ap.add_argument("--kernel", default="wave", choices=["plain", "wave"])
     ap.add_argument("--plain-steps", type=int, default=2, help="passes of the plain FP64 kernel per smoother")
-    ap.add_argument("--cpu-n", type=int, default=30, help="cube size of the CPU sample")
+    ap.add_argument("--cpu-n", type=int, default=80, help="cube size of the CPU sample")
+    ap.add_argument("--cpu-passes", type=int, default=2, help="passes per smoother of the CPU sample")
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
 
@@ -179,7 +180,7 @@ def main():
             achieved = flop_per_launch / avg_s / 1e12
             roofline = {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                         "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
-                        "kernel": f"smoothWaveKernel<float> ({best['smoother']})", "avg_launch_ms": avg_s * 1e3,
+                        "kernel": f"{'spawnLaneKernel' if best['smoother'] == 'spawn' else ('gdLaneKernel' if best['smoother'] == 'gd' else 'smoothWaveKernel')}<float> ({best['smoother']})", "avg_launch_ms": avg_s * 1e3,
                         "flop_per_launch": flop_per_launch}
         elif best["class"] == "evaluate":
             bytes_per = 16.0 * gm.ntets + 32.0 * gm.nverts
@@ -202,9 +203,9 @@ def main():
     if not args.no_cpu:
         threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
         threads = max(1, min(threads, 16))
-        cpu_value, cpu_s, cm = cpu_baseline(args.cpu_n, 1, threads, smoothers, spawn_coeff)
+        cpu_value, cpu_s, cm = cpu_baseline(args.cpu_n, args.cpu_passes, threads, smoothers, spawn_coeff)
         cpu = {"value": cpu_value, "unit": "smoothed vertices/s", "cores": threads, "kind": "port",
-               "sample": f"FP64 oracle, Thread mode, 1 pass each of {','.join(smoothers)} on the same "
+               "sample": f"FP64 oracle, Thread mode, {args.cpu_passes} passes each of {','.join(smoothers)} on the same "
                          f"construction at {args.cpu_n}^3 cells ({cm.nverts} vertices), glibc libm; "
                          f"{cpu_s:.1f} s"}
         log(f"cpu oracle: {cpu_value:.0f} vertices/s on {threads} threads ({cpu_s:.1f} s)")

@@ -247,6 +247,199 @@ __global__ void __launch_bounds__(64) smoothWaveKernel(DevMesh m, gmq::Crew c, c
 }
 
 // ---------------------------------------------------------------------------
+// Candidate-parallel kernels: the patch of a vertex is staged once in LDS
+// (every element's vertex positions and the centre's local index), and each
+// lane evaluates the whole patch for its own trial position, summing in patch
+// order exactly like the sequential code.  Trial positions that the
+// reference evaluates one after another but independently (the 64 spawn
+// lattice points, the 6 gradient samples and the 8 line-search points) run
+// side by side; the argmax keeps the reference's rule (strictly greater
+// wins, so the first maximum; NaN never wins; all -inf/NaN -> index 0).
+//   EK: 0 = tetrahedra only (4 slots per element), 1 = hexahedra only (8 slots).
+// ---------------------------------------------------------------------------
+template<int EK> struct Slots { static const int N = EK == 0 ? 4 : 8; };
+
+template<typename T, int EK, typename P>
+__device__ __forceinline__ void stageElement(const DevMesh& m, const P* pos, uint32_t ref, T* dst, uint8_t* k)
+{
+    uint32_t id = ref >> 4;
+    *k = (uint8_t)(ref & 7);
+    uint32_t vi[8];
+    if(EK == 0)
+    {
+        uint4 t = m.tets[id];
+        vi[0] = t.x; vi[1] = t.y; vi[2] = t.z; vi[3] = t.w;
+    }
+    else
+    {
+        uint4 a = m.hexs[2 * id], b = m.hexs[2 * id + 1];
+        vi[0] = a.x; vi[1] = a.y; vi[2] = a.z; vi[3] = a.w;
+        vi[4] = b.x; vi[5] = b.y; vi[6] = b.z; vi[7] = b.w;
+    }
+#pragma unroll
+    for(int j = 0; j < Slots<EK>::N; ++j)
+    {
+        v3<T> q = ld<T>(pos, vi[j]);
+        dst[3 * j] = q.x; dst[3 * j + 1] = q.y; dst[3 * j + 2] = q.z;
+    }
+}
+
+// Patch quality (AbstractEvaluator::patchQuality order) from an LDS-staged patch.
+template<typename T, int EV, int SA, int EK>
+__device__ __forceinline__ T ldsPatchQuality(const gmq::Crew& c, const T* sp, const uint8_t* sk, int nE, v3<T> p)
+{
+    T pq = T(0), pw = T(0);
+    for(int e = 0; e < nE; ++e)
+    {
+        const T* b = sp + e * Slots<EK>::N * 3;
+        const int k = sk[e];
+        T q;
+        if(EK == 0)
+        {
+            v3<T> vp[4];
+#pragma unroll
+            for(int j = 0; j < 4; ++j) vp[j] = (j == k) ? p : mk<T>(b[3 * j], b[3 * j + 1], b[3 * j + 2]);
+            q = gmq::tetQuality<T, EV, SA>(c, vp);
+        }
+        else
+        {
+            v3<T> vp[8];
+#pragma unroll
+            for(int j = 0; j < 8; ++j) vp[j] = (j == k) ? p : mk<T>(b[3 * j], b[3 * j + 1], b[3 * j + 2]);
+            q = gmq::hexQuality<T, EV, SA>(c, vp);
+        }
+        gmq::accumulatePatch<T>(pq, pw, q);
+    }
+    return gmq::finalizePatch<T>(pq, pw);
+}
+
+// Argmax over `width` lanes (power of two, aligned groups): the largest key,
+// lowest lane among equals.  NaN keys count as -inf.
+template<typename T>
+__device__ __forceinline__ int groupArgmax(T q, int lane, int width)
+{
+    T key = (q == q) ? q : (T)(-__builtin_huge_val());
+    int idx = lane;
+    for(int off = width >> 1; off > 0; off >>= 1)
+    {
+        T ok = __shfl_xor(key, off, 64);
+        int oi = __shfl_xor(idx, off, 64);
+        if(ok > key || (ok == key && oi < idx)) { key = ok; idx = oi; }
+    }
+    return idx;
+}
+
+template<typename T>
+__device__ __forceinline__ v3<T> shflv(v3<T> a, int src)
+{
+    return mk<T>(__shfl(a.x, src, 64), __shfl(a.y, src, 64), __shfl(a.z, src, 64));
+}
+
+// Spawn search (SpawnSearchSmoother::smoothVertices, SpawnSearchSmoother.cpp:161-226):
+// one vertex per wavefront, lane p evaluates lattice point p.
+template<typename T, int EV, int SA, int EK, int MAXP>
+__global__ void __launch_bounds__(64) spawnLaneKernel(DevMesh m, gmq::Crew c, const uint32_t* __restrict__ nodes,
+                                                      uint32_t count, T moveCoeff, int32_t* evalCounts)
+{
+    typedef typename PosOf<T>::type P;
+    __shared__ T sp[MAXP * Slots<EK>::N * 3];
+    __shared__ uint8_t sk[MAXP];
+    const uint32_t i = blockIdx.x;
+    const int lane = threadIdx.x;
+    P* pos = PosOf<T>::get(m);
+    const uint32_t v = nodes[i];
+    const uint32_t eb = m.eoff[v];
+    const int nE = (int)(m.eoff[v + 1] - eb);
+    for(int e = lane; e < nE; e += 64)
+        stageElement<T, EK>(m, pos, m.eref[eb + e], sp + e * Slots<EK>::N * 3, sk + e);
+    __syncthreads();
+    v3<T> p0 = ld<T>(pos, v);
+    T scale = moveCoeff * localSize<T>(m, pos, v, p0);
+    for(int iter = 0; iter < 2; ++iter)
+    {
+        v3<T> prop = p0 + gms::spawnOffset<T>(lane) * scale;
+        T q = ldsPatchQuality<T, EV, SA, EK>(c, sp, sk, nE, prop);
+        int best = groupArgmax<T>(q, lane, 64);
+        p0 = shflv<T>(prop, best);
+        scale /= T(3);
+    }
+    if(lane == 0)
+    {
+        st(pos, v, p0);
+        if(evalCounts)
+            evalCounts[v] += 128;
+    }
+}
+
+// Gradient descent (GradientDescentSmoother::smoothVertices,
+// GradientDescentSmoother.cpp:73-202): 8 vertices per wavefront, 8 lanes each;
+// lanes 0-5 evaluate the 6 gradient samples, lanes 0-7 the 8 line-search points.
+template<typename T, int EV, int SA, int EK, int MAXP>
+__global__ void __launch_bounds__(64) gdLaneKernel(DevMesh m, gmq::Crew c, const uint32_t* __restrict__ nodes,
+                                                   uint32_t count, int32_t* evalCounts)
+{
+    typedef typename PosOf<T>::type P;
+    const int SL = Slots<EK>::N * 3;
+    __shared__ T sp[8 * MAXP * Slots<EK>::N * 3];
+    __shared__ uint8_t sk[8 * MAXP];
+    const int lane = threadIdx.x;
+    const int g = lane >> 3, s = lane & 7, base = lane & ~7;
+    const uint32_t i = blockIdx.x * 8 + g;
+    const bool present = i < count;
+    P* pos = PosOf<T>::get(m);
+    const uint32_t v = present ? nodes[i] : 0;
+    const uint32_t eb = m.eoff[v];
+    const int nE = present ? (int)(m.eoff[v + 1] - eb) : 0;
+    T* mySp = sp + g * MAXP * SL;
+    uint8_t* mySk = sk + g * MAXP;
+    for(int e = s; e < nE; e += 8)
+        stageElement<T, EK>(m, pos, m.eref[eb + e], mySp + e * SL, mySk + e);
+    __syncthreads();
+    if(!present)
+        return;
+
+    v3<T> p = ld<T>(pos, v);
+    T nodeShift = localSize<T>(m, pos, v, p) * T(1.0 / 75.0);
+    const T originalNodeShift = nodeShift;
+    int evals = 0;
+    for(int cyc = 0; cyc < 5; ++cyc)
+    {
+        // Samples -x, +x, -y, +y, -z, +z on lanes 0..5.
+        const T sx = s == 0 ? -nodeShift : (s == 1 ? nodeShift : T(0));
+        const T sy = s == 2 ? -nodeShift : (s == 3 ? nodeShift : T(0));
+        const T sz = s == 4 ? -nodeShift : (s == 5 ? nodeShift : T(0));
+        T q = T(0);
+        if(s < 6)
+            q = ldsPatchQuality<T, EV, SA, EK>(c, mySp, mySk, nE, p + mk<T>(sx, sy, sz));
+        evals += 6;
+        const T q0 = __shfl(q, base + 0, 64), q1 = __shfl(q, base + 1, 64), q2 = __shfl(q, base + 2, 64);
+        const T q3 = __shfl(q, base + 3, 64), q4 = __shfl(q, base + 4, 64), q5 = __shfl(q, base + 5, 64);
+        v3<T> gq = mk<T>(q1 - q0, q3 - q2, q5 - q4);
+        T gn = gmq::tsqrt(gmq::dot(gq, gq));
+        if(gn == T(0))
+            break;
+        v3<T> shift = gq * (nodeShift / gn);
+        const T off = s == 0 ? T(-0.25) : s == 1 ? T(0.00) : s == 2 ? T(0.10) : s == 3 ? T(0.20)
+                    : s == 4 ? T(0.40) : s == 5 ? T(0.80) : s == 6 ? T(1.20) : T(1.60);
+        v3<T> prop = p + shift * off;
+        T qp = ldsPatchQuality<T, EV, SA, EK>(c, mySp, mySk, nE, prop);
+        evals += 8;
+        int best = groupArgmax<T>(qp, s, 8);
+        p = shflv<T>(prop, base + best);
+        T bestOff = __shfl(off, base + best, 64);
+        nodeShift *= gmq::tabs(bestOff);
+        if(nodeShift < originalNodeShift / T(10))
+            break;
+    }
+    if(s == 0)
+    {
+        st(pos, v, p);
+        if(evalCounts)
+            evalCounts[v] += evals;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // GETMe element step (Smoothers/ElementWise/GetmeSmoother.cpp:55-127, 245-374):
 // every element's transformed positions and weight against the pass-start
 // positions; out[n_vertices_per_element * id + k] = (vpp_k, weight).
@@ -529,6 +722,16 @@ static __global__ void __launch_bounds__(EVAL_THREADS) evalFinalKernel(EvalScrat
 // combination is instantiated in its own translation unit (parallel build)
 // and selected at run time by dispatch.hip.
 // ---------------------------------------------------------------------------
+template<typename T, int EV, int SA, int SM, int EK, int MAXP>
+static void launchLaneKernel(const DevMesh& m, const gmq::Crew& c, const uint32_t* nodes, uint32_t count,
+                             double moveCoeff, int32_t* evalCounts, hipStream_t stream)
+{
+    if(SM == SM_SPAWN_SEARCH)
+        spawnLaneKernel<T, EV, SA, EK, MAXP><<<count, 64, 0, stream>>>(m, c, nodes, count, (T)moveCoeff, evalCounts);
+    else
+        gdLaneKernel<T, EV, SA, EK, MAXP><<<(count + 7) / 8, 64, 0, stream>>>(m, c, nodes, count, evalCounts);
+}
+
 template<typename T, int EV, int SA, int SM>
 static hipError_t launchSmoothTyped(const DevMesh& m, const gmq::Crew& c, int shape, const uint32_t* nodes,
                                     uint32_t count, double moveCoeff, int32_t* evalCounts, int maxPatch,
@@ -536,7 +739,22 @@ static hipError_t launchSmoothTyped(const DevMesh& m, const gmq::Crew& c, int sh
 {
     if(count == 0)
         return hipSuccess;
-    if(shape == KS_WAVE && maxPatch <= 64)
+    // Candidate-parallel kernels for spawn search and gradient descent on
+    // single-element-kind meshes with patches of at most 64 elements.
+    const bool laneKernel = shape == KS_WAVE && (SM == SM_SPAWN_SEARCH || SM == SM_GRADIENT_DESCENT) &&
+                            maxPatch <= 64 && (m.ntets == 0 || m.nhexs == 0);
+    if(laneKernel)
+    {
+        const bool hex = m.ntets == 0;
+        if(SM == SM_SPAWN_SEARCH || SM == SM_GRADIENT_DESCENT)
+        {
+            if(!hex && maxPatch <= 24) launchLaneKernel<T, EV, SA, SM, 0, 24>(m, c, nodes, count, moveCoeff, evalCounts, stream);
+            else if(!hex) launchLaneKernel<T, EV, SA, SM, 0, 64>(m, c, nodes, count, moveCoeff, evalCounts, stream);
+            else if(maxPatch <= 8) launchLaneKernel<T, EV, SA, SM, 1, 8>(m, c, nodes, count, moveCoeff, evalCounts, stream);
+            else launchLaneKernel<T, EV, SA, SM, 1, 64>(m, c, nodes, count, moveCoeff, evalCounts, stream);
+        }
+    }
+    else if(shape == KS_WAVE && maxPatch <= 64)
     {
         if(maxPatch <= 8)
             smoothWaveKernel<T, EV, SA, SM, 8><<<(count + 7) / 8, 64, 0, stream>>>(m, c, nodes, count, (T)moveCoeff, evalCounts);
