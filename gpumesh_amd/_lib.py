@@ -23,7 +23,7 @@ EXPORTS = [
     "gm_last_error", "gm_version", "gm_device_count", "gm_mesh_create", "gm_mesh_create_host",
     "gm_mesh_destroy", "gm_mesh_info", "gm_mesh_permutation", "gm_mesh_groups", "gm_mesh_elements",
     "gm_mesh_topology", "gm_mesh_get_positions", "gm_mesh_set_positions", "gm_mesh_set_crew",
-    "gm_mesh_evaluate", "gm_mesh_smooth", "gm_mesh_last_timing", "gm_mesh_kernel_times", "gm_math_probe",
+    "gm_mesh_evaluate", "gm_mesh_smooth", "gm_mesh_smooth_groups", "gm_mesh_last_timing", "gm_mesh_kernel_times", "gm_math_probe",
 ]
 
 
@@ -59,6 +59,7 @@ def lib():
     L.gm_mesh_set_crew.argtypes = [vp, C.c_int, C.c_int, dbl, vp, vp, vp, vp]
     L.gm_mesh_evaluate.argtypes = [vp, vp, vp, vp]
     L.gm_mesh_smooth.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, dbl, vp, vp, vp]
+    L.gm_mesh_smooth_groups.argtypes = [vp, C.c_int, i64, i64, C.c_int, C.c_int, dbl, vp]
     L.gm_mesh_last_timing.argtypes = [vp, vp]
     L.gm_mesh_kernel_times.argtypes = [vp, vp, vp]
     L.gm_math_probe.argtypes = [C.c_int, vp, vp, vp, i64]

@@ -456,6 +456,9 @@ int gm_mesh_smooth(gm_mesh* m, int smoother, int passes, int precision, int kern
     if(precision < 0 || precision > 2) return fail(GM_ERR_ARG, "unknown precision");
     if(kernel < 0 || kernel > 1) return fail(GM_ERR_ARG, "unknown kernel shape");
     if(!(spawnCoeff > 0.0)) return fail(GM_ERR_ARG, "spawn coefficient must be positive");
+    if(m->sampler == GM_TEXTURE && (kernel == GM_KERNEL_PLAIN || smoother == gmd::SM_GETME))
+        return fail(GM_ERR_UNSUPPORTED, "the texture sampler runs only in the wavefront kernels of the vertex-wise "
+                                        "smoothers (GM_KERNEL_WAVE with Nelder-Mead, gradient descent or spawn search)");
     GM_HIP(hipSetDevice(m->device));
     int rc = ensureResults(m, passes + 1);
     if(rc) return rc;
@@ -565,6 +568,42 @@ int gm_mesh_smooth(gm_mesh* m, int smoother, int passes, int precision, int kern
         }
         if(hist) std::memcpy(hist + 20 * p, h.data() + 20 * p, 20 * sizeof(int32_t));
     }
+    return GM_OK;
+}
+
+int gm_mesh_smooth_groups(gm_mesh* m, int smoother, int64_t groupBegin, int64_t groupEnd, int precision,
+                          int kernel, double spawnCoeff, int32_t* evalCounts)
+{
+    if(!m) return fail(GM_ERR_ARG, "null mesh");
+    if(!m->onDevice) return fail(GM_ERR_ARG, "mesh has no device copy (created with gm_mesh_create_host)");
+    if(smoother < 0 || smoother > 3) return fail(GM_ERR_ARG, "unknown smoother");
+    if(smoother == gmd::SM_GETME)
+        return fail(GM_ERR_UNSUPPORTED, "GETMe sweeps start with an element step over the whole mesh; use gm_mesh_smooth");
+    if(precision != GM_FP64 && precision != GM_FP32) return fail(GM_ERR_ARG, "precision must be GM_FP64 or GM_FP32");
+    if(kernel < 0 || kernel > 1) return fail(GM_ERR_ARG, "unknown kernel shape");
+    if(!(spawnCoeff > 0.0)) return fail(GM_ERR_ARG, "spawn coefficient must be positive");
+    if(kernel == GM_KERNEL_PLAIN && m->sampler == GM_TEXTURE)
+        return fail(GM_ERR_UNSUPPORTED, "the texture sampler runs only in the wavefront kernels of the vertex-wise "
+                                        "smoothers (GM_KERNEL_WAVE with Nelder-Mead, gradient descent or spawn search)");
+    const gmt::Topology& t = m->topo;
+    if(groupBegin < 0 || groupEnd > t.ngroups() || groupBegin > groupEnd) return fail(GM_ERR_ARG, "bad group range");
+    GM_HIP(hipSetDevice(m->device));
+    hipStream_t s = m->stream;
+    size_t V = (size_t)t.nverts;
+    if(V) GM_HIP(hipMemsetAsync(m->evalCounts, 0, V * sizeof(int32_t), s));
+    const bool f64 = precision == GM_FP64;
+    if(!f64) GM_HIP(gmd::launchToF32(m->dm, s));
+    for(int64_t g = groupBegin; g < groupEnd; ++g)
+    {
+        const uint32_t* nodes = m->groupNodes + t.groupOff[g];
+        uint32_t cnt = (uint32_t)(t.groupOff[g + 1] - t.groupOff[g]);
+        hipError_t e = f64 ? gmd::launchSmoothGroup<double>(m->dm, m->crew, m->evaluator, m->sampler, smoother, kernel, nodes, cnt, spawnCoeff, m->evalCounts, t.groupMaxPatch[g], s)
+                           : gmd::launchSmoothGroup<float>(m->dm, m->crew, m->evaluator, m->sampler, smoother, kernel, nodes, cnt, spawnCoeff, m->evalCounts, t.groupMaxPatch[g], s);
+        if(e != hipSuccess) return fail(GM_ERR_HIP, std::string("smoothing launch: ") + hipGetErrorString(e));
+    }
+    if(!f64) GM_HIP(gmd::launchToF64(m->dm, s));
+    if(evalCounts && V) GM_HIP(hipMemcpyAsync(evalCounts, m->evalCounts, V * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    GM_HIP(hipStreamSynchronize(s));
     return GM_OK;
 }
 

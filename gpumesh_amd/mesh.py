@@ -121,6 +121,16 @@ class GpuMesh:
             r["elem_qual"] = q[: self.ntets + self.nhexs]
         return r
 
+    def smooth_groups(self, smoother: str, group_begin: int, group_end: int, precision: str = "fp64",
+                      kernel: str = "wave", spawn_coeff: float = 1.0 / 20.0) -> np.ndarray:
+        """One partial sweep over colour groups [group_begin, group_end); returns
+        the per-vertex patch-evaluation counts."""
+        cnt = np.zeros(self.nverts, np.int32)
+        check(lib().gm_mesh_smooth_groups(self._h, SMOOTHERS[smoother], int(group_begin), int(group_end),
+                                          PRECISIONS[precision], KERNELS[kernel], float(spawn_coeff), ptr(cnt)),
+              "gm_mesh_smooth_groups")
+        return cnt
+
     def smooth(self, smoother: str, passes: int, precision: str = "fp64", kernel: str = "plain",
                spawn_coeff: float = 1.0 / 20.0, counts: bool = True):
         stats = np.zeros((passes + 1, 3), np.float64)

@@ -121,6 +121,16 @@ int gm_mesh_evaluate(gm_mesh* mesh, double* out3, int32_t* hist20, double* elem_
 int gm_mesh_smooth(gm_mesh* mesh, int smoother, int passes, int precision, int kernel,
                    double spawn_coeff, double* stats, int32_t* hist, int32_t* eval_counts);
 
+/* One partial sweep of a vertex-wise smoother (NM, GD, spawn search) over the
+ * colour groups [group_begin, group_end), with no quality evaluation: the
+ * body of one pass of AbstractVertexWiseSmoother::smoothMeshThread
+ * (Smoothers/VertexWise/AbstractVertexWiseSmoother.cpp:114-138) restricted to
+ * those groups.  precision: GM_FP64 or GM_FP32 (FP32 rounds the positions
+ * once before and widens them after).  GETMe returns GM_ERR_UNSUPPORTED (its
+ * sweep starts with an element step over the whole mesh). */
+int gm_mesh_smooth_groups(gm_mesh* mesh, int smoother, int64_t group_begin, int64_t group_end, int precision,
+                          int kernel, double spawn_coeff, int32_t* eval_counts);
+
 /* Wall time of the last gm_mesh_smooth on the device (HIP events around the
  * sweeps and evaluations), in milliseconds; out[0] = total, out[1] = sweeps only. */
 int gm_mesh_last_timing(const gm_mesh* mesh, double* out2);

@@ -1,0 +1,49 @@
+"""FP32 modes against the FP64 oracle: the stated error bounds.
+
+FP32 mode runs every pass in FP32; the polish mode runs all but the last
+pass in FP32 and the last in FP64; both reduce min/mean quality in FP64.
+The optimisers branch on quality comparisons, so FP32 trajectories leave the
+FP64 ones vertex by vertex; the bounds below are on the outcome that matters
+(final positions relative to the cell size h, final min and mean quality).
+The measured values are written to gpurun_out/precision_errors.json when a
+GPU run has that folder.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from gpumesh_amd import GpuMesh, cube_mesh
+
+pytestmark = pytest.mark.gpu
+
+# (smoother, passes, bound on max |dx|/h, bound on |d mean|, bound on |d min|)
+CASES = [("nm", 10, 0.5, 5e-5, 2e-2), ("gd", 10, 0.5, 5e-5, 2e-2), ("spawn", 10, 0.5, 5e-5, 2e-2),
+         ("getme", 10, 0.5, 5e-5, 2e-2)]
+_RESULTS = {}
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp32_polish"])
+@pytest.mark.parametrize("smoother,passes,bx,bmean,bmin", CASES)
+def test_fp32_error_against_fp64_oracle(oracle, smoother, passes, bx, bmean, bmin, precision):
+    n = 20
+    m = cube_mesh(n, "tet", 0.3, seed=1)          # the small plain-method workload
+    coeff = 0.25 if smoother == "spawn" else 1.0 / 20.0
+    om = oracle.OracleMesh.from_cube(m)
+    ref = om.smooth(smoother, passes, threads=8, spawn_coeff=coeff)
+    gm = GpuMesh.from_cube(m)
+    out = gm.smooth(smoother, passes, precision=precision, kernel="wave", spawn_coeff=coeff)
+    h = 1.0 / n
+    dx = float(np.max(np.abs(gm.positions() - om.positions())) / h)
+    dmean = float(abs(out["mean"][-1] - ref["mean"][-1]))
+    dmin = float(abs(out["min"][-1] - ref["min"][-1]))
+    _RESULTS[f"{smoother}/{precision}"] = {"max_dx_over_h": dx, "mean_err": dmean, "min_err": dmin,
+                                           "fp64_mean": float(ref["mean"][-1]), "fp64_min": float(ref["min"][-1])}
+    out_dir = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "gpurun_out")
+    if os.path.isdir(out_dir):
+        with open(os.path.join(out_dir, "precision_errors.json"), "w") as f:
+            json.dump(_RESULTS, f, indent=1)
+    assert dx < bx
+    assert dmean < bmean
+    assert dmin < bmin
