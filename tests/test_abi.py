@@ -1,0 +1,58 @@
+"""The C-ABI library loads and exports every symbol include/gpumesh.h
+declares, and the host-side entry points work without a GPU."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+from gpumesh_amd import _lib, cube_mesh
+from gpumesh_amd.mesh import GpuMesh
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    text = open(os.path.join(ROOT, "include", "gpumesh.h")).read()
+    return sorted(set(re.findall(r"\b(gm_[a-z_0-9]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = _lib.lib()
+    names = declared_functions()
+    assert len(names) >= 15
+    for name in names:
+        assert hasattr(L, name), name
+    assert sorted(names) == sorted(_lib.EXPORTS)
+
+
+def test_version_and_error_strings():
+    L = _lib.lib()
+    assert b"gfx950" in L.gm_version()
+    n = _lib.device_count()
+    assert n >= 0
+
+
+def test_host_mesh_rejects_device_calls():
+    m = cube_mesh(3, "tet", 0.2, seed=1)
+    gm = GpuMesh.from_cube(m, device=False)
+    with pytest.raises(_lib.GpuMeshError) as e:
+        gm.positions()
+    assert e.value.status == 1
+
+
+def test_invalid_inputs_are_rejected():
+    m = cube_mesh(2, "tet", 0.0, seed=1)
+    bad = m.tets.copy()
+    bad[0, 0] = m.nverts + 5
+    with pytest.raises(_lib.GpuMeshError) as e:
+        GpuMesh(m.pos, m.fixed, bad, m.hexs, device=False)
+    assert e.value.status == 1
+    with pytest.raises(_lib.GpuMeshError):
+        GpuMesh(m.pos, m.fixed, m.tets.astype(np.int64) * 0 - 1, m.hexs, device=False)
+
+
+def test_empty_mesh_builds():
+    gm = GpuMesh(np.zeros((0, 3)), np.zeros(0, np.int32), np.zeros((0, 4), np.uint32),
+                 np.zeros((0, 8), np.uint32), device=False)
+    assert gm.nverts == 0 and gm.ngroups == 0

@@ -147,3 +147,22 @@ def test_mixed_element_mesh_bitexact(oracle):
         out = gm.smooth(smoother, 2, precision="fp64", kernel="wave")
         assert np.array_equal(gm.positions(), om.positions()), smoother
         assert np.array_equal(out["eval_counts"], ref["eval_counts"].astype(np.int32))
+
+
+def test_c1_plain_method_workload_against_fixture(oracle):
+    """The plain-method workload (20^3 Kuhn cube, jitter 0.3 h, seed 1, NM x 10,
+    FP64) on the GPU reproduces the committed oracle fixture bit for bit."""
+    import hashlib
+    import json
+    import os
+    golden = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "c1_golden.json")))
+    m = cube_mesh(20, "tet", 0.3, seed=1)
+    for kernel in ("plain", "wave"):
+        gm = GpuMesh.from_cube(m)
+        out = gm.smooth("nm", 10, precision="fp64", kernel=kernel)
+        pos = gm.positions()
+        assert hashlib.sha256(np.ascontiguousarray(pos).tobytes()).hexdigest() == \
+            golden["gm_libm"]["final_positions_sha256"], kernel
+        assert out["min"].tolist() == golden["gm_libm"]["min"]
+        assert np.max(np.abs(out["mean"] - np.array(golden["gm_libm"]["mean"]))) <= MEAN_TOL
+        assert int(out["eval_counts"].sum()) == golden["gm_libm"]["eval_count_total"]

@@ -1,0 +1,78 @@
+"""Product topology and colouring (gm_topology.cpp, host only) against the
+oracle's literal restatement of Mesh::compileNeighborhoods and
+NodeGroups::build: bit-exact permutation, patch lists, neighbour lists,
+renumbered elements and colour groups."""
+import numpy as np
+import pytest
+
+from gpumesh_amd import cube_mesh
+from gpumesh_amd.mesh import GpuMesh
+
+
+def _compare(oracle, pos, fixed, tets, hexs):
+    om = oracle.OracleMesh(pos, fixed, tets, hexs)
+    gm = GpuMesh(pos, fixed, tets, hexs, device=False)
+    assert np.array_equal(om.permutation(), gm.permutation())
+    eo1, er1, no1, nb1 = om.topology()
+    eo2, er2, no2, nb2 = gm.topology()
+    # oracle packs (id << 5) | (type << 3) | local; product (id << 4) | (is_hex << 3) | local
+    er1c = ((er1 >> 5) << 4) | ((((er1 >> 3) & 3) == 3).astype(np.uint64) << 3) | (er1 & 7)
+    assert np.array_equal(eo1, eo2.astype(np.int64))
+    assert np.array_equal(er1c.astype(np.uint32), er2)
+    assert np.array_equal(no1, no2.astype(np.int64))
+    assert np.array_equal(nb1, nb2)
+    t1, h1 = om.elements()
+    t2, h2 = gm.elements()
+    assert np.array_equal(t1, t2) and np.array_equal(h1, h2)
+    groups, tr = om.groups()
+    off, nodes = gm.groups()
+    assert gm.ngroups == len(groups)
+    nfixed = tr[0, 1]
+    for g in range(gm.ngroups):
+        if groups[g, 5] > groups[g, 4]:
+            assert off[g] + nfixed == groups[g, 4] and off[g + 1] + nfixed == groups[g, 5]
+        else:
+            assert off[g] == off[g + 1]
+    return om, gm
+
+
+@pytest.mark.parametrize("kind,n", [("tet", 1), ("tet", 5), ("tet", 12), ("hex", 1), ("hex", 7), ("hex", 15)])
+def test_cube_meshes(oracle, kind, n):
+    m = cube_mesh(n, kind, 0.3, seed=n)
+    _compare(oracle, m.pos, m.fixed, m.tets, m.hexs)
+
+
+def test_shuffled_numbering_and_mixed_elements(oracle):
+    """Random vertex ids, random element order, tets and hexes together,
+    free boundary vertices: exercises BFS restarts, ties in the sort and
+    colours used only by fixed vertices."""
+    rng = np.random.default_rng(11)
+    t = cube_mesh(6, "tet", 0.0, seed=1)
+    h = cube_mesh(6, "hex", 0.0, seed=1)
+    cx_t = t.pos[t.tets].mean(axis=1)[:, 0]
+    cx_h = h.pos[h.hexs].mean(axis=1)[:, 0]
+    tets = t.tets[cx_t > 0.5]
+    hexs = h.hexs[cx_h < 0.5]
+    perm = rng.permutation(t.nverts)
+    inv = np.empty_like(perm)
+    inv[perm] = np.arange(t.nverts)
+    pos = t.pos[perm]
+    fixed = (rng.uniform(size=t.nverts) < 0.2).astype(np.int32)
+    tets = inv[tets][rng.permutation(len(tets))].astype(np.uint32)
+    hexs = inv[hexs][rng.permutation(len(hexs))].astype(np.uint32)
+    _compare(oracle, pos, fixed, tets, hexs)
+
+
+def test_disconnected_components(oracle):
+    a = cube_mesh(3, "tet", 0.1, seed=2)
+    b = cube_mesh(2, "hex", 0.1, seed=3)
+    pos = np.concatenate([a.pos, b.pos + 5.0])
+    fixed = np.concatenate([a.fixed, b.fixed])
+    hexs = b.hexs + a.nverts
+    _compare(oracle, pos, fixed, a.tets, hexs.astype(np.uint32))
+
+
+def test_large_kuhn_cube_groups(oracle):
+    m = cube_mesh(40, "tet", 0.35, seed=2)
+    om, gm = _compare(oracle, m.pos, m.fixed, m.tets, m.hexs)
+    assert gm.max_patch == 24 and gm.nmovable == 39 ** 3
