@@ -23,7 +23,9 @@ EXPORTS = [
     "gm_last_error", "gm_version", "gm_device_count", "gm_mesh_create", "gm_mesh_create_host",
     "gm_mesh_destroy", "gm_mesh_info", "gm_mesh_permutation", "gm_mesh_groups", "gm_mesh_elements",
     "gm_mesh_topology", "gm_mesh_get_positions", "gm_mesh_set_positions", "gm_mesh_set_crew",
-    "gm_mesh_evaluate", "gm_mesh_smooth", "gm_mesh_smooth_groups", "gm_mesh_last_timing", "gm_mesh_kernel_times", "gm_math_probe",
+    "gm_mesh_evaluate", "gm_mesh_smooth", "gm_mesh_smooth_groups", "gm_mesh_last_timing", "gm_mesh_kernel_times",
+    "gm_mesh_device_bytes", "gm_batch_create", "gm_batch_destroy", "gm_batch_set_shape", "gm_batch_smooth",
+    "gm_batch_last_timing", "gm_batch_kernel_times", "gm_batch_autotune", "gm_math_probe",
 ]
 
 
@@ -62,6 +64,14 @@ def lib():
     L.gm_mesh_smooth_groups.argtypes = [vp, C.c_int, i64, i64, C.c_int, C.c_int, dbl, vp]
     L.gm_mesh_last_timing.argtypes = [vp, vp]
     L.gm_mesh_kernel_times.argtypes = [vp, vp, vp]
+    L.gm_mesh_device_bytes.argtypes = [vp, vp]
+    L.gm_batch_create.argtypes = [vp, i64, C.POINTER(vp)]
+    L.gm_batch_destroy.argtypes = [vp]
+    L.gm_batch_set_shape.argtypes = [vp, C.c_int, C.c_int, C.c_int]
+    L.gm_batch_smooth.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, dbl, C.c_int, vp, vp]
+    L.gm_batch_last_timing.argtypes = [vp, vp]
+    L.gm_batch_kernel_times.argtypes = [vp, vp, vp]
+    L.gm_batch_autotune.argtypes = [vp, C.c_int, C.c_int, C.c_int, vp, vp]
     L.gm_math_probe.argtypes = [C.c_int, vp, vp, vp, i64]
     for name in EXPORTS:
         if name not in ("gm_last_error", "gm_version"):

@@ -1,288 +1,111 @@
-// api.hip -- host runtime behind the C ABI declared in include/gpumesh.h.
+// api.hip -- the C ABI of include/gpumesh.h: mesh handles (a plan of one
+// mesh each), batches (a plan of many) and the launch-shape autotuner.
 #include <hip/hip_runtime.h>
 
-#include <cstdio>
+#include <algorithm>
 #include <cstring>
 #include <string>
 #include <vector>
 
-#include "../../include/gpumesh.h"
-#include "gm_device.h"
-#include "gm_topology.h"
+#include "gm_runtime.h"
 
-namespace
-{
-
-thread_local std::string g_lastError;
-
-int fail(int code, const std::string& msg)
-{
-    g_lastError = msg;
-    return code;
-}
+using gmr::fail;
 
 #define GM_HIP(call)                                                                   \
     do {                                                                               \
         hipError_t e_ = (call);                                                        \
         if(e_ != hipSuccess)                                                           \
-            return fail(GM_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+            return fail(e_ == hipErrorOutOfMemory ? GM_ERR_NOMEM : GM_ERR_HIP,         \
+                        std::string(#call) + ": " + hipGetErrorString(e_));            \
     } while(0)
 
-template<typename X> hipError_t devAlloc(X** p, size_t count)
-{
-    *p = nullptr;
-    if(count == 0)
-        count = 1;
-    return hipMalloc((void**)p, count * sizeof(X));
-}
+#define GM_NEED_DEVICE(m)                                                                                   \
+    do {                                                                                                    \
+        if(!(m)) return fail(GM_ERR_ARG, "null mesh");                                                      \
+        if(!(m)->onDevice) return fail(GM_ERR_ARG, "mesh has no device copy (created with gm_mesh_create_host)"); \
+    } while(0)
 
-} // namespace
-
-struct gm_mesh
+struct gm_batch
 {
-    bool onDevice = false;   // false: topology only (gm_mesh_create_host)
     int device = 0;
     hipStream_t stream = nullptr;
-    gmt::Topology topo;
-    gmd::DevMesh dm{};
-    uint32_t* groupNodes = nullptr;
-    int32_t* evalCounts = nullptr;
-    void* getmeTet = nullptr;
-    void* getmeHex = nullptr;
-    size_t getmeBytes = 0;
-    gmd::EvalScratch es{};
-    int resultCap = 0;
-    double* elemQual = nullptr;
-    // crew
-    int evaluator = gmq::EV_MEAN_RATIO;
-    int sampler = gmq::SA_UNIFORM;
-    gmq::Crew crew{};
-    float* texels = nullptr;
-    int maxPatch = 0;
-    hipEvent_t ev[2] = {nullptr, nullptr};
-    std::vector<hipEvent_t> sweepEv;
-    double lastTotalMs = 0.0, lastSweepMs = 0.0;
-    // Per-launch timing: event pairs around every kernel launch of a run,
-    // tagged by kernel class (KT_*), summed after the run.
-    std::vector<hipEvent_t> pool;
-    std::vector<int> poolTag;
-    size_t poolUsed = 0;
-    double kernMs[GM_KT_COUNT] = {0};
-    int64_t kernLaunches[GM_KT_COUNT] = {0};
+    gmr::Plan* plan = nullptr;
+    // Launch shape per smoother; unset (ppg 0) = the default shape.
+    gmd::LaneShape lane[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
 };
 
 namespace
 {
 
-// Opens a timed launch of class `tag`; returns the pool slot (or -1).
-int timedBegin(gm_mesh* m, int tag)
+template<typename X> hipError_t devAlloc(X** p, size_t count)
 {
-    if(m->poolUsed + 2 > m->pool.size())
-    {
-        size_t old = m->pool.size();
-        m->pool.resize(old + 256, nullptr);
-        m->poolTag.resize(old + 256, -1);
-        for(size_t i = old; i < m->pool.size(); ++i)
-            if(hipEventCreate(&m->pool[i]) != hipSuccess) { m->pool.resize(i); m->poolTag.resize(i); return -1; }
-        if(m->poolUsed + 2 > m->pool.size()) return -1;
-    }
-    int slot = (int)m->poolUsed;
-    m->poolUsed += 2;
-    m->poolTag[slot] = tag;
-    (void)hipEventRecord(m->pool[slot], m->stream);
-    return slot;
+    *p = nullptr;
+    return hipMalloc((void**)p, (count ? count : 1) * sizeof(X));
 }
 
-void timedEnd(gm_mesh* m, int slot)
+void freeMeshBuffers(gm_mesh* m)
 {
-    if(slot >= 0) (void)hipEventRecord(m->pool[slot + 1], m->stream);
+    gmd::DevMesh& d = m->dm;
+    void* ptrs[] = {d.pos64, d.pos32, (void*)d.tets, (void*)d.hexs, (void*)d.eoff, (void*)d.eref,
+                    (void*)d.noff, (void*)d.nbr, (void*)d.fixed, d.evalCounts, d.getmeTet, d.getmeHex,
+                    d.evalPartials, d.evalFlags, d.evalHist, d.evalResult, d.evalHistOut, d.passSlot,
+                    m->elemQualBuf, m->texels};
+    for(void* p : ptrs)
+        if(p) (void)hipFree(p);
+    d = gmd::DevMesh{};
+    m->elemQualBuf = nullptr;
+    m->texels = nullptr;
 }
 
-// After the stream has drained: sums the durations per class.
-void collectTimings(gm_mesh* m)
-{
-    for(int k = 0; k < GM_KT_COUNT; ++k) { m->kernMs[k] = 0.0; m->kernLaunches[k] = 0; }
-    for(size_t i = 0; i + 1 < m->poolUsed; i += 2)
-    {
-        float ms = 0.0f;
-        if(hipEventElapsedTime(&ms, m->pool[i], m->pool[i + 1]) == hipSuccess && m->poolTag[i] >= 0)
-        {
-            m->kernMs[m->poolTag[i]] += ms;
-            m->kernLaunches[m->poolTag[i]] += 1;
-        }
-    }
-    m->poolUsed = 0;
-}
-
-void freeMesh(gm_mesh* m)
+void destroyMesh(gm_mesh* m)
 {
     if(!m) return;
-    if(!m->onDevice) { delete m; return; }
-    (void)hipSetDevice(m->device);
-    if(m->stream) (void)hipStreamSynchronize(m->stream);
-    (void)hipFree(m->dm.pos64); (void)hipFree(m->dm.pos32); (void)hipFree((void*)m->dm.tets); (void)hipFree((void*)m->dm.hexs);
-    (void)hipFree((void*)m->dm.eoff); (void)hipFree((void*)m->dm.eref); (void)hipFree((void*)m->dm.noff); (void)hipFree((void*)m->dm.nbr);
-    (void)hipFree((void*)m->dm.fixed); (void)hipFree(m->groupNodes); (void)hipFree(m->evalCounts);
-    (void)hipFree(m->getmeTet); (void)hipFree(m->getmeHex);
-    (void)hipFree(m->es.partials); (void)hipFree(m->es.flags); (void)hipFree(m->es.result); (void)hipFree(m->es.hist);
-    (void)hipFree(m->es.histOut); (void)hipFree(m->elemQual); (void)hipFree(m->texels);
-    for(hipEvent_t e : m->ev) if(e) (void)hipEventDestroy(e);
-    for(hipEvent_t e : m->sweepEv) if(e) (void)hipEventDestroy(e);
-    for(hipEvent_t e : m->pool) if(e) (void)hipEventDestroy(e);
-    if(m->stream) (void)hipStreamDestroy(m->stream);
+    if(m->onDevice)
+    {
+        (void)hipSetDevice(m->device);
+        if(m->stream) (void)hipStreamSynchronize(m->stream);
+        gmr::freePlan(m->plan);
+        freeMeshBuffers(m);
+        if(m->stream) (void)hipStreamDestroy(m->stream);
+    }
     delete m;
 }
 
-int ensureResults(gm_mesh* m, int records)
+int checkRun(int smoother, int passes, int precision, int kernel, double spawnCoeff)
 {
-    if(records <= m->resultCap)
-        return GM_OK;
-    (void)hipFree(m->es.result);
-    (void)hipFree(m->es.histOut);
-    m->es.result = nullptr;
-    m->es.histOut = nullptr;
-    GM_HIP(devAlloc(&m->es.result, 3 * (size_t)records));
-    GM_HIP(devAlloc(&m->es.histOut, 20 * (size_t)records));
-    m->resultCap = records;
+    if(smoother < 0 || smoother > 3) return fail(GM_ERR_ARG, "unknown smoother");
+    if(passes < 0) return fail(GM_ERR_ARG, "negative pass count");
+    if(precision < 0 || precision > 2) return fail(GM_ERR_ARG, "unknown precision");
+    if(kernel < 0 || kernel > 1) return fail(GM_ERR_ARG, "unknown kernel shape");
+    if(!(spawnCoeff > 0.0)) return fail(GM_ERR_ARG, "spawn coefficient must be positive");
     return GM_OK;
 }
 
-int ensureGetme(gm_mesh* m, size_t scalarBytes)
+int checkSampler(int sampler, int smoother, int kernel)
 {
-    size_t need = (4 * (size_t)m->topo.ntets + 8 * (size_t)m->topo.nhexs) * 4 * scalarBytes;
-    if(need <= m->getmeBytes)
-        return GM_OK;
-    (void)hipFree(m->getmeTet);
-    (void)hipFree(m->getmeHex);
-    m->getmeTet = m->getmeHex = nullptr;
-    size_t tetBytes = 4 * (size_t)m->topo.ntets * 4 * 8;   // sized for FP64
-    size_t hexBytes = 8 * (size_t)m->topo.nhexs * 4 * 8;
-    GM_HIP(hipMalloc(&m->getmeTet, tetBytes ? tetBytes : 32));
-    GM_HIP(hipMalloc(&m->getmeHex, hexBytes ? hexBytes : 32));
-    m->getmeBytes = tetBytes + hexBytes;
+    if(sampler == GM_TEXTURE && (kernel == GM_KERNEL_PLAIN || smoother == gmd::SM_GETME))
+        return fail(GM_ERR_UNSUPPORTED, "the texture sampler runs only in the wavefront kernels of the vertex-wise "
+                                        "smoothers (GM_KERNEL_WAVE with Nelder-Mead, gradient descent or spawn search)");
     return GM_OK;
 }
 
-int evaluate(gm_mesh* m, bool fromF32, int record)
+// Default launch shape of the candidate-parallel kernels: one vertex per
+// wavefront with the smallest tile that holds the largest patch (spawn
+// search); 8 vertices per wavefront, tile / 8 slots each (gradient descent).
+gmd::LaneShape defaultLane(const std::vector<gm_mesh*>& meshes, int smoother)
 {
-    int slot = timedBegin(m, GM_KT_EVALUATE);
-    hipError_t e = fromF32
-        ? gmd::launchEvaluate<float>(m->dm, m->crew, m->evaluator, m->sampler, m->es, record, m->stream)
-        : gmd::launchEvaluate<double>(m->dm, m->crew, m->evaluator, m->sampler, m->es, record, m->stream);
-    timedEnd(m, slot);
-    if(e != hipSuccess)
-        return fail(GM_ERR_HIP, std::string("evaluate: ") + hipGetErrorString(e));
-    return GM_OK;
+    int maxPatch = 0;
+    for(const gm_mesh* m : meshes) maxPatch = std::max(maxPatch, m->maxPatch);
+    gmd::LaneShape l{1, 64};
+    if(smoother == gmd::SM_GRADIENT_DESCENT)
+        l.tileElems = maxPatch <= 8 ? 64 : (maxPatch <= 16 ? 128 : 256);
+    else
+        l.tileElems = maxPatch <= 64 ? 64 : (maxPatch <= 128 ? 128 : 256);
+    return l;
 }
 
-} // namespace
-
-extern "C" {
-
-const char* gm_last_error(void) { return g_lastError.c_str(); }
-
-const char* gm_version(void) { return "gpumesh_amd 0.1 (gfx950)"; }
-
-int gm_device_count(int* count)
-{
-    if(!count) return fail(GM_ERR_ARG, "count is null");
-    int n = 0;
-    hipError_t e = hipGetDeviceCount(&n);
-    *count = (e == hipSuccess) ? n : 0;
-    return GM_OK;
-}
-
-int gm_mesh_create(int64_t nverts, const double* pos, const int32_t* fixed,
-                   int64_t ntets, const uint32_t* tets, int64_t nhexs, const uint32_t* hexs,
-                   gm_mesh** out)
-{
-    if(!out) return fail(GM_ERR_ARG, "out is null");
-    *out = nullptr;
-    if(nverts < 0 || ntets < 0 || nhexs < 0) return fail(GM_ERR_ARG, "negative size");
-    if((nverts && (!pos || !fixed)) || (ntets && !tets) || (nhexs && !hexs)) return fail(GM_ERR_ARG, "null array");
-    int ndev = 0;
-    if(hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
-        return fail(GM_ERR_HIP, "no HIP device visible");
-
-    gm_mesh* m = new gm_mesh();
-    std::string err;
-    if(!gmt::build(nverts, pos, fixed, ntets, tets, nhexs, hexs, m->topo, err))
-    {
-        delete m;
-        return fail(GM_ERR_ARG, err);
-    }
-    const gmt::Topology& t = m->topo;
-    for(int64_t v = 0; v < nverts; ++v)
-        if(!t.fixed[v] && t.eoff[v + 1] == t.eoff[v])
-        {
-            delete m;
-            return fail(GM_ERR_ARG, "a free vertex belongs to no element");
-        }
-    for(int32_t p : t.groupMaxPatch) m->maxPatch = std::max(m->maxPatch, (int)p);
-
-    auto cleanupFail = [&](int code) { freeMesh(m); return code; };
-#define GM_TRY(call) do { hipError_t e_ = (call); if(e_ != hipSuccess) { fail(GM_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); return cleanupFail(e_ == hipErrorOutOfMemory ? GM_ERR_NOMEM : GM_ERR_HIP); } } while(0)
-
-    m->onDevice = true;
-    GM_TRY(hipGetDevice(&m->device));
-    GM_TRY(hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking));
-    GM_TRY(hipEventCreate(&m->ev[0]));
-    GM_TRY(hipEventCreate(&m->ev[1]));
-
-    const size_t V = (size_t)nverts;
-    std::vector<double> p64(4 * V);
-    std::vector<float> p32(4 * V);
-    for(size_t v = 0; v < V; ++v)
-    {
-        for(int d = 0; d < 3; ++d) { p64[4 * v + d] = t.pos[3 * v + d]; p32[4 * v + d] = (float)t.pos[3 * v + d]; }
-        p64[4 * v + 3] = 0.0; p32[4 * v + 3] = 0.0f;
-    }
-    double4* pos64; float4* pos32; uint4* tb; uint4* hb; uint32_t *eo, *er, *no, *nb; uint8_t* fx;
-    GM_TRY(devAlloc(&pos64, V)); m->dm.pos64 = pos64;
-    GM_TRY(devAlloc(&pos32, V)); m->dm.pos32 = pos32;
-    GM_TRY(devAlloc(&tb, (size_t)ntets)); m->dm.tets = tb;
-    GM_TRY(devAlloc(&hb, 2 * (size_t)nhexs)); m->dm.hexs = hb;
-    GM_TRY(devAlloc(&eo, V + 1)); m->dm.eoff = eo;
-    GM_TRY(devAlloc(&er, t.eref.size())); m->dm.eref = er;
-    GM_TRY(devAlloc(&no, V + 1)); m->dm.noff = no;
-    GM_TRY(devAlloc(&nb, t.nbr.size())); m->dm.nbr = nb;
-    GM_TRY(devAlloc(&fx, V)); m->dm.fixed = fx;
-    GM_TRY(devAlloc(&m->groupNodes, t.groupNodes.size()));
-    GM_TRY(devAlloc(&m->evalCounts, V));
-    GM_TRY(devAlloc(&m->es.partials, 3 * (size_t)gmd::EVAL_BLOCKS));
-    GM_TRY(devAlloc(&m->es.flags, (size_t)gmd::EVAL_BLOCKS));
-    GM_TRY(devAlloc(&m->es.hist, 20));
-
-    hipStream_t s = m->stream;
-    if(V)
-    {
-        GM_TRY(hipMemcpyAsync(pos64, p64.data(), V * sizeof(double4), hipMemcpyHostToDevice, s));
-        GM_TRY(hipMemcpyAsync(pos32, p32.data(), V * sizeof(float4), hipMemcpyHostToDevice, s));
-        GM_TRY(hipMemcpyAsync(eo, t.eoff.data(), (V + 1) * 4, hipMemcpyHostToDevice, s));
-        GM_TRY(hipMemcpyAsync(no, t.noff.data(), (V + 1) * 4, hipMemcpyHostToDevice, s));
-        GM_TRY(hipMemcpyAsync(fx, t.fixed.data(), V, hipMemcpyHostToDevice, s));
-    }
-    if(ntets) GM_TRY(hipMemcpyAsync(tb, t.tets.data(), (size_t)ntets * 16, hipMemcpyHostToDevice, s));
-    if(nhexs) GM_TRY(hipMemcpyAsync(hb, t.hexs.data(), (size_t)nhexs * 32, hipMemcpyHostToDevice, s));
-    if(!t.eref.empty()) GM_TRY(hipMemcpyAsync(er, t.eref.data(), t.eref.size() * 4, hipMemcpyHostToDevice, s));
-    if(!t.nbr.empty()) GM_TRY(hipMemcpyAsync(nb, t.nbr.data(), t.nbr.size() * 4, hipMemcpyHostToDevice, s));
-    if(!t.groupNodes.empty())
-        GM_TRY(hipMemcpyAsync(m->groupNodes, t.groupNodes.data(), t.groupNodes.size() * 4, hipMemcpyHostToDevice, s));
-    GM_TRY(hipMemsetAsync(m->es.hist, 0, 20 * sizeof(int32_t), s));
-    GM_TRY(hipStreamSynchronize(s));
-#undef GM_TRY
-
-    m->dm.nverts = (uint32_t)nverts;
-    m->dm.ntets = (uint32_t)ntets;
-    m->dm.nhexs = (uint32_t)nhexs;
-    m->crew.scaling = m->crew.scaling2 = m->crew.scaling3 = 1.0;
-    *out = m;
-    return GM_OK;
-}
-
-int gm_mesh_create_host(int64_t nverts, const double* pos, const int32_t* fixed,
-                        int64_t ntets, const uint32_t* tets, int64_t nhexs, const uint32_t* hexs,
-                        gm_mesh** out)
+int buildTopology(int64_t nverts, const double* pos, const int32_t* fixed, int64_t ntets, const uint32_t* tets,
+                  int64_t nhexs, const uint32_t* hexs, gm_mesh** out, gm_mesh** made)
 {
     if(!out) return fail(GM_ERR_ARG, "out is null");
     *out = nullptr;
@@ -296,13 +119,160 @@ int gm_mesh_create_host(int64_t nverts, const double* pos, const int32_t* fixed,
         return fail(GM_ERR_ARG, err);
     }
     for(int32_t p : m->topo.groupMaxPatch) m->maxPatch = std::max(m->maxPatch, (int)p);
+    m->kind = m->topo.nhexs == 0 ? gmd::EK_TET : (m->topo.ntets == 0 ? gmd::EK_HEX : gmd::EK_MIXED);
+    m->crew.scaling = m->crew.scaling2 = m->crew.scaling3 = 1.0;
+    *made = m;
+    return GM_OK;
+}
+
+int runAndFetch(gmr::Plan* p, const gm_mesh* m0, const gmr::RunParams& r, double* stats, int32_t* hist,
+                int32_t* evalCounts)
+{
+    int rc = gmr::runPlan(p, m0->crew, m0->evaluator, m0->sampler, r);
+    if(rc) return rc;
+    const size_t per = (size_t)r.passes + 1;
+    for(size_t i = 0; i < p->meshes.size(); ++i)
+        if((rc = gmr::fetchRecords(p, i, (int)per, stats ? stats + i * 3 * per : nullptr,
+                                   hist ? hist + i * 20 * per : nullptr, evalCounts)))
+            return rc;
+    return GM_OK;
+}
+
+double sweepMs(const gmr::Plan* p)
+{
+    double s = 0.0;
+    for(int k = 0; k < GM_KT_EVALUATE; ++k) s += p->kernMs[k];
+    return s;
+}
+
+} // namespace
+
+extern "C" {
+
+const char* gm_last_error(void) { return gmr::lastError(); }
+
+const char* gm_version(void) { return "gpumesh_amd 0.2 (gfx950)"; }
+
+int gm_device_count(int* count)
+{
+    if(!count) return fail(GM_ERR_ARG, "count is null");
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    *count = (e == hipSuccess) ? n : 0;
+    return GM_OK;
+}
+
+int gm_mesh_create_host(int64_t nverts, const double* pos, const int32_t* fixed,
+                        int64_t ntets, const uint32_t* tets, int64_t nhexs, const uint32_t* hexs,
+                        gm_mesh** out)
+{
+    gm_mesh* m = nullptr;
+    int rc = buildTopology(nverts, pos, fixed, ntets, tets, nhexs, hexs, out, &m);
+    if(rc) return rc;
+    *out = m;
+    return GM_OK;
+}
+
+int gm_mesh_create(int64_t nverts, const double* pos, const int32_t* fixed,
+                   int64_t ntets, const uint32_t* tets, int64_t nhexs, const uint32_t* hexs,
+                   gm_mesh** out)
+{
+    if(!out) return fail(GM_ERR_ARG, "out is null");
+    *out = nullptr;
+    int ndev = 0;
+    if(hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+        return fail(GM_ERR_HIP, "no HIP device visible");
+    gm_mesh* m = nullptr;
+    int rc = buildTopology(nverts, pos, fixed, ntets, tets, nhexs, hexs, out, &m);
+    if(rc) return rc;
+    const gmt::Topology& t = m->topo;
+    for(int64_t v = 0; v < nverts; ++v)
+        if(!t.fixed[v] && t.eoff[v + 1] == t.eoff[v])
+        {
+            delete m;
+            return fail(GM_ERR_ARG, "a free vertex belongs to no element");
+        }
+    if((uint64_t)nverts >= (1ull << 32))
+    {
+        delete m;
+        return fail(GM_ERR_ARG, "too many vertices for 32-bit ids");
+    }
+
+    m->onDevice = true;
+    auto bail = [&](int code) { destroyMesh(m); return code; };
+#define GM_TRY(call)                                                                                    \
+    do {                                                                                                \
+        hipError_t e_ = (call);                                                                         \
+        if(e_ != hipSuccess)                                                                            \
+            return bail(fail(e_ == hipErrorOutOfMemory ? GM_ERR_NOMEM : GM_ERR_HIP,                     \
+                             std::string(#call) + ": " + hipGetErrorString(e_)));                       \
+    } while(0)
+    GM_TRY(hipGetDevice(&m->device));
+    GM_TRY(hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking));
+
+    const size_t V = (size_t)nverts;
+    gmd::DevMesh& d = m->dm;
+    d.nverts = (uint32_t)nverts;
+    d.ntets = (uint32_t)ntets;
+    d.nhexs = (uint32_t)nhexs;
+    d.evalBlocks = gmd::evalBlocksFor((uint64_t)ntets + (uint64_t)nhexs);
+    std::vector<double> p64(4 * V);
+    std::vector<float> p32(4 * V);
+    for(size_t v = 0; v < V; ++v)
+    {
+        for(int k = 0; k < 3; ++k)
+        {
+            p64[4 * v + k] = t.pos[3 * v + k];
+            p32[4 * v + k] = (float)t.pos[3 * v + k];
+        }
+        p64[4 * v + 3] = 0.0;
+        p32[4 * v + 3] = 0.0f;
+    }
+    uint4 *tb, *hb;
+    uint32_t *eo, *er, *no, *nb;
+    uint8_t* fx;
+    GM_TRY(devAlloc(&d.pos64, V));
+    GM_TRY(devAlloc(&d.pos32, V));
+    GM_TRY(devAlloc(&tb, (size_t)ntets));      d.tets = tb;
+    GM_TRY(devAlloc(&hb, 2 * (size_t)nhexs));  d.hexs = hb;
+    GM_TRY(devAlloc(&eo, V + 1));              d.eoff = eo;
+    GM_TRY(devAlloc(&er, t.eref.size()));      d.eref = er;
+    GM_TRY(devAlloc(&no, V + 1));              d.noff = no;
+    GM_TRY(devAlloc(&nb, t.nbr.size()));       d.nbr = nb;
+    GM_TRY(devAlloc(&fx, V));                  d.fixed = fx;
+    GM_TRY(devAlloc(&d.evalCounts, V));
+    GM_TRY(devAlloc(&d.evalPartials, 3 * (size_t)d.evalBlocks));
+    GM_TRY(devAlloc(&d.evalFlags, (size_t)d.evalBlocks));
+    GM_TRY(devAlloc(&d.evalHist, 20));
+    GM_TRY(devAlloc(&d.passSlot, 1));
+    m->deviceBytes = V * (sizeof(double4) + sizeof(float4) + 4 + 4 + 4 + 1) + (size_t)ntets * 16 +
+                     (size_t)nhexs * 32 + t.eref.size() * 4 + t.nbr.size() * 4 +
+                     (size_t)d.evalBlocks * 28 + 84;
+
+    hipStream_t s = m->stream;
+    if(V)
+    {
+        GM_TRY(hipMemcpyAsync(d.pos64, p64.data(), V * sizeof(double4), hipMemcpyHostToDevice, s));
+        GM_TRY(hipMemcpyAsync(d.pos32, p32.data(), V * sizeof(float4), hipMemcpyHostToDevice, s));
+        GM_TRY(hipMemcpyAsync(eo, t.eoff.data(), (V + 1) * 4, hipMemcpyHostToDevice, s));
+        GM_TRY(hipMemcpyAsync(no, t.noff.data(), (V + 1) * 4, hipMemcpyHostToDevice, s));
+        GM_TRY(hipMemcpyAsync(fx, t.fixed.data(), V, hipMemcpyHostToDevice, s));
+    }
+    if(ntets) GM_TRY(hipMemcpyAsync(tb, t.tets.data(), (size_t)ntets * 16, hipMemcpyHostToDevice, s));
+    if(nhexs) GM_TRY(hipMemcpyAsync(hb, t.hexs.data(), (size_t)nhexs * 32, hipMemcpyHostToDevice, s));
+    if(!t.eref.empty()) GM_TRY(hipMemcpyAsync(er, t.eref.data(), t.eref.size() * 4, hipMemcpyHostToDevice, s));
+    if(!t.nbr.empty()) GM_TRY(hipMemcpyAsync(nb, t.nbr.data(), t.nbr.size() * 4, hipMemcpyHostToDevice, s));
+    GM_TRY(hipStreamSynchronize(s));
+#undef GM_TRY
+    std::vector<gm_mesh*> one{m};
+    if((rc = gmr::buildPlan(one, m->stream, &m->plan))) return bail(rc);
     *out = m;
     return GM_OK;
 }
 
 int gm_mesh_destroy(gm_mesh* mesh)
 {
-    freeMesh(mesh);
+    destroyMesh(mesh);
     return GM_OK;
 }
 
@@ -310,9 +280,21 @@ int gm_mesh_info(const gm_mesh* m, int64_t* info)
 {
     if(!m || !info) return fail(GM_ERR_ARG, "null argument");
     const gmt::Topology& t = m->topo;
-    info[0] = t.nverts; info[1] = t.ntets; info[2] = t.nhexs;
-    info[3] = (int64_t)t.groupNodes.size(); info[4] = t.ngroups();
-    info[5] = (int64_t)t.eref.size(); info[6] = (int64_t)t.nbr.size(); info[7] = m->maxPatch;
+    info[0] = t.nverts;
+    info[1] = t.ntets;
+    info[2] = t.nhexs;
+    info[3] = (int64_t)t.groupNodes.size();
+    info[4] = t.ngroups();
+    info[5] = (int64_t)t.eref.size();
+    info[6] = (int64_t)t.nbr.size();
+    info[7] = m->maxPatch;
+    return GM_OK;
+}
+
+int gm_mesh_device_bytes(const gm_mesh* m, int64_t* bytes)
+{
+    if(!m || !bytes) return fail(GM_ERR_ARG, "null argument");
+    *bytes = m->onDevice ? (int64_t)m->deviceBytes : 0;
     return GM_OK;
 }
 
@@ -352,29 +334,35 @@ int gm_mesh_topology(const gm_mesh* m, uint32_t* eoff, uint32_t* eref, uint32_t*
 
 int gm_mesh_get_positions(gm_mesh* m, double* pos)
 {
-    if(!m || !pos) return fail(GM_ERR_ARG, "null argument");
-    if(!m->onDevice) return fail(GM_ERR_ARG, "mesh has no device copy (created with gm_mesh_create_host)");
+    GM_NEED_DEVICE(m);
+    if(!pos) return fail(GM_ERR_ARG, "null argument");
     GM_HIP(hipSetDevice(m->device));
-    size_t V = (size_t)m->topo.nverts;
+    const size_t V = (size_t)m->topo.nverts;
     std::vector<double> p64(4 * V);
     if(V) GM_HIP(hipMemcpyAsync(p64.data(), m->dm.pos64, V * sizeof(double4), hipMemcpyDeviceToHost, m->stream));
     GM_HIP(hipStreamSynchronize(m->stream));
-    for(size_t v = 0; v < V; ++v) for(int d = 0; d < 3; ++d) pos[3 * v + d] = p64[4 * v + d];
+    for(size_t v = 0; v < V; ++v)
+        for(int k = 0; k < 3; ++k) pos[3 * v + k] = p64[4 * v + k];
     return GM_OK;
 }
 
 int gm_mesh_set_positions(gm_mesh* m, const double* pos)
 {
-    if(!m || !pos) return fail(GM_ERR_ARG, "null argument");
-    if(!m->onDevice) return fail(GM_ERR_ARG, "mesh has no device copy (created with gm_mesh_create_host)");
+    GM_NEED_DEVICE(m);
+    if(!pos) return fail(GM_ERR_ARG, "null argument");
     GM_HIP(hipSetDevice(m->device));
-    size_t V = (size_t)m->topo.nverts;
+    const size_t V = (size_t)m->topo.nverts;
     std::vector<double> p64(4 * V);
     std::vector<float> p32(4 * V);
     for(size_t v = 0; v < V; ++v)
     {
-        for(int d = 0; d < 3; ++d) { p64[4 * v + d] = pos[3 * v + d]; p32[4 * v + d] = (float)pos[3 * v + d]; }
-        p64[4 * v + 3] = 0.0; p32[4 * v + 3] = 0.0f;
+        for(int k = 0; k < 3; ++k)
+        {
+            p64[4 * v + k] = pos[3 * v + k];
+            p32[4 * v + k] = (float)pos[3 * v + k];
+        }
+        p64[4 * v + 3] = 0.0;
+        p32[4 * v + 3] = 0.0f;
     }
     if(V)
     {
@@ -388,9 +376,8 @@ int gm_mesh_set_positions(gm_mesh* m, const double* pos)
 int gm_mesh_set_crew(gm_mesh* m, int evaluator, int sampler, double scaling,
                      const int32_t* tsize, const double* tmin, const double* text, const float* texels)
 {
-    if(!m) return fail(GM_ERR_ARG, "null mesh");
+    GM_NEED_DEVICE(m);
     if(evaluator < 0 || evaluator > 1) return fail(GM_ERR_ARG, "unknown evaluator");
-    if(!m->onDevice) return fail(GM_ERR_ARG, "mesh has no device copy (created with gm_mesh_create_host)");
     if(sampler < 0 || sampler > 2) return fail(GM_ERR_ARG, "unknown sampler");
     if(!(scaling > 0.0)) return fail(GM_ERR_ARG, "scaling must be positive");
     GM_HIP(hipSetDevice(m->device));
@@ -398,16 +385,18 @@ int gm_mesh_set_crew(gm_mesh* m, int evaluator, int sampler, double scaling,
     c.scaling = scaling;
     c.scaling2 = scaling * scaling;
     c.scaling3 = c.scaling2 * scaling;
-    if(sampler == gmq::SA_TEXTURE)
+    if(sampler == GM_TEXTURE)
     {
         if(!tsize || !tmin || !text || !texels) return fail(GM_ERR_ARG, "texture sampler needs a grid");
-        for(int d = 0; d < 3; ++d)
+        for(int k = 0; k < 3; ++k)
         {
-            if(tsize[d] < 1) return fail(GM_ERR_ARG, "texture size must be >= 1");
-            if(!(text[d] > 0.0)) return fail(GM_ERR_ARG, "texture extents must be positive");
-            c.tsize[d] = tsize[d]; c.tmin[d] = tmin[d]; c.text[d] = text[d];
+            if(tsize[k] < 1) return fail(GM_ERR_ARG, "texture size must be >= 1");
+            if(!(text[k] > 0.0)) return fail(GM_ERR_ARG, "texture extents must be positive");
+            c.tsize[k] = tsize[k];
+            c.tmin[k] = tmin[k];
+            c.text[k] = text[k];
         }
-        size_t n = 6 * (size_t)tsize[0] * tsize[1] * tsize[2];
+        const size_t n = 6 * (size_t)tsize[0] * tsize[1] * tsize[2];
         (void)hipFree(m->texels);
         m->texels = nullptr;
         GM_HIP(devAlloc(&m->texels, n));
@@ -423,24 +412,27 @@ int gm_mesh_set_crew(gm_mesh* m, int evaluator, int sampler, double scaling,
 
 int gm_mesh_evaluate(gm_mesh* m, double* out3, int32_t* hist20, double* elemQual)
 {
-    if(!m) return fail(GM_ERR_ARG, "null mesh");
-    if(!m->onDevice) return fail(GM_ERR_ARG, "mesh has no device copy (created with gm_mesh_create_host)");
+    GM_NEED_DEVICE(m);
     GM_HIP(hipSetDevice(m->device));
-    int rc = ensureResults(m, 1);
+    const size_t E = (size_t)(m->topo.ntets + m->topo.nhexs);
+    if(elemQual && !m->elemQualBuf) GM_HIP(devAlloc(&m->elemQualBuf, E));
+    gmr::Plan* p = m->plan;
+    // Per-element output is switched on in the descriptor for this call only.
+    m->dm.elemQual = elemQual ? m->elemQualBuf : nullptr;
+    int rc = gmr::preparePlan(p, 1, false);
+    m->dm.elemQual = nullptr;
     if(rc) return rc;
-    size_t E = (size_t)(m->topo.ntets + m->topo.nhexs);
-    if(elemQual && !m->elemQual) GM_HIP(devAlloc(&m->elemQual, E));
-    m->es.elemQual = elemQual ? m->elemQual : nullptr;
-    m->poolUsed = 0;
-    rc = evaluate(m, false, 0);
-    m->es.elemQual = nullptr;
-    if(rc) return rc;
+    GM_HIP(gmd::launchResetRun(p->dTable, p->vertBlocks, p->nVertBlocks, p->meshList, p->nMeshes, p->stream));
+    hipError_t e = gmd::launchEvaluate<double>(p->dTable, p->evalBlocks, p->nEvalBlocks, p->meshList, p->nMeshes,
+                                               m->crew, m->evaluator, m->sampler, p->stream);
+    if(e != hipSuccess) return fail(GM_ERR_HIP, std::string("evaluate: ") + hipGetErrorString(e));
     double r[3];
     int32_t h[20];
-    GM_HIP(hipMemcpyAsync(r, m->es.result, sizeof(r), hipMemcpyDeviceToHost, m->stream));
-    GM_HIP(hipMemcpyAsync(h, m->es.histOut, sizeof(h), hipMemcpyDeviceToHost, m->stream));
-    if(elemQual && E) GM_HIP(hipMemcpyAsync(elemQual, m->elemQual, E * sizeof(double), hipMemcpyDeviceToHost, m->stream));
-    GM_HIP(hipStreamSynchronize(m->stream));
+    GM_HIP(hipMemcpyAsync(r, m->dm.evalResult, sizeof(r), hipMemcpyDeviceToHost, p->stream));
+    GM_HIP(hipMemcpyAsync(h, m->dm.evalHistOut, sizeof(h), hipMemcpyDeviceToHost, p->stream));
+    if(elemQual && E)
+        GM_HIP(hipMemcpyAsync(elemQual, m->elemQualBuf, E * sizeof(double), hipMemcpyDeviceToHost, p->stream));
+    GM_HIP(hipStreamSynchronize(p->stream));
     if(out3) std::memcpy(out3, r, sizeof(r));
     if(hist20) std::memcpy(hist20, h, sizeof(h));
     return GM_OK;
@@ -449,180 +441,254 @@ int gm_mesh_evaluate(gm_mesh* m, double* out3, int32_t* hist20, double* elemQual
 int gm_mesh_smooth(gm_mesh* m, int smoother, int passes, int precision, int kernel,
                    double spawnCoeff, double* stats, int32_t* hist, int32_t* evalCounts)
 {
-    if(!m) return fail(GM_ERR_ARG, "null mesh");
-    if(smoother < 0 || smoother > 3) return fail(GM_ERR_ARG, "unknown smoother");
-    if(!m->onDevice) return fail(GM_ERR_ARG, "mesh has no device copy (created with gm_mesh_create_host)");
-    if(passes < 0) return fail(GM_ERR_ARG, "negative pass count");
-    if(precision < 0 || precision > 2) return fail(GM_ERR_ARG, "unknown precision");
-    if(kernel < 0 || kernel > 1) return fail(GM_ERR_ARG, "unknown kernel shape");
-    if(!(spawnCoeff > 0.0)) return fail(GM_ERR_ARG, "spawn coefficient must be positive");
-    if(m->sampler == GM_TEXTURE && (kernel == GM_KERNEL_PLAIN || smoother == gmd::SM_GETME))
-        return fail(GM_ERR_UNSUPPORTED, "the texture sampler runs only in the wavefront kernels of the vertex-wise "
-                                        "smoothers (GM_KERNEL_WAVE with Nelder-Mead, gradient descent or spawn search)");
-    GM_HIP(hipSetDevice(m->device));
-    int rc = ensureResults(m, passes + 1);
+    GM_NEED_DEVICE(m);
+    int rc = checkRun(smoother, passes, precision, kernel, spawnCoeff);
     if(rc) return rc;
-    if(smoother == gmd::SM_GETME)
-    {
-        rc = ensureGetme(m, 8);
-        if(rc) return rc;
-    }
-    const gmt::Topology& t = m->topo;
-    hipStream_t s = m->stream;
-    size_t V = (size_t)t.nverts;
-    if(V) GM_HIP(hipMemsetAsync(m->evalCounts, 0, V * sizeof(int32_t), s));
-    if((int)m->sweepEv.size() < 2 * passes)
-    {
-        size_t old = m->sweepEv.size();
-        m->sweepEv.resize(2 * passes, nullptr);
-        for(size_t i = old; i < m->sweepEv.size(); ++i) GM_HIP(hipEventCreate(&m->sweepEv[i]));
-    }
-    m->poolUsed = 0;
-    GM_HIP(hipEventRecord(m->ev[0], s));
-
-    bool inF32 = false;
-    if(precision != GM_FP64)
-    {
-        GM_HIP(gmd::launchToF32(m->dm, s));
-        inF32 = true;
-    }
-    for(int pass = 0; pass <= passes; ++pass)
-    {
-        rc = evaluate(m, inF32, pass);
-        if(rc) return rc;
-        if(pass == passes)
-            break;
-        bool f64 = precision == GM_FP64 || (precision == GM_FP32_POLISH && pass == passes - 1);
-        if(f64 && inF32)
-        {
-            GM_HIP(gmd::launchToF64(m->dm, s));
-            inF32 = false;
-        }
-        GM_HIP(hipEventRecord(m->sweepEv[2 * pass], s));
-        hipError_t e = hipSuccess;
-        if(smoother == gmd::SM_GETME)
-        {
-            void* to = m->getmeTet;
-            void* ho = m->getmeHex;
-            int slot = timedBegin(m, f64 ? GM_KT_GETME_ELEM_F64 : GM_KT_GETME_ELEM_F32);
-            e = f64 ? gmd::launchGetmeElements<double>(m->dm, m->crew, m->evaluator, m->sampler, to, ho, s)
-                    : gmd::launchGetmeElements<float>(m->dm, m->crew, m->evaluator, m->sampler, to, ho, s);
-            timedEnd(m, slot);
-            for(int64_t g = 0; e == hipSuccess && g < t.ngroups(); ++g)
-            {
-                const uint32_t* nodes = m->groupNodes + t.groupOff[g];
-                uint32_t cnt = (uint32_t)(t.groupOff[g + 1] - t.groupOff[g]);
-                if(cnt == 0) continue;
-                slot = timedBegin(m, f64 ? GM_KT_GETME_VERT_F64 : GM_KT_GETME_VERT_F32);
-                e = f64 ? gmd::launchGetmeVertices<double>(m->dm, m->crew, m->evaluator, m->sampler, nodes, cnt, to, ho, m->evalCounts, s)
-                        : gmd::launchGetmeVertices<float>(m->dm, m->crew, m->evaluator, m->sampler, nodes, cnt, to, ho, m->evalCounts, s);
-                timedEnd(m, slot);
-            }
-        }
-        else
-        {
-            for(int64_t g = 0; e == hipSuccess && g < t.ngroups(); ++g)
-            {
-                const uint32_t* nodes = m->groupNodes + t.groupOff[g];
-                uint32_t cnt = (uint32_t)(t.groupOff[g + 1] - t.groupOff[g]);
-                int mp = t.groupMaxPatch[g];
-                if(cnt == 0) continue;
-                int slot = timedBegin(m, f64 ? GM_KT_SMOOTH_F64 : GM_KT_SMOOTH_F32);
-                e = f64 ? gmd::launchSmoothGroup<double>(m->dm, m->crew, m->evaluator, m->sampler, smoother, kernel, nodes, cnt, spawnCoeff, m->evalCounts, mp, s)
-                        : gmd::launchSmoothGroup<float>(m->dm, m->crew, m->evaluator, m->sampler, smoother, kernel, nodes, cnt, spawnCoeff, m->evalCounts, mp, s);
-                timedEnd(m, slot);
-            }
-        }
-        if(e != hipSuccess)
-            return fail(GM_ERR_HIP, std::string("smoothing launch: ") + hipGetErrorString(e));
-        GM_HIP(hipEventRecord(m->sweepEv[2 * pass + 1], s));
-    }
-    if(inF32)
-        GM_HIP(gmd::launchToF64(m->dm, s));
-    GM_HIP(hipEventRecord(m->ev[1], s));
-
-    std::vector<double> r(3 * (size_t)(passes + 1));
-    std::vector<int32_t> h(20 * (size_t)(passes + 1));
-    GM_HIP(hipMemcpyAsync(r.data(), m->es.result, r.size() * sizeof(double), hipMemcpyDeviceToHost, s));
-    GM_HIP(hipMemcpyAsync(h.data(), m->es.histOut, h.size() * sizeof(int32_t), hipMemcpyDeviceToHost, s));
-    if(evalCounts && V) GM_HIP(hipMemcpyAsync(evalCounts, m->evalCounts, V * sizeof(int32_t), hipMemcpyDeviceToHost, s));
-    GM_HIP(hipStreamSynchronize(s));
-
-    collectTimings(m);
-    float ms = 0.0f;
-    GM_HIP(hipEventElapsedTime(&ms, m->ev[0], m->ev[1]));
-    m->lastTotalMs = ms;
-    m->lastSweepMs = 0.0;
-    for(int p = 0; p < passes; ++p)
-    {
-        GM_HIP(hipEventElapsedTime(&ms, m->sweepEv[2 * p], m->sweepEv[2 * p + 1]));
-        m->lastSweepMs += ms;
-    }
-    for(int p = 0; p <= passes; ++p)
-    {
-        if(stats)
-        {
-            stats[3 * p] = r[3 * p];
-            stats[3 * p + 1] = r[3 * p + 2] / r[3 * p + 1];
-            stats[3 * p + 2] = r[3 * p + 1];
-        }
-        if(hist) std::memcpy(hist + 20 * p, h.data() + 20 * p, 20 * sizeof(int32_t));
-    }
-    return GM_OK;
+    if((rc = checkSampler(m->sampler, smoother, kernel))) return rc;
+    GM_HIP(hipSetDevice(m->device));
+    gmr::RunParams r;
+    r.smoother = smoother;
+    r.passes = passes;
+    r.precision = precision;
+    r.kernel = kernel;
+    r.spawnCoeff = spawnCoeff;
+    r.lane = defaultLane(m->plan->meshes, smoother);
+    return runAndFetch(m->plan, m, r, stats, hist, evalCounts);
 }
 
 int gm_mesh_smooth_groups(gm_mesh* m, int smoother, int64_t groupBegin, int64_t groupEnd, int precision,
                           int kernel, double spawnCoeff, int32_t* evalCounts)
 {
-    if(!m) return fail(GM_ERR_ARG, "null mesh");
-    if(!m->onDevice) return fail(GM_ERR_ARG, "mesh has no device copy (created with gm_mesh_create_host)");
-    if(smoother < 0 || smoother > 3) return fail(GM_ERR_ARG, "unknown smoother");
+    GM_NEED_DEVICE(m);
+    int rc = checkRun(smoother, 0, precision, kernel, spawnCoeff);
+    if(rc) return rc;
     if(smoother == gmd::SM_GETME)
         return fail(GM_ERR_UNSUPPORTED, "GETMe sweeps start with an element step over the whole mesh; use gm_mesh_smooth");
     if(precision != GM_FP64 && precision != GM_FP32) return fail(GM_ERR_ARG, "precision must be GM_FP64 or GM_FP32");
-    if(kernel < 0 || kernel > 1) return fail(GM_ERR_ARG, "unknown kernel shape");
-    if(!(spawnCoeff > 0.0)) return fail(GM_ERR_ARG, "spawn coefficient must be positive");
-    if(kernel == GM_KERNEL_PLAIN && m->sampler == GM_TEXTURE)
-        return fail(GM_ERR_UNSUPPORTED, "the texture sampler runs only in the wavefront kernels of the vertex-wise "
-                                        "smoothers (GM_KERNEL_WAVE with Nelder-Mead, gradient descent or spawn search)");
-    const gmt::Topology& t = m->topo;
-    if(groupBegin < 0 || groupEnd > t.ngroups() || groupBegin > groupEnd) return fail(GM_ERR_ARG, "bad group range");
+    if((rc = checkSampler(m->sampler, smoother, kernel))) return rc;
+    if(groupBegin < 0 || groupEnd > m->topo.ngroups() || groupBegin > groupEnd)
+        return fail(GM_ERR_ARG, "bad group range");
     GM_HIP(hipSetDevice(m->device));
-    hipStream_t s = m->stream;
-    size_t V = (size_t)t.nverts;
-    if(V) GM_HIP(hipMemsetAsync(m->evalCounts, 0, V * sizeof(int32_t), s));
-    const bool f64 = precision == GM_FP64;
-    if(!f64) GM_HIP(gmd::launchToF32(m->dm, s));
-    for(int64_t g = groupBegin; g < groupEnd; ++g)
+    gmr::RunParams r;
+    r.smoother = smoother;
+    r.precision = precision;
+    r.kernel = kernel;
+    r.spawnCoeff = spawnCoeff;
+    r.lane = defaultLane(m->plan->meshes, smoother);
+    if((rc = gmr::sweepColours(m->plan, m->crew, m->evaluator, m->sampler, r, groupBegin, groupEnd))) return rc;
+    if(evalCounts && m->topo.nverts)
     {
-        const uint32_t* nodes = m->groupNodes + t.groupOff[g];
-        uint32_t cnt = (uint32_t)(t.groupOff[g + 1] - t.groupOff[g]);
-        hipError_t e = f64 ? gmd::launchSmoothGroup<double>(m->dm, m->crew, m->evaluator, m->sampler, smoother, kernel, nodes, cnt, spawnCoeff, m->evalCounts, t.groupMaxPatch[g], s)
-                           : gmd::launchSmoothGroup<float>(m->dm, m->crew, m->evaluator, m->sampler, smoother, kernel, nodes, cnt, spawnCoeff, m->evalCounts, t.groupMaxPatch[g], s);
-        if(e != hipSuccess) return fail(GM_ERR_HIP, std::string("smoothing launch: ") + hipGetErrorString(e));
+        GM_HIP(hipMemcpyAsync(evalCounts, m->dm.evalCounts, (size_t)m->topo.nverts * 4, hipMemcpyDeviceToHost,
+                              m->stream));
+        GM_HIP(hipStreamSynchronize(m->stream));
     }
-    if(!f64) GM_HIP(gmd::launchToF64(m->dm, s));
-    if(evalCounts && V) GM_HIP(hipMemcpyAsync(evalCounts, m->evalCounts, V * sizeof(int32_t), hipMemcpyDeviceToHost, s));
-    GM_HIP(hipStreamSynchronize(s));
     return GM_OK;
 }
 
 int gm_mesh_last_timing(const gm_mesh* m, double* out2)
 {
     if(!m || !out2) return fail(GM_ERR_ARG, "null argument");
-    out2[0] = m->lastTotalMs;
-    out2[1] = m->lastSweepMs;
+    if(!m->plan) return fail(GM_ERR_ARG, "mesh has no device copy");
+    out2[0] = m->plan->totalMs;
+    out2[1] = sweepMs(m->plan);
     return GM_OK;
 }
 
 int gm_mesh_kernel_times(const gm_mesh* m, double* ms, int64_t* launches)
 {
     if(!m) return fail(GM_ERR_ARG, "null mesh");
+    if(!m->plan) return fail(GM_ERR_ARG, "mesh has no device copy");
     for(int k = 0; k < GM_KT_COUNT; ++k)
     {
-        if(ms) ms[k] = m->kernMs[k];
-        if(launches) launches[k] = m->kernLaunches[k];
+        if(ms) ms[k] = m->plan->kernMs[k];
+        if(launches) launches[k] = m->plan->kernLaunches[k];
     }
+    return GM_OK;
+}
+
+// ----- batches
+
+int gm_batch_create(gm_mesh* const* meshes, int64_t n, gm_batch** out)
+{
+    if(!out) return fail(GM_ERR_ARG, "out is null");
+    *out = nullptr;
+    if(n < 1 || !meshes) return fail(GM_ERR_ARG, "a batch needs at least one mesh");
+    std::vector<gm_mesh*> ms(meshes, meshes + n);
+    for(gm_mesh* m : ms)
+    {
+        if(!m || !m->onDevice) return fail(GM_ERR_ARG, "batch member without a device copy");
+        if(m->device != ms[0]->device) return fail(GM_ERR_ARG, "batch members live on different devices");
+        if(m->sampler == GM_TEXTURE) return fail(GM_ERR_UNSUPPORTED, "batches do not take the texture sampler");
+        if(m->evaluator != ms[0]->evaluator || m->sampler != ms[0]->sampler || m->crew.scaling != ms[0]->crew.scaling)
+            return fail(GM_ERR_ARG, "batch members need the same evaluator, sampler and scaling");
+    }
+    std::vector<gm_mesh*> sorted = ms;
+    std::sort(sorted.begin(), sorted.end());
+    if(std::adjacent_find(sorted.begin(), sorted.end()) != sorted.end())
+        return fail(GM_ERR_ARG, "a mesh appears twice in the batch");
+    GM_HIP(hipSetDevice(ms[0]->device));
+    gm_batch* b = new gm_batch();
+    b->device = ms[0]->device;
+    hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+    if(e != hipSuccess)
+    {
+        delete b;
+        return fail(GM_ERR_HIP, std::string("stream creation: ") + hipGetErrorString(e));
+    }
+    int rc = gmr::buildPlan(ms, b->stream, &b->plan);
+    if(rc)
+    {
+        (void)hipStreamDestroy(b->stream);
+        delete b;
+        return rc;
+    }
+    *out = b;
+    return GM_OK;
+}
+
+int gm_batch_destroy(gm_batch* b)
+{
+    if(!b) return GM_OK;
+    (void)hipSetDevice(b->device);
+    (void)hipStreamSynchronize(b->stream);
+    gmr::freePlan(b->plan);
+    (void)hipStreamDestroy(b->stream);
+    delete b;
+    return GM_OK;
+}
+
+int gm_batch_set_shape(gm_batch* b, int smoother, int patchesPerGroup, int tileElems)
+{
+    if(!b) return fail(GM_ERR_ARG, "null batch");
+    if(smoother < 0 || smoother > 3) return fail(GM_ERR_ARG, "unknown smoother");
+    if(patchesPerGroup != 1 && patchesPerGroup != 2 && patchesPerGroup != 4 && patchesPerGroup != 8)
+        return fail(GM_ERR_ARG, "patches per workgroup must be 1, 2, 4 or 8");
+    if(tileElems != 64 && tileElems != 128 && tileElems != 256)
+        return fail(GM_ERR_ARG, "LDS tile must be 64, 128 or 256 elements");
+    b->lane[smoother] = gmd::LaneShape{patchesPerGroup, tileElems};
+    return GM_OK;
+}
+
+static int checkBatchCrew(const gm_batch* b)
+{
+    const std::vector<gm_mesh*>& ms = b->plan->meshes;
+    for(const gm_mesh* m : ms)
+        if(m->sampler == GM_TEXTURE || m->evaluator != ms[0]->evaluator || m->sampler != ms[0]->sampler ||
+           m->crew.scaling != ms[0]->crew.scaling)
+            return fail(GM_ERR_ARG, "batch members' crews diverged after the batch was made");
+    return GM_OK;
+}
+
+int gm_batch_smooth(gm_batch* b, int smoother, int passes, int precision, int kernel, double spawnCoeff,
+                    int useGraph, double* stats, int32_t* hist)
+{
+    if(!b) return fail(GM_ERR_ARG, "null batch");
+    int rc = checkRun(smoother, passes, precision, kernel, spawnCoeff);
+    if(rc) return rc;
+    if((rc = checkBatchCrew(b))) return rc;
+    GM_HIP(hipSetDevice(b->device));
+    gmr::RunParams r;
+    r.smoother = smoother;
+    r.passes = passes;
+    r.precision = precision;
+    r.kernel = kernel;
+    r.spawnCoeff = spawnCoeff;
+    r.lane = b->lane[smoother].patchesPerGroup ? b->lane[smoother] : defaultLane(b->plan->meshes, smoother);
+    r.graph = useGraph != 0;
+    return runAndFetch(b->plan, b->plan->meshes[0], r, stats, hist, nullptr);
+}
+
+int gm_batch_last_timing(const gm_batch* b, double* out2)
+{
+    if(!b || !out2) return fail(GM_ERR_ARG, "null argument");
+    out2[0] = b->plan->totalMs;
+    out2[1] = sweepMs(b->plan);
+    return GM_OK;
+}
+
+int gm_batch_kernel_times(const gm_batch* b, double* ms, int64_t* launches)
+{
+    if(!b) return fail(GM_ERR_ARG, "null batch");
+    for(int k = 0; k < GM_KT_COUNT; ++k)
+    {
+        if(ms) ms[k] = b->plan->kernMs[k];
+        if(launches) launches[k] = b->plan->kernLaunches[k];
+    }
+    return GM_OK;
+}
+
+int gm_batch_autotune(gm_batch* b, int smoother, int precision, int passes, int32_t* best, double* times)
+{
+    if(!b || !best) return fail(GM_ERR_ARG, "null argument");
+    int rc = checkRun(smoother, passes, precision, GM_KERNEL_WAVE, 1.0);
+    if(rc) return rc;
+    if(passes < 1) return fail(GM_ERR_ARG, "autotuning needs at least one pass");
+    if((rc = checkBatchCrew(b))) return rc;
+    GM_HIP(hipSetDevice(b->device));
+    gmr::Plan* p = b->plan;
+    // Every trial starts from the same positions; they are restored at the end.
+    std::vector<std::vector<double4>> saved(p->meshes.size());
+    for(size_t i = 0; i < p->meshes.size(); ++i)
+    {
+        saved[i].resize(p->meshes[i]->topo.nverts);
+        if(!saved[i].empty())
+            GM_HIP(hipMemcpyAsync(saved[i].data(), p->meshes[i]->dm.pos64, saved[i].size() * sizeof(double4),
+                                  hipMemcpyDeviceToHost, p->stream));
+    }
+    GM_HIP(hipStreamSynchronize(p->stream));
+    auto restore = [&]() -> int {
+        for(size_t i = 0; i < p->meshes.size(); ++i)
+            if(!saved[i].empty())
+                GM_HIP(hipMemcpyAsync(p->meshes[i]->dm.pos64, saved[i].data(), saved[i].size() * sizeof(double4),
+                                      hipMemcpyHostToDevice, p->stream));
+        GM_HIP(hipStreamSynchronize(p->stream));
+        return GM_OK;
+    };
+    const int PPG[4] = {1, 2, 4, 8};
+    const int TILE[3] = {64, 128, 256};
+    double bestMs = 1e300;
+    gmd::LaneShape bestShape = b->lane[smoother].patchesPerGroup ? b->lane[smoother] : defaultLane(p->meshes, smoother);
+    for(int a = 0; a < 4; ++a)
+        for(int t = 0; t < 3; ++t)
+        {
+            const gmd::LaneShape shape{PPG[a], TILE[t]};
+            // A shape is a candidate when the candidate-parallel kernel takes
+            // every single-kind list with it.
+            bool fits = smoother == gmd::SM_SPAWN_SEARCH || smoother == gmd::SM_GRADIENT_DESCENT;
+            for(const auto& col : p->colours)
+                for(const gmr::VList& l : col)
+                {
+                    gmd::VertexWork w{p->dTable, l.items, l.count, l.kind, l.maxPatch};
+                    if(l.kind != gmd::EK_MIXED && !gmd::laneFits(w, smoother, shape, precision == GM_FP64))
+                        fits = false;
+                }
+            double ms = -1.0;
+            if(fits)
+            {
+                gmr::RunParams r;
+                r.smoother = smoother;
+                r.passes = passes;
+                r.precision = precision;
+                r.kernel = GM_KERNEL_WAVE;
+                r.spawnCoeff = 1.0 / 20.0;
+                r.lane = shape;
+                rc = gmr::runPlan(p, p->meshes[0]->crew, p->meshes[0]->evaluator, p->meshes[0]->sampler, r);
+                if(rc) return rc;
+                ms = sweepMs(p);
+                if(ms < bestMs)
+                {
+                    bestMs = ms;
+                    bestShape = shape;
+                }
+                if((rc = restore())) return rc;
+            }
+            if(times) times[a * 3 + t] = ms;
+        }
+    // FP32 runs leave pos32 stale relative to the restored pos64; every run
+    // converts from pos64 first, so nothing else needs restoring.
+    b->lane[smoother] = bestShape;
+    best[0] = bestShape.patchesPerGroup;
+    best[1] = bestShape.tileElems;
     return GM_OK;
 }
 
@@ -640,7 +706,9 @@ int gm_math_probe(int op, const double* a, const double* b, double* out, int64_t
     GM_HIP(hipMemcpy(db, b ? b : a, n * sizeof(double), hipMemcpyHostToDevice));
     GM_HIP(gmd::launchMathProbe(op, da, db, dout, (int)n, nullptr));
     GM_HIP(hipMemcpy(out, dout, n * sizeof(double), hipMemcpyDeviceToHost));
-    (void)hipFree(da); (void)hipFree(db); (void)hipFree(dout);
+    (void)hipFree(da);
+    (void)hipFree(db);
+    (void)hipFree(dout);
     return GM_OK;
 }
 

@@ -16,32 +16,6 @@ namespace gmd
     default: return hipErrorInvalidValue;                                           \
     }
 
-template<typename T>
-hipError_t launchSmoothGroup(const DevMesh& m, const gmq::Crew& c, int evaluator, int sampler, int smoother,
-                             int shape, const uint32_t* nodes, uint32_t count, double moveCoeff,
-                             int32_t* evalCounts, int maxPatch, hipStream_t stream)
-{
-    if(evaluator < 0 || evaluator > 1 || sampler < 0 || sampler > 2) return hipErrorInvalidValue;
-    GMD_SELECT(evaluator, sampler, T, smooth(m, c, smoother, shape, nodes, count, moveCoeff, evalCounts, maxPatch, stream))
-}
-
-template<typename T>
-hipError_t launchGetmeElements(const DevMesh& m, const gmq::Crew& c, int evaluator, int sampler,
-                               void* tetOut, void* hexOut, hipStream_t stream)
-{
-    if(evaluator < 0 || evaluator > 1 || sampler < 0 || sampler > 2) return hipErrorInvalidValue;
-    GMD_SELECT(evaluator, sampler, T, getmeElements(m, c, tetOut, hexOut, stream))
-}
-
-template<typename T>
-hipError_t launchGetmeVertices(const DevMesh& m, const gmq::Crew& c, int evaluator, int sampler,
-                               const uint32_t* nodes, uint32_t count, const void* tetOut, const void* hexOut,
-                               int32_t* evalCounts, hipStream_t stream)
-{
-    if(evaluator < 0 || evaluator > 1 || sampler < 0 || sampler > 2) return hipErrorInvalidValue;
-    GMD_SELECT(evaluator, sampler, T, getmeVertices(m, c, nodes, count, tetOut, hexOut, evalCounts, stream))
-}
-
 #define GMD_SELECT_EVAL(EVV, SAV, CALL)                                             \
     switch((EVV) * 3 + (SAV))                                                       \
     {                                                                               \
@@ -54,29 +28,53 @@ hipError_t launchGetmeVertices(const DevMesh& m, const gmq::Crew& c, int evaluat
     default: return hipErrorInvalidValue;                                           \
     }
 
-template<>
-hipError_t launchEvaluate<double>(const DevMesh& m, const gmq::Crew& c, int evaluator, int sampler,
-                                  const EvalScratch& s, int pass, hipStream_t stream)
+template<typename T>
+hipError_t launchSmooth(const VertexWork& w, const gmq::Crew& c, int evaluator, int sampler, int smoother,
+                        int shape, LaneShape lane, double moveCoeff, hipStream_t stream)
 {
-    GMD_SELECT_EVAL(evaluator, sampler, evaluateF64(m, c, s, pass, stream))
+    GMD_SELECT(evaluator, sampler, T, smooth(w, c, smoother, shape, lane, moveCoeff, stream))
+}
+
+template<typename T>
+hipError_t launchGetmeElements(const DevMesh* meshes, const uint2* tetBlocks, uint32_t nTetBlocks,
+                               const uint2* hexBlocks, uint32_t nHexBlocks, const gmq::Crew& c,
+                               int evaluator, int sampler, hipStream_t stream)
+{
+    GMD_SELECT(evaluator, sampler, T, getmeElements(meshes, tetBlocks, nTetBlocks, hexBlocks, nHexBlocks, c, stream))
+}
+
+template<typename T>
+hipError_t launchGetmeVertices(const VertexWork& w, const gmq::Crew& c, int evaluator, int sampler,
+                               hipStream_t stream)
+{
+    GMD_SELECT(evaluator, sampler, T, getmeVertices(w, c, stream))
 }
 
 template<>
-hipError_t launchEvaluate<float>(const DevMesh& m, const gmq::Crew& c, int evaluator, int sampler,
-                                 const EvalScratch& s, int pass, hipStream_t stream)
+hipError_t launchEvaluate<double>(const DevMesh* meshes, const uint2* blocks, uint32_t nblocks,
+                                  const uint32_t* meshList, uint32_t nmeshes, const gmq::Crew& c, int evaluator,
+                                  int sampler, hipStream_t stream)
 {
-    GMD_SELECT_EVAL(evaluator, sampler, evaluateF32(m, c, s, pass, stream))
+    GMD_SELECT_EVAL(evaluator, sampler, evaluateF64(meshes, blocks, nblocks, meshList, nmeshes, c, stream))
 }
 
-template hipError_t launchSmoothGroup<double>(const DevMesh&, const gmq::Crew&, int, int, int, int, const uint32_t*,
-                                              uint32_t, double, int32_t*, int, hipStream_t);
-template hipError_t launchSmoothGroup<float>(const DevMesh&, const gmq::Crew&, int, int, int, int, const uint32_t*,
-                                             uint32_t, double, int32_t*, int, hipStream_t);
-template hipError_t launchGetmeElements<double>(const DevMesh&, const gmq::Crew&, int, int, void*, void*, hipStream_t);
-template hipError_t launchGetmeElements<float>(const DevMesh&, const gmq::Crew&, int, int, void*, void*, hipStream_t);
-template hipError_t launchGetmeVertices<double>(const DevMesh&, const gmq::Crew&, int, int, const uint32_t*, uint32_t,
-                                                const void*, const void*, int32_t*, hipStream_t);
-template hipError_t launchGetmeVertices<float>(const DevMesh&, const gmq::Crew&, int, int, const uint32_t*, uint32_t,
-                                               const void*, const void*, int32_t*, hipStream_t);
+template<>
+hipError_t launchEvaluate<float>(const DevMesh* meshes, const uint2* blocks, uint32_t nblocks,
+                                 const uint32_t* meshList, uint32_t nmeshes, const gmq::Crew& c, int evaluator,
+                                 int sampler, hipStream_t stream)
+{
+    GMD_SELECT_EVAL(evaluator, sampler, evaluateF32(meshes, blocks, nblocks, meshList, nmeshes, c, stream))
+}
+
+template hipError_t launchSmooth<double>(const VertexWork&, const gmq::Crew&, int, int, int, int, LaneShape, double,
+                                         hipStream_t);
+template hipError_t launchSmooth<float>(const VertexWork&, const gmq::Crew&, int, int, int, int, LaneShape, double,
+                                        hipStream_t);
+template hipError_t launchGetmeElements<double>(const DevMesh*, const uint2*, uint32_t, const uint2*, uint32_t,
+                                                const gmq::Crew&, int, int, hipStream_t);
+template hipError_t launchGetmeElements<float>(const DevMesh*, const uint2*, uint32_t, const uint2*, uint32_t,
+                                               const gmq::Crew&, int, int, hipStream_t);
+template hipError_t launchGetmeVertices<double>(const VertexWork&, const gmq::Crew&, int, int, hipStream_t);
+template hipError_t launchGetmeVertices<float>(const VertexWork&, const gmq::Crew&, int, int, hipStream_t);
 
 } // namespace gmd

@@ -1,5 +1,11 @@
-// gm_device.h -- the interface between the host runtime (api.hip) and the
-// precision-specific kernel translation units (kernels_f64.hip, kernels_f32.hip).
+// gm_device.h -- the interface between the host runtime (runtime.hip, api.hip)
+// and the kernel translation units (kernels_inst.hip, dispatch.hip).
+//
+// Every launch is a fused launch over a table of resident meshes: a single
+// mesh is a batch of one.  Vertex work is a list of (mesh, vertex) items per
+// colour step; element and evaluation work is a list of (mesh, first index)
+// blocks.  Meshes never share vertices, so items of different meshes are
+// independent and any union of same-colour vertices is an independent set.
 #ifndef GM_DEVICE_H
 #define GM_DEVICE_H
 
@@ -10,6 +16,21 @@
 
 namespace gmd
 {
+
+const int EVAL_MAX_BLOCKS = 1024;
+const int EVAL_THREADS = 256;
+const int ELEM_BLOCK = 256;        // elements per block of the GETMe element kernels
+
+// Number of evaluation blocks of a mesh with E elements.  The same value is
+// used alone and in a batch, so a mesh's min/mean reduction tree (and hence
+// its result bits) does not depend on what it is batched with.
+inline uint32_t evalBlocksFor(uint64_t E)
+{
+    uint64_t b = (E + 2047) / 2048;
+    if(b < 1) b = 1;
+    if(b > (uint64_t)EVAL_MAX_BLOCKS) b = EVAL_MAX_BLOCKS;
+    return (uint32_t)b;
+}
 
 // Resident mesh, as the kernels see it.  Positions are 16/32-byte AoS
 // (x, y, z, pad) so that a vertex gather is one aligned vector load.
@@ -27,77 +48,102 @@ struct DevMesh
     const uint32_t* noff;     // V+1
     const uint32_t* nbr;
     const uint8_t* fixed;     // 1 = fixed vertex (never moved; snaps to itself)
+    int32_t* evalCounts;      // patch evaluations per vertex (accumulated)
+    void* getmeTet;           // GETMe (vpp, weight) per tet corner, sized for FP64
+    void* getmeHex;           // same per hex corner
+    // Quality reduction of this mesh.
+    double* evalPartials;     // 3 per block: min, sum, compensation
+    int32_t* evalFlags;       // per block: 1 when an element quality <= 0
+    int32_t* evalHist;        // 20 running bucket counts (reset by the final kernel)
+    double* evalResult;       // [min, invSum, n] per recorded pass
+    int32_t* evalHistOut;     // 20 per recorded pass
+    double* elemQual;         // optional per-element qualities (nullptr = off)
+    int32_t* passSlot;        // next pass record to write (advanced by the final kernel)
     uint32_t nverts, ntets, nhexs;
+    uint32_t evalBlocks;
+    uint32_t resultCap;       // pass records evalResult / evalHistOut hold
 };
 
 enum { SM_NELDER_MEAD = 0, SM_GRADIENT_DESCENT = 1, SM_SPAWN_SEARCH = 2, SM_GETME = 3 };
 enum { KS_PLAIN = 0, KS_WAVE = 1 };
+// Element kind of a work list: the candidate-parallel kernels stage patches
+// of one kind (4 or 8 slots per element); mixed lists use the wave kernel.
+enum { EK_TET = 0, EK_HEX = 1, EK_MIXED = 2 };
 
-// Scratch for the mesh-quality reduction.
-struct EvalScratch
+// Launch shape of the candidate-parallel kernels: patches (vertices) per
+// workgroup and the LDS tile (elements staged per workgroup).
+struct LaneShape
 {
-    double* partials;   // 3 per block: min, sum, compensation
-    int32_t* flags;     // per block: 1 when an element quality <= 0 (sum is +inf)
-    double* result;     // [min, invSum, n] per recorded pass (stride 3)
-    int32_t* hist;      // 20 running bucket counts (reset by the final kernel)
-    int32_t* histOut;   // 20 per recorded pass
-    double* elemQual;   // optional per-element qualities (nullptr = off)
+    int patchesPerGroup;   // 1, 2, 4, 8 wavefronts per block (spawn) / 8-vertex groups per block (GD)
+    int tileElems;         // LDS elements staged per wavefront-vertex slot budget: 64, 128, 256 per block
 };
 
-const int EVAL_BLOCKS = 1024;
-const int EVAL_THREADS = 256;
+// One fused vertex launch.
+struct VertexWork
+{
+    const DevMesh* meshes;     // device table
+    const uint2* items;        // (mesh index, vertex id)
+    uint32_t count;
+    int kind;                  // EK_*
+    int maxPatch;              // largest patch in the list
+};
 
 // Kernel launchers of one (precision T, evaluator EV, sampler SA)
 // combination.  All launches go on `stream`; nothing here synchronises.
 template<typename T, int EV, int SA>
 struct Launchers
 {
-    static hipError_t smooth(const DevMesh& m, const gmq::Crew& c, int smoother, int shape,
-                             const uint32_t* nodes, uint32_t count, double moveCoeff,
-                             int32_t* evalCounts, int maxPatch, hipStream_t stream);
-    static hipError_t getmeElements(const DevMesh& m, const gmq::Crew& c, void* tetOut, void* hexOut,
+    static hipError_t smooth(const VertexWork& w, const gmq::Crew& c, int smoother, int shape,
+                             LaneShape lane, double moveCoeff, hipStream_t stream);
+    static hipError_t getmeElements(const DevMesh* meshes, const uint2* tetBlocks, uint32_t nTetBlocks,
+                                    const uint2* hexBlocks, uint32_t nHexBlocks, const gmq::Crew& c,
                                     hipStream_t stream);
-    static hipError_t getmeVertices(const DevMesh& m, const gmq::Crew& c, const uint32_t* nodes, uint32_t count,
-                                    const void* tetOut, const void* hexOut, int32_t* evalCounts,
-                                    hipStream_t stream);
+    static hipError_t getmeVertices(const VertexWork& w, const gmq::Crew& c, hipStream_t stream);
 };
 
 // FP64 mesh-quality reduction of one (evaluator, sampler) combination, reading
-// the FP64 (evaluateF64) or FP32 (evaluateF32) position copy.  Always built
-// without contraction, with the FP64 kernels.
+// the FP64 (evaluateF64) or FP32 (evaluateF32) position copy.  `blocks` lists
+// (mesh index, block index within the mesh); `meshList` the meshes to finish.
+// Always built without contraction, with the FP64 kernels.
 template<int EV, int SA>
 struct Evaluators
 {
-    static hipError_t evaluateF64(const DevMesh& m, const gmq::Crew& c, const EvalScratch& s, int pass,
+    static hipError_t evaluateF64(const DevMesh* meshes, const uint2* blocks, uint32_t nblocks,
+                                  const uint32_t* meshList, uint32_t nmeshes, const gmq::Crew& c,
                                   hipStream_t stream);
-    static hipError_t evaluateF32(const DevMesh& m, const gmq::Crew& c, const EvalScratch& s, int pass,
+    static hipError_t evaluateF32(const DevMesh* meshes, const uint2* blocks, uint32_t nblocks,
+                                  const uint32_t* meshList, uint32_t nmeshes, const gmq::Crew& c,
                                   hipStream_t stream);
 };
 
 // Run-time selection over (evaluator, sampler) (dispatch.hip).
 template<typename T>
-hipError_t launchSmoothGroup(const DevMesh& m, const gmq::Crew& c, int evaluator, int sampler,
-                             int smoother, int shape, const uint32_t* nodes, uint32_t count,
-                             double moveCoeff, int32_t* evalCounts, int maxPatch, hipStream_t stream);
-
+hipError_t launchSmooth(const VertexWork& w, const gmq::Crew& c, int evaluator, int sampler, int smoother,
+                        int shape, LaneShape lane, double moveCoeff, hipStream_t stream);
 template<typename T>
-hipError_t launchGetmeElements(const DevMesh& m, const gmq::Crew& c, int evaluator, int sampler,
-                               void* tetOut, void* hexOut, hipStream_t stream);
-
+hipError_t launchGetmeElements(const DevMesh* meshes, const uint2* tetBlocks, uint32_t nTetBlocks,
+                               const uint2* hexBlocks, uint32_t nHexBlocks, const gmq::Crew& c,
+                               int evaluator, int sampler, hipStream_t stream);
 template<typename T>
-hipError_t launchGetmeVertices(const DevMesh& m, const gmq::Crew& c, int evaluator, int sampler,
-                               const uint32_t* nodes, uint32_t count, const void* tetOut,
-                               const void* hexOut, int32_t* evalCounts, hipStream_t stream);
-
-// Mesh quality in FP64 from the positions stored in precision P (double:
-// pos64, float: pos32); writes slot `pass` of scratch.result / scratch.histOut.
+hipError_t launchGetmeVertices(const VertexWork& w, const gmq::Crew& c, int evaluator, int sampler,
+                               hipStream_t stream);
+// P = position precision read (double: pos64, float: pos32); always FP64 math.
 template<typename P>
-hipError_t launchEvaluate(const DevMesh& m, const gmq::Crew& c, int evaluator, int sampler,
-                          const EvalScratch& s, int pass, hipStream_t stream);
+hipError_t launchEvaluate(const DevMesh* meshes, const uint2* blocks, uint32_t nblocks, const uint32_t* meshList,
+                          uint32_t nmeshes, const gmq::Crew& c, int evaluator, int sampler, hipStream_t stream);
 
-// Position precision conversions.
-hipError_t launchToF32(const DevMesh& m, hipStream_t stream);
-hipError_t launchToF64(const DevMesh& m, hipStream_t stream);
+// Position precision conversions over (mesh, vertex block) lists: blocks of
+// 256 vertices, (mesh index, first vertex).
+hipError_t launchToF32(const DevMesh* meshes, const uint2* blocks, uint32_t nblocks, hipStream_t stream);
+hipError_t launchToF64(const DevMesh* meshes, const uint2* blocks, uint32_t nblocks, hipStream_t stream);
+// Zeroes evalCounts and sets passSlot = 0 for the meshes of a batch (fill
+// kernels, so that the reset can be captured in a graph).
+hipError_t launchResetRun(const DevMesh* meshes, const uint2* vertexBlocks, uint32_t nblocks,
+                          const uint32_t* meshList, uint32_t nmeshes, hipStream_t stream);
+
+// True when the candidate-parallel kernel of `smoother` runs `w` with `lane`
+// (otherwise the wave kernel does).
+bool laneFits(const VertexWork& w, int smoother, LaneShape lane, bool fp64);
 
 // Math self-test: op 0 sqrt, 1 div (a/b), 2 pow23, 3 cbrt, 4 exp; n values.
 hipError_t launchMathProbe(int op, const double* a, const double* b, double* out, int n, hipStream_t stream);

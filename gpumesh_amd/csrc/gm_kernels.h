@@ -7,6 +7,8 @@
 #include "gm_device.h"
 #include "gm_smoothers.h"
 
+#include <type_traits>
+
 namespace gmd
 {
 using gmq::v3;
@@ -103,17 +105,20 @@ struct PlainPatch
     }
 };
 
-// Plain kernel: one vertex of the colour group per lane.
+// Plain kernel: one vertex of the work list per lane.
 template<typename T, int EV, int SA, int SM>
-__global__ void __launch_bounds__(256) smoothPlainKernel(DevMesh m, gmq::Crew c, const uint32_t* __restrict__ nodes,
-                                                         uint32_t count, T moveCoeff, int32_t* evalCounts)
+__global__ void __launch_bounds__(256) smoothPlainKernel(const DevMesh* __restrict__ meshes,
+                                                         const uint2* __restrict__ items, uint32_t count,
+                                                         gmq::Crew c, T moveCoeff)
 {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if(i >= count)
         return;
     typedef typename PosOf<T>::type P;
+    const uint2 it = items[i];
+    const DevMesh& m = meshes[it.x];
     P* pos = PosOf<T>::get(m);
-    uint32_t v = nodes[i];
+    const uint32_t v = it.y;
     v3<T> p0 = ld<T>(pos, v);
     T ls = localSize<T>(m, pos, v, p0);
     PlainPatch<T, EV, SA> f{m, c, pos, v, 0};
@@ -122,8 +127,7 @@ __global__ void __launch_bounds__(256) smoothPlainKernel(DevMesh m, gmq::Crew c,
     else if(SM == SM_GRADIENT_DESCENT) np = gms::gradientDescent<T>(p0, ls, f);
     else np = gms::spawnSearch<T>(p0, ls, moveCoeff, f);
     st(pos, v, np);
-    if(evalCounts)
-        evalCounts[v] += f.count;
+    m.evalCounts[v] += f.count;
 }
 
 // ---------------------------------------------------------------------------
@@ -189,8 +193,9 @@ struct WavePatch
 };
 
 template<typename T, int EV, int SA, int SM, int G>
-__global__ void __launch_bounds__(64) smoothWaveKernel(DevMesh m, gmq::Crew c, const uint32_t* __restrict__ nodes,
-                                                       uint32_t count, T moveCoeff, int32_t* evalCounts)
+__global__ void __launch_bounds__(64) smoothWaveKernel(const DevMesh* __restrict__ meshes,
+                                                       const uint2* __restrict__ items, uint32_t count,
+                                                       gmq::Crew c, T moveCoeff)
 {
     typedef typename PosOf<T>::type P;
     __shared__ T red[2 * 64];
@@ -200,8 +205,10 @@ __global__ void __launch_bounds__(64) smoothWaveKernel(DevMesh m, gmq::Crew c, c
     // Groups past the end leave; groups never synchronise with each other.
     if(i >= count)
         return;
+    const uint2 it = items[i];
+    const DevMesh& m = meshes[it.x];
     P* pos = PosOf<T>::get(m);
-    uint32_t v = nodes[i];
+    const uint32_t v = it.y;
     WavePatch<T, EV, SA, G> f;
     f.c = &c;
     f.red = red + 2 * G * grp;
@@ -241,8 +248,7 @@ __global__ void __launch_bounds__(64) smoothWaveKernel(DevMesh m, gmq::Crew c, c
     if(lane == 0)
     {
         st(pos, v, np);
-        if(evalCounts)
-            evalCounts[v] += f.count;
+        m.evalCounts[v] += f.count;
     }
 }
 
@@ -335,24 +341,42 @@ __device__ __forceinline__ v3<T> shflv(v3<T> a, int src)
     return mk<T>(__shfl(a.x, src, 64), __shfl(a.y, src, 64), __shfl(a.z, src, 64));
 }
 
+// LDS bytes of one wavefront's staging area: `elems` elements of kind EK.
+template<typename T, int EK> __host__ __device__ inline size_t stageBytes(int elems)
+{
+    size_t b = (size_t)elems * Slots<EK>::N * 3 * sizeof(T);
+    return b + (((size_t)elems + 15) & ~(size_t)15);
+}
+
 // Spawn search (SpawnSearchSmoother::smoothVertices, SpawnSearchSmoother.cpp:161-226):
-// one vertex per wavefront, lane p evaluates lattice point p.
-template<typename T, int EV, int SA, int EK, int MAXP>
-__global__ void __launch_bounds__(64) spawnLaneKernel(DevMesh m, gmq::Crew c, const uint32_t* __restrict__ nodes,
-                                                      uint32_t count, T moveCoeff, int32_t* evalCounts)
+// one vertex per wavefront, lane p evaluates lattice point p; `ppg`
+// wavefronts (vertices) per block, each staging up to `cap` elements in LDS.
+template<typename T, int EV, int SA, int EK>
+__global__ void __launch_bounds__(512) spawnLaneKernel(const DevMesh* __restrict__ meshes,
+                                                       const uint2* __restrict__ items, uint32_t count,
+                                                       gmq::Crew c, T moveCoeff, int cap)
 {
     typedef typename PosOf<T>::type P;
-    __shared__ T sp[MAXP * Slots<EK>::N * 3];
-    __shared__ uint8_t sk[MAXP];
-    const uint32_t i = blockIdx.x;
-    const int lane = threadIdx.x;
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int ppg = blockDim.x >> 6;
+    unsigned char* base = smem + (size_t)wave * stageBytes<T, EK>(cap);
+    T* sp = (T*)base;
+    uint8_t* sk = base + (size_t)cap * Slots<EK>::N * 3 * sizeof(T);
+    const uint32_t i = blockIdx.x * ppg + wave;
+    const bool present = i < count;
+    const uint2 it = present ? items[i] : make_uint2(0, 0);
+    const DevMesh& m = meshes[it.x];
     P* pos = PosOf<T>::get(m);
-    const uint32_t v = nodes[i];
+    const uint32_t v = it.y;
     const uint32_t eb = m.eoff[v];
-    const int nE = (int)(m.eoff[v + 1] - eb);
+    const int nE = present ? (int)(m.eoff[v + 1] - eb) : 0;
     for(int e = lane; e < nE; e += 64)
         stageElement<T, EK>(m, pos, m.eref[eb + e], sp + e * Slots<EK>::N * 3, sk + e);
     __syncthreads();
+    if(!present)
+        return;
     v3<T> p0 = ld<T>(pos, v);
     T scale = moveCoeff * localSize<T>(m, pos, v, p0);
     for(int iter = 0; iter < 2; ++iter)
@@ -366,32 +390,35 @@ __global__ void __launch_bounds__(64) spawnLaneKernel(DevMesh m, gmq::Crew c, co
     if(lane == 0)
     {
         st(pos, v, p0);
-        if(evalCounts)
-            evalCounts[v] += 128;
+        m.evalCounts[v] += 128;
     }
 }
 
 // Gradient descent (GradientDescentSmoother::smoothVertices,
 // GradientDescentSmoother.cpp:73-202): 8 vertices per wavefront, 8 lanes each;
-// lanes 0-5 evaluate the 6 gradient samples, lanes 0-7 the 8 line-search points.
-template<typename T, int EV, int SA, int EK, int MAXP>
-__global__ void __launch_bounds__(64) gdLaneKernel(DevMesh m, gmq::Crew c, const uint32_t* __restrict__ nodes,
-                                                   uint32_t count, int32_t* evalCounts)
+// lanes 0-5 evaluate the 6 gradient samples, lanes 0-7 the 8 line-search
+// points.  `ppg` wavefronts per block; each vertex stages up to `cap` elements.
+template<typename T, int EV, int SA, int EK>
+__global__ void __launch_bounds__(512) gdLaneKernel(const DevMesh* __restrict__ meshes,
+                                                    const uint2* __restrict__ items, uint32_t count,
+                                                    gmq::Crew c, int cap)
 {
     typedef typename PosOf<T>::type P;
+    extern __shared__ __align__(16) unsigned char smem[];
     const int SL = Slots<EK>::N * 3;
-    __shared__ T sp[8 * MAXP * Slots<EK>::N * 3];
-    __shared__ uint8_t sk[8 * MAXP];
     const int lane = threadIdx.x;
     const int g = lane >> 3, s = lane & 7, base = lane & ~7;
-    const uint32_t i = blockIdx.x * 8 + g;
+    unsigned char* mine = smem + (size_t)g * stageBytes<T, EK>(cap);
+    T* mySp = (T*)mine;
+    uint8_t* mySk = mine + (size_t)cap * SL * sizeof(T);
+    const uint32_t i = blockIdx.x * (blockDim.x >> 3) + g;
     const bool present = i < count;
+    const uint2 it = present ? items[i] : make_uint2(0, 0);
+    const DevMesh& m = meshes[it.x];
     P* pos = PosOf<T>::get(m);
-    const uint32_t v = present ? nodes[i] : 0;
+    const uint32_t v = it.y;
     const uint32_t eb = m.eoff[v];
     const int nE = present ? (int)(m.eoff[v + 1] - eb) : 0;
-    T* mySp = sp + g * MAXP * SL;
-    uint8_t* mySk = sk + g * MAXP;
     for(int e = s; e < nE; e += 8)
         stageElement<T, EK>(m, pos, m.eref[eb + e], mySp + e * SL, mySk + e);
     __syncthreads();
@@ -425,8 +452,8 @@ __global__ void __launch_bounds__(64) gdLaneKernel(DevMesh m, gmq::Crew c, const
         T qp = ldsPatchQuality<T, EV, SA, EK>(c, mySp, mySk, nE, prop);
         evals += 8;
         int best = groupArgmax<T>(qp, s, 8);
-        p = shflv<T>(prop, base + best);
-        T bestOff = __shfl(off, base + best, 64);
+        p = shflv<T>(prop, (base & 63) + best);
+        T bestOff = __shfl(off, (base & 63) + best, 64);
         nodeShift *= gmq::tabs(bestOff);
         if(nodeShift < originalNodeShift / T(10))
             break;
@@ -434,8 +461,7 @@ __global__ void __launch_bounds__(64) gdLaneKernel(DevMesh m, gmq::Crew c, const
     if(s == 0)
     {
         st(pos, v, p);
-        if(evalCounts)
-            evalCounts[v] += evals;
+        m.evalCounts[v] += evals;
     }
 }
 
@@ -445,13 +471,17 @@ __global__ void __launch_bounds__(64) gdLaneKernel(DevMesh m, gmq::Crew c, const
 // positions; out[n_vertices_per_element * id + k] = (vpp_k, weight).
 // ---------------------------------------------------------------------------
 template<typename T, int EV, int SA>
-__global__ void __launch_bounds__(256) getmeTetKernel(DevMesh m, gmq::Crew c, typename PosOf<T>::type* out)
+__global__ void __launch_bounds__(ELEM_BLOCK) getmeTetKernel(const DevMesh* __restrict__ meshes,
+                                                             const uint2* __restrict__ blocks, gmq::Crew c)
 {
-    uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint2 bl = blocks[blockIdx.x];
+    const DevMesh& m = meshes[bl.x];
+    const uint32_t e = bl.y + threadIdx.x;
     if(e >= m.ntets)
         return;
     typedef typename PosOf<T>::type P;
     const P* pos = PosOf<T>::get(m);
+    P* out = (P*)m.getmeTet;
     const T lambda = T(0.78);
     uint4 t = m.tets[e];
     uint32_t vi[4] = {t.x, t.y, t.z, t.w};
@@ -484,13 +514,17 @@ __global__ void __launch_bounds__(256) getmeTetKernel(DevMesh m, gmq::Crew c, ty
 }
 
 template<typename T, int EV, int SA>
-__global__ void __launch_bounds__(256) getmeHexKernel(DevMesh m, gmq::Crew c, typename PosOf<T>::type* out)
+__global__ void __launch_bounds__(ELEM_BLOCK) getmeHexKernel(const DevMesh* __restrict__ meshes,
+                                                             const uint2* __restrict__ blocks, gmq::Crew c)
 {
-    uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint2 bl = blocks[blockIdx.x];
+    const DevMesh& m = meshes[bl.x];
+    const uint32_t e = bl.y + threadIdx.x;
     if(e >= m.nhexs)
         return;
     typedef typename PosOf<T>::type P;
     const P* pos = PosOf<T>::get(m);
+    P* out = (P*)m.getmeHex;
     const T lambda = T(0.78);
     uint4 ha = m.hexs[2 * e], hb = m.hexs[2 * e + 1];
     uint32_t vi[8] = {ha.x, ha.y, ha.z, ha.w, hb.x, hb.y, hb.z, hb.w};
@@ -538,16 +572,20 @@ __global__ void __launch_bounds__(256) getmeHexKernel(DevMesh m, gmq::Crew c, ty
 // of the element loop), move to their weighted mean, keep the move unless the
 // patch quality drops.
 template<typename T, int EV, int SA>
-__global__ void __launch_bounds__(256) getmeVertexKernel(DevMesh m, gmq::Crew c, const uint32_t* __restrict__ nodes,
-                                                         uint32_t count, const typename PosOf<T>::type* tetOut,
-                                                         const typename PosOf<T>::type* hexOut, int32_t* evalCounts)
+__global__ void __launch_bounds__(512) getmeVertexKernel(const DevMesh* __restrict__ meshes,
+                                                         const uint2* __restrict__ items, uint32_t count,
+                                                         gmq::Crew c)
 {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if(i >= count)
         return;
     typedef typename PosOf<T>::type P;
+    const uint2 it = items[i];
+    const DevMesh& m = meshes[it.x];
     P* pos = PosOf<T>::get(m);
-    uint32_t v = nodes[i];
+    const P* tetOut = (const P*)m.getmeTet;
+    const P* hexOut = (const P*)m.getmeHex;
+    const uint32_t v = it.y;
     v3<T> acc = mk<T>(T(0), T(0), T(0));
     T wacc = T(0);
     uint32_t b = m.eoff[v], e = m.eoff[v + 1];
@@ -570,8 +608,7 @@ __global__ void __launch_bounds__(256) getmeVertexKernel(DevMesh m, gmq::Crew c,
         T q1 = f(p1);
         if(!(q1 < q0))
             st(pos, v, p1);
-        if(evalCounts)
-            evalCounts[v] += f.count;
+        m.evalCounts[v] += f.count;
     }
 }
 
@@ -599,7 +636,8 @@ __device__ __forceinline__ void twoSum(double a, double b, double& s, double& e)
 }
 
 template<int EV, int SA, typename P>
-__global__ void __launch_bounds__(EVAL_THREADS) evalKernel(DevMesh m, gmq::Crew c, const P* pos, EvalScratch s)
+__global__ void __launch_bounds__(EVAL_THREADS) evalKernel(const DevMesh* __restrict__ meshes,
+                                                           const uint2* __restrict__ blocks, gmq::Crew c)
 {
     __shared__ double smin[EVAL_THREADS], ssum[EVAL_THREADS], scmp[EVAL_THREADS];
     __shared__ int shist[20];
@@ -607,9 +645,12 @@ __global__ void __launch_bounds__(EVAL_THREADS) evalKernel(DevMesh m, gmq::Crew 
     if(threadIdx.x < 20) shist[threadIdx.x] = 0;
     if(threadIdx.x == 0) sbad = 0;
     __syncthreads();
+    const uint2 bl = blocks[blockIdx.x];
+    const DevMesh& m = meshes[bl.x];
+    const P* pos = PosOf<typename std::conditional<std::is_same<P, double4>::value, double, float>::type>::get(m);
     const uint64_t E = (uint64_t)m.ntets + m.nhexs;
-    const uint64_t chunk = (E + gridDim.x - 1) / gridDim.x;
-    const uint64_t b0 = (uint64_t)blockIdx.x * chunk;
+    const uint64_t chunk = (E + m.evalBlocks - 1) / m.evalBlocks;
+    const uint64_t b0 = (uint64_t)bl.y * chunk;
     const uint64_t b1 = (b0 + chunk < E) ? b0 + chunk : E;
     double mn = 1.0, sum = 0.0, cmp = 0.0;
     int bad = 0;
@@ -630,7 +671,7 @@ __global__ void __launch_bounds__(EVAL_THREADS) evalKernel(DevMesh m, gmq::Crew 
                                 ld<double>(pos, bq.x), ld<double>(pos, bq.y), ld<double>(pos, bq.z), ld<double>(pos, bq.w)};
             q = gmq::hexQuality<double, EV, SA>(c, vp);
         }
-        if(s.elemQual) s.elemQual[e] = q;
+        if(m.elemQual) m.elemQual[e] = q;
         mn = gmq::tmin(mn, q);
         if(q > 0.0)
         {
@@ -663,30 +704,39 @@ __global__ void __launch_bounds__(EVAL_THREADS) evalKernel(DevMesh m, gmq::Crew 
     }
     if(threadIdx.x == 0)
     {
-        s.partials[3 * blockIdx.x] = smin[0];
-        s.partials[3 * blockIdx.x + 1] = ssum[0];
-        s.partials[3 * blockIdx.x + 2] = scmp[0];
-        s.flags[blockIdx.x] = sbad;
+        m.evalPartials[3 * bl.y] = smin[0];
+        m.evalPartials[3 * bl.y + 1] = ssum[0];
+        m.evalPartials[3 * bl.y + 2] = scmp[0];
+        m.evalFlags[bl.y] = sbad;
     }
     if(threadIdx.x < 20 && shist[threadIdx.x])
-        atomicAdd(&s.hist[threadIdx.x], shist[threadIdx.x]);
+        atomicAdd(&m.evalHist[threadIdx.x], shist[threadIdx.x]);
 }
 
-static __global__ void __launch_bounds__(EVAL_THREADS) evalFinalKernel(EvalScratch s, int nblocks, double n, int pass)
+// One block per mesh: reduces the mesh's block partials in a fixed order and
+// writes pass record *passSlot, then advances the slot.
+static __global__ void __launch_bounds__(EVAL_THREADS) evalFinalKernel(const DevMesh* __restrict__ meshes,
+                                                                       const uint32_t* __restrict__ meshList)
 {
     __shared__ double smin[EVAL_THREADS], ssum[EVAL_THREADS], scmp[EVAL_THREADS];
     __shared__ int sbad;
+    const DevMesh& m = meshes[meshList[blockIdx.x]];
+    const int slot = *m.passSlot;
+    // The host sizes the record buffers for the run; a slot past them is
+    // never written.
+    if(slot < 0 || (uint32_t)slot >= m.resultCap)
+        return;
     if(threadIdx.x == 0) sbad = 0;
     __syncthreads();
     double mn = 1.0, sum = 0.0, cmp = 0.0;
-    for(int b = threadIdx.x; b < nblocks; b += blockDim.x)
+    for(uint32_t b = threadIdx.x; b < m.evalBlocks; b += blockDim.x)
     {
-        mn = gmq::tmin(mn, s.partials[3 * b]);
+        mn = gmq::tmin(mn, m.evalPartials[3 * b]);
         double t, err;
-        twoSum(sum, s.partials[3 * b + 1], t, err);
+        twoSum(sum, m.evalPartials[3 * b + 1], t, err);
         sum = t;
-        cmp += err + s.partials[3 * b + 2];
-        if(s.flags[b]) sbad = 1;
+        cmp += err + m.evalPartials[3 * b + 2];
+        if(m.evalFlags[b]) sbad = 1;
     }
     smin[threadIdx.x] = mn;
     ssum[threadIdx.x] = sum;
@@ -706,15 +756,18 @@ static __global__ void __launch_bounds__(EVAL_THREADS) evalFinalKernel(EvalScrat
     }
     if(threadIdx.x == 0)
     {
-        s.result[3 * pass] = smin[0];
-        s.result[3 * pass + 1] = sbad ? (double)__builtin_huge_val() : ssum[0] + scmp[0];
-        s.result[3 * pass + 2] = n;
+        m.evalResult[3 * slot] = smin[0];
+        m.evalResult[3 * slot + 1] = sbad ? (double)__builtin_huge_val() : ssum[0] + scmp[0];
+        m.evalResult[3 * slot + 2] = (double)m.ntets + (double)m.nhexs;
     }
     if(threadIdx.x < 20)
     {
-        s.histOut[20 * pass + threadIdx.x] = s.hist[threadIdx.x];
-        s.hist[threadIdx.x] = 0;
+        m.evalHistOut[20 * slot + threadIdx.x] = m.evalHist[threadIdx.x];
+        m.evalHist[threadIdx.x] = 0;
     }
+    __syncthreads();
+    if(threadIdx.x == 0)
+        *m.passSlot = slot + 1;
 }
 
 // ---------------------------------------------------------------------------
@@ -722,117 +775,128 @@ static __global__ void __launch_bounds__(EVAL_THREADS) evalFinalKernel(EvalScrat
 // combination is instantiated in its own translation unit (parallel build)
 // and selected at run time by dispatch.hip.
 // ---------------------------------------------------------------------------
-template<typename T, int EV, int SA, int SM, int EK, int MAXP>
-static void launchLaneKernel(const DevMesh& m, const gmq::Crew& c, const uint32_t* nodes, uint32_t count,
-                             double moveCoeff, int32_t* evalCounts, hipStream_t stream)
+template<typename T, int EV, int SA, int SM, int EK>
+static hipError_t launchLane(const VertexWork& w, const gmq::Crew& c, LaneShape lane, double moveCoeff,
+                             hipStream_t stream)
 {
+    const int ppg = lane.patchesPerGroup;
     if(SM == SM_SPAWN_SEARCH)
-        spawnLaneKernel<T, EV, SA, EK, MAXP><<<count, 64, 0, stream>>>(m, c, nodes, count, (T)moveCoeff, evalCounts);
-    else
-        gdLaneKernel<T, EV, SA, EK, MAXP><<<(count + 7) / 8, 64, 0, stream>>>(m, c, nodes, count, evalCounts);
-}
-
-template<typename T, int EV, int SA, int SM>
-static hipError_t launchSmoothTyped(const DevMesh& m, const gmq::Crew& c, int shape, const uint32_t* nodes,
-                                    uint32_t count, double moveCoeff, int32_t* evalCounts, int maxPatch,
-                                    hipStream_t stream)
-{
-    if(count == 0)
-        return hipSuccess;
-    // Candidate-parallel kernels for spawn search and gradient descent on
-    // single-element-kind meshes with patches of at most 64 elements.
-    const bool laneKernel = shape == KS_WAVE && (SM == SM_SPAWN_SEARCH || SM == SM_GRADIENT_DESCENT) &&
-                            maxPatch <= 64 && (m.ntets == 0 || m.nhexs == 0);
-    if(laneKernel)
     {
-        const bool hex = m.ntets == 0;
-        if(SM == SM_SPAWN_SEARCH || SM == SM_GRADIENT_DESCENT)
-        {
-            if(!hex && maxPatch <= 24) launchLaneKernel<T, EV, SA, SM, 0, 24>(m, c, nodes, count, moveCoeff, evalCounts, stream);
-            else if(!hex) launchLaneKernel<T, EV, SA, SM, 0, 64>(m, c, nodes, count, moveCoeff, evalCounts, stream);
-            else if(maxPatch <= 8) launchLaneKernel<T, EV, SA, SM, 1, 8>(m, c, nodes, count, moveCoeff, evalCounts, stream);
-            else launchLaneKernel<T, EV, SA, SM, 1, 64>(m, c, nodes, count, moveCoeff, evalCounts, stream);
-        }
-    }
-    else if(shape == KS_WAVE && maxPatch <= 64)
-    {
-        if(maxPatch <= 8)
-            smoothWaveKernel<T, EV, SA, SM, 8><<<(count + 7) / 8, 64, 0, stream>>>(m, c, nodes, count, (T)moveCoeff, evalCounts);
-        else if(maxPatch <= 16)
-            smoothWaveKernel<T, EV, SA, SM, 16><<<(count + 3) / 4, 64, 0, stream>>>(m, c, nodes, count, (T)moveCoeff, evalCounts);
-        else if(maxPatch <= 32)
-            smoothWaveKernel<T, EV, SA, SM, 32><<<(count + 1) / 2, 64, 0, stream>>>(m, c, nodes, count, (T)moveCoeff, evalCounts);
-        else
-            smoothWaveKernel<T, EV, SA, SM, 64><<<count, 64, 0, stream>>>(m, c, nodes, count, (T)moveCoeff, evalCounts);
+        // LDS tile per block = ppg wavefronts x tileElems elements.
+        const int cap = lane.tileElems;
+        const size_t lds = (size_t)ppg * stageBytes<T, EK>(cap);
+        spawnLaneKernel<T, EV, SA, EK><<<(w.count + ppg - 1) / ppg, 64 * ppg, lds, stream>>>(
+            w.meshes, w.items, w.count, c, (T)moveCoeff, cap);
     }
     else
     {
-        smoothPlainKernel<T, EV, SA, SM><<<(count + 255) / 256, 256, 0, stream>>>(m, c, nodes, count, (T)moveCoeff, evalCounts);
+        // 8 vertices per wavefront share the wavefront's tile.
+        const int cap = lane.tileElems / 8;
+        const size_t lds = (size_t)8 * ppg * stageBytes<T, EK>(cap);
+        gdLaneKernel<T, EV, SA, EK><<<(w.count + 8 * ppg - 1) / (8 * ppg), 64 * ppg, lds, stream>>>(
+            w.meshes, w.items, w.count, c, cap);
     }
     return hipGetLastError();
 }
 
+// True when the candidate-parallel kernel of `smoother` can run `w` with `lane`.
+template<typename T>
+inline bool laneFits(const VertexWork& w, int smoother, LaneShape lane)
+{
+    if(!(smoother == SM_SPAWN_SEARCH || smoother == SM_GRADIENT_DESCENT) || w.kind == EK_MIXED)
+        return false;
+    const int ppg = lane.patchesPerGroup;
+    const int cap = smoother == SM_SPAWN_SEARCH ? lane.tileElems : lane.tileElems / 8;
+    if(cap < w.maxPatch || ppg < 1 || ppg > 8)
+        return false;
+    const size_t per = w.kind == EK_TET ? stageBytes<T, EK_TET>(cap) : stageBytes<T, EK_HEX>(cap);
+    const size_t lds = (smoother == SM_SPAWN_SEARCH ? (size_t)ppg : (size_t)8 * ppg) * per;
+    return lds <= 64 * 1024;
+}
+
+template<typename T, int EV, int SA, int SM>
+static hipError_t launchSmoothTyped(const VertexWork& w, const gmq::Crew& c, int shape, LaneShape lane,
+                                    double moveCoeff, hipStream_t stream)
+{
+    if(w.count == 0)
+        return hipSuccess;
+    if(shape == KS_WAVE && laneFits<T>(w, SM, lane))
+    {
+        if(w.kind == EK_TET) return launchLane<T, EV, SA, SM, EK_TET>(w, c, lane, moveCoeff, stream);
+        return launchLane<T, EV, SA, SM, EK_HEX>(w, c, lane, moveCoeff, stream);
+    }
+    if(shape == KS_WAVE && w.maxPatch <= 64)
+    {
+        if(w.maxPatch <= 8)
+            smoothWaveKernel<T, EV, SA, SM, 8><<<(w.count + 7) / 8, 64, 0, stream>>>(w.meshes, w.items, w.count, c, (T)moveCoeff);
+        else if(w.maxPatch <= 16)
+            smoothWaveKernel<T, EV, SA, SM, 16><<<(w.count + 3) / 4, 64, 0, stream>>>(w.meshes, w.items, w.count, c, (T)moveCoeff);
+        else if(w.maxPatch <= 32)
+            smoothWaveKernel<T, EV, SA, SM, 32><<<(w.count + 1) / 2, 64, 0, stream>>>(w.meshes, w.items, w.count, c, (T)moveCoeff);
+        else
+            smoothWaveKernel<T, EV, SA, SM, 64><<<w.count, 64, 0, stream>>>(w.meshes, w.items, w.count, c, (T)moveCoeff);
+        return hipGetLastError();
+    }
+    smoothPlainKernel<T, EV, SA, SM><<<(w.count + 255) / 256, 256, 0, stream>>>(w.meshes, w.items, w.count, c, (T)moveCoeff);
+    return hipGetLastError();
+}
+
 template<typename T, int EV, int SA>
-hipError_t Launchers<T, EV, SA>::smooth(const DevMesh& m, const gmq::Crew& c, int smoother, int shape,
-                                        const uint32_t* nodes, uint32_t count, double moveCoeff,
-                                        int32_t* evalCounts, int maxPatch, hipStream_t stream)
+hipError_t Launchers<T, EV, SA>::smooth(const VertexWork& w, const gmq::Crew& c, int smoother, int shape,
+                                        LaneShape lane, double moveCoeff, hipStream_t stream)
 {
     if(smoother == SM_NELDER_MEAD)
-        return launchSmoothTyped<T, EV, SA, SM_NELDER_MEAD>(m, c, shape, nodes, count, moveCoeff, evalCounts, maxPatch, stream);
+        return launchSmoothTyped<T, EV, SA, SM_NELDER_MEAD>(w, c, shape, lane, moveCoeff, stream);
     if(smoother == SM_GRADIENT_DESCENT)
-        return launchSmoothTyped<T, EV, SA, SM_GRADIENT_DESCENT>(m, c, shape, nodes, count, moveCoeff, evalCounts, maxPatch, stream);
+        return launchSmoothTyped<T, EV, SA, SM_GRADIENT_DESCENT>(w, c, shape, lane, moveCoeff, stream);
     if(smoother == SM_SPAWN_SEARCH)
-        return launchSmoothTyped<T, EV, SA, SM_SPAWN_SEARCH>(m, c, shape, nodes, count, moveCoeff, evalCounts, maxPatch, stream);
+        return launchSmoothTyped<T, EV, SA, SM_SPAWN_SEARCH>(w, c, shape, lane, moveCoeff, stream);
     return hipErrorInvalidValue;
 }
 
 template<typename T, int EV, int SA>
-hipError_t Launchers<T, EV, SA>::getmeElements(const DevMesh& m, const gmq::Crew& c, void* tetOut, void* hexOut,
+hipError_t Launchers<T, EV, SA>::getmeElements(const DevMesh* meshes, const uint2* tetBlocks, uint32_t nTetBlocks,
+                                               const uint2* hexBlocks, uint32_t nHexBlocks, const gmq::Crew& c,
                                                hipStream_t stream)
 {
-    typedef typename PosOf<T>::type P;
-    if(m.ntets)
-        getmeTetKernel<T, EV, SA><<<(m.ntets + 255) / 256, 256, 0, stream>>>(m, c, (P*)tetOut);
-    if(m.nhexs)
-        getmeHexKernel<T, EV, SA><<<(m.nhexs + 255) / 256, 256, 0, stream>>>(m, c, (P*)hexOut);
+    if(nTetBlocks)
+        getmeTetKernel<T, EV, SA><<<nTetBlocks, ELEM_BLOCK, 0, stream>>>(meshes, tetBlocks, c);
+    if(nHexBlocks)
+        getmeHexKernel<T, EV, SA><<<nHexBlocks, ELEM_BLOCK, 0, stream>>>(meshes, hexBlocks, c);
     return hipGetLastError();
 }
 
 template<typename T, int EV, int SA>
-hipError_t Launchers<T, EV, SA>::getmeVertices(const DevMesh& m, const gmq::Crew& c, const uint32_t* nodes,
-                                               uint32_t count, const void* tetOut, const void* hexOut,
-                                               int32_t* evalCounts, hipStream_t stream)
+hipError_t Launchers<T, EV, SA>::getmeVertices(const VertexWork& w, const gmq::Crew& c, hipStream_t stream)
 {
-    typedef typename PosOf<T>::type P;
-    if(count == 0)
+    if(w.count == 0)
         return hipSuccess;
-    getmeVertexKernel<T, EV, SA><<<(count + 255) / 256, 256, 0, stream>>>(m, c, nodes, count, (const P*)tetOut,
-                                                                         (const P*)hexOut, evalCounts);
-    return hipGetLastError();
-}
-
-template<int EV, int SA, typename P>
-static hipError_t launchEvalTyped(const DevMesh& m, const gmq::Crew& c, const P* pos, const EvalScratch& s, int pass,
-                                  hipStream_t stream)
-{
-    const double n = (double)m.ntets + (double)m.nhexs;
-    evalKernel<EV, SA, P><<<EVAL_BLOCKS, EVAL_THREADS, 0, stream>>>(m, c, pos, s);
-    evalFinalKernel<<<1, EVAL_THREADS, 0, stream>>>(s, EVAL_BLOCKS, n, pass);
+    getmeVertexKernel<T, EV, SA><<<(w.count + 255) / 256, 256, 0, stream>>>(w.meshes, w.items, w.count, c);
     return hipGetLastError();
 }
 
 template<int EV, int SA>
-hipError_t Evaluators<EV, SA>::evaluateF64(const DevMesh& m, const gmq::Crew& c, const EvalScratch& s, int pass,
+hipError_t Evaluators<EV, SA>::evaluateF64(const DevMesh* meshes, const uint2* blocks, uint32_t nblocks,
+                                           const uint32_t* meshList, uint32_t nmeshes, const gmq::Crew& c,
                                            hipStream_t stream)
 {
-    return launchEvalTyped<EV, SA, double4>(m, c, m.pos64, s, pass, stream);
+    if(nblocks == 0 || nmeshes == 0)
+        return hipSuccess;
+    evalKernel<EV, SA, double4><<<nblocks, EVAL_THREADS, 0, stream>>>(meshes, blocks, c);
+    evalFinalKernel<<<nmeshes, EVAL_THREADS, 0, stream>>>(meshes, meshList);
+    return hipGetLastError();
 }
 
 template<int EV, int SA>
-hipError_t Evaluators<EV, SA>::evaluateF32(const DevMesh& m, const gmq::Crew& c, const EvalScratch& s, int pass,
+hipError_t Evaluators<EV, SA>::evaluateF32(const DevMesh* meshes, const uint2* blocks, uint32_t nblocks,
+                                           const uint32_t* meshList, uint32_t nmeshes, const gmq::Crew& c,
                                            hipStream_t stream)
 {
-    return launchEvalTyped<EV, SA, float4>(m, c, m.pos32, s, pass, stream);
+    if(nblocks == 0 || nmeshes == 0)
+        return hipSuccess;
+    evalKernel<EV, SA, float4><<<nblocks, EVAL_THREADS, 0, stream>>>(meshes, blocks, c);
+    evalFinalKernel<<<nmeshes, EVAL_THREADS, 0, stream>>>(meshes, meshList);
+    return hipGetLastError();
 }
 
 } // namespace gmd

@@ -1,0 +1,122 @@
+// gm_runtime.h -- host runtime internals shared by runtime.hip (plans, pass
+// loop) and api.hip (the C ABI).  Not part of the C ABI.
+#ifndef GM_RUNTIME_H
+#define GM_RUNTIME_H
+
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/gpumesh.h"
+#include "gm_device.h"
+#include "gm_topology.h"
+
+namespace gmr
+{
+
+// Records the message for gm_last_error() and returns `code`.
+int fail(int code, const std::string& msg);
+// Message of the calling thread's last failure.
+const char* lastError();
+
+// A vertex work list of one colour step and one element kind.
+struct VList
+{
+    uint2* items = nullptr;
+    uint32_t count = 0;
+    int kind = gmd::EK_TET;
+    int maxPatch = 0;
+};
+
+struct RunParams
+{
+    int smoother = 0;
+    int passes = 0;
+    int precision = GM_FP64;
+    int kernel = GM_KERNEL_WAVE;
+    double spawnCoeff = 1.0 / 20.0;
+    gmd::LaneShape lane{1, 64};
+    bool graph = false;
+    bool timeKernels = true;   // per-launch events (not inside graphs)
+};
+
+struct Plan;
+
+} // namespace gmr
+
+struct gm_mesh
+{
+    bool onDevice = false;     // false: topology only (gm_mesh_create_host)
+    int device = 0;
+    hipStream_t stream = nullptr;
+    gmt::Topology topo;
+    gmd::DevMesh dm{};         // host copy of the device descriptor (owns its pointers)
+    int kind = gmd::EK_TET;    // element kind of the whole mesh
+    int maxPatch = 0;
+    int resultCap = 0;
+    bool getmeReady = false;
+    double* elemQualBuf = nullptr;
+    int evaluator = GM_MEAN_RATIO;
+    int sampler = GM_UNIFORM;
+    gmq::Crew crew{};
+    float* texels = nullptr;
+    gmr::Plan* plan = nullptr; // the mesh as a batch of one
+    size_t deviceBytes = 0;    // device memory owned by the mesh
+};
+
+namespace gmr
+{
+
+struct Plan
+{
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::vector<gm_mesh*> meshes;
+    gmd::DevMesh* dTable = nullptr;
+    std::vector<std::vector<VList>> colours;      // [colour][list]
+    uint2* tetBlocks = nullptr; uint32_t nTetBlocks = 0;
+    uint2* hexBlocks = nullptr; uint32_t nHexBlocks = 0;
+    uint2* evalBlocks = nullptr; uint32_t nEvalBlocks = 0;
+    uint2* vertBlocks = nullptr; uint32_t nVertBlocks = 0;
+    uint32_t* meshList = nullptr; uint32_t nMeshes = 0;
+    std::vector<void*> owned;                     // device allocations of this plan
+    // Graphs of one pass body, keyed by (smoother, fp64, kernel, ppg, tile, spawn coeff bits, eval f32).
+    std::map<std::tuple<int, int, int, int, int, uint64_t, int>, hipGraphExec_t> graphs;
+    std::vector<hipGraph_t> graphDefs;
+    // Timing.
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    std::vector<hipEvent_t> pool;
+    std::vector<int> poolTag;
+    size_t poolUsed = 0;
+    double totalMs = 0.0;
+    double kernMs[GM_KT_COUNT] = {0};
+    int64_t kernLaunches[GM_KT_COUNT] = {0};
+    // Graphs hold the crew by value: they are dropped when it changes.
+    gmq::Crew graphCrew{};
+    int graphEv = -1, graphSa = -1;
+    // Host copy of the descriptor table last uploaded.
+    std::vector<gmd::DevMesh> table;
+};
+
+void clearGraphs(Plan* p);
+
+// Builds the fused plan of `meshes` (all resident on one device) on `stream`.
+int buildPlan(const std::vector<gm_mesh*>& meshes, hipStream_t stream, Plan** out);
+void freePlan(Plan* p);
+// Makes sure every mesh has room for `records` pass records and (for GETMe)
+// its element scratch, then uploads the descriptors.
+int preparePlan(Plan* p, int records, bool getme);
+// Runs the pass loop of `r` over the plan; synchronises at the end.
+int runPlan(Plan* p, const gmq::Crew& crew, int evaluator, int sampler, const RunParams& r);
+// One sweep restricted to colours [g0, g1) (no evaluation), FP64 or FP32.
+int sweepColours(Plan* p, const gmq::Crew& crew, int evaluator, int sampler, const RunParams& r, int64_t g0,
+                 int64_t g1);
+// Copies the pass records of mesh i (stats 3 per record, hist 20 per record).
+int fetchRecords(Plan* p, size_t i, int records, double* stats, int32_t* hist, int32_t* evalCounts);
+
+} // namespace gmr
+
+#endif // GM_RUNTIME_H

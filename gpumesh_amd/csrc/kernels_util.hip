@@ -1,38 +1,73 @@
-// kernels_util.hip -- precision conversions and the device math probe.
-// Built with -ffp-contract=off like every FP64 kernel.
+// kernels_util.hip -- precision conversions, run resets and the device math
+// probe.  Built with -ffp-contract=off like every FP64 kernel.
 #include "gm_kernels.h"
 
 namespace gmd
 {
 
-__global__ void toF32Kernel(const double4* __restrict__ in, float4* __restrict__ out, uint32_t n)
+__global__ void toF32Kernel(const DevMesh* __restrict__ meshes, const uint2* __restrict__ blocks)
 {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if(i >= n) return;
-    double4 a = in[i];
-    out[i] = make_float4((float)a.x, (float)a.y, (float)a.z, 0.0f);
+    const uint2 bl = blocks[blockIdx.x];
+    const DevMesh& m = meshes[bl.x];
+    uint32_t i = bl.y + threadIdx.x;
+    if(i >= m.nverts) return;
+    double4 a = m.pos64[i];
+    m.pos32[i] = make_float4((float)a.x, (float)a.y, (float)a.z, 0.0f);
 }
 
-__global__ void toF64Kernel(const float4* __restrict__ in, double4* __restrict__ out, uint32_t n)
+__global__ void toF64Kernel(const DevMesh* __restrict__ meshes, const uint2* __restrict__ blocks)
 {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if(i >= n) return;
-    float4 a = in[i];
-    out[i] = make_double4((double)a.x, (double)a.y, (double)a.z, 0.0);
+    const uint2 bl = blocks[blockIdx.x];
+    const DevMesh& m = meshes[bl.x];
+    uint32_t i = bl.y + threadIdx.x;
+    if(i >= m.nverts) return;
+    float4 a = m.pos32[i];
+    m.pos64[i] = make_double4((double)a.x, (double)a.y, (double)a.z, 0.0);
 }
 
-hipError_t launchToF32(const DevMesh& m, hipStream_t stream)
+__global__ void resetCountsKernel(const DevMesh* __restrict__ meshes, const uint2* __restrict__ blocks)
 {
-    if(m.nverts == 0) return hipSuccess;
-    toF32Kernel<<<(m.nverts + 255) / 256, 256, 0, stream>>>(m.pos64, m.pos32, m.nverts);
+    const uint2 bl = blocks[blockIdx.x];
+    const DevMesh& m = meshes[bl.x];
+    uint32_t i = bl.y + threadIdx.x;
+    if(i < m.nverts) m.evalCounts[i] = 0;
+}
+
+__global__ void resetSlotsKernel(const DevMesh* __restrict__ meshes, const uint32_t* __restrict__ meshList,
+                                 uint32_t nmeshes)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if(i >= nmeshes) return;
+    const DevMesh& m = meshes[meshList[i]];
+    *m.passSlot = 0;
+    for(int b = 0; b < 20; ++b) m.evalHist[b] = 0;
+}
+
+hipError_t launchToF32(const DevMesh* meshes, const uint2* blocks, uint32_t nblocks, hipStream_t stream)
+{
+    if(nblocks == 0) return hipSuccess;
+    toF32Kernel<<<nblocks, 256, 0, stream>>>(meshes, blocks);
     return hipGetLastError();
 }
 
-hipError_t launchToF64(const DevMesh& m, hipStream_t stream)
+hipError_t launchToF64(const DevMesh* meshes, const uint2* blocks, uint32_t nblocks, hipStream_t stream)
 {
-    if(m.nverts == 0) return hipSuccess;
-    toF64Kernel<<<(m.nverts + 255) / 256, 256, 0, stream>>>(m.pos32, m.pos64, m.nverts);
+    if(nblocks == 0) return hipSuccess;
+    toF64Kernel<<<nblocks, 256, 0, stream>>>(meshes, blocks);
     return hipGetLastError();
+}
+
+hipError_t launchResetRun(const DevMesh* meshes, const uint2* vertexBlocks, uint32_t nblocks,
+                          const uint32_t* meshList, uint32_t nmeshes, hipStream_t stream)
+{
+    if(nblocks) resetCountsKernel<<<nblocks, 256, 0, stream>>>(meshes, vertexBlocks);
+    if(nmeshes) resetSlotsKernel<<<(nmeshes + 255) / 256, 256, 0, stream>>>(meshes, meshList, nmeshes);
+    return hipGetLastError();
+}
+
+bool laneFits(const VertexWork& w, int smoother, LaneShape lane, bool fp64)
+{
+    return fp64 ? laneFits<double>(w, smoother, lane) : laneFits<float>(w, smoother, lane);
 }
 
 __global__ void mathProbeKernel(int op, const double* a, const double* b, double* out, int n)

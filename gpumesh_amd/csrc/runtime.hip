@@ -1,0 +1,450 @@
+// runtime.hip -- fused execution plans over resident meshes and the pass
+// loop of AbstractSmoother::smoothMesh (Smoothers/AbstractSmoother.cpp:57-211)
+// on the device.  A mesh handle runs as a plan of one; a batch is a plan of
+// many.  Everything a plan launches reads the per-mesh descriptors from a
+// device table, so a mesh's kernels, launch order within the mesh and
+// reduction trees are the same alone and batched.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "gm_runtime.h"
+
+namespace gmr
+{
+
+static thread_local std::string g_lastError;
+
+int fail(int code, const std::string& msg)
+{
+    g_lastError = msg;
+    return code;
+}
+
+const char* lastError() { return g_lastError.c_str(); }
+
+#define GMR_HIP(call)                                                                  \
+    do {                                                                               \
+        hipError_t e_ = (call);                                                        \
+        if(e_ != hipSuccess)                                                           \
+            return fail(e_ == hipErrorOutOfMemory ? GM_ERR_NOMEM : GM_ERR_HIP,         \
+                        std::string(#call) + ": " + hipGetErrorString(e_));            \
+    } while(0)
+
+template<typename X> static hipError_t devAlloc(X** p, size_t count)
+{
+    *p = nullptr;
+    return hipMalloc((void**)p, (count ? count : 1) * sizeof(X));
+}
+
+template<typename X> static int planUpload(Plan* p, X** ptr, const std::vector<X>& host)
+{
+    GMR_HIP(devAlloc(ptr, host.size()));
+    p->owned.push_back(*ptr);
+    if(!host.empty())
+        GMR_HIP(hipMemcpyAsync(*ptr, host.data(), host.size() * sizeof(X), hipMemcpyHostToDevice, p->stream));
+    return GM_OK;
+}
+
+// Patch-size boundaries at which the vertex kernels change shape: wave
+// kernel lanes per vertex (8, 16, 32, 64), gradient-descent tile slots
+// (8, 16, 32) and spawn-search tiles (64, 128, 256).
+static const int PATCH_CLASSES = 7;
+static int patchClass(int maxPatch)
+{
+    const int bounds[PATCH_CLASSES - 1] = {8, 16, 32, 64, 128, 256};
+    int c = 0;
+    while(c < PATCH_CLASSES - 1 && maxPatch > bounds[c]) ++c;
+    return c;
+}
+
+static int buildPlanInto(Plan* p, const std::vector<gm_mesh*>& meshes)
+{
+    int64_t maxColours = 0;
+    for(gm_mesh* m : meshes) maxColours = std::max(maxColours, m->topo.ngroups());
+    p->colours.resize((size_t)maxColours);
+    std::vector<uint2> tb, hb, eb, vb;
+    std::vector<uint32_t> ml;
+    for(size_t i = 0; i < meshes.size(); ++i)
+    {
+        const gm_mesh* m = meshes[i];
+        const gmt::Topology& t = m->topo;
+        for(uint32_t e = 0; e < (uint32_t)t.ntets; e += gmd::ELEM_BLOCK) tb.push_back(make_uint2((uint32_t)i, e));
+        for(uint32_t e = 0; e < (uint32_t)t.nhexs; e += gmd::ELEM_BLOCK) hb.push_back(make_uint2((uint32_t)i, e));
+        for(uint32_t b = 0; b < m->dm.evalBlocks; ++b) eb.push_back(make_uint2((uint32_t)i, b));
+        for(uint32_t v = 0; v < (uint32_t)t.nverts; v += 256) vb.push_back(make_uint2((uint32_t)i, v));
+        ml.push_back((uint32_t)i);
+    }
+    int rc;
+    for(int64_t g = 0; g < maxColours; ++g)
+    {
+        // One list per (element kind, patch class).  The kind keeps the
+        // candidate-parallel kernels on patches of a single kind; the patch
+        // class (largest patch of the member's colour group, rounded up to
+        // the kernels' tile boundaries) makes each list pick, for every
+        // member, the kernel that member picks when it runs alone.  Within a
+        // list, items keep the member order and each member's group order.
+        for(int kind = gmd::EK_TET; kind <= gmd::EK_MIXED; ++kind)
+            for(int cls = 0; cls < PATCH_CLASSES; ++cls)
+            {
+                std::vector<uint2> items;
+                int maxPatch = 0;
+                for(size_t i = 0; i < meshes.size(); ++i)
+                {
+                    const gm_mesh* m = meshes[i];
+                    const gmt::Topology& t = m->topo;
+                    if(m->kind != kind || g >= t.ngroups() || patchClass(t.groupMaxPatch[g]) != cls) continue;
+                    for(int64_t k = t.groupOff[g]; k < t.groupOff[g + 1]; ++k)
+                        items.push_back(make_uint2((uint32_t)i, t.groupNodes[k]));
+                    maxPatch = std::max(maxPatch, (int)t.groupMaxPatch[g]);
+                }
+                if(items.empty()) continue;
+                VList l;
+                l.count = (uint32_t)items.size();
+                l.kind = kind;
+                l.maxPatch = maxPatch;
+                if((rc = planUpload(p, &l.items, items))) return rc;
+                p->colours[g].push_back(l);
+            }
+    }
+    if((rc = planUpload(p, &p->tetBlocks, tb))) return rc;
+    if((rc = planUpload(p, &p->hexBlocks, hb))) return rc;
+    if((rc = planUpload(p, &p->evalBlocks, eb))) return rc;
+    if((rc = planUpload(p, &p->vertBlocks, vb))) return rc;
+    if((rc = planUpload(p, &p->meshList, ml))) return rc;
+    p->nTetBlocks = (uint32_t)tb.size();
+    p->nHexBlocks = (uint32_t)hb.size();
+    p->nEvalBlocks = (uint32_t)eb.size();
+    p->nVertBlocks = (uint32_t)vb.size();
+    p->nMeshes = (uint32_t)ml.size();
+    GMR_HIP(devAlloc(&p->dTable, meshes.size()));
+    GMR_HIP(hipEventCreate(&p->ev[0]));
+    GMR_HIP(hipEventCreate(&p->ev[1]));
+    GMR_HIP(hipStreamSynchronize(p->stream));
+    return GM_OK;
+}
+
+int buildPlan(const std::vector<gm_mesh*>& meshes, hipStream_t stream, Plan** out)
+{
+    *out = nullptr;
+    if(meshes.empty()) return fail(GM_ERR_ARG, "a plan needs at least one mesh");
+    for(const gm_mesh* m : meshes)
+        if((uint64_t)m->topo.nverts >= (1ull << 32) || (uint64_t)m->topo.ntets >= (1ull << 32) ||
+           (uint64_t)m->topo.nhexs >= (1ull << 32))
+            return fail(GM_ERR_ARG, "mesh too large for 32-bit work items");
+    if(meshes.size() >= (1ull << 32)) return fail(GM_ERR_ARG, "too many meshes");
+    Plan* p = new Plan();
+    p->stream = stream;
+    p->meshes = meshes;
+    p->device = meshes[0]->device;
+    int rc = buildPlanInto(p, meshes);
+    if(rc)
+    {
+        freePlan(p);
+        return rc;
+    }
+    *out = p;
+    return GM_OK;
+}
+
+void clearGraphs(Plan* p)
+{
+    for(auto& kv : p->graphs) (void)hipGraphExecDestroy(kv.second);
+    for(hipGraph_t g : p->graphDefs) (void)hipGraphDestroy(g);
+    p->graphs.clear();
+    p->graphDefs.clear();
+}
+
+void freePlan(Plan* p)
+{
+    if(!p) return;
+    clearGraphs(p);
+    for(void* q : p->owned) (void)hipFree(q);
+    if(p->dTable) (void)hipFree(p->dTable);
+    for(hipEvent_t e : p->ev) if(e) (void)hipEventDestroy(e);
+    for(hipEvent_t e : p->pool) if(e) (void)hipEventDestroy(e);
+    delete p;
+}
+
+static int ensureRecords(gm_mesh* m, int records)
+{
+    if(records <= m->resultCap) return GM_OK;
+    (void)hipFree(m->dm.evalResult);
+    (void)hipFree(m->dm.evalHistOut);
+    m->dm.evalResult = nullptr;
+    m->dm.evalHistOut = nullptr;
+    m->resultCap = 0;
+    m->dm.resultCap = 0;
+    GMR_HIP(devAlloc(&m->dm.evalResult, 3 * (size_t)records));
+    GMR_HIP(devAlloc(&m->dm.evalHistOut, 20 * (size_t)records));
+    m->resultCap = records;
+    m->dm.resultCap = (uint32_t)records;
+    return GM_OK;
+}
+
+static int ensureGetme(gm_mesh* m)
+{
+    if(m->getmeReady) return GM_OK;
+    // (vpp, weight) per element corner, sized for FP64; FP32 runs use the
+    // front half of each buffer.
+    const size_t tetBytes = 4 * (size_t)m->topo.ntets * sizeof(double4);
+    const size_t hexBytes = 8 * (size_t)m->topo.nhexs * sizeof(double4);
+    GMR_HIP(hipMalloc(&m->dm.getmeTet, std::max<size_t>(32, tetBytes)));
+    GMR_HIP(hipMalloc(&m->dm.getmeHex, std::max<size_t>(32, hexBytes)));
+    m->deviceBytes += tetBytes + hexBytes;
+    m->getmeReady = true;
+    return GM_OK;
+}
+
+int preparePlan(Plan* p, int records, bool getme)
+{
+    for(gm_mesh* m : p->meshes)
+    {
+        int rc = ensureRecords(m, records);
+        if(rc) return rc;
+        if(getme && (rc = ensureGetme(m))) return rc;
+    }
+    p->table.resize(p->meshes.size());
+    for(size_t i = 0; i < p->meshes.size(); ++i) p->table[i] = p->meshes[i]->dm;
+    GMR_HIP(hipMemcpyAsync(p->dTable, p->table.data(), p->table.size() * sizeof(gmd::DevMesh),
+                           hipMemcpyHostToDevice, p->stream));
+    GMR_HIP(hipStreamSynchronize(p->stream));
+    return GM_OK;
+}
+
+// ----- per-launch timing (eager launches only)
+static int timedBegin(Plan* p, bool on, int tag)
+{
+    if(!on) return -1;
+    if(p->poolUsed + 2 > p->pool.size())
+    {
+        size_t old = p->pool.size();
+        p->pool.resize(old + 256, nullptr);
+        p->poolTag.resize(old + 256, -1);
+        for(size_t i = old; i < p->pool.size(); ++i)
+            if(hipEventCreate(&p->pool[i]) != hipSuccess)
+            {
+                p->pool.resize(i);
+                p->poolTag.resize(i);
+                break;
+            }
+        if(p->poolUsed + 2 > p->pool.size()) return -1;
+    }
+    int slot = (int)p->poolUsed;
+    p->poolUsed += 2;
+    p->poolTag[slot] = tag;
+    (void)hipEventRecord(p->pool[slot], p->stream);
+    return slot;
+}
+
+static void timedEnd(Plan* p, int slot)
+{
+    if(slot >= 0) (void)hipEventRecord(p->pool[slot + 1], p->stream);
+}
+
+static void collectTimings(Plan* p)
+{
+    for(int k = 0; k < GM_KT_COUNT; ++k)
+    {
+        p->kernMs[k] = 0.0;
+        p->kernLaunches[k] = 0;
+    }
+    for(size_t i = 0; i + 1 < p->poolUsed; i += 2)
+    {
+        float ms = 0.0f;
+        if(p->poolTag[i] >= 0 && hipEventElapsedTime(&ms, p->pool[i], p->pool[i + 1]) == hipSuccess)
+        {
+            p->kernMs[p->poolTag[i]] += ms;
+            p->kernLaunches[p->poolTag[i]] += 1;
+        }
+    }
+    p->poolUsed = 0;
+}
+
+static hipError_t evaluateAll(Plan* p, const gmq::Crew& c, int ev, int sa, bool fromF32, bool timed)
+{
+    int slot = timedBegin(p, timed, GM_KT_EVALUATE);
+    hipError_t e = fromF32
+        ? gmd::launchEvaluate<float>(p->dTable, p->evalBlocks, p->nEvalBlocks, p->meshList, p->nMeshes, c, ev, sa,
+                                     p->stream)
+        : gmd::launchEvaluate<double>(p->dTable, p->evalBlocks, p->nEvalBlocks, p->meshList, p->nMeshes, c, ev, sa,
+                                      p->stream);
+    timedEnd(p, slot);
+    return e;
+}
+
+// One sweep over colours [g0, g1) of every member (GETMe: the element step
+// over all members first, as GetmeSmoother::smoothMeshThread does per mesh).
+static hipError_t sweep(Plan* p, const gmq::Crew& c, int ev, int sa, const RunParams& r, bool f64, int64_t g0,
+                        int64_t g1, bool timed)
+{
+    hipError_t e = hipSuccess;
+    const bool getme = r.smoother == gmd::SM_GETME;
+    if(getme)
+    {
+        int slot = timedBegin(p, timed, f64 ? GM_KT_GETME_ELEM_F64 : GM_KT_GETME_ELEM_F32);
+        e = f64 ? gmd::launchGetmeElements<double>(p->dTable, p->tetBlocks, p->nTetBlocks, p->hexBlocks,
+                                                   p->nHexBlocks, c, ev, sa, p->stream)
+                : gmd::launchGetmeElements<float>(p->dTable, p->tetBlocks, p->nTetBlocks, p->hexBlocks,
+                                                  p->nHexBlocks, c, ev, sa, p->stream);
+        timedEnd(p, slot);
+    }
+    for(int64_t g = g0; e == hipSuccess && g < g1; ++g)
+    {
+        for(const VList& l : p->colours[g])
+        {
+            gmd::VertexWork w{p->dTable, l.items, l.count, l.kind, l.maxPatch};
+            int tag = getme ? (f64 ? GM_KT_GETME_VERT_F64 : GM_KT_GETME_VERT_F32)
+                            : (f64 ? GM_KT_SMOOTH_F64 : GM_KT_SMOOTH_F32);
+            int slot = timedBegin(p, timed, tag);
+            if(getme)
+                e = f64 ? gmd::launchGetmeVertices<double>(w, c, ev, sa, p->stream)
+                        : gmd::launchGetmeVertices<float>(w, c, ev, sa, p->stream);
+            else
+                e = f64 ? gmd::launchSmooth<double>(w, c, ev, sa, r.smoother, r.kernel, r.lane, r.spawnCoeff,
+                                                    p->stream)
+                        : gmd::launchSmooth<float>(w, c, ev, sa, r.smoother, r.kernel, r.lane, r.spawnCoeff,
+                                                   p->stream);
+            timedEnd(p, slot);
+            if(e != hipSuccess) break;
+        }
+    }
+    return e;
+}
+
+// One pass body: the sweep (FP64 or FP32) and the FP64 quality reduction of
+// its result.  With r.graph the body is captured once per configuration and
+// replayed; the reduction's record slot advances on the device, so every
+// replay writes the next record.
+static int passBody(Plan* p, const gmq::Crew& c, int ev, int sa, const RunParams& r, bool f64, bool evalF32)
+{
+    const int64_t G = (int64_t)p->colours.size();
+    if(!r.graph)
+    {
+        hipError_t e = sweep(p, c, ev, sa, r, f64, 0, G, r.timeKernels);
+        if(e == hipSuccess) e = evaluateAll(p, c, ev, sa, evalF32, r.timeKernels);
+        if(e != hipSuccess) return fail(GM_ERR_HIP, std::string("pass launch: ") + hipGetErrorString(e));
+        return GM_OK;
+    }
+    if(!p->graphs.empty() && (p->graphEv != ev || p->graphSa != sa || std::memcmp(&p->graphCrew, &c, sizeof(c)) != 0))
+        clearGraphs(p);
+    p->graphCrew = c;
+    p->graphEv = ev;
+    p->graphSa = sa;
+    uint64_t coeffBits;
+    std::memcpy(&coeffBits, &r.spawnCoeff, sizeof(coeffBits));
+    auto key = std::make_tuple(r.smoother, (int)f64, r.kernel, r.lane.patchesPerGroup, r.lane.tileElems, coeffBits,
+                               (int)evalF32);
+    auto it = p->graphs.find(key);
+    if(it == p->graphs.end())
+    {
+        hipGraph_t graph = nullptr;
+        GMR_HIP(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal));
+        hipError_t e = sweep(p, c, ev, sa, r, f64, 0, G, false);
+        if(e == hipSuccess) e = evaluateAll(p, c, ev, sa, evalF32, false);
+        hipError_t e2 = hipStreamEndCapture(p->stream, &graph);
+        if(e != hipSuccess || e2 != hipSuccess)
+        {
+            if(graph) (void)hipGraphDestroy(graph);
+            return fail(GM_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e != hipSuccess ? e : e2));
+        }
+        hipGraphExec_t exec = nullptr;
+        e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        if(e != hipSuccess)
+        {
+            (void)hipGraphDestroy(graph);
+            return fail(GM_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(e));
+        }
+        p->graphDefs.push_back(graph);
+        it = p->graphs.emplace(key, exec).first;
+    }
+    GMR_HIP(hipGraphLaunch(it->second, p->stream));
+    return GM_OK;
+}
+
+int runPlan(Plan* p, const gmq::Crew& c, int ev, int sa, const RunParams& r)
+{
+    int rc = preparePlan(p, r.passes + 1, r.smoother == gmd::SM_GETME);
+    if(rc) return rc;
+    hipStream_t s = p->stream;
+    const bool timed = r.timeKernels && !r.graph;
+    p->poolUsed = 0;
+    GMR_HIP(hipEventRecord(p->ev[0], s));
+    GMR_HIP(gmd::launchResetRun(p->dTable, p->vertBlocks, p->nVertBlocks, p->meshList, p->nMeshes, s));
+    bool inF32 = false;
+    if(r.precision != GM_FP64)
+    {
+        GMR_HIP(gmd::launchToF32(p->dTable, p->vertBlocks, p->nVertBlocks, s));
+        inF32 = true;
+    }
+    GMR_HIP(evaluateAll(p, c, ev, sa, inF32, timed));
+    for(int pass = 0; pass < r.passes; ++pass)
+    {
+        const bool f64 = r.precision == GM_FP64 || (r.precision == GM_FP32_POLISH && pass == r.passes - 1);
+        if(f64 && inF32)
+        {
+            GMR_HIP(gmd::launchToF64(p->dTable, p->vertBlocks, p->nVertBlocks, s));
+            inF32 = false;
+        }
+        RunParams rr = r;
+        rr.timeKernels = timed;
+        if((rc = passBody(p, c, ev, sa, rr, f64, inF32))) return rc;
+    }
+    if(inF32)
+        GMR_HIP(gmd::launchToF64(p->dTable, p->vertBlocks, p->nVertBlocks, s));
+    GMR_HIP(hipEventRecord(p->ev[1], s));
+    GMR_HIP(hipStreamSynchronize(s));
+    float ms = 0.0f;
+    GMR_HIP(hipEventElapsedTime(&ms, p->ev[0], p->ev[1]));
+    p->totalMs = ms;
+    collectTimings(p);
+    return GM_OK;
+}
+
+int sweepColours(Plan* p, const gmq::Crew& c, int ev, int sa, const RunParams& r, int64_t g0, int64_t g1)
+{
+    int rc = preparePlan(p, 1, false);
+    if(rc) return rc;
+    hipStream_t s = p->stream;
+    p->poolUsed = 0;
+    GMR_HIP(gmd::launchResetRun(p->dTable, p->vertBlocks, p->nVertBlocks, p->meshList, p->nMeshes, s));
+    const bool f64 = r.precision == GM_FP64;
+    if(!f64) GMR_HIP(gmd::launchToF32(p->dTable, p->vertBlocks, p->nVertBlocks, s));
+    hipError_t e = sweep(p, c, ev, sa, r, f64, std::min<int64_t>(g0, (int64_t)p->colours.size()),
+                         std::min<int64_t>(g1, (int64_t)p->colours.size()), false);
+    if(e != hipSuccess) return fail(GM_ERR_HIP, std::string("sweep: ") + hipGetErrorString(e));
+    if(!f64) GMR_HIP(gmd::launchToF64(p->dTable, p->vertBlocks, p->nVertBlocks, s));
+    GMR_HIP(hipStreamSynchronize(s));
+    return GM_OK;
+}
+
+int fetchRecords(Plan* p, size_t i, int records, double* stats, int32_t* hist, int32_t* evalCounts)
+{
+    gm_mesh* m = p->meshes[i];
+    std::vector<double> r(3 * (size_t)records);
+    std::vector<int32_t> h(20 * (size_t)records);
+    GMR_HIP(hipMemcpyAsync(r.data(), m->dm.evalResult, r.size() * sizeof(double), hipMemcpyDeviceToHost, p->stream));
+    GMR_HIP(hipMemcpyAsync(h.data(), m->dm.evalHistOut, h.size() * sizeof(int32_t), hipMemcpyDeviceToHost,
+                           p->stream));
+    if(evalCounts && m->topo.nverts)
+        GMR_HIP(hipMemcpyAsync(evalCounts, m->dm.evalCounts, (size_t)m->topo.nverts * sizeof(int32_t),
+                               hipMemcpyDeviceToHost, p->stream));
+    GMR_HIP(hipStreamSynchronize(p->stream));
+    for(int k = 0; k < records; ++k)
+    {
+        if(stats)
+        {
+            // result record = {min, sum 1/q, n}; harmonic mean = n / sum 1/q.
+            stats[3 * k] = r[3 * k];
+            stats[3 * k + 1] = r[3 * k + 2] / r[3 * k + 1];
+            stats[3 * k + 2] = r[3 * k + 1];
+        }
+        if(hist) std::memcpy(hist + 20 * k, h.data() + 20 * k, 20 * sizeof(int32_t));
+    }
+    return GM_OK;
+}
+
+} // namespace gmr

@@ -47,6 +47,12 @@ class GpuMesh:
         (self.nverts, self.ntets, self.nhexs, self.nmovable, self.ngroups, self.sum_patch,
          self.sum_nbr, self.max_patch) = (int(x) for x in info)
 
+    def device_bytes(self) -> int:
+        """Device memory held by the handle."""
+        b = np.zeros(1, np.int64)
+        check(lib().gm_mesh_device_bytes(self._h, ptr(b)), "gm_mesh_device_bytes")
+        return int(b[0])
+
     @classmethod
     def from_cube(cls, m, device: bool = True):
         return cls(m.pos, m.fixed, m.tets, m.hexs, device=device)
@@ -150,4 +156,64 @@ class GpuMesh:
              "kernel_launches": dict(zip(KERNEL_CLASSES, kn.tolist()))}
         if counts:
             r["eval_counts"] = cnt
+        return r
+
+
+def _timings(L, fn_t, fn_k, h):
+    t = np.zeros(2, np.float64)
+    check(fn_t(h, ptr(t)), "last_timing")
+    kms = np.zeros(KERNEL_CLASSES_N, np.float64)
+    kn = np.zeros(KERNEL_CLASSES_N, np.int64)
+    check(fn_k(h, ptr(kms), ptr(kn)), "kernel_times")
+    return {"total_ms": float(t[0]), "sweep_ms": float(t[1]),
+            "kernel_ms": dict(zip(KERNEL_CLASSES, kms.tolist())),
+            "kernel_launches": dict(zip(KERNEL_CLASSES, kn.tolist()))}
+
+
+class GpuBatch:
+    """Several resident meshes smoothed by fused launches (one launch per
+    colour step and element kind over all members).  The members stay owned
+    by their GpuMesh objects, which must outlive the batch."""
+
+    def __init__(self, meshes):
+        self.meshes = list(meshes)
+        arr = (C.c_void_p * len(self.meshes))(*[m._h.value for m in self.meshes])
+        h = C.c_void_p()
+        check(lib().gm_batch_create(arr, len(self.meshes), C.byref(h)), "gm_batch_create")
+        self._h = h
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib().gm_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_shape(self, smoother: str, patches_per_group: int, tile_elems: int) -> None:
+        check(lib().gm_batch_set_shape(self._h, SMOOTHERS[smoother], int(patches_per_group), int(tile_elems)),
+              "gm_batch_set_shape")
+
+    def autotune(self, smoother: str, precision: str = "fp32", passes: int = 1):
+        """Times every launch shape; keeps the fastest.  Returns ((ppg, tile), times[4, 3] ms)."""
+        best = np.zeros(2, np.int32)
+        times = np.zeros(12, np.float64)
+        check(lib().gm_batch_autotune(self._h, SMOOTHERS[smoother], PRECISIONS[precision], int(passes), ptr(best),
+                                      ptr(times)), "gm_batch_autotune")
+        return (int(best[0]), int(best[1])), times.reshape(4, 3)
+
+    def smooth(self, smoother: str, passes: int, precision: str = "fp64", kernel: str = "wave",
+               spawn_coeff: float = 1.0 / 20.0, graph: bool = False):
+        n = len(self.meshes)
+        stats = np.zeros((n, passes + 1, 3), np.float64)
+        hist = np.zeros((n, passes + 1, 20), np.int32)
+        L = lib()
+        check(L.gm_batch_smooth(self._h, SMOOTHERS[smoother], int(passes), PRECISIONS[precision], KERNELS[kernel],
+                                float(spawn_coeff), int(bool(graph)), ptr(stats), ptr(hist)), "gm_batch_smooth")
+        r = {"min": stats[:, :, 0].copy(), "mean": stats[:, :, 1].copy(), "inv_sum": stats[:, :, 2].copy(),
+             "hist": hist}
+        r.update(_timings(L, L.gm_batch_last_timing, L.gm_batch_kernel_times, self._h))
         return r

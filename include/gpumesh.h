@@ -153,6 +153,50 @@ enum gm_kernel_class
  * launches[GM_KT_COUNT] (either may be NULL). */
 int gm_mesh_kernel_times(const gm_mesh* mesh, double* ms, int64_t* launches);
 
+/* Device memory owned by a mesh handle (positions, connectivity, patch
+ * lists, reduction buffers, GETMe scratch once allocated), in bytes. */
+int gm_mesh_device_bytes(const gm_mesh* mesh, int64_t* bytes);
+
+/* ---- Batches: many resident meshes smoothed by fused launches.
+ *
+ * The reference smooths one mesh per AbstractSmoother::smoothMesh call
+ * (Smoothers/AbstractSmoother.cpp:57-211); a batch runs that pass loop for
+ * n meshes at once: every colour step of every member goes into one launch
+ * per element kind, the GETMe element step and the quality reductions are
+ * one launch each over all members.  Members keep their own positions,
+ * statistics and pass records; each member's results are bit-identical to
+ * gm_mesh_smooth on that mesh alone (same kernels, same per-mesh reduction
+ * tree).  Members need the same evaluator, sampler and scaling (the texture
+ * sampler is not batched) and must live on one device.  A mesh may belong to
+ * several batches, but runs that touch one mesh must not overlap. */
+typedef struct gm_batch gm_batch;
+
+int gm_batch_create(gm_mesh* const* meshes, int64_t n, gm_batch** out);
+int gm_batch_destroy(gm_batch* batch);
+/* Launch shape of `smoother`'s candidate-parallel kernel (spawn search,
+ * gradient descent; the others ignore it): patches_per_group in {1,2,4,8}
+ * (wavefronts per workgroup), tile_elems in {64,128,256} (LDS elements per
+ * wavefront; gradient descent gives each of its 8 vertices per wavefront
+ * tile_elems/8).  A work list whose largest patch does not fit the tile runs
+ * on the wave kernel instead.  Until set (or autotuned), a batch uses the
+ * smallest tile that holds its largest patch, one patch group per workgroup. */
+int gm_batch_set_shape(gm_batch* batch, int smoother, int patches_per_group, int tile_elems);
+/* gm_mesh_smooth over every member.  use_graph != 0 captures one pass body
+ * (all sweeps + the quality reduction) in a HIP graph on first use and
+ * replays it.  stats[n][3*(passes+1)], hist[n][20*(passes+1)] (either may be
+ * NULL), member order. */
+int gm_batch_smooth(gm_batch* batch, int smoother, int passes, int precision, int kernel, double spawn_coeff,
+                    int use_graph, double* stats, int32_t* hist);
+/* out2 = {total device ms of the last gm_batch_smooth, sweep ms (0 with graphs)}. */
+int gm_batch_last_timing(const gm_batch* batch, double* out2);
+/* Per-class device time of the last gm_batch_smooth (not recorded with graphs). */
+int gm_batch_kernel_times(const gm_batch* batch, double* ms, int64_t* launches);
+/* Times `passes` sweeps of `smoother` for every launch shape (4 x 3 grid
+ * patches_per_group x tile_elems, times[12] = summed sweep kernel ms, -1
+ * where a shape does not fit every list), restores the positions after each
+ * trial, keeps the fastest shape for `smoother` and returns it in best[2]. */
+int gm_batch_autotune(gm_batch* batch, int smoother, int precision, int passes, int32_t* best, double* times);
+
 /* Device math probe (for tests): op 0 sqrt(a), 1 a/b, 2 a^(2/3), 3 a^(1/3),
  * 4 exp(a), all in FP64 on the device; n values. */
 int gm_math_probe(int op, const double* a, const double* b, double* out, int64_t n);
