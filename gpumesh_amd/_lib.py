@@ -25,7 +25,7 @@ EXPORTS = [
     "gm_mesh_topology", "gm_mesh_get_positions", "gm_mesh_set_positions", "gm_mesh_set_crew",
     "gm_mesh_evaluate", "gm_mesh_smooth", "gm_mesh_smooth_groups", "gm_mesh_last_timing", "gm_mesh_kernel_times",
     "gm_mesh_device_bytes", "gm_batch_create", "gm_batch_destroy", "gm_batch_set_shape", "gm_batch_smooth",
-    "gm_batch_last_timing", "gm_batch_kernel_times", "gm_batch_autotune", "gm_math_probe",
+    "gm_batch_last_timing", "gm_batch_kernel_times", "gm_batch_autotune", "gm_cube_mesh", "gm_math_probe",
 ]
 
 
@@ -72,6 +72,7 @@ def lib():
     L.gm_batch_last_timing.argtypes = [vp, vp]
     L.gm_batch_kernel_times.argtypes = [vp, vp, vp]
     L.gm_batch_autotune.argtypes = [vp, C.c_int, C.c_int, C.c_int, vp, vp]
+    L.gm_cube_mesh.argtypes = [C.c_int, C.c_int, dbl, C.c_uint64, dbl, vp, vp, vp]
     L.gm_math_probe.argtypes = [C.c_int, vp, vp, vp, i64]
     for name in EXPORTS:
         if name not in ("gm_last_error", "gm_version"):

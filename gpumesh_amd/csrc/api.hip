@@ -47,7 +47,7 @@ void freeMeshBuffers(gm_mesh* m)
 {
     gmd::DevMesh& d = m->dm;
     void* ptrs[] = {d.pos64, d.pos32, (void*)d.tets, (void*)d.hexs, (void*)d.eoff, (void*)d.eref,
-                    (void*)d.noff, (void*)d.nbr, (void*)d.fixed, d.evalCounts, d.getmeTet, d.getmeHex,
+                    (void*)d.noff, (void*)d.nbr, (void*)d.fixed, (void*)d.groupNodes, d.evalCounts, d.getmeTet, d.getmeHex,
                     d.evalPartials, d.evalFlags, d.evalHist, d.evalResult, d.evalHistOut, d.passSlot,
                     m->elemQualBuf, m->texels};
     for(void* p : ptrs)
@@ -240,13 +240,15 @@ int gm_mesh_create(int64_t nverts, const double* pos, const int32_t* fixed,
     GM_TRY(devAlloc(&no, V + 1));              d.noff = no;
     GM_TRY(devAlloc(&nb, t.nbr.size()));       d.nbr = nb;
     GM_TRY(devAlloc(&fx, V));                  d.fixed = fx;
+    uint32_t* gn;
+    GM_TRY(devAlloc(&gn, t.groupNodes.size())); d.groupNodes = gn;
     GM_TRY(devAlloc(&d.evalCounts, V));
     GM_TRY(devAlloc(&d.evalPartials, 3 * (size_t)d.evalBlocks));
     GM_TRY(devAlloc(&d.evalFlags, (size_t)d.evalBlocks));
     GM_TRY(devAlloc(&d.evalHist, 20));
     GM_TRY(devAlloc(&d.passSlot, 1));
     m->deviceBytes = V * (sizeof(double4) + sizeof(float4) + 4 + 4 + 4 + 1) + (size_t)ntets * 16 +
-                     (size_t)nhexs * 32 + t.eref.size() * 4 + t.nbr.size() * 4 +
+                     (size_t)nhexs * 32 + t.eref.size() * 4 + t.nbr.size() * 4 + t.groupNodes.size() * 4 +
                      (size_t)d.evalBlocks * 28 + 84;
 
     hipStream_t s = m->stream;
@@ -262,6 +264,8 @@ int gm_mesh_create(int64_t nverts, const double* pos, const int32_t* fixed,
     if(nhexs) GM_TRY(hipMemcpyAsync(hb, t.hexs.data(), (size_t)nhexs * 32, hipMemcpyHostToDevice, s));
     if(!t.eref.empty()) GM_TRY(hipMemcpyAsync(er, t.eref.data(), t.eref.size() * 4, hipMemcpyHostToDevice, s));
     if(!t.nbr.empty()) GM_TRY(hipMemcpyAsync(nb, t.nbr.data(), t.nbr.size() * 4, hipMemcpyHostToDevice, s));
+    if(!t.groupNodes.empty())
+        GM_TRY(hipMemcpyAsync(gn, t.groupNodes.data(), t.groupNodes.size() * 4, hipMemcpyHostToDevice, s));
     GM_TRY(hipStreamSynchronize(s));
 #undef GM_TRY
     std::vector<gm_mesh*> one{m};
@@ -422,9 +426,9 @@ int gm_mesh_evaluate(gm_mesh* m, double* out3, int32_t* hist20, double* elemQual
     int rc = gmr::preparePlan(p, 1, false);
     m->dm.elemQual = nullptr;
     if(rc) return rc;
-    GM_HIP(gmd::launchResetRun(p->dTable, p->vertBlocks, p->nVertBlocks, p->meshList, p->nMeshes, p->stream));
-    hipError_t e = gmd::launchEvaluate<double>(p->dTable, p->evalBlocks, p->nEvalBlocks, p->meshList, p->nMeshes,
-                                               m->crew, m->evaluator, m->sampler, p->stream);
+    GM_HIP(gmd::launchResetRun(p->dTable, p->vertBlocks, p->meshList, p->nMeshes, p->stream));
+    hipError_t e = gmd::launchEvaluate<double>(p->dTable, p->evalBlocks, p->meshList, p->nMeshes, m->crew,
+                                               m->evaluator, m->sampler, p->stream);
     if(e != hipSuccess) return fail(GM_ERR_HIP, std::string("evaluate: ") + hipGetErrorString(e));
     double r[3];
     int32_t h[20];
@@ -658,7 +662,7 @@ int gm_batch_autotune(gm_batch* b, int smoother, int precision, int passes, int3
             for(const auto& col : p->colours)
                 for(const gmr::VList& l : col)
                 {
-                    gmd::VertexWork w{p->dTable, l.items, l.count, l.kind, l.maxPatch};
+                    gmd::VertexWork w{p->dTable, l.segs, l.nsegs, l.count, l.kind, l.maxPatch};
                     if(l.kind != gmd::EK_MIXED && !gmd::laneFits(w, smoother, shape, precision == GM_FP64))
                         fits = false;
                 }

@@ -36,11 +36,10 @@ hipError_t launchSmooth(const VertexWork& w, const gmq::Crew& c, int evaluator, 
 }
 
 template<typename T>
-hipError_t launchGetmeElements(const DevMesh* meshes, const uint2* tetBlocks, uint32_t nTetBlocks,
-                               const uint2* hexBlocks, uint32_t nHexBlocks, const gmq::Crew& c,
+hipError_t launchGetmeElements(const DevMesh* meshes, BlockWork tets, BlockWork hexs, const gmq::Crew& c,
                                int evaluator, int sampler, hipStream_t stream)
 {
-    GMD_SELECT(evaluator, sampler, T, getmeElements(meshes, tetBlocks, nTetBlocks, hexBlocks, nHexBlocks, c, stream))
+    GMD_SELECT(evaluator, sampler, T, getmeElements(meshes, tets, hexs, c, stream))
 }
 
 template<typename T>
@@ -51,29 +50,29 @@ hipError_t launchGetmeVertices(const VertexWork& w, const gmq::Crew& c, int eval
 }
 
 template<>
-hipError_t launchEvaluate<double>(const DevMesh* meshes, const uint2* blocks, uint32_t nblocks,
-                                  const uint32_t* meshList, uint32_t nmeshes, const gmq::Crew& c, int evaluator,
-                                  int sampler, hipStream_t stream)
+hipError_t launchEvaluate<double>(const DevMesh* meshes, BlockWork blocks, const uint32_t* meshList,
+                                  uint32_t nmeshes, const gmq::Crew& c, int evaluator, int sampler,
+                                  hipStream_t stream)
 {
-    GMD_SELECT_EVAL(evaluator, sampler, evaluateF64(meshes, blocks, nblocks, meshList, nmeshes, c, stream))
+    GMD_SELECT_EVAL(evaluator, sampler, evaluateF64(meshes, blocks, meshList, nmeshes, c, stream))
 }
 
 template<>
-hipError_t launchEvaluate<float>(const DevMesh* meshes, const uint2* blocks, uint32_t nblocks,
-                                 const uint32_t* meshList, uint32_t nmeshes, const gmq::Crew& c, int evaluator,
-                                 int sampler, hipStream_t stream)
+hipError_t launchEvaluate<float>(const DevMesh* meshes, BlockWork blocks, const uint32_t* meshList,
+                                 uint32_t nmeshes, const gmq::Crew& c, int evaluator, int sampler,
+                                 hipStream_t stream)
 {
-    GMD_SELECT_EVAL(evaluator, sampler, evaluateF32(meshes, blocks, nblocks, meshList, nmeshes, c, stream))
+    GMD_SELECT_EVAL(evaluator, sampler, evaluateF32(meshes, blocks, meshList, nmeshes, c, stream))
 }
 
 template hipError_t launchSmooth<double>(const VertexWork&, const gmq::Crew&, int, int, int, int, LaneShape, double,
                                          hipStream_t);
 template hipError_t launchSmooth<float>(const VertexWork&, const gmq::Crew&, int, int, int, int, LaneShape, double,
                                         hipStream_t);
-template hipError_t launchGetmeElements<double>(const DevMesh*, const uint2*, uint32_t, const uint2*, uint32_t,
-                                                const gmq::Crew&, int, int, hipStream_t);
-template hipError_t launchGetmeElements<float>(const DevMesh*, const uint2*, uint32_t, const uint2*, uint32_t,
-                                               const gmq::Crew&, int, int, hipStream_t);
+template hipError_t launchGetmeElements<double>(const DevMesh*, BlockWork, BlockWork, const gmq::Crew&, int, int,
+                                                hipStream_t);
+template hipError_t launchGetmeElements<float>(const DevMesh*, BlockWork, BlockWork, const gmq::Crew&, int, int,
+                                               hipStream_t);
 template hipError_t launchGetmeVertices<double>(const VertexWork&, const gmq::Crew&, int, int, hipStream_t);
 template hipError_t launchGetmeVertices<float>(const VertexWork&, const gmq::Crew&, int, int, hipStream_t);
 

@@ -2,10 +2,13 @@
 // and the kernel translation units (kernels_inst.hip, dispatch.hip).
 //
 // Every launch is a fused launch over a table of resident meshes: a single
-// mesh is a batch of one.  Vertex work is a list of (mesh, vertex) items per
-// colour step; element and evaluation work is a list of (mesh, first index)
-// blocks.  Meshes never share vertices, so items of different meshes are
-// independent and any union of same-colour vertices is an independent set.
+// mesh is a batch of one.  Work is described by span tables, one span per
+// member: vertex work of a colour step is the concatenation of the members'
+// colour groups, block work (element, evaluation and vertex blocks) the
+// concatenation of the members' blocks.  A thread finds its span by binary
+// search, so the tables stay O(members) however large the meshes are, and a
+// new batch is planned without touching per-vertex data.  Meshes never share
+// vertices, so any union of same-colour groups is an independent set.
 #ifndef GM_DEVICE_H
 #define GM_DEVICE_H
 
@@ -48,6 +51,7 @@ struct DevMesh
     const uint32_t* noff;     // V+1
     const uint32_t* nbr;
     const uint8_t* fixed;     // 1 = fixed vertex (never moved; snaps to itself)
+    const uint32_t* groupNodes; // movable vertices, colour group by colour group
     int32_t* evalCounts;      // patch evaluations per vertex (accumulated)
     void* getmeTet;           // GETMe (vpp, weight) per tet corner, sized for FP64
     void* getmeHex;           // same per hex corner
@@ -78,14 +82,38 @@ struct LaneShape
     int tileElems;         // LDS elements staged per wavefront-vertex slot budget: 64, 128, 256 per block
 };
 
+// Span (x, y, z, w) = (mesh index, first group node, first index of the
+// span in the launch, unused).  Block spans use y = 0 and z = first block.
+// Spans are sorted by z; index i belongs to the last span with z <= i.
+__host__ __device__ inline uint32_t spanOf(const uint4* spans, uint32_t nspans, uint32_t i)
+{
+    uint32_t lo = 0, hi = nspans - 1;
+    while(lo < hi)
+    {
+        const uint32_t mid = (lo + hi + 1) >> 1;
+        if(spans[mid].z <= i) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
 // One fused vertex launch.
 struct VertexWork
 {
     const DevMesh* meshes;     // device table
-    const uint2* items;        // (mesh index, vertex id)
-    uint32_t count;
+    const uint4* segs;         // spans over the members' colour groups
+    uint32_t nsegs;
+    uint32_t count;            // vertices in the launch
     int kind;                  // EK_*
     int maxPatch;              // largest patch in the list
+};
+
+// Block work: `count` blocks spread over `nspans` members.
+struct BlockWork
+{
+    const uint4* spans;
+    uint32_t nspans;
+    uint32_t count;
 };
 
 // Kernel launchers of one (precision T, evaluator EV, sampler SA)
@@ -95,25 +123,22 @@ struct Launchers
 {
     static hipError_t smooth(const VertexWork& w, const gmq::Crew& c, int smoother, int shape,
                              LaneShape lane, double moveCoeff, hipStream_t stream);
-    static hipError_t getmeElements(const DevMesh* meshes, const uint2* tetBlocks, uint32_t nTetBlocks,
-                                    const uint2* hexBlocks, uint32_t nHexBlocks, const gmq::Crew& c,
+    static hipError_t getmeElements(const DevMesh* meshes, BlockWork tets, BlockWork hexs, const gmq::Crew& c,
                                     hipStream_t stream);
     static hipError_t getmeVertices(const VertexWork& w, const gmq::Crew& c, hipStream_t stream);
 };
 
 // FP64 mesh-quality reduction of one (evaluator, sampler) combination, reading
-// the FP64 (evaluateF64) or FP32 (evaluateF32) position copy.  `blocks` lists
-// (mesh index, block index within the mesh); `meshList` the meshes to finish.
-// Always built without contraction, with the FP64 kernels.
+// the FP64 (evaluateF64) or FP32 (evaluateF32) position copy.  `blocks` spans
+// the members' evaluation blocks; `meshList` the meshes to finish.  Always
+// built without contraction, with the FP64 kernels.
 template<int EV, int SA>
 struct Evaluators
 {
-    static hipError_t evaluateF64(const DevMesh* meshes, const uint2* blocks, uint32_t nblocks,
-                                  const uint32_t* meshList, uint32_t nmeshes, const gmq::Crew& c,
-                                  hipStream_t stream);
-    static hipError_t evaluateF32(const DevMesh* meshes, const uint2* blocks, uint32_t nblocks,
-                                  const uint32_t* meshList, uint32_t nmeshes, const gmq::Crew& c,
-                                  hipStream_t stream);
+    static hipError_t evaluateF64(const DevMesh* meshes, BlockWork blocks, const uint32_t* meshList,
+                                  uint32_t nmeshes, const gmq::Crew& c, hipStream_t stream);
+    static hipError_t evaluateF32(const DevMesh* meshes, BlockWork blocks, const uint32_t* meshList,
+                                  uint32_t nmeshes, const gmq::Crew& c, hipStream_t stream);
 };
 
 // Run-time selection over (evaluator, sampler) (dispatch.hip).
@@ -121,25 +146,24 @@ template<typename T>
 hipError_t launchSmooth(const VertexWork& w, const gmq::Crew& c, int evaluator, int sampler, int smoother,
                         int shape, LaneShape lane, double moveCoeff, hipStream_t stream);
 template<typename T>
-hipError_t launchGetmeElements(const DevMesh* meshes, const uint2* tetBlocks, uint32_t nTetBlocks,
-                               const uint2* hexBlocks, uint32_t nHexBlocks, const gmq::Crew& c,
+hipError_t launchGetmeElements(const DevMesh* meshes, BlockWork tets, BlockWork hexs, const gmq::Crew& c,
                                int evaluator, int sampler, hipStream_t stream);
 template<typename T>
 hipError_t launchGetmeVertices(const VertexWork& w, const gmq::Crew& c, int evaluator, int sampler,
                                hipStream_t stream);
 // P = position precision read (double: pos64, float: pos32); always FP64 math.
 template<typename P>
-hipError_t launchEvaluate(const DevMesh* meshes, const uint2* blocks, uint32_t nblocks, const uint32_t* meshList,
-                          uint32_t nmeshes, const gmq::Crew& c, int evaluator, int sampler, hipStream_t stream);
+hipError_t launchEvaluate(const DevMesh* meshes, BlockWork blocks, const uint32_t* meshList, uint32_t nmeshes,
+                          const gmq::Crew& c, int evaluator, int sampler, hipStream_t stream);
 
-// Position precision conversions over (mesh, vertex block) lists: blocks of
-// 256 vertices, (mesh index, first vertex).
-hipError_t launchToF32(const DevMesh* meshes, const uint2* blocks, uint32_t nblocks, hipStream_t stream);
-hipError_t launchToF64(const DevMesh* meshes, const uint2* blocks, uint32_t nblocks, hipStream_t stream);
+// Position precision conversions over blocks of VERT_BLOCK vertices.
+const int VERT_BLOCK = 256;
+hipError_t launchToF32(const DevMesh* meshes, BlockWork vertexBlocks, hipStream_t stream);
+hipError_t launchToF64(const DevMesh* meshes, BlockWork vertexBlocks, hipStream_t stream);
 // Zeroes evalCounts and sets passSlot = 0 for the meshes of a batch (fill
 // kernels, so that the reset can be captured in a graph).
-hipError_t launchResetRun(const DevMesh* meshes, const uint2* vertexBlocks, uint32_t nblocks,
-                          const uint32_t* meshList, uint32_t nmeshes, hipStream_t stream);
+hipError_t launchResetRun(const DevMesh* meshes, BlockWork vertexBlocks, const uint32_t* meshList,
+                          uint32_t nmeshes, hipStream_t stream);
 
 // True when the candidate-parallel kernel of `smoother` runs `w` with `lane`
 // (otherwise the wave kernel does).

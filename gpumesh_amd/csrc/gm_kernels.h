@@ -28,6 +28,20 @@ template<> struct PosOf<float>
     static float4* host(const DevMesh& m) { return m.pos32; }
 };
 
+// Work item i of a vertex launch: (mesh index, vertex id).
+__device__ __forceinline__ uint2 workItem(const DevMesh* meshes, const uint4* segs, uint32_t nsegs, uint32_t i)
+{
+    const uint4 s = segs[spanOf(segs, nsegs, i)];
+    return make_uint2(s.x, meshes[s.x].groupNodes[s.y + (i - s.z)]);
+}
+
+// Block b of a block launch: (mesh index, block index within the mesh).
+__device__ __forceinline__ uint2 blockItem(const uint4* spans, uint32_t nspans, uint32_t b)
+{
+    const uint4 s = spans[spanOf(spans, nspans, b)];
+    return make_uint2(s.x, b - s.z);
+}
+
 template<typename T, typename P> __device__ __forceinline__ v3<T> ld(const P* pos, uint32_t v)
 {
     P q = pos[v];
@@ -108,14 +122,14 @@ struct PlainPatch
 // Plain kernel: one vertex of the work list per lane.
 template<typename T, int EV, int SA, int SM>
 __global__ void __launch_bounds__(256) smoothPlainKernel(const DevMesh* __restrict__ meshes,
-                                                         const uint2* __restrict__ items, uint32_t count,
+                                                         const uint4* __restrict__ segs, uint32_t nsegs, uint32_t count,
                                                          gmq::Crew c, T moveCoeff)
 {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if(i >= count)
         return;
     typedef typename PosOf<T>::type P;
-    const uint2 it = items[i];
+    const uint2 it = workItem(meshes, segs, nsegs, i);
     const DevMesh& m = meshes[it.x];
     P* pos = PosOf<T>::get(m);
     const uint32_t v = it.y;
@@ -194,7 +208,7 @@ struct WavePatch
 
 template<typename T, int EV, int SA, int SM, int G>
 __global__ void __launch_bounds__(64) smoothWaveKernel(const DevMesh* __restrict__ meshes,
-                                                       const uint2* __restrict__ items, uint32_t count,
+                                                       const uint4* __restrict__ segs, uint32_t nsegs, uint32_t count,
                                                        gmq::Crew c, T moveCoeff)
 {
     typedef typename PosOf<T>::type P;
@@ -205,7 +219,7 @@ __global__ void __launch_bounds__(64) smoothWaveKernel(const DevMesh* __restrict
     // Groups past the end leave; groups never synchronise with each other.
     if(i >= count)
         return;
-    const uint2 it = items[i];
+    const uint2 it = workItem(meshes, segs, nsegs, i);
     const DevMesh& m = meshes[it.x];
     P* pos = PosOf<T>::get(m);
     const uint32_t v = it.y;
@@ -353,7 +367,7 @@ template<typename T, int EK> __host__ __device__ inline size_t stageBytes(int el
 // wavefronts (vertices) per block, each staging up to `cap` elements in LDS.
 template<typename T, int EV, int SA, int EK>
 __global__ void __launch_bounds__(512) spawnLaneKernel(const DevMesh* __restrict__ meshes,
-                                                       const uint2* __restrict__ items, uint32_t count,
+                                                       const uint4* __restrict__ segs, uint32_t nsegs, uint32_t count,
                                                        gmq::Crew c, T moveCoeff, int cap)
 {
     typedef typename PosOf<T>::type P;
@@ -366,7 +380,7 @@ __global__ void __launch_bounds__(512) spawnLaneKernel(const DevMesh* __restrict
     uint8_t* sk = base + (size_t)cap * Slots<EK>::N * 3 * sizeof(T);
     const uint32_t i = blockIdx.x * ppg + wave;
     const bool present = i < count;
-    const uint2 it = present ? items[i] : make_uint2(0, 0);
+    const uint2 it = present ? workItem(meshes, segs, nsegs, i) : make_uint2(0, 0);
     const DevMesh& m = meshes[it.x];
     P* pos = PosOf<T>::get(m);
     const uint32_t v = it.y;
@@ -400,7 +414,7 @@ __global__ void __launch_bounds__(512) spawnLaneKernel(const DevMesh* __restrict
 // points.  `ppg` wavefronts per block; each vertex stages up to `cap` elements.
 template<typename T, int EV, int SA, int EK>
 __global__ void __launch_bounds__(512) gdLaneKernel(const DevMesh* __restrict__ meshes,
-                                                    const uint2* __restrict__ items, uint32_t count,
+                                                    const uint4* __restrict__ segs, uint32_t nsegs, uint32_t count,
                                                     gmq::Crew c, int cap)
 {
     typedef typename PosOf<T>::type P;
@@ -413,7 +427,7 @@ __global__ void __launch_bounds__(512) gdLaneKernel(const DevMesh* __restrict__ 
     uint8_t* mySk = mine + (size_t)cap * SL * sizeof(T);
     const uint32_t i = blockIdx.x * (blockDim.x >> 3) + g;
     const bool present = i < count;
-    const uint2 it = present ? items[i] : make_uint2(0, 0);
+    const uint2 it = present ? workItem(meshes, segs, nsegs, i) : make_uint2(0, 0);
     const DevMesh& m = meshes[it.x];
     P* pos = PosOf<T>::get(m);
     const uint32_t v = it.y;
@@ -472,11 +486,12 @@ __global__ void __launch_bounds__(512) gdLaneKernel(const DevMesh* __restrict__ 
 // ---------------------------------------------------------------------------
 template<typename T, int EV, int SA>
 __global__ void __launch_bounds__(ELEM_BLOCK) getmeTetKernel(const DevMesh* __restrict__ meshes,
-                                                             const uint2* __restrict__ blocks, gmq::Crew c)
+                                                             const uint4* __restrict__ spans, uint32_t nspans,
+                                                             gmq::Crew c)
 {
-    const uint2 bl = blocks[blockIdx.x];
+    const uint2 bl = blockItem(spans, nspans, blockIdx.x);
     const DevMesh& m = meshes[bl.x];
-    const uint32_t e = bl.y + threadIdx.x;
+    const uint32_t e = bl.y * ELEM_BLOCK + threadIdx.x;
     if(e >= m.ntets)
         return;
     typedef typename PosOf<T>::type P;
@@ -515,11 +530,12 @@ __global__ void __launch_bounds__(ELEM_BLOCK) getmeTetKernel(const DevMesh* __re
 
 template<typename T, int EV, int SA>
 __global__ void __launch_bounds__(ELEM_BLOCK) getmeHexKernel(const DevMesh* __restrict__ meshes,
-                                                             const uint2* __restrict__ blocks, gmq::Crew c)
+                                                             const uint4* __restrict__ spans, uint32_t nspans,
+                                                             gmq::Crew c)
 {
-    const uint2 bl = blocks[blockIdx.x];
+    const uint2 bl = blockItem(spans, nspans, blockIdx.x);
     const DevMesh& m = meshes[bl.x];
-    const uint32_t e = bl.y + threadIdx.x;
+    const uint32_t e = bl.y * ELEM_BLOCK + threadIdx.x;
     if(e >= m.nhexs)
         return;
     typedef typename PosOf<T>::type P;
@@ -573,14 +589,14 @@ __global__ void __launch_bounds__(ELEM_BLOCK) getmeHexKernel(const DevMesh* __re
 // patch quality drops.
 template<typename T, int EV, int SA>
 __global__ void __launch_bounds__(512) getmeVertexKernel(const DevMesh* __restrict__ meshes,
-                                                         const uint2* __restrict__ items, uint32_t count,
+                                                         const uint4* __restrict__ segs, uint32_t nsegs, uint32_t count,
                                                          gmq::Crew c)
 {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if(i >= count)
         return;
     typedef typename PosOf<T>::type P;
-    const uint2 it = items[i];
+    const uint2 it = workItem(meshes, segs, nsegs, i);
     const DevMesh& m = meshes[it.x];
     P* pos = PosOf<T>::get(m);
     const P* tetOut = (const P*)m.getmeTet;
@@ -637,7 +653,8 @@ __device__ __forceinline__ void twoSum(double a, double b, double& s, double& e)
 
 template<int EV, int SA, typename P>
 __global__ void __launch_bounds__(EVAL_THREADS) evalKernel(const DevMesh* __restrict__ meshes,
-                                                           const uint2* __restrict__ blocks, gmq::Crew c)
+                                                           const uint4* __restrict__ spans, uint32_t nspans,
+                                                           gmq::Crew c)
 {
     __shared__ double smin[EVAL_THREADS], ssum[EVAL_THREADS], scmp[EVAL_THREADS];
     __shared__ int shist[20];
@@ -645,7 +662,7 @@ __global__ void __launch_bounds__(EVAL_THREADS) evalKernel(const DevMesh* __rest
     if(threadIdx.x < 20) shist[threadIdx.x] = 0;
     if(threadIdx.x == 0) sbad = 0;
     __syncthreads();
-    const uint2 bl = blocks[blockIdx.x];
+    const uint2 bl = blockItem(spans, nspans, blockIdx.x);
     const DevMesh& m = meshes[bl.x];
     const P* pos = PosOf<typename std::conditional<std::is_same<P, double4>::value, double, float>::type>::get(m);
     const uint64_t E = (uint64_t)m.ntets + m.nhexs;
@@ -786,7 +803,7 @@ static hipError_t launchLane(const VertexWork& w, const gmq::Crew& c, LaneShape 
         const int cap = lane.tileElems;
         const size_t lds = (size_t)ppg * stageBytes<T, EK>(cap);
         spawnLaneKernel<T, EV, SA, EK><<<(w.count + ppg - 1) / ppg, 64 * ppg, lds, stream>>>(
-            w.meshes, w.items, w.count, c, (T)moveCoeff, cap);
+            w.meshes, w.segs, w.nsegs, w.count, c, (T)moveCoeff, cap);
     }
     else
     {
@@ -794,7 +811,7 @@ static hipError_t launchLane(const VertexWork& w, const gmq::Crew& c, LaneShape 
         const int cap = lane.tileElems / 8;
         const size_t lds = (size_t)8 * ppg * stageBytes<T, EK>(cap);
         gdLaneKernel<T, EV, SA, EK><<<(w.count + 8 * ppg - 1) / (8 * ppg), 64 * ppg, lds, stream>>>(
-            w.meshes, w.items, w.count, c, cap);
+            w.meshes, w.segs, w.nsegs, w.count, c, cap);
     }
     return hipGetLastError();
 }
@@ -828,16 +845,16 @@ static hipError_t launchSmoothTyped(const VertexWork& w, const gmq::Crew& c, int
     if(shape == KS_WAVE && w.maxPatch <= 64)
     {
         if(w.maxPatch <= 8)
-            smoothWaveKernel<T, EV, SA, SM, 8><<<(w.count + 7) / 8, 64, 0, stream>>>(w.meshes, w.items, w.count, c, (T)moveCoeff);
+            smoothWaveKernel<T, EV, SA, SM, 8><<<(w.count + 7) / 8, 64, 0, stream>>>(w.meshes, w.segs, w.nsegs, w.count, c, (T)moveCoeff);
         else if(w.maxPatch <= 16)
-            smoothWaveKernel<T, EV, SA, SM, 16><<<(w.count + 3) / 4, 64, 0, stream>>>(w.meshes, w.items, w.count, c, (T)moveCoeff);
+            smoothWaveKernel<T, EV, SA, SM, 16><<<(w.count + 3) / 4, 64, 0, stream>>>(w.meshes, w.segs, w.nsegs, w.count, c, (T)moveCoeff);
         else if(w.maxPatch <= 32)
-            smoothWaveKernel<T, EV, SA, SM, 32><<<(w.count + 1) / 2, 64, 0, stream>>>(w.meshes, w.items, w.count, c, (T)moveCoeff);
+            smoothWaveKernel<T, EV, SA, SM, 32><<<(w.count + 1) / 2, 64, 0, stream>>>(w.meshes, w.segs, w.nsegs, w.count, c, (T)moveCoeff);
         else
-            smoothWaveKernel<T, EV, SA, SM, 64><<<w.count, 64, 0, stream>>>(w.meshes, w.items, w.count, c, (T)moveCoeff);
+            smoothWaveKernel<T, EV, SA, SM, 64><<<w.count, 64, 0, stream>>>(w.meshes, w.segs, w.nsegs, w.count, c, (T)moveCoeff);
         return hipGetLastError();
     }
-    smoothPlainKernel<T, EV, SA, SM><<<(w.count + 255) / 256, 256, 0, stream>>>(w.meshes, w.items, w.count, c, (T)moveCoeff);
+    smoothPlainKernel<T, EV, SA, SM><<<(w.count + 255) / 256, 256, 0, stream>>>(w.meshes, w.segs, w.nsegs, w.count, c, (T)moveCoeff);
     return hipGetLastError();
 }
 
@@ -855,14 +872,13 @@ hipError_t Launchers<T, EV, SA>::smooth(const VertexWork& w, const gmq::Crew& c,
 }
 
 template<typename T, int EV, int SA>
-hipError_t Launchers<T, EV, SA>::getmeElements(const DevMesh* meshes, const uint2* tetBlocks, uint32_t nTetBlocks,
-                                               const uint2* hexBlocks, uint32_t nHexBlocks, const gmq::Crew& c,
-                                               hipStream_t stream)
+hipError_t Launchers<T, EV, SA>::getmeElements(const DevMesh* meshes, BlockWork tets, BlockWork hexs,
+                                               const gmq::Crew& c, hipStream_t stream)
 {
-    if(nTetBlocks)
-        getmeTetKernel<T, EV, SA><<<nTetBlocks, ELEM_BLOCK, 0, stream>>>(meshes, tetBlocks, c);
-    if(nHexBlocks)
-        getmeHexKernel<T, EV, SA><<<nHexBlocks, ELEM_BLOCK, 0, stream>>>(meshes, hexBlocks, c);
+    if(tets.count)
+        getmeTetKernel<T, EV, SA><<<tets.count, ELEM_BLOCK, 0, stream>>>(meshes, tets.spans, tets.nspans, c);
+    if(hexs.count)
+        getmeHexKernel<T, EV, SA><<<hexs.count, ELEM_BLOCK, 0, stream>>>(meshes, hexs.spans, hexs.nspans, c);
     return hipGetLastError();
 }
 
@@ -871,30 +887,28 @@ hipError_t Launchers<T, EV, SA>::getmeVertices(const VertexWork& w, const gmq::C
 {
     if(w.count == 0)
         return hipSuccess;
-    getmeVertexKernel<T, EV, SA><<<(w.count + 255) / 256, 256, 0, stream>>>(w.meshes, w.items, w.count, c);
+    getmeVertexKernel<T, EV, SA><<<(w.count + 255) / 256, 256, 0, stream>>>(w.meshes, w.segs, w.nsegs, w.count, c);
     return hipGetLastError();
 }
 
 template<int EV, int SA>
-hipError_t Evaluators<EV, SA>::evaluateF64(const DevMesh* meshes, const uint2* blocks, uint32_t nblocks,
-                                           const uint32_t* meshList, uint32_t nmeshes, const gmq::Crew& c,
-                                           hipStream_t stream)
+hipError_t Evaluators<EV, SA>::evaluateF64(const DevMesh* meshes, BlockWork blocks, const uint32_t* meshList,
+                                           uint32_t nmeshes, const gmq::Crew& c, hipStream_t stream)
 {
-    if(nblocks == 0 || nmeshes == 0)
+    if(blocks.count == 0 || nmeshes == 0)
         return hipSuccess;
-    evalKernel<EV, SA, double4><<<nblocks, EVAL_THREADS, 0, stream>>>(meshes, blocks, c);
+    evalKernel<EV, SA, double4><<<blocks.count, EVAL_THREADS, 0, stream>>>(meshes, blocks.spans, blocks.nspans, c);
     evalFinalKernel<<<nmeshes, EVAL_THREADS, 0, stream>>>(meshes, meshList);
     return hipGetLastError();
 }
 
 template<int EV, int SA>
-hipError_t Evaluators<EV, SA>::evaluateF32(const DevMesh* meshes, const uint2* blocks, uint32_t nblocks,
-                                           const uint32_t* meshList, uint32_t nmeshes, const gmq::Crew& c,
-                                           hipStream_t stream)
+hipError_t Evaluators<EV, SA>::evaluateF32(const DevMesh* meshes, BlockWork blocks, const uint32_t* meshList,
+                                           uint32_t nmeshes, const gmq::Crew& c, hipStream_t stream)
 {
-    if(nblocks == 0 || nmeshes == 0)
+    if(blocks.count == 0 || nmeshes == 0)
         return hipSuccess;
-    evalKernel<EV, SA, float4><<<nblocks, EVAL_THREADS, 0, stream>>>(meshes, blocks, c);
+    evalKernel<EV, SA, float4><<<blocks.count, EVAL_THREADS, 0, stream>>>(meshes, blocks.spans, blocks.nspans, c);
     evalFinalKernel<<<nmeshes, EVAL_THREADS, 0, stream>>>(meshes, meshList);
     return hipGetLastError();
 }

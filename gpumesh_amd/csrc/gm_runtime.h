@@ -22,13 +22,15 @@ int fail(int code, const std::string& msg);
 // Message of the calling thread's last failure.
 const char* lastError();
 
-// A vertex work list of one colour step and one element kind.
+// A vertex work list of one colour step, element kind and patch class.
 struct VList
 {
-    uint2* items = nullptr;
+    const uint4* segs = nullptr;   // device spans (one per member with vertices in the list)
+    uint32_t nsegs = 0;
     uint32_t count = 0;
     int kind = gmd::EK_TET;
     int maxPatch = 0;
+    size_t firstSpan = 0;          // offset of segs in the plan's span buffer
 };
 
 struct RunParams
@@ -77,11 +79,9 @@ struct Plan
     std::vector<gm_mesh*> meshes;
     gmd::DevMesh* dTable = nullptr;
     std::vector<std::vector<VList>> colours;      // [colour][list]
-    uint2* tetBlocks = nullptr; uint32_t nTetBlocks = 0;
-    uint2* hexBlocks = nullptr; uint32_t nHexBlocks = 0;
-    uint2* evalBlocks = nullptr; uint32_t nEvalBlocks = 0;
-    uint2* vertBlocks = nullptr; uint32_t nVertBlocks = 0;
+    gmd::BlockWork tetBlocks{}, hexBlocks{}, evalBlocks{}, vertBlocks{};
     uint32_t* meshList = nullptr; uint32_t nMeshes = 0;
+    uint4* spans = nullptr;                       // every span table of the plan, one buffer
     std::vector<void*> owned;                     // device allocations of this plan
     // Graphs of one pass body, keyed by (smoother, fp64, kernel, ppg, tile, spawn coeff bits, eval f32).
     std::map<std::tuple<int, int, int, int, int, uint64_t, int>, hipGraphExec_t> graphs;

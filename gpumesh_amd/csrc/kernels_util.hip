@@ -5,31 +5,34 @@
 namespace gmd
 {
 
-__global__ void toF32Kernel(const DevMesh* __restrict__ meshes, const uint2* __restrict__ blocks)
+__global__ void __launch_bounds__(VERT_BLOCK) toF32Kernel(const DevMesh* __restrict__ meshes,
+                                                  const uint4* __restrict__ spans, uint32_t nspans)
 {
-    const uint2 bl = blocks[blockIdx.x];
+    const uint2 bl = blockItem(spans, nspans, blockIdx.x);
     const DevMesh& m = meshes[bl.x];
-    uint32_t i = bl.y + threadIdx.x;
+    uint32_t i = bl.y * VERT_BLOCK + threadIdx.x;
     if(i >= m.nverts) return;
     double4 a = m.pos64[i];
     m.pos32[i] = make_float4((float)a.x, (float)a.y, (float)a.z, 0.0f);
 }
 
-__global__ void toF64Kernel(const DevMesh* __restrict__ meshes, const uint2* __restrict__ blocks)
+__global__ void __launch_bounds__(VERT_BLOCK) toF64Kernel(const DevMesh* __restrict__ meshes,
+                                                  const uint4* __restrict__ spans, uint32_t nspans)
 {
-    const uint2 bl = blocks[blockIdx.x];
+    const uint2 bl = blockItem(spans, nspans, blockIdx.x);
     const DevMesh& m = meshes[bl.x];
-    uint32_t i = bl.y + threadIdx.x;
+    uint32_t i = bl.y * VERT_BLOCK + threadIdx.x;
     if(i >= m.nverts) return;
     float4 a = m.pos32[i];
     m.pos64[i] = make_double4((double)a.x, (double)a.y, (double)a.z, 0.0);
 }
 
-__global__ void resetCountsKernel(const DevMesh* __restrict__ meshes, const uint2* __restrict__ blocks)
+__global__ void __launch_bounds__(VERT_BLOCK) resetCountsKernel(const DevMesh* __restrict__ meshes,
+                                                  const uint4* __restrict__ spans, uint32_t nspans)
 {
-    const uint2 bl = blocks[blockIdx.x];
+    const uint2 bl = blockItem(spans, nspans, blockIdx.x);
     const DevMesh& m = meshes[bl.x];
-    uint32_t i = bl.y + threadIdx.x;
+    uint32_t i = bl.y * VERT_BLOCK + threadIdx.x;
     if(i < m.nverts) m.evalCounts[i] = 0;
 }
 
@@ -43,24 +46,24 @@ __global__ void resetSlotsKernel(const DevMesh* __restrict__ meshes, const uint3
     for(int b = 0; b < 20; ++b) m.evalHist[b] = 0;
 }
 
-hipError_t launchToF32(const DevMesh* meshes, const uint2* blocks, uint32_t nblocks, hipStream_t stream)
+hipError_t launchToF32(const DevMesh* meshes, BlockWork vb, hipStream_t stream)
 {
-    if(nblocks == 0) return hipSuccess;
-    toF32Kernel<<<nblocks, 256, 0, stream>>>(meshes, blocks);
+    if(vb.count == 0) return hipSuccess;
+    toF32Kernel<<<vb.count, VERT_BLOCK, 0, stream>>>(meshes, vb.spans, vb.nspans);
     return hipGetLastError();
 }
 
-hipError_t launchToF64(const DevMesh* meshes, const uint2* blocks, uint32_t nblocks, hipStream_t stream)
+hipError_t launchToF64(const DevMesh* meshes, BlockWork vb, hipStream_t stream)
 {
-    if(nblocks == 0) return hipSuccess;
-    toF64Kernel<<<nblocks, 256, 0, stream>>>(meshes, blocks);
+    if(vb.count == 0) return hipSuccess;
+    toF64Kernel<<<vb.count, VERT_BLOCK, 0, stream>>>(meshes, vb.spans, vb.nspans);
     return hipGetLastError();
 }
 
-hipError_t launchResetRun(const DevMesh* meshes, const uint2* vertexBlocks, uint32_t nblocks,
-                          const uint32_t* meshList, uint32_t nmeshes, hipStream_t stream)
+hipError_t launchResetRun(const DevMesh* meshes, BlockWork vb, const uint32_t* meshList, uint32_t nmeshes,
+                          hipStream_t stream)
 {
-    if(nblocks) resetCountsKernel<<<nblocks, 256, 0, stream>>>(meshes, vertexBlocks);
+    if(vb.count) resetCountsKernel<<<vb.count, VERT_BLOCK, 0, stream>>>(meshes, vb.spans, vb.nspans);
     if(nmeshes) resetSlotsKernel<<<(nmeshes + 255) / 256, 256, 0, stream>>>(meshes, meshList, nmeshes);
     return hipGetLastError();
 }

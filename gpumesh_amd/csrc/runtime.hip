@@ -40,15 +40,6 @@ template<typename X> static hipError_t devAlloc(X** p, size_t count)
     return hipMalloc((void**)p, (count ? count : 1) * sizeof(X));
 }
 
-template<typename X> static int planUpload(Plan* p, X** ptr, const std::vector<X>& host)
-{
-    GMR_HIP(devAlloc(ptr, host.size()));
-    p->owned.push_back(*ptr);
-    if(!host.empty())
-        GMR_HIP(hipMemcpyAsync(*ptr, host.data(), host.size() * sizeof(X), hipMemcpyHostToDevice, p->stream));
-    return GM_OK;
-}
-
 // Patch-size boundaries at which the vertex kernels change shape: wave
 // kernel lanes per vertex (8, 16, 32, 64), gradient-descent tile slots
 // (8, 16, 32) and spawn-search tiles (64, 128, 256).
@@ -61,24 +52,35 @@ static int patchClass(int maxPatch)
     return c;
 }
 
+static uint32_t ceilDiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+
+// Appends one block span per member with blocks; returns the work description
+// (span pointer filled in after upload).
+template<typename F>
+static gmd::BlockWork blockSpans(const std::vector<gm_mesh*>& meshes, std::vector<uint4>& spans, size_t& first,
+                                 F blocksOf)
+{
+    first = spans.size();
+    uint32_t total = 0;
+    for(size_t i = 0; i < meshes.size(); ++i)
+    {
+        const uint32_t nb = blocksOf(meshes[i]);
+        if(nb == 0) continue;
+        spans.push_back(make_uint4((uint32_t)i, 0, total, 0));
+        total += nb;
+    }
+    gmd::BlockWork w{};
+    w.nspans = (uint32_t)(spans.size() - first);
+    w.count = total;
+    return w;
+}
+
 static int buildPlanInto(Plan* p, const std::vector<gm_mesh*>& meshes)
 {
     int64_t maxColours = 0;
     for(gm_mesh* m : meshes) maxColours = std::max(maxColours, m->topo.ngroups());
     p->colours.resize((size_t)maxColours);
-    std::vector<uint2> tb, hb, eb, vb;
-    std::vector<uint32_t> ml;
-    for(size_t i = 0; i < meshes.size(); ++i)
-    {
-        const gm_mesh* m = meshes[i];
-        const gmt::Topology& t = m->topo;
-        for(uint32_t e = 0; e < (uint32_t)t.ntets; e += gmd::ELEM_BLOCK) tb.push_back(make_uint2((uint32_t)i, e));
-        for(uint32_t e = 0; e < (uint32_t)t.nhexs; e += gmd::ELEM_BLOCK) hb.push_back(make_uint2((uint32_t)i, e));
-        for(uint32_t b = 0; b < m->dm.evalBlocks; ++b) eb.push_back(make_uint2((uint32_t)i, b));
-        for(uint32_t v = 0; v < (uint32_t)t.nverts; v += 256) vb.push_back(make_uint2((uint32_t)i, v));
-        ml.push_back((uint32_t)i);
-    }
-    int rc;
+    std::vector<uint4> spans;
     for(int64_t g = 0; g < maxColours; ++g)
     {
         // One list per (element kind, patch class).  The kind keeps the
@@ -86,40 +88,49 @@ static int buildPlanInto(Plan* p, const std::vector<gm_mesh*>& meshes)
         // class (largest patch of the member's colour group, rounded up to
         // the kernels' tile boundaries) makes each list pick, for every
         // member, the kernel that member picks when it runs alone.  Within a
-        // list, items keep the member order and each member's group order.
+        // list, vertices keep the member order and each member's group order.
         for(int kind = gmd::EK_TET; kind <= gmd::EK_MIXED; ++kind)
             for(int cls = 0; cls < PATCH_CLASSES; ++cls)
             {
-                std::vector<uint2> items;
-                int maxPatch = 0;
+                VList l;
+                l.kind = kind;
+                l.firstSpan = spans.size();
                 for(size_t i = 0; i < meshes.size(); ++i)
                 {
                     const gm_mesh* m = meshes[i];
                     const gmt::Topology& t = m->topo;
                     if(m->kind != kind || g >= t.ngroups() || patchClass(t.groupMaxPatch[g]) != cls) continue;
-                    for(int64_t k = t.groupOff[g]; k < t.groupOff[g + 1]; ++k)
-                        items.push_back(make_uint2((uint32_t)i, t.groupNodes[k]));
-                    maxPatch = std::max(maxPatch, (int)t.groupMaxPatch[g]);
+                    const uint64_t n = (uint64_t)(t.groupOff[g + 1] - t.groupOff[g]);
+                    if(n == 0) continue;
+                    if((uint64_t)l.count + n >= (1ull << 32)) return fail(GM_ERR_ARG, "colour step too large");
+                    spans.push_back(make_uint4((uint32_t)i, (uint32_t)t.groupOff[g], l.count, 0));
+                    l.count += (uint32_t)n;
+                    l.maxPatch = std::max(l.maxPatch, (int)t.groupMaxPatch[g]);
                 }
-                if(items.empty()) continue;
-                VList l;
-                l.count = (uint32_t)items.size();
-                l.kind = kind;
-                l.maxPatch = maxPatch;
-                if((rc = planUpload(p, &l.items, items))) return rc;
-                p->colours[g].push_back(l);
+                l.nsegs = (uint32_t)(spans.size() - l.firstSpan);
+                if(l.count) p->colours[g].push_back(l);
             }
     }
-    if((rc = planUpload(p, &p->tetBlocks, tb))) return rc;
-    if((rc = planUpload(p, &p->hexBlocks, hb))) return rc;
-    if((rc = planUpload(p, &p->evalBlocks, eb))) return rc;
-    if((rc = planUpload(p, &p->vertBlocks, vb))) return rc;
-    if((rc = planUpload(p, &p->meshList, ml))) return rc;
-    p->nTetBlocks = (uint32_t)tb.size();
-    p->nHexBlocks = (uint32_t)hb.size();
-    p->nEvalBlocks = (uint32_t)eb.size();
-    p->nVertBlocks = (uint32_t)vb.size();
+    size_t fTet, fHex, fEval, fVert;
+    p->tetBlocks = blockSpans(meshes, spans, fTet, [](const gm_mesh* m) { return ceilDiv(m->topo.ntets, gmd::ELEM_BLOCK); });
+    p->hexBlocks = blockSpans(meshes, spans, fHex, [](const gm_mesh* m) { return ceilDiv(m->topo.nhexs, gmd::ELEM_BLOCK); });
+    p->evalBlocks = blockSpans(meshes, spans, fEval, [](const gm_mesh* m) { return m->dm.evalBlocks; });
+    p->vertBlocks = blockSpans(meshes, spans, fVert, [](const gm_mesh* m) { return ceilDiv(m->topo.nverts, gmd::VERT_BLOCK); });
+    std::vector<uint32_t> ml(meshes.size());
+    for(size_t i = 0; i < meshes.size(); ++i) ml[i] = (uint32_t)i;
+
+    GMR_HIP(devAlloc(&p->spans, spans.size()));
+    GMR_HIP(devAlloc(&p->meshList, ml.size()));
     p->nMeshes = (uint32_t)ml.size();
+    if(!spans.empty())
+        GMR_HIP(hipMemcpyAsync(p->spans, spans.data(), spans.size() * sizeof(uint4), hipMemcpyHostToDevice, p->stream));
+    GMR_HIP(hipMemcpyAsync(p->meshList, ml.data(), ml.size() * sizeof(uint32_t), hipMemcpyHostToDevice, p->stream));
+    for(auto& col : p->colours)
+        for(VList& l : col) l.segs = p->spans + l.firstSpan;
+    p->tetBlocks.spans = p->spans + fTet;
+    p->hexBlocks.spans = p->spans + fHex;
+    p->evalBlocks.spans = p->spans + fEval;
+    p->vertBlocks.spans = p->spans + fVert;
     GMR_HIP(devAlloc(&p->dTable, meshes.size()));
     GMR_HIP(hipEventCreate(&p->ev[0]));
     GMR_HIP(hipEventCreate(&p->ev[1]));
@@ -164,6 +175,8 @@ void freePlan(Plan* p)
     clearGraphs(p);
     for(void* q : p->owned) (void)hipFree(q);
     if(p->dTable) (void)hipFree(p->dTable);
+    if(p->spans) (void)hipFree(p->spans);
+    if(p->meshList) (void)hipFree(p->meshList);
     for(hipEvent_t e : p->ev) if(e) (void)hipEventDestroy(e);
     for(hipEvent_t e : p->pool) if(e) (void)hipEventDestroy(e);
     delete p;
@@ -268,10 +281,8 @@ static hipError_t evaluateAll(Plan* p, const gmq::Crew& c, int ev, int sa, bool 
 {
     int slot = timedBegin(p, timed, GM_KT_EVALUATE);
     hipError_t e = fromF32
-        ? gmd::launchEvaluate<float>(p->dTable, p->evalBlocks, p->nEvalBlocks, p->meshList, p->nMeshes, c, ev, sa,
-                                     p->stream)
-        : gmd::launchEvaluate<double>(p->dTable, p->evalBlocks, p->nEvalBlocks, p->meshList, p->nMeshes, c, ev, sa,
-                                      p->stream);
+        ? gmd::launchEvaluate<float>(p->dTable, p->evalBlocks, p->meshList, p->nMeshes, c, ev, sa, p->stream)
+        : gmd::launchEvaluate<double>(p->dTable, p->evalBlocks, p->meshList, p->nMeshes, c, ev, sa, p->stream);
     timedEnd(p, slot);
     return e;
 }
@@ -286,17 +297,15 @@ static hipError_t sweep(Plan* p, const gmq::Crew& c, int ev, int sa, const RunPa
     if(getme)
     {
         int slot = timedBegin(p, timed, f64 ? GM_KT_GETME_ELEM_F64 : GM_KT_GETME_ELEM_F32);
-        e = f64 ? gmd::launchGetmeElements<double>(p->dTable, p->tetBlocks, p->nTetBlocks, p->hexBlocks,
-                                                   p->nHexBlocks, c, ev, sa, p->stream)
-                : gmd::launchGetmeElements<float>(p->dTable, p->tetBlocks, p->nTetBlocks, p->hexBlocks,
-                                                  p->nHexBlocks, c, ev, sa, p->stream);
+        e = f64 ? gmd::launchGetmeElements<double>(p->dTable, p->tetBlocks, p->hexBlocks, c, ev, sa, p->stream)
+                : gmd::launchGetmeElements<float>(p->dTable, p->tetBlocks, p->hexBlocks, c, ev, sa, p->stream);
         timedEnd(p, slot);
     }
     for(int64_t g = g0; e == hipSuccess && g < g1; ++g)
     {
         for(const VList& l : p->colours[g])
         {
-            gmd::VertexWork w{p->dTable, l.items, l.count, l.kind, l.maxPatch};
+            gmd::VertexWork w{p->dTable, l.segs, l.nsegs, l.count, l.kind, l.maxPatch};
             int tag = getme ? (f64 ? GM_KT_GETME_VERT_F64 : GM_KT_GETME_VERT_F32)
                             : (f64 ? GM_KT_SMOOTH_F64 : GM_KT_SMOOTH_F32);
             int slot = timedBegin(p, timed, tag);
@@ -373,11 +382,11 @@ int runPlan(Plan* p, const gmq::Crew& c, int ev, int sa, const RunParams& r)
     const bool timed = r.timeKernels && !r.graph;
     p->poolUsed = 0;
     GMR_HIP(hipEventRecord(p->ev[0], s));
-    GMR_HIP(gmd::launchResetRun(p->dTable, p->vertBlocks, p->nVertBlocks, p->meshList, p->nMeshes, s));
+    GMR_HIP(gmd::launchResetRun(p->dTable, p->vertBlocks, p->meshList, p->nMeshes, s));
     bool inF32 = false;
     if(r.precision != GM_FP64)
     {
-        GMR_HIP(gmd::launchToF32(p->dTable, p->vertBlocks, p->nVertBlocks, s));
+        GMR_HIP(gmd::launchToF32(p->dTable, p->vertBlocks, s));
         inF32 = true;
     }
     GMR_HIP(evaluateAll(p, c, ev, sa, inF32, timed));
@@ -386,7 +395,7 @@ int runPlan(Plan* p, const gmq::Crew& c, int ev, int sa, const RunParams& r)
         const bool f64 = r.precision == GM_FP64 || (r.precision == GM_FP32_POLISH && pass == r.passes - 1);
         if(f64 && inF32)
         {
-            GMR_HIP(gmd::launchToF64(p->dTable, p->vertBlocks, p->nVertBlocks, s));
+            GMR_HIP(gmd::launchToF64(p->dTable, p->vertBlocks, s));
             inF32 = false;
         }
         RunParams rr = r;
@@ -394,7 +403,7 @@ int runPlan(Plan* p, const gmq::Crew& c, int ev, int sa, const RunParams& r)
         if((rc = passBody(p, c, ev, sa, rr, f64, inF32))) return rc;
     }
     if(inF32)
-        GMR_HIP(gmd::launchToF64(p->dTable, p->vertBlocks, p->nVertBlocks, s));
+        GMR_HIP(gmd::launchToF64(p->dTable, p->vertBlocks, s));
     GMR_HIP(hipEventRecord(p->ev[1], s));
     GMR_HIP(hipStreamSynchronize(s));
     float ms = 0.0f;
@@ -410,13 +419,13 @@ int sweepColours(Plan* p, const gmq::Crew& c, int ev, int sa, const RunParams& r
     if(rc) return rc;
     hipStream_t s = p->stream;
     p->poolUsed = 0;
-    GMR_HIP(gmd::launchResetRun(p->dTable, p->vertBlocks, p->nVertBlocks, p->meshList, p->nMeshes, s));
+    GMR_HIP(gmd::launchResetRun(p->dTable, p->vertBlocks, p->meshList, p->nMeshes, s));
     const bool f64 = r.precision == GM_FP64;
-    if(!f64) GMR_HIP(gmd::launchToF32(p->dTable, p->vertBlocks, p->nVertBlocks, s));
+    if(!f64) GMR_HIP(gmd::launchToF32(p->dTable, p->vertBlocks, s));
     hipError_t e = sweep(p, c, ev, sa, r, f64, std::min<int64_t>(g0, (int64_t)p->colours.size()),
                          std::min<int64_t>(g1, (int64_t)p->colours.size()), false);
     if(e != hipSuccess) return fail(GM_ERR_HIP, std::string("sweep: ") + hipGetErrorString(e));
-    if(!f64) GMR_HIP(gmd::launchToF64(p->dTable, p->vertBlocks, p->nVertBlocks, s));
+    if(!f64) GMR_HIP(gmd::launchToF64(p->dTable, p->vertBlocks, s));
     GMR_HIP(hipStreamSynchronize(s));
     return GM_OK;
 }

@@ -111,12 +111,24 @@ def _cell_base(n: int) -> np.ndarray:
     return (ii + jj * w + kk * w * w).reshape(-1).astype(np.int64)
 
 
+def _dot3(a: np.ndarray, b: np.ndarray) -> np.ndarray:
+    """Row-wise dot product summed left to right, (a0 b0 + a1 b1) + a2 b2.
+    (np.einsum may fuse or reorder the sum depending on the host CPU, which
+    would make generated meshes machine-dependent.)"""
+    return (a[:, 0] * b[:, 0] + a[:, 1] * b[:, 1]) + a[:, 2] * b[:, 2]
+
+
+def _cross3(b: np.ndarray, c: np.ndarray) -> np.ndarray:
+    return np.stack([b[:, 1] * c[:, 2] - b[:, 2] * c[:, 1], b[:, 2] * c[:, 0] - b[:, 0] * c[:, 2],
+                     b[:, 0] * c[:, 1] - b[:, 1] * c[:, 0]], axis=1)
+
+
 def _tet_dets(pos: np.ndarray, tets: np.ndarray) -> np.ndarray:
     p = pos[tets]
     a = p[:, 3] - p[:, 0]
     b = p[:, 3] - p[:, 1]
     c = p[:, 3] - p[:, 2]
-    return np.einsum("ij,ij->i", a, np.cross(b, c))
+    return _dot3(a, _cross3(b, c))
 
 
 def _hex_min_corner_det(pos: np.ndarray, hexs: np.ndarray) -> np.ndarray:
@@ -131,12 +143,36 @@ def _hex_min_corner_det(pos: np.ndarray, hexs: np.ndarray) -> np.ndarray:
               (e(1, 2), e(2, 6), -e(2, 3)), (e(0, 3), e(2, 3), e(3, 7)),
               (e(0, 4), e(4, 5), e(4, 7)), (e(1, 5), -e(5, 6), e(4, 5)),
               (e(2, 6), e(5, 6), e(6, 7)), (e(3, 7), e(6, 7), -e(4, 7))]
-    dets = [np.einsum("ij,ij->i", a, np.cross(b, c)) for a, b, c in frames]
+    dets = [_dot3(a, _cross3(b, c)) for a, b, c in frames]
     return np.min(np.stack(dets, axis=1), axis=1)
 
 
 def cube_mesh(n: int, kind: str = "tet", jitter: float = 0.3, seed: int = 1,
               min_det_fraction: float = 0.05) -> CubeMesh:
+    """Builds an n^3-cell unit-cube mesh ("tet" Kuhn split or "hex") with the
+    library's native generator (gm_cube_mesh); the output is identical to
+    cube_mesh_numpy, the restatement below."""
+    if n < 1:
+        raise ValueError("n must be >= 1")
+    if kind not in ("tet", "hex"):
+        raise ValueError("kind must be 'tet' or 'hex'")
+    from ._lib import check, lib, ptr
+    w = n + 1
+    pos = np.empty((w ** 3, 3), np.float64)
+    fixed = np.empty(w ** 3, np.int32)
+    per = 4 if kind == "tet" else 8
+    ne = 6 * n ** 3 if kind == "tet" else n ** 3
+    elems = np.empty((ne, per), np.uint32)
+    check(lib().gm_cube_mesh(int(n), 0 if kind == "tet" else 1, float(jitter), int(seed) & 0xFFFFFFFFFFFFFFFF,
+                             float(min_det_fraction), ptr(pos), ptr(fixed), ptr(elems)), "gm_cube_mesh")
+    empty4 = np.zeros((0, 4), np.uint32)
+    empty8 = np.zeros((0, 8), np.uint32)
+    return CubeMesh(pos=pos, fixed=fixed, tets=elems if kind == "tet" else empty4,
+                    hexs=elems if kind == "hex" else empty8, n=n, kind=kind, seed=seed, jitter=jitter)
+
+
+def cube_mesh_numpy(n: int, kind: str = "tet", jitter: float = 0.3, seed: int = 1,
+                    min_det_fraction: float = 0.05) -> CubeMesh:
     """Builds an n^3-cell unit-cube mesh ("tet" Kuhn split or "hex").
 
     Interior vertices are displaced by U[-jitter h, +jitter h] per axis.

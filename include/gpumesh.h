@@ -197,6 +197,18 @@ int gm_batch_kernel_times(const gm_batch* batch, double* ms, int64_t* launches);
  * trial, keeps the fastest shape for `smoother` and returns it in best[2]. */
 int gm_batch_autotune(gm_batch* batch, int smoother, int precision, int passes, int32_t* best, double* times);
 
+/* Synthetic input: an n^3-cell unit-cube mesh, kind 0 = Kuhn tetrahedra
+ * (6 per cell, det(v3-v0, v3-v1, v3-v2) > 0), 1 = hexahedra (v0..v3 bottom
+ * counter-clockwise, v4..v7 top, Meshers/DebugMesher.cpp:285-293); vertices
+ * numbered x-fastest (DebugMesher.cpp:173-181), boundary vertices fixed,
+ * interior vertices jittered per axis by U[-jitter/n, jitter/n] from
+ * SplitMix64(seed, 3 v + axis), with the jitter of an element's vertices
+ * halved until every orientation determinant is at least min_det_fraction
+ * of the unjittered one.  Host only.  pos[3 (n+1)^3], fixed[(n+1)^3],
+ * elems[4 * 6 n^3] (tets) or [8 n^3] (hexes). */
+int gm_cube_mesh(int n, int kind, double jitter, uint64_t seed, double min_det_fraction, double* pos,
+                 int32_t* fixed, uint32_t* elems);
+
 /* Device math probe (for tests): op 0 sqrt(a), 1 a/b, 2 a^(2/3), 3 a^(1/3),
  * 4 exp(a), all in FP64 on the device; n values. */
 int gm_math_probe(int op, const double* a, const double* b, double* out, int64_t n);
