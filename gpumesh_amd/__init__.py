@@ -7,5 +7,6 @@ cube-mesh generator used by the tests and the benchmark.
 from ._lib import GpuMeshError, device_count, lib, math_probe  # noqa: F401
 from .mesh import GpuBatch, GpuMesh  # noqa: F401
 from .meshgen import CubeMesh, cube_mesh  # noqa: F401
+from .serving import GpuServer  # noqa: F401
 
-__all__ = ["GpuBatch", "GpuMesh", "GpuMeshError", "CubeMesh", "cube_mesh", "device_count", "lib", "math_probe"]
+__all__ = ["GpuBatch", "GpuMesh", "GpuServer", "GpuMeshError", "CubeMesh", "cube_mesh", "device_count", "lib", "math_probe"]

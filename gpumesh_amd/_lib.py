@@ -25,7 +25,9 @@ EXPORTS = [
     "gm_mesh_topology", "gm_mesh_get_positions", "gm_mesh_set_positions", "gm_mesh_set_crew",
     "gm_mesh_evaluate", "gm_mesh_smooth", "gm_mesh_smooth_groups", "gm_mesh_last_timing", "gm_mesh_kernel_times",
     "gm_mesh_device_bytes", "gm_batch_create", "gm_batch_destroy", "gm_batch_set_shape", "gm_batch_smooth",
-    "gm_batch_last_timing", "gm_batch_kernel_times", "gm_batch_autotune", "gm_cube_mesh", "gm_math_probe",
+    "gm_batch_last_timing", "gm_batch_kernel_times", "gm_batch_autotune", "gm_server_create", "gm_server_destroy", "gm_server_set_shape", "gm_server_submit", "gm_server_wait",
+    "gm_server_result", "gm_server_release", "gm_server_stats", "gm_cube_mesh", "gm_set_topology_cache",
+    "gm_topology_cache_stats", "gm_math_probe",
 ]
 
 
@@ -72,6 +74,17 @@ def lib():
     L.gm_batch_last_timing.argtypes = [vp, vp]
     L.gm_batch_kernel_times.argtypes = [vp, vp, vp]
     L.gm_batch_autotune.argtypes = [vp, C.c_int, C.c_int, C.c_int, vp, vp]
+    L.gm_server_create.argtypes = [i64, C.c_int, C.c_int, C.c_int, C.c_int, dbl, C.POINTER(vp)]
+    L.gm_server_destroy.argtypes = [vp]
+    L.gm_server_set_shape.argtypes = [vp, C.c_int, C.c_int, C.c_int]
+    L.gm_server_submit.argtypes = [vp, i64, vp, vp, i64, vp, i64, vp, C.c_int, C.c_int, C.c_int,
+                                   C.POINTER(i64)]
+    L.gm_server_wait.argtypes = [vp, i64, dbl, C.POINTER(C.c_int)]
+    L.gm_server_result.argtypes = [vp, i64, vp, vp, vp, vp]
+    L.gm_server_release.argtypes = [vp, i64]
+    L.gm_server_stats.argtypes = [vp, vp]
+    L.gm_set_topology_cache.argtypes = [i64]
+    L.gm_topology_cache_stats.argtypes = [vp]
     L.gm_cube_mesh.argtypes = [C.c_int, C.c_int, dbl, C.c_uint64, dbl, vp, vp, vp]
     L.gm_math_probe.argtypes = [C.c_int, vp, vp, vp, i64]
     for name in EXPORTS:

@@ -43,34 +43,6 @@ template<typename X> hipError_t devAlloc(X** p, size_t count)
     return hipMalloc((void**)p, (count ? count : 1) * sizeof(X));
 }
 
-void freeMeshBuffers(gm_mesh* m)
-{
-    gmd::DevMesh& d = m->dm;
-    void* ptrs[] = {d.pos64, d.pos32, (void*)d.tets, (void*)d.hexs, (void*)d.eoff, (void*)d.eref,
-                    (void*)d.noff, (void*)d.nbr, (void*)d.fixed, (void*)d.groupNodes, d.evalCounts, d.getmeTet, d.getmeHex,
-                    d.evalPartials, d.evalFlags, d.evalHist, d.evalResult, d.evalHistOut, d.passSlot,
-                    m->elemQualBuf, m->texels};
-    for(void* p : ptrs)
-        if(p) (void)hipFree(p);
-    d = gmd::DevMesh{};
-    m->elemQualBuf = nullptr;
-    m->texels = nullptr;
-}
-
-void destroyMesh(gm_mesh* m)
-{
-    if(!m) return;
-    if(m->onDevice)
-    {
-        (void)hipSetDevice(m->device);
-        if(m->stream) (void)hipStreamSynchronize(m->stream);
-        gmr::freePlan(m->plan);
-        freeMeshBuffers(m);
-        if(m->stream) (void)hipStreamDestroy(m->stream);
-    }
-    delete m;
-}
-
 int checkRun(int smoother, int passes, int precision, int kernel, double spawnCoeff)
 {
     if(smoother < 0 || smoother > 3) return fail(GM_ERR_ARG, "unknown smoother");
@@ -102,27 +74,6 @@ gmd::LaneShape defaultLane(const std::vector<gm_mesh*>& meshes, int smoother)
     else
         l.tileElems = maxPatch <= 64 ? 64 : (maxPatch <= 128 ? 128 : 256);
     return l;
-}
-
-int buildTopology(int64_t nverts, const double* pos, const int32_t* fixed, int64_t ntets, const uint32_t* tets,
-                  int64_t nhexs, const uint32_t* hexs, gm_mesh** out, gm_mesh** made)
-{
-    if(!out) return fail(GM_ERR_ARG, "out is null");
-    *out = nullptr;
-    if(nverts < 0 || ntets < 0 || nhexs < 0) return fail(GM_ERR_ARG, "negative size");
-    if((nverts && (!pos || !fixed)) || (ntets && !tets) || (nhexs && !hexs)) return fail(GM_ERR_ARG, "null array");
-    gm_mesh* m = new gm_mesh();
-    std::string err;
-    if(!gmt::build(nverts, pos, fixed, ntets, tets, nhexs, hexs, m->topo, err))
-    {
-        delete m;
-        return fail(GM_ERR_ARG, err);
-    }
-    for(int32_t p : m->topo.groupMaxPatch) m->maxPatch = std::max(m->maxPatch, (int)p);
-    m->kind = m->topo.nhexs == 0 ? gmd::EK_TET : (m->topo.ntets == 0 ? gmd::EK_HEX : gmd::EK_MIXED);
-    m->crew.scaling = m->crew.scaling2 = m->crew.scaling3 = 1.0;
-    *made = m;
-    return GM_OK;
 }
 
 int runAndFetch(gmr::Plan* p, const gm_mesh* m0, const gmr::RunParams& r, double* stats, int32_t* hist,
@@ -166,11 +117,7 @@ int gm_mesh_create_host(int64_t nverts, const double* pos, const int32_t* fixed,
                         int64_t ntets, const uint32_t* tets, int64_t nhexs, const uint32_t* hexs,
                         gm_mesh** out)
 {
-    gm_mesh* m = nullptr;
-    int rc = buildTopology(nverts, pos, fixed, ntets, tets, nhexs, hexs, out, &m);
-    if(rc) return rc;
-    *out = m;
-    return GM_OK;
+    return gmr::makeMesh(nverts, pos, fixed, ntets, tets, nhexs, hexs, out);
 }
 
 int gm_mesh_create(int64_t nverts, const double* pos, const int32_t* fixed,
@@ -183,107 +130,27 @@ int gm_mesh_create(int64_t nverts, const double* pos, const int32_t* fixed,
     if(hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
         return fail(GM_ERR_HIP, "no HIP device visible");
     gm_mesh* m = nullptr;
-    int rc = buildTopology(nverts, pos, fixed, ntets, tets, nhexs, hexs, out, &m);
+    int rc = gmr::makeMesh(nverts, pos, fixed, ntets, tets, nhexs, hexs, &m);
     if(rc) return rc;
-    const gmt::Topology& t = m->topo;
-    for(int64_t v = 0; v < nverts; ++v)
-        if(!t.fixed[v] && t.eoff[v + 1] == t.eoff[v])
-        {
-            delete m;
-            return fail(GM_ERR_ARG, "a free vertex belongs to no element");
-        }
-    if((uint64_t)nverts >= (1ull << 32))
+    if((rc = gmr::uploadMesh(m)))
     {
-        delete m;
-        return fail(GM_ERR_ARG, "too many vertices for 32-bit ids");
+        gmr::destroyMesh(m);
+        return rc;
     }
-
-    m->onDevice = true;
-    auto bail = [&](int code) { destroyMesh(m); return code; };
-#define GM_TRY(call)                                                                                    \
-    do {                                                                                                \
-        hipError_t e_ = (call);                                                                         \
-        if(e_ != hipSuccess)                                                                            \
-            return bail(fail(e_ == hipErrorOutOfMemory ? GM_ERR_NOMEM : GM_ERR_HIP,                     \
-                             std::string(#call) + ": " + hipGetErrorString(e_)));                       \
-    } while(0)
-    GM_TRY(hipGetDevice(&m->device));
-    GM_TRY(hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking));
-
-    const size_t V = (size_t)nverts;
-    gmd::DevMesh& d = m->dm;
-    d.nverts = (uint32_t)nverts;
-    d.ntets = (uint32_t)ntets;
-    d.nhexs = (uint32_t)nhexs;
-    d.evalBlocks = gmd::evalBlocksFor((uint64_t)ntets + (uint64_t)nhexs);
-    std::vector<double> p64(4 * V);
-    std::vector<float> p32(4 * V);
-    for(size_t v = 0; v < V; ++v)
-    {
-        for(int k = 0; k < 3; ++k)
-        {
-            p64[4 * v + k] = t.pos[3 * v + k];
-            p32[4 * v + k] = (float)t.pos[3 * v + k];
-        }
-        p64[4 * v + 3] = 0.0;
-        p32[4 * v + 3] = 0.0f;
-    }
-    uint4 *tb, *hb;
-    uint32_t *eo, *er, *no, *nb;
-    uint8_t* fx;
-    GM_TRY(devAlloc(&d.pos64, V));
-    GM_TRY(devAlloc(&d.pos32, V));
-    GM_TRY(devAlloc(&tb, (size_t)ntets));      d.tets = tb;
-    GM_TRY(devAlloc(&hb, 2 * (size_t)nhexs));  d.hexs = hb;
-    GM_TRY(devAlloc(&eo, V + 1));              d.eoff = eo;
-    GM_TRY(devAlloc(&er, t.eref.size()));      d.eref = er;
-    GM_TRY(devAlloc(&no, V + 1));              d.noff = no;
-    GM_TRY(devAlloc(&nb, t.nbr.size()));       d.nbr = nb;
-    GM_TRY(devAlloc(&fx, V));                  d.fixed = fx;
-    uint32_t* gn;
-    GM_TRY(devAlloc(&gn, t.groupNodes.size())); d.groupNodes = gn;
-    GM_TRY(devAlloc(&d.evalCounts, V));
-    GM_TRY(devAlloc(&d.evalPartials, 3 * (size_t)d.evalBlocks));
-    GM_TRY(devAlloc(&d.evalFlags, (size_t)d.evalBlocks));
-    GM_TRY(devAlloc(&d.evalHist, 20));
-    GM_TRY(devAlloc(&d.passSlot, 1));
-    m->deviceBytes = V * (sizeof(double4) + sizeof(float4) + 4 + 4 + 4 + 1) + (size_t)ntets * 16 +
-                     (size_t)nhexs * 32 + t.eref.size() * 4 + t.nbr.size() * 4 + t.groupNodes.size() * 4 +
-                     (size_t)d.evalBlocks * 28 + 84;
-
-    hipStream_t s = m->stream;
-    if(V)
-    {
-        GM_TRY(hipMemcpyAsync(d.pos64, p64.data(), V * sizeof(double4), hipMemcpyHostToDevice, s));
-        GM_TRY(hipMemcpyAsync(d.pos32, p32.data(), V * sizeof(float4), hipMemcpyHostToDevice, s));
-        GM_TRY(hipMemcpyAsync(eo, t.eoff.data(), (V + 1) * 4, hipMemcpyHostToDevice, s));
-        GM_TRY(hipMemcpyAsync(no, t.noff.data(), (V + 1) * 4, hipMemcpyHostToDevice, s));
-        GM_TRY(hipMemcpyAsync(fx, t.fixed.data(), V, hipMemcpyHostToDevice, s));
-    }
-    if(ntets) GM_TRY(hipMemcpyAsync(tb, t.tets.data(), (size_t)ntets * 16, hipMemcpyHostToDevice, s));
-    if(nhexs) GM_TRY(hipMemcpyAsync(hb, t.hexs.data(), (size_t)nhexs * 32, hipMemcpyHostToDevice, s));
-    if(!t.eref.empty()) GM_TRY(hipMemcpyAsync(er, t.eref.data(), t.eref.size() * 4, hipMemcpyHostToDevice, s));
-    if(!t.nbr.empty()) GM_TRY(hipMemcpyAsync(nb, t.nbr.data(), t.nbr.size() * 4, hipMemcpyHostToDevice, s));
-    if(!t.groupNodes.empty())
-        GM_TRY(hipMemcpyAsync(gn, t.groupNodes.data(), t.groupNodes.size() * 4, hipMemcpyHostToDevice, s));
-    GM_TRY(hipStreamSynchronize(s));
-#undef GM_TRY
-    std::vector<gm_mesh*> one{m};
-    if((rc = gmr::buildPlan(one, m->stream, &m->plan))) return bail(rc);
     *out = m;
     return GM_OK;
 }
 
 int gm_mesh_destroy(gm_mesh* mesh)
 {
-    destroyMesh(mesh);
+    gmr::destroyMesh(mesh);
     return GM_OK;
 }
 
 int gm_mesh_info(const gm_mesh* m, int64_t* info)
 {
     if(!m || !info) return fail(GM_ERR_ARG, "null argument");
-    const gmt::Topology& t = m->topo;
+    const gmt::Topology& t = *m->topo;
     info[0] = t.nverts;
     info[1] = t.ntets;
     info[2] = t.nhexs;
@@ -295,40 +162,54 @@ int gm_mesh_info(const gm_mesh* m, int64_t* info)
     return GM_OK;
 }
 
+int gm_set_topology_cache(int64_t max_bytes)
+{
+    if(max_bytes < 0) return fail(GM_ERR_ARG, "negative cache size");
+    gmt::setCacheCapacity((size_t)max_bytes);
+    return GM_OK;
+}
+
+int gm_topology_cache_stats(int64_t* out4)
+{
+    if(!out4) return fail(GM_ERR_ARG, "null argument");
+    gmt::cacheStats(out4);
+    return GM_OK;
+}
+
 int gm_mesh_device_bytes(const gm_mesh* m, int64_t* bytes)
 {
     if(!m || !bytes) return fail(GM_ERR_ARG, "null argument");
-    *bytes = m->onDevice ? (int64_t)m->deviceBytes : 0;
+    *bytes = m->onDevice ? (int64_t)gmr::residentBytes(m, m->resultCap, m->getmeReady) : 0;
     return GM_OK;
 }
 
 int gm_mesh_permutation(const gm_mesh* m, int64_t* perm)
 {
     if(!m || !perm) return fail(GM_ERR_ARG, "null argument");
-    std::memcpy(perm, m->topo.permutation.data(), m->topo.permutation.size() * sizeof(int64_t));
+    std::memcpy(perm, m->topo->permutation.data(), m->topo->permutation.size() * sizeof(int64_t));
     return GM_OK;
 }
 
 int gm_mesh_groups(const gm_mesh* m, int64_t* off, uint32_t* nodes)
 {
     if(!m) return fail(GM_ERR_ARG, "null mesh");
-    if(off) std::memcpy(off, m->topo.groupOff.data(), m->topo.groupOff.size() * sizeof(int64_t));
-    if(nodes) std::memcpy(nodes, m->topo.groupNodes.data(), m->topo.groupNodes.size() * 4);
+    if(off) std::memcpy(off, m->topo->groupOff.data(), m->topo->groupOff.size() * sizeof(int64_t));
+    if(nodes) std::memcpy(nodes, m->topo->groupNodes.data(), m->topo->groupNodes.size() * 4);
     return GM_OK;
 }
 
 int gm_mesh_elements(const gm_mesh* m, uint32_t* tets, uint32_t* hexs)
 {
     if(!m) return fail(GM_ERR_ARG, "null mesh");
-    if(tets) std::memcpy(tets, m->topo.tets.data(), m->topo.tets.size() * 4);
-    if(hexs) std::memcpy(hexs, m->topo.hexs.data(), m->topo.hexs.size() * 4);
+    if(tets) std::memcpy(tets, m->topo->tets.data(), m->topo->tets.size() * 4);
+    if(hexs) std::memcpy(hexs, m->topo->hexs.data(), m->topo->hexs.size() * 4);
     return GM_OK;
 }
 
 int gm_mesh_topology(const gm_mesh* m, uint32_t* eoff, uint32_t* eref, uint32_t* noff, uint32_t* nbr)
 {
     if(!m) return fail(GM_ERR_ARG, "null mesh");
-    const gmt::Topology& t = m->topo;
+    const gmt::Topology& t = *m->topo;
     if(eoff) std::memcpy(eoff, t.eoff.data(), t.eoff.size() * 4);
     if(eref) std::memcpy(eref, t.eref.data(), t.eref.size() * 4);
     if(noff) std::memcpy(noff, t.noff.data(), t.noff.size() * 4);
@@ -341,7 +222,7 @@ int gm_mesh_get_positions(gm_mesh* m, double* pos)
     GM_NEED_DEVICE(m);
     if(!pos) return fail(GM_ERR_ARG, "null argument");
     GM_HIP(hipSetDevice(m->device));
-    const size_t V = (size_t)m->topo.nverts;
+    const size_t V = (size_t)m->topo->nverts;
     std::vector<double> p64(4 * V);
     if(V) GM_HIP(hipMemcpyAsync(p64.data(), m->dm.pos64, V * sizeof(double4), hipMemcpyDeviceToHost, m->stream));
     GM_HIP(hipStreamSynchronize(m->stream));
@@ -355,7 +236,7 @@ int gm_mesh_set_positions(gm_mesh* m, const double* pos)
     GM_NEED_DEVICE(m);
     if(!pos) return fail(GM_ERR_ARG, "null argument");
     GM_HIP(hipSetDevice(m->device));
-    const size_t V = (size_t)m->topo.nverts;
+    const size_t V = (size_t)m->topo->nverts;
     std::vector<double> p64(4 * V);
     std::vector<float> p32(4 * V);
     for(size_t v = 0; v < V; ++v)
@@ -418,7 +299,7 @@ int gm_mesh_evaluate(gm_mesh* m, double* out3, int32_t* hist20, double* elemQual
 {
     GM_NEED_DEVICE(m);
     GM_HIP(hipSetDevice(m->device));
-    const size_t E = (size_t)(m->topo.ntets + m->topo.nhexs);
+    const size_t E = (size_t)(m->topo->ntets + m->topo->nhexs);
     if(elemQual && !m->elemQualBuf) GM_HIP(devAlloc(&m->elemQualBuf, E));
     gmr::Plan* p = m->plan;
     // Per-element output is switched on in the descriptor for this call only.
@@ -470,7 +351,7 @@ int gm_mesh_smooth_groups(gm_mesh* m, int smoother, int64_t groupBegin, int64_t 
         return fail(GM_ERR_UNSUPPORTED, "GETMe sweeps start with an element step over the whole mesh; use gm_mesh_smooth");
     if(precision != GM_FP64 && precision != GM_FP32) return fail(GM_ERR_ARG, "precision must be GM_FP64 or GM_FP32");
     if((rc = checkSampler(m->sampler, smoother, kernel))) return rc;
-    if(groupBegin < 0 || groupEnd > m->topo.ngroups() || groupBegin > groupEnd)
+    if(groupBegin < 0 || groupEnd > m->topo->ngroups() || groupBegin > groupEnd)
         return fail(GM_ERR_ARG, "bad group range");
     GM_HIP(hipSetDevice(m->device));
     gmr::RunParams r;
@@ -480,9 +361,9 @@ int gm_mesh_smooth_groups(gm_mesh* m, int smoother, int64_t groupBegin, int64_t 
     r.spawnCoeff = spawnCoeff;
     r.lane = defaultLane(m->plan->meshes, smoother);
     if((rc = gmr::sweepColours(m->plan, m->crew, m->evaluator, m->sampler, r, groupBegin, groupEnd))) return rc;
-    if(evalCounts && m->topo.nverts)
+    if(evalCounts && m->topo->nverts)
     {
-        GM_HIP(hipMemcpyAsync(evalCounts, m->dm.evalCounts, (size_t)m->topo.nverts * 4, hipMemcpyDeviceToHost,
+        GM_HIP(hipMemcpyAsync(evalCounts, m->dm.evalCounts, (size_t)m->topo->nverts * 4, hipMemcpyDeviceToHost,
                               m->stream));
         GM_HIP(hipStreamSynchronize(m->stream));
     }
@@ -634,7 +515,7 @@ int gm_batch_autotune(gm_batch* b, int smoother, int precision, int passes, int3
     std::vector<std::vector<double4>> saved(p->meshes.size());
     for(size_t i = 0; i < p->meshes.size(); ++i)
     {
-        saved[i].resize(p->meshes[i]->topo.nverts);
+        saved[i].resize(p->meshes[i]->topo->nverts);
         if(!saved[i].empty())
             GM_HIP(hipMemcpyAsync(saved[i].data(), p->meshes[i]->dm.pos64, saved[i].size() * sizeof(double4),
                                   hipMemcpyDeviceToHost, p->stream));

@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <map>
+#include <memory>
 #include <string>
 #include <tuple>
 #include <vector>
@@ -54,7 +55,8 @@ struct gm_mesh
     bool onDevice = false;     // false: topology only (gm_mesh_create_host)
     int device = 0;
     hipStream_t stream = nullptr;
-    gmt::Topology topo;
+    std::shared_ptr<const gmt::Topology> topo;   // shared by meshes of equal connectivity
+    std::vector<double> hostPos;  // renumbered input positions, until uploaded
     gmd::DevMesh dm{};         // host copy of the device descriptor (owns its pointers)
     int kind = gmd::EK_TET;    // element kind of the whole mesh
     int maxPatch = 0;
@@ -66,7 +68,6 @@ struct gm_mesh
     gmq::Crew crew{};
     float* texels = nullptr;
     gmr::Plan* plan = nullptr; // the mesh as a batch of one
-    size_t deviceBytes = 0;    // device memory owned by the mesh
 };
 
 namespace gmr
@@ -103,6 +104,17 @@ struct Plan
 
 void clearGraphs(Plan* p);
 
+// Host topology of a new handle (through the topology cache).
+int makeMesh(int64_t nverts, const double* pos, const int32_t* fixed, int64_t ntets, const uint32_t* tets,
+             int64_t nhexs, const uint32_t* hexs, gm_mesh** out);
+// Device memory a mesh needs when resident (connectivity, positions, reduction
+// buffers, `records` pass records, GETMe scratch when `getme`).
+size_t residentBytes(const gm_mesh* m, int records, bool getme);
+// Allocates and uploads the device copy on the current device, with its own
+// stream and a plan of one.  On failure everything allocated is released.
+int uploadMesh(gm_mesh* m);
+void destroyMesh(gm_mesh* m);
+
 // Builds the fused plan of `meshes` (all resident on one device) on `stream`.
 int buildPlan(const std::vector<gm_mesh*>& meshes, hipStream_t stream, Plan** out);
 void freePlan(Plan* p);
@@ -114,6 +126,15 @@ int runPlan(Plan* p, const gmq::Crew& crew, int evaluator, int sampler, const Ru
 // One sweep restricted to colours [g0, g1) (no evaluation), FP64 or FP32.
 int sweepColours(Plan* p, const gmq::Crew& crew, int evaluator, int sampler, const RunParams& r, int64_t g0,
                  int64_t g1);
+// Pieces of the pass loop for the serving loop, launched on the plan's stream
+// without synchronising: the run start (counters reset, FP32 copy when
+// `toF32`, record 0), one pass (FP64 widening first when `convertFirst`; the
+// reduction reads the FP32 copy unless `f64`), and the final widening.
+// The plan must have been prepared (preparePlan) for its members.
+int launchBegin(Plan* p, const gmq::Crew& crew, int evaluator, int sampler, bool toF32);
+int launchPass(Plan* p, const gmq::Crew& crew, int evaluator, int sampler, const RunParams& r, bool f64,
+               bool convertFirst);
+int launchToF64All(Plan* p);
 // Copies the pass records of mesh i (stats 3 per record, hist 20 per record).
 int fetchRecords(Plan* p, size_t i, int records, double* stats, int32_t* hist, int32_t* evalCounts);
 

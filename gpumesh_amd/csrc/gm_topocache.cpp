@@ -1,0 +1,171 @@
+// gm_topocache.cpp -- shared, size-bounded cache of built topologies.
+//
+// Meshes that differ only in vertex positions (the same connectivity and the
+// same fixed flags) have the same patch lists, neighbour lists, colouring and
+// renumbering: everything NodeGroups::build and Mesh::compileTopology derive
+// depends on connectivity alone.  A serving stream of jittered cube meshes
+// repeats a few hundred connectivities, so the host-side build (the costliest
+// host step of admitting a mesh) runs once per connectivity.  Lookups compare
+// the full input arrays, never a hash alone, so a hit is always exact.
+#include <algorithm>
+#include <cstring>
+#include <list>
+#include <memory>
+#include <mutex>
+
+#include "gm_topology.h"
+
+namespace gmt
+{
+
+namespace
+{
+
+uint64_t mix(uint64_t h, uint64_t w)
+{
+    h ^= w * 0x9E3779B97F4A7C15ull;
+    h = (h << 27) | (h >> 37);
+    return h * 0xC2B2AE3D27D4EB4Full + 0x165667B19E3779F9ull;
+}
+
+uint64_t hashBytes(uint64_t h, const void* data, size_t n)
+{
+    const unsigned char* p = (const unsigned char*)data;
+    size_t i = 0;
+    for(; i + 8 <= n; i += 8)
+    {
+        uint64_t w;
+        std::memcpy(&w, p + i, 8);
+        h = mix(h, w);
+    }
+    uint64_t tail = 0;
+    std::memcpy(&tail, p + i, n - i);
+    return mix(h, tail ^ (uint64_t)n);
+}
+
+struct Entry
+{
+    uint64_t key;
+    int64_t nverts, ntets, nhexs;
+    std::vector<uint8_t> fixed;
+    std::vector<uint32_t> tets, hexs;
+    std::shared_ptr<const Topology> topo;
+    size_t bytes;
+};
+
+std::mutex g_mutex;
+std::list<Entry> g_entries;     // most recently used first
+size_t g_capacity = 0, g_bytes = 0;
+int64_t g_hits = 0, g_misses = 0;
+
+size_t topologyBytes(const Topology& t)
+{
+    return t.fixed.size() + 4 * (t.tets.size() + t.hexs.size() + t.eoff.size() + t.eref.size() + t.noff.size() +
+                                 t.nbr.size() + t.groupNodes.size() + t.serialGroup.size()) +
+           8 * (t.permutation.size() + t.groupOff.size()) + 4 * t.groupMaxPatch.size();
+}
+
+void evictTo(size_t cap)
+{
+    while(g_bytes > cap && !g_entries.empty())
+    {
+        g_bytes -= g_entries.back().bytes;
+        g_entries.pop_back();
+    }
+}
+
+} // namespace
+
+void setCacheCapacity(size_t bytes)
+{
+    std::lock_guard<std::mutex> lock(g_mutex);
+    g_capacity = bytes;
+    evictTo(bytes);
+}
+
+void cacheStats(int64_t* out4)
+{
+    std::lock_guard<std::mutex> lock(g_mutex);
+    out4[0] = g_hits;
+    out4[1] = g_misses;
+    out4[2] = (int64_t)g_entries.size();
+    out4[3] = (int64_t)g_bytes;
+}
+
+std::shared_ptr<const Topology> buildShared(int64_t nverts, const double* pos, const int32_t* fixed, int64_t ntets,
+                                            const uint32_t* tets, int64_t nhexs, const uint32_t* hexs,
+                                            std::vector<double>& renumbered, std::string& err)
+{
+    std::vector<uint8_t> fx((size_t)std::max<int64_t>(nverts, 0));
+    for(int64_t v = 0; v < nverts; ++v) fx[v] = fixed[v] != 0;
+    size_t cap;
+    {
+        std::lock_guard<std::mutex> lock(g_mutex);
+        cap = g_capacity;
+    }
+    uint64_t key = 0;
+    if(cap > 0 && nverts > 0)
+    {
+        key = hashBytes(0x243F6A8885A308D3ull, &nverts, sizeof(nverts));
+        key = hashBytes(key, fx.data(), fx.size());
+        key = hashBytes(key, tets, 16 * (size_t)ntets);
+        key = hashBytes(key, hexs, 32 * (size_t)nhexs);
+        std::shared_ptr<const Topology> hit;
+        {
+            std::lock_guard<std::mutex> lock(g_mutex);
+            for(auto it = g_entries.begin(); it != g_entries.end(); ++it)
+            {
+                if(it->key != key || it->nverts != nverts || it->ntets != ntets || it->nhexs != nhexs) continue;
+                if(it->fixed != fx) continue;
+                if(ntets && std::memcmp(it->tets.data(), tets, 16 * (size_t)ntets) != 0) continue;
+                if(nhexs && std::memcmp(it->hexs.data(), hexs, 32 * (size_t)nhexs) != 0) continue;
+                hit = it->topo;
+                g_entries.splice(g_entries.begin(), g_entries, it);
+                ++g_hits;
+                break;
+            }
+        }
+        if(hit)
+        {
+            renumbered.resize(3 * (size_t)nverts);
+            for(int64_t v = 0; v < nverts; ++v)
+            {
+                const int64_t p = hit->permutation[v];
+                renumbered[3 * p] = pos[3 * v];
+                renumbered[3 * p + 1] = pos[3 * v + 1];
+                renumbered[3 * p + 2] = pos[3 * v + 2];
+            }
+            return hit;
+        }
+    }
+    auto t = std::make_shared<Topology>();
+    if(!build(nverts, pos, fixed, ntets, tets, nhexs, hexs, *t, err)) return nullptr;
+    renumbered = std::move(t->pos);
+    t->pos.clear();
+    t->pos.shrink_to_fit();
+    std::shared_ptr<const Topology> ct = t;
+    if(cap > 0 && nverts > 0)
+    {
+        Entry e;
+        e.key = key;
+        e.nverts = nverts;
+        e.ntets = ntets;
+        e.nhexs = nhexs;
+        e.fixed = std::move(fx);
+        e.tets.assign(tets, tets + 4 * (size_t)ntets);
+        e.hexs.assign(hexs, hexs + 8 * (size_t)nhexs);
+        e.topo = ct;
+        e.bytes = topologyBytes(*ct) + e.fixed.size() + 4 * (e.tets.size() + e.hexs.size());
+        std::lock_guard<std::mutex> lock(g_mutex);
+        ++g_misses;
+        if(e.bytes <= g_capacity)
+        {
+            g_bytes += e.bytes;
+            g_entries.push_front(std::move(e));
+            evictTo(g_capacity);
+        }
+    }
+    return ct;
+}
+
+} // namespace gmt

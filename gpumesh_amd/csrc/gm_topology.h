@@ -11,6 +11,7 @@
 #define GM_TOPOLOGY_H
 
 #include <stdint.h>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -47,6 +48,17 @@ struct Topology
 bool build(int64_t nverts, const double* pos, const int32_t* fixed,
            int64_t ntets, const uint32_t* tets, int64_t nhexs, const uint32_t* hexs,
            Topology& out, std::string& err);
+
+// build() through the shared topology cache (gm_topocache.cpp): returns the
+// topology with `pos` left empty and the renumbered positions in
+// `renumbered`; nullptr (and `err`) on invalid input.  With a zero capacity
+// (the default) every call builds.
+std::shared_ptr<const Topology> buildShared(int64_t nverts, const double* pos, const int32_t* fixed, int64_t ntets,
+                                            const uint32_t* tets, int64_t nhexs, const uint32_t* hexs,
+                                            std::vector<double>& renumbered, std::string& err);
+void setCacheCapacity(size_t bytes);
+// {hits, misses, entries, bytes}
+void cacheStats(int64_t* out4);
 
 } // namespace gmt
 

@@ -40,6 +40,182 @@ template<typename X> static hipError_t devAlloc(X** p, size_t count)
     return hipMalloc((void**)p, (count ? count : 1) * sizeof(X));
 }
 
+// Rounded-up allocation size of devAlloc (the accounting of residentBytes).
+template<typename X> static size_t allocBytes(size_t count)
+{
+    return ((count ? count : 1) * sizeof(X) + 255) & ~(size_t)255;
+}
+
+// ---------------------------------------------------------------------------
+// Mesh handles.
+// ---------------------------------------------------------------------------
+int makeMesh(int64_t nverts, const double* pos, const int32_t* fixed, int64_t ntets, const uint32_t* tets,
+             int64_t nhexs, const uint32_t* hexs, gm_mesh** out)
+{
+    if(!out) return fail(GM_ERR_ARG, "out is null");
+    *out = nullptr;
+    if(nverts < 0 || ntets < 0 || nhexs < 0) return fail(GM_ERR_ARG, "negative size");
+    if((nverts && (!pos || !fixed)) || (ntets && !tets) || (nhexs && !hexs)) return fail(GM_ERR_ARG, "null array");
+    gm_mesh* m = new gm_mesh();
+    std::string err;
+    m->topo = gmt::buildShared(nverts, pos, fixed, ntets, tets, nhexs, hexs, m->hostPos, err);
+    if(!m->topo)
+    {
+        delete m;
+        return fail(GM_ERR_ARG, err);
+    }
+    for(int32_t p : m->topo->groupMaxPatch) m->maxPatch = std::max(m->maxPatch, (int)p);
+    m->kind = m->topo->nhexs == 0 ? gmd::EK_TET : (m->topo->ntets == 0 ? gmd::EK_HEX : gmd::EK_MIXED);
+    m->crew.scaling = m->crew.scaling2 = m->crew.scaling3 = 1.0;
+    *out = m;
+    return GM_OK;
+}
+
+size_t residentBytes(const gm_mesh* m, int records, bool getme)
+{
+    const gmt::Topology& t = *m->topo;
+    const size_t V = (size_t)t.nverts;
+    const uint32_t eb = gmd::evalBlocksFor((uint64_t)t.ntets + (uint64_t)t.nhexs);
+    size_t b = allocBytes<double4>(V) + allocBytes<float4>(V) + allocBytes<uint4>((size_t)t.ntets) +
+               allocBytes<uint4>(2 * (size_t)t.nhexs) + 2 * allocBytes<uint32_t>(V + 1) +
+               allocBytes<uint32_t>(t.eref.size()) + allocBytes<uint32_t>(t.nbr.size()) + allocBytes<uint8_t>(V) +
+               allocBytes<uint32_t>(t.groupNodes.size()) + allocBytes<int32_t>(V) +
+               allocBytes<double>(3 * (size_t)eb) + allocBytes<int32_t>(eb) + allocBytes<int32_t>(20) +
+               allocBytes<int32_t>(1);
+    if(records > 0) b += allocBytes<double>(3 * (size_t)records) + allocBytes<int32_t>(20 * (size_t)records);
+    if(getme)
+        b += std::max<size_t>(256, 4 * (size_t)t.ntets * sizeof(double4)) +
+             std::max<size_t>(256, 8 * (size_t)t.nhexs * sizeof(double4));
+    return b;
+}
+
+static void freeMeshBuffers(gm_mesh* m)
+{
+    gmd::DevMesh& d = m->dm;
+    void* ptrs[] = {d.pos64, d.pos32, (void*)d.tets, (void*)d.hexs, (void*)d.eoff, (void*)d.eref,
+                    (void*)d.noff, (void*)d.nbr, (void*)d.fixed, (void*)d.groupNodes, d.evalCounts, d.getmeTet,
+                    d.getmeHex, d.evalPartials, d.evalFlags, d.evalHist, d.evalResult, d.evalHistOut, d.passSlot,
+                    m->elemQualBuf, m->texels};
+    for(void* p : ptrs)
+        if(p) (void)hipFree(p);
+    d = gmd::DevMesh{};
+    m->elemQualBuf = nullptr;
+    m->texels = nullptr;
+    m->resultCap = 0;
+    m->getmeReady = false;
+}
+
+void destroyMesh(gm_mesh* m)
+{
+    if(!m) return;
+    if(m->onDevice)
+    {
+        (void)hipSetDevice(m->device);
+        if(m->stream) (void)hipStreamSynchronize(m->stream);
+        freePlan(m->plan);
+        m->plan = nullptr;
+        freeMeshBuffers(m);
+        if(m->stream) (void)hipStreamDestroy(m->stream);
+        m->stream = nullptr;
+        m->onDevice = false;
+    }
+    delete m;
+}
+
+int uploadMesh(gm_mesh* m)
+{
+    const gmt::Topology& t = *m->topo;
+    for(int64_t v = 0; v < t.nverts; ++v)
+        if(!t.fixed[v] && t.eoff[v + 1] == t.eoff[v])
+            return fail(GM_ERR_ARG, "a free vertex belongs to no element");
+    if((uint64_t)t.nverts >= (1ull << 32)) return fail(GM_ERR_ARG, "too many vertices for 32-bit ids");
+    if(m->onDevice) return fail(GM_ERR_ARG, "mesh is already resident");
+    if(m->hostPos.size() != 3 * (size_t)t.nverts) return fail(GM_ERR_ARG, "mesh has no initial positions");
+
+    m->onDevice = true;
+    auto bail = [&](int code) {
+        if(m->stream) (void)hipStreamSynchronize(m->stream);
+        freePlan(m->plan);
+        m->plan = nullptr;
+        freeMeshBuffers(m);
+        if(m->stream) (void)hipStreamDestroy(m->stream);
+        m->stream = nullptr;
+        m->onDevice = false;
+        return code;
+    };
+#define GM_TRY(call)                                                                                    \
+    do {                                                                                                \
+        hipError_t e_ = (call);                                                                         \
+        if(e_ != hipSuccess)                                                                            \
+            return bail(fail(e_ == hipErrorOutOfMemory ? GM_ERR_NOMEM : GM_ERR_HIP,                     \
+                             std::string(#call) + ": " + hipGetErrorString(e_)));                       \
+    } while(0)
+    GM_TRY(hipGetDevice(&m->device));
+    GM_TRY(hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking));
+
+    const size_t V = (size_t)t.nverts;
+    gmd::DevMesh& d = m->dm;
+    d = gmd::DevMesh{};
+    d.nverts = (uint32_t)t.nverts;
+    d.ntets = (uint32_t)t.ntets;
+    d.nhexs = (uint32_t)t.nhexs;
+    d.evalBlocks = gmd::evalBlocksFor((uint64_t)t.ntets + (uint64_t)t.nhexs);
+    std::vector<double> p64(4 * V);
+    std::vector<float> p32(4 * V);
+    for(size_t v = 0; v < V; ++v)
+    {
+        for(int k = 0; k < 3; ++k)
+        {
+            p64[4 * v + k] = m->hostPos[3 * v + k];
+            p32[4 * v + k] = (float)m->hostPos[3 * v + k];
+        }
+        p64[4 * v + 3] = 0.0;
+        p32[4 * v + 3] = 0.0f;
+    }
+    uint4 *tb, *hb;
+    uint32_t *eo, *er, *no, *nb, *gn;
+    uint8_t* fx;
+    GM_TRY(devAlloc(&d.pos64, V));
+    GM_TRY(devAlloc(&d.pos32, V));
+    GM_TRY(devAlloc(&tb, (size_t)t.ntets));        d.tets = tb;
+    GM_TRY(devAlloc(&hb, 2 * (size_t)t.nhexs));    d.hexs = hb;
+    GM_TRY(devAlloc(&eo, V + 1));                  d.eoff = eo;
+    GM_TRY(devAlloc(&er, t.eref.size()));          d.eref = er;
+    GM_TRY(devAlloc(&no, V + 1));                  d.noff = no;
+    GM_TRY(devAlloc(&nb, t.nbr.size()));           d.nbr = nb;
+    GM_TRY(devAlloc(&fx, V));                      d.fixed = fx;
+    GM_TRY(devAlloc(&gn, t.groupNodes.size()));    d.groupNodes = gn;
+    GM_TRY(devAlloc(&d.evalCounts, V));
+    GM_TRY(devAlloc(&d.evalPartials, 3 * (size_t)d.evalBlocks));
+    GM_TRY(devAlloc(&d.evalFlags, (size_t)d.evalBlocks));
+    GM_TRY(devAlloc(&d.evalHist, 20));
+    GM_TRY(devAlloc(&d.passSlot, 1));
+
+    hipStream_t s = m->stream;
+    if(V)
+    {
+        GM_TRY(hipMemcpyAsync(d.pos64, p64.data(), V * sizeof(double4), hipMemcpyHostToDevice, s));
+        GM_TRY(hipMemcpyAsync(d.pos32, p32.data(), V * sizeof(float4), hipMemcpyHostToDevice, s));
+        GM_TRY(hipMemcpyAsync(eo, t.eoff.data(), (V + 1) * 4, hipMemcpyHostToDevice, s));
+        GM_TRY(hipMemcpyAsync(no, t.noff.data(), (V + 1) * 4, hipMemcpyHostToDevice, s));
+        GM_TRY(hipMemcpyAsync(fx, t.fixed.data(), V, hipMemcpyHostToDevice, s));
+    }
+    if(t.ntets) GM_TRY(hipMemcpyAsync(tb, t.tets.data(), (size_t)t.ntets * 16, hipMemcpyHostToDevice, s));
+    if(t.nhexs) GM_TRY(hipMemcpyAsync(hb, t.hexs.data(), (size_t)t.nhexs * 32, hipMemcpyHostToDevice, s));
+    if(!t.eref.empty()) GM_TRY(hipMemcpyAsync(er, t.eref.data(), t.eref.size() * 4, hipMemcpyHostToDevice, s));
+    if(!t.nbr.empty()) GM_TRY(hipMemcpyAsync(nb, t.nbr.data(), t.nbr.size() * 4, hipMemcpyHostToDevice, s));
+    if(!t.groupNodes.empty())
+        GM_TRY(hipMemcpyAsync(gn, t.groupNodes.data(), t.groupNodes.size() * 4, hipMemcpyHostToDevice, s));
+    GM_TRY(hipStreamSynchronize(s));
+#undef GM_TRY
+    std::vector<gm_mesh*> one{m};
+    int rc = buildPlan(one, m->stream, &m->plan);
+    if(rc) return bail(rc);
+    m->hostPos.clear();
+    m->hostPos.shrink_to_fit();
+    return GM_OK;
+}
+
 // Patch-size boundaries at which the vertex kernels change shape: wave
 // kernel lanes per vertex (8, 16, 32, 64), gradient-descent tile slots
 // (8, 16, 32) and spawn-search tiles (64, 128, 256).
@@ -78,7 +254,7 @@ static gmd::BlockWork blockSpans(const std::vector<gm_mesh*>& meshes, std::vecto
 static int buildPlanInto(Plan* p, const std::vector<gm_mesh*>& meshes)
 {
     int64_t maxColours = 0;
-    for(gm_mesh* m : meshes) maxColours = std::max(maxColours, m->topo.ngroups());
+    for(gm_mesh* m : meshes) maxColours = std::max(maxColours, m->topo->ngroups());
     p->colours.resize((size_t)maxColours);
     std::vector<uint4> spans;
     for(int64_t g = 0; g < maxColours; ++g)
@@ -98,7 +274,7 @@ static int buildPlanInto(Plan* p, const std::vector<gm_mesh*>& meshes)
                 for(size_t i = 0; i < meshes.size(); ++i)
                 {
                     const gm_mesh* m = meshes[i];
-                    const gmt::Topology& t = m->topo;
+                    const gmt::Topology& t = *m->topo;
                     if(m->kind != kind || g >= t.ngroups() || patchClass(t.groupMaxPatch[g]) != cls) continue;
                     const uint64_t n = (uint64_t)(t.groupOff[g + 1] - t.groupOff[g]);
                     if(n == 0) continue;
@@ -112,10 +288,10 @@ static int buildPlanInto(Plan* p, const std::vector<gm_mesh*>& meshes)
             }
     }
     size_t fTet, fHex, fEval, fVert;
-    p->tetBlocks = blockSpans(meshes, spans, fTet, [](const gm_mesh* m) { return ceilDiv(m->topo.ntets, gmd::ELEM_BLOCK); });
-    p->hexBlocks = blockSpans(meshes, spans, fHex, [](const gm_mesh* m) { return ceilDiv(m->topo.nhexs, gmd::ELEM_BLOCK); });
+    p->tetBlocks = blockSpans(meshes, spans, fTet, [](const gm_mesh* m) { return ceilDiv(m->topo->ntets, gmd::ELEM_BLOCK); });
+    p->hexBlocks = blockSpans(meshes, spans, fHex, [](const gm_mesh* m) { return ceilDiv(m->topo->nhexs, gmd::ELEM_BLOCK); });
     p->evalBlocks = blockSpans(meshes, spans, fEval, [](const gm_mesh* m) { return m->dm.evalBlocks; });
-    p->vertBlocks = blockSpans(meshes, spans, fVert, [](const gm_mesh* m) { return ceilDiv(m->topo.nverts, gmd::VERT_BLOCK); });
+    p->vertBlocks = blockSpans(meshes, spans, fVert, [](const gm_mesh* m) { return ceilDiv(m->topo->nverts, gmd::VERT_BLOCK); });
     std::vector<uint32_t> ml(meshes.size());
     for(size_t i = 0; i < meshes.size(); ++i) ml[i] = (uint32_t)i;
 
@@ -143,8 +319,8 @@ int buildPlan(const std::vector<gm_mesh*>& meshes, hipStream_t stream, Plan** ou
     *out = nullptr;
     if(meshes.empty()) return fail(GM_ERR_ARG, "a plan needs at least one mesh");
     for(const gm_mesh* m : meshes)
-        if((uint64_t)m->topo.nverts >= (1ull << 32) || (uint64_t)m->topo.ntets >= (1ull << 32) ||
-           (uint64_t)m->topo.nhexs >= (1ull << 32))
+        if((uint64_t)m->topo->nverts >= (1ull << 32) || (uint64_t)m->topo->ntets >= (1ull << 32) ||
+           (uint64_t)m->topo->nhexs >= (1ull << 32))
             return fail(GM_ERR_ARG, "mesh too large for 32-bit work items");
     if(meshes.size() >= (1ull << 32)) return fail(GM_ERR_ARG, "too many meshes");
     Plan* p = new Plan();
@@ -203,11 +379,10 @@ static int ensureGetme(gm_mesh* m)
     if(m->getmeReady) return GM_OK;
     // (vpp, weight) per element corner, sized for FP64; FP32 runs use the
     // front half of each buffer.
-    const size_t tetBytes = 4 * (size_t)m->topo.ntets * sizeof(double4);
-    const size_t hexBytes = 8 * (size_t)m->topo.nhexs * sizeof(double4);
+    const size_t tetBytes = 4 * (size_t)m->topo->ntets * sizeof(double4);
+    const size_t hexBytes = 8 * (size_t)m->topo->nhexs * sizeof(double4);
     GMR_HIP(hipMalloc(&m->dm.getmeTet, std::max<size_t>(32, tetBytes)));
     GMR_HIP(hipMalloc(&m->dm.getmeHex, std::max<size_t>(32, hexBytes)));
-    m->deviceBytes += tetBytes + hexBytes;
     m->getmeReady = true;
     return GM_OK;
 }
@@ -413,6 +588,30 @@ int runPlan(Plan* p, const gmq::Crew& c, int ev, int sa, const RunParams& r)
     return GM_OK;
 }
 
+int launchBegin(Plan* p, const gmq::Crew& c, int ev, int sa, bool toF32)
+{
+    hipStream_t s = p->stream;
+    GMR_HIP(gmd::launchResetRun(p->dTable, p->vertBlocks, p->meshList, p->nMeshes, s));
+    if(toF32) GMR_HIP(gmd::launchToF32(p->dTable, p->vertBlocks, s));
+    GMR_HIP(evaluateAll(p, c, ev, sa, toF32, false));
+    return GM_OK;
+}
+
+int launchPass(Plan* p, const gmq::Crew& c, int ev, int sa, const RunParams& r, bool f64, bool convertFirst)
+{
+    if(convertFirst) GMR_HIP(gmd::launchToF64(p->dTable, p->vertBlocks, p->stream));
+    RunParams rr = r;
+    rr.graph = false;
+    rr.timeKernels = false;
+    return passBody(p, c, ev, sa, rr, f64, !f64);
+}
+
+int launchToF64All(Plan* p)
+{
+    GMR_HIP(gmd::launchToF64(p->dTable, p->vertBlocks, p->stream));
+    return GM_OK;
+}
+
 int sweepColours(Plan* p, const gmq::Crew& c, int ev, int sa, const RunParams& r, int64_t g0, int64_t g1)
 {
     int rc = preparePlan(p, 1, false);
@@ -438,8 +637,8 @@ int fetchRecords(Plan* p, size_t i, int records, double* stats, int32_t* hist, i
     GMR_HIP(hipMemcpyAsync(r.data(), m->dm.evalResult, r.size() * sizeof(double), hipMemcpyDeviceToHost, p->stream));
     GMR_HIP(hipMemcpyAsync(h.data(), m->dm.evalHistOut, h.size() * sizeof(int32_t), hipMemcpyDeviceToHost,
                            p->stream));
-    if(evalCounts && m->topo.nverts)
-        GMR_HIP(hipMemcpyAsync(evalCounts, m->dm.evalCounts, (size_t)m->topo.nverts * sizeof(int32_t),
+    if(evalCounts && m->topo->nverts)
+        GMR_HIP(hipMemcpyAsync(evalCounts, m->dm.evalCounts, (size_t)m->topo->nverts * sizeof(int32_t),
                                hipMemcpyDeviceToHost, p->stream));
     GMR_HIP(hipStreamSynchronize(p->stream));
     for(int k = 0; k < records; ++k)

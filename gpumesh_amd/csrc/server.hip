@@ -1,0 +1,621 @@
+// server.hip -- the serving loop: smoothing jobs for meshes of uneven size
+// are admitted by free device memory, stay resident until their last pass,
+// and advance together one pass per round (continuous batching).
+//
+// Threads:
+//  * builders (host only) turn submitted jobs into host topologies through the
+//    shared topology cache and compute their device footprint;
+//  * one scheduler owns every HIP call.  A round launches one pass for every
+//    active job -- jobs are grouped by (smoother, FP64 pass, widening first)
+//    and each group is one fused plan on its own stream -- then, while those
+//    kernels run, uploads and starts newly admitted jobs on their own
+//    streams, and after the round retires finished jobs and frees their
+//    memory.  Jobs that arrive while others run join at the next round.
+//
+// A job's kernels, their order within the job and its reduction trees are the
+// ones gm_mesh_smooth runs for the same mesh alone, so every result is
+// bit-identical to smoothing that mesh alone with the same precision mode.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <tuple>
+#include <vector>
+
+#include "gm_runtime.h"
+
+using gmr::fail;
+using Clock = std::chrono::steady_clock;
+
+namespace
+{
+
+enum JobState { JOB_QUEUED = 0, JOB_BUILT = 1, JOB_ACTIVE = 2, JOB_DONE = 3, JOB_FAILED = 4 };
+
+struct Job
+{
+    int64_t id = 0;
+    // Inputs (released once the topology is built).
+    int64_t nverts = 0, ntets = 0, nhexs = 0;
+    std::vector<double> pos;
+    std::vector<int32_t> fixed;
+    std::vector<uint32_t> tets, hexs;
+    int smoother = 0, passes = 0, precision = 0;
+    // State.
+    int state = JOB_QUEUED;
+    gm_mesh* mesh = nullptr;
+    size_t bytes = 0;
+    int pass = 0;
+    bool inF32 = false;
+    // Results.
+    std::vector<double> outPos, stats;
+    std::vector<int32_t> hist;
+    int errCode = 0;
+    std::string err;
+    Clock::time_point tSubmit, tBuilt, tAdmit, tDone;
+};
+
+double ms(Clock::time_point a, Clock::time_point b)
+{
+    return std::chrono::duration<double, std::milli>(b - a).count();
+}
+
+} // namespace
+
+struct gm_server
+{
+    int device = 0;
+    size_t budget = 0;
+    int maxActive = 0;
+    int evaluator = 0, sampler = 0;
+    gmq::Crew crew{};
+    gmd::LaneShape lane[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+
+    std::mutex mu;
+    std::condition_variable cvSched, cvBuild, cvDone;
+    bool stop = false;
+    int64_t nextId = 1;
+    std::map<int64_t, std::unique_ptr<Job>> jobs;
+    std::deque<Job*> buildQueue;
+    std::deque<Job*> pending;        // submitted, not yet admitted, in submission order
+    size_t reserved = 0;
+
+    // Scheduler-owned.
+    std::vector<Job*> active;
+    struct Group
+    {
+        std::vector<int64_t> ids;
+        gmr::Plan* plan = nullptr;
+        hipStream_t stream = nullptr;
+    };
+    std::map<std::tuple<int, int, int>, Group> groups;
+
+    // Statistics (under mu).
+    int64_t submitted = 0, done = 0, failed = 0, rounds = 0, peakActive = 0;
+    size_t peakReserved = 0;
+    double deviceBusyMs = 0.0;
+
+    std::vector<std::thread> builders;
+    std::thread scheduler;
+};
+
+namespace
+{
+
+void finishJob(gm_server* s, Job* j, int code, const std::string& msg)
+{
+    // Called with s->mu held.
+    j->errCode = code;
+    j->err = msg;
+    j->state = code ? JOB_FAILED : JOB_DONE;
+    j->tDone = Clock::now();
+    if(code) ++s->failed;
+    else ++s->done;
+    s->cvDone.notify_all();
+}
+
+void builderLoop(gm_server* s)
+{
+    for(;;)
+    {
+        Job* j = nullptr;
+        {
+            std::unique_lock<std::mutex> lock(s->mu);
+            s->cvBuild.wait(lock, [&] { return s->stop || !s->buildQueue.empty(); });
+            if(s->stop) return;
+            j = s->buildQueue.front();
+            s->buildQueue.pop_front();
+        }
+        gm_mesh* m = nullptr;
+        int rc = gmr::makeMesh(j->nverts, j->pos.data(), j->fixed.data(), j->ntets, j->tets.data(), j->nhexs,
+                               j->hexs.data(), &m);
+        std::string msg = rc ? gmr::lastError() : "";
+        size_t bytes = 0;
+        if(!rc)
+        {
+            const gmt::Topology& t = *m->topo;
+            for(int64_t v = 0; v < t.nverts && !rc; ++v)
+                if(!t.fixed[v] && t.eoff[v + 1] == t.eoff[v])
+                {
+                    rc = GM_ERR_ARG;
+                    msg = "a free vertex belongs to no element";
+                }
+            bytes = gmr::residentBytes(m, j->passes + 1, j->smoother == gmd::SM_GETME);
+        }
+        std::lock_guard<std::mutex> lock(s->mu);
+        j->pos.clear(); j->pos.shrink_to_fit();
+        j->fixed.clear(); j->fixed.shrink_to_fit();
+        j->tets.clear(); j->tets.shrink_to_fit();
+        j->hexs.clear(); j->hexs.shrink_to_fit();
+        j->tBuilt = Clock::now();
+        if(!rc && bytes > s->budget)
+        {
+            rc = GM_ERR_NOMEM;
+            msg = "the job needs more device memory than the server's budget";
+        }
+        if(rc)
+        {
+            if(m) gmr::destroyMesh(m);
+            s->pending.erase(std::find(s->pending.begin(), s->pending.end(), j));
+            finishJob(s, j, rc, msg);
+        }
+        else
+        {
+            j->mesh = m;
+            j->bytes = bytes;
+            j->state = JOB_BUILT;
+        }
+        s->cvSched.notify_all();
+    }
+}
+
+// Admission under s->mu: the pending jobs in submission order; a built job
+// that does not fit the memory left blocks those behind it (no overtaking of
+// large jobs), a job still building is skipped.
+std::vector<Job*> takeAdmissible(gm_server* s)
+{
+    std::vector<Job*> out;
+    size_t slots = s->active.size();
+    for(auto it = s->pending.begin(); it != s->pending.end();)
+    {
+        Job* j = *it;
+        if(j->state != JOB_BUILT) { ++it; continue; }
+        if((int)(slots + out.size()) >= s->maxActive || s->reserved + j->bytes > s->budget) break;
+        s->reserved += j->bytes;
+        s->peakReserved = std::max(s->peakReserved, s->reserved);
+        j->state = JOB_ACTIVE;
+        j->tAdmit = Clock::now();
+        out.push_back(j);
+        it = s->pending.erase(it);
+    }
+    return out;
+}
+
+// Uploads and starts a job on its own stream (no wait).
+int startJob(gm_server* s, Job* j)
+{
+    gm_mesh* m = j->mesh;
+    int rc = gmr::uploadMesh(m);
+    if(rc) return rc;
+    m->crew = s->crew;
+    m->evaluator = s->evaluator;
+    m->sampler = s->sampler;
+    if((rc = gmr::preparePlan(m->plan, j->passes + 1, j->smoother == gmd::SM_GETME))) return rc;
+    j->inF32 = j->precision != GM_FP64;
+    j->pass = 0;
+    return gmr::launchBegin(m->plan, s->crew, s->evaluator, s->sampler, j->inF32);
+}
+
+// Copies a finished job's results (pass records, positions in the caller's
+// numbering) and releases its device memory.
+int retireJob(gm_server* s, Job* j)
+{
+    gm_mesh* m = j->mesh;
+    int rc = GM_OK;
+    if(j->inF32)
+    {
+        rc = gmr::launchToF64All(m->plan);
+        j->inF32 = false;
+    }
+    const int records = j->passes + 1;
+    j->stats.assign(3 * (size_t)records, 0.0);
+    j->hist.assign(20 * (size_t)records, 0);
+    if(!rc) rc = gmr::fetchRecords(m->plan, 0, records, j->stats.data(), j->hist.data(), nullptr);
+    const gmt::Topology& t = *m->topo;
+    const size_t V = (size_t)t.nverts;
+    if(!rc)
+    {
+        std::vector<double4> p(V);
+        hipError_t e = V ? hipMemcpyAsync(p.data(), m->dm.pos64, V * sizeof(double4), hipMemcpyDeviceToHost,
+                                          m->stream)
+                         : hipSuccess;
+        if(e == hipSuccess) e = hipStreamSynchronize(m->stream);
+        if(e != hipSuccess) rc = fail(GM_ERR_HIP, std::string("result copy: ") + hipGetErrorString(e));
+        j->outPos.resize(3 * V);
+        for(size_t old = 0; old < V && !rc; ++old)
+        {
+            const double4 q = p[t.permutation[old]];
+            j->outPos[3 * old] = q.x;
+            j->outPos[3 * old + 1] = q.y;
+            j->outPos[3 * old + 2] = q.z;
+        }
+    }
+    gmr::destroyMesh(m);
+    j->mesh = nullptr;
+    return rc;
+}
+
+// Brings the group plans in line with the active set and launches one pass
+// per group.  Returns the streams used.
+int launchRound(gm_server* s, const gmd::LaneShape* lanes, std::vector<hipStream_t>& used)
+{
+    std::map<std::tuple<int, int, int>, std::vector<Job*>> members;
+    for(Job* j : s->active)
+    {
+        const bool f64 = j->precision == GM_FP64 || (j->precision == GM_FP32_POLISH && j->pass == j->passes - 1);
+        const bool convert = f64 && j->inF32;
+        members[std::make_tuple(j->smoother, (int)f64, (int)convert)].push_back(j);
+    }
+    // Drop plans whose membership changed.
+    for(auto& kv : s->groups)
+    {
+        auto it = members.find(kv.first);
+        std::vector<int64_t> ids;
+        if(it != members.end())
+            for(Job* j : it->second) ids.push_back(j->id);
+        if(ids != kv.second.ids && kv.second.plan)
+        {
+            gmr::freePlan(kv.second.plan);
+            kv.second.plan = nullptr;
+            kv.second.ids.clear();
+        }
+    }
+    for(auto& kv : members)
+    {
+        gm_server::Group& g = s->groups[kv.first];
+        if(!g.stream && hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking) != hipSuccess)
+            return fail(GM_ERR_HIP, "stream creation failed");
+        if(!g.plan)
+        {
+            std::vector<gm_mesh*> ms;
+            for(Job* j : kv.second)
+            {
+                ms.push_back(j->mesh);
+                g.ids.push_back(j->id);
+            }
+            int rc = gmr::buildPlan(ms, g.stream, &g.plan);
+            if(!rc) rc = gmr::preparePlan(g.plan, 0, false);
+            if(rc)
+            {
+                g.ids.clear();
+                return rc;
+            }
+        }
+        const int smoother = std::get<0>(kv.first);
+        gmr::RunParams r;
+        r.smoother = smoother;
+        r.kernel = GM_KERNEL_WAVE;
+        r.spawnCoeff = 1.0 / 20.0;
+        int maxPatch = 0;
+        for(gm_mesh* m : g.plan->meshes) maxPatch = std::max(maxPatch, m->maxPatch);
+        if(lanes[smoother].patchesPerGroup)
+            r.lane = lanes[smoother];
+        else
+        {
+            r.lane = gmd::LaneShape{1, 64};
+            if(smoother == gmd::SM_GRADIENT_DESCENT)
+                r.lane.tileElems = maxPatch <= 8 ? 64 : (maxPatch <= 16 ? 128 : 256);
+            else
+                r.lane.tileElems = maxPatch <= 64 ? 64 : (maxPatch <= 128 ? 128 : 256);
+        }
+        int rc = gmr::launchPass(g.plan, s->crew, s->evaluator, s->sampler, r, std::get<1>(kv.first) != 0,
+                                 std::get<2>(kv.first) != 0);
+        if(rc) return rc;
+        used.push_back(g.stream);
+    }
+    return GM_OK;
+}
+
+void schedulerLoop(gm_server* s)
+{
+    (void)hipSetDevice(s->device);
+    for(;;)
+    {
+        std::vector<Job*> admit;
+        gmd::LaneShape lanes[4];
+        {
+            std::unique_lock<std::mutex> lock(s->mu);
+            s->cvSched.wait(lock, [&] {
+                if(s->stop || !s->active.empty()) return true;
+                for(Job* j : s->pending)
+                    if(j->state == JOB_BUILT) return true;
+                return false;
+            });
+            if(s->stop) break;
+            admit = takeAdmissible(s);
+            for(int k = 0; k < 4; ++k) lanes[k] = s->lane[k];
+        }
+        const Clock::time_point t0 = Clock::now();
+        std::vector<hipStream_t> used;
+        int rc = s->active.empty() ? GM_OK : launchRound(s, lanes, used);
+        if(rc)
+        {
+            // A launch failure poisons every active job of the round; wait for
+            // whatever was launched before releasing memory.
+            const std::string msg = gmr::lastError();
+            for(auto& kv : s->groups)
+                if(kv.second.stream) (void)hipStreamSynchronize(kv.second.stream);
+            used.clear();
+            std::lock_guard<std::mutex> lock(s->mu);
+            for(Job* j : s->active)
+            {
+                gmr::destroyMesh(j->mesh);
+                j->mesh = nullptr;
+                s->reserved -= j->bytes;
+                finishJob(s, j, rc, msg);
+            }
+            s->active.clear();
+        }
+        // Start the admitted jobs while the round runs.
+        std::vector<Job*> started;
+        for(Job* j : admit)
+        {
+            int src = startJob(s, j);
+            std::lock_guard<std::mutex> lock(s->mu);
+            if(src)
+            {
+                if(j->mesh)
+                {
+                    gmr::destroyMesh(j->mesh);
+                    j->mesh = nullptr;
+                }
+                s->reserved -= j->bytes;
+                finishJob(s, j, src, gmr::lastError());
+            }
+            else
+                started.push_back(j);
+        }
+        for(hipStream_t st : used) (void)hipStreamSynchronize(st);
+        for(Job* j : started) (void)hipStreamSynchronize(j->mesh->stream);
+        const Clock::time_point t1 = Clock::now();
+        // Retire what finished.
+        std::vector<Job*> still;
+        for(Job* j : s->active)
+        {
+            const bool f64 = j->precision == GM_FP64 || (j->precision == GM_FP32_POLISH && j->pass == j->passes - 1);
+            if(f64) j->inF32 = false;
+            ++j->pass;
+            if(j->pass < j->passes)
+            {
+                still.push_back(j);
+                continue;
+            }
+            int rrc = retireJob(s, j);
+            std::lock_guard<std::mutex> lock(s->mu);
+            s->reserved -= j->bytes;
+            finishJob(s, j, rrc, rrc ? gmr::lastError() : "");
+        }
+        // Jobs with no passes are finished at admission.
+        for(Job* j : started)
+        {
+            if(j->passes > 0)
+            {
+                still.push_back(j);
+                continue;
+            }
+            int rrc = retireJob(s, j);
+            std::lock_guard<std::mutex> lock(s->mu);
+            s->reserved -= j->bytes;
+            finishJob(s, j, rrc, rrc ? gmr::lastError() : "");
+        }
+        s->active.swap(still);
+        std::lock_guard<std::mutex> lock(s->mu);
+        if(!used.empty())
+        {
+            ++s->rounds;
+            s->deviceBusyMs += ms(t0, t1);
+        }
+        s->peakActive = std::max<int64_t>(s->peakActive, (int64_t)s->active.size());
+    }
+    // Shutdown: release every resident job.
+    for(Job* j : s->active)
+    {
+        gmr::destroyMesh(j->mesh);
+        j->mesh = nullptr;
+        std::lock_guard<std::mutex> lock(s->mu);
+        s->reserved -= j->bytes;
+        finishJob(s, j, GM_ERR_ARG, "server shut down");
+    }
+    s->active.clear();
+    for(auto& kv : s->groups)
+    {
+        gmr::freePlan(kv.second.plan);
+        if(kv.second.stream) (void)hipStreamDestroy(kv.second.stream);
+    }
+    s->groups.clear();
+}
+
+} // namespace
+
+extern "C" {
+
+int gm_server_create(int64_t budgetBytes, int maxActive, int buildThreads, int evaluator, int sampler,
+                     double scaling, gm_server** out)
+{
+    if(!out) return fail(GM_ERR_ARG, "out is null");
+    *out = nullptr;
+    if(evaluator < 0 || evaluator > 1) return fail(GM_ERR_ARG, "unknown evaluator");
+    if(sampler < 0 || sampler > 1)
+        return fail(sampler == GM_TEXTURE ? GM_ERR_UNSUPPORTED : GM_ERR_ARG,
+                    "the server takes the uniform or analytic sampler");
+    if(!(scaling > 0.0)) return fail(GM_ERR_ARG, "scaling must be positive");
+    if(maxActive < 1 || buildThreads < 1 || buildThreads > 64) return fail(GM_ERR_ARG, "bad slot or thread count");
+    int ndev = 0;
+    if(hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(GM_ERR_HIP, "no HIP device visible");
+    auto s = std::make_unique<gm_server>();
+    if(hipGetDevice(&s->device) != hipSuccess) return fail(GM_ERR_HIP, "hipGetDevice failed");
+    size_t freeB = 0, totalB = 0;
+    if(hipMemGetInfo(&freeB, &totalB) != hipSuccess) return fail(GM_ERR_HIP, "hipMemGetInfo failed");
+    // The budget never exceeds what is free now, less 1 GiB for plans,
+    // streams and the runtime.
+    const size_t headroom = (size_t)1 << 30;
+    const size_t avail = freeB > headroom ? freeB - headroom : 0;
+    s->budget = budgetBytes > 0 ? std::min((size_t)budgetBytes, avail) : avail;
+    s->maxActive = maxActive;
+    s->evaluator = evaluator;
+    s->sampler = sampler;
+    s->crew.scaling = scaling;
+    s->crew.scaling2 = scaling * scaling;
+    s->crew.scaling3 = s->crew.scaling2 * scaling;
+    gm_server* raw = s.get();
+    for(int i = 0; i < buildThreads; ++i) raw->builders.emplace_back(builderLoop, raw);
+    raw->scheduler = std::thread(schedulerLoop, raw);
+    *out = s.release();
+    return GM_OK;
+}
+
+int gm_server_destroy(gm_server* s)
+{
+    if(!s) return GM_OK;
+    {
+        std::lock_guard<std::mutex> lock(s->mu);
+        s->stop = true;
+    }
+    s->cvBuild.notify_all();
+    s->cvSched.notify_all();
+    for(std::thread& t : s->builders) t.join();
+    s->scheduler.join();
+    for(auto& kv : s->jobs)
+        if(kv.second->mesh) gmr::destroyMesh(kv.second->mesh);
+    delete s;
+    return GM_OK;
+}
+
+int gm_server_set_shape(gm_server* s, int smoother, int patchesPerGroup, int tileElems)
+{
+    if(!s) return fail(GM_ERR_ARG, "null server");
+    if(smoother < 0 || smoother > 3) return fail(GM_ERR_ARG, "unknown smoother");
+    if(patchesPerGroup != 1 && patchesPerGroup != 2 && patchesPerGroup != 4 && patchesPerGroup != 8)
+        return fail(GM_ERR_ARG, "patches per workgroup must be 1, 2, 4 or 8");
+    if(tileElems != 64 && tileElems != 128 && tileElems != 256)
+        return fail(GM_ERR_ARG, "LDS tile must be 64, 128 or 256 elements");
+    std::lock_guard<std::mutex> lock(s->mu);
+    s->lane[smoother] = gmd::LaneShape{patchesPerGroup, tileElems};
+    return GM_OK;
+}
+
+int gm_server_submit(gm_server* s, int64_t nverts, const double* pos, const int32_t* fixed, int64_t ntets,
+                     const uint32_t* tets, int64_t nhexs, const uint32_t* hexs, int smoother, int passes,
+                     int precision, int64_t* jobId)
+{
+    if(!s || !jobId) return fail(GM_ERR_ARG, "null argument");
+    if(nverts < 0 || ntets < 0 || nhexs < 0) return fail(GM_ERR_ARG, "negative size");
+    if((nverts && (!pos || !fixed)) || (ntets && !tets) || (nhexs && !hexs)) return fail(GM_ERR_ARG, "null array");
+    if(smoother < 0 || smoother > 3) return fail(GM_ERR_ARG, "unknown smoother");
+    if(passes < 0) return fail(GM_ERR_ARG, "negative pass count");
+    if(precision < 0 || precision > 2) return fail(GM_ERR_ARG, "unknown precision");
+    auto j = std::make_unique<Job>();
+    j->nverts = nverts;
+    j->ntets = ntets;
+    j->nhexs = nhexs;
+    j->pos.assign(pos, pos + 3 * (size_t)nverts);
+    j->fixed.assign(fixed, fixed + (size_t)nverts);
+    j->tets.assign(tets, tets + 4 * (size_t)ntets);
+    j->hexs.assign(hexs, hexs + 8 * (size_t)nhexs);
+    j->smoother = smoother;
+    j->passes = passes;
+    j->precision = precision;
+    j->tSubmit = Clock::now();
+    std::lock_guard<std::mutex> lock(s->mu);
+    if(s->stop) return fail(GM_ERR_ARG, "server is shutting down");
+    j->id = s->nextId++;
+    *jobId = j->id;
+    Job* raw = j.get();
+    s->jobs.emplace(raw->id, std::move(j));
+    s->buildQueue.push_back(raw);
+    s->pending.push_back(raw);
+    ++s->submitted;
+    s->cvBuild.notify_one();
+    return GM_OK;
+}
+
+int gm_server_wait(gm_server* s, int64_t jobId, double timeoutMs, int* state)
+{
+    if(!s) return fail(GM_ERR_ARG, "null server");
+    std::unique_lock<std::mutex> lock(s->mu);
+    auto it = s->jobs.find(jobId);
+    if(it == s->jobs.end()) return fail(GM_ERR_ARG, "unknown job");
+    Job* j = it->second.get();
+    auto finished = [&] { return j->state == JOB_DONE || j->state == JOB_FAILED; };
+    if(timeoutMs < 0)
+        s->cvDone.wait(lock, finished);
+    else
+        s->cvDone.wait_for(lock, std::chrono::duration<double, std::milli>(timeoutMs), finished);
+    if(state) *state = j->state;
+    return GM_OK;
+}
+
+int gm_server_result(gm_server* s, int64_t jobId, double* pos, double* stats, int32_t* hist, double* times4)
+{
+    if(!s) return fail(GM_ERR_ARG, "null server");
+    std::lock_guard<std::mutex> lock(s->mu);
+    auto it = s->jobs.find(jobId);
+    if(it == s->jobs.end()) return fail(GM_ERR_ARG, "unknown job");
+    Job* j = it->second.get();
+    if(j->state == JOB_FAILED) return fail(j->errCode, "job failed: " + j->err);
+    if(j->state != JOB_DONE) return fail(GM_ERR_ARG, "job has not finished");
+    if(pos) std::memcpy(pos, j->outPos.data(), j->outPos.size() * sizeof(double));
+    if(stats)
+        for(int k = 0; k <= j->passes; ++k)
+        {
+            stats[3 * k] = j->stats[3 * k];
+            stats[3 * k + 1] = j->stats[3 * k + 1];
+            stats[3 * k + 2] = j->stats[3 * k + 2];
+        }
+    if(hist) std::memcpy(hist, j->hist.data(), j->hist.size() * sizeof(int32_t));
+    if(times4)
+    {
+        times4[0] = ms(j->tSubmit, j->tBuilt);
+        times4[1] = ms(j->tBuilt, j->tAdmit);
+        times4[2] = ms(j->tAdmit, j->tDone);
+        times4[3] = ms(j->tSubmit, j->tDone);
+    }
+    return GM_OK;
+}
+
+int gm_server_release(gm_server* s, int64_t jobId)
+{
+    if(!s) return fail(GM_ERR_ARG, "null server");
+    std::lock_guard<std::mutex> lock(s->mu);
+    auto it = s->jobs.find(jobId);
+    if(it == s->jobs.end()) return fail(GM_ERR_ARG, "unknown job");
+    if(it->second->state != JOB_DONE && it->second->state != JOB_FAILED)
+        return fail(GM_ERR_ARG, "job has not finished");
+    s->jobs.erase(it);
+    return GM_OK;
+}
+
+int gm_server_stats(gm_server* s, double* out8)
+{
+    if(!s || !out8) return fail(GM_ERR_ARG, "null argument");
+    std::lock_guard<std::mutex> lock(s->mu);
+    out8[0] = (double)s->submitted;
+    out8[1] = (double)s->done;
+    out8[2] = (double)s->failed;
+    out8[3] = (double)s->rounds;
+    out8[4] = (double)s->peakActive;
+    out8[5] = (double)s->peakReserved;
+    out8[6] = (double)s->budget;
+    out8[7] = s->deviceBusyMs;
+    return GM_OK;
+}
+
+} // extern "C"

@@ -153,6 +153,15 @@ enum gm_kernel_class
  * launches[GM_KT_COUNT] (either may be NULL). */
 int gm_mesh_kernel_times(const gm_mesh* mesh, double* ms, int64_t* launches);
 
+/* Host topology cache shared by every mesh created afterwards: meshes with
+ * identical connectivity and fixed flags (differing only in positions) reuse
+ * one built topology (patch lists, colour groups, renumbering), found by an
+ * exact comparison of the inputs.  max_bytes bounds the host memory it holds
+ * (least recently used entries go first); 0, the default, disables it. */
+int gm_set_topology_cache(int64_t max_bytes);
+/* out4 = {hits, misses, entries, bytes held}. */
+int gm_topology_cache_stats(int64_t* out4);
+
 /* Device memory owned by a mesh handle (positions, connectivity, patch
  * lists, reduction buffers, GETMe scratch once allocated), in bytes. */
 int gm_mesh_device_bytes(const gm_mesh* mesh, int64_t* bytes);
@@ -196,6 +205,51 @@ int gm_batch_kernel_times(const gm_batch* batch, double* ms, int64_t* launches);
  * where a shape does not fit every list), restores the positions after each
  * trial, keeps the fastest shape for `smoother` and returns it in best[2]. */
 int gm_batch_autotune(gm_batch* batch, int smoother, int precision, int passes, int32_t* best, double* times);
+
+/* ---- Serving: a request loop over resident meshes.
+ *
+ * Jobs (a mesh, a smoother, a pass count, a precision mode) are submitted
+ * from any thread and return at once.  Builder threads build each job's host
+ * topology (through the topology cache, gm_set_topology_cache); a scheduler
+ * thread admits built jobs in submission order while their device footprint
+ * fits the memory budget and a slot is free, keeps them resident, and runs
+ * rounds: every active job advances one pass per round, the jobs of one
+ * (smoother, FP64 pass) group in fused launches, groups on their own streams,
+ * and jobs admitted meanwhile join the next round (continuous batching).  A
+ * finished job's memory is released at once.  Each job's result is
+ * bit-identical to gm_mesh_smooth on that mesh alone (same smoother, passes
+ * and precision, GM_KERNEL_WAVE, spawn coefficient 1/20).  The reference has
+ * no serving layer; a job is one AbstractSmoother::smoothMesh call
+ * (Smoothers/AbstractSmoother.cpp:57-211).
+ *
+ * budget_bytes <= 0: all device memory free at creation less 1 GiB (a larger
+ * request is clamped to that).  A job whose footprint exceeds the budget fails
+ * with GM_ERR_NOMEM.  Crew: evaluator, sampler (uniform or analytic) and
+ * scaling for every job.  Destroying the server cancels unfinished jobs. */
+typedef struct gm_server gm_server;
+
+int gm_server_create(int64_t budget_bytes, int max_active, int build_threads, int evaluator, int sampler,
+                     double scaling, gm_server** out);
+int gm_server_destroy(gm_server* server);
+/* Launch shape for `smoother` (as gm_batch_set_shape). */
+int gm_server_set_shape(gm_server* server, int smoother, int patches_per_group, int tile_elems);
+/* Copies the mesh (caller numbering) and queues the job; *job = its id. */
+int gm_server_submit(gm_server* server, int64_t nverts, const double* pos, const int32_t* fixed, int64_t ntets,
+                     const uint32_t* tets, int64_t nhexs, const uint32_t* hexs, int smoother, int passes,
+                     int precision, int64_t* job);
+/* Waits until the job finished or timeout_ms passed (< 0: no limit);
+ * *state = 0 queued, 1 built, 2 active, 3 done, 4 failed. */
+int gm_server_wait(gm_server* server, int64_t job, double timeout_ms, int* state);
+/* Results of a finished job: pos[3 nverts] in the caller's numbering,
+ * stats[3 (passes+1)] and hist[20 (passes+1)] as gm_mesh_smooth, times4 =
+ * {submit->built, built->admitted, admitted->done, submit->done} in ms (any
+ * may be NULL).  A failed job returns its error. */
+int gm_server_result(gm_server* server, int64_t job, double* pos, double* stats, int32_t* hist, double* times4);
+/* Forgets a finished job. */
+int gm_server_release(gm_server* server, int64_t job);
+/* out8 = {submitted, done, failed, rounds, peak active jobs, peak reserved
+ * bytes, budget bytes, wall ms of the rounds}. */
+int gm_server_stats(gm_server* server, double* out8);
 
 /* Synthetic input: an n^3-cell unit-cube mesh, kind 0 = Kuhn tetrahedra
  * (6 per cell, det(v3-v0, v3-v1, v3-v2) > 0), 1 = hexahedra (v0..v3 bottom

@@ -56,3 +56,12 @@ def test_empty_mesh_builds():
     gm = GpuMesh(np.zeros((0, 3)), np.zeros(0, np.int32), np.zeros((0, 4), np.uint32),
                  np.zeros((0, 8), np.uint32), device=False)
     assert gm.nverts == 0 and gm.ngroups == 0
+
+
+def test_server_needs_a_device():
+    from gpumesh_amd import GpuServer, device_count
+    if device_count() > 0:
+        pytest.skip("a device is visible")
+    with pytest.raises(_lib.GpuMeshError) as e:
+        GpuServer()
+    assert e.value.status == 2

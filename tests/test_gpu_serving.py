@@ -1,0 +1,101 @@
+"""Serving loop: jobs that arrive while others run are admitted as memory
+and slots free up and advance together in fused rounds; every result must
+be bit-identical to smoothing that mesh alone with gm_mesh_smooth (same
+smoother, passes and precision), whatever else was in the mix."""
+import time
+
+import numpy as np
+import pytest
+
+from gpumesh_amd import GpuMesh, GpuServer, cube_mesh
+from gpumesh_amd._lib import GpuMeshError
+
+pytestmark = pytest.mark.gpu
+
+JOBS = [  # (n, kind, jitter, seed, smoother, passes, precision)
+    (9, "tet", 0.3, 101, "nm", 4, "fp32_polish"),
+    (14, "hex", 0.25, 102, "gd", 6, "fp32_polish"),
+    (6, "tet", 0.4, 103, "getme", 5, "fp32"),
+    (17, "tet", 0.2, 104, "gd", 3, "fp64"),
+    (11, "hex", 0.35, 105, "getme", 7, "fp32_polish"),
+    (4, "tet", 0.3, 106, "spawn", 2, "fp64"),
+    (20, "tet", 0.3, 107, "getme", 5, "fp32_polish"),
+    (8, "hex", 0.3, 108, "nm", 3, "fp32"),
+    (9, "tet", 0.3, 109, "nm", 1, "fp32_polish"),
+    (12, "tet", 0.3, 110, "gd", 0, "fp64"),
+]
+
+
+def _alone(spec):
+    n, kind, jitter, seed, smoother, passes, precision = spec
+    m = cube_mesh(n, kind, jitter, seed)
+    gm = GpuMesh.from_cube(m)
+    r = gm.smooth(smoother, passes, precision=precision, kernel="wave")
+    pos = gm.positions()[gm.permutation()]       # back to the caller's numbering
+    return pos, r
+
+
+def _same(a, b):
+    return np.array_equal(np.asarray(a).view(np.uint64), np.asarray(b).view(np.uint64))
+
+
+def _check(res, ref, tag):
+    pos, r = ref
+    assert _same(res["pos"], pos), tag
+    assert _same(res["min"], r["min"]), tag
+    assert _same(res["inv_sum"], r["inv_sum"]), tag
+    assert np.array_equal(res["hist"], r["hist"]), tag
+
+
+def test_stream_results_match_alone():
+    refs = [_alone(s) for s in JOBS]
+    with GpuServer(build_threads=4) as srv:
+        ids = []
+        for k, (n, kind, jitter, seed, smoother, passes, precision) in enumerate(JOBS):
+            ids.append(srv.submit_mesh(cube_mesh(n, kind, jitter, seed), smoother, passes, precision))
+            time.sleep(0.002 * (k % 3))          # staggered arrivals: later jobs join running rounds
+        for k, j in enumerate(ids):
+            assert srv.wait(j, 60000) == "done", k
+            _check(srv.result(j), refs[k], k)
+        st = srv.stats()
+        assert st["done"] == len(JOBS) and st["failed"] == 0
+        assert st["peak_active"] >= 2            # jobs overlapped on the device
+
+
+def test_admission_by_memory_budget():
+    """A budget that holds about two of the jobs: jobs wait for memory, never
+    exceed the budget, and still match their solo results; a job larger than
+    the whole budget fails with GM_ERR_NOMEM instead of blocking the queue."""
+    specs = [(12, "tet", 0.3, 200 + k, "getme", 3, "fp32_polish") for k in range(6)]
+    refs = [_alone(s) for s in specs]
+    probe = GpuMesh.from_cube(cube_mesh(12, "tet", 0.3, 200))
+    probe.smooth("getme", 3, precision="fp32_polish", kernel="wave")
+    one = probe.device_bytes()
+    budget = int(2.6 * one)
+    with GpuServer(budget_bytes=budget, build_threads=2) as srv:
+        big = cube_mesh(30, "tet", 0.3, 1)
+        jb = srv.submit_mesh(big, "getme", 2, "fp32")
+        ids = [srv.submit_mesh(cube_mesh(*s[:4]), s[4], s[5], s[6]) for s in specs]
+        assert srv.wait(jb, 60000) == "failed"
+        with pytest.raises(GpuMeshError) as e:
+            srv.result(jb)
+        assert e.value.status == 4
+        for k, j in enumerate(ids):
+            assert srv.wait(j, 60000) == "done"
+            _check(srv.result(j), refs[k], k)
+        st = srv.stats()
+        assert st["peak_reserved_bytes"] <= budget
+        assert st["peak_active"] <= 2
+
+
+def test_slot_limit_and_release():
+    with GpuServer(max_active=1, build_threads=1) as srv:
+        specs = [(5, "hex", 0.3, 300 + k, "nm", 2, "fp64") for k in range(3)]
+        refs = [_alone(s) for s in specs]
+        ids = [srv.submit_mesh(cube_mesh(*s[:4]), s[4], s[5], s[6]) for s in specs]
+        for k, j in enumerate(ids):
+            assert srv.wait(j) == "done"
+            _check(srv.result(j), refs[k], k)
+        assert srv.stats()["peak_active"] == 1
+        with pytest.raises(GpuMeshError):
+            srv.wait(ids[0])                     # released with its result

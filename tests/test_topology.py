@@ -76,3 +76,40 @@ def test_large_kuhn_cube_groups(oracle):
     m = cube_mesh(40, "tet", 0.35, seed=2)
     om, gm = _compare(oracle, m.pos, m.fixed, m.tets, m.hexs)
     assert gm.max_patch == 24 and gm.nmovable == 39 ** 3
+
+
+def test_topology_cache_hits_are_exact(oracle):
+    """With the topology cache on, meshes that share connectivity reuse one
+    build; a hit must give exactly the topology of a fresh build, and a mesh
+    whose connectivity or fixed flags differ must miss."""
+    from gpumesh_amd._lib import check, lib
+    import ctypes
+
+    def stats():
+        out = (ctypes.c_int64 * 4)()
+        check(lib().gm_topology_cache_stats(out), "stats")
+        return list(out)
+
+    check(lib().gm_set_topology_cache(1 << 30), "cache on")
+    try:
+        a = cube_mesh(9, "tet", 0.3, seed=1)
+        b = cube_mesh(9, "tet", 0.3, seed=2)        # same connectivity, other positions
+        s0 = stats()
+        ga = GpuMesh.from_cube(a, device=False)
+        s1 = stats()
+        _, gb = _compare(oracle, b.pos, b.fixed, b.tets, b.hexs)
+        s2 = stats()
+        assert s1[1] == s0[1] + 1 and s2[0] == s1[0] + 1       # a misses, b hits
+        assert np.array_equal(ga.permutation(), gb.permutation())
+        fixed2 = b.fixed.copy()
+        fixed2[np.nonzero(fixed2 == 0)[0][0]] = 1               # one more fixed vertex: must miss
+        _compare(oracle, b.pos, fixed2, b.tets, b.hexs)
+        s3 = stats()
+        assert s3[1] == s2[1] + 1
+        tets2 = b.tets.copy()
+        tets2[[0, 1]] = tets2[[1, 0]]                           # element order matters: must miss
+        _compare(oracle, b.pos, b.fixed, tets2, b.hexs)
+        assert stats()[1] == s3[1] + 1
+    finally:
+        check(lib().gm_set_topology_cache(0), "cache off")
+    assert stats()[2] == 0
