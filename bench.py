@@ -15,6 +15,18 @@ timed region).  Also reported: the plain FP64 single-mesh kernel (one lane
 per vertex) on the same workload, the roofline of the dominant kernel and the
 CPU oracle on a bounded sample of the same work.
 
+Two more legs run after it and are reported beside the headline value:
+  * batched: 256 meshes (Kuhn tet or, with probability 0.3, hex cubes of n^3
+    cells, n log-uniform in [8, 60], seeds 1000..1255, jitter 0.3 h) smoothed
+    together by GETMe for 10 FP32 passes, one fused launch per colour step and
+    element kind, each pass replayed from a HIP graph, launch shape chosen by
+    the autotuner; compared with the same meshes smoothed one by one;
+  * serving: a stream of jobs (tet or hex cubes, n log-uniform in [8, 100],
+    jitter U[0.2, 0.4] h, 5-20 passes, Nelder-Mead, gradient descent or
+    GETMe, FP32 with an FP64 polish pass) arriving as a Poisson process at
+    several rates, admitted by free device memory and batched continuously;
+    p50/p99 latency, jobs/s and vertices/s per rate.
+
 Usage: python bench.py [--gpus 1] [--steps 20] [--warmup 2]
 """
 import argparse
@@ -79,6 +91,141 @@ def cpu_baseline(n, passes, threads, smoothers, spawn_coeff):
     return moved / t_total, t_total, m
 
 
+def batched_leg(args):
+    """256 uneven meshes, GETMe, FP32, fused launches, graph replay, autotuned."""
+    from gpumesh_amd import GpuBatch, GpuMesh, cube_mesh
+    specs = []
+    for s in range(1000, 1000 + args.batch_meshes):
+        rng = np.random.default_rng(s)
+        n = int(round(np.exp(rng.uniform(np.log(8.0), np.log(60.0)))))
+        specs.append((n, "hex" if rng.random() < 0.3 else "tet", s))
+    t0 = time.perf_counter()
+    meshes = [cube_mesh(n, kind, 0.3, seed=s) for n, kind, s in specs]
+    t_gen = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    gms = [GpuMesh.from_cube(m) for m in meshes]
+    t_up = time.perf_counter() - t0
+    starts = [g.positions() for g in gms]
+    movable = sum(g.nmovable for g in gms)
+    elems = sum(g.ntets + g.nhexs for g in gms)
+    passes = args.batch_passes
+    log(f"batched: {len(gms)} meshes, {sum(m.nverts for m in meshes)} vertices ({movable} movable), {elems} elements, "
+        f"{sum(1 for _, k, _ in specs if k == 'hex')} hex meshes; generate {t_gen:.2f}s, topology+upload {t_up:.2f}s")
+
+    def reset():
+        for g, p in zip(gms, starts):
+            g.set_positions(p)
+
+    b = GpuBatch(gms)
+    (ppg, tile), times = b.autotune("getme", precision="fp32", passes=1)
+    log(f"batched autotune getme: best {ppg} wavefronts/block; sweep ms (tile 64 column) "
+        f"{[round(float(x), 3) for x in times[:, 0]]}")
+    b.smooth("getme", passes, precision="fp32", graph=True)          # captures the pass graph
+    reset()
+    t0 = time.perf_counter()
+    r = b.smooth("getme", passes, precision="fp32", graph=True)
+    wall = time.perf_counter() - t0
+    reset()
+    t0 = time.perf_counter()
+    b.smooth("getme", passes, precision="fp32", graph=False)
+    wall_eager = time.perf_counter() - t0
+    # The same meshes one by one (FP32, then the plain FP64 kernel).
+    reset()
+    t0 = time.perf_counter()
+    for g in gms:
+        g.smooth("getme", passes, precision="fp32", kernel="wave", counts=False)
+    wall_single = time.perf_counter() - t0
+    reset()
+    t0 = time.perf_counter()
+    for g in gms:
+        g.smooth("getme", passes, precision="fp64", kernel="plain", counts=False)
+    wall_plain = time.perf_counter() - t0
+    log(f"batched getme x{passes}: graph {wall * 1e3:.1f} ms, eager {wall_eager * 1e3:.1f} ms, one by one "
+        f"{wall_single * 1e3:.1f} ms, plain fp64 one by one {wall_plain * 1e3:.1f} ms")
+    b.close()
+    return {"workload": f"{len(gms)} cube meshes, n log-uniform [8,60], 30% hex, seeds 1000..{999 + len(gms)}, "
+                        f"jitter 0.3h, mean ratio, GETMe x {passes} passes, fp32, graph-replayed fused passes",
+            "meshes": len(gms), "movable_vertices": movable, "elements": elems, "passes": passes,
+            "vertices_per_s": movable * passes / wall, "ms_per_pass": wall / passes * 1e3,
+            "elements_per_s": elems * passes / wall,
+            "eager_vertices_per_s": movable * passes / wall_eager,
+            "one_by_one_fp32_vertices_per_s": movable * passes / wall_single,
+            "one_by_one_plain_fp64_vertices_per_s": movable * passes / wall_plain,
+            "speedup_vs_plain_fp64": wall_plain / wall,
+            "autotune": {"best_patches_per_group": ppg, "sweep_ms_by_ppg": [float(x) for x in times[:, 0]]},
+            "final_min_mean": [float(np.min(r["min"][:, -1])), float(np.mean(r["mean"][:, -1]))]}
+
+
+def serving_leg(args):
+    """Poisson job stream through the serving loop at several rates."""
+    from gpumesh_amd import GpuMesh, GpuServer, cube_mesh
+    rng = np.random.default_rng(5)
+    jobs = []
+    for k in range(args.serve_jobs):
+        n = int(round(np.exp(rng.uniform(np.log(8.0), np.log(100.0)))))
+        kind = "tet" if rng.random() < 0.5 else "hex"
+        jitter = float(rng.uniform(0.2, 0.4))
+        passes = int(rng.integers(5, 21))
+        smoother = ["nm", "gd", "getme"][int(rng.integers(0, 3))]
+        jobs.append((n, kind, jitter, 5000 + k, smoother, passes))
+    t0 = time.perf_counter()
+    meshes = [cube_mesh(n, kind, jit, seed=sd) for n, kind, jit, sd, _, _ in jobs]
+    log(f"serving: {len(jobs)} jobs, {sum(m.nverts for m in meshes)} vertices in total, generated in "
+        f"{time.perf_counter() - t0:.2f}s")
+    work = sum(m.nmovable * j[5] for m, j in zip(meshes, jobs))
+    rates = [float(x) for x in args.serve_rates.split(",") if x]
+    out = {"workload": f"{len(jobs)} jobs per rate: tet/hex 50/50 cubes, n log-uniform [8,100], jitter U[0.2,0.4]h, "
+                       f"5-20 passes, NM/GD/GETMe uniform, fp32 + fp64 polish, Poisson arrivals",
+           "jobs": len(jobs), "vertex_passes_per_run": work, "rates": {}}
+    with GpuServer(budget_bytes=288 * 10**9, build_threads=8) as srv:
+        # Untimed warm-up jobs (first launches of each kernel).
+        warm = [srv.submit_mesh(meshes[0], sm, 1) for sm in ("nm", "gd", "getme")]
+        for w in warm:
+            srv.wait(w)
+            srv.release(w)
+        for rate in rates:
+            arr = np.cumsum(np.random.default_rng(int(rate) + 5).exponential(1.0 / rate, len(jobs)))
+            ids = []
+            t0 = time.perf_counter()
+            for (n, kind, jit, sd, sm, p), m, at in zip(jobs, meshes, arr):
+                delay = t0 + at - time.perf_counter()
+                if delay > 0:
+                    time.sleep(delay)
+                ids.append(srv.submit_mesh(m, sm, p, "fp32_polish"))
+            lat = []
+            for j in ids:
+                st = srv.wait(j, 120000)
+                if st != "done":
+                    raise RuntimeError(f"serving job ended {st}")
+                lat.append(srv.result(j)["latency_ms"])
+            span = time.perf_counter() - t0
+            lat = np.array(lat)
+            out["rates"][str(int(rate))] = {
+                "offered_jobs_per_s": rate, "jobs_per_s": len(jobs) / span, "vertices_per_s": work / span,
+                "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+                "max_ms": float(lat.max()), "span_s": span}
+            log(f"serving rate {rate:.0f}/s: {len(jobs) / span:.1f} jobs/s, {work / span / 1e6:.1f} M vertices/s, "
+                f"p50 {np.percentile(lat, 50):.1f} ms, p99 {np.percentile(lat, 99):.1f} ms")
+        st = srv.stats()
+        out["server"] = {"rounds": st["rounds"], "peak_active": st["peak_active"],
+                         "peak_reserved_gb": st["peak_reserved_bytes"] / 1e9, "budget_gb": st["budget_bytes"] / 1e9}
+    # The same jobs one by one through the plain FP64 kernel (a bounded sample).
+    sample = list(range(min(args.serve_plain_sample, len(jobs))))
+    t_plain = 0.0
+    w_plain = 0
+    for k in sample:
+        g = GpuMesh.from_cube(meshes[k])
+        t0 = time.perf_counter()
+        g.smooth(jobs[k][4], jobs[k][5], precision="fp64", kernel="plain", counts=False)
+        t_plain += time.perf_counter() - t0
+        w_plain += meshes[k].nmovable * jobs[k][5]
+        g.close()
+    out["plain_fp64_one_by_one_vertices_per_s"] = w_plain / t_plain if t_plain else None
+    out["plain_sample_jobs"] = len(sample)
+    log(f"serving plain fp64 sample: {len(sample)} jobs, {w_plain / max(t_plain, 1e-9) / 1e6:.1f} M vertices/s")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -92,6 +239,13 @@ def main():
     ap.add_argument("--cpu-n", type=int, default=80, help="cube size of the CPU sample")
     ap.add_argument("--cpu-passes", type=int, default=2, help="passes per smoother of the CPU sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batch-meshes", type=int, default=256)
+    ap.add_argument("--batch-passes", type=int, default=10)
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--serve-jobs", type=int, default=48)
+    ap.add_argument("--serve-rates", default="50,200,1000,2000")
+    ap.add_argument("--serve-plain-sample", type=int, default=8)
+    ap.add_argument("--no-serve", action="store_true")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -107,6 +261,8 @@ def main():
         sys.exit(1)
     smoothers = [s for s in args.smoothers.split(",") if s]
     spawn_coeff = 0.25
+    from gpumesh_amd._lib import check, lib
+    check(lib().gm_set_topology_cache(8 << 30), "gm_set_topology_cache")
 
     t0 = time.perf_counter()
     m = cube_mesh(args.n, "tet", 0.35, seed=2)
@@ -199,6 +355,20 @@ def main():
                         "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None, "kernel": best["class"],
                         "avg_launch_ms": avg_s * 1e3, "flop_per_launch": flop_per_launch}
 
+    # Final quality of the mixed-precision runs against FP64 runs of the same
+    # passes (the FP64 wave kernels are bit-identical to the plain FP64 ones).
+    quality = {}
+    for sm, p in per.items():
+        gm.set_positions(start)
+        r64 = gm.smooth(sm, args.steps, precision="fp64", kernel="wave", spawn_coeff=spawn_coeff, counts=False)
+        quality[sm] = {"fp64_final_min": float(r64["min"][-1]), "fp64_final_mean": float(r64["mean"][-1]),
+                       "min_diff": float(p["res"]["min"][-1] - r64["min"][-1]),
+                       "mean_diff": float(p["res"]["mean"][-1] - r64["mean"][-1])}
+        log(f"{sm}: final min/mean vs fp64: {quality[sm]['min_diff']:+.2e} / {quality[sm]['mean_diff']:+.2e}")
+
+    batched = None if args.no_batch else batched_leg(args)
+    serving = None if args.no_serve else serving_leg(args)
+
     cpu = None
     if not args.no_cpu:
         threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
@@ -212,7 +382,7 @@ def main():
 
     ms_per_step = total_wall / (args.steps * len(smoothers)) * 1e3
     out = {
-        "metric": "smoothed vertices/s on 1M-vertex tet mesh",
+        "metric": "smoothed vertices/s on 1M-vertex tet mesh; p50/p99 latency batched stream",
         "value": value,
         "unit": "vertices/s",
         "n_gpus": 1,
@@ -239,6 +409,9 @@ def main():
                          for sm, p in per.items()},
         "plain_fp64_vertices_per_s": plain_value,
         "speedup_vs_plain_fp64": (value / plain_value) if plain_value else None,
+        "quality_vs_fp64": quality,
+        "batched": batched,
+        "serving": serving,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -69,7 +69,9 @@ gmd::LaneShape defaultLane(const std::vector<gm_mesh*>& meshes, int smoother)
     int maxPatch = 0;
     for(const gm_mesh* m : meshes) maxPatch = std::max(maxPatch, m->maxPatch);
     gmd::LaneShape l{1, 64};
-    if(smoother == gmd::SM_GRADIENT_DESCENT)
+    if(smoother == gmd::SM_GETME)
+        l.patchesPerGroup = 4;
+    else if(smoother == gmd::SM_GRADIENT_DESCENT)
         l.tileElems = maxPatch <= 8 ? 64 : (maxPatch <= 16 ? 128 : 256);
     else
         l.tileElems = maxPatch <= 64 ? 64 : (maxPatch <= 128 ? 128 : 256);
@@ -508,6 +510,7 @@ int gm_batch_autotune(gm_batch* b, int smoother, int precision, int passes, int3
     int rc = checkRun(smoother, passes, precision, GM_KERNEL_WAVE, 1.0);
     if(rc) return rc;
     if(passes < 1) return fail(GM_ERR_ARG, "autotuning needs at least one pass");
+    if(smoother == gmd::SM_NELDER_MEAD) return fail(GM_ERR_ARG, "Nelder-Mead has no tunable launch shape");
     if((rc = checkBatchCrew(b))) return rc;
     GM_HIP(hipSetDevice(b->device));
     gmr::Plan* p = b->plan;
@@ -538,13 +541,16 @@ int gm_batch_autotune(gm_batch* b, int smoother, int precision, int passes, int3
         {
             const gmd::LaneShape shape{PPG[a], TILE[t]};
             // A shape is a candidate when the candidate-parallel kernel takes
-            // every single-kind list with it.
-            bool fits = smoother == gmd::SM_SPAWN_SEARCH || smoother == gmd::SM_GRADIENT_DESCENT;
+            // every single-kind list with it; GETMe has no LDS tile, so only
+            // its block size (patches per group) is swept.
+            const bool getme = smoother == gmd::SM_GETME;
+            bool fits = getme ? TILE[t] == 64 : true;
             for(const auto& col : p->colours)
                 for(const gmr::VList& l : col)
                 {
                     gmd::VertexWork w{p->dTable, l.segs, l.nsegs, l.count, l.kind, l.maxPatch};
-                    if(l.kind != gmd::EK_MIXED && !gmd::laneFits(w, smoother, shape, precision == GM_FP64))
+                    if(!getme && l.kind != gmd::EK_MIXED &&
+                       !gmd::laneFits(w, smoother, shape, precision == GM_FP64))
                         fits = false;
                 }
             double ms = -1.0;

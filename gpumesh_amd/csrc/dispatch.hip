@@ -43,10 +43,10 @@ hipError_t launchGetmeElements(const DevMesh* meshes, BlockWork tets, BlockWork 
 }
 
 template<typename T>
-hipError_t launchGetmeVertices(const VertexWork& w, const gmq::Crew& c, int evaluator, int sampler,
+hipError_t launchGetmeVertices(const VertexWork& w, const gmq::Crew& c, int evaluator, int sampler, LaneShape lane,
                                hipStream_t stream)
 {
-    GMD_SELECT(evaluator, sampler, T, getmeVertices(w, c, stream))
+    GMD_SELECT(evaluator, sampler, T, getmeVertices(w, c, lane, stream))
 }
 
 template<>
@@ -73,7 +73,9 @@ template hipError_t launchGetmeElements<double>(const DevMesh*, BlockWork, Block
                                                 hipStream_t);
 template hipError_t launchGetmeElements<float>(const DevMesh*, BlockWork, BlockWork, const gmq::Crew&, int, int,
                                                hipStream_t);
-template hipError_t launchGetmeVertices<double>(const VertexWork&, const gmq::Crew&, int, int, hipStream_t);
-template hipError_t launchGetmeVertices<float>(const VertexWork&, const gmq::Crew&, int, int, hipStream_t);
+template hipError_t launchGetmeVertices<double>(const VertexWork&, const gmq::Crew&, int, int, LaneShape,
+                                                hipStream_t);
+template hipError_t launchGetmeVertices<float>(const VertexWork&, const gmq::Crew&, int, int, LaneShape,
+                                               hipStream_t);
 
 } // namespace gmd

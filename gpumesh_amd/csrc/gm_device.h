@@ -78,8 +78,9 @@ enum { EK_TET = 0, EK_HEX = 1, EK_MIXED = 2 };
 // workgroup and the LDS tile (elements staged per workgroup).
 struct LaneShape
 {
-    int patchesPerGroup;   // 1, 2, 4, 8 wavefronts per block (spawn) / 8-vertex groups per block (GD)
-    int tileElems;         // LDS elements staged per wavefront-vertex slot budget: 64, 128, 256 per block
+    int patchesPerGroup;   // wavefronts per block: 1, 2, 4, 8 (spawn: 1 vertex each, GD: 8 vertices each,
+                           // GETMe vertex update: 64 vertices each)
+    int tileElems;         // LDS elements staged per wavefront: 64, 128, 256 (spawn, GD)
 };
 
 // Span (x, y, z, w) = (mesh index, first group node, first index of the
@@ -125,7 +126,9 @@ struct Launchers
                              LaneShape lane, double moveCoeff, hipStream_t stream);
     static hipError_t getmeElements(const DevMesh* meshes, BlockWork tets, BlockWork hexs, const gmq::Crew& c,
                                     hipStream_t stream);
-    static hipError_t getmeVertices(const VertexWork& w, const gmq::Crew& c, hipStream_t stream);
+    // GETMe vertex updates: one vertex per thread, 64 x lane.patchesPerGroup
+    // threads per block.
+    static hipError_t getmeVertices(const VertexWork& w, const gmq::Crew& c, LaneShape lane, hipStream_t stream);
 };
 
 // FP64 mesh-quality reduction of one (evaluator, sampler) combination, reading
@@ -149,7 +152,7 @@ template<typename T>
 hipError_t launchGetmeElements(const DevMesh* meshes, BlockWork tets, BlockWork hexs, const gmq::Crew& c,
                                int evaluator, int sampler, hipStream_t stream);
 template<typename T>
-hipError_t launchGetmeVertices(const VertexWork& w, const gmq::Crew& c, int evaluator, int sampler,
+hipError_t launchGetmeVertices(const VertexWork& w, const gmq::Crew& c, int evaluator, int sampler, LaneShape lane,
                                hipStream_t stream);
 // P = position precision read (double: pos64, float: pos32); always FP64 math.
 template<typename P>

@@ -883,11 +883,15 @@ hipError_t Launchers<T, EV, SA>::getmeElements(const DevMesh* meshes, BlockWork 
 }
 
 template<typename T, int EV, int SA>
-hipError_t Launchers<T, EV, SA>::getmeVertices(const VertexWork& w, const gmq::Crew& c, hipStream_t stream)
+hipError_t Launchers<T, EV, SA>::getmeVertices(const VertexWork& w, const gmq::Crew& c, LaneShape lane,
+                                               hipStream_t stream)
 {
     if(w.count == 0)
         return hipSuccess;
-    getmeVertexKernel<T, EV, SA><<<(w.count + 255) / 256, 256, 0, stream>>>(w.meshes, w.segs, w.nsegs, w.count, c);
+    const int ppg = lane.patchesPerGroup >= 1 && lane.patchesPerGroup <= 8 ? lane.patchesPerGroup : 4;
+    const int threads = 64 * ppg;
+    getmeVertexKernel<T, EV, SA><<<(w.count + threads - 1) / threads, threads, 0, stream>>>(w.meshes, w.segs,
+                                                                                            w.nsegs, w.count, c);
     return hipGetLastError();
 }
 

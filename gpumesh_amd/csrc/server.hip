@@ -311,7 +311,9 @@ int launchRound(gm_server* s, const gmd::LaneShape* lanes, std::vector<hipStream
         else
         {
             r.lane = gmd::LaneShape{1, 64};
-            if(smoother == gmd::SM_GRADIENT_DESCENT)
+            if(smoother == gmd::SM_GETME)
+                r.lane.patchesPerGroup = 4;
+            else if(smoother == gmd::SM_GRADIENT_DESCENT)
                 r.lane.tileElems = maxPatch <= 8 ? 64 : (maxPatch <= 16 ? 128 : 256);
             else
                 r.lane.tileElems = maxPatch <= 64 ? 64 : (maxPatch <= 128 ? 128 : 256);
@@ -522,6 +524,7 @@ int gm_server_submit(gm_server* s, int64_t nverts, const double* pos, const int3
     if(smoother < 0 || smoother > 3) return fail(GM_ERR_ARG, "unknown smoother");
     if(passes < 0) return fail(GM_ERR_ARG, "negative pass count");
     if(precision < 0 || precision > 2) return fail(GM_ERR_ARG, "unknown precision");
+    const Clock::time_point now = Clock::now();
     auto j = std::make_unique<Job>();
     j->nverts = nverts;
     j->ntets = ntets;
@@ -533,7 +536,7 @@ int gm_server_submit(gm_server* s, int64_t nverts, const double* pos, const int3
     j->smoother = smoother;
     j->passes = passes;
     j->precision = precision;
-    j->tSubmit = Clock::now();
+    j->tSubmit = now;
     std::lock_guard<std::mutex> lock(s->mu);
     if(s->stop) return fail(GM_ERR_ARG, "server is shutting down");
     j->id = s->nextId++;

@@ -182,13 +182,15 @@ typedef struct gm_batch gm_batch;
 
 int gm_batch_create(gm_mesh* const* meshes, int64_t n, gm_batch** out);
 int gm_batch_destroy(gm_batch* batch);
-/* Launch shape of `smoother`'s candidate-parallel kernel (spawn search,
- * gradient descent; the others ignore it): patches_per_group in {1,2,4,8}
- * (wavefronts per workgroup), tile_elems in {64,128,256} (LDS elements per
- * wavefront; gradient descent gives each of its 8 vertices per wavefront
- * tile_elems/8).  A work list whose largest patch does not fit the tile runs
- * on the wave kernel instead.  Until set (or autotuned), a batch uses the
- * smallest tile that holds its largest patch, one patch group per workgroup. */
+/* Launch shape of `smoother`: patches_per_group in {1,2,4,8} wavefronts per
+ * workgroup (spawn search: one vertex per wavefront; gradient descent: 8;
+ * GETMe vertex update: 64), tile_elems in {64,128,256} LDS elements per
+ * wavefront for the candidate-parallel spawn-search and gradient-descent
+ * kernels (gradient descent gives each of its 8 vertices tile_elems/8); a
+ * work list whose largest patch does not fit the tile runs on the wave kernel
+ * instead.  Nelder-Mead ignores it.  Until set (or autotuned) a batch uses
+ * one wavefront per workgroup with the smallest tile that holds its largest
+ * patch (GETMe: 4 wavefronts). */
 int gm_batch_set_shape(gm_batch* batch, int smoother, int patches_per_group, int tile_elems);
 /* gm_mesh_smooth over every member.  use_graph != 0 captures one pass body
  * (all sweeps + the quality reduction) in a HIP graph on first use and
@@ -200,10 +202,12 @@ int gm_batch_smooth(gm_batch* batch, int smoother, int passes, int precision, in
 int gm_batch_last_timing(const gm_batch* batch, double* out2);
 /* Per-class device time of the last gm_batch_smooth (not recorded with graphs). */
 int gm_batch_kernel_times(const gm_batch* batch, double* ms, int64_t* launches);
-/* Times `passes` sweeps of `smoother` for every launch shape (4 x 3 grid
- * patches_per_group x tile_elems, times[12] = summed sweep kernel ms, -1
- * where a shape does not fit every list), restores the positions after each
- * trial, keeps the fastest shape for `smoother` and returns it in best[2]. */
+/* Times `passes` sweeps of `smoother` (spawn search, gradient descent or
+ * GETMe) for every launch shape of the 4 x 3 grid patches_per_group x
+ * tile_elems (times[12], summed sweep kernel ms; -1 where a shape does not
+ * fit every list, and for GETMe where tile_elems != 64, which it does not
+ * use), restores the positions after each trial, keeps the fastest shape for
+ * `smoother` and returns it in best[2]. */
 int gm_batch_autotune(gm_batch* batch, int smoother, int precision, int passes, int32_t* best, double* times);
 
 /* ---- Serving: a request loop over resident meshes.
