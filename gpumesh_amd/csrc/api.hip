@@ -550,7 +550,8 @@ int gm_batch_autotune(gm_batch* b, int smoother, int precision, int passes, int3
                 {
                     gmd::VertexWork w{p->dTable, l.segs, l.nsegs, l.count, l.kind, l.maxPatch};
                     if(!getme && l.kind != gmd::EK_MIXED &&
-                       !gmd::laneFits(w, smoother, shape, precision == GM_FP64))
+                       !gmd::laneFits(w, smoother, shape, precision == GM_FP64, p->meshes[0]->evaluator,
+                                      p->meshes[0]->sampler))
                         fits = false;
                 }
             double ms = -1.0;

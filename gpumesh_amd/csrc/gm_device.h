@@ -170,7 +170,7 @@ hipError_t launchResetRun(const DevMesh* meshes, BlockWork vertexBlocks, const u
 
 // True when the candidate-parallel kernel of `smoother` runs `w` with `lane`
 // (otherwise the wave kernel does).
-bool laneFits(const VertexWork& w, int smoother, LaneShape lane, bool fp64);
+bool laneFits(const VertexWork& w, int smoother, LaneShape lane, bool fp64, int evaluator, int sampler);
 
 // Math self-test: op 0 sqrt, 1 div (a/b), 2 pow23, 3 cbrt, 4 exp; n values.
 hipError_t launchMathProbe(int op, const double* a, const double* b, double* out, int n, hipStream_t stream);

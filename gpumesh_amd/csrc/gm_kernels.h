@@ -1,10 +1,12 @@
 // gm_kernels.h -- kernel bodies of the smoothing path, instantiated once per
-// precision by kernels_f64.hip (built without contraction: bit-comparable to
-// an FP64 CPU build of the same operation order) and kernels_f32.hip.
+// (precision, evaluator, sampler) by kernels_inst.hip: FP64 objects are built
+// without contraction (bit-comparable to an FP64 CPU build of the same
+// operation order), FP32 objects with it.
 #ifndef GM_KERNELS_H
 #define GM_KERNELS_H
 
 #include "gm_device.h"
+#include "gm_affine.h"
 #include "gm_smoothers.h"
 
 #include <type_traits>
@@ -480,6 +482,166 @@ __global__ void __launch_bounds__(512) gdLaneKernel(const DevMesh* __restrict__ 
 }
 
 // ---------------------------------------------------------------------------
+// FP32 mean ratio with the metric-free measurer: candidate-parallel kernels on
+// affine corner terms (gm_affine.h) staged relative to the vertex's position.
+// Same search as spawnLaneKernel / gdLaneKernel; trial points are offsets x
+// from the start position o, and the vertex moves to o + x.
+// ---------------------------------------------------------------------------
+template<typename T, int EV, int SA> struct Affine { static const bool on = false; };
+template<> struct Affine<float, gmq::EV_MEAN_RATIO, gmq::SA_UNIFORM> { static const bool on = true; };
+
+template<int EK> __host__ __device__ inline size_t affineBytes()
+{
+    return EK == 0 ? sizeof(gma::Corner) : sizeof(gma::Hex);
+}
+
+// LDS bytes of `elems` staged elements of kind EK in the (T, EV, SA) kernels.
+template<typename T, int EV, int SA, int EK> __host__ __device__ inline size_t laneStageBytes(int elems)
+{
+    return Affine<T, EV, SA>::on ? (size_t)elems * affineBytes<EK>() : stageBytes<T, EK>(elems);
+}
+
+template<int EK>
+__device__ __forceinline__ void stageAffine(const DevMesh& m, const float4* pos, uint32_t ref, v3<float> o,
+                                            unsigned char* dst)
+{
+    const uint32_t id = ref >> 4;
+    const int k = (int)(ref & 7);
+    if(EK == 0)
+    {
+        const uint4 t = m.tets[id];
+        v3<float> u[4] = {ld<float>(pos, t.x) - o, ld<float>(pos, t.y) - o, ld<float>(pos, t.z) - o,
+                          ld<float>(pos, t.w) - o};
+        *(gma::Corner*)dst = gma::tetCorner(u, k);
+    }
+    else
+    {
+        const uint4 a = m.hexs[2 * id], b = m.hexs[2 * id + 1];
+        v3<float> u[8] = {ld<float>(pos, a.x) - o, ld<float>(pos, a.y) - o, ld<float>(pos, a.z) - o,
+                          ld<float>(pos, a.w) - o, ld<float>(pos, b.x) - o, ld<float>(pos, b.y) - o,
+                          ld<float>(pos, b.z) - o, ld<float>(pos, b.w) - o};
+        *(gma::Hex*)dst = gma::stageHex(u, k);
+    }
+}
+
+template<int EK>
+__device__ __forceinline__ float affinePatch(const unsigned char* sp, int nE, v3<float> x)
+{
+    const float xx = gmq::dot(x, x);
+    gma::Acc a;
+    for(int e = 0; e < nE; ++e)
+    {
+        if(EK == 0) gma::addTet(a, ((const gma::Corner*)sp)[e], x, xx);
+        else gma::addHex(a, ((const gma::Hex*)sp)[e], x, xx);
+    }
+    return a.finish();
+}
+
+template<int EK>
+__global__ void __launch_bounds__(512) spawnAffineKernel(const DevMesh* __restrict__ meshes,
+                                                         const uint4* __restrict__ segs, uint32_t nsegs,
+                                                         uint32_t count, float moveCoeff, int cap)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int ppg = blockDim.x >> 6;
+    unsigned char* sp = smem + (size_t)wave * cap * affineBytes<EK>();
+    const uint32_t i = blockIdx.x * ppg + wave;
+    const bool present = i < count;
+    const uint2 it = present ? workItem(meshes, segs, nsegs, i) : make_uint2(0, 0);
+    const DevMesh& m = meshes[it.x];
+    float4* pos = m.pos32;
+    const uint32_t v = it.y;
+    const uint32_t eb = m.eoff[v];
+    const int nE = present ? (int)(m.eoff[v + 1] - eb) : 0;
+    const v3<float> o = present ? ld<float>(pos, v) : mk<float>(0.0f, 0.0f, 0.0f);
+    for(int e = lane; e < nE; e += 64)
+        stageAffine<EK>(m, pos, m.eref[eb + e], o, sp + (size_t)e * affineBytes<EK>());
+    __syncthreads();
+    if(!present)
+        return;
+    float scale = moveCoeff * localSize<float>(m, pos, v, o);
+    v3<float> xb = mk<float>(0.0f, 0.0f, 0.0f);
+    for(int iter = 0; iter < 2; ++iter)
+    {
+        const v3<float> x = xb + gms::spawnOffset<float>(lane) * scale;
+        const float q = affinePatch<EK>(sp, nE, x);
+        const int best = groupArgmax<float>(q, lane, 64);
+        xb = shflv<float>(x, best);
+        scale /= 3.0f;
+    }
+    if(lane == 0)
+    {
+        st(pos, v, o + xb);
+        m.evalCounts[v] += 128;
+    }
+}
+
+template<int EK>
+__global__ void __launch_bounds__(512) gdAffineKernel(const DevMesh* __restrict__ meshes,
+                                                      const uint4* __restrict__ segs, uint32_t nsegs,
+                                                      uint32_t count, int cap)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int lane = threadIdx.x;
+    const int g = lane >> 3, s = lane & 7, base = lane & ~7;
+    unsigned char* mySp = smem + (size_t)g * cap * affineBytes<EK>();
+    const uint32_t i = blockIdx.x * (blockDim.x >> 3) + g;
+    const bool present = i < count;
+    const uint2 it = present ? workItem(meshes, segs, nsegs, i) : make_uint2(0, 0);
+    const DevMesh& m = meshes[it.x];
+    float4* pos = m.pos32;
+    const uint32_t v = it.y;
+    const uint32_t eb = m.eoff[v];
+    const int nE = present ? (int)(m.eoff[v + 1] - eb) : 0;
+    const v3<float> o = present ? ld<float>(pos, v) : mk<float>(0.0f, 0.0f, 0.0f);
+    for(int e = s; e < nE; e += 8)
+        stageAffine<EK>(m, pos, m.eref[eb + e], o, mySp + (size_t)e * affineBytes<EK>());
+    __syncthreads();
+    if(!present)
+        return;
+
+    v3<float> x = mk<float>(0.0f, 0.0f, 0.0f);
+    float nodeShift = localSize<float>(m, pos, v, o) * (float)(1.0 / 75.0);
+    const float originalNodeShift = nodeShift;
+    int evals = 0;
+    for(int cyc = 0; cyc < 5; ++cyc)
+    {
+        const float sx = s == 0 ? -nodeShift : (s == 1 ? nodeShift : 0.0f);
+        const float sy = s == 2 ? -nodeShift : (s == 3 ? nodeShift : 0.0f);
+        const float sz = s == 4 ? -nodeShift : (s == 5 ? nodeShift : 0.0f);
+        float q = 0.0f;
+        if(s < 6)
+            q = affinePatch<EK>(mySp, nE, x + mk<float>(sx, sy, sz));
+        evals += 6;
+        const float q0 = __shfl(q, base + 0, 64), q1 = __shfl(q, base + 1, 64), q2 = __shfl(q, base + 2, 64);
+        const float q3 = __shfl(q, base + 3, 64), q4 = __shfl(q, base + 4, 64), q5 = __shfl(q, base + 5, 64);
+        const v3<float> gq = mk<float>(q1 - q0, q3 - q2, q5 - q4);
+        const float gn = gmq::tsqrt(gmq::dot(gq, gq));
+        if(gn == 0.0f)
+            break;
+        const v3<float> shift = gq * (nodeShift / gn);
+        const float off = s == 0 ? -0.25f : s == 1 ? 0.00f : s == 2 ? 0.10f : s == 3 ? 0.20f
+                        : s == 4 ? 0.40f : s == 5 ? 0.80f : s == 6 ? 1.20f : 1.60f;
+        const v3<float> prop = x + shift * off;
+        const float qp = affinePatch<EK>(mySp, nE, prop);
+        evals += 8;
+        const int best = groupArgmax<float>(qp, s, 8);
+        x = shflv<float>(prop, (base & 63) + best);
+        const float bestOff = __shfl(off, (base & 63) + best, 64);
+        nodeShift *= gmq::tabs(bestOff);
+        if(nodeShift < originalNodeShift / 10.0f)
+            break;
+    }
+    if(s == 0)
+    {
+        st(pos, v, o + x);
+        m.evalCounts[v] += evals;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // GETMe element step (Smoothers/ElementWise/GetmeSmoother.cpp:55-127, 245-374):
 // every element's transformed positions and weight against the pass-start
 // positions; out[n_vertices_per_element * id + k] = (vpp_k, weight).
@@ -797,27 +959,37 @@ static hipError_t launchLane(const VertexWork& w, const gmq::Crew& c, LaneShape 
                              hipStream_t stream)
 {
     const int ppg = lane.patchesPerGroup;
+    const bool aff = Affine<T, EV, SA>::on;
     if(SM == SM_SPAWN_SEARCH)
     {
         // LDS tile per block = ppg wavefronts x tileElems elements.
         const int cap = lane.tileElems;
-        const size_t lds = (size_t)ppg * stageBytes<T, EK>(cap);
-        spawnLaneKernel<T, EV, SA, EK><<<(w.count + ppg - 1) / ppg, 64 * ppg, lds, stream>>>(
-            w.meshes, w.segs, w.nsegs, w.count, c, (T)moveCoeff, cap);
+        const size_t lds = (size_t)ppg * laneStageBytes<T, EV, SA, EK>(cap);
+        const uint32_t blocks = (w.count + ppg - 1) / ppg;
+        if(aff)
+            spawnAffineKernel<EK><<<blocks, 64 * ppg, lds, stream>>>(w.meshes, w.segs, w.nsegs, w.count,
+                                                                     (float)moveCoeff, cap);
+        else
+            spawnLaneKernel<T, EV, SA, EK><<<blocks, 64 * ppg, lds, stream>>>(w.meshes, w.segs, w.nsegs, w.count, c,
+                                                                              (T)moveCoeff, cap);
     }
     else
     {
         // 8 vertices per wavefront share the wavefront's tile.
         const int cap = lane.tileElems / 8;
-        const size_t lds = (size_t)8 * ppg * stageBytes<T, EK>(cap);
-        gdLaneKernel<T, EV, SA, EK><<<(w.count + 8 * ppg - 1) / (8 * ppg), 64 * ppg, lds, stream>>>(
-            w.meshes, w.segs, w.nsegs, w.count, c, cap);
+        const size_t lds = (size_t)8 * ppg * laneStageBytes<T, EV, SA, EK>(cap);
+        const uint32_t blocks = (w.count + 8 * ppg - 1) / (8 * ppg);
+        if(aff)
+            gdAffineKernel<EK><<<blocks, 64 * ppg, lds, stream>>>(w.meshes, w.segs, w.nsegs, w.count, cap);
+        else
+            gdLaneKernel<T, EV, SA, EK><<<blocks, 64 * ppg, lds, stream>>>(w.meshes, w.segs, w.nsegs, w.count, c,
+                                                                           cap);
     }
     return hipGetLastError();
 }
 
 // True when the candidate-parallel kernel of `smoother` can run `w` with `lane`.
-template<typename T>
+template<typename T, int EV, int SA>
 inline bool laneFits(const VertexWork& w, int smoother, LaneShape lane)
 {
     if(!(smoother == SM_SPAWN_SEARCH || smoother == SM_GRADIENT_DESCENT) || w.kind == EK_MIXED)
@@ -826,7 +998,8 @@ inline bool laneFits(const VertexWork& w, int smoother, LaneShape lane)
     const int cap = smoother == SM_SPAWN_SEARCH ? lane.tileElems : lane.tileElems / 8;
     if(cap < w.maxPatch || ppg < 1 || ppg > 8)
         return false;
-    const size_t per = w.kind == EK_TET ? stageBytes<T, EK_TET>(cap) : stageBytes<T, EK_HEX>(cap);
+    const size_t per = w.kind == EK_TET ? laneStageBytes<T, EV, SA, EK_TET>(cap)
+                                        : laneStageBytes<T, EV, SA, EK_HEX>(cap);
     const size_t lds = (smoother == SM_SPAWN_SEARCH ? (size_t)ppg : (size_t)8 * ppg) * per;
     return lds <= 64 * 1024;
 }
@@ -837,7 +1010,7 @@ static hipError_t launchSmoothTyped(const VertexWork& w, const gmq::Crew& c, int
 {
     if(w.count == 0)
         return hipSuccess;
-    if(shape == KS_WAVE && laneFits<T>(w, SM, lane))
+    if(shape == KS_WAVE && laneFits<T, EV, SA>(w, SM, lane))
     {
         if(w.kind == EK_TET) return launchLane<T, EV, SA, SM, EK_TET>(w, c, lane, moveCoeff, stream);
         return launchLane<T, EV, SA, SM, EK_HEX>(w, c, lane, moveCoeff, stream);

@@ -68,9 +68,12 @@ hipError_t launchResetRun(const DevMesh* meshes, BlockWork vb, const uint32_t* m
     return hipGetLastError();
 }
 
-bool laneFits(const VertexWork& w, int smoother, LaneShape lane, bool fp64)
+bool laneFits(const VertexWork& w, int smoother, LaneShape lane, bool fp64, int evaluator, int sampler)
 {
-    return fp64 ? laneFits<double>(w, smoother, lane) : laneFits<float>(w, smoother, lane);
+    if(fp64) return laneFits<double, 0, 0>(w, smoother, lane);
+    if(evaluator == gmq::EV_MEAN_RATIO && sampler == gmq::SA_UNIFORM)
+        return laneFits<float, gmq::EV_MEAN_RATIO, gmq::SA_UNIFORM>(w, smoother, lane);
+    return laneFits<float, 1, 1>(w, smoother, lane);
 }
 
 __global__ void mathProbeKernel(int op, const double* a, const double* b, double* out, int n)
