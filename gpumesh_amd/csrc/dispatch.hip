@@ -43,10 +43,10 @@ hipError_t launchGetmeElements(const DevMesh* meshes, BlockWork tets, BlockWork 
 }
 
 template<typename T>
-hipError_t launchGetmeVertices(const VertexWork& w, const gmq::Crew& c, int evaluator, int sampler, LaneShape lane,
-                               hipStream_t stream)
+hipError_t launchGetmeVertices(const VertexWork& w, const gmq::Crew& c, int evaluator, int sampler, int shape,
+                               LaneShape lane, hipStream_t stream)
 {
-    GMD_SELECT(evaluator, sampler, T, getmeVertices(w, c, lane, stream))
+    GMD_SELECT(evaluator, sampler, T, getmeVertices(w, c, shape, lane, stream))
 }
 
 template<>
@@ -73,9 +73,9 @@ template hipError_t launchGetmeElements<double>(const DevMesh*, BlockWork, Block
                                                 hipStream_t);
 template hipError_t launchGetmeElements<float>(const DevMesh*, BlockWork, BlockWork, const gmq::Crew&, int, int,
                                                hipStream_t);
-template hipError_t launchGetmeVertices<double>(const VertexWork&, const gmq::Crew&, int, int, LaneShape,
+template hipError_t launchGetmeVertices<double>(const VertexWork&, const gmq::Crew&, int, int, int, LaneShape,
                                                 hipStream_t);
-template hipError_t launchGetmeVertices<float>(const VertexWork&, const gmq::Crew&, int, int, LaneShape,
+template hipError_t launchGetmeVertices<float>(const VertexWork&, const gmq::Crew&, int, int, int, LaneShape,
                                                hipStream_t);
 
 } // namespace gmd

@@ -126,9 +126,11 @@ struct Launchers
                              LaneShape lane, double moveCoeff, hipStream_t stream);
     static hipError_t getmeElements(const DevMesh* meshes, BlockWork tets, BlockWork hexs, const gmq::Crew& c,
                                     hipStream_t stream);
-    // GETMe vertex updates: one vertex per thread, 64 x lane.patchesPerGroup
-    // threads per block.
-    static hipError_t getmeVertices(const VertexWork& w, const gmq::Crew& c, LaneShape lane, hipStream_t stream);
+    // GETMe vertex updates: KS_WAVE, one vertex per group of lanes (patches
+    // of up to 64 elements); otherwise one vertex per thread, 64 x
+    // lane.patchesPerGroup threads per block.
+    static hipError_t getmeVertices(const VertexWork& w, const gmq::Crew& c, int shape, LaneShape lane,
+                                    hipStream_t stream);
 };
 
 // FP64 mesh-quality reduction of one (evaluator, sampler) combination, reading
@@ -152,8 +154,8 @@ template<typename T>
 hipError_t launchGetmeElements(const DevMesh* meshes, BlockWork tets, BlockWork hexs, const gmq::Crew& c,
                                int evaluator, int sampler, hipStream_t stream);
 template<typename T>
-hipError_t launchGetmeVertices(const VertexWork& w, const gmq::Crew& c, int evaluator, int sampler, LaneShape lane,
-                               hipStream_t stream);
+hipError_t launchGetmeVertices(const VertexWork& w, const gmq::Crew& c, int evaluator, int sampler, int shape,
+                               LaneShape lane, hipStream_t stream);
 // P = position precision read (double: pos64, float: pos32); always FP64 math.
 template<typename P>
 hipError_t launchEvaluate(const DevMesh* meshes, BlockWork blocks, const uint32_t* meshList, uint32_t nmeshes,

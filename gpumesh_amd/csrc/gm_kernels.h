@@ -150,9 +150,10 @@ __global__ void __launch_bounds__(256) smoothPlainKernel(const DevMesh* __restri
 // Wavefront kernel: one vertex per group of G lanes (G a power of two >= the
 // patch size), lane j of the group holding patch element j's other vertices
 // in registers.  A patch evaluation computes the G element qualities in
-// parallel and reduces them in patch order through LDS, so every lane of the
-// group obtains the bit-identical sequential result and runs the optimiser
-// in lock-step.  64 / G vertices share a wavefront; a block is one wavefront.
+// parallel and, in FP64, reduces them in patch order through LDS, so every
+// lane of the group obtains the bit-identical sequential result and runs the
+// optimiser in lock-step (FP32 reduces over a shuffle butterfly instead).
+// 64 / G vertices share a wavefront; a block is one wavefront.
 // ---------------------------------------------------------------------------
 template<typename T, int EV, int SA, int G>
 struct WavePatch
@@ -186,6 +187,27 @@ struct WavePatch
                 for(int i = 0; i < 8; ++i) e[i] = (i == k) ? p : vp[i];
                 q = gmq::hexQuality<T, EV, SA>(*c, e);
             }
+        }
+        if constexpr(std::is_same<T, float>::value)
+        {
+            // FP32: the same patch rule over a butterfly of the group's
+            // lanes (tree order; deterministic, not the sequential sum).
+            const bool valid = lane < nElem;
+            const bool pos = valid && q > 0.0f;
+            float sum = pos ? __builtin_amdgcn_rcpf(q) : 0.0f;
+            float cnt = pos ? 1.0f : 0.0f;
+            int bad = (valid && !pos) ? 1 : 0;
+            float mn = (bad && q == q) ? q : __builtin_huge_valf();
+#pragma unroll
+            for(int off = G / 2; off > 0; off >>= 1)
+            {
+                sum += __shfl_xor(sum, off, 64);
+                cnt += __shfl_xor(cnt, off, 64);
+                bad |= __shfl_xor(bad, off, 64);
+                const float om = __shfl_xor(mn, off, 64);
+                mn = om < mn ? om : mn;
+            }
+            return bad ? mn : cnt / sum;
         }
         T inv = T(1) / q;
         // LDS accesses of one wavefront complete in issue order, so the reads
@@ -790,6 +812,95 @@ __global__ void __launch_bounds__(512) getmeVertexKernel(const DevMesh* __restri
     }
 }
 
+// GETMe vertex step, one vertex per group of G lanes (G >= patch size): lane j
+// pulls element j's contribution and holds its vertices; the weighted sum is
+// formed in patch order through LDS (bit-identical to getmeVertexKernel) and
+// the two patch evaluations run lane-parallel as in smoothWaveKernel.
+template<typename T, int EV, int SA, int G>
+__global__ void __launch_bounds__(64) getmeWaveKernel(const DevMesh* __restrict__ meshes,
+                                                      const uint4* __restrict__ segs, uint32_t nsegs,
+                                                      uint32_t count, gmq::Crew c)
+{
+    typedef typename PosOf<T>::type P;
+    __shared__ T red[2 * 64];
+    __shared__ T pull[4 * 64];
+    const int lane = threadIdx.x % G;
+    const int grp = threadIdx.x / G;
+    const uint32_t i = blockIdx.x * (64 / G) + grp;
+    if(i >= count)
+        return;
+    const uint2 it = workItem(meshes, segs, nsegs, i);
+    const DevMesh& m = meshes[it.x];
+    P* pos = PosOf<T>::get(m);
+    const P* tetOut = (const P*)m.getmeTet;
+    const P* hexOut = (const P*)m.getmeHex;
+    const uint32_t v = it.y;
+    WavePatch<T, EV, SA, G> f;
+    f.c = &c;
+    f.red = red + 2 * G * grp;
+    f.lane = lane;
+    f.count = 0;
+    const uint32_t eb = m.eoff[v], ee = m.eoff[v + 1];
+    f.nElem = (int)(ee - eb);
+    f.isHex = 0;
+    f.k = 0;
+    T* pl = pull + 4 * G * grp;
+    if(lane < f.nElem)
+    {
+        const uint32_t ref = m.eref[eb + lane];
+        const uint32_t id = ref >> 4;
+        f.k = (int)(ref & 7);
+        f.isHex = (ref >> 3) & 1;
+        P o;
+        if(!f.isHex)
+        {
+            uint4 t = m.tets[id];
+            f.vp[0] = ld<T>(pos, t.x); f.vp[1] = ld<T>(pos, t.y);
+            f.vp[2] = ld<T>(pos, t.z); f.vp[3] = ld<T>(pos, t.w);
+            o = tetOut[4 * (size_t)id + f.k];
+        }
+        else
+        {
+            uint4 a = m.hexs[2 * id], b = m.hexs[2 * id + 1];
+            f.vp[0] = ld<T>(pos, a.x); f.vp[1] = ld<T>(pos, a.y);
+            f.vp[2] = ld<T>(pos, a.z); f.vp[3] = ld<T>(pos, a.w);
+            f.vp[4] = ld<T>(pos, b.x); f.vp[5] = ld<T>(pos, b.y);
+            f.vp[6] = ld<T>(pos, b.z); f.vp[7] = ld<T>(pos, b.w);
+            o = hexOut[8 * (size_t)id + f.k];
+        }
+        const T w = (T)o.w;
+        const v3<T> pw = mk<T>((T)o.x, (T)o.y, (T)o.z) * w;
+        pl[4 * lane] = pw.x;
+        pl[4 * lane + 1] = pw.y;
+        pl[4 * lane + 2] = pw.z;
+        pl[4 * lane + 3] = w;
+    }
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_wave_barrier();
+    v3<T> acc = mk<T>(T(0), T(0), T(0));
+    T wacc = T(0);
+    for(int j = 0; j < f.nElem; ++j)
+    {
+        acc = acc + mk<T>(pl[4 * j], pl[4 * j + 1], pl[4 * j + 2]);
+        wacc += pl[4 * j + 3];
+    }
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_wave_barrier();
+    if(wacc != T(0))
+    {
+        const v3<T> p0 = ld<T>(pos, v);
+        const v3<T> p1 = acc / wacc;
+        const T q0 = f(p0);
+        const T q1 = f(p1);
+        if(lane == 0)
+        {
+            if(!(q1 < q0))
+                st(pos, v, p1);
+            m.evalCounts[v] += f.count;
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Mesh quality (AbstractEvaluator::evaluateMeshQuality*, AbstractEvaluator.cpp:384-453;
 // QualityHistogram::add, DataStructures/QualityHistogram.cpp:63-92) in FP64.
@@ -1056,11 +1167,23 @@ hipError_t Launchers<T, EV, SA>::getmeElements(const DevMesh* meshes, BlockWork 
 }
 
 template<typename T, int EV, int SA>
-hipError_t Launchers<T, EV, SA>::getmeVertices(const VertexWork& w, const gmq::Crew& c, LaneShape lane,
+hipError_t Launchers<T, EV, SA>::getmeVertices(const VertexWork& w, const gmq::Crew& c, int shape, LaneShape lane,
                                                hipStream_t stream)
 {
     if(w.count == 0)
         return hipSuccess;
+    if(shape == KS_WAVE && w.maxPatch <= 64)
+    {
+        if(w.maxPatch <= 8)
+            getmeWaveKernel<T, EV, SA, 8><<<(w.count + 7) / 8, 64, 0, stream>>>(w.meshes, w.segs, w.nsegs, w.count, c);
+        else if(w.maxPatch <= 16)
+            getmeWaveKernel<T, EV, SA, 16><<<(w.count + 3) / 4, 64, 0, stream>>>(w.meshes, w.segs, w.nsegs, w.count, c);
+        else if(w.maxPatch <= 32)
+            getmeWaveKernel<T, EV, SA, 32><<<(w.count + 1) / 2, 64, 0, stream>>>(w.meshes, w.segs, w.nsegs, w.count, c);
+        else
+            getmeWaveKernel<T, EV, SA, 64><<<w.count, 64, 0, stream>>>(w.meshes, w.segs, w.nsegs, w.count, c);
+        return hipGetLastError();
+    }
     const int ppg = lane.patchesPerGroup >= 1 && lane.patchesPerGroup <= 8 ? lane.patchesPerGroup : 4;
     const int threads = 64 * ppg;
     getmeVertexKernel<T, EV, SA><<<(w.count + threads - 1) / threads, threads, 0, stream>>>(w.meshes, w.segs,

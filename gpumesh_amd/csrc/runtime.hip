@@ -485,8 +485,8 @@ static hipError_t sweep(Plan* p, const gmq::Crew& c, int ev, int sa, const RunPa
                             : (f64 ? GM_KT_SMOOTH_F64 : GM_KT_SMOOTH_F32);
             int slot = timedBegin(p, timed, tag);
             if(getme)
-                e = f64 ? gmd::launchGetmeVertices<double>(w, c, ev, sa, r.lane, p->stream)
-                        : gmd::launchGetmeVertices<float>(w, c, ev, sa, r.lane, p->stream);
+                e = f64 ? gmd::launchGetmeVertices<double>(w, c, ev, sa, r.kernel, r.lane, p->stream)
+                        : gmd::launchGetmeVertices<float>(w, c, ev, sa, r.kernel, r.lane, p->stream);
             else
                 e = f64 ? gmd::launchSmooth<double>(w, c, ev, sa, r.smoother, r.kernel, r.lane, r.spawnCoeff,
                                                     p->stream)

@@ -49,8 +49,6 @@ def test_element_qualities_bitexact(oracle, kind):
 @pytest.mark.parametrize("kernel", ["plain", "wave"])
 @pytest.mark.parametrize("kind", ["tet", "hex"])
 def test_smoothing_fp64_bitexact(oracle, smoother, kernel, kind):
-    if smoother == "getme" and kernel == "wave":
-        pytest.skip("GETMe has a single kernel shape")
     n = 10 if kind == "tet" else 12
     passes = 3
     m, om, gm = _pair(oracle, n, kind, 0.3, 5)
@@ -88,7 +86,7 @@ def test_anisotropic_element_qualities_bitexact(oracle, evaluator, sampler):
 
 
 @pytest.mark.parametrize("smoother,kernel", [("nm", "wave"), ("nm", "plain"), ("gd", "wave"), ("spawn", "wave"),
-                                             ("getme", "plain")])
+                                             ("getme", "plain"), ("getme", "wave")])
 @pytest.mark.parametrize("sampler", ["analytic", "texture"])
 def test_anisotropic_smoothing_fp64_bitexact(oracle, smoother, kernel, sampler):
     """The anisotropic hex workload's crew (metric conformity, metric-wise
