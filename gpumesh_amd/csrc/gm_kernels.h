@@ -663,6 +663,110 @@ __global__ void __launch_bounds__(512) gdAffineKernel(const DevMesh* __restrict_
     }
 }
 
+// Nelder-Mead on affine terms: one vertex per group of G lanes, lane j holding
+// element j's staged corner terms in registers; a patch evaluation is one
+// affine element evaluation per lane and a butterfly over the group.
+template<int EK, int G>
+struct AffineWavePatch
+{
+    int lane;
+    int nElem;
+    int count;
+    v3<float> o;
+    gma::Corner cn;   // EK == 0
+    gma::Hex hx;      // EK == 1
+
+    __device__ float operator()(v3<float> p)
+    {
+        ++count;
+        const v3<float> x = p - o;
+        const float xx = gmq::dot(x, x);
+        gma::Acc a;
+        if(lane < nElem)
+        {
+            if(EK == 0) gma::addTet(a, cn, x, xx);
+            else gma::addHex(a, hx, x, xx);
+        }
+        float sum = a.inv3;
+        float cnt = (float)a.n;
+        int bad = a.bad ? 1 : 0;
+        float mn = (a.bad && a.mn == a.mn) ? a.mn : __builtin_huge_valf();
+#pragma unroll
+        for(int off = G / 2; off > 0; off >>= 1)
+        {
+            sum += __shfl_xor(sum, off, 64);
+            cnt += __shfl_xor(cnt, off, 64);
+            bad |= __shfl_xor(bad, off, 64);
+            const float om = __shfl_xor(mn, off, 64);
+            mn = om < mn ? om : mn;
+        }
+        return bad ? mn : 3.0f * cnt / sum;
+    }
+};
+
+template<int EK, int G>
+__global__ void __launch_bounds__(64) nmAffineWaveKernel(const DevMesh* __restrict__ meshes,
+                                                         const uint4* __restrict__ segs, uint32_t nsegs,
+                                                         uint32_t count)
+{
+    const int lane = threadIdx.x % G;
+    const int grp = threadIdx.x / G;
+    const uint32_t i = blockIdx.x * (64 / G) + grp;
+    if(i >= count)
+        return;
+    const uint2 it = workItem(meshes, segs, nsegs, i);
+    const DevMesh& m = meshes[it.x];
+    float4* pos = m.pos32;
+    const uint32_t v = it.y;
+    const uint32_t eb = m.eoff[v], ee = m.eoff[v + 1];
+    AffineWavePatch<EK, G> f;
+    f.lane = lane;
+    f.nElem = (int)(ee - eb);
+    f.count = 0;
+    f.o = ld<float>(pos, v);
+    if(lane < f.nElem)
+    {
+        const uint32_t ref = m.eref[eb + lane];
+        const uint32_t id = ref >> 4;
+        const int k = (int)(ref & 7);
+        if(EK == 0)
+        {
+            const uint4 t = m.tets[id];
+            v3<float> u[4] = {ld<float>(pos, t.x) - f.o, ld<float>(pos, t.y) - f.o, ld<float>(pos, t.z) - f.o,
+                              ld<float>(pos, t.w) - f.o};
+            f.cn = gma::tetCorner(u, k);
+        }
+        else
+        {
+            const uint4 a = m.hexs[2 * id], b = m.hexs[2 * id + 1];
+            v3<float> u[8] = {ld<float>(pos, a.x) - f.o, ld<float>(pos, a.y) - f.o, ld<float>(pos, a.z) - f.o,
+                              ld<float>(pos, a.w) - f.o, ld<float>(pos, b.x) - f.o, ld<float>(pos, b.y) - f.o,
+                              ld<float>(pos, b.z) - f.o, ld<float>(pos, b.w) - f.o};
+            f.hx = gma::stageHex(u, k);
+        }
+    }
+    const float ls = localSize<float>(m, pos, v, f.o);
+    const v3<float> np = gms::nelderMead<float>(f.o, ls, f);
+    if(lane == 0)
+    {
+        st(pos, v, np);
+        m.evalCounts[v] += f.count;
+    }
+}
+
+template<int EK>
+static void launchNmAffine(const VertexWork& w, hipStream_t stream)
+{
+    if(w.maxPatch <= 8)
+        nmAffineWaveKernel<EK, 8><<<(w.count + 7) / 8, 64, 0, stream>>>(w.meshes, w.segs, w.nsegs, w.count);
+    else if(w.maxPatch <= 16)
+        nmAffineWaveKernel<EK, 16><<<(w.count + 3) / 4, 64, 0, stream>>>(w.meshes, w.segs, w.nsegs, w.count);
+    else if(w.maxPatch <= 32)
+        nmAffineWaveKernel<EK, 32><<<(w.count + 1) / 2, 64, 0, stream>>>(w.meshes, w.segs, w.nsegs, w.count);
+    else
+        nmAffineWaveKernel<EK, 64><<<w.count, 64, 0, stream>>>(w.meshes, w.segs, w.nsegs, w.count);
+}
+
 // ---------------------------------------------------------------------------
 // GETMe element step (Smoothers/ElementWise/GetmeSmoother.cpp:55-127, 245-374):
 // every element's transformed positions and weight against the pass-start
@@ -1121,6 +1225,15 @@ static hipError_t launchSmoothTyped(const VertexWork& w, const gmq::Crew& c, int
 {
     if(w.count == 0)
         return hipSuccess;
+    if constexpr(Affine<T, EV, SA>::on && SM == SM_NELDER_MEAD)
+    {
+        if(shape == KS_WAVE && w.maxPatch <= 64 && w.kind != EK_MIXED)
+        {
+            if(w.kind == EK_TET) launchNmAffine<EK_TET>(w, stream);
+            else launchNmAffine<EK_HEX>(w, stream);
+            return hipGetLastError();
+        }
+    }
     if(shape == KS_WAVE && laneFits<T, EV, SA>(w, SM, lane))
     {
         if(w.kind == EK_TET) return launchLane<T, EV, SA, SM, EK_TET>(w, c, lane, moveCoeff, stream);
