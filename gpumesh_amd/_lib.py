@@ -26,7 +26,7 @@ EXPORTS = [
     "gm_mesh_evaluate", "gm_mesh_smooth", "gm_mesh_smooth_groups", "gm_mesh_last_timing", "gm_mesh_kernel_times",
     "gm_mesh_device_bytes", "gm_batch_create", "gm_batch_destroy", "gm_batch_set_shape", "gm_batch_smooth",
     "gm_batch_last_timing", "gm_batch_kernel_times", "gm_batch_autotune", "gm_server_create", "gm_server_destroy", "gm_server_set_shape", "gm_server_submit", "gm_server_wait",
-    "gm_server_result", "gm_server_release", "gm_server_stats", "gm_cube_mesh", "gm_set_topology_cache",
+    "gm_server_result", "gm_server_release", "gm_server_stats", "gm_cube_mesh", "gm_set_topology_cache", "gm_set_device_pool",
     "gm_topology_cache_stats", "gm_math_probe",
 ]
 
@@ -84,6 +84,7 @@ def lib():
     L.gm_server_release.argtypes = [vp, i64]
     L.gm_server_stats.argtypes = [vp, vp]
     L.gm_set_topology_cache.argtypes = [i64]
+    L.gm_set_device_pool.argtypes = [i64]
     L.gm_topology_cache_stats.argtypes = [vp]
     L.gm_cube_mesh.argtypes = [C.c_int, C.c_int, dbl, C.c_uint64, dbl, vp, vp, vp]
     L.gm_math_probe.argtypes = [C.c_int, vp, vp, vp, i64]

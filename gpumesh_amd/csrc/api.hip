@@ -39,8 +39,7 @@ namespace
 
 template<typename X> hipError_t devAlloc(X** p, size_t count)
 {
-    *p = nullptr;
-    return hipMalloc((void**)p, (count ? count : 1) * sizeof(X));
+    return gmr::poolAlloc((void**)p, (count ? count : 1) * sizeof(X));
 }
 
 int checkRun(int smoother, int passes, int precision, int kernel, double spawnCoeff)
@@ -171,6 +170,13 @@ int gm_set_topology_cache(int64_t max_bytes)
     return GM_OK;
 }
 
+int gm_set_device_pool(int64_t max_cached_bytes)
+{
+    if(max_cached_bytes < 0) return fail(GM_ERR_ARG, "negative pool size");
+    gmr::setPoolCap((size_t)max_cached_bytes);
+    return GM_OK;
+}
+
 int gm_topology_cache_stats(int64_t* out4)
 {
     if(!out4) return fail(GM_ERR_ARG, "null argument");
@@ -284,7 +290,7 @@ int gm_mesh_set_crew(gm_mesh* m, int evaluator, int sampler, double scaling,
             c.text[k] = text[k];
         }
         const size_t n = 6 * (size_t)tsize[0] * tsize[1] * tsize[2];
-        (void)hipFree(m->texels);
+        gmr::poolFree(m->texels);
         m->texels = nullptr;
         GM_HIP(devAlloc(&m->texels, n));
         GM_HIP(hipMemcpyAsync(m->texels, texels, n * sizeof(float), hipMemcpyHostToDevice, m->stream));
@@ -598,9 +604,9 @@ int gm_math_probe(int op, const double* a, const double* b, double* out, int64_t
     GM_HIP(hipMemcpy(db, b ? b : a, n * sizeof(double), hipMemcpyHostToDevice));
     GM_HIP(gmd::launchMathProbe(op, da, db, dout, (int)n, nullptr));
     GM_HIP(hipMemcpy(out, dout, n * sizeof(double), hipMemcpyDeviceToHost));
-    (void)hipFree(da);
-    (void)hipFree(db);
-    (void)hipFree(dout);
+    gmr::poolFree(da);
+    gmr::poolFree(db);
+    gmr::poolFree(dout);
     return GM_OK;
 }
 

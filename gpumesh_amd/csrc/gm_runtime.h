@@ -104,6 +104,13 @@ struct Plan
 
 void clearGraphs(Plan* p);
 
+// Caching device allocator (runtime.hip): poolFree of a pointer that did not
+// come from poolAlloc falls back to hipFree.
+hipError_t poolAlloc(void** out, size_t bytes);
+void poolFree(void* p);
+void setPoolCap(size_t bytes);
+size_t poolCachedBytes();
+
 // Host topology of a new handle (through the topology cache).
 int makeMesh(int64_t nverts, const double* pos, const int32_t* fixed, int64_t ntets, const uint32_t* tets,
              int64_t nhexs, const uint32_t* hexs, gm_mesh** out);

@@ -8,6 +8,8 @@
 
 #include <algorithm>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -34,16 +36,130 @@ const char* lastError() { return g_lastError.c_str(); }
                         std::string(#call) + ": " + hipGetErrorString(e_));            \
     } while(0)
 
-template<typename X> static hipError_t devAlloc(X** p, size_t count)
+// ---------------------------------------------------------------------------
+// Caching device allocator.  Freed blocks are kept per (device, size class)
+// and handed out again, so that meshes, pass records and plans that come and
+// go in a serving loop cost neither hipMalloc latency nor the device-wide
+// synchronisation of hipFree.  Size classes are 256 B, then four per power of
+// two (at most 25 % rounding).  Blocks are only freed or reused after the
+// streams that used them have been synchronised by their owners.
+// ---------------------------------------------------------------------------
+namespace
 {
-    *p = nullptr;
-    return hipMalloc((void**)p, (count ? count : 1) * sizeof(X));
+std::mutex g_poolMutex;
+std::map<std::pair<int, size_t>, std::vector<void*>> g_poolFree;
+std::map<void*, std::pair<int, size_t>> g_poolLive;
+size_t g_poolCached = 0, g_poolCap = (size_t)16 << 30;
+
+size_t sizeClass(size_t bytes)
+{
+    if(bytes <= 256) return 256;
+    int e = 63 - __builtin_clzll((unsigned long long)bytes);
+    const size_t step = (size_t)1 << (e - 2);
+    return (bytes + step - 1) / step * step;
+}
+} // namespace
+
+hipError_t poolAlloc(void** out, size_t bytes)
+{
+    *out = nullptr;
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if(e != hipSuccess) return e;
+    const size_t sz = sizeClass(bytes ? bytes : 1);
+    {
+        std::lock_guard<std::mutex> lock(g_poolMutex);
+        auto it = g_poolFree.find({dev, sz});
+        if(it != g_poolFree.end() && !it->second.empty())
+        {
+            *out = it->second.back();
+            it->second.pop_back();
+            g_poolCached -= sz;
+            g_poolLive[*out] = {dev, sz};
+            return hipSuccess;
+        }
+    }
+    e = hipMalloc(out, sz);
+    if(e == hipErrorOutOfMemory)
+    {
+        // Give the cached blocks of this device back and try once more.
+        (void)hipGetLastError();
+        std::vector<void*> drop;
+        {
+            std::lock_guard<std::mutex> lock(g_poolMutex);
+            for(auto& kv : g_poolFree)
+                if(kv.first.first == dev)
+                {
+                    for(void* q : kv.second) { drop.push_back(q); g_poolCached -= kv.first.second; }
+                    kv.second.clear();
+                }
+        }
+        for(void* q : drop) (void)hipFree(q);
+        e = hipMalloc(out, sz);
+    }
+    if(e != hipSuccess)
+    {
+        *out = nullptr;
+        return e;
+    }
+    std::lock_guard<std::mutex> lock(g_poolMutex);
+    g_poolLive[*out] = {dev, sz};
+    return hipSuccess;
 }
 
-// Rounded-up allocation size of devAlloc (the accounting of residentBytes).
+void poolFree(void* p)
+{
+    if(!p) return;
+    {
+        std::lock_guard<std::mutex> lock(g_poolMutex);
+        auto it = g_poolLive.find(p);
+        if(it != g_poolLive.end())
+        {
+            const auto key = it->second;
+            g_poolLive.erase(it);
+            if(g_poolCached + key.second <= g_poolCap)
+            {
+                g_poolFree[key].push_back(p);
+                g_poolCached += key.second;
+                return;
+            }
+        }
+    }
+    (void)hipFree(p);
+}
+
+void setPoolCap(size_t bytes)
+{
+    std::vector<void*> drop;
+    {
+        std::lock_guard<std::mutex> lock(g_poolMutex);
+        g_poolCap = bytes;
+        for(auto& kv : g_poolFree)
+            while(g_poolCached > g_poolCap && !kv.second.empty())
+            {
+                drop.push_back(kv.second.back());
+                kv.second.pop_back();
+                g_poolCached -= kv.first.second;
+            }
+    }
+    for(void* q : drop) (void)hipFree(q);
+}
+
+size_t poolCachedBytes()
+{
+    std::lock_guard<std::mutex> lock(g_poolMutex);
+    return g_poolCached;
+}
+
+template<typename X> static hipError_t devAlloc(X** p, size_t count)
+{
+    return poolAlloc((void**)p, (count ? count : 1) * sizeof(X));
+}
+
+// Allocation size of devAlloc (the accounting of residentBytes).
 template<typename X> static size_t allocBytes(size_t count)
 {
-    return ((count ? count : 1) * sizeof(X) + 255) & ~(size_t)255;
+    return sizeClass((count ? count : 1) * sizeof(X));
 }
 
 // ---------------------------------------------------------------------------
@@ -84,8 +200,8 @@ size_t residentBytes(const gm_mesh* m, int records, bool getme)
                allocBytes<int32_t>(1);
     if(records > 0) b += allocBytes<double>(3 * (size_t)records) + allocBytes<int32_t>(20 * (size_t)records);
     if(getme)
-        b += std::max<size_t>(256, 4 * (size_t)t.ntets * sizeof(double4)) +
-             std::max<size_t>(256, 8 * (size_t)t.nhexs * sizeof(double4));
+        b += sizeClass(std::max<size_t>(32, 4 * (size_t)t.ntets * sizeof(double4))) +
+             sizeClass(std::max<size_t>(32, 8 * (size_t)t.nhexs * sizeof(double4)));
     return b;
 }
 
@@ -97,7 +213,7 @@ static void freeMeshBuffers(gm_mesh* m)
                     d.getmeHex, d.evalPartials, d.evalFlags, d.evalHist, d.evalResult, d.evalHistOut, d.passSlot,
                     m->elemQualBuf, m->texels};
     for(void* p : ptrs)
-        if(p) (void)hipFree(p);
+        poolFree(p);
     d = gmd::DevMesh{};
     m->elemQualBuf = nullptr;
     m->texels = nullptr;
@@ -349,10 +465,10 @@ void freePlan(Plan* p)
 {
     if(!p) return;
     clearGraphs(p);
-    for(void* q : p->owned) (void)hipFree(q);
-    if(p->dTable) (void)hipFree(p->dTable);
-    if(p->spans) (void)hipFree(p->spans);
-    if(p->meshList) (void)hipFree(p->meshList);
+    for(void* q : p->owned) poolFree(q);
+    poolFree(p->dTable);
+    poolFree(p->spans);
+    poolFree(p->meshList);
     for(hipEvent_t e : p->ev) if(e) (void)hipEventDestroy(e);
     for(hipEvent_t e : p->pool) if(e) (void)hipEventDestroy(e);
     delete p;
@@ -361,8 +477,8 @@ void freePlan(Plan* p)
 static int ensureRecords(gm_mesh* m, int records)
 {
     if(records <= m->resultCap) return GM_OK;
-    (void)hipFree(m->dm.evalResult);
-    (void)hipFree(m->dm.evalHistOut);
+    poolFree(m->dm.evalResult);
+    poolFree(m->dm.evalHistOut);
     m->dm.evalResult = nullptr;
     m->dm.evalHistOut = nullptr;
     m->resultCap = 0;
@@ -381,8 +497,8 @@ static int ensureGetme(gm_mesh* m)
     // front half of each buffer.
     const size_t tetBytes = 4 * (size_t)m->topo->ntets * sizeof(double4);
     const size_t hexBytes = 8 * (size_t)m->topo->nhexs * sizeof(double4);
-    GMR_HIP(hipMalloc(&m->dm.getmeTet, std::max<size_t>(32, tetBytes)));
-    GMR_HIP(hipMalloc(&m->dm.getmeHex, std::max<size_t>(32, hexBytes)));
+    GMR_HIP(poolAlloc(&m->dm.getmeTet, std::max<size_t>(32, tetBytes)));
+    GMR_HIP(poolAlloc(&m->dm.getmeHex, std::max<size_t>(32, hexBytes)));
     m->getmeReady = true;
     return GM_OK;
 }

@@ -329,6 +329,9 @@ int launchRound(gm_server* s, const gmd::LaneShape* lanes, std::vector<hipStream
 void schedulerLoop(gm_server* s)
 {
     (void)hipSetDevice(s->device);
+    // Jobs whose last pass ran in the previous round: their results are
+    // copied out while the next round's kernels run.
+    std::vector<Job*> retiring;
     for(;;)
     {
         std::vector<Job*> admit;
@@ -336,7 +339,7 @@ void schedulerLoop(gm_server* s)
         {
             std::unique_lock<std::mutex> lock(s->mu);
             s->cvSched.wait(lock, [&] {
-                if(s->stop || !s->active.empty()) return true;
+                if(s->stop || !s->active.empty() || !retiring.empty()) return true;
                 for(Job* j : s->pending)
                     if(j->state == JOB_BUILT) return true;
                 return false;
@@ -366,21 +369,31 @@ void schedulerLoop(gm_server* s)
             }
             s->active.clear();
         }
-        // Start the admitted jobs while the round runs.
+        // While the round runs: retire the previous round's finished jobs and
+        // start the admitted ones, each on its own stream.
+        for(Job* j : retiring)
+        {
+            const int rrc = retireJob(s, j);
+            std::lock_guard<std::mutex> lock(s->mu);
+            s->reserved -= j->bytes;
+            finishJob(s, j, rrc, rrc ? gmr::lastError() : "");
+        }
+        retiring.clear();
         std::vector<Job*> started;
         for(Job* j : admit)
         {
-            int src = startJob(s, j);
-            std::lock_guard<std::mutex> lock(s->mu);
+            const int src = startJob(s, j);
             if(src)
             {
+                const std::string msg = gmr::lastError();
                 if(j->mesh)
                 {
                     gmr::destroyMesh(j->mesh);
                     j->mesh = nullptr;
                 }
+                std::lock_guard<std::mutex> lock(s->mu);
                 s->reserved -= j->bytes;
-                finishJob(s, j, src, gmr::lastError());
+                finishJob(s, j, src, msg);
             }
             else
                 started.push_back(j);
@@ -388,35 +401,19 @@ void schedulerLoop(gm_server* s)
         for(hipStream_t st : used) (void)hipStreamSynchronize(st);
         for(Job* j : started) (void)hipStreamSynchronize(j->mesh->stream);
         const Clock::time_point t1 = Clock::now();
-        // Retire what finished.
         std::vector<Job*> still;
         for(Job* j : s->active)
         {
             const bool f64 = j->precision == GM_FP64 || (j->precision == GM_FP32_POLISH && j->pass == j->passes - 1);
             if(f64) j->inF32 = false;
             ++j->pass;
-            if(j->pass < j->passes)
-            {
-                still.push_back(j);
-                continue;
-            }
-            int rrc = retireJob(s, j);
-            std::lock_guard<std::mutex> lock(s->mu);
-            s->reserved -= j->bytes;
-            finishJob(s, j, rrc, rrc ? gmr::lastError() : "");
+            if(j->pass < j->passes) still.push_back(j);
+            else retiring.push_back(j);
         }
-        // Jobs with no passes are finished at admission.
         for(Job* j : started)
         {
-            if(j->passes > 0)
-            {
-                still.push_back(j);
-                continue;
-            }
-            int rrc = retireJob(s, j);
-            std::lock_guard<std::mutex> lock(s->mu);
-            s->reserved -= j->bytes;
-            finishJob(s, j, rrc, rrc ? gmr::lastError() : "");
+            if(j->passes > 0) still.push_back(j);
+            else retiring.push_back(j);
         }
         s->active.swap(still);
         std::lock_guard<std::mutex> lock(s->mu);
@@ -428,6 +425,7 @@ void schedulerLoop(gm_server* s)
         s->peakActive = std::max<int64_t>(s->peakActive, (int64_t)s->active.size());
     }
     // Shutdown: release every resident job.
+    for(Job* j : retiring) s->active.push_back(j);
     for(Job* j : s->active)
     {
         gmr::destroyMesh(j->mesh);

@@ -162,6 +162,14 @@ int gm_set_topology_cache(int64_t max_bytes);
 /* out4 = {hits, misses, entries, bytes held}. */
 int gm_topology_cache_stats(int64_t* out4);
 
+/* Device memory is allocated through a caching allocator: blocks freed by
+ * destroyed meshes, batches and finished serving jobs are kept (per device,
+ * in size classes of at most 25 % rounding) and reused, so that a serving
+ * loop neither waits on hipMalloc nor synchronises the device in hipFree.
+ * max_cached_bytes bounds the free blocks kept (default 16 GiB); 0 returns
+ * them all to HIP now and disables caching. */
+int gm_set_device_pool(int64_t max_cached_bytes);
+
 /* Device memory owned by a mesh handle (positions, connectivity, patch
  * lists, reduction buffers, GETMe scratch once allocated), in bytes. */
 int gm_mesh_device_bytes(const gm_mesh* mesh, int64_t* bytes);
