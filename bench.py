@@ -56,6 +56,14 @@ FLOP_MR_TET_PATCH = 101.0
 FLOP_GETME_TET = 415.0
 # GETMe vertex update, per patch element pulled: 3 mul + 4 add.
 FLOP_GETME_PULL = 7.0
+# FLOP the FP32 kernels execute per tetrahedron evaluation on the affine
+# corner terms: determinant 6, squared norm 8, |d|^(-2/3) and 3/q 4,
+# accumulation 2.
+FLOP_AFFINE_TET = 20.0
+# HBM bytes per launch of the dominant kernel, measured with rocprofv3 --pmc
+# FETCH_SIZE and WRITE_SIZE (each scaled by the same counter's reading for a
+# 1 GiB device copy: x2.0 and x1.0) on the 100^3 Kuhn cube.
+MEASURED_TRAFFIC = {("spawnAffineKernel<0>", 100): 345680936.0}
 
 
 def log(msg):
@@ -246,7 +254,18 @@ def main():
     ap.add_argument("--serve-rates", default="50,200,1000,2000")
     ap.add_argument("--serve-plain-sample", type=int, default=8)
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB device to device 5 times (the reference kernel for HBM counters) and exit")
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+        a = torch.ones(args.copy_probe << 18, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": a.numel() * 4, "repeats": 5}))
+        return
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: multi-GPU runs are not part of this build "
@@ -328,16 +347,20 @@ def main():
     if best is not None:
         avg_s = best["ms"] / best["launches"] / 1e3
         steps = args.steps
+        f32_passes = steps - 1 if args.precision == "fp32_polish" else steps
         if best["class"] == "smooth_f32":
             # FP32 passes only (the last pass is the FP64 polish): scale the
             # counted FLOP to the FP32 share of the passes.
-            f32_passes = steps - 1 if args.precision == "fp32_polish" else steps
             flop_per_launch = best["f_smooth"] * (f32_passes / steps) / best["launches"]
             achieved = flop_per_launch / avg_s / 1e12
+            kname = {"spawn": "spawnAffineKernel<0>", "gd": "gdAffineKernel<0>",
+                     "nm": "nmAffineWaveKernel<0, 32>"}.get(best["smoother"], "smoothWaveKernel<float>")
+            executed = flop_per_launch * FLOP_AFFINE_TET / FLOP_MR_TET_PATCH / avg_s / 1e12
             roofline = {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                        "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
-                        "kernel": f"{'spawnLaneKernel' if best['smoother'] == 'spawn' else ('gdLaneKernel' if best['smoother'] == 'gd' else 'smoothWaveKernel')}<float> ({best['smoother']})", "avg_launch_ms": avg_s * 1e3,
-                        "flop_per_launch": flop_per_launch}
+                        "frac": achieved / PEAK_FP32_TFLOPS,
+                        "traffic": MEASURED_TRAFFIC.get((kname, args.n)), "kernel": kname,
+                        "avg_launch_ms": avg_s * 1e3, "flop_per_launch": flop_per_launch,
+                        "executed_tflops": executed, "executed_frac_of_nonpacked_fp32": executed / (PEAK_FP32_TFLOPS / 2)}
         elif best["class"] == "evaluate":
             bytes_per = 16.0 * gm.ntets + 32.0 * gm.nverts
             achieved = bytes_per / avg_s / 1e9
@@ -345,14 +368,15 @@ def main():
                         "frac": achieved / PEAK_HBM_GBS, "traffic": None, "kernel": "evalKernel",
                         "avg_launch_ms": avg_s * 1e3, "bytes_per_launch": bytes_per}
         else:
-            f32_passes = steps - 1 if args.precision == "fp32_polish" else steps
             if best["class"] == "getme_elem_f32":
                 flop_per_launch = FLOP_GETME_TET * gm.ntets
+                kname = "getmeTetKernel<float, 0, 0>"
             else:
                 flop_per_launch = (2 * FLOP_MR_TET_PATCH + FLOP_GETME_PULL) * gm.sum_patch / best["launches"] * f32_passes
+                kname = "getmeWaveKernel<float, 0, 0, 32>"
             achieved = flop_per_launch / avg_s / 1e12
             roofline = {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                        "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None, "kernel": best["class"],
+                        "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None, "kernel": kname,
                         "avg_launch_ms": avg_s * 1e3, "flop_per_launch": flop_per_launch}
 
     # Final quality of the mixed-precision runs against FP64 runs of the same

@@ -34,12 +34,22 @@ namespace gma
 using gmq::v3;
 using gmq::mk;
 
-// One corner: d0, g (3), c0, h (3), s, padded to three 16-byte rows so a
-// staged corner is read with three vector LDS loads.
+// One corner: d0, g (3), c0, h (3), s, then the near-degenerate threshold
+// and the corner's identity (element reference bits, hex corner index) for
+// the FP64 determinant recheck; three 16-byte rows, read with three vector
+// LDS loads.
 struct alignas(16) Corner
 {
-    float d0, gx, gy, gz, c0, hx, hy, hz, s, pad0, pad1, pad2;
+    float d0, gx, gy, gz, c0, hx, hy, hz, s, thr, ref, corner;
 };
+
+// A corner whose FP32 determinant is below thr = NEAR_DEGENERATE * c0^(3/2)
+// in magnitude (quality below about 0.006) is near-degenerate: its sign --
+// valid or inverted -- is decided in FP64 from the FP32 positions (for the
+// constant corners of a staged hexahedron, and for every corner at a
+// vertex's chosen new position).  FP32 rounding of d is about 1e-7 c0^(3/2),
+// far inside the band.
+constexpr float NEAR_DEGENERATE = 1.0e-4f;
 
 #define GMA __host__ __device__ inline
 
@@ -56,7 +66,9 @@ GMA Corner makeCorner(v3<float> c0, v3<float> c1, v3<float> c2, float r0, float 
     k.hy = 2.0f * (r0 * c0.y + r1 * c1.y + r2 * c2.y);
     k.hz = 2.0f * (r0 * c0.z + r1 * c1.z + r2 * c2.z);
     k.s = r0 * r0 + r1 * r1 + r2 * r2;
-    k.pad0 = k.pad1 = k.pad2 = 0.0f;
+    k.thr = NEAR_DEGENERATE * k.c0 * __builtin_sqrtf(k.c0);
+    k.ref = 0.0f;
+    k.corner = 0.0f;
     return k;
 }
 
@@ -69,6 +81,8 @@ GMA float flog2(float x) { return __builtin_log2f(x); }
 GMA float fexp2(float x) { return __builtin_exp2f(x); }
 GMA float frcp(float x) { return 1.0f / x; }
 #endif
+GMA float asFloat(uint32_t u) { float f; __builtin_memcpy(&f, &u, 4); return f; }
+GMA uint32_t asBits(float f) { uint32_t u; __builtin_memcpy(&u, &f, 4); return u; }
 
 // Corner at offset x (|x|^2 = xx): returns d and f2.
 GMA void cornerAt(const Corner& k, v3<float> x, float xx, float& d, float& f2)
@@ -109,7 +123,7 @@ struct Acc
 // ---- Tetrahedra: one corner; columns e_i3 = v3 - v_i (metric-free
 // segments), F = E Fr_TET_INV.  Moving vertex k: k == 3 adds x to every
 // column of E, k < 3 subtracts it from column k; r = Fr^T a.
-GMA Corner tetCorner(const v3<float> u[4], int k)
+GMA Corner tetCorner(const v3<float> u[4], int k, uint32_t ref)
 {
     const float R01 = -0.5773502691896257645091f, R11 = 1.154700538379251529018f;
     const float R02 = -0.4082482904638630163662f, R12 = -0.4082482904638630163662f, R22 = 1.224744871391589049099f;
@@ -125,23 +139,40 @@ GMA Corner tetCorner(const v3<float> u[4], int k)
     const float r0 = a0;
     const float r1 = a0 * R01 + a1 * R11;
     const float r2 = a0 * R02 + a1 * R12 + a2 * R22;
-    return makeCorner(f0, f1, f2, r0, r1, r2);
+    Corner c = makeCorner(f0, f1, f2, r0, r1, r2);
+    c.ref = asFloat(ref);
+    return c;
 }
 
 // ---- Hexahedra: corner c has columns v[P] - v[M] for the (P, M) pairs
 // below (MeanRatioEvaluator::hexQuality corner frames).
+// Packed as 4 bits per vertex index: column j's P at bits 8j+4, M at 8j.
+GMA uint32_t hexCornerCode(int c)
+{
+    switch(c)
+    {
+    case 0: return 0x034010u;  // (1,0) (4,0) (0,3)
+    case 1: return 0x512110u;  // (1,0) (2,1) (5,1)
+    case 2: return 0x236221u;  // (2,1) (6,2) (2,3)
+    case 3: return 0x733230u;  // (3,0) (3,2) (7,3)
+    case 4: return 0x745440u;  // (4,0) (5,4) (7,4)
+    case 5: return 0x545651u;  // (5,1) (5,6) (5,4)
+    case 6: return 0x766562u;  // (6,2) (6,5) (7,6)
+    default: return 0x477673u; // (7,3) (7,6) (4,7)
+    }
+}
+
 GMA void hexCornerPairs(int c, int j, int& P, int& M)
 {
-    const int T[8][3][2] = {{{1, 0}, {4, 0}, {0, 3}}, {{1, 0}, {2, 1}, {5, 1}}, {{2, 1}, {6, 2}, {2, 3}},
-                            {{3, 0}, {3, 2}, {7, 3}}, {{4, 0}, {5, 4}, {7, 4}}, {{5, 1}, {5, 6}, {5, 4}},
-                            {{6, 2}, {6, 5}, {7, 6}}, {{7, 3}, {7, 6}, {4, 7}}};
-    P = T[c][j][0];
-    M = T[c][j][1];
+    const uint32_t code = hexCornerCode(c) >> (8 * j);
+    P = (int)((code >> 4) & 15u);
+    M = (int)(code & 15u);
 }
 
 // Staged hexahedron with moving vertex k: the 4 corners that involve k, and
 // the 4 that do not folded into their 3/q sum (or their smallest quality
-// when one is non-positive).
+// when one is non-positive).  `fix(corner)` returns a constant corner's FP64
+// determinant, used when it is near-degenerate.
 struct alignas(16) Hex
 {
     Corner k[4];
@@ -150,17 +181,19 @@ struct alignas(16) Hex
     float pad[2];
 };
 
-GMA Hex stageHex(const v3<float> u[8], int k)
+template<class Fix> GMA Hex stageHex(const v3<float> u[8], int k, uint32_t ref, const Fix& fix)
 {
     Hex h;
     int nv = 0;
     float inv3 = 0.0f, mn = 1.0f;
     bool bad = false;
+#pragma unroll
     for(int c = 0; c < 8; ++c)
     {
         v3<float> col[3];
         float r[3];
         bool moves = false;
+#pragma unroll
         for(int j = 0; j < 3; ++j)
         {
             int P, M;
@@ -169,13 +202,17 @@ GMA Hex stageHex(const v3<float> u[8], int k)
             r[j] = P == k ? 1.0f : (M == k ? -1.0f : 0.0f);
             moves = moves || r[j] != 0.0f;
         }
-        const Corner cc = makeCorner(col[0], col[1], col[2], r[0], r[1], r[2]);
+        Corner cc = makeCorner(col[0], col[1], col[2], r[0], r[1], r[2]);
+        cc.ref = asFloat(ref);
+        cc.corner = asFloat((uint32_t)c);
         if(moves)
         {
             if(nv < 4) h.k[nv] = cc;
             ++nv;
+            continue;
         }
-        else if(cc.d0 > 0.0f)
+        if(__builtin_fabsf(cc.d0) < cc.thr) cc.d0 = fix(cc);
+        if(cc.d0 > 0.0f)
             inv3 += invQ3(cc.d0, cc.c0);
         else
         {
@@ -206,6 +243,7 @@ GMA void addHex(Acc& a, const Hex& h, v3<float> x, float xx)
 {
     float inv3 = h.constInv3, mn = h.constMin;
     bool bad = h.constMin <= 0.0f;
+#pragma unroll
     for(int c = 0; c < 4; ++c)
     {
         float d, f2;

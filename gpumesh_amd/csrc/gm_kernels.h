@@ -523,6 +523,89 @@ template<typename T, int EV, int SA, int EK> __host__ __device__ inline size_t l
     return Affine<T, EV, SA>::on ? (size_t)elems * affineBytes<EK>() : stageBytes<T, EK>(elems);
 }
 
+// FP64 determinant of a corner, from the FP32 positions with the moving
+// vertex at p (widened exactly).
+template<int EK>
+struct DetFix
+{
+    const DevMesh* m;
+    const float4* pos;
+
+    __device__ v3<double> at(uint32_t v, int slot, int kk, v3<double> p) const
+    {
+        if(slot == kk) return p;
+        const float4 q = pos[v];
+        return mk<double>((double)q.x, (double)q.y, (double)q.z);
+    }
+
+    __device__ float det64(const gma::Corner& k, v3<float> pf) const
+    {
+        const uint32_t ref = gma::asBits(k.ref);
+        const uint32_t id = ref >> 4;
+        const int kk = (int)(ref & 7);
+        const v3<double> p = mk<double>((double)pf.x, (double)pf.y, (double)pf.z);
+        if(EK == 0)
+        {
+            const uint4 t = m->tets[id];
+            const v3<double> v0 = at(t.x, 0, kk, p), v1 = at(t.y, 1, kk, p), v2 = at(t.z, 2, kk, p),
+                             v3_ = at(t.w, 3, kk, p);
+            const double dE = gmq::dot(v3_ - v0, gmq::cross(v3_ - v1, v3_ - v2));
+            // det F = det E * det Fr_TET_INV, and det Fr_TET_INV = sqrt(2).
+            return (float)(dE * 1.4142135623730951);
+        }
+        const uint4 a = m->hexs[2 * id], b = m->hexs[2 * id + 1];
+        const uint32_t ids[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+        const uint32_t code = gma::hexCornerCode((int)gma::asBits(k.corner));
+        v3<double> col[3];
+#pragma unroll
+        for(int j = 0; j < 3; ++j)
+        {
+            const int P = (int)((code >> (8 * j + 4)) & 15u), M = (int)((code >> (8 * j)) & 15u);
+            uint32_t vp = ids[0], vm = ids[0];
+#pragma unroll
+            for(int q = 1; q < 8; ++q)
+            {
+                vp = P == q ? ids[q] : vp;
+                vm = M == q ? ids[q] : vm;
+            }
+            col[j] = at(vp, P, kk, p) - at(vm, M, kk, p);
+        }
+        return (float)gmq::dot(col[0], gmq::cross(col[1], col[2]));
+    }
+};
+
+// Constant corners of a staged hexahedron: the moving vertex at its position.
+template<int EK>
+struct ConstFix
+{
+    DetFix<EK> d;
+    v3<float> o;
+    __device__ float operator()(const gma::Corner& k) const { return d.det64(k, o); }
+};
+
+// FP64 inversion test of a chosen move (one staged element): corners FP32
+// calls non-positive set `nonPos32`; corners FP32 calls positive but inside
+// the near-degenerate band are recomputed in FP64, and set `flip` when FP64
+// finds them non-positive.
+template<int EK>
+__device__ __forceinline__ void inversionTest(const unsigned char* sp, int e, v3<float> x, v3<float> pf,
+                                              const DetFix<EK>& fix, bool& nonPos32, bool& flip)
+{
+    auto one = [&](const gma::Corner& k) {
+        const float d = k.d0 + (k.gx * x.x + k.gy * x.y + k.gz * x.z);
+        if(!(d > 0.0f)) nonPos32 = true;
+        else if(d < k.thr && !(fix.det64(k, pf) > 0.0f)) flip = true;
+    };
+    if(EK == 0)
+        one(((const gma::Corner*)sp)[e]);
+    else
+    {
+        const gma::Hex& h = ((const gma::Hex*)sp)[e];
+        for(int c = 0; c < 4; ++c) one(h.k[c]);
+        if(h.constMin <= 0.0f) nonPos32 = true;
+    }
+}
+
 template<int EK>
 __device__ __forceinline__ void stageAffine(const DevMesh& m, const float4* pos, uint32_t ref, v3<float> o,
                                             unsigned char* dst)
@@ -534,7 +617,7 @@ __device__ __forceinline__ void stageAffine(const DevMesh& m, const float4* pos,
         const uint4 t = m.tets[id];
         v3<float> u[4] = {ld<float>(pos, t.x) - o, ld<float>(pos, t.y) - o, ld<float>(pos, t.z) - o,
                           ld<float>(pos, t.w) - o};
-        *(gma::Corner*)dst = gma::tetCorner(u, k);
+        *(gma::Corner*)dst = gma::tetCorner(u, k, ref);
     }
     else
     {
@@ -542,7 +625,7 @@ __device__ __forceinline__ void stageAffine(const DevMesh& m, const float4* pos,
         v3<float> u[8] = {ld<float>(pos, a.x) - o, ld<float>(pos, a.y) - o, ld<float>(pos, a.z) - o,
                           ld<float>(pos, a.w) - o, ld<float>(pos, b.x) - o, ld<float>(pos, b.y) - o,
                           ld<float>(pos, b.z) - o, ld<float>(pos, b.w) - o};
-        *(gma::Hex*)dst = gma::stageHex(u, k);
+        *(gma::Hex*)dst = gma::stageHex(u, k, ref, ConstFix<1>{DetFix<1>{&m, pos}, o});
     }
 }
 
@@ -593,9 +676,16 @@ __global__ void __launch_bounds__(512) spawnAffineKernel(const DevMesh* __restri
         xb = shflv<float>(x, best);
         scale /= 3.0f;
     }
+    // FP64 inversion test of the move, one element per lane.
+    const v3<float> pf = o + xb;
+    const DetFix<EK> fix{&m, pos};
+    bool nonPos32 = false, flip = false;
+    for(int e = lane; e < nE; e += 64)
+        inversionTest<EK>(sp, e, xb, pf, fix, nonPos32, flip);
+    const bool reject = __ballot(flip) != 0 && __ballot(nonPos32) == 0;
     if(lane == 0)
     {
-        st(pos, v, o + xb);
+        st(pos, v, reject ? o : pf);
         m.evalCounts[v] += 128;
     }
 }
@@ -656,9 +746,17 @@ __global__ void __launch_bounds__(512) gdAffineKernel(const DevMesh* __restrict_
         if(nodeShift < originalNodeShift / 10.0f)
             break;
     }
+    // FP64 inversion test of the move over the vertex's 8 lanes.
+    const v3<float> pf = o + x;
+    const DetFix<EK> fix{&m, pos};
+    bool nonPos32 = false, flip = false;
+    for(int e = s; e < nE; e += 8)
+        inversionTest<EK>(mySp, e, x, pf, fix, nonPos32, flip);
+    const int sh = (threadIdx.x & 63) & ~7;
+    const bool reject = ((__ballot(flip) >> sh) & 0xFFull) != 0 && ((__ballot(nonPos32) >> sh) & 0xFFull) == 0;
     if(s == 0)
     {
-        st(pos, v, o + x);
+        st(pos, v, reject ? o : pf);
         m.evalCounts[v] += evals;
     }
 }
@@ -734,7 +832,7 @@ __global__ void __launch_bounds__(64) nmAffineWaveKernel(const DevMesh* __restri
             const uint4 t = m.tets[id];
             v3<float> u[4] = {ld<float>(pos, t.x) - f.o, ld<float>(pos, t.y) - f.o, ld<float>(pos, t.z) - f.o,
                               ld<float>(pos, t.w) - f.o};
-            f.cn = gma::tetCorner(u, k);
+            f.cn = gma::tetCorner(u, k, ref);
         }
         else
         {
@@ -742,14 +840,26 @@ __global__ void __launch_bounds__(64) nmAffineWaveKernel(const DevMesh* __restri
             v3<float> u[8] = {ld<float>(pos, a.x) - f.o, ld<float>(pos, a.y) - f.o, ld<float>(pos, a.z) - f.o,
                               ld<float>(pos, a.w) - f.o, ld<float>(pos, b.x) - f.o, ld<float>(pos, b.y) - f.o,
                               ld<float>(pos, b.z) - f.o, ld<float>(pos, b.w) - f.o};
-            f.hx = gma::stageHex(u, k);
+            f.hx = gma::stageHex(u, k, ref, ConstFix<1>{DetFix<1>{&m, pos}, f.o});
         }
     }
     const float ls = localSize<float>(m, pos, v, f.o);
     const v3<float> np = gms::nelderMead<float>(f.o, ls, f);
+    // FP64 inversion test of the move, one element per lane of the group.
+    bool nonPos32 = false, flip = false;
+    if(lane < f.nElem)
+    {
+        const DetFix<EK> fix{&m, pos};
+        const v3<float> x = np - f.o;
+        inversionTest<EK>(EK == 0 ? (const unsigned char*)&f.cn : (const unsigned char*)&f.hx, 0, x, np, fix,
+                          nonPos32, flip);
+    }
+    const int sh = (threadIdx.x & 63) & ~(G - 1);
+    const unsigned long long gmask = G == 64 ? ~0ull : ((1ull << G) - 1);
+    const bool reject = ((__ballot(flip) >> sh) & gmask) != 0 && ((__ballot(nonPos32) >> sh) & gmask) == 0;
     if(lane == 0)
     {
-        st(pos, v, np);
+        st(pos, v, reject ? f.o : np);
         m.evalCounts[v] += f.count;
     }
 }

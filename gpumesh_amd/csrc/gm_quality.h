@@ -300,6 +300,17 @@ template<typename T> GMQ T cornerQuality(const m3<T>& F)
 {
     T d = det(F);
     T f2 = dot(F.c[0], F.c[0]) + dot(F.c[1], F.c[1]) + dot(F.c[2], F.c[2]);
+    if constexpr(sizeof(T) == 4)
+    {
+        // FP32: a near-degenerate corner (|d| < 1e-4 f2^(3/2), quality below
+        // about 0.006) gets its determinant, and so its sign, in FP64.
+        if(d * d < T(1e-8) * f2 * f2 * f2)
+        {
+            m3<double> Fd;
+            for(int j = 0; j < 3; ++j) Fd.c[j] = mk<double>((double)F.c[j].x, (double)F.c[j].y, (double)F.c[j].z);
+            d = (T)det(Fd);
+        }
+    }
     return tsign(d) * T(3) * tpow23(tabs(d)) / f2;
 }
 
