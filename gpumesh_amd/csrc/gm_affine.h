@@ -227,6 +227,29 @@ template<class Fix> GMA Hex stageHex(const v3<float> u[8], int k, uint32_t ref, 
     return h;
 }
 
+// Branch-free sums for the common case: 3/q of a corner assuming d > 0, and
+// whether that assumption failed.  A patch whose elements are all positive
+// has quality 3 n / sum; otherwise the caller evaluates it again with the
+// exact rule (addTet / addHex), which is rare.
+GMA float fastInv3(const Corner& k, v3<float> x, float xx, bool& bad)
+{
+    float d, f2;
+    cornerAt(k, x, xx, d, f2);
+    bad = bad || !(d > 0.0f);
+    return f2 * fexp2(-0.6666666666666666f * flog2(__builtin_fabsf(d)));
+}
+
+GMA float fastTet(const Corner& k, v3<float> x, float xx, bool& bad) { return fastInv3(k, x, xx, bad); }
+
+GMA float fastHex(const Hex& h, v3<float> x, float xx, bool& bad)
+{
+    float s = h.constInv3;
+    bad = bad || !(h.constMin > 0.0f);
+#pragma unroll
+    for(int c = 0; c < 4; ++c) s += fastInv3(h.k[c], x, xx, bad);
+    return 0.125f * s;
+}
+
 // Adds one staged tetrahedron at offset x to the accumulator.
 GMA void addTet(Acc& a, const Corner& k, v3<float> x, float xx)
 {

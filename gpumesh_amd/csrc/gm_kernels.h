@@ -629,10 +629,24 @@ __device__ __forceinline__ void stageAffine(const DevMesh& m, const float4* pos,
     }
 }
 
+// Patch quality assuming every element is positive (branch-free); `bad`
+// reports that the assumption failed.
 template<int EK>
-__device__ __forceinline__ float affinePatch(const unsigned char* sp, int nE, v3<float> x)
+__device__ __forceinline__ float affinePatchFast(const unsigned char* sp, int nE, v3<float> x, float xx, bool& bad)
 {
-    const float xx = gmq::dot(x, x);
+    float inv3 = 0.0f;
+    for(int e = 0; e < nE; ++e)
+    {
+        if(EK == 0) inv3 += gma::fastTet(((const gma::Corner*)sp)[e], x, xx, bad);
+        else inv3 += gma::fastHex(((const gma::Hex*)sp)[e], x, xx, bad);
+    }
+    return 3.0f * (float)nE / inv3;
+}
+
+// The exact patch rule (smallest non-positive quality when one exists).
+template<int EK>
+__device__ __forceinline__ float affinePatchExact(const unsigned char* sp, int nE, v3<float> x, float xx)
+{
     gma::Acc a;
     for(int e = 0; e < nE; ++e)
     {
@@ -640,6 +654,15 @@ __device__ __forceinline__ float affinePatch(const unsigned char* sp, int nE, v3
         else gma::addHex(a, ((const gma::Hex*)sp)[e], x, xx);
     }
     return a.finish();
+}
+
+template<int EK>
+__device__ __forceinline__ float affinePatch(const unsigned char* sp, int nE, v3<float> x)
+{
+    const float xx = gmq::dot(x, x);
+    bool bad = false;
+    const float q = affinePatchFast<EK>(sp, nE, x, xx, bad);
+    return bad ? affinePatchExact<EK>(sp, nE, x, xx) : q;
 }
 
 template<int EK>
@@ -671,7 +694,16 @@ __global__ void __launch_bounds__(512) spawnAffineKernel(const DevMesh* __restri
     for(int iter = 0; iter < 2; ++iter)
     {
         const v3<float> x = xb + gms::spawnOffset<float>(lane) * scale;
-        const float q = affinePatch<EK>(sp, nE, x);
+        // A candidate with a non-positive element has quality <= 0 and loses
+        // to any candidate without one; its exact value matters only when
+        // every candidate of the vertex has one.
+        const float xx = gmq::dot(x, x);
+        bool bad = false;
+        float q = affinePatchFast<EK>(sp, nE, x, xx, bad);
+        if(__ballot(!bad) == 0)
+            q = affinePatchExact<EK>(sp, nE, x, xx);
+        else if(bad)
+            q = -__builtin_huge_valf();
         const int best = groupArgmax<float>(q, lane, 64);
         xb = shflv<float>(x, best);
         scale /= 3.0f;
@@ -779,26 +811,34 @@ struct AffineWavePatch
         ++count;
         const v3<float> x = p - o;
         const float xx = gmq::dot(x, x);
+        float sum = 0.0f;
+        bool badLane = false;
+        if(lane < nElem)
+            sum = EK == 0 ? gma::fastTet(cn, x, xx, badLane) : gma::fastHex(hx, x, xx, badLane);
+        int bad = badLane ? 1 : 0;
+#pragma unroll
+        for(int off = G / 2; off > 0; off >>= 1)
+        {
+            sum += __shfl_xor(sum, off, 64);
+            bad |= __shfl_xor(bad, off, 64);
+        }
+        if(!bad)
+            return 3.0f * (float)nElem / sum;
+        // A non-positive element: the smallest non-positive quality.
         gma::Acc a;
         if(lane < nElem)
         {
             if(EK == 0) gma::addTet(a, cn, x, xx);
             else gma::addHex(a, hx, x, xx);
         }
-        float sum = a.inv3;
-        float cnt = (float)a.n;
-        int bad = a.bad ? 1 : 0;
         float mn = (a.bad && a.mn == a.mn) ? a.mn : __builtin_huge_valf();
 #pragma unroll
         for(int off = G / 2; off > 0; off >>= 1)
         {
-            sum += __shfl_xor(sum, off, 64);
-            cnt += __shfl_xor(cnt, off, 64);
-            bad |= __shfl_xor(bad, off, 64);
             const float om = __shfl_xor(mn, off, 64);
             mn = om < mn ? om : mn;
         }
-        return bad ? mn : 3.0f * cnt / sum;
+        return mn;
     }
 };
 
