@@ -60,21 +60,12 @@ int checkSampler(int sampler, int smoother, int kernel)
     return GM_OK;
 }
 
-// Default launch shape of the candidate-parallel kernels: one vertex per
-// wavefront with the smallest tile that holds the largest patch (spawn
-// search); 8 vertices per wavefront, tile / 8 slots each (gradient descent).
-gmd::LaneShape defaultLane(const std::vector<gm_mesh*>& meshes, int smoother)
+// Default launch shape: one wavefront per workgroup (GETMe vertex update: 4)
+// and the LDS tile chosen per work list from its largest patch (tile 0), so
+// that a list picks the same kernel alone and batched.
+gmd::LaneShape defaultLane(const std::vector<gm_mesh*>&, int smoother)
 {
-    int maxPatch = 0;
-    for(const gm_mesh* m : meshes) maxPatch = std::max(maxPatch, m->maxPatch);
-    gmd::LaneShape l{1, 64};
-    if(smoother == gmd::SM_GETME)
-        l.patchesPerGroup = 4;
-    else if(smoother == gmd::SM_GRADIENT_DESCENT)
-        l.tileElems = maxPatch <= 8 ? 64 : (maxPatch <= 16 ? 128 : 256);
-    else
-        l.tileElems = maxPatch <= 64 ? 64 : (maxPatch <= 128 ? 128 : 256);
-    return l;
+    return gmd::LaneShape{smoother == gmd::SM_GETME ? 4 : 1, 0};
 }
 
 int runAndFetch(gmr::Plan* p, const gm_mesh* m0, const gmr::RunParams& r, double* stats, int32_t* hist,
@@ -554,7 +545,7 @@ int gm_batch_autotune(gm_batch* b, int smoother, int precision, int passes, int3
             for(const auto& col : p->colours)
                 for(const gmr::VList& l : col)
                 {
-                    gmd::VertexWork w{p->dTable, l.segs, l.nsegs, l.count, l.kind, l.maxPatch};
+                    gmd::VertexWork w{p->dTable, l.segs, l.nsegs, l.count, l.kind, l.maxPatch, 0};
                     if(!getme && l.kind != gmd::EK_MIXED &&
                        !gmd::laneFits(w, smoother, shape, precision == GM_FP64, p->meshes[0]->evaluator,
                                       p->meshes[0]->sampler))

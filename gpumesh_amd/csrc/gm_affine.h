@@ -1,6 +1,7 @@
-// gm_affine.h -- FP32 patch evaluation of the mean-ratio evaluator with the
-// metric-free measurer, for optimisers that try many positions of one vertex
-// while the rest of its patch stays put.
+// gm_affine.h -- patch evaluation of the mean-ratio evaluator with the
+// metric-free measurer for optimisers that try many positions of one vertex
+// while the rest of its patch stays put: the FP32 passes and the FP64 polish
+// pass of the mixed-precision mode.
 //
 // Every mean-ratio corner of an element (the tetrahedron's single corner,
 // MeanRatioEvaluator::tetQuality :34-50, or a hexahedron corner, :106-167) is
@@ -14,10 +15,10 @@
 // with d0 = det F0, g = sum_j r_j (c_{j+1} x c_{j+2}), c0 = |F0|^2,
 // h = 2 sum_j r_j c_j, s = |r|^2, c_j the columns of F0.  Staging these nine
 // numbers per corner once, with coordinates taken relative to the vertex's
-// position, turns a corner evaluation into a handful of FMAs, one log and one
-// exp.  The identity is exact; the results differ from the direct formula
-// only by FP32 rounding.  Mean ratio is invariant under uniform scaling, so
-// the metric scaling factor drops out.
+// position, turns a corner evaluation into a handful of FMAs and one
+// |d|^(-2/3).  The identity is exact; results differ from the direct formula
+// only by rounding.  Mean ratio is invariant under uniform scaling, so the
+// metric scaling factor drops out.
 //
 // Patch quality (AbstractEvaluator::patchQuality, accumulatePatchQuality and
 // finalizePatchQuality, AbstractEvaluator.cpp:778-803) is the harmonic mean of
@@ -34,15 +35,6 @@ namespace gma
 using gmq::v3;
 using gmq::mk;
 
-// One corner: d0, g (3), c0, h (3), s, then the near-degenerate threshold
-// and the corner's identity (element reference bits, hex corner index) for
-// the FP64 determinant recheck; three 16-byte rows, read with three vector
-// LDS loads.
-struct alignas(16) Corner
-{
-    float d0, gx, gy, gz, c0, hx, hy, hz, s, thr, ref, corner;
-};
-
 // A corner whose FP32 determinant is below thr = NEAR_DEGENERATE * c0^(3/2)
 // in magnitude (quality below about 0.006) is near-degenerate: its sign --
 // valid or inverted -- is decided in FP64 from the FP32 positions (for the
@@ -51,26 +43,20 @@ struct alignas(16) Corner
 // far inside the band.
 constexpr float NEAR_DEGENERATE = 1.0e-4f;
 
-#define GMA __host__ __device__ inline
-
-GMA Corner makeCorner(v3<float> c0, v3<float> c1, v3<float> c2, float r0, float r1, float r2)
+// One corner: d0, g (3), c0, h (3), s, then (FP32 only) the near-degenerate
+// threshold and the corner's identity (element reference, hex corner index)
+// for the FP64 determinant recheck.  48 bytes in FP32, 96 in FP64: whole
+// 16-byte rows for vector LDS loads.
+template<typename T>
+struct alignas(16) Corner
 {
-    const v3<float> x12 = gmq::cross(c1, c2), x20 = gmq::cross(c2, c0), x01 = gmq::cross(c0, c1);
-    Corner k;
-    k.d0 = gmq::dot(c0, x12);
-    k.gx = r0 * x12.x + r1 * x20.x + r2 * x01.x;
-    k.gy = r0 * x12.y + r1 * x20.y + r2 * x01.y;
-    k.gz = r0 * x12.z + r1 * x20.z + r2 * x01.z;
-    k.c0 = gmq::dot(c0, c0) + gmq::dot(c1, c1) + gmq::dot(c2, c2);
-    k.hx = 2.0f * (r0 * c0.x + r1 * c1.x + r2 * c2.x);
-    k.hy = 2.0f * (r0 * c0.y + r1 * c1.y + r2 * c2.y);
-    k.hz = 2.0f * (r0 * c0.z + r1 * c1.z + r2 * c2.z);
-    k.s = r0 * r0 + r1 * r1 + r2 * r2;
-    k.thr = NEAR_DEGENERATE * k.c0 * __builtin_sqrtf(k.c0);
-    k.ref = 0.0f;
-    k.corner = 0.0f;
-    return k;
-}
+    T d0, gx, gy, gz, c0, hx, hy, hz, s;
+    float thr;
+    uint32_t ref;
+    uint32_t corner;
+};
+
+#define GMA __host__ __device__ inline
 
 #if defined(__HIP_DEVICE_COMPILE__)
 GMA float flog2(float x) { return __builtin_amdgcn_logf(x); }
@@ -81,72 +67,128 @@ GMA float flog2(float x) { return __builtin_log2f(x); }
 GMA float fexp2(float x) { return __builtin_exp2f(x); }
 GMA float frcp(float x) { return 1.0f / x; }
 #endif
-GMA float asFloat(uint32_t u) { float f; __builtin_memcpy(&f, &u, 4); return f; }
-GMA uint32_t asBits(float f) { uint32_t u; __builtin_memcpy(&u, &f, 4); return u; }
+
+// a^(-2/3) for a > 0.
+GMA float powm23(float a) { return fexp2(-0.6666666666666666f * flog2(a)); }
+
+// FP64: a seed from the FP32 hardware log/exp on the mantissa (exponent
+// handled exactly), then two Newton steps on y^3 a^2 = 1, each of which
+// squares the relative error (1e-7 -> 1e-14 -> below the FP64 rounding).
+GMA double powm23(double a)
+{
+    const uint64_t b = gm_dbits(a);
+    const int e = (int)((b >> 52) & 0x7ff) - 1023;
+    if(e == -1023 || e == 1024)
+        return 1.0 / gm_pow23(a);   // zero, subnormal, inf, NaN: exact slow path
+    const double m = gm_dfrom((b & 0x000FFFFFFFFFFFFFull) | 0x3FF0000000000000ull);
+    const double L = -0.6666666666666666 * ((double)e + (double)flog2((float)m));
+    const double fl = __builtin_floor(L);
+    const int i = (int)fl;
+    double y = (double)fexp2((float)(L - fl)) * gm_pow2i(i);
+#pragma unroll
+    for(int it = 0; it < 2; ++it)
+    {
+        double t = a * y;
+        t = t * t * y;                   // a^2 y^3
+        y = y + y * (1.0 - t) * (1.0 / 3.0);
+    }
+    return y;
+}
+
+// |a|^(2/3) (the rare non-positive path).
+GMA float pow23abs(float a) { return fexp2(0.6666666666666666f * flog2(__builtin_fabsf(a))); }
+GMA double pow23abs(double a) { return gm_pow23(__builtin_fabs(a)); }
+
+GMA float rcpT(float x) { return frcp(x); }
+GMA double rcpT(double x) { return 1.0 / x; }
+
+template<typename T>
+GMA Corner<T> makeCorner(v3<T> c0, v3<T> c1, v3<T> c2, T r0, T r1, T r2)
+{
+    const v3<T> x12 = gmq::cross(c1, c2), x20 = gmq::cross(c2, c0), x01 = gmq::cross(c0, c1);
+    Corner<T> k;
+    k.d0 = gmq::dot(c0, x12);
+    k.gx = r0 * x12.x + r1 * x20.x + r2 * x01.x;
+    k.gy = r0 * x12.y + r1 * x20.y + r2 * x01.y;
+    k.gz = r0 * x12.z + r1 * x20.z + r2 * x01.z;
+    k.c0 = gmq::dot(c0, c0) + gmq::dot(c1, c1) + gmq::dot(c2, c2);
+    k.hx = T(2) * (r0 * c0.x + r1 * c1.x + r2 * c2.x);
+    k.hy = T(2) * (r0 * c0.y + r1 * c1.y + r2 * c2.y);
+    k.hz = T(2) * (r0 * c0.z + r1 * c1.z + r2 * c2.z);
+    k.s = r0 * r0 + r1 * r1 + r2 * r2;
+    const float c0f = (float)k.c0;
+    k.thr = NEAR_DEGENERATE * c0f * __builtin_sqrtf(c0f);
+    k.ref = 0;
+    k.corner = 0;
+    return k;
+}
 
 // Corner at offset x (|x|^2 = xx): returns d and f2.
-GMA void cornerAt(const Corner& k, v3<float> x, float xx, float& d, float& f2)
+template<typename T>
+GMA void cornerAt(const Corner<T>& k, v3<T> x, T xx, T& d, T& f2)
 {
     d = k.d0 + (k.gx * x.x + k.gy * x.y + k.gz * x.z);
     f2 = k.c0 + (k.hx * x.x + k.hy * x.y + k.hz * x.z) + k.s * xx;
 }
 
 // 3 / q of a corner with d > 0 (the factor 3 is applied once per patch).
-GMA float invQ3(float d, float f2) { return f2 * fexp2(-0.6666666666666666f * flog2(d)); }
+template<typename T> GMA T invQ3(T d, T f2) { return f2 * powm23(d); }
 
 // q of a corner with d <= 0 (0 when d == 0 and f2 > 0, NaN when F == 0).
-GMA float qNonPositive(float d, float f2)
+template<typename T> GMA T qNonPositive(T d, T f2)
 {
-    const float m = d < 0.0f ? -3.0f : 0.0f;
-    return m * fexp2(0.6666666666666666f * flog2(-d)) * frcp(f2);
+    const T m = d < T(0) ? T(-3) : T(0);
+    return m * pow23abs(d) * rcpT(f2);
 }
 
 // Patch accumulator: sum of 3/q over positive elements, smallest
 // non-positive q, element count, and whether any element was non-positive.
+template<typename T>
 struct Acc
 {
-    float inv3 = 0.0f;
-    float mn = 0.0f;
+    T inv3 = T(0);
+    T mn = T(0);
     int n = 0;
     bool bad = false;
-    GMA void positive(float i3) { inv3 += i3; ++n; }
-    GMA void nonPositive(float q)
+    GMA void positive(T i3) { inv3 += i3; ++n; }
+    GMA void nonPositive(T q)
     {
         // The first non-positive quality replaces the running value; later
         // ones keep the minimum (a NaN never replaces it).
         if(!bad) { mn = q; bad = true; }
         else if(q < mn) mn = q;
     }
-    GMA float finish() const { return bad ? mn : (n ? 3.0f * (float)n / inv3 : 0.0f); }
+    GMA T finish() const { return bad ? mn : (n ? T(3) * (T)n / inv3 : T(0)); }
 };
 
 // ---- Tetrahedra: one corner; columns e_i3 = v3 - v_i (metric-free
 // segments), F = E Fr_TET_INV.  Moving vertex k: k == 3 adds x to every
 // column of E, k < 3 subtracts it from column k; r = Fr^T a.
-GMA Corner tetCorner(const v3<float> u[4], int k, uint32_t ref)
+template<typename T>
+GMA Corner<T> tetCorner(const v3<T> u[4], int k, uint32_t ref)
 {
-    const float R01 = -0.5773502691896257645091f, R11 = 1.154700538379251529018f;
-    const float R02 = -0.4082482904638630163662f, R12 = -0.4082482904638630163662f, R22 = 1.224744871391589049099f;
-    const v3<float> e0 = u[3] - u[0], e1 = u[3] - u[1], e2 = u[3] - u[2];
+    const T R01 = T(-0.5773502691896257645091), R11 = T(1.154700538379251529018);
+    const T R02 = T(-0.4082482904638630163662), R12 = T(-0.4082482904638630163662), R22 = T(1.224744871391589049099);
+    const v3<T> e0 = u[3] - u[0], e1 = u[3] - u[1], e2 = u[3] - u[2];
     // F columns: f0 = e0, f1 = R01 e0 + R11 e1, f2 = R02 e0 + R12 e1 + R22 e2.
-    const v3<float> f0 = e0;
-    const v3<float> f1 = e0 * R01 + e1 * R11;
-    const v3<float> f2 = e0 * R02 + e1 * R12 + e2 * R22;
-    const float a0 = k == 3 ? 1.0f : (k == 0 ? -1.0f : 0.0f);
-    const float a1 = k == 3 ? 1.0f : (k == 1 ? -1.0f : 0.0f);
-    const float a2 = k == 3 ? 1.0f : (k == 2 ? -1.0f : 0.0f);
+    const v3<T> f0 = e0;
+    const v3<T> f1 = e0 * R01 + e1 * R11;
+    const v3<T> f2 = e0 * R02 + e1 * R12 + e2 * R22;
+    const T a0 = k == 3 ? T(1) : (k == 0 ? T(-1) : T(0));
+    const T a1 = k == 3 ? T(1) : (k == 1 ? T(-1) : T(0));
+    const T a2 = k == 3 ? T(1) : (k == 2 ? T(-1) : T(0));
     // r_j = sum_i a_i R_ij (R upper triangular, R00 = 1).
-    const float r0 = a0;
-    const float r1 = a0 * R01 + a1 * R11;
-    const float r2 = a0 * R02 + a1 * R12 + a2 * R22;
-    Corner c = makeCorner(f0, f1, f2, r0, r1, r2);
-    c.ref = asFloat(ref);
+    const T r0 = a0;
+    const T r1 = a0 * R01 + a1 * R11;
+    const T r2 = a0 * R02 + a1 * R12 + a2 * R22;
+    Corner<T> c = makeCorner<T>(f0, f1, f2, r0, r1, r2);
+    c.ref = ref;
     return c;
 }
 
-// ---- Hexahedra: corner c has columns v[P] - v[M] for the (P, M) pairs
-// below (MeanRatioEvaluator::hexQuality corner frames).
-// Packed as 4 bits per vertex index: column j's P at bits 8j+4, M at 8j.
+// ---- Hexahedra: corner c has columns v[P] - v[M] for these (P, M) pairs
+// (MeanRatioEvaluator::hexQuality corner frames), packed as 4 bits per
+// vertex index: column j's P at bits 8j+4, M at 8j.
 GMA uint32_t hexCornerCode(int c)
 {
     switch(c)
@@ -172,26 +214,26 @@ GMA void hexCornerPairs(int c, int j, int& P, int& M)
 // Staged hexahedron with moving vertex k: the 4 corners that involve k, and
 // the 4 that do not folded into their 3/q sum (or their smallest quality
 // when one is non-positive).  `fix(corner)` returns a constant corner's FP64
-// determinant, used when it is near-degenerate.
+// determinant, used in FP32 when it is near-degenerate.
+template<typename T>
 struct alignas(16) Hex
 {
-    Corner k[4];
-    float constInv3;   // sum of 3/q over the constant corners (all positive)
-    float constMin;    // smallest constant-corner q if one is non-positive, else +1
-    float pad[2];
+    Corner<T> k[4];
+    T constInv3;   // sum of 3/q over the constant corners (all positive)
+    T constMin;    // smallest constant-corner q if one is non-positive, else +1
 };
 
-template<class Fix> GMA Hex stageHex(const v3<float> u[8], int k, uint32_t ref, const Fix& fix)
+template<typename T, class Fix> GMA Hex<T> stageHex(const v3<T> u[8], int k, uint32_t ref, const Fix& fix)
 {
-    Hex h;
+    Hex<T> h;
     int nv = 0;
-    float inv3 = 0.0f, mn = 1.0f;
+    T inv3 = T(0), mn = T(1);
     bool bad = false;
 #pragma unroll
     for(int c = 0; c < 8; ++c)
     {
-        v3<float> col[3];
-        float r[3];
+        v3<T> col[3];
+        T r[3];
         bool moves = false;
 #pragma unroll
         for(int j = 0; j < 3; ++j)
@@ -199,31 +241,30 @@ template<class Fix> GMA Hex stageHex(const v3<float> u[8], int k, uint32_t ref, 
             int P, M;
             hexCornerPairs(c, j, P, M);
             col[j] = u[P] - u[M];
-            r[j] = P == k ? 1.0f : (M == k ? -1.0f : 0.0f);
-            moves = moves || r[j] != 0.0f;
+            r[j] = P == k ? T(1) : (M == k ? T(-1) : T(0));
+            moves = moves || r[j] != T(0);
         }
-        Corner cc = makeCorner(col[0], col[1], col[2], r[0], r[1], r[2]);
-        cc.ref = asFloat(ref);
-        cc.corner = asFloat((uint32_t)c);
+        Corner<T> cc = makeCorner<T>(col[0], col[1], col[2], r[0], r[1], r[2]);
+        cc.ref = ref;
+        cc.corner = (uint32_t)c;
         if(moves)
         {
             if(nv < 4) h.k[nv] = cc;
             ++nv;
             continue;
         }
-        if(__builtin_fabsf(cc.d0) < cc.thr) cc.d0 = fix(cc);
-        if(cc.d0 > 0.0f)
-            inv3 += invQ3(cc.d0, cc.c0);
+        if(sizeof(T) == 4 && __builtin_fabsf((float)cc.d0) < cc.thr) cc.d0 = (T)fix(cc);
+        if(cc.d0 > T(0))
+            inv3 += invQ3<T>(cc.d0, cc.c0);
         else
         {
-            const float q = qNonPositive(cc.d0, cc.c0);
+            const T q = qNonPositive<T>(cc.d0, cc.c0);
             if(!bad || q < mn) mn = q;
             bad = true;
         }
     }
     h.constInv3 = inv3;
-    h.constMin = bad ? mn : 1.0f;
-    h.pad[0] = h.pad[1] = 0.0f;
+    h.constMin = bad ? mn : T(1);
     return h;
 }
 
@@ -231,57 +272,62 @@ template<class Fix> GMA Hex stageHex(const v3<float> u[8], int k, uint32_t ref, 
 // whether that assumption failed.  A patch whose elements are all positive
 // has quality 3 n / sum; otherwise the caller evaluates it again with the
 // exact rule (addTet / addHex), which is rare.
-GMA float fastInv3(const Corner& k, v3<float> x, float xx, bool& bad)
+template<typename T>
+GMA T fastInv3(const Corner<T>& k, v3<T> x, T xx, bool& bad)
 {
-    float d, f2;
-    cornerAt(k, x, xx, d, f2);
-    bad = bad || !(d > 0.0f);
-    return f2 * fexp2(-0.6666666666666666f * flog2(__builtin_fabsf(d)));
+    T d, f2;
+    cornerAt<T>(k, x, xx, d, f2);
+    bad = bad || !(d > T(0));
+    return f2 * powm23(d > T(0) ? d : T(1));
 }
 
-GMA float fastTet(const Corner& k, v3<float> x, float xx, bool& bad) { return fastInv3(k, x, xx, bad); }
+template<typename T>
+GMA T fastTet(const Corner<T>& k, v3<T> x, T xx, bool& bad) { return fastInv3<T>(k, x, xx, bad); }
 
-GMA float fastHex(const Hex& h, v3<float> x, float xx, bool& bad)
+template<typename T>
+GMA T fastHex(const Hex<T>& h, v3<T> x, T xx, bool& bad)
 {
-    float s = h.constInv3;
-    bad = bad || !(h.constMin > 0.0f);
+    T s = h.constInv3;
+    bad = bad || !(h.constMin > T(0));
 #pragma unroll
-    for(int c = 0; c < 4; ++c) s += fastInv3(h.k[c], x, xx, bad);
-    return 0.125f * s;
+    for(int c = 0; c < 4; ++c) s += fastInv3<T>(h.k[c], x, xx, bad);
+    return T(0.125) * s;
 }
 
 // Adds one staged tetrahedron at offset x to the accumulator.
-GMA void addTet(Acc& a, const Corner& k, v3<float> x, float xx)
+template<typename T>
+GMA void addTet(Acc<T>& a, const Corner<T>& k, v3<T> x, T xx)
 {
-    float d, f2;
-    cornerAt(k, x, xx, d, f2);
-    if(d > 0.0f) a.positive(invQ3(d, f2));
-    else a.nonPositive(qNonPositive(d, f2));
+    T d, f2;
+    cornerAt<T>(k, x, xx, d, f2);
+    if(d > T(0)) a.positive(invQ3<T>(d, f2));
+    else a.nonPositive(qNonPositive<T>(d, f2));
 }
 
 // Adds one staged hexahedron: its quality is 8 / sum(1/q_c) over its corners
 // when all are positive (so 3/q_hex = (1/8) sum 3/q_c), else its smallest
 // corner quality (MeanRatioEvaluator::hexQuality's mean-or-min).
-GMA void addHex(Acc& a, const Hex& h, v3<float> x, float xx)
+template<typename T>
+GMA void addHex(Acc<T>& a, const Hex<T>& h, v3<T> x, T xx)
 {
-    float inv3 = h.constInv3, mn = h.constMin;
-    bool bad = h.constMin <= 0.0f;
+    T inv3 = h.constInv3, mn = h.constMin;
+    bool bad = h.constMin <= T(0);
 #pragma unroll
     for(int c = 0; c < 4; ++c)
     {
-        float d, f2;
-        cornerAt(h.k[c], x, xx, d, f2);
-        if(d > 0.0f)
-            inv3 += invQ3(d, f2);
+        T d, f2;
+        cornerAt<T>(h.k[c], x, xx, d, f2);
+        if(d > T(0))
+            inv3 += invQ3<T>(d, f2);
         else
         {
-            const float q = qNonPositive(d, f2);
+            const T q = qNonPositive<T>(d, f2);
             if(!bad || q < mn) mn = q;
             bad = true;
         }
     }
     if(bad) a.nonPositive(mn);
-    else a.positive(0.125f * inv3);
+    else a.positive(T(0.125) * inv3);
 }
 
 #undef GMA

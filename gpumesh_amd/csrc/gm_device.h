@@ -107,6 +107,7 @@ struct VertexWork
     uint32_t count;            // vertices in the launch
     int kind;                  // EK_*
     int maxPatch;              // largest patch in the list
+    int polish;                // FP64 polish pass of the mixed-precision mode (not the exact FP64 mode)
 };
 
 // Block work: `count` blocks spread over `nspans` members.

@@ -504,23 +504,33 @@ __global__ void __launch_bounds__(512) gdLaneKernel(const DevMesh* __restrict__ 
 }
 
 // ---------------------------------------------------------------------------
-// FP32 mean ratio with the metric-free measurer: candidate-parallel kernels on
-// affine corner terms (gm_affine.h) staged relative to the vertex's position.
-// Same search as spawnLaneKernel / gdLaneKernel; trial points are offsets x
-// from the start position o, and the vertex moves to o + x.
+// Mean ratio with the metric-free measurer in the mixed-precision mode:
+// candidate-parallel kernels on affine corner terms (gm_affine.h) staged
+// relative to the vertex's position, in FP32 for the bulk passes and in FP64
+// for the polish pass.  Same searches as spawnLaneKernel / gdLaneKernel /
+// Nelder-Mead; trial points are offsets x from the start position o, and the
+// vertex moves to o + x.  (The exact FP64 mode keeps the kernels above, whose
+// operation order matches the oracle bit for bit.)
 // ---------------------------------------------------------------------------
-template<typename T, int EV, int SA> struct Affine { static const bool on = false; };
-template<> struct Affine<float, gmq::EV_MEAN_RATIO, gmq::SA_UNIFORM> { static const bool on = true; };
+template<int EV, int SA> struct AffineCrew { static const bool on = false; };
+template<> struct AffineCrew<gmq::EV_MEAN_RATIO, gmq::SA_UNIFORM> { static const bool on = true; };
 
-template<int EK> __host__ __device__ inline size_t affineBytes()
+// True when the list runs on the affine kernels: every FP32 pass, and the
+// FP64 polish pass, of the mean-ratio / metric-free crew.
+template<typename T, int EV, int SA> __host__ __device__ inline bool useAffine(const VertexWork& w)
 {
-    return EK == 0 ? sizeof(gma::Corner) : sizeof(gma::Hex);
+    return AffineCrew<EV, SA>::on && (sizeof(T) == 4 || w.polish);
 }
 
-// LDS bytes of `elems` staged elements of kind EK in the (T, EV, SA) kernels.
-template<typename T, int EV, int SA, int EK> __host__ __device__ inline size_t laneStageBytes(int elems)
+template<typename T, int EK> __host__ __device__ inline size_t affineBytes()
 {
-    return Affine<T, EV, SA>::on ? (size_t)elems * affineBytes<EK>() : stageBytes<T, EK>(elems);
+    return EK == 0 ? sizeof(gma::Corner<T>) : sizeof(gma::Hex<T>);
+}
+
+// LDS bytes of `elems` staged elements of kind EK.
+template<typename T, int EK> __host__ __device__ inline size_t laneStageBytes(int elems, bool affine)
+{
+    return affine ? (size_t)elems * affineBytes<T, EK>() : stageBytes<T, EK>(elems);
 }
 
 // FP64 determinant of a corner, from the FP32 positions with the moving
@@ -538,9 +548,9 @@ struct DetFix
         return mk<double>((double)q.x, (double)q.y, (double)q.z);
     }
 
-    __device__ float det64(const gma::Corner& k, v3<float> pf) const
+    __device__ float det64(const gma::Corner<float>& k, v3<float> pf) const
     {
-        const uint32_t ref = gma::asBits(k.ref);
+        const uint32_t ref = k.ref;
         const uint32_t id = ref >> 4;
         const int kk = (int)(ref & 7);
         const v3<double> p = mk<double>((double)pf.x, (double)pf.y, (double)pf.z);
@@ -555,7 +565,7 @@ struct DetFix
         }
         const uint4 a = m->hexs[2 * id], b = m->hexs[2 * id + 1];
         const uint32_t ids[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
-        const uint32_t code = gma::hexCornerCode((int)gma::asBits(k.corner));
+        const uint32_t code = gma::hexCornerCode((int)k.corner);
         v3<double> col[3];
 #pragma unroll
         for(int j = 0; j < 3; ++j)
@@ -574,221 +584,238 @@ struct DetFix
     }
 };
 
-// Constant corners of a staged hexahedron: the moving vertex at its position.
-template<int EK>
+// Constant corners of a staged hexahedron: the moving vertex at its position
+// (FP32); FP64 staging needs no recheck.
+template<typename T, int EK>
 struct ConstFix
 {
-    DetFix<EK> d;
-    v3<float> o;
-    __device__ float operator()(const gma::Corner& k) const { return d.det64(k, o); }
+    const DevMesh* m;
+    const float4* pos32;
+    v3<T> o;
+    __device__ float operator()(const gma::Corner<T>& k) const
+    {
+        if constexpr(sizeof(T) == 4)
+            return DetFix<EK>{m, pos32}.det64(k, o);
+        else
+            return (float)k.d0;
+    }
 };
 
-// FP64 inversion test of a chosen move (one staged element): corners FP32
-// calls non-positive set `nonPos32`; corners FP32 calls positive but inside
-// the near-degenerate band are recomputed in FP64, and set `flip` when FP64
-// finds them non-positive.
+// FP64 inversion test of a chosen FP32 move (one staged element): corners
+// FP32 calls non-positive set `nonPos32`; corners FP32 calls positive but
+// inside the near-degenerate band are recomputed in FP64, and set `flip`
+// when FP64 finds them non-positive.
 template<int EK>
 __device__ __forceinline__ void inversionTest(const unsigned char* sp, int e, v3<float> x, v3<float> pf,
                                               const DetFix<EK>& fix, bool& nonPos32, bool& flip)
 {
-    auto one = [&](const gma::Corner& k) {
+    auto one = [&](const gma::Corner<float>& k) {
         const float d = k.d0 + (k.gx * x.x + k.gy * x.y + k.gz * x.z);
         if(!(d > 0.0f)) nonPos32 = true;
         else if(d < k.thr && !(fix.det64(k, pf) > 0.0f)) flip = true;
     };
     if(EK == 0)
-        one(((const gma::Corner*)sp)[e]);
+        one(((const gma::Corner<float>*)sp)[e]);
     else
     {
-        const gma::Hex& h = ((const gma::Hex*)sp)[e];
+        const gma::Hex<float>& h = ((const gma::Hex<float>*)sp)[e];
         for(int c = 0; c < 4; ++c) one(h.k[c]);
         if(h.constMin <= 0.0f) nonPos32 = true;
     }
 }
 
-template<int EK>
-__device__ __forceinline__ void stageAffine(const DevMesh& m, const float4* pos, uint32_t ref, v3<float> o,
-                                            unsigned char* dst)
+// Applies the FP64 inversion test to a move of the vertex whose lanes are the
+// `width` lanes starting at `first` (within the wavefront); returns the
+// position to store.  FP64 moves need no test.
+template<typename T, int EK>
+__device__ __forceinline__ v3<T> testedMove(const DevMesh& m, const unsigned char* sp, int nE, int lane, int width,
+                                            int first, v3<T> o, v3<T> x)
+{
+    const v3<T> pf = o + x;
+    if constexpr(sizeof(T) == 8)
+        return pf;
+    else
+    {
+        const DetFix<EK> fix{&m, m.pos32};
+        bool nonPos32 = false, flip = false;
+        for(int e = lane; e < nE; e += width)
+            inversionTest<EK>(sp, e, x, pf, fix, nonPos32, flip);
+        const unsigned long long mask = width == 64 ? ~0ull : ((1ull << width) - 1);
+        const bool reject = ((__ballot(flip) >> first) & mask) != 0 && ((__ballot(nonPos32) >> first) & mask) == 0;
+        return reject ? o : pf;
+    }
+}
+
+template<typename T, int EK>
+__device__ __forceinline__ void stageAffine(const DevMesh& m, const typename PosOf<T>::type* pos, uint32_t ref,
+                                            v3<T> o, unsigned char* dst)
 {
     const uint32_t id = ref >> 4;
     const int k = (int)(ref & 7);
     if(EK == 0)
     {
         const uint4 t = m.tets[id];
-        v3<float> u[4] = {ld<float>(pos, t.x) - o, ld<float>(pos, t.y) - o, ld<float>(pos, t.z) - o,
-                          ld<float>(pos, t.w) - o};
-        *(gma::Corner*)dst = gma::tetCorner(u, k, ref);
+        v3<T> u[4] = {ld<T>(pos, t.x) - o, ld<T>(pos, t.y) - o, ld<T>(pos, t.z) - o, ld<T>(pos, t.w) - o};
+        *(gma::Corner<T>*)dst = gma::tetCorner<T>(u, k, ref);
     }
     else
     {
         const uint4 a = m.hexs[2 * id], b = m.hexs[2 * id + 1];
-        v3<float> u[8] = {ld<float>(pos, a.x) - o, ld<float>(pos, a.y) - o, ld<float>(pos, a.z) - o,
-                          ld<float>(pos, a.w) - o, ld<float>(pos, b.x) - o, ld<float>(pos, b.y) - o,
-                          ld<float>(pos, b.z) - o, ld<float>(pos, b.w) - o};
-        *(gma::Hex*)dst = gma::stageHex(u, k, ref, ConstFix<1>{DetFix<1>{&m, pos}, o});
+        v3<T> u[8] = {ld<T>(pos, a.x) - o, ld<T>(pos, a.y) - o, ld<T>(pos, a.z) - o, ld<T>(pos, a.w) - o,
+                      ld<T>(pos, b.x) - o, ld<T>(pos, b.y) - o, ld<T>(pos, b.z) - o, ld<T>(pos, b.w) - o};
+        *(gma::Hex<T>*)dst = gma::stageHex<T>(u, k, ref, ConstFix<T, 1>{&m, m.pos32, o});
     }
 }
 
 // Patch quality assuming every element is positive (branch-free); `bad`
 // reports that the assumption failed.
-template<int EK>
-__device__ __forceinline__ float affinePatchFast(const unsigned char* sp, int nE, v3<float> x, float xx, bool& bad)
+template<typename T, int EK>
+__device__ __forceinline__ T affinePatchFast(const unsigned char* sp, int nE, v3<T> x, T xx, bool& bad)
 {
-    float inv3 = 0.0f;
+    T inv3 = T(0);
     for(int e = 0; e < nE; ++e)
     {
-        if(EK == 0) inv3 += gma::fastTet(((const gma::Corner*)sp)[e], x, xx, bad);
-        else inv3 += gma::fastHex(((const gma::Hex*)sp)[e], x, xx, bad);
+        if(EK == 0) inv3 += gma::fastTet<T>(((const gma::Corner<T>*)sp)[e], x, xx, bad);
+        else inv3 += gma::fastHex<T>(((const gma::Hex<T>*)sp)[e], x, xx, bad);
     }
-    return 3.0f * (float)nE / inv3;
+    return T(3) * (T)nE / inv3;
 }
 
 // The exact patch rule (smallest non-positive quality when one exists).
-template<int EK>
-__device__ __forceinline__ float affinePatchExact(const unsigned char* sp, int nE, v3<float> x, float xx)
+template<typename T, int EK>
+__device__ __forceinline__ T affinePatchExact(const unsigned char* sp, int nE, v3<T> x, T xx)
 {
-    gma::Acc a;
+    gma::Acc<T> a;
     for(int e = 0; e < nE; ++e)
     {
-        if(EK == 0) gma::addTet(a, ((const gma::Corner*)sp)[e], x, xx);
-        else gma::addHex(a, ((const gma::Hex*)sp)[e], x, xx);
+        if(EK == 0) gma::addTet<T>(a, ((const gma::Corner<T>*)sp)[e], x, xx);
+        else gma::addHex<T>(a, ((const gma::Hex<T>*)sp)[e], x, xx);
     }
     return a.finish();
 }
 
-template<int EK>
-__device__ __forceinline__ float affinePatch(const unsigned char* sp, int nE, v3<float> x)
+template<typename T, int EK>
+__device__ __forceinline__ T affinePatch(const unsigned char* sp, int nE, v3<T> x)
 {
-    const float xx = gmq::dot(x, x);
+    const T xx = gmq::dot(x, x);
     bool bad = false;
-    const float q = affinePatchFast<EK>(sp, nE, x, xx, bad);
-    return bad ? affinePatchExact<EK>(sp, nE, x, xx) : q;
+    const T q = affinePatchFast<T, EK>(sp, nE, x, xx, bad);
+    return bad ? affinePatchExact<T, EK>(sp, nE, x, xx) : q;
 }
 
-template<int EK>
+template<typename T, int EK>
 __global__ void __launch_bounds__(512) spawnAffineKernel(const DevMesh* __restrict__ meshes,
                                                          const uint4* __restrict__ segs, uint32_t nsegs,
-                                                         uint32_t count, float moveCoeff, int cap)
+                                                         uint32_t count, T moveCoeff, int cap)
 {
+    typedef typename PosOf<T>::type P;
     extern __shared__ __align__(16) unsigned char smem[];
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int ppg = blockDim.x >> 6;
-    unsigned char* sp = smem + (size_t)wave * cap * affineBytes<EK>();
+    unsigned char* sp = smem + (size_t)wave * cap * affineBytes<T, EK>();
     const uint32_t i = blockIdx.x * ppg + wave;
     const bool present = i < count;
     const uint2 it = present ? workItem(meshes, segs, nsegs, i) : make_uint2(0, 0);
     const DevMesh& m = meshes[it.x];
-    float4* pos = m.pos32;
+    P* pos = PosOf<T>::get(m);
     const uint32_t v = it.y;
     const uint32_t eb = m.eoff[v];
     const int nE = present ? (int)(m.eoff[v + 1] - eb) : 0;
-    const v3<float> o = present ? ld<float>(pos, v) : mk<float>(0.0f, 0.0f, 0.0f);
+    const v3<T> o = present ? ld<T>(pos, v) : mk<T>(T(0), T(0), T(0));
     for(int e = lane; e < nE; e += 64)
-        stageAffine<EK>(m, pos, m.eref[eb + e], o, sp + (size_t)e * affineBytes<EK>());
+        stageAffine<T, EK>(m, pos, m.eref[eb + e], o, sp + (size_t)e * affineBytes<T, EK>());
     __syncthreads();
     if(!present)
         return;
-    float scale = moveCoeff * localSize<float>(m, pos, v, o);
-    v3<float> xb = mk<float>(0.0f, 0.0f, 0.0f);
+    T scale = moveCoeff * localSize<T>(m, pos, v, o);
+    v3<T> xb = mk<T>(T(0), T(0), T(0));
     for(int iter = 0; iter < 2; ++iter)
     {
-        const v3<float> x = xb + gms::spawnOffset<float>(lane) * scale;
+        const v3<T> x = xb + gms::spawnOffset<T>(lane) * scale;
         // A candidate with a non-positive element has quality <= 0 and loses
         // to any candidate without one; its exact value matters only when
         // every candidate of the vertex has one.
-        const float xx = gmq::dot(x, x);
+        const T xx = gmq::dot(x, x);
         bool bad = false;
-        float q = affinePatchFast<EK>(sp, nE, x, xx, bad);
+        T q = affinePatchFast<T, EK>(sp, nE, x, xx, bad);
         if(__ballot(!bad) == 0)
-            q = affinePatchExact<EK>(sp, nE, x, xx);
+            q = affinePatchExact<T, EK>(sp, nE, x, xx);
         else if(bad)
-            q = -__builtin_huge_valf();
-        const int best = groupArgmax<float>(q, lane, 64);
-        xb = shflv<float>(x, best);
-        scale /= 3.0f;
+            q = -(T)__builtin_huge_val();
+        const int best = groupArgmax<T>(q, lane, 64);
+        xb = shflv<T>(x, best);
+        scale /= T(3);
     }
-    // FP64 inversion test of the move, one element per lane.
-    const v3<float> pf = o + xb;
-    const DetFix<EK> fix{&m, pos};
-    bool nonPos32 = false, flip = false;
-    for(int e = lane; e < nE; e += 64)
-        inversionTest<EK>(sp, e, xb, pf, fix, nonPos32, flip);
-    const bool reject = __ballot(flip) != 0 && __ballot(nonPos32) == 0;
+    const v3<T> np = testedMove<T, EK>(m, sp, nE, lane, 64, 0, o, xb);
     if(lane == 0)
     {
-        st(pos, v, reject ? o : pf);
+        st(pos, v, np);
         m.evalCounts[v] += 128;
     }
 }
 
-template<int EK>
+template<typename T, int EK>
 __global__ void __launch_bounds__(512) gdAffineKernel(const DevMesh* __restrict__ meshes,
                                                       const uint4* __restrict__ segs, uint32_t nsegs,
                                                       uint32_t count, int cap)
 {
+    typedef typename PosOf<T>::type P;
     extern __shared__ __align__(16) unsigned char smem[];
     const int lane = threadIdx.x;
     const int g = lane >> 3, s = lane & 7, base = lane & ~7;
-    unsigned char* mySp = smem + (size_t)g * cap * affineBytes<EK>();
+    unsigned char* mySp = smem + (size_t)g * cap * affineBytes<T, EK>();
     const uint32_t i = blockIdx.x * (blockDim.x >> 3) + g;
     const bool present = i < count;
     const uint2 it = present ? workItem(meshes, segs, nsegs, i) : make_uint2(0, 0);
     const DevMesh& m = meshes[it.x];
-    float4* pos = m.pos32;
+    P* pos = PosOf<T>::get(m);
     const uint32_t v = it.y;
     const uint32_t eb = m.eoff[v];
     const int nE = present ? (int)(m.eoff[v + 1] - eb) : 0;
-    const v3<float> o = present ? ld<float>(pos, v) : mk<float>(0.0f, 0.0f, 0.0f);
+    const v3<T> o = present ? ld<T>(pos, v) : mk<T>(T(0), T(0), T(0));
     for(int e = s; e < nE; e += 8)
-        stageAffine<EK>(m, pos, m.eref[eb + e], o, mySp + (size_t)e * affineBytes<EK>());
+        stageAffine<T, EK>(m, pos, m.eref[eb + e], o, mySp + (size_t)e * affineBytes<T, EK>());
     __syncthreads();
     if(!present)
         return;
 
-    v3<float> x = mk<float>(0.0f, 0.0f, 0.0f);
-    float nodeShift = localSize<float>(m, pos, v, o) * (float)(1.0 / 75.0);
-    const float originalNodeShift = nodeShift;
+    v3<T> x = mk<T>(T(0), T(0), T(0));
+    T nodeShift = localSize<T>(m, pos, v, o) * T(1.0 / 75.0);
+    const T originalNodeShift = nodeShift;
     int evals = 0;
     for(int cyc = 0; cyc < 5; ++cyc)
     {
-        const float sx = s == 0 ? -nodeShift : (s == 1 ? nodeShift : 0.0f);
-        const float sy = s == 2 ? -nodeShift : (s == 3 ? nodeShift : 0.0f);
-        const float sz = s == 4 ? -nodeShift : (s == 5 ? nodeShift : 0.0f);
-        float q = 0.0f;
+        const T sx = s == 0 ? -nodeShift : (s == 1 ? nodeShift : T(0));
+        const T sy = s == 2 ? -nodeShift : (s == 3 ? nodeShift : T(0));
+        const T sz = s == 4 ? -nodeShift : (s == 5 ? nodeShift : T(0));
+        T q = T(0);
         if(s < 6)
-            q = affinePatch<EK>(mySp, nE, x + mk<float>(sx, sy, sz));
+            q = affinePatch<T, EK>(mySp, nE, x + mk<T>(sx, sy, sz));
         evals += 6;
-        const float q0 = __shfl(q, base + 0, 64), q1 = __shfl(q, base + 1, 64), q2 = __shfl(q, base + 2, 64);
-        const float q3 = __shfl(q, base + 3, 64), q4 = __shfl(q, base + 4, 64), q5 = __shfl(q, base + 5, 64);
-        const v3<float> gq = mk<float>(q1 - q0, q3 - q2, q5 - q4);
-        const float gn = gmq::tsqrt(gmq::dot(gq, gq));
-        if(gn == 0.0f)
+        const T q0 = __shfl(q, base + 0, 64), q1 = __shfl(q, base + 1, 64), q2 = __shfl(q, base + 2, 64);
+        const T q3 = __shfl(q, base + 3, 64), q4 = __shfl(q, base + 4, 64), q5 = __shfl(q, base + 5, 64);
+        const v3<T> gq = mk<T>(q1 - q0, q3 - q2, q5 - q4);
+        const T gn = gmq::tsqrt(gmq::dot(gq, gq));
+        if(gn == T(0))
             break;
-        const v3<float> shift = gq * (nodeShift / gn);
-        const float off = s == 0 ? -0.25f : s == 1 ? 0.00f : s == 2 ? 0.10f : s == 3 ? 0.20f
-                        : s == 4 ? 0.40f : s == 5 ? 0.80f : s == 6 ? 1.20f : 1.60f;
-        const v3<float> prop = x + shift * off;
-        const float qp = affinePatch<EK>(mySp, nE, prop);
+        const v3<T> shift = gq * (nodeShift / gn);
+        const T off = s == 0 ? T(-0.25) : s == 1 ? T(0.00) : s == 2 ? T(0.10) : s == 3 ? T(0.20)
+                    : s == 4 ? T(0.40) : s == 5 ? T(0.80) : s == 6 ? T(1.20) : T(1.60);
+        const v3<T> prop = x + shift * off;
+        const T qp = affinePatch<T, EK>(mySp, nE, prop);
         evals += 8;
-        const int best = groupArgmax<float>(qp, s, 8);
-        x = shflv<float>(prop, (base & 63) + best);
-        const float bestOff = __shfl(off, (base & 63) + best, 64);
+        const int best = groupArgmax<T>(qp, s, 8);
+        x = shflv<T>(prop, (base & 63) + best);
+        const T bestOff = __shfl(off, (base & 63) + best, 64);
         nodeShift *= gmq::tabs(bestOff);
-        if(nodeShift < originalNodeShift / 10.0f)
+        if(nodeShift < originalNodeShift / T(10))
             break;
     }
-    // FP64 inversion test of the move over the vertex's 8 lanes.
-    const v3<float> pf = o + x;
-    const DetFix<EK> fix{&m, pos};
-    bool nonPos32 = false, flip = false;
-    for(int e = s; e < nE; e += 8)
-        inversionTest<EK>(mySp, e, x, pf, fix, nonPos32, flip);
-    const int sh = (threadIdx.x & 63) & ~7;
-    const bool reject = ((__ballot(flip) >> sh) & 0xFFull) != 0 && ((__ballot(nonPos32) >> sh) & 0xFFull) == 0;
+    const v3<T> np = testedMove<T, EK>(m, mySp, nE, s, 8, (threadIdx.x & 63) & ~7, o, x);
     if(s == 0)
     {
-        st(pos, v, reject ? o : pf);
+        st(pos, v, np);
         m.evalCounts[v] += evals;
     }
 }
@@ -796,25 +823,25 @@ __global__ void __launch_bounds__(512) gdAffineKernel(const DevMesh* __restrict_
 // Nelder-Mead on affine terms: one vertex per group of G lanes, lane j holding
 // element j's staged corner terms in registers; a patch evaluation is one
 // affine element evaluation per lane and a butterfly over the group.
-template<int EK, int G>
+template<typename T, int EK, int G>
 struct AffineWavePatch
 {
     int lane;
     int nElem;
     int count;
-    v3<float> o;
-    gma::Corner cn;   // EK == 0
-    gma::Hex hx;      // EK == 1
+    v3<T> o;
+    gma::Corner<T> cn;   // EK == 0
+    gma::Hex<T> hx;      // EK == 1
 
-    __device__ float operator()(v3<float> p)
+    __device__ T operator()(v3<T> p)
     {
         ++count;
-        const v3<float> x = p - o;
-        const float xx = gmq::dot(x, x);
-        float sum = 0.0f;
+        const v3<T> x = p - o;
+        const T xx = gmq::dot(x, x);
+        T sum = T(0);
         bool badLane = false;
         if(lane < nElem)
-            sum = EK == 0 ? gma::fastTet(cn, x, xx, badLane) : gma::fastHex(hx, x, xx, badLane);
+            sum = EK == 0 ? gma::fastTet<T>(cn, x, xx, badLane) : gma::fastHex<T>(hx, x, xx, badLane);
         int bad = badLane ? 1 : 0;
 #pragma unroll
         for(int off = G / 2; off > 0; off >>= 1)
@@ -823,30 +850,31 @@ struct AffineWavePatch
             bad |= __shfl_xor(bad, off, 64);
         }
         if(!bad)
-            return 3.0f * (float)nElem / sum;
+            return T(3) * (T)nElem / sum;
         // A non-positive element: the smallest non-positive quality.
-        gma::Acc a;
+        gma::Acc<T> a;
         if(lane < nElem)
         {
-            if(EK == 0) gma::addTet(a, cn, x, xx);
-            else gma::addHex(a, hx, x, xx);
+            if(EK == 0) gma::addTet<T>(a, cn, x, xx);
+            else gma::addHex<T>(a, hx, x, xx);
         }
-        float mn = (a.bad && a.mn == a.mn) ? a.mn : __builtin_huge_valf();
+        T mn = (a.bad && a.mn == a.mn) ? a.mn : (T)__builtin_huge_val();
 #pragma unroll
         for(int off = G / 2; off > 0; off >>= 1)
         {
-            const float om = __shfl_xor(mn, off, 64);
+            const T om = __shfl_xor(mn, off, 64);
             mn = om < mn ? om : mn;
         }
         return mn;
     }
 };
 
-template<int EK, int G>
+template<typename T, int EK, int G>
 __global__ void __launch_bounds__(64) nmAffineWaveKernel(const DevMesh* __restrict__ meshes,
                                                          const uint4* __restrict__ segs, uint32_t nsegs,
                                                          uint32_t count)
 {
+    typedef typename PosOf<T>::type P;
     const int lane = threadIdx.x % G;
     const int grp = threadIdx.x / G;
     const uint32_t i = blockIdx.x * (64 / G) + grp;
@@ -854,67 +882,53 @@ __global__ void __launch_bounds__(64) nmAffineWaveKernel(const DevMesh* __restri
         return;
     const uint2 it = workItem(meshes, segs, nsegs, i);
     const DevMesh& m = meshes[it.x];
-    float4* pos = m.pos32;
+    P* pos = PosOf<T>::get(m);
     const uint32_t v = it.y;
     const uint32_t eb = m.eoff[v], ee = m.eoff[v + 1];
-    AffineWavePatch<EK, G> f;
+    AffineWavePatch<T, EK, G> f;
     f.lane = lane;
     f.nElem = (int)(ee - eb);
     f.count = 0;
-    f.o = ld<float>(pos, v);
+    f.o = ld<T>(pos, v);
     if(lane < f.nElem)
+        stageAffine<T, EK>(m, pos, m.eref[eb + lane], f.o,
+                           EK == 0 ? (unsigned char*)&f.cn : (unsigned char*)&f.hx);
+    const T ls = localSize<T>(m, pos, v, f.o);
+    const v3<T> np = gms::nelderMead<T>(f.o, ls, f);
+    // FP64 inversion test of an FP32 move, one element per lane of the group.
+    v3<T> out = np;
+    if constexpr(sizeof(T) == 4)
     {
-        const uint32_t ref = m.eref[eb + lane];
-        const uint32_t id = ref >> 4;
-        const int k = (int)(ref & 7);
-        if(EK == 0)
+        bool nonPos32 = false, flip = false;
+        if(lane < f.nElem)
         {
-            const uint4 t = m.tets[id];
-            v3<float> u[4] = {ld<float>(pos, t.x) - f.o, ld<float>(pos, t.y) - f.o, ld<float>(pos, t.z) - f.o,
-                              ld<float>(pos, t.w) - f.o};
-            f.cn = gma::tetCorner(u, k, ref);
+            const DetFix<EK> fix{&m, m.pos32};
+            inversionTest<EK>(EK == 0 ? (const unsigned char*)&f.cn : (const unsigned char*)&f.hx, 0, np - f.o, np,
+                              fix, nonPos32, flip);
         }
-        else
-        {
-            const uint4 a = m.hexs[2 * id], b = m.hexs[2 * id + 1];
-            v3<float> u[8] = {ld<float>(pos, a.x) - f.o, ld<float>(pos, a.y) - f.o, ld<float>(pos, a.z) - f.o,
-                              ld<float>(pos, a.w) - f.o, ld<float>(pos, b.x) - f.o, ld<float>(pos, b.y) - f.o,
-                              ld<float>(pos, b.z) - f.o, ld<float>(pos, b.w) - f.o};
-            f.hx = gma::stageHex(u, k, ref, ConstFix<1>{DetFix<1>{&m, pos}, f.o});
-        }
+        const int sh = (threadIdx.x & 63) & ~(G - 1);
+        const unsigned long long gmask = G == 64 ? ~0ull : ((1ull << G) - 1);
+        const bool reject = ((__ballot(flip) >> sh) & gmask) != 0 && ((__ballot(nonPos32) >> sh) & gmask) == 0;
+        out = reject ? f.o : np;
     }
-    const float ls = localSize<float>(m, pos, v, f.o);
-    const v3<float> np = gms::nelderMead<float>(f.o, ls, f);
-    // FP64 inversion test of the move, one element per lane of the group.
-    bool nonPos32 = false, flip = false;
-    if(lane < f.nElem)
-    {
-        const DetFix<EK> fix{&m, pos};
-        const v3<float> x = np - f.o;
-        inversionTest<EK>(EK == 0 ? (const unsigned char*)&f.cn : (const unsigned char*)&f.hx, 0, x, np, fix,
-                          nonPos32, flip);
-    }
-    const int sh = (threadIdx.x & 63) & ~(G - 1);
-    const unsigned long long gmask = G == 64 ? ~0ull : ((1ull << G) - 1);
-    const bool reject = ((__ballot(flip) >> sh) & gmask) != 0 && ((__ballot(nonPos32) >> sh) & gmask) == 0;
     if(lane == 0)
     {
-        st(pos, v, reject ? f.o : np);
+        st(pos, v, out);
         m.evalCounts[v] += f.count;
     }
 }
 
-template<int EK>
+template<typename T, int EK>
 static void launchNmAffine(const VertexWork& w, hipStream_t stream)
 {
     if(w.maxPatch <= 8)
-        nmAffineWaveKernel<EK, 8><<<(w.count + 7) / 8, 64, 0, stream>>>(w.meshes, w.segs, w.nsegs, w.count);
+        nmAffineWaveKernel<T, EK, 8><<<(w.count + 7) / 8, 64, 0, stream>>>(w.meshes, w.segs, w.nsegs, w.count);
     else if(w.maxPatch <= 16)
-        nmAffineWaveKernel<EK, 16><<<(w.count + 3) / 4, 64, 0, stream>>>(w.meshes, w.segs, w.nsegs, w.count);
+        nmAffineWaveKernel<T, EK, 16><<<(w.count + 3) / 4, 64, 0, stream>>>(w.meshes, w.segs, w.nsegs, w.count);
     else if(w.maxPatch <= 32)
-        nmAffineWaveKernel<EK, 32><<<(w.count + 1) / 2, 64, 0, stream>>>(w.meshes, w.segs, w.nsegs, w.count);
+        nmAffineWaveKernel<T, EK, 32><<<(w.count + 1) / 2, 64, 0, stream>>>(w.meshes, w.segs, w.nsegs, w.count);
     else
-        nmAffineWaveKernel<EK, 64><<<w.count, 64, 0, stream>>>(w.meshes, w.segs, w.nsegs, w.count);
+        nmAffineWaveKernel<T, EK, 64><<<w.count, 64, 0, stream>>>(w.meshes, w.segs, w.nsegs, w.count);
 }
 
 // ---------------------------------------------------------------------------
@@ -1324,31 +1338,36 @@ static hipError_t launchLane(const VertexWork& w, const gmq::Crew& c, LaneShape 
                              hipStream_t stream)
 {
     const int ppg = lane.patchesPerGroup;
-    const bool aff = Affine<T, EV, SA>::on;
+    const bool aff = useAffine<T, EV, SA>(w);
     if(SM == SM_SPAWN_SEARCH)
     {
         // LDS tile per block = ppg wavefronts x tileElems elements.
         const int cap = lane.tileElems;
-        const size_t lds = (size_t)ppg * laneStageBytes<T, EV, SA, EK>(cap);
+        const size_t lds = (size_t)ppg * laneStageBytes<T, EK>(cap, aff);
         const uint32_t blocks = (w.count + ppg - 1) / ppg;
-        if(aff)
-            spawnAffineKernel<EK><<<blocks, 64 * ppg, lds, stream>>>(w.meshes, w.segs, w.nsegs, w.count,
-                                                                     (float)moveCoeff, cap);
-        else
-            spawnLaneKernel<T, EV, SA, EK><<<blocks, 64 * ppg, lds, stream>>>(w.meshes, w.segs, w.nsegs, w.count, c,
-                                                                              (T)moveCoeff, cap);
+        if constexpr(AffineCrew<EV, SA>::on)
+            if(aff)
+            {
+                spawnAffineKernel<T, EK><<<blocks, 64 * ppg, lds, stream>>>(w.meshes, w.segs, w.nsegs, w.count,
+                                                                            (T)moveCoeff, cap);
+                return hipGetLastError();
+            }
+        spawnLaneKernel<T, EV, SA, EK><<<blocks, 64 * ppg, lds, stream>>>(w.meshes, w.segs, w.nsegs, w.count, c,
+                                                                          (T)moveCoeff, cap);
     }
     else
     {
         // 8 vertices per wavefront share the wavefront's tile.
         const int cap = lane.tileElems / 8;
-        const size_t lds = (size_t)8 * ppg * laneStageBytes<T, EV, SA, EK>(cap);
+        const size_t lds = (size_t)8 * ppg * laneStageBytes<T, EK>(cap, aff);
         const uint32_t blocks = (w.count + 8 * ppg - 1) / (8 * ppg);
-        if(aff)
-            gdAffineKernel<EK><<<blocks, 64 * ppg, lds, stream>>>(w.meshes, w.segs, w.nsegs, w.count, cap);
-        else
-            gdLaneKernel<T, EV, SA, EK><<<blocks, 64 * ppg, lds, stream>>>(w.meshes, w.segs, w.nsegs, w.count, c,
-                                                                           cap);
+        if constexpr(AffineCrew<EV, SA>::on)
+            if(aff)
+            {
+                gdAffineKernel<T, EK><<<blocks, 64 * ppg, lds, stream>>>(w.meshes, w.segs, w.nsegs, w.count, cap);
+                return hipGetLastError();
+            }
+        gdLaneKernel<T, EV, SA, EK><<<blocks, 64 * ppg, lds, stream>>>(w.meshes, w.segs, w.nsegs, w.count, c, cap);
     }
     return hipGetLastError();
 }
@@ -1363,8 +1382,8 @@ inline bool laneFits(const VertexWork& w, int smoother, LaneShape lane)
     const int cap = smoother == SM_SPAWN_SEARCH ? lane.tileElems : lane.tileElems / 8;
     if(cap < w.maxPatch || ppg < 1 || ppg > 8)
         return false;
-    const size_t per = w.kind == EK_TET ? laneStageBytes<T, EV, SA, EK_TET>(cap)
-                                        : laneStageBytes<T, EV, SA, EK_HEX>(cap);
+    const bool aff = useAffine<T, EV, SA>(w);
+    const size_t per = w.kind == EK_TET ? laneStageBytes<T, EK_TET>(cap, aff) : laneStageBytes<T, EK_HEX>(cap, aff);
     const size_t lds = (smoother == SM_SPAWN_SEARCH ? (size_t)ppg : (size_t)8 * ppg) * per;
     return lds <= 64 * 1024;
 }
@@ -1375,12 +1394,12 @@ static hipError_t launchSmoothTyped(const VertexWork& w, const gmq::Crew& c, int
 {
     if(w.count == 0)
         return hipSuccess;
-    if constexpr(Affine<T, EV, SA>::on && SM == SM_NELDER_MEAD)
+    if constexpr(AffineCrew<EV, SA>::on && SM == SM_NELDER_MEAD)
     {
-        if(shape == KS_WAVE && w.maxPatch <= 64 && w.kind != EK_MIXED)
+        if(shape == KS_WAVE && useAffine<T, EV, SA>(w) && w.maxPatch <= 64 && w.kind != EK_MIXED)
         {
-            if(w.kind == EK_TET) launchNmAffine<EK_TET>(w, stream);
-            else launchNmAffine<EK_HEX>(w, stream);
+            if(w.kind == EK_TET) launchNmAffine<T, EK_TET>(w, stream);
+            else launchNmAffine<T, EK_HEX>(w, stream);
             return hipGetLastError();
         }
     }

@@ -41,7 +41,7 @@ struct RunParams
     int precision = GM_FP64;
     int kernel = GM_KERNEL_WAVE;
     double spawnCoeff = 1.0 / 20.0;
-    gmd::LaneShape lane{1, 64};
+    gmd::LaneShape lane{1, 0};   // tileElems 0: chosen per work list
     bool graph = false;
     bool timeKernels = true;   // per-launch events (not inside graphs)
 };

@@ -70,10 +70,11 @@ hipError_t launchResetRun(const DevMesh* meshes, BlockWork vb, const uint32_t* m
 
 bool laneFits(const VertexWork& w, int smoother, LaneShape lane, bool fp64, int evaluator, int sampler)
 {
-    if(fp64) return laneFits<double, 0, 0>(w, smoother, lane);
-    if(evaluator == gmq::EV_MEAN_RATIO && sampler == gmq::SA_UNIFORM)
-        return laneFits<float, gmq::EV_MEAN_RATIO, gmq::SA_UNIFORM>(w, smoother, lane);
-    return laneFits<float, 1, 1>(w, smoother, lane);
+    const bool mr = evaluator == gmq::EV_MEAN_RATIO && sampler == gmq::SA_UNIFORM;
+    if(fp64) return mr ? laneFits<double, gmq::EV_MEAN_RATIO, gmq::SA_UNIFORM>(w, smoother, lane)
+                       : laneFits<double, 1, 1>(w, smoother, lane);
+    return mr ? laneFits<float, gmq::EV_MEAN_RATIO, gmq::SA_UNIFORM>(w, smoother, lane)
+              : laneFits<float, 1, 1>(w, smoother, lane);
 }
 
 __global__ void mathProbeKernel(int op, const double* a, const double* b, double* out, int n)

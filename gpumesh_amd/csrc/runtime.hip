@@ -596,17 +596,26 @@ static hipError_t sweep(Plan* p, const gmq::Crew& c, int ev, int sa, const RunPa
     {
         for(const VList& l : p->colours[g])
         {
-            gmd::VertexWork w{p->dTable, l.segs, l.nsegs, l.count, l.kind, l.maxPatch};
+            gmd::VertexWork w{p->dTable, l.segs, l.nsegs, l.count, l.kind, l.maxPatch,
+                              f64 && r.precision == GM_FP32_POLISH ? 1 : 0};
+            gmd::LaneShape lane = r.lane;
+            if(lane.tileElems == 0)
+            {
+                // Smallest tile that holds the list's patches: spawn search
+                // stages one patch per wavefront, gradient descent eight.
+                const int need = r.smoother == gmd::SM_GRADIENT_DESCENT ? 8 * l.maxPatch : l.maxPatch;
+                lane.tileElems = need <= 64 ? 64 : (need <= 128 ? 128 : 256);
+            }
             int tag = getme ? (f64 ? GM_KT_GETME_VERT_F64 : GM_KT_GETME_VERT_F32)
                             : (f64 ? GM_KT_SMOOTH_F64 : GM_KT_SMOOTH_F32);
             int slot = timedBegin(p, timed, tag);
             if(getme)
-                e = f64 ? gmd::launchGetmeVertices<double>(w, c, ev, sa, r.kernel, r.lane, p->stream)
-                        : gmd::launchGetmeVertices<float>(w, c, ev, sa, r.kernel, r.lane, p->stream);
+                e = f64 ? gmd::launchGetmeVertices<double>(w, c, ev, sa, r.kernel, lane, p->stream)
+                        : gmd::launchGetmeVertices<float>(w, c, ev, sa, r.kernel, lane, p->stream);
             else
-                e = f64 ? gmd::launchSmooth<double>(w, c, ev, sa, r.smoother, r.kernel, r.lane, r.spawnCoeff,
+                e = f64 ? gmd::launchSmooth<double>(w, c, ev, sa, r.smoother, r.kernel, lane, r.spawnCoeff,
                                                     p->stream)
-                        : gmd::launchSmooth<float>(w, c, ev, sa, r.smoother, r.kernel, r.lane, r.spawnCoeff,
+                        : gmd::launchSmooth<float>(w, c, ev, sa, r.smoother, r.kernel, lane, r.spawnCoeff,
                                                    p->stream);
             timedEnd(p, slot);
             if(e != hipSuccess) break;

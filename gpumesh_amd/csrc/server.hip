@@ -96,7 +96,7 @@ struct gm_server
         gmr::Plan* plan = nullptr;
         hipStream_t stream = nullptr;
     };
-    std::map<std::tuple<int, int, int>, Group> groups;
+    std::map<std::tuple<int, int, int, int>, Group> groups;
 
     // Statistics (under mu).
     int64_t submitted = 0, done = 0, failed = 0, rounds = 0, peakActive = 0;
@@ -257,12 +257,16 @@ int retireJob(gm_server* s, Job* j)
 // per group.  Returns the streams used.
 int launchRound(gm_server* s, const gmd::LaneShape* lanes, std::vector<hipStream_t>& used)
 {
-    std::map<std::tuple<int, int, int>, std::vector<Job*>> members;
+    // Groups: (smoother, FP64 pass, widening first, polish pass).  A polish
+    // pass runs the mixed-precision FP64 kernels, an exact-mode pass the
+    // bit-exact ones, as gm_mesh_smooth does.
+    std::map<std::tuple<int, int, int, int>, std::vector<Job*>> members;
     for(Job* j : s->active)
     {
-        const bool f64 = j->precision == GM_FP64 || (j->precision == GM_FP32_POLISH && j->pass == j->passes - 1);
+        const bool polish = j->precision == GM_FP32_POLISH && j->pass == j->passes - 1;
+        const bool f64 = j->precision == GM_FP64 || polish;
         const bool convert = f64 && j->inF32;
-        members[std::make_tuple(j->smoother, (int)f64, (int)convert)].push_back(j);
+        members[std::make_tuple(j->smoother, (int)f64, (int)convert, (int)polish)].push_back(j);
     }
     // Drop plans whose membership changed.
     for(auto& kv : s->groups)
@@ -302,22 +306,13 @@ int launchRound(gm_server* s, const gmd::LaneShape* lanes, std::vector<hipStream
         const int smoother = std::get<0>(kv.first);
         gmr::RunParams r;
         r.smoother = smoother;
+        r.precision = std::get<3>(kv.first) ? GM_FP32_POLISH : (std::get<1>(kv.first) ? GM_FP64 : GM_FP32);
         r.kernel = GM_KERNEL_WAVE;
         r.spawnCoeff = 1.0 / 20.0;
-        int maxPatch = 0;
-        for(gm_mesh* m : g.plan->meshes) maxPatch = std::max(maxPatch, m->maxPatch);
         if(lanes[smoother].patchesPerGroup)
             r.lane = lanes[smoother];
         else
-        {
-            r.lane = gmd::LaneShape{1, 64};
-            if(smoother == gmd::SM_GETME)
-                r.lane.patchesPerGroup = 4;
-            else if(smoother == gmd::SM_GRADIENT_DESCENT)
-                r.lane.tileElems = maxPatch <= 8 ? 64 : (maxPatch <= 16 ? 128 : 256);
-            else
-                r.lane.tileElems = maxPatch <= 64 ? 64 : (maxPatch <= 128 ? 128 : 256);
-        }
+            r.lane = gmd::LaneShape{smoother == gmd::SM_GETME ? 4 : 1, 0};   // tile per work list
         int rc = gmr::launchPass(g.plan, s->crew, s->evaluator, s->sampler, r, std::get<1>(kv.first) != 0,
                                  std::get<2>(kv.first) != 0);
         if(rc) return rc;

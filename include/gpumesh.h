@@ -197,8 +197,9 @@ int gm_batch_destroy(gm_batch* batch);
  * kernels (gradient descent gives each of its 8 vertices tile_elems/8); a
  * work list whose largest patch does not fit the tile runs on the wave kernel
  * instead.  Nelder-Mead ignores it.  Until set (or autotuned) a batch uses
- * one wavefront per workgroup with the smallest tile that holds its largest
- * patch (GETMe: 4 wavefronts). */
+ * one wavefront per workgroup (GETMe: 4) and, for each work list, the
+ * smallest tile that holds the list's largest patch -- the choice a member
+ * makes alone. */
 int gm_batch_set_shape(gm_batch* batch, int smoother, int patches_per_group, int tile_elems);
 /* gm_mesh_smooth over every member.  use_graph != 0 captures one pass body
  * (all sweeps + the quality reduction) in a HIP graph on first use and
@@ -215,7 +216,8 @@ int gm_batch_kernel_times(const gm_batch* batch, double* ms, int64_t* launches);
  * tile_elems (times[12], summed sweep kernel ms; -1 where a shape does not
  * fit every list, and for GETMe where tile_elems != 64, which it does not
  * use), restores the positions after each trial, keeps the fastest shape for
- * `smoother` and returns it in best[2]. */
+ * `smoother` and returns it in best[2] (tile 0 if no shape fit: the default
+ * per-list choice stays). */
 int gm_batch_autotune(gm_batch* batch, int smoother, int precision, int passes, int32_t* best, double* times);
 
 /* ---- Serving: a request loop over resident meshes.
