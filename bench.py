@@ -185,7 +185,7 @@ def serving_leg(args):
     out = {"workload": f"{len(jobs)} jobs per rate: tet/hex 50/50 cubes, n log-uniform [8,100], jitter U[0.2,0.4]h, "
                        f"5-20 passes, NM/GD/GETMe uniform, fp32 + fp64 polish, Poisson arrivals",
            "jobs": len(jobs), "vertex_passes_per_run": work, "rates": {}}
-    with GpuServer(budget_bytes=288 * 10**9, build_threads=8) as srv:
+    with GpuServer(budget_bytes=288 * 10**9, worker_threads=8) as srv:
         # Untimed warm-up jobs (first launches of each kernel).
         warm = [srv.submit_mesh(meshes[0], sm, 1) for sm in ("nm", "gd", "getme")]
         for w in warm:
@@ -215,7 +215,7 @@ def serving_leg(args):
             log(f"serving rate {rate:.0f}/s: {len(jobs) / span:.1f} jobs/s, {work / span / 1e6:.1f} M vertices/s, "
                 f"p50 {np.percentile(lat, 50):.1f} ms, p99 {np.percentile(lat, 99):.1f} ms")
         st = srv.stats()
-        out["server"] = {"rounds": st["rounds"], "peak_active": st["peak_active"],
+        out["server"] = {"rounds": st["rounds"], "round_wall_s": st["round_ms"] / 1e3, "peak_active": st["peak_active"],
                          "peak_reserved_gb": st["peak_reserved_bytes"] / 1e9, "budget_gb": st["budget_bytes"] / 1e9}
     # The same jobs one by one through the plain FP64 kernel (a bounded sample).
     sample = list(range(min(args.serve_plain_sample, len(jobs))))
@@ -254,6 +254,7 @@ def main():
     ap.add_argument("--serve-rates", default="50,200,1000,2000")
     ap.add_argument("--serve-plain-sample", type=int, default=8)
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-only", action="store_true", help="run only the serving leg and print its JSON")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device to device 5 times (the reference kernel for HBM counters) and exit")
     args = ap.parse_args()
@@ -282,6 +283,9 @@ def main():
     spawn_coeff = 0.25
     from gpumesh_amd._lib import check, lib
     check(lib().gm_set_topology_cache(8 << 30), "gm_set_topology_cache")
+    if args.serve_only:
+        print(json.dumps(serving_leg(args)))
+        return
 
     t0 = time.perf_counter()
     m = cube_mesh(args.n, "tet", 0.35, seed=2)

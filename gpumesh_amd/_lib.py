@@ -77,7 +77,7 @@ def lib():
     L.gm_server_create.argtypes = [i64, C.c_int, C.c_int, C.c_int, C.c_int, dbl, C.POINTER(vp)]
     L.gm_server_destroy.argtypes = [vp]
     L.gm_server_set_shape.argtypes = [vp, C.c_int, C.c_int, C.c_int]
-    L.gm_server_submit.argtypes = [vp, i64, vp, vp, i64, vp, i64, vp, C.c_int, C.c_int, C.c_int,
+    L.gm_server_submit.argtypes = [vp, i64, vp, vp, i64, vp, i64, vp, C.c_int, C.c_int, C.c_int, C.c_int,
                                    C.POINTER(i64)]
     L.gm_server_wait.argtypes = [vp, i64, dbl, C.POINTER(C.c_int)]
     L.gm_server_result.argtypes = [vp, i64, vp, vp, vp, vp]

@@ -3,14 +3,18 @@
 // and advance together one pass per round (continuous batching).
 //
 // Threads:
-//  * builders (host only) turn submitted jobs into host topologies through the
-//    shared topology cache and compute their device footprint;
-//  * one scheduler owns every HIP call.  A round launches one pass for every
-//    active job -- jobs are grouped by (smoother, FP64 pass, widening first)
-//    and each group is one fused plan on its own stream -- then, while those
-//    kernels run, uploads and starts newly admitted jobs on their own
-//    streams, and after the round retires finished jobs and frees their
-//    memory.  Jobs that arrive while others run join at the next round.
+//  * workers: build a submitted job's host topology through the shared
+//    topology cache and compute its device footprint; once the scheduler has
+//    admitted the job, upload it and start it (pass record 0) on the job's
+//    own stream; once its last pass ran, copy its results out and release its
+//    device memory.  Retirements go first, then uploads, then builds, so the
+//    memory of finished jobs comes back before new work is taken on.
+//  * one scheduler: admits built jobs in submission order against the memory
+//    budget and the slot count, and runs rounds.  A round launches one pass
+//    for every running job -- jobs grouped by (smoother, FP64 pass, widening
+//    first, polish pass), each group one fused plan on its own stream -- and
+//    waits for it.  Jobs the workers started meanwhile join the next round,
+//    so uploads and result copies overlap the rounds' kernels.
 //
 // A job's kernels, their order within the job and its reduction trees are the
 // ones gm_mesh_smooth runs for the same mesh alone, so every result is
@@ -38,16 +42,22 @@ using Clock = std::chrono::steady_clock;
 namespace
 {
 
+// Public states (gm_server_wait): queued, built, active, done, failed.
 enum JobState { JOB_QUEUED = 0, JOB_BUILT = 1, JOB_ACTIVE = 2, JOB_DONE = 3, JOB_FAILED = 4 };
 
 struct Job
 {
     int64_t id = 0;
-    // Inputs (released once the topology is built).
+    // Inputs: owned copies, or the caller's arrays when borrowed; dropped
+    // once the topology is built.
     int64_t nverts = 0, ntets = 0, nhexs = 0;
-    std::vector<double> pos;
-    std::vector<int32_t> fixed;
-    std::vector<uint32_t> tets, hexs;
+    std::vector<double> posCopy;
+    std::vector<int32_t> fixedCopy;
+    std::vector<uint32_t> tetsCopy, hexsCopy;
+    const double* pos = nullptr;
+    const int32_t* fixed = nullptr;
+    const uint32_t* tets = nullptr;
+    const uint32_t* hexs = nullptr;
     int smoother = 0, passes = 0, precision = 0;
     // State.
     int state = JOB_QUEUED;
@@ -61,6 +71,18 @@ struct Job
     int errCode = 0;
     std::string err;
     Clock::time_point tSubmit, tBuilt, tAdmit, tDone;
+
+    void dropInputs()
+    {
+        std::vector<double>().swap(posCopy);
+        std::vector<int32_t>().swap(fixedCopy);
+        std::vector<uint32_t>().swap(tetsCopy);
+        std::vector<uint32_t>().swap(hexsCopy);
+        pos = nullptr;
+        fixed = nullptr;
+        tets = nullptr;
+        hexs = nullptr;
+    }
 };
 
 double ms(Clock::time_point a, Clock::time_point b)
@@ -80,13 +102,17 @@ struct gm_server
     gmd::LaneShape lane[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
 
     std::mutex mu;
-    std::condition_variable cvSched, cvBuild, cvDone;
+    std::condition_variable cvSched, cvWork, cvDone;
     bool stop = false;
     int64_t nextId = 1;
     std::map<int64_t, std::unique_ptr<Job>> jobs;
-    std::deque<Job*> buildQueue;
-    std::deque<Job*> pending;        // submitted, not yet admitted, in submission order
-    size_t reserved = 0;
+    std::deque<Job*> buildQueue;     // submitted, topology not built
+    std::deque<Job*> pending;        // submitted, not admitted, in submission order
+    std::deque<Job*> uploadQueue;    // admitted, to upload and start
+    std::deque<Job*> retireQueue;    // last pass run, results to copy out
+    std::vector<Job*> ready;         // started, waiting for the next round
+    size_t reserved = 0;             // device bytes of the admitted jobs
+    int admitted = 0;                // jobs holding a slot: uploading, ready, running or retiring
 
     // Scheduler-owned.
     std::vector<Job*> active;
@@ -101,18 +127,18 @@ struct gm_server
     // Statistics (under mu).
     int64_t submitted = 0, done = 0, failed = 0, rounds = 0, peakActive = 0;
     size_t peakReserved = 0;
-    double deviceBusyMs = 0.0;
+    double roundMs = 0.0;
 
-    std::vector<std::thread> builders;
+    std::vector<std::thread> workers;
     std::thread scheduler;
 };
 
 namespace
 {
 
+// Both with s->mu held.
 void finishJob(gm_server* s, Job* j, int code, const std::string& msg)
 {
-    // Called with s->mu held.
     j->errCode = code;
     j->err = msg;
     j->state = code ? JOB_FAILED : JOB_DONE;
@@ -122,101 +148,92 @@ void finishJob(gm_server* s, Job* j, int code, const std::string& msg)
     s->cvDone.notify_all();
 }
 
-void builderLoop(gm_server* s)
+void releaseSlot(gm_server* s, Job* j)
 {
-    for(;;)
-    {
-        Job* j = nullptr;
-        {
-            std::unique_lock<std::mutex> lock(s->mu);
-            s->cvBuild.wait(lock, [&] { return s->stop || !s->buildQueue.empty(); });
-            if(s->stop) return;
-            j = s->buildQueue.front();
-            s->buildQueue.pop_front();
-        }
-        gm_mesh* m = nullptr;
-        int rc = gmr::makeMesh(j->nverts, j->pos.data(), j->fixed.data(), j->ntets, j->tets.data(), j->nhexs,
-                               j->hexs.data(), &m);
-        std::string msg = rc ? gmr::lastError() : "";
-        size_t bytes = 0;
-        if(!rc)
-        {
-            const gmt::Topology& t = *m->topo;
-            for(int64_t v = 0; v < t.nverts && !rc; ++v)
-                if(!t.fixed[v] && t.eoff[v + 1] == t.eoff[v])
-                {
-                    rc = GM_ERR_ARG;
-                    msg = "a free vertex belongs to no element";
-                }
-            bytes = gmr::residentBytes(m, j->passes + 1, j->smoother == gmd::SM_GETME);
-        }
-        std::lock_guard<std::mutex> lock(s->mu);
-        j->pos.clear(); j->pos.shrink_to_fit();
-        j->fixed.clear(); j->fixed.shrink_to_fit();
-        j->tets.clear(); j->tets.shrink_to_fit();
-        j->hexs.clear(); j->hexs.shrink_to_fit();
-        j->tBuilt = Clock::now();
-        if(!rc && bytes > s->budget)
-        {
-            rc = GM_ERR_NOMEM;
-            msg = "the job needs more device memory than the server's budget";
-        }
-        if(rc)
-        {
-            if(m) gmr::destroyMesh(m);
-            s->pending.erase(std::find(s->pending.begin(), s->pending.end(), j));
-            finishJob(s, j, rc, msg);
-        }
-        else
-        {
-            j->mesh = m;
-            j->bytes = bytes;
-            j->state = JOB_BUILT;
-        }
-        s->cvSched.notify_all();
-    }
+    s->reserved -= j->bytes;
+    --s->admitted;
+    s->cvSched.notify_all();
 }
 
-// Admission under s->mu: the pending jobs in submission order; a built job
-// that does not fit the memory left blocks those behind it (no overtaking of
-// large jobs), a job still building is skipped.
-std::vector<Job*> takeAdmissible(gm_server* s)
+// ---- worker tasks, run without the lock
+
+void buildJob(gm_server* s, Job* j)
 {
-    std::vector<Job*> out;
-    size_t slots = s->active.size();
-    for(auto it = s->pending.begin(); it != s->pending.end();)
+    gm_mesh* m = nullptr;
+    int rc = gmr::makeMesh(j->nverts, j->pos, j->fixed, j->ntets, j->tets, j->nhexs, j->hexs, &m);
+    std::string msg = rc ? gmr::lastError() : "";
+    size_t bytes = 0;
+    if(!rc)
     {
-        Job* j = *it;
-        if(j->state != JOB_BUILT) { ++it; continue; }
-        if((int)(slots + out.size()) >= s->maxActive || s->reserved + j->bytes > s->budget) break;
-        s->reserved += j->bytes;
-        s->peakReserved = std::max(s->peakReserved, s->reserved);
-        j->state = JOB_ACTIVE;
-        j->tAdmit = Clock::now();
-        out.push_back(j);
-        it = s->pending.erase(it);
+        const gmt::Topology& t = *m->topo;
+        for(int64_t v = 0; v < t.nverts && !rc; ++v)
+            if(!t.fixed[v] && t.eoff[v + 1] == t.eoff[v])
+            {
+                rc = GM_ERR_ARG;
+                msg = "a free vertex belongs to no element";
+            }
+        bytes = gmr::residentBytes(m, j->passes + 1, j->smoother == gmd::SM_GETME);
     }
-    return out;
+    std::lock_guard<std::mutex> lock(s->mu);
+    j->dropInputs();
+    j->tBuilt = Clock::now();
+    if(!rc && bytes > s->budget)
+    {
+        rc = GM_ERR_NOMEM;
+        msg = "the job needs more device memory than the server's budget";
+    }
+    if(rc)
+    {
+        if(m) gmr::destroyMesh(m);
+        s->pending.erase(std::find(s->pending.begin(), s->pending.end(), j));
+        finishJob(s, j, rc, msg);
+    }
+    else
+    {
+        j->mesh = m;
+        j->bytes = bytes;
+        j->state = JOB_BUILT;
+    }
+    s->cvSched.notify_all();
 }
 
-// Uploads and starts a job on its own stream (no wait).
-int startJob(gm_server* s, Job* j)
+// Uploads an admitted job, sets its crew and runs its start (counters, FP32
+// copy, pass record 0) on the job's own stream.
+void uploadJob(gm_server* s, Job* j)
 {
     gm_mesh* m = j->mesh;
     int rc = gmr::uploadMesh(m);
-    if(rc) return rc;
-    m->crew = s->crew;
-    m->evaluator = s->evaluator;
-    m->sampler = s->sampler;
-    if((rc = gmr::preparePlan(m->plan, j->passes + 1, j->smoother == gmd::SM_GETME))) return rc;
+    if(!rc)
+    {
+        m->crew = s->crew;
+        m->evaluator = s->evaluator;
+        m->sampler = s->sampler;
+        rc = gmr::preparePlan(m->plan, j->passes + 1, j->smoother == gmd::SM_GETME);
+    }
     j->inF32 = j->precision != GM_FP64;
     j->pass = 0;
-    return gmr::launchBegin(m->plan, s->crew, s->evaluator, s->sampler, j->inF32);
+    if(!rc) rc = gmr::launchBegin(m->plan, s->crew, s->evaluator, s->sampler, j->inF32);
+    if(!rc && hipStreamSynchronize(m->stream) != hipSuccess) rc = fail(GM_ERR_HIP, "job start failed");
+    const std::string msg = rc ? gmr::lastError() : "";
+    if(rc)
+    {
+        gmr::destroyMesh(m);
+        j->mesh = nullptr;
+    }
+    std::lock_guard<std::mutex> lock(s->mu);
+    if(rc)
+    {
+        releaseSlot(s, j);
+        finishJob(s, j, rc, msg);
+    }
+    else
+        s->ready.push_back(j);
+    s->cvSched.notify_all();
 }
 
 // Copies a finished job's results (pass records, positions in the caller's
 // numbering) and releases its device memory.
-int retireJob(gm_server* s, Job* j)
+void retireJob(gm_server* s, Job* j)
 {
     gm_mesh* m = j->mesh;
     int rc = GM_OK;
@@ -248,14 +265,70 @@ int retireJob(gm_server* s, Job* j)
             j->outPos[3 * old + 2] = q.z;
         }
     }
+    const std::string msg = rc ? gmr::lastError() : "";
     gmr::destroyMesh(m);
     j->mesh = nullptr;
-    return rc;
+    std::lock_guard<std::mutex> lock(s->mu);
+    releaseSlot(s, j);
+    finishJob(s, j, rc, msg);
+}
+
+void workerLoop(gm_server* s)
+{
+    (void)hipSetDevice(s->device);
+    for(;;)
+    {
+        Job* j = nullptr;
+        int task = 0;
+        {
+            std::unique_lock<std::mutex> lock(s->mu);
+            s->cvWork.wait(lock, [&] {
+                return s->stop || !s->retireQueue.empty() || !s->uploadQueue.empty() || !s->buildQueue.empty();
+            });
+            if(s->stop) return;
+            std::deque<Job*>& q = !s->retireQueue.empty() ? s->retireQueue
+                                : !s->uploadQueue.empty() ? s->uploadQueue
+                                                          : s->buildQueue;
+            task = &q == &s->retireQueue ? 2 : &q == &s->uploadQueue ? 1 : 0;
+            j = q.front();
+            q.pop_front();
+        }
+        if(task == 2) retireJob(s, j);
+        else if(task == 1) uploadJob(s, j);
+        else buildJob(s, j);
+    }
+}
+
+// ---- admission (s->mu held): pending jobs in submission order; a built job
+// that does not fit the memory left blocks those behind it (no overtaking of
+// large jobs), a job still building is skipped.
+
+Job* nextAdmissible(gm_server* s)
+{
+    if(s->admitted >= s->maxActive) return nullptr;
+    for(Job* j : s->pending)
+        if(j->state == JOB_BUILT) return s->reserved + j->bytes <= s->budget ? j : nullptr;
+    return nullptr;
+}
+
+void admit(gm_server* s)
+{
+    while(Job* j = nextAdmissible(s))
+    {
+        s->pending.erase(std::find(s->pending.begin(), s->pending.end(), j));
+        s->reserved += j->bytes;
+        s->peakReserved = std::max(s->peakReserved, s->reserved);
+        ++s->admitted;
+        j->state = JOB_ACTIVE;
+        j->tAdmit = Clock::now();
+        s->uploadQueue.push_back(j);
+        s->cvWork.notify_one();
+    }
 }
 
 // Brings the group plans in line with the active set and launches one pass
-// per group.  Returns the streams used.
-int launchRound(gm_server* s, const gmd::LaneShape* lanes, std::vector<hipStream_t>& used)
+// per group.
+int launchRound(gm_server* s, const gmd::LaneShape* lanes)
 {
     // Groups: (smoother, FP64 pass, widening first, polish pass).  A polish
     // pass runs the mixed-precision FP64 kernels, an exact-mode pass the
@@ -313,10 +386,9 @@ int launchRound(gm_server* s, const gmd::LaneShape* lanes, std::vector<hipStream
             r.lane = lanes[smoother];
         else
             r.lane = gmd::LaneShape{smoother == gmd::SM_GETME ? 4 : 1, 0};   // tile per work list
-        int rc = gmr::launchPass(g.plan, s->crew, s->evaluator, s->sampler, r, std::get<1>(kv.first) != 0,
-                                 std::get<2>(kv.first) != 0);
+        const int rc = gmr::launchPass(g.plan, s->crew, s->evaluator, s->sampler, r, std::get<1>(kv.first) != 0,
+                                       std::get<2>(kv.first) != 0);
         if(rc) return rc;
-        used.push_back(g.stream);
     }
     return GM_OK;
 }
@@ -324,78 +396,53 @@ int launchRound(gm_server* s, const gmd::LaneShape* lanes, std::vector<hipStream
 void schedulerLoop(gm_server* s)
 {
     (void)hipSetDevice(s->device);
-    // Jobs whose last pass ran in the previous round: their results are
-    // copied out while the next round's kernels run.
-    std::vector<Job*> retiring;
     for(;;)
     {
-        std::vector<Job*> admit;
         gmd::LaneShape lanes[4];
         {
             std::unique_lock<std::mutex> lock(s->mu);
             s->cvSched.wait(lock, [&] {
-                if(s->stop || !s->active.empty() || !retiring.empty()) return true;
-                for(Job* j : s->pending)
-                    if(j->state == JOB_BUILT) return true;
-                return false;
+                return s->stop || !s->active.empty() || !s->ready.empty() || nextAdmissible(s) != nullptr;
             });
             if(s->stop) break;
-            admit = takeAdmissible(s);
+            admit(s);
+            // Started jobs join this round; a job with no passes retires at once.
+            for(Job* j : s->ready)
+            {
+                if(j->passes > 0) s->active.push_back(j);
+                else
+                {
+                    s->retireQueue.push_back(j);
+                    s->cvWork.notify_one();
+                }
+            }
+            s->ready.clear();
             for(int k = 0; k < 4; ++k) lanes[k] = s->lane[k];
+            s->peakActive = std::max<int64_t>(s->peakActive, (int64_t)s->active.size());
         }
+        if(s->active.empty()) continue;
         const Clock::time_point t0 = Clock::now();
-        std::vector<hipStream_t> used;
-        int rc = s->active.empty() ? GM_OK : launchRound(s, lanes, used);
+        const int rc = launchRound(s, lanes);
+        // Also after a failed launch: whatever was launched must end before
+        // the memory goes back.
+        for(auto& kv : s->groups)
+            if(kv.second.stream) (void)hipStreamSynchronize(kv.second.stream);
+        const Clock::time_point t1 = Clock::now();
+        std::lock_guard<std::mutex> lock(s->mu);
         if(rc)
         {
-            // A launch failure poisons every active job of the round; wait for
-            // whatever was launched before releasing memory.
+            // A launch failure fails every job of the round.
             const std::string msg = gmr::lastError();
-            for(auto& kv : s->groups)
-                if(kv.second.stream) (void)hipStreamSynchronize(kv.second.stream);
-            used.clear();
-            std::lock_guard<std::mutex> lock(s->mu);
             for(Job* j : s->active)
             {
                 gmr::destroyMesh(j->mesh);
                 j->mesh = nullptr;
-                s->reserved -= j->bytes;
+                releaseSlot(s, j);
                 finishJob(s, j, rc, msg);
             }
             s->active.clear();
+            continue;
         }
-        // While the round runs: retire the previous round's finished jobs and
-        // start the admitted ones, each on its own stream.
-        for(Job* j : retiring)
-        {
-            const int rrc = retireJob(s, j);
-            std::lock_guard<std::mutex> lock(s->mu);
-            s->reserved -= j->bytes;
-            finishJob(s, j, rrc, rrc ? gmr::lastError() : "");
-        }
-        retiring.clear();
-        std::vector<Job*> started;
-        for(Job* j : admit)
-        {
-            const int src = startJob(s, j);
-            if(src)
-            {
-                const std::string msg = gmr::lastError();
-                if(j->mesh)
-                {
-                    gmr::destroyMesh(j->mesh);
-                    j->mesh = nullptr;
-                }
-                std::lock_guard<std::mutex> lock(s->mu);
-                s->reserved -= j->bytes;
-                finishJob(s, j, src, msg);
-            }
-            else
-                started.push_back(j);
-        }
-        for(hipStream_t st : used) (void)hipStreamSynchronize(st);
-        for(Job* j : started) (void)hipStreamSynchronize(j->mesh->stream);
-        const Clock::time_point t1 = Clock::now();
         std::vector<Job*> still;
         for(Job* j : s->active)
         {
@@ -403,33 +450,33 @@ void schedulerLoop(gm_server* s)
             if(f64) j->inF32 = false;
             ++j->pass;
             if(j->pass < j->passes) still.push_back(j);
-            else retiring.push_back(j);
-        }
-        for(Job* j : started)
-        {
-            if(j->passes > 0) still.push_back(j);
-            else retiring.push_back(j);
+            else
+            {
+                s->retireQueue.push_back(j);
+                s->cvWork.notify_one();
+            }
         }
         s->active.swap(still);
-        std::lock_guard<std::mutex> lock(s->mu);
-        if(!used.empty())
-        {
-            ++s->rounds;
-            s->deviceBusyMs += ms(t0, t1);
-        }
-        s->peakActive = std::max<int64_t>(s->peakActive, (int64_t)s->active.size());
+        ++s->rounds;
+        s->roundMs += ms(t0, t1);
     }
-    // Shutdown: release every resident job.
-    for(Job* j : retiring) s->active.push_back(j);
-    for(Job* j : s->active)
+    // Shutdown: cancel the jobs the scheduler holds; the device memory of
+    // jobs left in the worker queues is released by gm_server_destroy.
+    std::vector<Job*> held;
+    {
+        std::lock_guard<std::mutex> lock(s->mu);
+        held.swap(s->active);
+        held.insert(held.end(), s->ready.begin(), s->ready.end());
+        s->ready.clear();
+    }
+    for(Job* j : held)
     {
         gmr::destroyMesh(j->mesh);
         j->mesh = nullptr;
         std::lock_guard<std::mutex> lock(s->mu);
-        s->reserved -= j->bytes;
+        releaseSlot(s, j);
         finishJob(s, j, GM_ERR_ARG, "server shut down");
     }
-    s->active.clear();
     for(auto& kv : s->groups)
     {
         gmr::freePlan(kv.second.plan);
@@ -442,7 +489,7 @@ void schedulerLoop(gm_server* s)
 
 extern "C" {
 
-int gm_server_create(int64_t budgetBytes, int maxActive, int buildThreads, int evaluator, int sampler,
+int gm_server_create(int64_t budgetBytes, int maxActive, int workerThreads, int evaluator, int sampler,
                      double scaling, gm_server** out)
 {
     if(!out) return fail(GM_ERR_ARG, "out is null");
@@ -452,7 +499,7 @@ int gm_server_create(int64_t budgetBytes, int maxActive, int buildThreads, int e
         return fail(sampler == GM_TEXTURE ? GM_ERR_UNSUPPORTED : GM_ERR_ARG,
                     "the server takes the uniform or analytic sampler");
     if(!(scaling > 0.0)) return fail(GM_ERR_ARG, "scaling must be positive");
-    if(maxActive < 1 || buildThreads < 1 || buildThreads > 64) return fail(GM_ERR_ARG, "bad slot or thread count");
+    if(maxActive < 1 || workerThreads < 1 || workerThreads > 64) return fail(GM_ERR_ARG, "bad slot or thread count");
     int ndev = 0;
     if(hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(GM_ERR_HIP, "no HIP device visible");
     auto s = std::make_unique<gm_server>();
@@ -471,7 +518,7 @@ int gm_server_create(int64_t budgetBytes, int maxActive, int buildThreads, int e
     s->crew.scaling2 = scaling * scaling;
     s->crew.scaling3 = s->crew.scaling2 * scaling;
     gm_server* raw = s.get();
-    for(int i = 0; i < buildThreads; ++i) raw->builders.emplace_back(builderLoop, raw);
+    for(int i = 0; i < workerThreads; ++i) raw->workers.emplace_back(workerLoop, raw);
     raw->scheduler = std::thread(schedulerLoop, raw);
     *out = s.release();
     return GM_OK;
@@ -484,10 +531,12 @@ int gm_server_destroy(gm_server* s)
         std::lock_guard<std::mutex> lock(s->mu);
         s->stop = true;
     }
-    s->cvBuild.notify_all();
+    s->cvWork.notify_all();
     s->cvSched.notify_all();
-    for(std::thread& t : s->builders) t.join();
+    for(std::thread& t : s->workers) t.join();
     s->scheduler.join();
+    // Jobs left in the worker queues (or started after the scheduler ended)
+    // still hold a host topology or device memory.
     for(auto& kv : s->jobs)
         if(kv.second->mesh) gmr::destroyMesh(kv.second->mesh);
     delete s;
@@ -509,23 +558,40 @@ int gm_server_set_shape(gm_server* s, int smoother, int patchesPerGroup, int til
 
 int gm_server_submit(gm_server* s, int64_t nverts, const double* pos, const int32_t* fixed, int64_t ntets,
                      const uint32_t* tets, int64_t nhexs, const uint32_t* hexs, int smoother, int passes,
-                     int precision, int64_t* jobId)
+                     int precision, int borrow, int64_t* jobId)
 {
+    const Clock::time_point now = Clock::now();
     if(!s || !jobId) return fail(GM_ERR_ARG, "null argument");
     if(nverts < 0 || ntets < 0 || nhexs < 0) return fail(GM_ERR_ARG, "negative size");
     if((nverts && (!pos || !fixed)) || (ntets && !tets) || (nhexs && !hexs)) return fail(GM_ERR_ARG, "null array");
     if(smoother < 0 || smoother > 3) return fail(GM_ERR_ARG, "unknown smoother");
     if(passes < 0) return fail(GM_ERR_ARG, "negative pass count");
     if(precision < 0 || precision > 2) return fail(GM_ERR_ARG, "unknown precision");
-    const Clock::time_point now = Clock::now();
     auto j = std::make_unique<Job>();
     j->nverts = nverts;
     j->ntets = ntets;
     j->nhexs = nhexs;
-    j->pos.assign(pos, pos + 3 * (size_t)nverts);
-    j->fixed.assign(fixed, fixed + (size_t)nverts);
-    j->tets.assign(tets, tets + 4 * (size_t)ntets);
-    j->hexs.assign(hexs, hexs + 8 * (size_t)nhexs);
+    if(borrow)
+    {
+        j->pos = pos;
+        j->fixed = fixed;
+        j->tets = tets;
+        j->hexs = hexs;
+    }
+    else
+    {
+        if(nverts)
+        {
+            j->posCopy.assign(pos, pos + 3 * (size_t)nverts);
+            j->fixedCopy.assign(fixed, fixed + (size_t)nverts);
+        }
+        if(ntets) j->tetsCopy.assign(tets, tets + 4 * (size_t)ntets);
+        if(nhexs) j->hexsCopy.assign(hexs, hexs + 8 * (size_t)nhexs);
+        j->pos = j->posCopy.data();
+        j->fixed = j->fixedCopy.data();
+        j->tets = j->tetsCopy.data();
+        j->hexs = j->hexsCopy.data();
+    }
     j->smoother = smoother;
     j->passes = passes;
     j->precision = precision;
@@ -539,7 +605,7 @@ int gm_server_submit(gm_server* s, int64_t nverts, const double* pos, const int3
     s->buildQueue.push_back(raw);
     s->pending.push_back(raw);
     ++s->submitted;
-    s->cvBuild.notify_one();
+    s->cvWork.notify_one();
     return GM_OK;
 }
 
@@ -568,14 +634,8 @@ int gm_server_result(gm_server* s, int64_t jobId, double* pos, double* stats, in
     Job* j = it->second.get();
     if(j->state == JOB_FAILED) return fail(j->errCode, "job failed: " + j->err);
     if(j->state != JOB_DONE) return fail(GM_ERR_ARG, "job has not finished");
-    if(pos) std::memcpy(pos, j->outPos.data(), j->outPos.size() * sizeof(double));
-    if(stats)
-        for(int k = 0; k <= j->passes; ++k)
-        {
-            stats[3 * k] = j->stats[3 * k];
-            stats[3 * k + 1] = j->stats[3 * k + 1];
-            stats[3 * k + 2] = j->stats[3 * k + 2];
-        }
+    if(pos && !j->outPos.empty()) std::memcpy(pos, j->outPos.data(), j->outPos.size() * sizeof(double));
+    if(stats) std::memcpy(stats, j->stats.data(), j->stats.size() * sizeof(double));
     if(hist) std::memcpy(hist, j->hist.data(), j->hist.size() * sizeof(int32_t));
     if(times4)
     {
@@ -610,7 +670,7 @@ int gm_server_stats(gm_server* s, double* out8)
     out8[4] = (double)s->peakActive;
     out8[5] = (double)s->peakReserved;
     out8[6] = (double)s->budget;
-    out8[7] = s->deviceBusyMs;
+    out8[7] = s->roundMs;
     return GM_OK;
 }
 

@@ -14,16 +14,18 @@ JOB_STATES = {0: "queued", 1: "built", 2: "active", 3: "done", 4: "failed"}
 
 class GpuServer:
     """A request loop on the current HIP device.  budget_bytes <= 0 uses the
-    free device memory (less 1 GiB)."""
+    free device memory (less 1 GiB).  Submitted arrays are lent to the server
+    (no copy) and held here until the job is built."""
 
-    def __init__(self, budget_bytes: int = 0, max_active: int = 4096, build_threads: int = 8,
+    def __init__(self, budget_bytes: int = 0, max_active: int = 4096, worker_threads: int = 8,
                  evaluator: str = "mean_ratio", sampler: str = "uniform", scaling: float = 1.0):
         h = C.c_void_p()
-        check(lib().gm_server_create(int(budget_bytes), int(max_active), int(build_threads), EVALUATORS[evaluator],
+        check(lib().gm_server_create(int(budget_bytes), int(max_active), int(worker_threads), EVALUATORS[evaluator],
                                      SAMPLERS[sampler], float(scaling), C.byref(h)), "gm_server_create")
         self._h = h
         self._passes = {}
         self._nverts = {}
+        self._lent = {}
 
     def close(self) -> None:
         if getattr(self, "_h", None):
@@ -54,9 +56,10 @@ class GpuServer:
         job = C.c_int64(0)
         check(lib().gm_server_submit(self._h, pos.shape[0], ptr(pos), ptr(fixed), tets.shape[0], ptr(tets),
                                      hexs.shape[0], ptr(hexs), SMOOTHERS[smoother], int(passes),
-                                     PRECISIONS[precision], C.byref(job)), "gm_server_submit")
+                                     PRECISIONS[precision], 1, C.byref(job)), "gm_server_submit")
         self._passes[job.value] = int(passes)
         self._nverts[job.value] = int(pos.shape[0])
+        self._lent[job.value] = (pos, fixed, tets, hexs)
         return int(job.value)
 
     def submit_mesh(self, m, smoother: str, passes: int, precision: str = "fp32_polish") -> int:
@@ -65,6 +68,8 @@ class GpuServer:
     def wait(self, job: int, timeout_ms: float = -1.0) -> str:
         st = C.c_int(0)
         check(lib().gm_server_wait(self._h, int(job), float(timeout_ms), C.byref(st)), "gm_server_wait")
+        if st.value != 0:
+            self._lent.pop(job, None)
         return JOB_STATES[st.value]
 
     def result(self, job: int, release: bool = True):
@@ -85,6 +90,7 @@ class GpuServer:
         check(lib().gm_server_release(self._h, int(job)), "gm_server_release")
         self._passes.pop(job, None)
         self._nverts.pop(job, None)
+        self._lent.pop(job, None)
 
     def stats(self):
         out = np.zeros(8, np.float64)

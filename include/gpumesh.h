@@ -223,14 +223,15 @@ int gm_batch_autotune(gm_batch* batch, int smoother, int precision, int passes, 
 /* ---- Serving: a request loop over resident meshes.
  *
  * Jobs (a mesh, a smoother, a pass count, a precision mode) are submitted
- * from any thread and return at once.  Builder threads build each job's host
+ * from any thread and return at once.  Worker threads build each job's host
  * topology (through the topology cache, gm_set_topology_cache); a scheduler
  * thread admits built jobs in submission order while their device footprint
- * fits the memory budget and a slot is free, keeps them resident, and runs
- * rounds: every active job advances one pass per round, the jobs of one
- * (smoother, FP64 pass) group in fused launches, groups on their own streams,
- * and jobs admitted meanwhile join the next round (continuous batching).  A
- * finished job's memory is released at once.  Each job's result is
+ * fits the memory budget and a slot is free, and runs rounds: every running
+ * job advances one pass per round, the jobs of one (smoother, FP64 pass)
+ * group in fused launches, groups on their own streams.  The workers upload
+ * and start admitted jobs and copy out finished ones while rounds run;
+ * started jobs join the next round (continuous batching).  A finished job's
+ * device memory is released at once.  Each job's result is
  * bit-identical to gm_mesh_smooth on that mesh alone (same smoother, passes
  * and precision, GM_KERNEL_WAVE, spawn coefficient 1/20).  The reference has
  * no serving layer; a job is one AbstractSmoother::smoothMesh call
@@ -242,15 +243,18 @@ int gm_batch_autotune(gm_batch* batch, int smoother, int precision, int passes, 
  * scaling for every job.  Destroying the server cancels unfinished jobs. */
 typedef struct gm_server gm_server;
 
-int gm_server_create(int64_t budget_bytes, int max_active, int build_threads, int evaluator, int sampler,
+int gm_server_create(int64_t budget_bytes, int max_active, int worker_threads, int evaluator, int sampler,
                      double scaling, gm_server** out);
 int gm_server_destroy(gm_server* server);
 /* Launch shape for `smoother` (as gm_batch_set_shape). */
 int gm_server_set_shape(gm_server* server, int smoother, int patches_per_group, int tile_elems);
-/* Copies the mesh (caller numbering) and queues the job; *job = its id. */
+/* Queues a job for the mesh (caller numbering); *job = its id.  borrow = 0
+ * copies the arrays before returning; borrow != 0 reads them in place, so
+ * the caller keeps them alive and unchanged until gm_server_wait reports the
+ * job built, done or failed. */
 int gm_server_submit(gm_server* server, int64_t nverts, const double* pos, const int32_t* fixed, int64_t ntets,
                      const uint32_t* tets, int64_t nhexs, const uint32_t* hexs, int smoother, int passes,
-                     int precision, int64_t* job);
+                     int precision, int borrow, int64_t* job);
 /* Waits until the job finished or timeout_ms passed (< 0: no limit);
  * *state = 0 queued, 1 built, 2 active, 3 done, 4 failed. */
 int gm_server_wait(gm_server* server, int64_t job, double timeout_ms, int* state);

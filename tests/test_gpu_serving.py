@@ -49,7 +49,7 @@ def _check(res, ref, tag):
 
 def test_stream_results_match_alone():
     refs = [_alone(s) for s in JOBS]
-    with GpuServer(build_threads=4) as srv:
+    with GpuServer(worker_threads=4) as srv:
         ids = []
         for k, (n, kind, jitter, seed, smoother, passes, precision) in enumerate(JOBS):
             ids.append(srv.submit_mesh(cube_mesh(n, kind, jitter, seed), smoother, passes, precision))
@@ -72,7 +72,7 @@ def test_admission_by_memory_budget():
     probe.smooth("getme", 3, precision="fp32_polish", kernel="wave")
     one = probe.device_bytes()
     budget = int(2.6 * one)
-    with GpuServer(budget_bytes=budget, build_threads=2) as srv:
+    with GpuServer(budget_bytes=budget, worker_threads=2) as srv:
         big = cube_mesh(30, "tet", 0.3, 1)
         jb = srv.submit_mesh(big, "getme", 2, "fp32")
         ids = [srv.submit_mesh(cube_mesh(*s[:4]), s[4], s[5], s[6]) for s in specs]
@@ -89,7 +89,7 @@ def test_admission_by_memory_budget():
 
 
 def test_slot_limit_and_release():
-    with GpuServer(max_active=1, build_threads=1) as srv:
+    with GpuServer(max_active=1, worker_threads=1) as srv:
         specs = [(5, "hex", 0.3, 300 + k, "nm", 2, "fp64") for k in range(3)]
         refs = [_alone(s) for s in specs]
         ids = [srv.submit_mesh(cube_mesh(*s[:4]), s[4], s[5], s[6]) for s in specs]
