@@ -55,6 +55,7 @@ struct DevMesh
     int32_t* evalCounts;      // patch evaluations per vertex (accumulated)
     void* getmeTet;           // GETMe (vpp, weight) per tet corner, sized for FP64
     void* getmeHex;           // same per hex corner
+    void* getmeQual;          // GETMe: current quality per element (tets, then hexes), sized for FP64
     // Quality reduction of this mesh.
     double* evalPartials;     // 3 per block: min, sum, compensation
     int32_t* evalFlags;       // per block: 1 when an element quality <= 0
@@ -84,8 +85,13 @@ struct LaneShape
 };
 
 // Span (x, y, z, w) = (mesh index, first group node, first index of the
-// span in the launch, unused).  Block spans use y = 0 and z = first block.
-// Spans are sorted by z; index i belongs to the last span with z <= i.
+// span in the launch, items in the span).  Block spans use y = 0 and
+// z = first block.  Spans are sorted by z; index i belongs to the last span
+// with z <= i.  Vertex spans start at multiples of ITEM_ALIGN, so the items
+// of one wavefront of a wave or lane kernel (at most 8) lie in one span; the
+// indices between a span's last item and the next span are holes.
+constexpr uint32_t ITEM_ALIGN = 8;
+constexpr uint32_t NO_ITEM = 0xffffffffu;
 __host__ __device__ inline uint32_t spanOf(const uint4* spans, uint32_t nspans, uint32_t i)
 {
     uint32_t lo = 0, hi = nspans - 1;

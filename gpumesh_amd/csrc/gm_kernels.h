@@ -30,11 +30,30 @@ template<> struct PosOf<float>
     static float4* host(const DevMesh& m) { return m.pos32; }
 };
 
-// Work item i of a vertex launch: (mesh index, vertex id).
+// Work item i of a vertex launch: (mesh index, vertex id), vertex id NO_ITEM
+// for a hole.
 __device__ __forceinline__ uint2 workItem(const DevMesh* meshes, const uint4* segs, uint32_t nsegs, uint32_t i)
 {
     const uint4 s = segs[spanOf(segs, nsegs, i)];
-    return make_uint2(s.x, meshes[s.x].groupNodes[s.y + (i - s.z)]);
+    const uint32_t r = i - s.z;
+    return make_uint2(s.x, r < s.w ? meshes[s.x].groupNodes[s.y + r] : NO_ITEM);
+}
+
+// The same for a kernel whose wavefront's items lie in one ITEM_ALIGN window
+// (wave and lane kernels): the span is found once per wavefront from its
+// first active lane, so the search, the span and the mesh descriptor are
+// wavefront-uniform (scalar loads).
+__device__ __forceinline__ uint4 waveSpan(const uint4* segs, uint32_t nsegs, uint32_t i)
+{
+    const uint32_t first = __builtin_amdgcn_readfirstlane(i);
+    const uint32_t k = __builtin_amdgcn_readfirstlane(spanOf(segs, nsegs, first));
+    return segs[k];
+}
+
+__device__ __forceinline__ uint32_t spanVertex(const DevMesh& m, const uint4& s, uint32_t i)
+{
+    const uint32_t r = i - s.z;
+    return r < s.w ? m.groupNodes[s.y + r] : NO_ITEM;
 }
 
 // Block b of a block launch: (mesh index, block index within the mesh).
@@ -132,6 +151,8 @@ __global__ void __launch_bounds__(256) smoothPlainKernel(const DevMesh* __restri
         return;
     typedef typename PosOf<T>::type P;
     const uint2 it = workItem(meshes, segs, nsegs, i);
+    if(it.y == NO_ITEM)
+        return;
     const DevMesh& m = meshes[it.x];
     P* pos = PosOf<T>::get(m);
     const uint32_t v = it.y;
@@ -170,6 +191,13 @@ struct WavePatch
     __device__ T operator()(v3<T> p)
     {
         ++count;
+        return reduce(elemQuality(p));
+    }
+
+    // Quality of this lane's element with the centre vertex at p (1 past the
+    // patch).
+    __device__ T elemQuality(v3<T> p) const
+    {
         T q = T(1);
         if(lane < nElem)
         {
@@ -188,6 +216,13 @@ struct WavePatch
                 q = gmq::hexQuality<T, EV, SA>(*c, e);
             }
         }
+        return q;
+    }
+
+    // Patch quality from the lanes' element qualities
+    // (AbstractEvaluator::patchQuality rule); every lane of the group gets it.
+    __device__ T reduce(T q)
+    {
         if constexpr(std::is_same<T, float>::value)
         {
             // FP32: the same patch rule over a butterfly of the group's
@@ -243,10 +278,12 @@ __global__ void __launch_bounds__(64) smoothWaveKernel(const DevMesh* __restrict
     // Groups past the end leave; groups never synchronise with each other.
     if(i >= count)
         return;
-    const uint2 it = workItem(meshes, segs, nsegs, i);
-    const DevMesh& m = meshes[it.x];
+    const uint4 sp = waveSpan(segs, nsegs, i);
+    const DevMesh& m = meshes[sp.x];
+    const uint32_t v = spanVertex(m, sp, i);
+    if(v == NO_ITEM)
+        return;
     P* pos = PosOf<T>::get(m);
-    const uint32_t v = it.y;
     WavePatch<T, EV, SA, G> f;
     f.c = &c;
     f.red = red + 2 * G * grp;
@@ -403,11 +440,12 @@ __global__ void __launch_bounds__(512) spawnLaneKernel(const DevMesh* __restrict
     T* sp = (T*)base;
     uint8_t* sk = base + (size_t)cap * Slots<EK>::N * 3 * sizeof(T);
     const uint32_t i = blockIdx.x * ppg + wave;
-    const bool present = i < count;
-    const uint2 it = present ? workItem(meshes, segs, nsegs, i) : make_uint2(0, 0);
-    const DevMesh& m = meshes[it.x];
+    const uint4 span = waveSpan(segs, nsegs, i);
+    const DevMesh& m = meshes[span.x];
+    const uint32_t item = i < count ? spanVertex(m, span, i) : NO_ITEM;
+    const bool present = item != NO_ITEM;
+    const uint32_t v = present ? item : 0;
     P* pos = PosOf<T>::get(m);
-    const uint32_t v = it.y;
     const uint32_t eb = m.eoff[v];
     const int nE = present ? (int)(m.eoff[v + 1] - eb) : 0;
     for(int e = lane; e < nE; e += 64)
@@ -450,11 +488,12 @@ __global__ void __launch_bounds__(512) gdLaneKernel(const DevMesh* __restrict__ 
     T* mySp = (T*)mine;
     uint8_t* mySk = mine + (size_t)cap * SL * sizeof(T);
     const uint32_t i = blockIdx.x * (blockDim.x >> 3) + g;
-    const bool present = i < count;
-    const uint2 it = present ? workItem(meshes, segs, nsegs, i) : make_uint2(0, 0);
-    const DevMesh& m = meshes[it.x];
+    const uint4 span = waveSpan(segs, nsegs, i);
+    const DevMesh& m = meshes[span.x];
+    const uint32_t item = i < count ? spanVertex(m, span, i) : NO_ITEM;
+    const bool present = item != NO_ITEM;
+    const uint32_t v = present ? item : 0;
     P* pos = PosOf<T>::get(m);
-    const uint32_t v = it.y;
     const uint32_t eb = m.eoff[v];
     const int nE = present ? (int)(m.eoff[v + 1] - eb) : 0;
     for(int e = s; e < nE; e += 8)
@@ -715,11 +754,12 @@ __global__ void __launch_bounds__(512) spawnAffineKernel(const DevMesh* __restri
     const int ppg = blockDim.x >> 6;
     unsigned char* sp = smem + (size_t)wave * cap * affineBytes<T, EK>();
     const uint32_t i = blockIdx.x * ppg + wave;
-    const bool present = i < count;
-    const uint2 it = present ? workItem(meshes, segs, nsegs, i) : make_uint2(0, 0);
-    const DevMesh& m = meshes[it.x];
+    const uint4 span = waveSpan(segs, nsegs, i);
+    const DevMesh& m = meshes[span.x];
+    const uint32_t item = i < count ? spanVertex(m, span, i) : NO_ITEM;
+    const bool present = item != NO_ITEM;
+    const uint32_t v = present ? item : 0;
     P* pos = PosOf<T>::get(m);
-    const uint32_t v = it.y;
     const uint32_t eb = m.eoff[v];
     const int nE = present ? (int)(m.eoff[v + 1] - eb) : 0;
     const v3<T> o = present ? ld<T>(pos, v) : mk<T>(T(0), T(0), T(0));
@@ -766,11 +806,12 @@ __global__ void __launch_bounds__(512) gdAffineKernel(const DevMesh* __restrict_
     const int g = lane >> 3, s = lane & 7, base = lane & ~7;
     unsigned char* mySp = smem + (size_t)g * cap * affineBytes<T, EK>();
     const uint32_t i = blockIdx.x * (blockDim.x >> 3) + g;
-    const bool present = i < count;
-    const uint2 it = present ? workItem(meshes, segs, nsegs, i) : make_uint2(0, 0);
-    const DevMesh& m = meshes[it.x];
+    const uint4 span = waveSpan(segs, nsegs, i);
+    const DevMesh& m = meshes[span.x];
+    const uint32_t item = i < count ? spanVertex(m, span, i) : NO_ITEM;
+    const bool present = item != NO_ITEM;
+    const uint32_t v = present ? item : 0;
     P* pos = PosOf<T>::get(m);
-    const uint32_t v = it.y;
     const uint32_t eb = m.eoff[v];
     const int nE = present ? (int)(m.eoff[v + 1] - eb) : 0;
     const v3<T> o = present ? ld<T>(pos, v) : mk<T>(T(0), T(0), T(0));
@@ -880,10 +921,12 @@ __global__ void __launch_bounds__(64) nmAffineWaveKernel(const DevMesh* __restri
     const uint32_t i = blockIdx.x * (64 / G) + grp;
     if(i >= count)
         return;
-    const uint2 it = workItem(meshes, segs, nsegs, i);
-    const DevMesh& m = meshes[it.x];
+    const uint4 sp = waveSpan(segs, nsegs, i);
+    const DevMesh& m = meshes[sp.x];
+    const uint32_t v = spanVertex(m, sp, i);
+    if(v == NO_ITEM)
+        return;
     P* pos = PosOf<T>::get(m);
-    const uint32_t v = it.y;
     const uint32_t eb = m.eoff[v], ee = m.eoff[v + 1];
     AffineWavePatch<T, EK, G> f;
     f.lane = lane;
@@ -934,7 +977,9 @@ static void launchNmAffine(const VertexWork& w, hipStream_t stream)
 // ---------------------------------------------------------------------------
 // GETMe element step (Smoothers/ElementWise/GetmeSmoother.cpp:55-127, 245-374):
 // every element's transformed positions and weight against the pass-start
-// positions; out[n_vertices_per_element * id + k] = (vpp_k, weight).
+// positions; out[n_vertices_per_element * id + k] = (vpp_k, weight).  The
+// element's quality goes to getmeQual (tets, then hexes), which the vertex
+// steps of the pass keep current.
 // ---------------------------------------------------------------------------
 template<typename T, int EV, int SA>
 __global__ void __launch_bounds__(ELEM_BLOCK) getmeTetKernel(const DevMesh* __restrict__ meshes,
@@ -971,6 +1016,7 @@ __global__ void __launch_bounds__(ELEM_BLOCK) getmeTetKernel(const DevMesh* __re
     T quality = gmq::tetQuality<T, EV, SA>(c, vp);
     T qualityPrime = gmq::tetQuality<T, EV, SA>(c, vpp);
     T weight = qualityPrime / (T(1) + quality);
+    ((T*)m.getmeQual)[e] = quality;
 #pragma unroll
     for(int k = 0; k < 4; ++k)
     {
@@ -1025,6 +1071,7 @@ __global__ void __launch_bounds__(ELEM_BLOCK) getmeHexKernel(const DevMesh* __re
     T quality = gmq::hexQuality<T, EV, SA>(c, vp);
     T qualityPrime = gmq::hexQuality<T, EV, SA>(c, vpp);
     T weight = qualityPrime / (T(1) + quality);
+    ((T*)m.getmeQual)[m.ntets + e] = quality;
 #pragma unroll
     for(int k = 0; k < 8; ++k)
     {
@@ -1049,6 +1096,8 @@ __global__ void __launch_bounds__(512) getmeVertexKernel(const DevMesh* __restri
         return;
     typedef typename PosOf<T>::type P;
     const uint2 it = workItem(meshes, segs, nsegs, i);
+    if(it.y == NO_ITEM)
+        return;
     const DevMesh& m = meshes[it.x];
     P* pos = PosOf<T>::get(m);
     const P* tetOut = (const P*)m.getmeTet;
@@ -1082,8 +1131,13 @@ __global__ void __launch_bounds__(512) getmeVertexKernel(const DevMesh* __restri
 
 // GETMe vertex step, one vertex per group of G lanes (G >= patch size): lane j
 // pulls element j's contribution and holds its vertices; the weighted sum is
-// formed in patch order through LDS (bit-identical to getmeVertexKernel) and
-// the two patch evaluations run lane-parallel as in smoothWaveKernel.
+// formed in patch order through LDS (bit-identical to getmeVertexKernel).
+// The patch quality before the move is reduced from getmeQual: every element
+// quality there was computed from the element's current vertex positions
+// (by the element step, or by the accepted move that last changed one of
+// them), so it has the bits a fresh evaluation would give.  The quality after
+// the move is evaluated lane-parallel as in smoothWaveKernel and, when the
+// move is kept, written back.
 template<typename T, int EV, int SA, int G>
 __global__ void __launch_bounds__(64) getmeWaveKernel(const DevMesh* __restrict__ meshes,
                                                       const uint4* __restrict__ segs, uint32_t nsegs,
@@ -1097,12 +1151,14 @@ __global__ void __launch_bounds__(64) getmeWaveKernel(const DevMesh* __restrict_
     const uint32_t i = blockIdx.x * (64 / G) + grp;
     if(i >= count)
         return;
-    const uint2 it = workItem(meshes, segs, nsegs, i);
-    const DevMesh& m = meshes[it.x];
+    const uint4 sp = waveSpan(segs, nsegs, i);
+    const DevMesh& m = meshes[sp.x];
+    const uint32_t v = spanVertex(m, sp, i);
+    if(v == NO_ITEM)
+        return;
     P* pos = PosOf<T>::get(m);
     const P* tetOut = (const P*)m.getmeTet;
     const P* hexOut = (const P*)m.getmeHex;
-    const uint32_t v = it.y;
     WavePatch<T, EV, SA, G> f;
     f.c = &c;
     f.red = red + 2 * G * grp;
@@ -1113,12 +1169,17 @@ __global__ void __launch_bounds__(64) getmeWaveKernel(const DevMesh* __restrict_
     f.isHex = 0;
     f.k = 0;
     T* pl = pull + 4 * G * grp;
+    T* qual = (T*)m.getmeQual;
+    uint32_t qslot = 0;
+    T qOld = T(1);
     if(lane < f.nElem)
     {
         const uint32_t ref = m.eref[eb + lane];
         const uint32_t id = ref >> 4;
         f.k = (int)(ref & 7);
         f.isHex = (ref >> 3) & 1;
+        qslot = f.isHex ? m.ntets + id : id;
+        qOld = qual[qslot];
         P o;
         if(!f.isHex)
         {
@@ -1156,15 +1217,18 @@ __global__ void __launch_bounds__(64) getmeWaveKernel(const DevMesh* __restrict_
     __builtin_amdgcn_wave_barrier();
     if(wacc != T(0))
     {
-        const v3<T> p0 = ld<T>(pos, v);
         const v3<T> p1 = acc / wacc;
-        const T q0 = f(p0);
-        const T q1 = f(p1);
+        const T q0 = f.reduce(qOld);
+        const T qNew = f.elemQuality(p1);
+        const T q1 = f.reduce(qNew);
+        const bool keep = !(q1 < q0);
+        if(keep && lane < f.nElem)
+            qual[qslot] = qNew;
         if(lane == 0)
         {
-            if(!(q1 < q0))
+            if(keep)
                 st(pos, v, p1);
-            m.evalCounts[v] += f.count;
+            m.evalCounts[v] += 2;
         }
     }
 }

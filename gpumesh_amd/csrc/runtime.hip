@@ -201,7 +201,8 @@ size_t residentBytes(const gm_mesh* m, int records, bool getme)
     if(records > 0) b += allocBytes<double>(3 * (size_t)records) + allocBytes<int32_t>(20 * (size_t)records);
     if(getme)
         b += sizeClass(std::max<size_t>(32, 4 * (size_t)t.ntets * sizeof(double4))) +
-             sizeClass(std::max<size_t>(32, 8 * (size_t)t.nhexs * sizeof(double4)));
+             sizeClass(std::max<size_t>(32, 8 * (size_t)t.nhexs * sizeof(double4))) +
+             sizeClass(std::max<size_t>(32, ((size_t)t.ntets + (size_t)t.nhexs) * sizeof(double)));
     return b;
 }
 
@@ -210,7 +211,7 @@ static void freeMeshBuffers(gm_mesh* m)
     gmd::DevMesh& d = m->dm;
     void* ptrs[] = {d.pos64, d.pos32, (void*)d.tets, (void*)d.hexs, (void*)d.eoff, (void*)d.eref,
                     (void*)d.noff, (void*)d.nbr, (void*)d.fixed, (void*)d.groupNodes, d.evalCounts, d.getmeTet,
-                    d.getmeHex, d.evalPartials, d.evalFlags, d.evalHist, d.evalResult, d.evalHistOut, d.passSlot,
+                    d.getmeHex, d.getmeQual, d.evalPartials, d.evalFlags, d.evalHist, d.evalResult, d.evalHistOut, d.passSlot,
                     m->elemQualBuf, m->texels};
     for(void* p : ptrs)
         poolFree(p);
@@ -394,9 +395,12 @@ static int buildPlanInto(Plan* p, const std::vector<gm_mesh*>& meshes)
                     if(m->kind != kind || g >= t.ngroups() || patchClass(t.groupMaxPatch[g]) != cls) continue;
                     const uint64_t n = (uint64_t)(t.groupOff[g + 1] - t.groupOff[g]);
                     if(n == 0) continue;
-                    if((uint64_t)l.count + n >= (1ull << 32)) return fail(GM_ERR_ARG, "colour step too large");
-                    spans.push_back(make_uint4((uint32_t)i, (uint32_t)t.groupOff[g], l.count, 0));
-                    l.count += (uint32_t)n;
+                    // Each member starts at a multiple of ITEM_ALIGN (the
+                    // items of a wavefront never span two members).
+                    const uint64_t z = ((uint64_t)l.count + gmd::ITEM_ALIGN - 1) / gmd::ITEM_ALIGN * gmd::ITEM_ALIGN;
+                    if(z + n >= (1ull << 32)) return fail(GM_ERR_ARG, "colour step too large");
+                    spans.push_back(make_uint4((uint32_t)i, (uint32_t)t.groupOff[g], (uint32_t)z, (uint32_t)n));
+                    l.count = (uint32_t)(z + n);
                     l.maxPatch = std::max(l.maxPatch, (int)t.groupMaxPatch[g]);
                 }
                 l.nsegs = (uint32_t)(spans.size() - l.firstSpan);
@@ -497,8 +501,10 @@ static int ensureGetme(gm_mesh* m)
     // front half of each buffer.
     const size_t tetBytes = 4 * (size_t)m->topo->ntets * sizeof(double4);
     const size_t hexBytes = 8 * (size_t)m->topo->nhexs * sizeof(double4);
+    const size_t qualBytes = ((size_t)m->topo->ntets + (size_t)m->topo->nhexs) * sizeof(double);
     GMR_HIP(poolAlloc(&m->dm.getmeTet, std::max<size_t>(32, tetBytes)));
     GMR_HIP(poolAlloc(&m->dm.getmeHex, std::max<size_t>(32, hexBytes)));
+    GMR_HIP(poolAlloc(&m->dm.getmeQual, std::max<size_t>(32, qualBytes)));
     m->getmeReady = true;
     return GM_OK;
 }
