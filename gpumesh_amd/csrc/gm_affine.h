@@ -123,12 +123,20 @@ GMA Corner<T> makeCorner(v3<T> c0, v3<T> c1, v3<T> c2, T r0, T r1, T r2)
     return k;
 }
 
-// Corner at offset x (|x|^2 = xx): returns d and f2.
+GMA float fmaT(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+GMA double fmaT(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
+// Corner at offset x (|x|^2 = xx): d and f2 as FMA chains from the constant.
+template<typename T> GMA T cornerDet(const Corner<T>& k, v3<T> x)
+{
+    return fmaT(k.gz, x.z, fmaT(k.gy, x.y, fmaT(k.gx, x.x, k.d0)));
+}
+
 template<typename T>
 GMA void cornerAt(const Corner<T>& k, v3<T> x, T xx, T& d, T& f2)
 {
-    d = k.d0 + (k.gx * x.x + k.gy * x.y + k.gz * x.z);
-    f2 = k.c0 + (k.hx * x.x + k.hy * x.y + k.hz * x.z) + k.s * xx;
+    d = cornerDet<T>(k, x);
+    f2 = fmaT(k.s, xx, fmaT(k.hz, x.z, fmaT(k.hy, x.y, fmaT(k.hx, x.x, k.c0))));
 }
 
 // 3 / q of a corner with d > 0 (the factor 3 is applied once per patch).
@@ -263,36 +271,41 @@ template<typename T, class Fix> GMA Hex<T> stageHex(const v3<T> u[8], int k, uin
             bad = true;
         }
     }
-    h.constInv3 = inv3;
+    // A non-positive constant corner makes the fast sum NaN (the exact rule
+    // reads constMin only).
+    h.constInv3 = bad ? T(__builtin_nan("")) : inv3;
     h.constMin = bad ? mn : T(1);
     return h;
 }
 
-// Branch-free sums for the common case: 3/q of a corner assuming d > 0, and
-// whether that assumption failed.  A patch whose elements are all positive
-// has quality 3 n / sum; otherwise the caller evaluates it again with the
-// exact rule (addTet / addHex), which is rare.
+// Branch-free sums for the common case: 3/q of a corner, which is NaN
+// (d < 0: the FP32 log of a negative number; FP64: explicit) or +inf (d == 0,
+// or d so small that |d|^(-2/3) overflows) when the corner is not positive.
+// A patch sum that is not finite (fastBad) sends the caller to the exact rule
+// (addTet / addHex), which is rare; otherwise the patch quality is 3 n / sum.
 template<typename T>
-GMA T fastInv3(const Corner<T>& k, v3<T> x, T xx, bool& bad)
+GMA T fastInv3(const Corner<T>& k, v3<T> x, T xx)
 {
     T d, f2;
     cornerAt<T>(k, x, xx, d, f2);
-    bad = bad || !(d > T(0));
-    return f2 * powm23(d > T(0) ? d : T(1));
+    if constexpr(sizeof(T) == 4)
+        return f2 * powm23(d);
+    else
+        return d > T(0) ? f2 * powm23(d) : T(__builtin_nan(""));
 }
 
-template<typename T>
-GMA T fastTet(const Corner<T>& k, v3<T> x, T xx, bool& bad) { return fastInv3<T>(k, x, xx, bad); }
+template<typename T> GMA T fastTet(const Corner<T>& k, v3<T> x, T xx) { return fastInv3<T>(k, x, xx); }
 
 template<typename T>
-GMA T fastHex(const Hex<T>& h, v3<T> x, T xx, bool& bad)
+GMA T fastHex(const Hex<T>& h, v3<T> x, T xx)
 {
     T s = h.constInv3;
-    bad = bad || !(h.constMin > T(0));
 #pragma unroll
-    for(int c = 0; c < 4; ++c) s += fastInv3<T>(h.k[c], x, xx, bad);
+    for(int c = 0; c < 4; ++c) s += fastInv3<T>(h.k[c], x, xx);
     return T(0.125) * s;
 }
+
+template<typename T> GMA bool fastBad(T sum) { return !(sum < T(__builtin_huge_val())); }
 
 // Adds one staged tetrahedron at offset x to the accumulator.
 template<typename T>

@@ -649,7 +649,7 @@ __device__ __forceinline__ void inversionTest(const unsigned char* sp, int e, v3
                                               const DetFix<EK>& fix, bool& nonPos32, bool& flip)
 {
     auto one = [&](const gma::Corner<float>& k) {
-        const float d = k.d0 + (k.gx * x.x + k.gy * x.y + k.gz * x.z);
+        const float d = gma::cornerDet<float>(k, x);
         if(!(d > 0.0f)) nonPos32 = true;
         else if(d < k.thr && !(fix.det64(k, pf) > 0.0f)) flip = true;
     };
@@ -714,9 +714,10 @@ __device__ __forceinline__ T affinePatchFast(const unsigned char* sp, int nE, v3
     T inv3 = T(0);
     for(int e = 0; e < nE; ++e)
     {
-        if(EK == 0) inv3 += gma::fastTet<T>(((const gma::Corner<T>*)sp)[e], x, xx, bad);
-        else inv3 += gma::fastHex<T>(((const gma::Hex<T>*)sp)[e], x, xx, bad);
+        if(EK == 0) inv3 += gma::fastTet<T>(((const gma::Corner<T>*)sp)[e], x, xx);
+        else inv3 += gma::fastHex<T>(((const gma::Hex<T>*)sp)[e], x, xx);
     }
+    bad = gma::fastBad<T>(inv3);
     return T(3) * (T)nE / inv3;
 }
 
@@ -788,6 +789,127 @@ __global__ void __launch_bounds__(512) spawnAffineKernel(const DevMesh* __restri
         scale /= T(3);
     }
     const v3<T> np = testedMove<T, EK>(m, sp, nE, lane, 64, 0, o, xb);
+    if(lane == 0)
+    {
+        st(pos, v, np);
+        m.evalCounts[v] += 128;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Matrix-core evaluation of FP32 affine corner terms.  A patch's corner rows
+// (a tetrahedron's corner, or the four corners of a hexahedron that move
+// with the vertex) go four at a time through v_mfma_f32_4x4x1f32 (16 blocks
+// of 4x4, K = 1): lane l supplies the coefficient of row 4g + (l & 3) as the
+// A operand and the coordinate of its own candidate as B, and receives
+// D[r] = A(lane 4 (l / 4) + r) * B(lane l) + C[r] -- rows 4g..4g+3 at its
+// candidate -- in every 4-lane block.  Two chains build
+//   d  = d0 + gx x + gy y + gz z,
+//   f2 = c0 + hx x + hy y + hz z + s |x|^2,
+// and the vector ALUs only form f2 |d|^(-2/3) and the sum.  Every lane of the
+// wavefront takes part (the matrix instructions run on the whole wavefront);
+// the row count is a multiple of 4 (tetrahedron patches are padded with
+// neutral rows, d = 1 and f2 = 0).
+// ---------------------------------------------------------------------------
+typedef float mfv4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ mfv4 mfma4x4(float a, float b, mfv4 c)
+{
+    return __builtin_amdgcn_mfma_f32_4x4x1f32(a, b, c, 0, 0, 0);
+}
+
+template<int EK>
+__device__ __forceinline__ const gma::Corner<float>& cornerRow(const unsigned char* sp, int r)
+{
+    if(EK == 0) return ((const gma::Corner<float>*)sp)[r];
+    return ((const gma::Hex<float>*)sp)[r >> 2].k[r & 3];
+}
+
+// Sum over the patch's corner rows of f2 |d|^(-2/3) at this lane's candidate
+// offset x (|x|^2 = xx); NaN or +inf when a corner is not positive.
+template<int EK>
+__device__ __forceinline__ float mfmaRowSum(const unsigned char* sp, int nRows, v3<float> x, float xx)
+{
+    const int sub = threadIdx.x & 3;
+    float sum = 0.0f;
+    for(int r0 = 0; r0 < nRows; r0 += 4)
+    {
+        const gma::Corner<float>& k = cornerRow<EK>(sp, r0 + sub);
+        mfv4 d = {0.0f, 0.0f, 0.0f, 0.0f}, f = {0.0f, 0.0f, 0.0f, 0.0f};
+        d = mfma4x4(k.d0, 1.0f, d);
+        f = mfma4x4(k.c0, 1.0f, f);
+        d = mfma4x4(k.gx, x.x, d);
+        f = mfma4x4(k.hx, x.x, f);
+        d = mfma4x4(k.gy, x.y, d);
+        f = mfma4x4(k.hy, x.y, f);
+        d = mfma4x4(k.gz, x.z, d);
+        f = mfma4x4(k.hz, x.z, f);
+        f = mfma4x4(k.s, xx, f);
+#pragma unroll
+        for(int r = 0; r < 4; ++r) sum += f[r] * gma::powm23(d[r]);
+    }
+    return sum;
+}
+
+// Spawn search, FP32 affine terms on the matrix cores (the schedule, the
+// candidates and the rules of spawnAffineKernel).
+template<int EK>
+__global__ void __launch_bounds__(512) spawnMfmaKernel(const DevMesh* __restrict__ meshes,
+                                                       const uint4* __restrict__ segs, uint32_t nsegs,
+                                                       uint32_t count, float moveCoeff, int cap)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int ppg = blockDim.x >> 6;
+    unsigned char* sp = smem + (size_t)wave * cap * affineBytes<float, EK>();
+    const uint32_t i = blockIdx.x * ppg + wave;
+    const uint4 span = waveSpan(segs, nsegs, i);
+    const DevMesh& m = meshes[span.x];
+    const uint32_t item = i < count ? spanVertex(m, span, i) : NO_ITEM;
+    const bool present = item != NO_ITEM;
+    const uint32_t v = present ? item : 0;
+    float4* pos = m.pos32;
+    const uint32_t eb = m.eoff[v];
+    const int nE = present ? (int)(m.eoff[v + 1] - eb) : 0;
+    const v3<float> o = present ? ld<float>(pos, v) : mk<float>(0.0f, 0.0f, 0.0f);
+    for(int e = lane; e < nE; e += 64)
+        stageAffine<float, EK>(m, pos, m.eref[eb + e], o, sp + (size_t)e * affineBytes<float, EK>());
+    // Neutral rows up to a multiple of 4 (the tile size is a multiple of 4).
+    const int nRows = EK == 0 ? (nE + 3) & ~3 : 4 * nE;
+    if(EK == 0 && nE + lane < nRows)
+    {
+        gma::Corner<float> z{};
+        z.d0 = 1.0f;
+        ((gma::Corner<float>*)sp)[nE + lane] = z;
+    }
+    __syncthreads();
+    if(!present)
+        return;
+    float constSum = 0.0f;
+    if(EK == 1)
+        for(int e = 0; e < nE; ++e) constSum += ((const gma::Hex<float>*)sp)[e].constInv3;
+    float scale = moveCoeff * localSize<float>(m, pos, v, o);
+    v3<float> xb = mk<float>(0.0f, 0.0f, 0.0f);
+    for(int iter = 0; iter < 2; ++iter)
+    {
+        const v3<float> x = xb + gms::spawnOffset<float>(lane) * scale;
+        const float xx = gmq::dot(x, x);
+        float sum = mfmaRowSum<EK>(sp, nRows, x, xx);
+        if(EK == 1) sum = 0.125f * (sum + constSum);
+        const bool bad = gma::fastBad<float>(sum);
+        float q = 3.0f * (float)nE / sum;
+        // As in spawnAffineKernel: a candidate with a non-positive element
+        // loses unless every candidate has one.
+        if(__ballot(!bad) == 0)
+            q = affinePatchExact<float, EK>(sp, nE, x, xx);
+        else if(bad)
+            q = -__builtin_huge_valf();
+        const int best = groupArgmax<float>(q, lane, 64);
+        xb = shflv<float>(x, best);
+        scale /= 3.0f;
+    }
+    const v3<float> np = testedMove<float, EK>(m, sp, nE, lane, 64, 0, o, xb);
     if(lane == 0)
     {
         st(pos, v, np);
@@ -880,17 +1002,12 @@ struct AffineWavePatch
         const v3<T> x = p - o;
         const T xx = gmq::dot(x, x);
         T sum = T(0);
-        bool badLane = false;
         if(lane < nElem)
-            sum = EK == 0 ? gma::fastTet<T>(cn, x, xx, badLane) : gma::fastHex<T>(hx, x, xx, badLane);
-        int bad = badLane ? 1 : 0;
+            sum = EK == 0 ? gma::fastTet<T>(cn, x, xx) : gma::fastHex<T>(hx, x, xx);
 #pragma unroll
         for(int off = G / 2; off > 0; off >>= 1)
-        {
             sum += __shfl_xor(sum, off, 64);
-            bad |= __shfl_xor(bad, off, 64);
-        }
-        if(!bad)
+        if(!gma::fastBad<T>(sum))
             return T(3) * (T)nElem / sum;
         // A non-positive element: the smallest non-positive quality.
         gma::Acc<T> a;
@@ -1412,8 +1529,12 @@ static hipError_t launchLane(const VertexWork& w, const gmq::Crew& c, LaneShape 
         if constexpr(AffineCrew<EV, SA>::on)
             if(aff)
             {
-                spawnAffineKernel<T, EK><<<blocks, 64 * ppg, lds, stream>>>(w.meshes, w.segs, w.nsegs, w.count,
-                                                                            (T)moveCoeff, cap);
+                if constexpr(sizeof(T) == 4)
+                    spawnMfmaKernel<EK><<<blocks, 64 * ppg, lds, stream>>>(w.meshes, w.segs, w.nsegs, w.count,
+                                                                           (float)moveCoeff, cap);
+                else
+                    spawnAffineKernel<T, EK><<<blocks, 64 * ppg, lds, stream>>>(w.meshes, w.segs, w.nsegs, w.count,
+                                                                                (T)moveCoeff, cap);
                 return hipGetLastError();
             }
         spawnLaneKernel<T, EV, SA, EK><<<blocks, 64 * ppg, lds, stream>>>(w.meshes, w.segs, w.nsegs, w.count, c,

@@ -18,17 +18,19 @@ from gpumesh_amd import GpuMesh, cube_mesh
 
 pytestmark = pytest.mark.gpu
 
-# (smoother, passes, bound on max |dx|/h, bound on |d mean|, bound on |d min|)
-CASES = [("nm", 10, 0.5, 5e-5, 2e-2), ("gd", 10, 0.5, 5e-5, 2e-2), ("spawn", 10, 0.5, 5e-5, 2e-2),
-         ("getme", 10, 0.5, 5e-5, 2e-2)]
+# (element kind, cube cells, smoother, passes, bound on max |dx|/h, bound on
+# |d mean|, bound on |d min|)
+CASES = [("tet", 20, "nm", 10, 0.5, 5e-5, 2e-2), ("tet", 20, "gd", 10, 0.5, 5e-5, 2e-2),
+         ("tet", 20, "spawn", 10, 0.5, 5e-5, 2e-2), ("tet", 20, "getme", 10, 0.5, 5e-5, 2e-2),
+         ("hex", 16, "spawn", 10, 0.5, 5e-5, 2e-2), ("hex", 16, "gd", 10, 0.5, 5e-5, 2e-2),
+         ("hex", 16, "nm", 10, 0.5, 5e-5, 2e-2)]
 _RESULTS = {}
 
 
 @pytest.mark.parametrize("precision", ["fp32", "fp32_polish"])
-@pytest.mark.parametrize("smoother,passes,bx,bmean,bmin", CASES)
-def test_fp32_error_against_fp64_oracle(oracle, smoother, passes, bx, bmean, bmin, precision):
-    n = 20
-    m = cube_mesh(n, "tet", 0.3, seed=1)          # the small plain-method workload
+@pytest.mark.parametrize("kind,n,smoother,passes,bx,bmean,bmin", CASES)
+def test_fp32_error_against_fp64_oracle(oracle, kind, n, smoother, passes, bx, bmean, bmin, precision):
+    m = cube_mesh(n, kind, 0.3, seed=1)           # the small plain-method workload (tets)
     coeff = 0.25 if smoother == "spawn" else 1.0 / 20.0
     om = oracle.OracleMesh.from_cube(m)
     ref = om.smooth(smoother, passes, threads=8, spawn_coeff=coeff)
@@ -38,7 +40,7 @@ def test_fp32_error_against_fp64_oracle(oracle, smoother, passes, bx, bmean, bmi
     dx = float(np.max(np.abs(gm.positions() - om.positions())) / h)
     dmean = float(abs(out["mean"][-1] - ref["mean"][-1]))
     dmin = float(abs(out["min"][-1] - ref["min"][-1]))
-    _RESULTS[f"{smoother}/{precision}"] = {"max_dx_over_h": dx, "mean_err": dmean, "min_err": dmin,
+    _RESULTS[f"{kind}/{smoother}/{precision}"] = {"max_dx_over_h": dx, "mean_err": dmean, "min_err": dmin,
                                            "fp64_mean": float(ref["mean"][-1]), "fp64_min": float(ref["min"][-1])}
     out_dir = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "gpurun_out")
     if os.path.isdir(out_dir):
