@@ -27,8 +27,13 @@ EXPORTS = [
     "gm_mesh_device_bytes", "gm_batch_create", "gm_batch_destroy", "gm_batch_set_shape", "gm_batch_smooth",
     "gm_batch_last_timing", "gm_batch_kernel_times", "gm_batch_autotune", "gm_server_create", "gm_server_destroy", "gm_server_set_shape", "gm_server_submit", "gm_server_wait",
     "gm_server_result", "gm_server_release", "gm_server_stats", "gm_cube_mesh", "gm_set_topology_cache", "gm_set_device_pool",
-    "gm_topology_cache_stats", "gm_math_probe",
+    "gm_topology_cache_stats", "gm_math_probe", "gm_mesh_smooth_schedule", "gm_batch_smooth_schedule",
 ]
+
+
+class Schedule(C.Structure):
+    """gm_schedule: autopilot flag and the two gain thresholds (reference names)."""
+    _fields_ = [("autopilot", C.c_int), ("min_qual_threshold", C.c_double), ("qual_mean_threshold", C.c_double)]
 
 
 class GpuMeshError(RuntimeError):
@@ -71,6 +76,10 @@ def lib():
     L.gm_batch_destroy.argtypes = [vp]
     L.gm_batch_set_shape.argtypes = [vp, C.c_int, C.c_int, C.c_int]
     L.gm_batch_smooth.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, dbl, C.c_int, vp, vp]
+    L.gm_mesh_smooth_schedule.argtypes = [vp, C.c_int, C.c_int, C.POINTER(Schedule), C.c_int, C.c_int, dbl, vp, vp,
+                                          vp, vp]
+    L.gm_batch_smooth_schedule.argtypes = [vp, C.c_int, C.c_int, C.POINTER(Schedule), C.c_int, C.c_int, dbl,
+                                           C.c_int, vp, vp, vp]
     L.gm_batch_last_timing.argtypes = [vp, vp]
     L.gm_batch_kernel_times.argtypes = [vp, vp, vp]
     L.gm_batch_autotune.argtypes = [vp, C.c_int, C.c_int, C.c_int, vp, vp]

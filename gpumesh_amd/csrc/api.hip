@@ -68,16 +68,39 @@ gmd::LaneShape defaultLane(const std::vector<gm_mesh*>&, int smoother)
     return gmd::LaneShape{smoother == gmd::SM_GETME ? 4 : 1, 0};
 }
 
+// Runs the plan and copies each member's records (stats/hist strided by
+// passes + 1 records per member); passesRun[i] = sweeps member i ran.
 int runAndFetch(gmr::Plan* p, const gm_mesh* m0, const gmr::RunParams& r, double* stats, int32_t* hist,
-                int32_t* evalCounts)
+                int32_t* evalCounts, int32_t* passesRun = nullptr)
 {
     int rc = gmr::runPlan(p, m0->crew, m0->evaluator, m0->sampler, r);
     if(rc) return rc;
     const size_t per = (size_t)r.passes + 1;
     for(size_t i = 0; i < p->meshes.size(); ++i)
-        if((rc = gmr::fetchRecords(p, i, (int)per, stats ? stats + i * 3 * per : nullptr,
+    {
+        int records = (int)per;
+        if(r.autopilot && (rc = gmr::recordsWritten(p, i, &records))) return rc;
+        if((rc = gmr::fetchRecords(p, i, records, stats ? stats + i * 3 * per : nullptr,
                                    hist ? hist + i * 20 * per : nullptr, evalCounts)))
             return rc;
+        if(passesRun) passesRun[i] = records - 1;
+    }
+    return GM_OK;
+}
+
+int scheduleParams(const gm_schedule* sched, int precision, gmr::RunParams& r)
+{
+    if(!sched) return fail(GM_ERR_ARG, "schedule is null");
+    if(!sched->autopilot) return GM_OK;
+    if(precision == GM_FP32_POLISH)
+        return fail(GM_ERR_UNSUPPORTED, "the autopilot decides after each pass whether another runs, so no pass "
+                                        "is known to be the last one (use GM_FP64 or GM_FP32)");
+    if(!(sched->min_qual_threshold == sched->min_qual_threshold) ||
+       !(sched->qual_mean_threshold == sched->qual_mean_threshold))
+        return fail(GM_ERR_ARG, "autopilot thresholds must not be NaN");
+    r.autopilot = true;
+    r.apMeanGain = sched->min_qual_threshold;
+    r.apMinGain = sched->qual_mean_threshold;
     return GM_OK;
 }
 
@@ -340,6 +363,29 @@ int gm_mesh_smooth(gm_mesh* m, int smoother, int passes, int precision, int kern
     return runAndFetch(m->plan, m, r, stats, hist, evalCounts);
 }
 
+int gm_mesh_smooth_schedule(gm_mesh* m, int smoother, int maxPasses, const gm_schedule* sched, int precision,
+                            int kernel, double spawnCoeff, double* stats, int32_t* hist, int32_t* evalCounts,
+                            int32_t* passesRun)
+{
+    GM_NEED_DEVICE(m);
+    int rc = checkRun(smoother, maxPasses, precision, kernel, spawnCoeff);
+    if(rc) return rc;
+    if((rc = checkSampler(m->sampler, smoother, kernel))) return rc;
+    GM_HIP(hipSetDevice(m->device));
+    gmr::RunParams r;
+    r.smoother = smoother;
+    r.passes = maxPasses;
+    r.precision = precision;
+    r.kernel = kernel;
+    r.spawnCoeff = spawnCoeff;
+    r.lane = defaultLane(m->plan->meshes, smoother);
+    if((rc = scheduleParams(sched, precision, r))) return rc;
+    int32_t run = 0;
+    rc = runAndFetch(m->plan, m, r, stats, hist, evalCounts, &run);
+    if(!rc && passesRun) *passesRun = run;
+    return rc;
+}
+
 int gm_mesh_smooth_groups(gm_mesh* m, int smoother, int64_t groupBegin, int64_t groupEnd, int precision,
                           int kernel, double spawnCoeff, int32_t* evalCounts)
 {
@@ -480,6 +526,30 @@ int gm_batch_smooth(gm_batch* b, int smoother, int passes, int precision, int ke
     r.lane = b->lane[smoother].patchesPerGroup ? b->lane[smoother] : defaultLane(b->plan->meshes, smoother);
     r.graph = useGraph != 0;
     return runAndFetch(b->plan, b->plan->meshes[0], r, stats, hist, nullptr);
+}
+
+int gm_batch_smooth_schedule(gm_batch* b, int smoother, int maxPasses, const gm_schedule* sched, int precision,
+                             int kernel, double spawnCoeff, int useGraph, double* stats, int32_t* hist,
+                             int32_t* passesRun)
+{
+    if(!b) return fail(GM_ERR_ARG, "null batch");
+    int rc = checkRun(smoother, maxPasses, precision, kernel, spawnCoeff);
+    if(rc) return rc;
+    if((rc = checkBatchCrew(b))) return rc;
+    GM_HIP(hipSetDevice(b->device));
+    gmr::RunParams r;
+    r.smoother = smoother;
+    r.passes = maxPasses;
+    r.precision = precision;
+    r.kernel = kernel;
+    r.spawnCoeff = spawnCoeff;
+    r.lane = b->lane[smoother].patchesPerGroup ? b->lane[smoother] : defaultLane(b->plan->meshes, smoother);
+    r.graph = useGraph != 0;
+    if((rc = scheduleParams(sched, precision, r))) return rc;
+    std::vector<int32_t> run(b->plan->meshes.size(), 0);
+    rc = runAndFetch(b->plan, b->plan->meshes[0], r, stats, hist, nullptr, run.data());
+    if(!rc && passesRun) std::memcpy(passesRun, run.data(), run.size() * sizeof(int32_t));
+    return rc;
 }
 
 int gm_batch_last_timing(const gm_batch* b, double* out2)

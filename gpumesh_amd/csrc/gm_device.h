@@ -63,11 +63,26 @@ struct DevMesh
     double* evalResult;       // [min, invSum, n] per recorded pass
     int32_t* evalHistOut;     // 20 per recorded pass
     double* elemQual;         // optional per-element qualities (nullptr = off)
-    int32_t* passSlot;        // next pass record to write (advanced by the final kernel)
+    int32_t* passSlot;        // [0] next pass record to write (advanced by the final kernel),
+                              // [1] 1 once the autopilot stopped the run (pass kernels then do nothing)
     uint32_t nverts, ntets, nhexs;
     uint32_t evalBlocks;
     uint32_t resultCap;       // pass records evalResult / evalHistOut hold
+    // Pass schedule (AbstractSmoother::evaluateMeshQuality,
+    // Smoothers/AbstractSmoother.cpp:186-197): with autopilot set, the record
+    // k >= 1 stops the run unless meanGain > apMeanGain or minGain > apMinGain
+    // (Schedule::minQualThreshold and Schedule::qualMeanThreshold, which the
+    // reference compares with those gains in that order).
+    int32_t autopilot;
+    double apMeanGain;
+    double apMinGain;
 };
+
+// True once the autopilot stopped this mesh's run: the pass kernels leave.
+__device__ __forceinline__ bool runStopped(const DevMesh& m)
+{
+    return m.autopilot && m.passSlot[1] != 0;
+}
 
 enum { SM_NELDER_MEAD = 0, SM_GRADIENT_DESCENT = 1, SM_SPAWN_SEARCH = 2, SM_GETME = 3 };
 enum { KS_PLAIN = 0, KS_WAVE = 1 };

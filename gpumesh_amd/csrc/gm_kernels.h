@@ -89,6 +89,23 @@ template<typename T, typename P> __device__ T localSize(const DevMesh& m, const 
     return total / (T)(e - b);
 }
 
+// The same in FP32 with the neighbours spread over a group of `width` lanes
+// (hardware square root, tree sum); every lane of the group gets the result.
+__device__ __forceinline__ float localSizeGroup(const DevMesh& m, const float4* pos, uint32_t v, v3<float> p,
+                                                int lane, int width)
+{
+    const uint32_t b = m.noff[v], e = m.noff[v + 1];
+    float total = 0.0f;
+    for(uint32_t r = b + lane; r < e; r += width)
+    {
+        const v3<float> d = p - ld<float>(pos, m.nbr[r]);
+        total += __builtin_amdgcn_sqrtf(gmq::dot(d, d));
+    }
+    for(int off = width >> 1; off > 0; off >>= 1)
+        total += __shfl_xor(total, off, 64);
+    return total * __builtin_amdgcn_rcpf((float)(e - b));
+}
+
 // Quality of the patch element `ref` with the patch's centre vertex at p.
 template<typename T, int EV, int SA, typename P>
 __device__ __forceinline__ T elementQualityAt(const DevMesh& m, const gmq::Crew& c, const P* pos, uint32_t ref, v3<T> p)
@@ -151,7 +168,7 @@ __global__ void __launch_bounds__(256) smoothPlainKernel(const DevMesh* __restri
         return;
     typedef typename PosOf<T>::type P;
     const uint2 it = workItem(meshes, segs, nsegs, i);
-    if(it.y == NO_ITEM)
+    if(it.y == NO_ITEM || runStopped(meshes[it.x]))
         return;
     const DevMesh& m = meshes[it.x];
     P* pos = PosOf<T>::get(m);
@@ -281,7 +298,7 @@ __global__ void __launch_bounds__(64) smoothWaveKernel(const DevMesh* __restrict
     const uint4 sp = waveSpan(segs, nsegs, i);
     const DevMesh& m = meshes[sp.x];
     const uint32_t v = spanVertex(m, sp, i);
-    if(v == NO_ITEM)
+    if(v == NO_ITEM || runStopped(m))
         return;
     P* pos = PosOf<T>::get(m);
     WavePatch<T, EV, SA, G> f;
@@ -410,6 +427,18 @@ __device__ __forceinline__ int groupArgmax(T q, int lane, int width)
     return idx;
 }
 
+// The same rule over a whole wavefront: the largest key, the lowest lane
+// among equal keys (a maximum reduction, then the first lane holding it).
+__device__ __forceinline__ int waveArgmax(float key)
+{
+    key = (key == key) ? key : -__builtin_huge_valf();
+    float mx = key;
+#pragma unroll
+    for(int off = 32; off > 0; off >>= 1)
+        mx = __builtin_fmaxf(mx, __shfl_xor(mx, off, 64));
+    return (int)__builtin_ctzll(__ballot(key == mx));
+}
+
 template<typename T>
 __device__ __forceinline__ v3<T> shflv(v3<T> a, int src)
 {
@@ -442,7 +471,7 @@ __global__ void __launch_bounds__(512) spawnLaneKernel(const DevMesh* __restrict
     const uint32_t i = blockIdx.x * ppg + wave;
     const uint4 span = waveSpan(segs, nsegs, i);
     const DevMesh& m = meshes[span.x];
-    const uint32_t item = i < count ? spanVertex(m, span, i) : NO_ITEM;
+    const uint32_t item = i < count && !runStopped(m) ? spanVertex(m, span, i) : NO_ITEM;
     const bool present = item != NO_ITEM;
     const uint32_t v = present ? item : 0;
     P* pos = PosOf<T>::get(m);
@@ -490,7 +519,7 @@ __global__ void __launch_bounds__(512) gdLaneKernel(const DevMesh* __restrict__ 
     const uint32_t i = blockIdx.x * (blockDim.x >> 3) + g;
     const uint4 span = waveSpan(segs, nsegs, i);
     const DevMesh& m = meshes[span.x];
-    const uint32_t item = i < count ? spanVertex(m, span, i) : NO_ITEM;
+    const uint32_t item = i < count && !runStopped(m) ? spanVertex(m, span, i) : NO_ITEM;
     const bool present = item != NO_ITEM;
     const uint32_t v = present ? item : 0;
     P* pos = PosOf<T>::get(m);
@@ -757,7 +786,7 @@ __global__ void __launch_bounds__(512) spawnAffineKernel(const DevMesh* __restri
     const uint32_t i = blockIdx.x * ppg + wave;
     const uint4 span = waveSpan(segs, nsegs, i);
     const DevMesh& m = meshes[span.x];
-    const uint32_t item = i < count ? spanVertex(m, span, i) : NO_ITEM;
+    const uint32_t item = i < count && !runStopped(m) ? spanVertex(m, span, i) : NO_ITEM;
     const bool present = item != NO_ITEM;
     const uint32_t v = present ? item : 0;
     P* pos = PosOf<T>::get(m);
@@ -866,7 +895,7 @@ __global__ void __launch_bounds__(512) spawnMfmaKernel(const DevMesh* __restrict
     const uint32_t i = blockIdx.x * ppg + wave;
     const uint4 span = waveSpan(segs, nsegs, i);
     const DevMesh& m = meshes[span.x];
-    const uint32_t item = i < count ? spanVertex(m, span, i) : NO_ITEM;
+    const uint32_t item = i < count && !runStopped(m) ? spanVertex(m, span, i) : NO_ITEM;
     const bool present = item != NO_ITEM;
     const uint32_t v = present ? item : 0;
     float4* pos = m.pos32;
@@ -889,7 +918,7 @@ __global__ void __launch_bounds__(512) spawnMfmaKernel(const DevMesh* __restrict
     float constSum = 0.0f;
     if(EK == 1)
         for(int e = 0; e < nE; ++e) constSum += ((const gma::Hex<float>*)sp)[e].constInv3;
-    float scale = moveCoeff * localSize<float>(m, pos, v, o);
+    float scale = moveCoeff * localSizeGroup(m, pos, v, o, lane, 64);
     v3<float> xb = mk<float>(0.0f, 0.0f, 0.0f);
     for(int iter = 0; iter < 2; ++iter)
     {
@@ -898,16 +927,18 @@ __global__ void __launch_bounds__(512) spawnMfmaKernel(const DevMesh* __restrict
         float sum = mfmaRowSum<EK>(sp, nRows, x, xx);
         if(EK == 1) sum = 0.125f * (sum + constSum);
         const bool bad = gma::fastBad<float>(sum);
-        float q = 3.0f * (float)nE / sum;
-        // As in spawnAffineKernel: a candidate with a non-positive element
-        // loses unless every candidate has one.
+        // The patch quality 3 n / sum falls as the sum grows, so the best
+        // candidate has the smallest sum.  As in spawnAffineKernel, a
+        // candidate with a non-positive element loses unless every candidate
+        // has one; then the exact rule decides.
+        float key;
         if(__ballot(!bad) == 0)
-            q = affinePatchExact<float, EK>(sp, nE, x, xx);
-        else if(bad)
-            q = -__builtin_huge_valf();
-        const int best = groupArgmax<float>(q, lane, 64);
+            key = affinePatchExact<float, EK>(sp, nE, x, xx);
+        else
+            key = bad ? -__builtin_huge_valf() : -sum;
+        const int best = waveArgmax(key);
         xb = shflv<float>(x, best);
-        scale /= 3.0f;
+        scale *= 1.0f / 3.0f;
     }
     const v3<float> np = testedMove<float, EK>(m, sp, nE, lane, 64, 0, o, xb);
     if(lane == 0)
@@ -930,7 +961,7 @@ __global__ void __launch_bounds__(512) gdAffineKernel(const DevMesh* __restrict_
     const uint32_t i = blockIdx.x * (blockDim.x >> 3) + g;
     const uint4 span = waveSpan(segs, nsegs, i);
     const DevMesh& m = meshes[span.x];
-    const uint32_t item = i < count ? spanVertex(m, span, i) : NO_ITEM;
+    const uint32_t item = i < count && !runStopped(m) ? spanVertex(m, span, i) : NO_ITEM;
     const bool present = item != NO_ITEM;
     const uint32_t v = present ? item : 0;
     P* pos = PosOf<T>::get(m);
@@ -1041,7 +1072,7 @@ __global__ void __launch_bounds__(64) nmAffineWaveKernel(const DevMesh* __restri
     const uint4 sp = waveSpan(segs, nsegs, i);
     const DevMesh& m = meshes[sp.x];
     const uint32_t v = spanVertex(m, sp, i);
-    if(v == NO_ITEM)
+    if(v == NO_ITEM || runStopped(m))
         return;
     P* pos = PosOf<T>::get(m);
     const uint32_t eb = m.eoff[v], ee = m.eoff[v + 1];
@@ -1106,7 +1137,7 @@ __global__ void __launch_bounds__(ELEM_BLOCK) getmeTetKernel(const DevMesh* __re
     const uint2 bl = blockItem(spans, nspans, blockIdx.x);
     const DevMesh& m = meshes[bl.x];
     const uint32_t e = bl.y * ELEM_BLOCK + threadIdx.x;
-    if(e >= m.ntets)
+    if(e >= m.ntets || runStopped(m))
         return;
     typedef typename PosOf<T>::type P;
     const P* pos = PosOf<T>::get(m);
@@ -1151,7 +1182,7 @@ __global__ void __launch_bounds__(ELEM_BLOCK) getmeHexKernel(const DevMesh* __re
     const uint2 bl = blockItem(spans, nspans, blockIdx.x);
     const DevMesh& m = meshes[bl.x];
     const uint32_t e = bl.y * ELEM_BLOCK + threadIdx.x;
-    if(e >= m.nhexs)
+    if(e >= m.nhexs || runStopped(m))
         return;
     typedef typename PosOf<T>::type P;
     const P* pos = PosOf<T>::get(m);
@@ -1213,7 +1244,7 @@ __global__ void __launch_bounds__(512) getmeVertexKernel(const DevMesh* __restri
         return;
     typedef typename PosOf<T>::type P;
     const uint2 it = workItem(meshes, segs, nsegs, i);
-    if(it.y == NO_ITEM)
+    if(it.y == NO_ITEM || runStopped(meshes[it.x]))
         return;
     const DevMesh& m = meshes[it.x];
     P* pos = PosOf<T>::get(m);
@@ -1271,7 +1302,7 @@ __global__ void __launch_bounds__(64) getmeWaveKernel(const DevMesh* __restrict_
     const uint4 sp = waveSpan(segs, nsegs, i);
     const DevMesh& m = meshes[sp.x];
     const uint32_t v = spanVertex(m, sp, i);
-    if(v == NO_ITEM)
+    if(v == NO_ITEM || runStopped(m))
         return;
     P* pos = PosOf<T>::get(m);
     const P* tetOut = (const P*)m.getmeTet;
@@ -1381,11 +1412,13 @@ __global__ void __launch_bounds__(EVAL_THREADS) evalKernel(const DevMesh* __rest
     __shared__ double smin[EVAL_THREADS], ssum[EVAL_THREADS], scmp[EVAL_THREADS];
     __shared__ int shist[20];
     __shared__ int sbad;
+    const uint2 bl = blockItem(spans, nspans, blockIdx.x);
+    const DevMesh& m = meshes[bl.x];
+    if(runStopped(m))
+        return;
     if(threadIdx.x < 20) shist[threadIdx.x] = 0;
     if(threadIdx.x == 0) sbad = 0;
     __syncthreads();
-    const uint2 bl = blockItem(spans, nspans, blockIdx.x);
-    const DevMesh& m = meshes[bl.x];
     const P* pos = PosOf<typename std::conditional<std::is_same<P, double4>::value, double, float>::type>::get(m);
     const uint64_t E = (uint64_t)m.ntets + m.nhexs;
     const uint64_t chunk = (E + m.evalBlocks - 1) / m.evalBlocks;
@@ -1460,10 +1493,10 @@ static __global__ void __launch_bounds__(EVAL_THREADS) evalFinalKernel(const Dev
     __shared__ double smin[EVAL_THREADS], ssum[EVAL_THREADS], scmp[EVAL_THREADS];
     __shared__ int sbad;
     const DevMesh& m = meshes[meshList[blockIdx.x]];
-    const int slot = *m.passSlot;
+    const int slot = m.passSlot[0];
     // The host sizes the record buffers for the run; a slot past them is
     // never written.
-    if(slot < 0 || (uint32_t)slot >= m.resultCap)
+    if(slot < 0 || (uint32_t)slot >= m.resultCap || runStopped(m))
         return;
     if(threadIdx.x == 0) sbad = 0;
     __syncthreads();
@@ -1495,9 +1528,21 @@ static __global__ void __launch_bounds__(EVAL_THREADS) evalFinalKernel(const Dev
     }
     if(threadIdx.x == 0)
     {
-        m.evalResult[3 * slot] = smin[0];
-        m.evalResult[3 * slot + 1] = sbad ? (double)__builtin_huge_val() : ssum[0] + scmp[0];
-        m.evalResult[3 * slot + 2] = (double)m.ntets + (double)m.nhexs;
+        const double mn0 = smin[0];
+        const double inv = sbad ? (double)__builtin_huge_val() : ssum[0] + scmp[0];
+        const double n = (double)m.ntets + (double)m.nhexs;
+        m.evalResult[3 * slot] = mn0;
+        m.evalResult[3 * slot + 1] = inv;
+        m.evalResult[3 * slot + 2] = n;
+        if(m.autopilot && slot > 0)
+        {
+            // Gains against the previous record, as AbstractSmoother does
+            // (harmonic mean = n / sum of 1/q).
+            const double minGain = mn0 - m.evalResult[3 * (slot - 1)];
+            const double geomGain = n / inv - m.evalResult[3 * (slot - 1) + 2] / m.evalResult[3 * (slot - 1) + 1];
+            if(!((geomGain > m.apMeanGain) || (minGain > m.apMinGain)))
+                m.passSlot[1] = 1;
+        }
     }
     if(threadIdx.x < 20)
     {
@@ -1506,7 +1551,7 @@ static __global__ void __launch_bounds__(EVAL_THREADS) evalFinalKernel(const Dev
     }
     __syncthreads();
     if(threadIdx.x == 0)
-        *m.passSlot = slot + 1;
+        m.passSlot[0] = slot + 1;
 }
 
 // ---------------------------------------------------------------------------

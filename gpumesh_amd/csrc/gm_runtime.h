@@ -44,6 +44,10 @@ struct RunParams
     gmd::LaneShape lane{1, 0};   // tileElems 0: chosen per work list
     bool graph = false;
     bool timeKernels = true;   // per-launch events (not inside graphs)
+    // Autopilot (at most `passes` sweeps; see DevMesh::autopilot).
+    bool autopilot = false;
+    double apMeanGain = 0.001;   // Schedule::minQualThreshold
+    double apMinGain = 0.0;      // Schedule::qualMeanThreshold
 };
 
 struct Plan;
@@ -144,6 +148,8 @@ int launchPass(Plan* p, const gmq::Crew& crew, int evaluator, int sampler, const
 int launchToF64All(Plan* p);
 // Copies the pass records of mesh i (stats 3 per record, hist 20 per record).
 int fetchRecords(Plan* p, size_t i, int records, double* stats, int32_t* hist, int32_t* evalCounts);
+// Pass records mesh i holds after the last run (sweeps run + 1).
+int recordsWritten(Plan* p, size_t i, int* records);
 
 } // namespace gmr
 

@@ -42,7 +42,8 @@ __global__ void resetSlotsKernel(const DevMesh* __restrict__ meshes, const uint3
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if(i >= nmeshes) return;
     const DevMesh& m = meshes[meshList[i]];
-    *m.passSlot = 0;
+    m.passSlot[0] = 0;
+    m.passSlot[1] = 0;
     for(int b = 0; b < 20; ++b) m.evalHist[b] = 0;
 }
 

@@ -197,7 +197,7 @@ size_t residentBytes(const gm_mesh* m, int records, bool getme)
                allocBytes<uint32_t>(t.eref.size()) + allocBytes<uint32_t>(t.nbr.size()) + allocBytes<uint8_t>(V) +
                allocBytes<uint32_t>(t.groupNodes.size()) + allocBytes<int32_t>(V) +
                allocBytes<double>(3 * (size_t)eb) + allocBytes<int32_t>(eb) + allocBytes<int32_t>(20) +
-               allocBytes<int32_t>(1);
+               allocBytes<int32_t>(2);
     if(records > 0) b += allocBytes<double>(3 * (size_t)records) + allocBytes<int32_t>(20 * (size_t)records);
     if(getme)
         b += sizeClass(std::max<size_t>(32, 4 * (size_t)t.ntets * sizeof(double4))) +
@@ -306,7 +306,7 @@ int uploadMesh(gm_mesh* m)
     GM_TRY(devAlloc(&d.evalPartials, 3 * (size_t)d.evalBlocks));
     GM_TRY(devAlloc(&d.evalFlags, (size_t)d.evalBlocks));
     GM_TRY(devAlloc(&d.evalHist, 20));
-    GM_TRY(devAlloc(&d.passSlot, 1));
+    GM_TRY(devAlloc(&d.passSlot, 2));
 
     hipStream_t s = m->stream;
     if(V)
@@ -682,6 +682,12 @@ static int passBody(Plan* p, const gmq::Crew& c, int ev, int sa, const RunParams
 
 int runPlan(Plan* p, const gmq::Crew& c, int ev, int sa, const RunParams& r)
 {
+    for(gm_mesh* m : p->meshes)
+    {
+        m->dm.autopilot = r.autopilot ? 1 : 0;
+        m->dm.apMeanGain = r.apMeanGain;
+        m->dm.apMinGain = r.apMinGain;
+    }
     int rc = preparePlan(p, r.passes + 1, r.smoother == gmd::SM_GETME);
     if(rc) return rc;
     hipStream_t s = p->stream;
@@ -745,6 +751,7 @@ int launchToF64All(Plan* p)
 
 int sweepColours(Plan* p, const gmq::Crew& c, int ev, int sa, const RunParams& r, int64_t g0, int64_t g1)
 {
+    for(gm_mesh* m : p->meshes) m->dm.autopilot = 0;
     int rc = preparePlan(p, 1, false);
     if(rc) return rc;
     hipStream_t s = p->stream;
@@ -757,6 +764,15 @@ int sweepColours(Plan* p, const gmq::Crew& c, int ev, int sa, const RunParams& r
     if(e != hipSuccess) return fail(GM_ERR_HIP, std::string("sweep: ") + hipGetErrorString(e));
     if(!f64) GMR_HIP(gmd::launchToF64(p->dTable, p->vertBlocks, s));
     GMR_HIP(hipStreamSynchronize(s));
+    return GM_OK;
+}
+
+int recordsWritten(Plan* p, size_t i, int* records)
+{
+    int32_t slot[2] = {0, 0};
+    GMR_HIP(hipMemcpyAsync(slot, p->meshes[i]->dm.passSlot, sizeof(slot), hipMemcpyDeviceToHost, p->stream));
+    GMR_HIP(hipStreamSynchronize(p->stream));
+    *records = slot[0];
     return GM_OK;
 }
 

@@ -12,7 +12,7 @@ import ctypes as C
 import numpy as np
 
 from . import _lib
-from ._lib import check, lib, ptr
+from ._lib import Schedule, check, lib, ptr
 
 SMOOTHERS = {"nm": 0, "nelder_mead": 0, "gd": 1, "gradient_descent": 1, "spawn": 2, "spawn_search": 2,
              "getme": 3}
@@ -158,6 +158,28 @@ class GpuMesh:
             r["eval_counts"] = cnt
         return r
 
+    def smooth_autopilot(self, smoother: str, max_passes: int, min_qual_threshold: float = 0.001,
+                         qual_mean_threshold: float = 0.0, precision: str = "fp64", kernel: str = "wave",
+                         spawn_coeff: float = 1.0 / 20.0, counts: bool = True):
+        """Smoothing under the reference's autopilot (gm_mesh_smooth_schedule):
+        sweeps continue while the harmonic-mean gain exceeds min_qual_threshold
+        or the minimum gain exceeds qual_mean_threshold, at most max_passes.
+        Returns the records of the passes run ("passes" = sweeps run)."""
+        stats = np.zeros((max_passes + 1, 3), np.float64)
+        hist = np.zeros((max_passes + 1, 20), np.int32)
+        cnt = np.zeros(self.nverts, np.int32) if counts else None
+        sched = Schedule(1, float(min_qual_threshold), float(qual_mean_threshold))
+        run = np.zeros(1, np.int32)
+        check(lib().gm_mesh_smooth_schedule(self._h, SMOOTHERS[smoother], int(max_passes), C.byref(sched),
+                                            PRECISIONS[precision], KERNELS[kernel], float(spawn_coeff), ptr(stats),
+                                            ptr(hist), ptr(cnt), ptr(run)), "gm_mesh_smooth_schedule")
+        k = int(run[0]) + 1
+        r = {"passes": int(run[0]), "min": stats[:k, 0].copy(), "mean": stats[:k, 1].copy(),
+             "inv_sum": stats[:k, 2].copy(), "hist": hist[:k].copy()}
+        if counts:
+            r["eval_counts"] = cnt
+        return r
+
 
 def _timings(L, fn_t, fn_k, h):
     t = np.zeros(2, np.float64)
@@ -215,5 +237,26 @@ class GpuBatch:
                                 float(spawn_coeff), int(bool(graph)), ptr(stats), ptr(hist)), "gm_batch_smooth")
         r = {"min": stats[:, :, 0].copy(), "mean": stats[:, :, 1].copy(), "inv_sum": stats[:, :, 2].copy(),
              "hist": hist}
+        r.update(_timings(L, L.gm_batch_last_timing, L.gm_batch_kernel_times, self._h))
+        return r
+
+    def smooth_autopilot(self, smoother: str, max_passes: int, min_qual_threshold: float = 0.001,
+                         qual_mean_threshold: float = 0.0, precision: str = "fp64", kernel: str = "wave",
+                         spawn_coeff: float = 1.0 / 20.0, graph: bool = False):
+        """GpuMesh.smooth_autopilot over every member (each stops on its own
+        records); "passes" holds the sweeps each member ran, records past a
+        member's last one are zero."""
+        n = len(self.meshes)
+        stats = np.zeros((n, max_passes + 1, 3), np.float64)
+        hist = np.zeros((n, max_passes + 1, 20), np.int32)
+        run = np.zeros(n, np.int32)
+        sched = Schedule(1, float(min_qual_threshold), float(qual_mean_threshold))
+        L = lib()
+        check(L.gm_batch_smooth_schedule(self._h, SMOOTHERS[smoother], int(max_passes), C.byref(sched),
+                                         PRECISIONS[precision], KERNELS[kernel], float(spawn_coeff),
+                                         int(bool(graph)), ptr(stats), ptr(hist), ptr(run)),
+              "gm_batch_smooth_schedule")
+        r = {"passes": run, "min": stats[:, :, 0].copy(), "mean": stats[:, :, 1].copy(),
+             "inv_sum": stats[:, :, 2].copy(), "hist": hist}
         r.update(_timings(L, L.gm_batch_last_timing, L.gm_batch_kernel_times, self._h))
         return r

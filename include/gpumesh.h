@@ -121,6 +121,37 @@ int gm_mesh_evaluate(gm_mesh* mesh, double* out3, int32_t* hist20, double* elem_
 int gm_mesh_smooth(gm_mesh* mesh, int smoother, int passes, int precision, int kernel,
                    double spawn_coeff, double* stats, int32_t* hist, int32_t* eval_counts);
 
+/* Pass schedule of AbstractSmoother::evaluateMeshQuality
+ * (Smoothers/AbstractSmoother.cpp:103-211; defaults Schedule.cpp:4-13:
+ * autopilot off, thresholds 0.001 and 0.0).  autopilot == 0: exactly
+ * max_passes sweeps (the schedule's relocationPassCount).  autopilot != 0:
+ * after pass record 0 a sweep runs if max_passes > 0; after record k >= 1 the
+ * next sweep runs while
+ *   mean_k - mean_{k-1} > min_qual_threshold  ||  min_k - min_{k-1} > qual_mean_threshold
+ * -- the reference compares the harmonic-mean gain with minQualThreshold and
+ * the minimum gain with qualMeanThreshold (AbstractSmoother.cpp:186-193), and
+ * these fields keep its names.  The reference's autopilot has no pass limit;
+ * here max_passes bounds it.  The decision is made on the device after each
+ * pass's quality reduction: the host queues max_passes passes without
+ * waiting, and the passes after the stop do no work. */
+typedef struct gm_schedule
+{
+    int autopilot;
+    double min_qual_threshold;
+    double qual_mean_threshold;
+} gm_schedule;
+
+/* gm_mesh_smooth under a schedule.  stats/hist are sized for max_passes + 1
+ * records; the first *passes_run + 1 are written.  The autopilot takes
+ * precision GM_FP64 or GM_FP32 (GM_FP32_POLISH needs a known last pass:
+ * GM_ERR_UNSUPPORTED).  Replaces the pass loop of
+ * AbstractVertexWiseSmoother::smoothMesh{Serial,Thread}
+ * (AbstractVertexWiseSmoother.cpp:53-152) for a Schedule with topology
+ * operations off. */
+int gm_mesh_smooth_schedule(gm_mesh* mesh, int smoother, int max_passes, const gm_schedule* schedule,
+                            int precision, int kernel, double spawn_coeff, double* stats, int32_t* hist,
+                            int32_t* eval_counts, int32_t* passes_run);
+
 /* One partial sweep of a vertex-wise smoother (NM, GD, spawn search) over the
  * colour groups [group_begin, group_end), with no quality evaluation: the
  * body of one pass of AbstractVertexWiseSmoother::smoothMeshThread
@@ -207,6 +238,12 @@ int gm_batch_set_shape(gm_batch* batch, int smoother, int patches_per_group, int
  * NULL), member order. */
 int gm_batch_smooth(gm_batch* batch, int smoother, int passes, int precision, int kernel, double spawn_coeff,
                     int use_graph, double* stats, int32_t* hist);
+/* gm_mesh_smooth_schedule over every member: each member stops on its own
+ * records; passes_run[n] (may be NULL), stats/hist strided by max_passes + 1
+ * records per member. */
+int gm_batch_smooth_schedule(gm_batch* batch, int smoother, int max_passes, const gm_schedule* schedule,
+                             int precision, int kernel, double spawn_coeff, int use_graph, double* stats,
+                             int32_t* hist, int32_t* passes_run);
 /* out2 = {total device ms of the last gm_batch_smooth, sweep ms (0 with graphs)}. */
 int gm_batch_last_timing(const gm_batch* batch, double* out2);
 /* Per-class device time of the last gm_batch_smooth (not recorded with graphs). */

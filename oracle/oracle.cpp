@@ -1327,15 +1327,25 @@ void orc_smooth_vertices(void* h, int smoother, double spawnCoeff, const uint32_
 }
 
 // Full smoothing run (AbstractSmoother::smoothMesh with topology operations
-// and autopilot off: exactly `passes` sweeps; AbstractSmoother.cpp:103-211).
+// off; AbstractSmoother.cpp:103-211).
 //   smoother: 0 NM, 1 GD, 2 spawn search, 3 GETMe.
+//   passes: Schedule::relocationPassCount.  With autopilot == 0 exactly
+//     `passes` sweeps run.  With autopilot != 0 the sweep after pass record
+//     k >= 1 runs only while
+//       mean_k - mean_{k-1} > minQualThreshold  or  min_k - min_{k-1} > qualMeanThreshold
+//     (AbstractSmoother.cpp:186-193: the harmonic-mean gain is compared with
+//     minQualThreshold and the minimum gain with qualMeanThreshold), and after
+//     record 0 while 0 < passes (:194-197); the reference has no other bound,
+//     here `passes` also caps the sweeps.
 //   order: 0 = serialGroup order on one thread (smoothMeshSerial),
 //          1 = colour-group-major, `threads` workers per group (smoothMeshThread).
 //   evalChunks: 1 = serial histogram sums, k > 1 = k-way chunked sums.
-// stats: (passes+1) x [min, harmonicMean, invSum]; hist: (passes+1) x 20;
-// evalCounts: per-vertex patch evaluation counts (may be null).
-int orc_smooth(void* h, int smoother, int passes, int threads, int order, int evalChunks,
-               double spawnCoeff, double* stats, int32_t* hist, int64_t* evalCounts)
+// stats: (passes+1) x [min, harmonicMean, invSum]; hist: (passes+1) x 20
+// (records past the last one run are left as they are); evalCounts:
+// per-vertex patch evaluation counts (may be null); *passesRun = sweeps run.
+int orc_smooth_schedule(void* h, int smoother, int passes, int autopilot, double minQualThreshold,
+                        double qualMeanThreshold, int threads, int order, int evalChunks, double spawnCoeff,
+                        double* stats, int32_t* hist, int64_t* evalCounts, int* passesRun)
 {
     Mesh& m = *(Mesh*)h;
     size_t V = m.verts.size();
@@ -1347,17 +1357,29 @@ int orc_smooth(void* h, int smoother, int passes, int threads, int order, int ev
 
     bool hasBoundaryTypes = (m.boundaryNodes.end > m.boundaryNodes.begin) ||
                             (m.subsurfaceNodes.end > m.subsurfaceNodes.begin);
+    double lastMin = 0.0, lastMean = 0.0;
+    bool keepGoing = true;
     auto record = [&](int pass) {
         Histogram hh = evaluateMesh(m, evalChunks);
         if(stats) { stats[3*pass] = hh.minQ; stats[3*pass+1] = hh.harmonicMean(); stats[3*pass+2] = hh.invSum; }
         if(hist) for(int i = 0; i < 20; ++i) hist[20*pass+i] = hh.buckets[i];
+        if(autopilot && pass > 0)
+        {
+            const double minGain = hh.minQ - lastMin;
+            const double geomGain = hh.harmonicMean() - lastMean;
+            keepGoing = (geomGain > minQualThreshold) || (minGain > qualMeanThreshold);
+        }
+        lastMin = hh.minQ;
+        lastMean = hh.harmonicMean();
     };
 
     std::vector<ElemOut> tetOut, hexOut;
+    int swept = 0;
     for(int pass = 0; pass <= passes; ++pass)
     {
         record(pass);
-        if(pass == passes) break;
+        if(pass == passes || !keepGoing) break;
+        ++swept;
         if(smoother == 3)
         {
             tetOut.resize(m.tets.size());
@@ -1388,7 +1410,16 @@ int orc_smooth(void* h, int smoother, int passes, int threads, int order, int ev
         }
     }
     if(evalCounts) for(size_t v = 0; v < V; ++v) evalCounts[v] = counts[v];
+    if(passesRun) *passesRun = swept;
     return 0;
+}
+
+// Exactly `passes` sweeps (autopilot off).
+int orc_smooth(void* h, int smoother, int passes, int threads, int order, int evalChunks,
+               double spawnCoeff, double* stats, int32_t* hist, int64_t* evalCounts)
+{
+    return orc_smooth_schedule(h, smoother, passes, 0, 0.0, 0.0, threads, order, evalChunks, spawnCoeff, stats,
+                               hist, evalCounts, nullptr);
 }
 
 } // extern "C"

@@ -70,6 +70,9 @@ def lib():
     L.orc_smooth_vertices.argtypes = [vp, C.c_int, C.c_double, _P(np.uint32), i64, vp]
     L.orc_smooth.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, vp, vp, vp]
     L.orc_smooth.restype = C.c_int
+    L.orc_smooth_schedule.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int,
+                                      C.c_int, C.c_double, vp, vp, vp, vp]
+    L.orc_smooth_schedule.restype = C.c_int
     _LIB = L
     return L
 
@@ -230,3 +233,22 @@ class OracleMesh:
             raise RuntimeError(f"oracle smooth failed: {rc}")
         return {"min": stats[:, 0].copy(), "mean": stats[:, 1].copy(), "inv_sum": stats[:, 2].copy(),
                 "hist": hist, "eval_counts": cnt}
+
+    def smooth_autopilot(self, smoother: str, max_passes: int, min_qual_threshold: float = 0.001,
+                         qual_mean_threshold: float = 0.0, threads: int = 1, order: int = 1, eval_chunks: int = 1,
+                         spawn_coeff: float = 1.0 / 20.0):
+        """The autopilot schedule (Schedule's thresholds, applied as AbstractSmoother does), at most
+        `max_passes` sweeps; records of the sweeps run plus the initial one."""
+        stats = np.zeros((max_passes + 1, 3), np.float64)
+        hist = np.zeros((max_passes + 1, 20), np.int32)
+        cnt = np.zeros(self.nverts, np.int64)
+        run = C.c_int(0)
+        rc = lib().orc_smooth_schedule(self._h, SMOOTHERS[smoother], int(max_passes), 1, float(min_qual_threshold),
+                                       float(qual_mean_threshold), int(threads), int(order), int(eval_chunks),
+                                       float(spawn_coeff), stats.ctypes.data, hist.ctypes.data, cnt.ctypes.data,
+                                       C.addressof(run))
+        if rc != 0:
+            raise RuntimeError(f"oracle smooth failed: {rc}")
+        k = run.value + 1
+        return {"passes": run.value, "min": stats[:k, 0].copy(), "mean": stats[:k, 1].copy(),
+                "inv_sum": stats[:k, 2].copy(), "hist": hist[:k].copy(), "eval_counts": cnt}

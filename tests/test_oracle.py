@@ -320,3 +320,50 @@ def test_libm_modes_agree_closely(oracle):
     d = np.max(np.abs(out[0][0] - out[1][0]))
     assert d < 1e-3 * (1.0 / 8)
     assert abs(out[0][1]["mean"][-1] - out[1][1]["mean"][-1]) < 1e-6
+
+
+@pytest.mark.parametrize("smoother", ["gd", "getme"])
+def test_autopilot_schedule(oracle, smoother):
+    """The autopilot (AbstractSmoother.cpp:186-197): the sweeps are those of
+    the fixed-count run, and the run stops at the first record k >= 1 whose
+    harmonic-mean gain is at most min_qual_threshold and whose minimum gain is
+    at most qual_mean_threshold.  The thresholds differ (a gain above one but
+    not the other), so comparing either gain with the other threshold moves
+    the stop."""
+    m = cube_mesh(6, "tet", 0.35, seed=4)
+    fixed = oracle.OracleMesh.from_cube(m)
+    ref = fixed.smooth(smoother, 12, threads=1)
+    t_mean, t_min = 2e-4, 5e-3
+    auto = oracle.OracleMesh.from_cube(m)
+    r = auto.smooth_autopilot(smoother, 12, min_qual_threshold=t_mean, qual_mean_threshold=t_min, threads=1)
+    k = r["passes"]
+    assert 1 <= k < 12
+    assert np.array_equal(r["min"], ref["min"][:k + 1])
+    assert np.array_equal(r["mean"], ref["mean"][:k + 1])
+
+    def goes_on(j):
+        return (ref["mean"][j] - ref["mean"][j - 1] > t_mean) or (ref["min"][j] - ref["min"][j - 1] > t_min)
+
+    assert all(goes_on(j) for j in range(1, k))
+    assert not goes_on(k)
+    # The stop depends on which gain meets which threshold.
+    swapped = oracle.OracleMesh.from_cube(m).smooth_autopilot(smoother, 12, min_qual_threshold=t_min,
+                                                              qual_mean_threshold=t_mean, threads=1)
+    first = next((j for j in range(1, 13) if not ((ref["mean"][j] - ref["mean"][j - 1] > t_min) or
+                                                   (ref["min"][j] - ref["min"][j - 1] > t_mean))), 12)
+    assert swapped["passes"] == first
+    assert first != k
+    # Positions after k autopilot sweeps are those after k fixed sweeps.
+    again = oracle.OracleMesh.from_cube(m)
+    again.smooth(smoother, k, threads=1)
+    assert np.array_equal(again.positions(), auto.positions())
+
+
+def test_autopilot_limits(oracle):
+    m = cube_mesh(4, "tet", 0.3, seed=2)
+    o = oracle.OracleMesh.from_cube(m)
+    assert o.smooth_autopilot("gd", 0)["passes"] == 0          # no sweep after record 0
+    o = oracle.OracleMesh.from_cube(m)
+    assert o.smooth_autopilot("gd", 3, min_qual_threshold=-1.0)["passes"] == 3   # never stops: the cap
+    o = oracle.OracleMesh.from_cube(m)
+    assert o.smooth_autopilot("gd", 5, min_qual_threshold=1.0, qual_mean_threshold=1.0)["passes"] == 1
