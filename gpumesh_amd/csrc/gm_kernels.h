@@ -747,7 +747,10 @@ __device__ __forceinline__ T affinePatchFast(const unsigned char* sp, int nE, v3
         else inv3 += gma::fastHex<T>(((const gma::Hex<T>*)sp)[e], x, xx);
     }
     bad = gma::fastBad<T>(inv3);
-    return T(3) * (T)nE / inv3;
+    if constexpr(sizeof(T) == 4)
+        return 3.0f * (float)nE * __builtin_amdgcn_rcpf(inv3);
+    else
+        return T(3) * (T)nE / inv3;
 }
 
 // The exact patch rule (smallest non-positive quality when one exists).
@@ -975,7 +978,11 @@ __global__ void __launch_bounds__(512) gdAffineKernel(const DevMesh* __restrict_
         return;
 
     v3<T> x = mk<T>(T(0), T(0), T(0));
-    T nodeShift = localSize<T>(m, pos, v, o) * T(1.0 / 75.0);
+    T nodeShift;
+    if constexpr(sizeof(T) == 4)
+        nodeShift = localSizeGroup(m, pos, v, o, s, 8) * (1.0f / 75.0f);
+    else
+        nodeShift = localSize<T>(m, pos, v, o) * T(1.0 / 75.0);
     const T originalNodeShift = nodeShift;
     int evals = 0;
     for(int cyc = 0; cyc < 5; ++cyc)
@@ -1014,6 +1021,125 @@ __global__ void __launch_bounds__(512) gdAffineKernel(const DevMesh* __restrict_
     }
 }
 
+// Patch quality of this lane's candidate from the FP32 affine rows on the
+// matrix cores (every lane of the wavefront takes part); the exact rule, on
+// the vector ALUs, for a lane whose patch has a non-positive element.
+template<int EK>
+__device__ __forceinline__ float mfmaPatchQuality(const unsigned char* sp, int nRows, int nE, v3<float> x,
+                                                  float constSum)
+{
+    const float xx = gmq::dot(x, x);
+    float sum = mfmaRowSum<EK>(sp, nRows, x, xx);
+    if(EK == 1) sum = 0.125f * (sum + constSum);
+    if(!gma::fastBad<float>(sum))
+        return 3.0f * (float)nE * __builtin_amdgcn_rcpf(sum);
+    return affinePatchExact<float, EK>(sp, nE, x, xx);
+}
+
+// Gradient descent, FP32 affine terms on the matrix cores: the schedule,
+// samples and proposals of gdAffineKernel (8 vertices per wavefront, 8 lanes
+// each).  The matrix instructions need the whole wavefront, so the cycle loop
+// runs while any of its vertices is still descending; a vertex that stopped
+// keeps its position, step and count.  Each vertex's rows are padded with
+// neutral ones up to the wavefront's largest patch.
+template<int EK>
+__global__ void __launch_bounds__(512) gdMfmaKernel(const DevMesh* __restrict__ meshes,
+                                                    const uint4* __restrict__ segs, uint32_t nsegs, uint32_t count,
+                                                    int cap)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int lane = threadIdx.x;
+    const int g = lane >> 3, s = lane & 7, base = lane & ~7;
+    unsigned char* mySp = smem + (size_t)g * cap * affineBytes<float, EK>();
+    const uint32_t i = blockIdx.x * (blockDim.x >> 3) + g;
+    const uint4 span = waveSpan(segs, nsegs, i);
+    const DevMesh& m = meshes[span.x];
+    const uint32_t item = i < count && !runStopped(m) ? spanVertex(m, span, i) : NO_ITEM;
+    const bool present = item != NO_ITEM;
+    const uint32_t v = present ? item : 0;
+    float4* pos = m.pos32;
+    const uint32_t eb = m.eoff[v];
+    const int nE = present ? (int)(m.eoff[v + 1] - eb) : 0;
+    const v3<float> o = present ? ld<float>(pos, v) : mk<float>(0.0f, 0.0f, 0.0f);
+    for(int e = s; e < nE; e += 8)
+        stageAffine<float, EK>(m, pos, m.eref[eb + e], o, mySp + (size_t)e * affineBytes<float, EK>());
+    // Rows up to the wavefront's largest patch (rounded to 4 rows); the tile
+    // holds them (a multiple of 4 elements, at least the largest patch).
+    int nEmax = nE;
+#pragma unroll
+    for(int off = 8; off < 64; off <<= 1)
+        nEmax = max(nEmax, __shfl_xor(nEmax, off, 64));
+    const int nPad = EK == 0 ? (nEmax + 3) & ~3 : nEmax;
+    for(int e = nE + s; e < nPad; e += 8)
+    {
+        if(EK == 0)
+        {
+            gma::Corner<float> z{};
+            z.d0 = 1.0f;
+            ((gma::Corner<float>*)mySp)[e] = z;
+        }
+        else
+        {
+            gma::Hex<float> z{};
+            for(int c = 0; c < 4; ++c) z.k[c].d0 = 1.0f;
+            z.constInv3 = 0.0f;
+            z.constMin = 1.0f;
+            ((gma::Hex<float>*)mySp)[e] = z;
+        }
+    }
+    __syncthreads();
+    const int nRows = EK == 0 ? nPad : 4 * nPad;
+    float constSum = 0.0f;
+    if(EK == 1)
+        for(int e = 0; e < nE; ++e) constSum += ((const gma::Hex<float>*)mySp)[e].constInv3;
+
+    v3<float> x = mk<float>(0.0f, 0.0f, 0.0f);
+    float nodeShift = localSizeGroup(m, pos, v, o, s, 8) * (1.0f / 75.0f);
+    const float originalNodeShift = nodeShift;
+    int evals = 0;
+    bool active = present;
+    const float off = s == 0 ? -0.25f : s == 1 ? 0.00f : s == 2 ? 0.10f : s == 3 ? 0.20f
+                    : s == 4 ? 0.40f : s == 5 ? 0.80f : s == 6 ? 1.20f : 1.60f;
+    for(int cyc = 0; cyc < 5; ++cyc)
+    {
+        if(__ballot(active) == 0)
+            break;
+        const float sx = s == 0 ? -nodeShift : (s == 1 ? nodeShift : 0.0f);
+        const float sy = s == 2 ? -nodeShift : (s == 3 ? nodeShift : 0.0f);
+        const float sz = s == 4 ? -nodeShift : (s == 5 ? nodeShift : 0.0f);
+        const float q = mfmaPatchQuality<EK>(mySp, nRows, nE, x + mk<float>(sx, sy, sz), constSum);
+        if(active) evals += 6;
+        const float q0 = __shfl(q, base + 0, 64), q1 = __shfl(q, base + 1, 64), q2 = __shfl(q, base + 2, 64);
+        const float q3 = __shfl(q, base + 3, 64), q4 = __shfl(q, base + 4, 64), q5 = __shfl(q, base + 5, 64);
+        const v3<float> gq = mk<float>(q1 - q0, q3 - q2, q5 - q4);
+        const float gn = gmq::tsqrt(gmq::dot(gq, gq));
+        if(gn == 0.0f)
+            active = false;
+        const v3<float> shift = gq * (nodeShift / gn);
+        const v3<float> prop = x + shift * off;
+        const float qp = mfmaPatchQuality<EK>(mySp, nRows, nE, prop, constSum);
+        const int best = groupArgmax<float>(qp, s, 8);
+        const v3<float> nx = shflv<float>(prop, (base & 63) + best);
+        const float bestOff = __shfl(off, (base & 63) + best, 64);
+        if(active)
+        {
+            evals += 8;
+            x = nx;
+            nodeShift *= gmq::tabs(bestOff);
+            if(nodeShift < originalNodeShift / 10.0f)
+                active = false;
+        }
+    }
+    if(!present)
+        return;
+    const v3<float> np = testedMove<float, EK>(m, mySp, nE, s, 8, (threadIdx.x & 63) & ~7, o, x);
+    if(s == 0)
+    {
+        st(pos, v, np);
+        m.evalCounts[v] += evals;
+    }
+}
+
 // Nelder-Mead on affine terms: one vertex per group of G lanes, lane j holding
 // element j's staged corner terms in registers; a patch evaluation is one
 // affine element evaluation per lane and a butterfly over the group.
@@ -1039,7 +1165,12 @@ struct AffineWavePatch
         for(int off = G / 2; off > 0; off >>= 1)
             sum += __shfl_xor(sum, off, 64);
         if(!gma::fastBad<T>(sum))
-            return T(3) * (T)nElem / sum;
+        {
+            if constexpr(sizeof(T) == 4)
+                return 3.0f * (float)nElem * __builtin_amdgcn_rcpf(sum);
+            else
+                return T(3) * (T)nElem / sum;
+        }
         // A non-positive element: the smallest non-positive quality.
         gma::Acc<T> a;
         if(lane < nElem)
@@ -1084,7 +1215,11 @@ __global__ void __launch_bounds__(64) nmAffineWaveKernel(const DevMesh* __restri
     if(lane < f.nElem)
         stageAffine<T, EK>(m, pos, m.eref[eb + lane], f.o,
                            EK == 0 ? (unsigned char*)&f.cn : (unsigned char*)&f.hx);
-    const T ls = localSize<T>(m, pos, v, f.o);
+    T ls;
+    if constexpr(sizeof(T) == 4)
+        ls = localSizeGroup(m, pos, v, f.o, lane, G);
+    else
+        ls = localSize<T>(m, pos, v, f.o);
     const v3<T> np = gms::nelderMead<T>(f.o, ls, f);
     // FP64 inversion test of an FP32 move, one element per lane of the group.
     v3<T> out = np;
@@ -1594,7 +1729,11 @@ static hipError_t launchLane(const VertexWork& w, const gmq::Crew& c, LaneShape 
         if constexpr(AffineCrew<EV, SA>::on)
             if(aff)
             {
-                gdAffineKernel<T, EK><<<blocks, 64 * ppg, lds, stream>>>(w.meshes, w.segs, w.nsegs, w.count, cap);
+                if constexpr(sizeof(T) == 4)
+                    gdMfmaKernel<EK><<<blocks, 64 * ppg, lds, stream>>>(w.meshes, w.segs, w.nsegs, w.count, cap);
+                else
+                    gdAffineKernel<T, EK><<<blocks, 64 * ppg, lds, stream>>>(w.meshes, w.segs, w.nsegs, w.count,
+                                                                            cap);
                 return hipGetLastError();
             }
         gdLaneKernel<T, EV, SA, EK><<<blocks, 64 * ppg, lds, stream>>>(w.meshes, w.segs, w.nsegs, w.count, c, cap);
