@@ -89,21 +89,58 @@ template<typename T, typename P> __device__ T localSize(const DevMesh& m, const 
     return total / (T)(e - b);
 }
 
-// The same in FP32 with the neighbours spread over a group of `width` lanes
+// FP32 reductions over aligned groups of G lanes (G a power of two): DPP
+// moves inside rows of 16 lanes (lane ^ 1, lane ^ 2, then the mirrored half
+// row and row, which pair lanes holding different partial results), a
+// shuffle across rows.  The operation is commutative, so every lane of the
+// group ends with the same bits.
+template<int CTRL> __device__ __forceinline__ float dppMove(float v)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, 0xF, 0xF,
+                                                                  false));
+}
+
+enum { DPP_XOR1 = 0xB1, DPP_XOR2 = 0x4E, DPP_HALF_MIRROR = 0x141, DPP_MIRROR = 0x140 };
+
+template<int G, class Op> __device__ __forceinline__ float groupReduce(float v, Op op)
+{
+    if constexpr(G >= 2) v = op(v, dppMove<DPP_XOR1>(v));
+    if constexpr(G >= 4) v = op(v, dppMove<DPP_XOR2>(v));
+    if constexpr(G >= 8) v = op(v, dppMove<DPP_HALF_MIRROR>(v));
+    if constexpr(G >= 16) v = op(v, dppMove<DPP_MIRROR>(v));
+    if constexpr(G >= 32) v = op(v, __shfl_xor(v, 16, 64));
+    if constexpr(G >= 64) v = op(v, __shfl_xor(v, 32, 64));
+    return v;
+}
+
+template<int G> __device__ __forceinline__ float groupSum(float v)
+{
+    return groupReduce<G>(v, [](float a, float b) { return a + b; });
+}
+
+template<int G> __device__ __forceinline__ float groupMax(float v)
+{
+    return groupReduce<G>(v, [](float a, float b) { return __builtin_fmaxf(a, b); });
+}
+
+template<int G> __device__ __forceinline__ float groupMin(float v)
+{
+    return groupReduce<G>(v, [](float a, float b) { return __builtin_fminf(a, b); });
+}
+
+// The same in FP32 with the neighbours spread over a group of W lanes
 // (hardware square root, tree sum); every lane of the group gets the result.
-__device__ __forceinline__ float localSizeGroup(const DevMesh& m, const float4* pos, uint32_t v, v3<float> p,
-                                                int lane, int width)
+template<int W>
+__device__ __forceinline__ float localSizeGroup(const DevMesh& m, const float4* pos, uint32_t v, v3<float> p, int lane)
 {
     const uint32_t b = m.noff[v], e = m.noff[v + 1];
     float total = 0.0f;
-    for(uint32_t r = b + lane; r < e; r += width)
+    for(uint32_t r = b + lane; r < e; r += W)
     {
         const v3<float> d = p - ld<float>(pos, m.nbr[r]);
         total += __builtin_amdgcn_sqrtf(gmq::dot(d, d));
     }
-    for(int off = width >> 1; off > 0; off >>= 1)
-        total += __shfl_xor(total, off, 64);
-    return total * __builtin_amdgcn_rcpf((float)(e - b));
+    return groupSum<W>(total) * __builtin_amdgcn_rcpf((float)(e - b));
 }
 
 // Quality of the patch element `ref` with the patch's centre vertex at p.
@@ -246,20 +283,12 @@ struct WavePatch
             // lanes (tree order; deterministic, not the sequential sum).
             const bool valid = lane < nElem;
             const bool pos = valid && q > 0.0f;
-            float sum = pos ? __builtin_amdgcn_rcpf(q) : 0.0f;
-            float cnt = pos ? 1.0f : 0.0f;
-            int bad = (valid && !pos) ? 1 : 0;
-            float mn = (bad && q == q) ? q : __builtin_huge_valf();
-#pragma unroll
-            for(int off = G / 2; off > 0; off >>= 1)
-            {
-                sum += __shfl_xor(sum, off, 64);
-                cnt += __shfl_xor(cnt, off, 64);
-                bad |= __shfl_xor(bad, off, 64);
-                const float om = __shfl_xor(mn, off, 64);
-                mn = om < mn ? om : mn;
-            }
-            return bad ? mn : cnt / sum;
+            const float sum = groupSum<G>(pos ? __builtin_amdgcn_rcpf(q) : 0.0f);
+            const float cnt = groupSum<G>(pos ? 1.0f : 0.0f);
+            const bool bad = groupMax<G>((valid && !pos) ? 1.0f : 0.0f) > 0.0f;
+            if(!bad)
+                return cnt / sum;
+            return groupMin<G>((valid && !pos && q == q) ? q : __builtin_huge_valf());
         }
         T inv = T(1) / q;
         // LDS accesses of one wavefront complete in issue order, so the reads
@@ -427,17 +456,21 @@ __device__ __forceinline__ int groupArgmax(T q, int lane, int width)
     return idx;
 }
 
-// The same rule over a whole wavefront: the largest key, the lowest lane
-// among equal keys (a maximum reduction, then the first lane holding it).
-__device__ __forceinline__ int waveArgmax(float key)
+// The same rule over an aligned group of G lanes: the largest key, the
+// lowest lane among equal keys (a maximum reduction, then the first lane of
+// the group holding it); returns the index within the group.  Every lane of
+// the wavefront must be active.
+template<int G> __device__ __forceinline__ int groupArgmaxFast(float key)
 {
     key = (key == key) ? key : -__builtin_huge_valf();
-    float mx = key;
-#pragma unroll
-    for(int off = 32; off > 0; off >>= 1)
-        mx = __builtin_fmaxf(mx, __shfl_xor(mx, off, 64));
-    return (int)__builtin_ctzll(__ballot(key == mx));
+    const float mx = groupMax<G>(key);
+    const unsigned long long hit = __ballot(key == mx);
+    const int first = (int)(threadIdx.x & 63) & ~(G - 1);
+    const unsigned long long mine = G == 64 ? hit : (hit >> first) & ((1ull << G) - 1);
+    return (int)__builtin_ctzll(mine);
 }
+
+__device__ __forceinline__ int waveArgmax(float key) { return groupArgmaxFast<64>(key); }
 
 template<typename T>
 __device__ __forceinline__ v3<T> shflv(v3<T> a, int src)
@@ -921,7 +954,7 @@ __global__ void __launch_bounds__(512) spawnMfmaKernel(const DevMesh* __restrict
     float constSum = 0.0f;
     if(EK == 1)
         for(int e = 0; e < nE; ++e) constSum += ((const gma::Hex<float>*)sp)[e].constInv3;
-    float scale = moveCoeff * localSizeGroup(m, pos, v, o, lane, 64);
+    float scale = moveCoeff * localSizeGroup<64>(m, pos, v, o, lane);
     v3<float> xb = mk<float>(0.0f, 0.0f, 0.0f);
     for(int iter = 0; iter < 2; ++iter)
     {
@@ -980,7 +1013,7 @@ __global__ void __launch_bounds__(512) gdAffineKernel(const DevMesh* __restrict_
     v3<T> x = mk<T>(T(0), T(0), T(0));
     T nodeShift;
     if constexpr(sizeof(T) == 4)
-        nodeShift = localSizeGroup(m, pos, v, o, s, 8) * (1.0f / 75.0f);
+        nodeShift = localSizeGroup<8>(m, pos, v, o, s) * (1.0f / 75.0f);
     else
         nodeShift = localSize<T>(m, pos, v, o) * T(1.0 / 75.0);
     const T originalNodeShift = nodeShift;
@@ -1094,7 +1127,7 @@ __global__ void __launch_bounds__(512) gdMfmaKernel(const DevMesh* __restrict__ 
         for(int e = 0; e < nE; ++e) constSum += ((const gma::Hex<float>*)mySp)[e].constInv3;
 
     v3<float> x = mk<float>(0.0f, 0.0f, 0.0f);
-    float nodeShift = localSizeGroup(m, pos, v, o, s, 8) * (1.0f / 75.0f);
+    float nodeShift = localSizeGroup<8>(m, pos, v, o, s) * (1.0f / 75.0f);
     const float originalNodeShift = nodeShift;
     int evals = 0;
     bool active = present;
@@ -1118,7 +1151,7 @@ __global__ void __launch_bounds__(512) gdMfmaKernel(const DevMesh* __restrict__ 
         const v3<float> shift = gq * (nodeShift / gn);
         const v3<float> prop = x + shift * off;
         const float qp = mfmaPatchQuality<EK>(mySp, nRows, nE, prop, constSum);
-        const int best = groupArgmax<float>(qp, s, 8);
+        const int best = groupArgmaxFast<8>(qp);
         const v3<float> nx = shflv<float>(prop, (base & 63) + best);
         const float bestOff = __shfl(off, (base & 63) + best, 64);
         if(active)
@@ -1161,9 +1194,11 @@ struct AffineWavePatch
         T sum = T(0);
         if(lane < nElem)
             sum = EK == 0 ? gma::fastTet<T>(cn, x, xx) : gma::fastHex<T>(hx, x, xx);
-#pragma unroll
-        for(int off = G / 2; off > 0; off >>= 1)
-            sum += __shfl_xor(sum, off, 64);
+        if constexpr(sizeof(T) == 4)
+            sum = groupSum<G>(sum);
+        else
+            for(int off = G / 2; off > 0; off >>= 1)
+                sum += __shfl_xor(sum, off, 64);
         if(!gma::fastBad<T>(sum))
         {
             if constexpr(sizeof(T) == 4)
@@ -1217,7 +1252,7 @@ __global__ void __launch_bounds__(64) nmAffineWaveKernel(const DevMesh* __restri
                            EK == 0 ? (unsigned char*)&f.cn : (unsigned char*)&f.hx);
     T ls;
     if constexpr(sizeof(T) == 4)
-        ls = localSizeGroup(m, pos, v, f.o, lane, G);
+        ls = localSizeGroup<G>(m, pos, v, f.o, lane);
     else
         ls = localSize<T>(m, pos, v, f.o);
     const v3<T> np = gms::nelderMead<T>(f.o, ls, f);
