@@ -231,9 +231,11 @@ struct alignas(16) Hex
     T constMin;    // smallest constant-corner q if one is non-positive, else +1
 };
 
-template<typename T, class Fix> GMA Hex<T> stageHex(const v3<T> u[8], int k, uint32_t ref, const Fix& fix)
+// Stages a hexahedron into h, which should live in memory (LDS): the slot of
+// a moving corner depends on the vertex, and an indexed write to a register
+// array would go through scratch.
+template<typename T, class Fix> GMA void stageHex(Hex<T>& h, const v3<T> u[8], int k, uint32_t ref, const Fix& fix)
 {
-    Hex<T> h;
     int nv = 0;
     T inv3 = T(0), mn = T(1);
     bool bad = false;
@@ -275,7 +277,6 @@ template<typename T, class Fix> GMA Hex<T> stageHex(const v3<T> u[8], int k, uin
     // reads constMin only).
     h.constInv3 = bad ? T(__builtin_nan("")) : inv3;
     h.constMin = bad ? mn : T(1);
-    return h;
 }
 
 // Branch-free sums for the common case: 3/q of a corner, which is NaN

@@ -702,27 +702,31 @@ struct ConstFix
     }
 };
 
-// FP64 inversion test of a chosen FP32 move (one staged element): corners
-// FP32 calls non-positive set `nonPos32`; corners FP32 calls positive but
-// inside the near-degenerate band are recomputed in FP64, and set `flip`
-// when FP64 finds them non-positive.
+// FP64 inversion test of a chosen FP32 move (one staged element): returns
+// INV_NONPOS32 when FP32 calls a corner non-positive, INV_FLIP when a corner
+// FP32 calls positive lies inside the near-degenerate band and its FP64
+// determinant (from the FP32 positions) is not positive.
+enum { INV_NONPOS32 = 1, INV_FLIP = 2 };
+
 template<int EK>
-__device__ __forceinline__ void inversionTest(const unsigned char* sp, int e, v3<float> x, v3<float> pf,
-                                              const DetFix<EK>& fix, bool& nonPos32, bool& flip)
+__device__ __forceinline__ int cornerInversion(const gma::Corner<float>& k, v3<float> x, v3<float> pf,
+                                               const DetFix<EK>& fix)
 {
-    auto one = [&](const gma::Corner<float>& k) {
-        const float d = gma::cornerDet<float>(k, x);
-        if(!(d > 0.0f)) nonPos32 = true;
-        else if(d < k.thr && !(fix.det64(k, pf) > 0.0f)) flip = true;
-    };
+    const float d = gma::cornerDet<float>(k, x);
+    if(!(d > 0.0f)) return INV_NONPOS32;
+    return (d < k.thr && !(fix.det64(k, pf) > 0.0f)) ? INV_FLIP : 0;
+}
+
+template<int EK>
+__device__ __forceinline__ int inversionTest(const unsigned char* sp, int e, v3<float> x, v3<float> pf,
+                                             const DetFix<EK>& fix)
+{
     if(EK == 0)
-        one(((const gma::Corner<float>*)sp)[e]);
-    else
-    {
-        const gma::Hex<float>& h = ((const gma::Hex<float>*)sp)[e];
-        for(int c = 0; c < 4; ++c) one(h.k[c]);
-        if(h.constMin <= 0.0f) nonPos32 = true;
-    }
+        return cornerInversion<EK>(((const gma::Corner<float>*)sp)[e], x, pf, fix);
+    const gma::Hex<float>& h = ((const gma::Hex<float>*)sp)[e];
+    int flags = h.constMin <= 0.0f ? INV_NONPOS32 : 0;
+    for(int c = 0; c < 4; ++c) flags |= cornerInversion<EK>(h.k[c], x, pf, fix);
+    return flags;
 }
 
 // Applies the FP64 inversion test to a move of the vertex whose lanes are the
@@ -738,11 +742,12 @@ __device__ __forceinline__ v3<T> testedMove(const DevMesh& m, const unsigned cha
     else
     {
         const DetFix<EK> fix{&m, m.pos32};
-        bool nonPos32 = false, flip = false;
+        int flags = 0;
         for(int e = lane; e < nE; e += width)
-            inversionTest<EK>(sp, e, x, pf, fix, nonPos32, flip);
+            flags |= inversionTest<EK>(sp, e, x, pf, fix);
         const unsigned long long mask = width == 64 ? ~0ull : ((1ull << width) - 1);
-        const bool reject = ((__ballot(flip) >> first) & mask) != 0 && ((__ballot(nonPos32) >> first) & mask) == 0;
+        const bool reject = ((__ballot((flags & INV_FLIP) != 0) >> first) & mask) != 0 &&
+                            ((__ballot((flags & INV_NONPOS32) != 0) >> first) & mask) == 0;
         return reject ? o : pf;
     }
 }
@@ -764,7 +769,7 @@ __device__ __forceinline__ void stageAffine(const DevMesh& m, const typename Pos
         const uint4 a = m.hexs[2 * id], b = m.hexs[2 * id + 1];
         v3<T> u[8] = {ld<T>(pos, a.x) - o, ld<T>(pos, a.y) - o, ld<T>(pos, a.z) - o, ld<T>(pos, a.w) - o,
                       ld<T>(pos, b.x) - o, ld<T>(pos, b.y) - o, ld<T>(pos, b.z) - o, ld<T>(pos, b.w) - o};
-        *(gma::Hex<T>*)dst = gma::stageHex<T>(u, k, ref, ConstFix<T, 1>{&m, m.pos32, o});
+        gma::stageHex<T>(*(gma::Hex<T>*)dst, u, k, ref, ConstFix<T, 1>{&m, m.pos32, o});
     }
 }
 
@@ -1183,8 +1188,8 @@ struct AffineWavePatch
     int nElem;
     int count;
     v3<T> o;
-    gma::Corner<T> cn;   // EK == 0
-    gma::Hex<T> hx;      // EK == 1
+    gma::Corner<T> cn;   // EK == 0: in registers
+    gma::Hex<T>* hx;     // EK == 1: this lane's slot in LDS
 
     __device__ T operator()(v3<T> p)
     {
@@ -1193,7 +1198,7 @@ struct AffineWavePatch
         const T xx = gmq::dot(x, x);
         T sum = T(0);
         if(lane < nElem)
-            sum = EK == 0 ? gma::fastTet<T>(cn, x, xx) : gma::fastHex<T>(hx, x, xx);
+            sum = EK == 0 ? gma::fastTet<T>(cn, x, xx) : gma::fastHex<T>(*hx, x, xx);
         if constexpr(sizeof(T) == 4)
             sum = groupSum<G>(sum);
         else
@@ -1211,7 +1216,7 @@ struct AffineWavePatch
         if(lane < nElem)
         {
             if(EK == 0) gma::addTet<T>(a, cn, x, xx);
-            else gma::addHex<T>(a, hx, x, xx);
+            else gma::addHex<T>(a, *hx, x, xx);
         }
         T mn = (a.bad && a.mn == a.mn) ? a.mn : (T)__builtin_huge_val();
 #pragma unroll
@@ -1230,6 +1235,9 @@ __global__ void __launch_bounds__(64) nmAffineWaveKernel(const DevMesh* __restri
                                                          uint32_t count)
 {
     typedef typename PosOf<T>::type P;
+    // Hexahedra are staged in LDS, one slot per lane (their moving corners
+    // land in vertex-dependent slots, which registers cannot index).
+    __shared__ __align__(16) unsigned char hexStage[EK == 1 ? 64 * sizeof(gma::Hex<T>) : 16];
     const int lane = threadIdx.x % G;
     const int grp = threadIdx.x / G;
     const uint32_t i = blockIdx.x * (64 / G) + grp;
@@ -1243,13 +1251,14 @@ __global__ void __launch_bounds__(64) nmAffineWaveKernel(const DevMesh* __restri
     P* pos = PosOf<T>::get(m);
     const uint32_t eb = m.eoff[v], ee = m.eoff[v + 1];
     AffineWavePatch<T, EK, G> f;
+    f.hx = (gma::Hex<T>*)hexStage + (EK == 1 ? threadIdx.x : 0);
     f.lane = lane;
     f.nElem = (int)(ee - eb);
     f.count = 0;
     f.o = ld<T>(pos, v);
     if(lane < f.nElem)
         stageAffine<T, EK>(m, pos, m.eref[eb + lane], f.o,
-                           EK == 0 ? (unsigned char*)&f.cn : (unsigned char*)&f.hx);
+                           EK == 0 ? (unsigned char*)&f.cn : (unsigned char*)f.hx);
     T ls;
     if constexpr(sizeof(T) == 4)
         ls = localSizeGroup<G>(m, pos, v, f.o, lane);
@@ -1260,16 +1269,17 @@ __global__ void __launch_bounds__(64) nmAffineWaveKernel(const DevMesh* __restri
     v3<T> out = np;
     if constexpr(sizeof(T) == 4)
     {
-        bool nonPos32 = false, flip = false;
+        int flags = 0;
         if(lane < f.nElem)
         {
             const DetFix<EK> fix{&m, m.pos32};
-            inversionTest<EK>(EK == 0 ? (const unsigned char*)&f.cn : (const unsigned char*)&f.hx, 0, np - f.o, np,
-                              fix, nonPos32, flip);
+            flags = inversionTest<EK>(EK == 0 ? (const unsigned char*)&f.cn : (const unsigned char*)f.hx, 0,
+                                      np - f.o, np, fix);
         }
         const int sh = (threadIdx.x & 63) & ~(G - 1);
         const unsigned long long gmask = G == 64 ? ~0ull : ((1ull << G) - 1);
-        const bool reject = ((__ballot(flip) >> sh) & gmask) != 0 && ((__ballot(nonPos32) >> sh) & gmask) == 0;
+        const bool reject = ((__ballot((flags & INV_FLIP) != 0) >> sh) & gmask) != 0 &&
+                            ((__ballot((flags & INV_NONPOS32) != 0) >> sh) & gmask) == 0;
         out = reject ? f.o : np;
     }
     if(lane == 0)
