@@ -28,6 +28,7 @@ EXPORTS = [
     "gm_batch_last_timing", "gm_batch_kernel_times", "gm_batch_autotune", "gm_server_create", "gm_server_destroy", "gm_server_set_shape", "gm_server_submit", "gm_server_wait",
     "gm_server_result", "gm_server_release", "gm_server_stats", "gm_cube_mesh", "gm_set_topology_cache", "gm_set_device_pool",
     "gm_topology_cache_stats", "gm_math_probe", "gm_mesh_smooth_schedule", "gm_batch_smooth_schedule",
+    "gm_job_size", "gm_job_encode", "gm_job_info", "gm_job_decode", "gm_server_submit_job",
 ]
 
 
@@ -76,6 +77,12 @@ def lib():
     L.gm_batch_destroy.argtypes = [vp]
     L.gm_batch_set_shape.argtypes = [vp, C.c_int, C.c_int, C.c_int]
     L.gm_batch_smooth.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, dbl, C.c_int, vp, vp]
+    L.gm_job_size.argtypes = [i64, i64, i64]
+    L.gm_job_size.restype = i64
+    L.gm_job_encode.argtypes = [i64, vp, vp, i64, vp, i64, vp, C.c_int, C.c_int, C.c_int, vp, i64]
+    L.gm_job_info.argtypes = [vp, i64, vp, vp]
+    L.gm_job_decode.argtypes = [vp, i64, vp, vp, vp, vp]
+    L.gm_server_submit_job.argtypes = [vp, vp, i64, C.POINTER(i64)]
     L.gm_mesh_smooth_schedule.argtypes = [vp, C.c_int, C.c_int, C.POINTER(Schedule), C.c_int, C.c_int, dbl, vp, vp,
                                           vp, vp]
     L.gm_batch_smooth_schedule.argtypes = [vp, C.c_int, C.c_int, C.POINTER(Schedule), C.c_int, C.c_int, dbl,

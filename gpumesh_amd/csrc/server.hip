@@ -485,6 +485,30 @@ void schedulerLoop(gm_server* s)
     s->groups.clear();
 }
 
+int checkRequest(int smoother, int passes, int precision)
+{
+    if(smoother < 0 || smoother > 3) return fail(GM_ERR_ARG, "unknown smoother");
+    if(passes < 0) return fail(GM_ERR_ARG, "negative pass count");
+    if(precision < 0 || precision > 2) return fail(GM_ERR_ARG, "unknown precision");
+    return GM_OK;
+}
+
+// Queues a job whose inputs are set (s->mu not held).
+int enqueue(gm_server* s, std::unique_ptr<Job> j, int64_t* jobId)
+{
+    std::lock_guard<std::mutex> lock(s->mu);
+    if(s->stop) return fail(GM_ERR_ARG, "server is shutting down");
+    j->id = s->nextId++;
+    *jobId = j->id;
+    Job* raw = j.get();
+    s->jobs.emplace(raw->id, std::move(j));
+    s->buildQueue.push_back(raw);
+    s->pending.push_back(raw);
+    ++s->submitted;
+    s->cvWork.notify_one();
+    return GM_OK;
+}
+
 } // namespace
 
 extern "C" {
@@ -556,6 +580,38 @@ int gm_server_set_shape(gm_server* s, int smoother, int patchesPerGroup, int til
     return GM_OK;
 }
 
+
+int gm_server_submit_job(gm_server* s, const void* blob, int64_t bytes, int64_t* jobId)
+{
+    const Clock::time_point now = Clock::now();
+    if(!s || !jobId) return fail(GM_ERR_ARG, "null argument");
+    int64_t counts[3];
+    int32_t request[3];
+    int rc = gm_job_info(blob, bytes, counts, request);
+    if(rc) return rc;
+    if((rc = checkRequest(request[0], request[1], request[2]))) return rc;
+    auto j = std::make_unique<Job>();
+    j->nverts = counts[0];
+    j->ntets = counts[1];
+    j->nhexs = counts[2];
+    j->posCopy.resize(3 * (size_t)j->nverts);
+    j->fixedCopy.resize((size_t)j->nverts);
+    j->tetsCopy.resize(4 * (size_t)j->ntets);
+    j->hexsCopy.resize(8 * (size_t)j->nhexs);
+    if((rc = gm_job_decode(blob, bytes, j->posCopy.data(), j->fixedCopy.data(), j->tetsCopy.data(),
+                           j->hexsCopy.data())))
+        return rc;
+    j->pos = j->posCopy.data();
+    j->fixed = j->fixedCopy.data();
+    j->tets = j->tetsCopy.data();
+    j->hexs = j->hexsCopy.data();
+    j->smoother = request[0];
+    j->passes = request[1];
+    j->precision = request[2];
+    j->tSubmit = now;
+    return enqueue(s, std::move(j), jobId);
+}
+
 int gm_server_submit(gm_server* s, int64_t nverts, const double* pos, const int32_t* fixed, int64_t ntets,
                      const uint32_t* tets, int64_t nhexs, const uint32_t* hexs, int smoother, int passes,
                      int precision, int borrow, int64_t* jobId)
@@ -564,9 +620,8 @@ int gm_server_submit(gm_server* s, int64_t nverts, const double* pos, const int3
     if(!s || !jobId) return fail(GM_ERR_ARG, "null argument");
     if(nverts < 0 || ntets < 0 || nhexs < 0) return fail(GM_ERR_ARG, "negative size");
     if((nverts && (!pos || !fixed)) || (ntets && !tets) || (nhexs && !hexs)) return fail(GM_ERR_ARG, "null array");
-    if(smoother < 0 || smoother > 3) return fail(GM_ERR_ARG, "unknown smoother");
-    if(passes < 0) return fail(GM_ERR_ARG, "negative pass count");
-    if(precision < 0 || precision > 2) return fail(GM_ERR_ARG, "unknown precision");
+    int rc = checkRequest(smoother, passes, precision);
+    if(rc) return rc;
     auto j = std::make_unique<Job>();
     j->nverts = nverts;
     j->ntets = ntets;
@@ -596,17 +651,7 @@ int gm_server_submit(gm_server* s, int64_t nverts, const double* pos, const int3
     j->passes = passes;
     j->precision = precision;
     j->tSubmit = now;
-    std::lock_guard<std::mutex> lock(s->mu);
-    if(s->stop) return fail(GM_ERR_ARG, "server is shutting down");
-    j->id = s->nextId++;
-    *jobId = j->id;
-    Job* raw = j.get();
-    s->jobs.emplace(raw->id, std::move(j));
-    s->buildQueue.push_back(raw);
-    s->pending.push_back(raw);
-    ++s->submitted;
-    s->cvWork.notify_one();
-    return GM_OK;
+    return enqueue(s, std::move(j), jobId);
 }
 
 int gm_server_wait(gm_server* s, int64_t jobId, double timeoutMs, int* state)

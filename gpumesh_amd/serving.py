@@ -62,6 +62,18 @@ class GpuServer:
         self._lent[job.value] = (pos, fixed, tets, hexs)
         return int(job.value)
 
+    def submit_job(self, blob: bytes) -> int:
+        """Queues a job file (gpumesh_amd.encode_job / gm_job_encode)."""
+        raw = np.frombuffer(blob, np.uint8)
+        counts = np.zeros(3, np.int64)
+        req = np.zeros(3, np.int32)
+        check(lib().gm_job_info(ptr(raw), raw.size, ptr(counts), ptr(req)), "gm_job_info")
+        job = C.c_int64(0)
+        check(lib().gm_server_submit_job(self._h, ptr(raw), raw.size, C.byref(job)), "gm_server_submit_job")
+        self._passes[job.value] = int(req[1])
+        self._nverts[job.value] = int(counts[0])
+        return int(job.value)
+
     def submit_mesh(self, m, smoother: str, passes: int, precision: str = "fp32_polish") -> int:
         return self.submit(m.pos, m.fixed, m.tets, m.hexs, smoother, passes, precision)
 

@@ -292,6 +292,8 @@ int gm_server_set_shape(gm_server* server, int smoother, int patches_per_group, 
 int gm_server_submit(gm_server* server, int64_t nverts, const double* pos, const int32_t* fixed, int64_t ntets,
                      const uint32_t* tets, int64_t nhexs, const uint32_t* hexs, int smoother, int passes,
                      int precision, int borrow, int64_t* job);
+/* Queues a job encoded with gm_job_encode (copied before returning). */
+int gm_server_submit_job(gm_server* server, const void* job_bytes, int64_t bytes, int64_t* job);
 /* Waits until the job finished or timeout_ms passed (< 0: no limit);
  * *state = 0 queued, 1 built, 2 active, 3 done, 4 failed. */
 int gm_server_wait(gm_server* server, int64_t job, double timeout_ms, int* state);
@@ -305,6 +307,27 @@ int gm_server_release(gm_server* server, int64_t job);
 /* out8 = {submitted, done, failed, rounds, peak active jobs, peak reserved
  * bytes, budget bytes, wall ms of the rounds}. */
 int gm_server_stats(gm_server* server, double* out8);
+
+/* ---- Job files: a lossless binary form of a smoothing request (mesh in
+ * the caller's numbering, smoother, passes, precision), for queues and the
+ * serving loop.  The reference's JSON meshes (Serialization/JsonSerializer.cpp:25-120,
+ * JsonDeserializer.cpp:38-115) print positions with 6 significant digits
+ * and keep a fixed vertex only as a constraint id; a job keeps every bit.
+ * Layout: "GMJ1", a 64-byte header (counts, request), positions f64,
+ * fixed flags u8, tetrahedra u32, hexahedra u32 (8-byte aligned sections),
+ * then an FNV-1a 64 checksum of everything before it; little-endian.
+ * Host only. */
+/* Bytes of a job with these counts (-1 for bad counts). */
+int64_t gm_job_size(int64_t nverts, int64_t ntets, int64_t nhexs);
+int gm_job_encode(int64_t nverts, const double* pos, const int32_t* fixed, int64_t ntets, const uint32_t* tets,
+                  int64_t nhexs, const uint32_t* hexs, int smoother, int passes, int precision, void* out,
+                  int64_t out_bytes);
+/* Checks magic, sizes and checksum; counts3 = {nverts, ntets, nhexs},
+ * request3 = {smoother, passes, precision} (either may be NULL). */
+int gm_job_info(const void* job_bytes, int64_t bytes, int64_t* counts3, int32_t* request3);
+/* pos[3 nverts], fixed[nverts] (0/1), tets[4 ntets], hexs[8 nhexs]. */
+int gm_job_decode(const void* job_bytes, int64_t bytes, double* pos, int32_t* fixed, uint32_t* tets,
+                  uint32_t* hexs);
 
 /* Synthetic input: an n^3-cell unit-cube mesh, kind 0 = Kuhn tetrahedra
  * (6 per cell, det(v3-v0, v3-v1, v3-v2) > 0), 1 = hexahedra (v0..v3 bottom

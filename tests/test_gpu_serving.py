@@ -99,3 +99,20 @@ def test_slot_limit_and_release():
         assert srv.stats()["peak_active"] == 1
         with pytest.raises(GpuMeshError):
             srv.wait(ids[0])                     # released with its result
+
+
+def test_job_files_match_alone():
+    """Jobs submitted as job files (gm_server_submit_job) give the same bits
+    as the same meshes submitted as arrays, and as smoothing them alone."""
+    from gpumesh_amd import encode_job
+    specs = JOBS[:6]
+    refs = [_alone(s) for s in specs]
+    with GpuServer(worker_threads=2) as srv:
+        ids = [srv.submit_job(encode_job(cube_mesh(*s[:4]), s[4], s[5], s[6])) for s in specs]
+        for k, j in enumerate(ids):
+            assert srv.wait(j, 60000) == "done", k
+            _check(srv.result(j), refs[k], k)
+        bad = bytearray(encode_job(cube_mesh(3, "tet", 0.3, 1), "nm", 2))
+        bad[-1] ^= 1
+        with pytest.raises(GpuMeshError):
+            srv.submit_job(bytes(bad))
