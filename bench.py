@@ -63,7 +63,7 @@ FLOP_AFFINE_TET = 20.0
 # HBM bytes per launch of the dominant kernel, measured with rocprofv3 --pmc
 # FETCH_SIZE and WRITE_SIZE (each scaled by the same counter's reading for a
 # 1 GiB device copy: x2.0 and x1.0) on the 100^3 Kuhn cube.
-MEASURED_TRAFFIC = {("spawnAffineKernel<0>", 100): 345680936.0}
+MEASURED_TRAFFIC = {("spawnMfmaKernel<0>", 100): 346038749.0}
 
 
 def log(msg):
