@@ -1465,18 +1465,21 @@ __global__ void __launch_bounds__(512) getmeVertexKernel(const DevMesh* __restri
 // (by the element step, or by the accepted move that last changed one of
 // them), so it has the bits a fresh evaluation would give.  The quality after
 // the move is evaluated lane-parallel as in smoothWaveKernel and, when the
-// move is kept, written back.
+// move is kept, written back.  GETME_WAVES wavefronts per workgroup: more
+// resident wavefronts to cover the dependent gathers.
+constexpr int GETME_WAVES = 4;
+
 template<typename T, int EV, int SA, int G>
-__global__ void __launch_bounds__(64) getmeWaveKernel(const DevMesh* __restrict__ meshes,
-                                                      const uint4* __restrict__ segs, uint32_t nsegs,
-                                                      uint32_t count, gmq::Crew c)
+__global__ void __launch_bounds__(64 * GETME_WAVES) getmeWaveKernel(const DevMesh* __restrict__ meshes,
+                                                                    const uint4* __restrict__ segs, uint32_t nsegs,
+                                                                    uint32_t count, gmq::Crew c)
 {
     typedef typename PosOf<T>::type P;
-    __shared__ T red[2 * 64];
-    __shared__ T pull[4 * 64];
+    __shared__ T red[2 * 64 * GETME_WAVES];
+    __shared__ T pull[4 * 64 * GETME_WAVES];
     const int lane = threadIdx.x % G;
     const int grp = threadIdx.x / G;
-    const uint32_t i = blockIdx.x * (64 / G) + grp;
+    const uint32_t i = blockIdx.x * (64 * GETME_WAVES / G) + grp;
     if(i >= count)
         return;
     const uint4 sp = waveSpan(segs, nsegs, i);
@@ -1870,14 +1873,19 @@ hipError_t Launchers<T, EV, SA>::getmeVertices(const VertexWork& w, const gmq::C
         return hipSuccess;
     if(shape == KS_WAVE && w.maxPatch <= 64)
     {
+        const int threads = 64 * GETME_WAVES;
         if(w.maxPatch <= 8)
-            getmeWaveKernel<T, EV, SA, 8><<<(w.count + 7) / 8, 64, 0, stream>>>(w.meshes, w.segs, w.nsegs, w.count, c);
+            getmeWaveKernel<T, EV, SA, 8><<<(w.count + threads / 8 - 1) / (threads / 8), threads, 0, stream>>>(
+                w.meshes, w.segs, w.nsegs, w.count, c);
         else if(w.maxPatch <= 16)
-            getmeWaveKernel<T, EV, SA, 16><<<(w.count + 3) / 4, 64, 0, stream>>>(w.meshes, w.segs, w.nsegs, w.count, c);
+            getmeWaveKernel<T, EV, SA, 16><<<(w.count + threads / 16 - 1) / (threads / 16), threads, 0, stream>>>(
+                w.meshes, w.segs, w.nsegs, w.count, c);
         else if(w.maxPatch <= 32)
-            getmeWaveKernel<T, EV, SA, 32><<<(w.count + 1) / 2, 64, 0, stream>>>(w.meshes, w.segs, w.nsegs, w.count, c);
+            getmeWaveKernel<T, EV, SA, 32><<<(w.count + threads / 32 - 1) / (threads / 32), threads, 0, stream>>>(
+                w.meshes, w.segs, w.nsegs, w.count, c);
         else
-            getmeWaveKernel<T, EV, SA, 64><<<w.count, 64, 0, stream>>>(w.meshes, w.segs, w.nsegs, w.count, c);
+            getmeWaveKernel<T, EV, SA, 64><<<(w.count + threads / 64 - 1) / (threads / 64), threads, 0, stream>>>(
+                w.meshes, w.segs, w.nsegs, w.count, c);
         return hipGetLastError();
     }
     const int ppg = lane.patchesPerGroup >= 1 && lane.patchesPerGroup <= 8 ? lane.patchesPerGroup : 4;

@@ -607,10 +607,12 @@ static hipError_t sweep(Plan* p, const gmq::Crew& c, int ev, int sa, const RunPa
             gmd::LaneShape lane = r.lane;
             if(lane.tileElems == 0)
             {
-                // Smallest tile that holds the list's patches: spawn search
-                // stages one patch per wavefront, gradient descent eight.
-                const int need = r.smoother == gmd::SM_GRADIENT_DESCENT ? 8 * l.maxPatch : l.maxPatch;
-                lane.tileElems = need <= 64 ? 64 : (need <= 128 ? 128 : 256);
+                // The tile that just holds the list's largest patch (rounded
+                // to 4 elements): spawn search stages one patch per
+                // wavefront, gradient descent eight.  Less LDS per wavefront
+                // keeps more of them resident.
+                const int patch = (l.maxPatch + 3) & ~3;
+                lane.tileElems = r.smoother == gmd::SM_GRADIENT_DESCENT ? 8 * patch : patch;
             }
             int tag = getme ? (f64 ? GM_KT_GETME_VERT_F64 : GM_KT_GETME_VERT_F32)
                             : (f64 ? GM_KT_SMOOTH_F64 : GM_KT_SMOOTH_F32);
