@@ -128,8 +128,33 @@ template<int G> __device__ __forceinline__ float groupMin(float v)
     return groupReduce<G>(v, [](float a, float b) { return __builtin_fminf(a, b); });
 }
 
-// The same in FP32 with the neighbours spread over a group of W lanes
-// (hardware square root, tree sum); every lane of the group gets the result.
+// FP64 group sum over shuffles (same tree, every lane gets the same bits).
+template<int G> __device__ __forceinline__ double groupSum(double v)
+{
+#pragma unroll
+    for(int off = G / 2; off > 0; off >>= 1)
+        v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// The local size with the neighbours spread over a group of W lanes (tree
+// sum; FP32 with the hardware square root); every lane of the group gets the
+// result.  Used by the FP32 kernels and the FP64 polish pass (not by the
+// bit-exact FP64 mode, which sums in the reference's order).
+template<int W>
+__device__ __forceinline__ double localSizeGroup(const DevMesh& m, const double4* pos, uint32_t v, v3<double> p,
+                                                 int lane)
+{
+    const uint32_t b = m.noff[v], e = m.noff[v + 1];
+    double total = 0.0;
+    for(uint32_t r = b + lane; r < e; r += W)
+    {
+        const v3<double> d = p - ld<double>(pos, m.nbr[r]);
+        total += gmq::tsqrt(gmq::dot(d, d));
+    }
+    return groupSum<W>(total) / (double)(e - b);
+}
+
 template<int W>
 __device__ __forceinline__ float localSizeGroup(const DevMesh& m, const float4* pos, uint32_t v, v3<float> p, int lane)
 {
@@ -472,6 +497,16 @@ template<int G> __device__ __forceinline__ int groupArgmaxFast(float key)
 
 __device__ __forceinline__ int waveArgmax(float key) { return groupArgmaxFast<64>(key); }
 
+__device__ __forceinline__ int waveArgmax(double key)
+{
+    key = (key == key) ? key : -__builtin_huge_val();
+    double mx = key;
+#pragma unroll
+    for(int off = 32; off > 0; off >>= 1)
+        mx = __builtin_fmax(mx, __shfl_xor(mx, off, 64));
+    return (int)__builtin_ctzll(__ballot(key == mx));
+}
+
 template<typename T>
 __device__ __forceinline__ v3<T> shflv(v3<T> a, int src)
 {
@@ -773,10 +808,10 @@ __device__ __forceinline__ void stageAffine(const DevMesh& m, const typename Pos
     }
 }
 
-// Patch quality assuming every element is positive (branch-free); `bad`
-// reports that the assumption failed.
+// Sum of 3/q over the patch assuming every element is positive
+// (branch-free; not finite when the assumption fails).
 template<typename T, int EK>
-__device__ __forceinline__ T affinePatchFast(const unsigned char* sp, int nE, v3<T> x, T xx, bool& bad)
+__device__ __forceinline__ T affineSumFast(const unsigned char* sp, int nE, v3<T> x, T xx)
 {
     T inv3 = T(0);
     for(int e = 0; e < nE; ++e)
@@ -784,6 +819,15 @@ __device__ __forceinline__ T affinePatchFast(const unsigned char* sp, int nE, v3
         if(EK == 0) inv3 += gma::fastTet<T>(((const gma::Corner<T>*)sp)[e], x, xx);
         else inv3 += gma::fastHex<T>(((const gma::Hex<T>*)sp)[e], x, xx);
     }
+    return inv3;
+}
+
+// Patch quality assuming every element is positive (branch-free); `bad`
+// reports that the assumption failed.
+template<typename T, int EK>
+__device__ __forceinline__ T affinePatchFast(const unsigned char* sp, int nE, v3<T> x, T xx, bool& bad)
+{
+    const T inv3 = affineSumFast<T, EK>(sp, nE, x, xx);
     bad = gma::fastBad<T>(inv3);
     if constexpr(sizeof(T) == 4)
         return 3.0f * (float)nE * __builtin_amdgcn_rcpf(inv3);
@@ -839,22 +883,24 @@ __global__ void __launch_bounds__(512) spawnAffineKernel(const DevMesh* __restri
     __syncthreads();
     if(!present)
         return;
-    T scale = moveCoeff * localSize<T>(m, pos, v, o);
+    T scale = moveCoeff * localSizeGroup<64>(m, pos, v, o, lane);
     v3<T> xb = mk<T>(T(0), T(0), T(0));
     for(int iter = 0; iter < 2; ++iter)
     {
         const v3<T> x = xb + gms::spawnOffset<T>(lane) * scale;
-        // A candidate with a non-positive element has quality <= 0 and loses
-        // to any candidate without one; its exact value matters only when
-        // every candidate of the vertex has one.
+        // The best candidate has the smallest sum of 3/q.  A candidate with a
+        // non-positive element has quality <= 0 and loses to any candidate
+        // without one; its exact value matters only when every candidate of
+        // the vertex has one.
         const T xx = gmq::dot(x, x);
-        bool bad = false;
-        T q = affinePatchFast<T, EK>(sp, nE, x, xx, bad);
+        const T inv3 = affineSumFast<T, EK>(sp, nE, x, xx);
+        const bool bad = gma::fastBad<T>(inv3);
+        T key;
         if(__ballot(!bad) == 0)
-            q = affinePatchExact<T, EK>(sp, nE, x, xx);
-        else if(bad)
-            q = -(T)__builtin_huge_val();
-        const int best = groupArgmax<T>(q, lane, 64);
+            key = affinePatchExact<T, EK>(sp, nE, x, xx);
+        else
+            key = bad ? -(T)__builtin_huge_val() : -inv3;
+        const int best = waveArgmax(key);
         xb = shflv<T>(x, best);
         scale /= T(3);
     }
@@ -1016,11 +1062,7 @@ __global__ void __launch_bounds__(512) gdAffineKernel(const DevMesh* __restrict_
         return;
 
     v3<T> x = mk<T>(T(0), T(0), T(0));
-    T nodeShift;
-    if constexpr(sizeof(T) == 4)
-        nodeShift = localSizeGroup<8>(m, pos, v, o, s) * (1.0f / 75.0f);
-    else
-        nodeShift = localSize<T>(m, pos, v, o) * T(1.0 / 75.0);
+    T nodeShift = localSizeGroup<8>(m, pos, v, o, s) * T(1.0 / 75.0);
     const T originalNodeShift = nodeShift;
     int evals = 0;
     for(int cyc = 0; cyc < 5; ++cyc)
@@ -1259,11 +1301,7 @@ __global__ void __launch_bounds__(64) nmAffineWaveKernel(const DevMesh* __restri
     if(lane < f.nElem)
         stageAffine<T, EK>(m, pos, m.eref[eb + lane], f.o,
                            EK == 0 ? (unsigned char*)&f.cn : (unsigned char*)f.hx);
-    T ls;
-    if constexpr(sizeof(T) == 4)
-        ls = localSizeGroup<G>(m, pos, v, f.o, lane);
-    else
-        ls = localSize<T>(m, pos, v, f.o);
+    const T ls = localSizeGroup<G>(m, pos, v, f.o, lane);
     const v3<T> np = gms::nelderMead<T>(f.o, ls, f);
     // FP64 inversion test of an FP32 move, one element per lane of the group.
     v3<T> out = np;
