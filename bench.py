@@ -358,7 +358,7 @@ def main():
             flop_per_launch = best["f_smooth"] * (f32_passes / steps) / best["launches"]
             achieved = flop_per_launch / avg_s / 1e12
             kname = {"spawn": "spawnMfmaKernel<0>", "gd": "gdMfmaKernel<0>",
-                     "nm": "nmAffineWaveKernel<float, 0, 32>"}.get(best["smoother"], "smoothWaveKernel<float>")
+                     "nm": "nmQuadKernel<6>"}.get(best["smoother"], "smoothWaveKernel<float>")
             executed = flop_per_launch * FLOP_AFFINE_TET / FLOP_MR_TET_PATCH / avg_s / 1e12
             # The spawn-search and gradient-descent kernels put the corner
             # terms' linear parts on the matrix cores; the FP32 peak is the

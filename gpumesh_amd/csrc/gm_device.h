@@ -103,9 +103,9 @@ struct LaneShape
 // span in the launch, items in the span).  Block spans use y = 0 and
 // z = first block.  Spans are sorted by z; index i belongs to the last span
 // with z <= i.  Vertex spans start at multiples of ITEM_ALIGN, so the items
-// of one wavefront of a wave or lane kernel (at most 8) lie in one span; the
+// of one wavefront of a wave or lane kernel (at most 16) lie in one span; the
 // indices between a span's last item and the next span are holes.
-constexpr uint32_t ITEM_ALIGN = 8;
+constexpr uint32_t ITEM_ALIGN = 16;
 constexpr uint32_t NO_ITEM = 0xffffffffu;
 __host__ __device__ inline uint32_t spanOf(const uint4* spans, uint32_t nspans, uint32_t i)
 {

@@ -1327,9 +1327,175 @@ __global__ void __launch_bounds__(64) nmAffineWaveKernel(const DevMesh* __restri
     }
 }
 
+// Nelder-Mead, FP32 affine terms of tetrahedral patches, four lanes per
+// vertex (16 vertices per wavefront): lane k of a vertex holds rows
+// k, k + 4, ... of its patch in registers (neutral rows, d = 1 and f2 = 0,
+// past the patch), an evaluation sums its rows and adds the four partial
+// sums (DPP inside the quad), and the simplex logic of gms::nelderMead runs in
+// all four lanes.  The vertices of a wavefront follow their own paths: the
+// loop below takes one simplex evaluation and one cycle per trip for every
+// vertex still running, and the second evaluation of a cycle only for the
+// vertices whose reflection calls for it, so every vertex makes the
+// reference's evaluations in the reference's order.
+template<int RPL>
+__global__ void __launch_bounds__(256) nmQuadKernel(const DevMesh* __restrict__ meshes,
+                                                    const uint4* __restrict__ segs, uint32_t nsegs, uint32_t count)
+{
+    const int sub = threadIdx.x & 3;
+    const uint32_t i = blockIdx.x * (blockDim.x >> 2) + (threadIdx.x >> 2);
+    if(i >= count)
+        return;
+    const uint4 sp = waveSpan(segs, nsegs, i);
+    const DevMesh& m = meshes[sp.x];
+    const uint32_t v = spanVertex(m, sp, i);
+    if(v == NO_ITEM || runStopped(m))
+        return;
+    float4* pos = m.pos32;
+    const uint32_t eb = m.eoff[v];
+    const int nE = (int)(m.eoff[v + 1] - eb);
+    const v3<float> o = ld<float>(pos, v);
+    gma::Corner<float> rows[RPL];
+#pragma unroll
+    for(int k = 0; k < RPL; ++k)
+    {
+        const int e = sub + 4 * k;
+        if(e < nE)
+        {
+            const uint32_t ref = m.eref[eb + e];
+            const uint4 t = m.tets[ref >> 4];
+            v3<float> u[4] = {ld<float>(pos, t.x) - o, ld<float>(pos, t.y) - o, ld<float>(pos, t.z) - o,
+                              ld<float>(pos, t.w) - o};
+            rows[k] = gma::tetCorner<float>(u, (int)(ref & 7), ref);
+        }
+        else
+        {
+            rows[k] = gma::Corner<float>{};
+            rows[k].d0 = 1.0f;
+        }
+    }
+    int evals = 0;
+    auto f = [&](v3<float> p) -> float {
+        ++evals;
+        const v3<float> x = p - o;
+        const float xx = gmq::dot(x, x);
+        float sum = 0.0f;
+#pragma unroll
+        for(int k = 0; k < RPL; ++k) sum += gma::fastTet<float>(rows[k], x, xx);
+        sum = groupSum<4>(sum);
+        if(!gma::fastBad<float>(sum))
+            return 3.0f * (float)nE * __builtin_amdgcn_rcpf(sum);
+        // A non-positive element: the smallest non-positive quality.
+        gma::Acc<float> a;
+#pragma unroll
+        for(int k = 0; k < RPL; ++k)
+            if(sub + 4 * k < nE) gma::addTet<float>(a, rows[k], x, xx);
+        return groupMin<4>((a.bad && a.mn == a.mn) ? a.mn : __builtin_huge_valf());
+    };
+
+    // gms::nelderMead (NelderMeadSmoother.cpp:100-261), one step per trip.
+    const float nodeShift = localSizeGroup<4>(m, pos, v, o, sub) * (1.0f / 24.0f);
+    gms::v4<float> vo;
+    vo.p = o;
+    vo.w = f(o);
+    gms::v4<float> s[4];
+    s[0].p = o + mk<float>(nodeShift, 0.0f, 0.0f); s[0].w = 0.0f;
+    s[1].p = o + mk<float>(0.0f, nodeShift, 0.0f); s[1].w = 0.0f;
+    s[2].p = o + mk<float>(0.0f, 0.0f, nodeShift); s[2].w = 0.0f;
+    s[3] = vo;
+    int cycle = 0;
+    bool reset = false;
+    int phase = 0;   // 0: evaluate the simplex, 1: cycle, 2: done
+    while(__ballot(phase != 2) != 0)
+    {
+        if(phase == 0)
+        {
+            for(int p = 0; p < 3; ++p) s[p].w = f(s[p].p);
+            if(s[0].w > s[1].w) gms::swap4(s[0], s[1]);
+            if(s[1].w > s[2].w) gms::swap4(s[1], s[2]);
+            if(s[2].w > s[3].w) gms::swap4(s[2], s[3]);
+            if(s[0].w > s[1].w) gms::swap4(s[0], s[1]);
+            if(s[1].w > s[2].w) gms::swap4(s[1], s[2]);
+            if(s[0].w > s[1].w) gms::swap4(s[0], s[1]);
+            phase = 1;
+        }
+        if(phase == 1)
+        {
+            const v3<float> c = (float)(1 / 3.0) * (s[1].p + s[2].p + s[3].p);
+            v3<float> x = c + 1.0f * (c - s[0].p);
+            const float fr = f(x);
+            float fv = fr;
+            const v3<float> xr = x;
+            const bool expand = s[3].w < fr;
+            const bool contract = !expand && s[1].w >= fr;
+            if(expand)
+                x = c + 2.0f * (x - c);
+            else if(contract)
+                x = fr > s[0].w ? c + 0.5f * (xr - c) : c + 0.5f * (s[0].p - c);
+            if(expand || contract)
+            {
+                const float f2 = f(x);
+                fv = f2;
+                if(expand && f2 <= fr)
+                {
+                    x = xr;
+                    fv = fr;
+                }
+            }
+            gms::v4<float> vtx;
+            vtx.p = x;
+            vtx.w = fv;
+            if(vtx.w > s[3].w) gms::swap4(s[3], vtx);
+            if(vtx.w > s[2].w) gms::swap4(s[2], vtx);
+            if(vtx.w > s[1].w) gms::swap4(s[1], vtx);
+            if(vtx.w > s[0].w) gms::swap4(s[0], vtx);
+            if((s[3].w - s[1].w) < 0.000100f)
+                phase = 2;
+            else if(++cycle >= 12)
+            {
+                if(reset)
+                    phase = 2;
+                else
+                {
+                    s[0].p = vo.p - mk<float>(nodeShift, 0.0f, 0.0f); s[0].w = vo.w - 0.0f;
+                    s[1].p = vo.p - mk<float>(0.0f, nodeShift, 0.0f); s[1].w = vo.w - 0.0f;
+                    s[2].p = vo.p - mk<float>(0.0f, 0.0f, nodeShift); s[2].w = vo.w - 0.0f;
+                    s[3] = vo;
+                    reset = true;
+                    cycle = 0;
+                    phase = 0;
+                }
+            }
+        }
+    }
+    const v3<float> np = s[3].p;
+    // FP64 inversion test of the move, the quad's rows.
+    const DetFix<0> fix{&m, m.pos32};
+    int flags = 0;
+#pragma unroll
+    for(int k = 0; k < RPL; ++k)
+        if(sub + 4 * k < nE) flags |= cornerInversion<0>(rows[k], np - o, np, fix);
+    const int sh = (threadIdx.x & 63) & ~3;
+    const bool reject = ((__ballot((flags & INV_FLIP) != 0) >> sh) & 15ull) != 0 &&
+                        ((__ballot((flags & INV_NONPOS32) != 0) >> sh) & 15ull) == 0;
+    if(sub == 0)
+    {
+        st(pos, v, reject ? o : np);
+        m.evalCounts[v] += evals;
+    }
+}
+
 template<typename T, int EK>
 static void launchNmAffine(const VertexWork& w, hipStream_t stream)
 {
+    if constexpr(sizeof(T) == 4 && EK == 0)
+    {
+        // Four lanes per vertex, 64 vertices per 256-thread workgroup.
+        const uint32_t blocks = (w.count + 63) / 64;
+        if(w.maxPatch <= 8) { nmQuadKernel<2><<<blocks, 256, 0, stream>>>(w.meshes, w.segs, w.nsegs, w.count); return; }
+        if(w.maxPatch <= 16) { nmQuadKernel<4><<<blocks, 256, 0, stream>>>(w.meshes, w.segs, w.nsegs, w.count); return; }
+        if(w.maxPatch <= 24) { nmQuadKernel<6><<<blocks, 256, 0, stream>>>(w.meshes, w.segs, w.nsegs, w.count); return; }
+        if(w.maxPatch <= 32) { nmQuadKernel<8><<<blocks, 256, 0, stream>>>(w.meshes, w.segs, w.nsegs, w.count); return; }
+    }
     if(w.maxPatch <= 8)
         nmAffineWaveKernel<T, EK, 8><<<(w.count + 7) / 8, 64, 0, stream>>>(w.meshes, w.segs, w.nsegs, w.count);
     else if(w.maxPatch <= 16)
