@@ -331,7 +331,7 @@ int gm_mesh_evaluate(gm_mesh* m, double* out3, int32_t* hist20, double* elemQual
     if(rc) return rc;
     GM_HIP(gmd::launchResetRun(p->dTable, p->vertBlocks, p->meshList, p->nMeshes, p->stream));
     hipError_t e = gmd::launchEvaluate<double>(p->dTable, p->evalBlocks, p->meshList, p->nMeshes, m->crew,
-                                               m->evaluator, m->sampler, p->stream);
+                                               m->evaluator, m->sampler, p->hexBlocks.count != 0, p->stream);
     if(e != hipSuccess) return fail(GM_ERR_HIP, std::string("evaluate: ") + hipGetErrorString(e));
     double r[3];
     int32_t h[20];

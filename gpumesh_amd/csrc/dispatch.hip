@@ -51,18 +51,18 @@ hipError_t launchGetmeVertices(const VertexWork& w, const gmq::Crew& c, int eval
 
 template<>
 hipError_t launchEvaluate<double>(const DevMesh* meshes, BlockWork blocks, const uint32_t* meshList,
-                                  uint32_t nmeshes, const gmq::Crew& c, int evaluator, int sampler,
+                                  uint32_t nmeshes, const gmq::Crew& c, int evaluator, int sampler, bool hexes,
                                   hipStream_t stream)
 {
-    GMD_SELECT_EVAL(evaluator, sampler, evaluateF64(meshes, blocks, meshList, nmeshes, c, stream))
+    GMD_SELECT_EVAL(evaluator, sampler, evaluateF64(meshes, blocks, meshList, nmeshes, c, hexes, stream))
 }
 
 template<>
 hipError_t launchEvaluate<float>(const DevMesh* meshes, BlockWork blocks, const uint32_t* meshList,
-                                 uint32_t nmeshes, const gmq::Crew& c, int evaluator, int sampler,
+                                 uint32_t nmeshes, const gmq::Crew& c, int evaluator, int sampler, bool hexes,
                                  hipStream_t stream)
 {
-    GMD_SELECT_EVAL(evaluator, sampler, evaluateF32(meshes, blocks, meshList, nmeshes, c, stream))
+    GMD_SELECT_EVAL(evaluator, sampler, evaluateF32(meshes, blocks, meshList, nmeshes, c, hexes, stream))
 }
 
 template hipError_t launchSmooth<double>(const VertexWork&, const gmq::Crew&, int, int, int, int, LaneShape, double,

@@ -20,7 +20,7 @@
 namespace gmd
 {
 
-const int EVAL_MAX_BLOCKS = 1024;
+const int EVAL_MAX_BLOCKS = 4096;
 const int EVAL_THREADS = 256;
 const int ELEM_BLOCK = 256;        // elements per block of the GETMe element kernels
 
@@ -29,7 +29,7 @@ const int ELEM_BLOCK = 256;        // elements per block of the GETMe element ke
 // its result bits) does not depend on what it is batched with.
 inline uint32_t evalBlocksFor(uint64_t E)
 {
-    uint64_t b = (E + 2047) / 2048;
+    uint64_t b = (E + 1023) / 1024;
     if(b < 1) b = 1;
     if(b > (uint64_t)EVAL_MAX_BLOCKS) b = EVAL_MAX_BLOCKS;
     return (uint32_t)b;
@@ -163,9 +163,9 @@ template<int EV, int SA>
 struct Evaluators
 {
     static hipError_t evaluateF64(const DevMesh* meshes, BlockWork blocks, const uint32_t* meshList,
-                                  uint32_t nmeshes, const gmq::Crew& c, hipStream_t stream);
+                                  uint32_t nmeshes, const gmq::Crew& c, bool hexes, hipStream_t stream);
     static hipError_t evaluateF32(const DevMesh* meshes, BlockWork blocks, const uint32_t* meshList,
-                                  uint32_t nmeshes, const gmq::Crew& c, hipStream_t stream);
+                                  uint32_t nmeshes, const gmq::Crew& c, bool hexes, hipStream_t stream);
 };
 
 // Run-time selection over (evaluator, sampler) (dispatch.hip).
@@ -179,9 +179,11 @@ template<typename T>
 hipError_t launchGetmeVertices(const VertexWork& w, const gmq::Crew& c, int evaluator, int sampler, int shape,
                                LaneShape lane, hipStream_t stream);
 // P = position precision read (double: pos64, float: pos32); always FP64 math.
+// `hexes`: some mesh of the batch has hexahedra (a tet-only batch runs a
+// kernel without the hexahedron branch, which needs fewer registers).
 template<typename P>
 hipError_t launchEvaluate(const DevMesh* meshes, BlockWork blocks, const uint32_t* meshList, uint32_t nmeshes,
-                          const gmq::Crew& c, int evaluator, int sampler, hipStream_t stream);
+                          const gmq::Crew& c, int evaluator, int sampler, bool hexes, hipStream_t stream);
 
 // Position precision conversions over blocks of VERT_BLOCK vertices.
 const int VERT_BLOCK = 256;

@@ -1791,7 +1791,65 @@ __device__ __forceinline__ void twoSum(double a, double b, double& s, double& e)
     e = (a - (s - bb)) + (b - bb);
 }
 
-template<int EV, int SA, typename P>
+// Records of the FP32 passes (mean ratio, metric-free): each element's
+// quality in FP64 from the FP32 positions, with det(F)^(-2/3) from the
+// hardware seed and two Newton steps (gma::powm23) and the reciprocal from
+// the hardware estimate and two Newton steps, in place of the correctly
+// rounded cube root and divisions of gmq::cornerQuality (a few ulp apart).
+// Elements with a non-positive corner take the exact routine.  Returns q
+// and sets inv = 1 / q for q > 0.
+__device__ __forceinline__ double rcpNewton(double x)
+{
+    double r = __builtin_amdgcn_rcp(x);
+    double e = __builtin_fma(-x, r, 1.0);
+    r = __builtin_fma(r, e, r);
+    e = __builtin_fma(-x, r, 1.0);
+    return __builtin_fma(r, e, r);
+}
+
+__device__ __forceinline__ double evalCornerInv3(const gmq::m3<double>& F)
+{
+    const double d = gmq::det(F);
+    const double f2 = gmq::dot(F.c[0], F.c[0]) + gmq::dot(F.c[1], F.c[1]) + gmq::dot(F.c[2], F.c[2]);
+    return d > 0.0 ? f2 * gma::powm23(d) : __builtin_nan("");
+}
+
+__device__ __forceinline__ double fastTetEval(const gmq::Crew& c, const v3<double> vp[4], double& inv)
+{
+    const double s = evalCornerInv3(
+        gmq::mmul<double>(gmq::mcols<double>(vp[3] - vp[0], vp[3] - vp[1], vp[3] - vp[2]), gmq::frTetInv<double>()));
+    if(s < __builtin_huge_val())
+    {
+        inv = s * (1.0 / 3.0);
+        return 3.0 * rcpNewton(s);
+    }
+    const double q = gmq::tetQuality<double, 0, 0>(c, vp);
+    inv = 1.0 / q;
+    return q;
+}
+
+__device__ __forceinline__ double fastHexEval(const gmq::Crew& c, const v3<double> vp[8], double& inv)
+{
+    const v3<double> e01 = vp[1] - vp[0], e03 = vp[3] - vp[0], e04 = vp[4] - vp[0], e12 = vp[2] - vp[1];
+    const v3<double> e15 = vp[5] - vp[1], e23 = vp[3] - vp[2], e26 = vp[6] - vp[2], e37 = vp[7] - vp[3];
+    const v3<double> e45 = vp[5] - vp[4], e47 = vp[7] - vp[4], e56 = vp[6] - vp[5], e67 = vp[7] - vp[6];
+    using gmq::mcols;
+    const double s = evalCornerInv3(mcols<double>(e01, e04, -e03)) + evalCornerInv3(mcols<double>(e01, e12, e15)) +
+                     evalCornerInv3(mcols<double>(e12, e26, -e23)) + evalCornerInv3(mcols<double>(e03, e23, e37)) +
+                     evalCornerInv3(mcols<double>(e04, e45, e47)) + evalCornerInv3(mcols<double>(e15, -e56, e45)) +
+                     evalCornerInv3(mcols<double>(e26, e56, e67)) + evalCornerInv3(mcols<double>(e37, e67, -e47));
+    if(s < __builtin_huge_val())
+    {
+        // q = 8 / sum(1 / q_k) = 24 / sum(3 / q_k)
+        inv = s * (1.0 / 24.0);
+        return 24.0 * rcpNewton(s);
+    }
+    const double q = gmq::hexQuality<double, 0, 0>(c, vp);
+    inv = 1.0 / q;
+    return q;
+}
+
+template<int EV, int SA, typename P, bool HEX>
 __global__ void __launch_bounds__(EVAL_THREADS) evalKernel(const DevMesh* __restrict__ meshes,
                                                            const uint4* __restrict__ spans, uint32_t nspans,
                                                            gmq::Crew c)
@@ -1811,31 +1869,39 @@ __global__ void __launch_bounds__(EVAL_THREADS) evalKernel(const DevMesh* __rest
     const uint64_t chunk = (E + m.evalBlocks - 1) / m.evalBlocks;
     const uint64_t b0 = (uint64_t)bl.y * chunk;
     const uint64_t b1 = (b0 + chunk < E) ? b0 + chunk : E;
+    // FP32 records of mean-ratio metric-free runs take the fast FP64 routines.
+    constexpr bool FAST = EV == 0 && SA == 0 && std::is_same<P, float4>::value;
     double mn = 1.0, sum = 0.0, cmp = 0.0;
     int bad = 0;
     for(uint64_t e = b0 + threadIdx.x; e < b1; e += blockDim.x)
     {
-        double q;
-        if(e < m.ntets)
+        double q, inv = 0.0;
+        if(!HEX || e < m.ntets)
         {
             uint4 t = m.tets[e];
             v3<double> vp[4] = {ld<double>(pos, t.x), ld<double>(pos, t.y), ld<double>(pos, t.z), ld<double>(pos, t.w)};
-            q = gmq::tetQuality<double, EV, SA>(c, vp);
+            if constexpr(FAST)
+                q = fastTetEval(c, vp, inv);
+            else
+                q = gmq::tetQuality<double, EV, SA>(c, vp);
         }
-        else
+        else if constexpr(HEX)
         {
             uint64_t h = e - m.ntets;
             uint4 a = m.hexs[2 * h], bq = m.hexs[2 * h + 1];
             v3<double> vp[8] = {ld<double>(pos, a.x), ld<double>(pos, a.y), ld<double>(pos, a.z), ld<double>(pos, a.w),
                                 ld<double>(pos, bq.x), ld<double>(pos, bq.y), ld<double>(pos, bq.z), ld<double>(pos, bq.w)};
-            q = gmq::hexQuality<double, EV, SA>(c, vp);
+            if constexpr(FAST)
+                q = fastHexEval(c, vp, inv);
+            else
+                q = gmq::hexQuality<double, EV, SA>(c, vp);
         }
         if(m.elemQual) m.elemQual[e] = q;
         mn = gmq::tmin(mn, q);
         if(q > 0.0)
         {
             double t, err;
-            twoSum(sum, 1.0 / q, t, err);
+            twoSum(sum, FAST ? inv : 1.0 / q, t, err);
             sum = t;
             cmp += err;
         }
@@ -2101,22 +2167,32 @@ hipError_t Launchers<T, EV, SA>::getmeVertices(const VertexWork& w, const gmq::C
 
 template<int EV, int SA>
 hipError_t Evaluators<EV, SA>::evaluateF64(const DevMesh* meshes, BlockWork blocks, const uint32_t* meshList,
-                                           uint32_t nmeshes, const gmq::Crew& c, hipStream_t stream)
+                                           uint32_t nmeshes, const gmq::Crew& c, bool hexes, hipStream_t stream)
 {
     if(blocks.count == 0 || nmeshes == 0)
         return hipSuccess;
-    evalKernel<EV, SA, double4><<<blocks.count, EVAL_THREADS, 0, stream>>>(meshes, blocks.spans, blocks.nspans, c);
+    if(hexes)
+        evalKernel<EV, SA, double4, true><<<blocks.count, EVAL_THREADS, 0, stream>>>(meshes, blocks.spans,
+                                                                                    blocks.nspans, c);
+    else
+        evalKernel<EV, SA, double4, false><<<blocks.count, EVAL_THREADS, 0, stream>>>(meshes, blocks.spans,
+                                                                                     blocks.nspans, c);
     evalFinalKernel<<<nmeshes, EVAL_THREADS, 0, stream>>>(meshes, meshList);
     return hipGetLastError();
 }
 
 template<int EV, int SA>
 hipError_t Evaluators<EV, SA>::evaluateF32(const DevMesh* meshes, BlockWork blocks, const uint32_t* meshList,
-                                           uint32_t nmeshes, const gmq::Crew& c, hipStream_t stream)
+                                           uint32_t nmeshes, const gmq::Crew& c, bool hexes, hipStream_t stream)
 {
     if(blocks.count == 0 || nmeshes == 0)
         return hipSuccess;
-    evalKernel<EV, SA, float4><<<blocks.count, EVAL_THREADS, 0, stream>>>(meshes, blocks.spans, blocks.nspans, c);
+    if(hexes)
+        evalKernel<EV, SA, float4, true><<<blocks.count, EVAL_THREADS, 0, stream>>>(meshes, blocks.spans,
+                                                                                   blocks.nspans, c);
+    else
+        evalKernel<EV, SA, float4, false><<<blocks.count, EVAL_THREADS, 0, stream>>>(meshes, blocks.spans,
+                                                                                    blocks.nspans, c);
     evalFinalKernel<<<nmeshes, EVAL_THREADS, 0, stream>>>(meshes, meshList);
     return hipGetLastError();
 }

@@ -578,8 +578,10 @@ static hipError_t evaluateAll(Plan* p, const gmq::Crew& c, int ev, int sa, bool 
 {
     int slot = timedBegin(p, timed, GM_KT_EVALUATE);
     hipError_t e = fromF32
-        ? gmd::launchEvaluate<float>(p->dTable, p->evalBlocks, p->meshList, p->nMeshes, c, ev, sa, p->stream)
-        : gmd::launchEvaluate<double>(p->dTable, p->evalBlocks, p->meshList, p->nMeshes, c, ev, sa, p->stream);
+        ? gmd::launchEvaluate<float>(p->dTable, p->evalBlocks, p->meshList, p->nMeshes, c, ev, sa,
+                                     p->hexBlocks.count != 0, p->stream)
+        : gmd::launchEvaluate<double>(p->dTable, p->evalBlocks, p->meshList, p->nMeshes, c, ev, sa,
+                                      p->hexBlocks.count != 0, p->stream);
     timedEnd(p, slot);
     return e;
 }
