@@ -71,20 +71,17 @@ GMA float frcp(float x) { return 1.0f / x; }
 // a^(-2/3) for a > 0.
 GMA float powm23(float a) { return fexp2(-0.6666666666666666f * flog2(a)); }
 
-// FP64: a seed from the FP32 hardware log/exp on the mantissa (exponent
-// handled exactly), then two Newton steps on y^3 a^2 = 1, each of which
-// squares the relative error (1e-7 -> 1e-14 -> below the FP64 rounding).
+// FP64: a seed from the FP32 hardware log/exp where a is a normal float
+// (relative error below 1e-5: the log's absolute error grows with |log2 a|
+// <= 126), then two Newton steps on y^3 a^2 = 1, each of which takes the
+// relative error e to about 2 e^2 (1e-5 -> 2e-10 -> below the FP64
+// rounding).  Other values (zero, subnormal, beyond the float range, inf,
+// NaN) take the exact slow path.
 GMA double powm23(double a)
 {
-    const uint64_t b = gm_dbits(a);
-    const int e = (int)((b >> 52) & 0x7ff) - 1023;
-    if(e == -1023 || e == 1024)
-        return 1.0 / gm_pow23(a);   // zero, subnormal, inf, NaN: exact slow path
-    const double m = gm_dfrom((b & 0x000FFFFFFFFFFFFFull) | 0x3FF0000000000000ull);
-    const double L = -0.6666666666666666 * ((double)e + (double)flog2((float)m));
-    const double fl = __builtin_floor(L);
-    const int i = (int)fl;
-    double y = (double)fexp2((float)(L - fl)) * gm_pow2i(i);
+    if(!(a >= 1.1754943508222875e-38 && a <= 3.4028234663852886e+38))
+        return 1.0 / gm_pow23(a);
+    double y = (double)fexp2(-0.6666666666666666f * flog2((float)a));
 #pragma unroll
     for(int it = 0; it < 2; ++it)
     {

@@ -857,59 +857,89 @@ __device__ __forceinline__ T affinePatch(const unsigned char* sp, int nE, v3<T> 
     return bad ? affinePatchExact<T, EK>(sp, nE, x, xx) : q;
 }
 
-template<typename T, int EK>
-__global__ void __launch_bounds__(512) spawnAffineKernel(const DevMesh* __restrict__ meshes,
-                                                         const uint4* __restrict__ segs, uint32_t nsegs,
-                                                         uint32_t count, T moveCoeff, int cap)
+// A value of lane `src` (wavefront-uniform; every lane must be active).
+__device__ __forceinline__ float readLane(float v, int src)
 {
-    typedef typename PosOf<T>::type P;
-    extern __shared__ __align__(16) unsigned char smem[];
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), src));
+}
+
+__device__ __forceinline__ double readLane(double v, int src)
+{
+    const uint64_t b = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)b, src);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), src);
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
+// Stages the affine terms of the patches of the VPW vertices of a wavefront
+// (vertex h's lanes h L .. h L + L - 1, L = 64 / VPW) into their tiles
+// waveSp + h tileBytes, all 64 lanes sharing the rows: row r of the
+// wavefront is row r - start(h) of vertex h.  Vertex h's element range and
+// position are read from its lane h L while every lane is active (a
+// cross-lane read from a lane that is off returns zero, and the staging loop
+// runs on fewer lanes).  Returns the wavefront's largest patch.
+template<typename T, int EK, int VPW>
+__device__ __forceinline__ int stageWaveRows(const DevMesh& m, const typename PosOf<T>::type* pos,
+                                             unsigned char* waveSp, size_t tileBytes, int nE, uint32_t eb, v3<T> o)
+{
+    constexpr int L = 64 / VPW;
     const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
-    const int ppg = blockDim.x >> 6;
-    unsigned char* sp = smem + (size_t)wave * cap * affineBytes<T, EK>();
-    const uint32_t i = blockIdx.x * ppg + wave;
-    const uint4 span = waveSpan(segs, nsegs, i);
-    const DevMesh& m = meshes[span.x];
-    const uint32_t item = i < count && !runStopped(m) ? spanVertex(m, span, i) : NO_ITEM;
-    const bool present = item != NO_ITEM;
-    const uint32_t v = present ? item : 0;
-    P* pos = PosOf<T>::get(m);
-    const uint32_t eb = m.eoff[v];
-    const int nE = present ? (int)(m.eoff[v + 1] - eb) : 0;
-    const v3<T> o = present ? ld<T>(pos, v) : mk<T>(T(0), T(0), T(0));
-    for(int e = lane; e < nE; e += 64)
-        stageAffine<T, EK>(m, pos, m.eref[eb + e], o, sp + (size_t)e * affineBytes<T, EK>());
-    __syncthreads();
-    if(!present)
-        return;
-    T scale = moveCoeff * localSizeGroup<64>(m, pos, v, o, lane);
-    v3<T> xb = mk<T>(T(0), T(0), T(0));
-    for(int iter = 0; iter < 2; ++iter)
+    int start[VPW + 1];
+    uint32_t ebOf[VPW];
+    v3<T> oOf[VPW];
+    start[0] = 0;
+#pragma unroll
+    for(int h = 0; h < VPW; ++h)
     {
-        const v3<T> x = xb + gms::spawnOffset<T>(lane) * scale;
-        // The best candidate has the smallest sum of 3/q.  A candidate with a
-        // non-positive element has quality <= 0 and loses to any candidate
-        // without one; its exact value matters only when every candidate of
-        // the vertex has one.
-        const T xx = gmq::dot(x, x);
-        const T inv3 = affineSumFast<T, EK>(sp, nE, x, xx);
-        const bool bad = gma::fastBad<T>(inv3);
-        T key;
-        if(__ballot(!bad) == 0)
-            key = affinePatchExact<T, EK>(sp, nE, x, xx);
-        else
-            key = bad ? -(T)__builtin_huge_val() : -inv3;
-        const int best = waveArgmax(key);
-        xb = shflv<T>(x, best);
-        scale /= T(3);
+        start[h + 1] = start[h] + __builtin_amdgcn_readlane(nE, h * L);
+        ebOf[h] = (uint32_t)__builtin_amdgcn_readlane((int)eb, h * L);
+        oOf[h] = mk<T>(readLane(o.x, h * L), readLane(o.y, h * L), readLane(o.z, h * L));
     }
-    const v3<T> np = testedMove<T, EK>(m, sp, nE, lane, 64, 0, o, xb);
-    if(lane == 0)
+    int nEmax = 0;
+#pragma unroll
+    for(int h = 0; h < VPW; ++h) nEmax = max(nEmax, start[h + 1] - start[h]);
+    for(int r = lane; r < start[VPW]; r += 64)
     {
-        st(pos, v, np);
-        m.evalCounts[v] += 128;
+        int h = 0;
+        uint32_t ebH = ebOf[0];
+        v3<T> oH = oOf[0];
+#pragma unroll
+        for(int t = 1; t < VPW; ++t)
+            if(r >= start[t])
+            {
+                h = t;
+                ebH = ebOf[t];
+                oH = oOf[t];
+            }
+        const int e = r - start[h];
+        stageAffine<T, EK>(m, pos, m.eref[ebH + e], oH, waveSp + (size_t)h * tileBytes + (size_t)e * affineBytes<T, EK>());
     }
+    return nEmax;
+}
+
+// Largest value over aligned groups of G lanes (FP64, shuffles).
+template<int G> __device__ __forceinline__ double groupMax(double v)
+{
+#pragma unroll
+    for(int off = G / 2; off > 0; off >>= 1)
+        v = __builtin_fmax(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+// The first lattice point, in the order j + L p of a vertex's lanes j and
+// sweep steps p, whose key equals the group's largest; `mask` selects the
+// vertex's lanes, starting at lane `first`.
+template<typename T, int P, int L>
+__device__ __forceinline__ int firstBest(const T (&key)[P], T mx, unsigned long long mask, int first)
+{
+    int cand = 0;
+#pragma unroll
+    for(int p = P - 1; p >= 0; --p)
+    {
+        const unsigned long long hit = (__ballot(key[p] == mx) & mask) >> first;
+        if(hit) cand = (int)__builtin_ctzll(hit) + L * p;
+    }
+    return cand;
 }
 
 // ---------------------------------------------------------------------------
@@ -967,19 +997,69 @@ __device__ __forceinline__ float mfmaRowSum(const unsigned char* sp, int nRows, 
     return sum;
 }
 
+// The sums of mfmaRowSum at P candidates per lane, each row's coefficients
+// read from LDS once; the constant terms (d0, c0) go through the matrix
+// cores once per row block and start every candidate's chains.
+template<int EK, int P>
+__device__ __forceinline__ void mfmaRowSums(const unsigned char* sp, int nRows, const v3<float> (&x)[P],
+                                            const float (&xx)[P], float (&sum)[P])
+{
+    const int sub = threadIdx.x & 3;
+#pragma unroll
+    for(int p = 0; p < P; ++p) sum[p] = 0.0f;
+    for(int r0 = 0; r0 < nRows; r0 += 4)
+    {
+        const gma::Corner<float>& k = cornerRow<EK>(sp, r0 + sub);
+        const float gx = k.gx, gy = k.gy, gz = k.gz, hx = k.hx, hy = k.hy, hz = k.hz, ks = k.s;
+        const mfv4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
+        const mfv4 d0 = mfma4x4(k.d0, 1.0f, zero);
+        const mfv4 c0 = mfma4x4(k.c0, 1.0f, zero);
+#pragma unroll
+        for(int p = 0; p < P; ++p)
+        {
+            mfv4 d = mfma4x4(gx, x[p].x, d0);
+            mfv4 f = mfma4x4(hx, x[p].x, c0);
+            d = mfma4x4(gy, x[p].y, d);
+            f = mfma4x4(hy, x[p].y, f);
+            d = mfma4x4(gz, x[p].z, d);
+            f = mfma4x4(hz, x[p].z, f);
+            f = mfma4x4(ks, xx[p], f);
+#pragma unroll
+            for(int r = 0; r < 4; ++r) sum[p] += f[r] * gma::powm23(d[r]);
+        }
+    }
+}
+
+// Vertices per wavefront of the FP32 spawn-search kernel.
+constexpr int SPAWN_VPW = 4;
+
 // Spawn search, FP32 affine terms on the matrix cores (the schedule, the
-// candidates and the rules of spawnAffineKernel).
-template<int EK>
-__global__ void __launch_bounds__(512) spawnMfmaKernel(const DevMesh* __restrict__ meshes,
+// candidates and the rules of spawnAffineKernel), VPW vertices per
+// wavefront.  The L = 64 / VPW lanes of vertex g (lanes g L ...) take its
+// lattice points j, j + L, ... (P = VPW per lane); a matrix instruction's
+// 4-lane blocks each read their own vertex's rows, so one instruction
+// serves every vertex of the wavefront.  All lanes of the wavefront stage
+// the rows of all its vertices together, and every vertex's rows are padded
+// with neutral ones (d = 1, f2 = 0: they add an exact zero) to the
+// wavefront's largest row count, so a vertex's result does not depend on
+// its neighbours in the wavefront.  The staging, the local size, the argmax
+// and the inversion test are shared by VPW vertices instead of one.
+template<int EK, int VPW>
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(EK == 0 ? 8 : 1))) spawnMfmaKernel(const DevMesh* __restrict__ meshes,
                                                        const uint4* __restrict__ segs, uint32_t nsegs,
                                                        uint32_t count, float moveCoeff, int cap)
 {
+    constexpr int L = 64 / VPW;
+    constexpr int P = VPW;
     extern __shared__ __align__(16) unsigned char smem[];
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int ppg = blockDim.x >> 6;
-    unsigned char* sp = smem + (size_t)wave * cap * affineBytes<float, EK>();
-    const uint32_t i = blockIdx.x * ppg + wave;
+    const int g = lane / L, j = lane % L;
+    const size_t tileBytes = (size_t)cap * affineBytes<float, EK>();
+    unsigned char* waveSp = smem + (size_t)wave * VPW * tileBytes;
+    unsigned char* sp = waveSp + (size_t)g * tileBytes;
+    const uint32_t i = (blockIdx.x * ppg + wave) * VPW + g;
     const uint4 span = waveSpan(segs, nsegs, i);
     const DevMesh& m = meshes[span.x];
     const uint32_t item = i < count && !runStopped(m) ? spanVertex(m, span, i) : NO_ITEM;
@@ -989,48 +1069,161 @@ __global__ void __launch_bounds__(512) spawnMfmaKernel(const DevMesh* __restrict
     const uint32_t eb = m.eoff[v];
     const int nE = present ? (int)(m.eoff[v + 1] - eb) : 0;
     const v3<float> o = present ? ld<float>(pos, v) : mk<float>(0.0f, 0.0f, 0.0f);
-    for(int e = lane; e < nE; e += 64)
-        stageAffine<float, EK>(m, pos, m.eref[eb + e], o, sp + (size_t)e * affineBytes<float, EK>());
-    // Neutral rows up to a multiple of 4 (the tile size is a multiple of 4).
-    const int nRows = EK == 0 ? (nE + 3) & ~3 : 4 * nE;
-    if(EK == 0 && nE + lane < nRows)
+    const int nEmax = stageWaveRows<float, EK, VPW>(m, pos, waveSp, tileBytes, nE, eb, o);
+    // Neutral rows (tetrahedra: up to a multiple of 4 rows; hexahedra: whole
+    // neutral elements) up to the wavefront's largest patch.
+    const int nPad = EK == 0 ? (nEmax + 3) & ~3 : nEmax;
+    for(int e = nE + j; e < nPad; e += L)
     {
-        gma::Corner<float> z{};
-        z.d0 = 1.0f;
-        ((gma::Corner<float>*)sp)[nE + lane] = z;
+        if(EK == 0)
+        {
+            gma::Corner<float> z{};
+            z.d0 = 1.0f;
+            ((gma::Corner<float>*)sp)[e] = z;
+        }
+        else
+        {
+            gma::Hex<float> z{};
+            for(int c = 0; c < 4; ++c) z.k[c].d0 = 1.0f;
+            z.constInv3 = 0.0f;
+            z.constMin = 1.0f;
+            ((gma::Hex<float>*)sp)[e] = z;
+        }
     }
     __syncthreads();
-    if(!present)
+    if(__ballot(present) == 0)
         return;
+    const int nRows = EK == 0 ? nPad : 4 * nPad;
     float constSum = 0.0f;
     if(EK == 1)
         for(int e = 0; e < nE; ++e) constSum += ((const gma::Hex<float>*)sp)[e].constInv3;
-    float scale = moveCoeff * localSizeGroup<64>(m, pos, v, o, lane);
+    float scale = present ? moveCoeff * localSizeGroup<L>(m, pos, v, o, j) : 0.0f;
+    const unsigned long long groupMask = (L == 64 ? ~0ull : ((1ull << L) - 1)) << (g * L);
     v3<float> xb = mk<float>(0.0f, 0.0f, 0.0f);
     for(int iter = 0; iter < 2; ++iter)
     {
-        const v3<float> x = xb + gms::spawnOffset<float>(lane) * scale;
-        const float xx = gmq::dot(x, x);
-        float sum = mfmaRowSum<EK>(sp, nRows, x, xx);
-        if(EK == 1) sum = 0.125f * (sum + constSum);
-        const bool bad = gma::fastBad<float>(sum);
+        v3<float> x[P];
+        float xx[P], sum[P];
+#pragma unroll
+        for(int p = 0; p < P; ++p)
+        {
+            x[p] = xb + gms::spawnOffset<float>(j + L * p) * scale;
+            xx[p] = gmq::dot(x[p], x[p]);
+        }
+        mfmaRowSums<EK, P>(sp, nRows, x, xx, sum);
         // The patch quality 3 n / sum falls as the sum grows, so the best
         // candidate has the smallest sum.  As in spawnAffineKernel, a
         // candidate with a non-positive element loses unless every candidate
-        // has one; then the exact rule decides.
-        float key;
-        if(__ballot(!bad) == 0)
-            key = affinePatchExact<float, EK>(sp, nE, x, xx);
-        else
-            key = bad ? -__builtin_huge_valf() : -sum;
-        const int best = waveArgmax(key);
-        xb = shflv<float>(x, best);
+        // of the vertex has one; then the exact rule decides.
+        bool anyGood = false;
+#pragma unroll
+        for(int p = 0; p < P; ++p)
+        {
+            if(EK == 1) sum[p] = 0.125f * (sum[p] + constSum);
+            anyGood |= !gma::fastBad<float>(sum[p]);
+        }
+        const bool exact = (__ballot(anyGood) & groupMask) == 0;
+        float key[P];
+        float best = -__builtin_huge_valf();
+#pragma unroll
+        for(int p = 0; p < P; ++p)
+        {
+            if(exact)
+                key[p] = affinePatchExact<float, EK>(sp, nE, x[p], xx[p]);
+            else
+                key[p] = gma::fastBad<float>(sum[p]) ? -__builtin_huge_valf() : -sum[p];
+            key[p] = key[p] == key[p] ? key[p] : -__builtin_huge_valf();
+            best = __builtin_fmaxf(best, key[p]);
+        }
+        const int cand = firstBest<float, P, L>(key, groupMax<L>(best), groupMask, g * L);
+        xb = xb + gms::spawnOffset<float>(cand) * scale;
         scale *= 1.0f / 3.0f;
     }
-    const v3<float> np = testedMove<float, EK>(m, sp, nE, lane, 64, 0, o, xb);
-    if(lane == 0)
+    if(!present)
+        return;
+    const v3<float> np = testedMove<float, EK>(m, sp, nE, j, L, g * L, o, xb);
+    if(j == 0)
     {
         st(pos, v, np);
+        m.evalCounts[v] += 128;
+    }
+}
+
+// Spawn search in FP64 on the vector ALUs (the polish pass), with the
+// layout of spawnMfmaKernel: VPW vertices per wavefront, L = 64 / VPW lanes
+// each taking lattice points j, j + L, ...; each lane sums its vertex's
+// staged rows in element order.
+template<int EK, int VPW>
+__global__ void __launch_bounds__(512) spawnAffineKernel(const DevMesh* __restrict__ meshes,
+                                                         const uint4* __restrict__ segs, uint32_t nsegs,
+                                                         uint32_t count, double moveCoeff, int cap)
+{
+    constexpr int L = 64 / VPW;
+    constexpr int P = VPW;
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int ppg = blockDim.x >> 6;
+    const int g = lane / L, j = lane % L;
+    const size_t tileBytes = (size_t)cap * affineBytes<double, EK>();
+    unsigned char* waveSp = smem + (size_t)wave * VPW * tileBytes;
+    const unsigned char* sp = waveSp + (size_t)g * tileBytes;
+    const uint32_t i = (blockIdx.x * ppg + wave) * VPW + g;
+    const uint4 span = waveSpan(segs, nsegs, i);
+    const DevMesh& m = meshes[span.x];
+    const uint32_t item = i < count && !runStopped(m) ? spanVertex(m, span, i) : NO_ITEM;
+    const bool present = item != NO_ITEM;
+    const uint32_t v = present ? item : 0;
+    double4* pos = m.pos64;
+    const uint32_t eb = m.eoff[v];
+    const int nE = present ? (int)(m.eoff[v + 1] - eb) : 0;
+    const v3<double> o = present ? ld<double>(pos, v) : mk<double>(0.0, 0.0, 0.0);
+    stageWaveRows<double, EK, VPW>(m, pos, waveSp, tileBytes, nE, eb, o);
+    __syncthreads();
+    if(__ballot(present) == 0)
+        return;
+    double scale = present ? moveCoeff * localSizeGroup<L>(m, pos, v, o, j) : 0.0;
+    const unsigned long long groupMask = (L == 64 ? ~0ull : ((1ull << L) - 1)) << (g * L);
+    v3<double> xb = mk<double>(0.0, 0.0, 0.0);
+    for(int iter = 0; iter < 2; ++iter)
+    {
+        // The best candidate has the smallest sum of 3/q.  A candidate with a
+        // non-positive element has quality <= 0 and loses to any candidate
+        // without one; its exact value matters only when every candidate of
+        // the vertex has one.
+        v3<double> x[P];
+        double xx[P], sum[P];
+        bool anyGood = false;
+#pragma unroll
+        for(int p = 0; p < P; ++p)
+        {
+            x[p] = xb + gms::spawnOffset<double>(j + L * p) * scale;
+            xx[p] = gmq::dot(x[p], x[p]);
+            sum[p] = affineSumFast<double, EK>(sp, nE, x[p], xx[p]);
+            anyGood |= !gma::fastBad<double>(sum[p]);
+        }
+        const bool exact = (__ballot(anyGood) & groupMask) == 0;
+        double key[P];
+        double best = -__builtin_huge_val();
+#pragma unroll
+        for(int p = 0; p < P; ++p)
+        {
+            if(exact)
+                key[p] = affinePatchExact<double, EK>(sp, nE, x[p], xx[p]);
+            else
+                key[p] = gma::fastBad<double>(sum[p]) ? -__builtin_huge_val() : -sum[p];
+            key[p] = key[p] == key[p] ? key[p] : -__builtin_huge_val();
+            best = __builtin_fmax(best, key[p]);
+        }
+        const int cand = firstBest<double, P, L>(key, groupMax<L>(best), groupMask, g * L);
+        xb = xb + gms::spawnOffset<double>(cand) * scale;
+        scale /= 3.0;
+    }
+    if(!present)
+        return;
+    if(j == 0)
+    {
+        st(pos, v, o + xb);
         m.evalCounts[v] += 128;
     }
 }
@@ -2028,11 +2221,18 @@ static hipError_t launchLane(const VertexWork& w, const gmq::Crew& c, LaneShape 
             if(aff)
             {
                 if constexpr(sizeof(T) == 4)
-                    spawnMfmaKernel<EK><<<blocks, 64 * ppg, lds, stream>>>(w.meshes, w.segs, w.nsegs, w.count,
-                                                                           (float)moveCoeff, cap);
+                {
+                    // SPAWN_VPW vertices (tiles) per wavefront.
+                    const uint32_t vb = (w.count + SPAWN_VPW * ppg - 1) / (SPAWN_VPW * ppg);
+                    spawnMfmaKernel<EK, SPAWN_VPW><<<vb, 64 * ppg, SPAWN_VPW * lds, stream>>>(
+                        w.meshes, w.segs, w.nsegs, w.count, (float)moveCoeff, cap);
+                }
                 else
-                    spawnAffineKernel<T, EK><<<blocks, 64 * ppg, lds, stream>>>(w.meshes, w.segs, w.nsegs, w.count,
-                                                                                (T)moveCoeff, cap);
+                {
+                    const uint32_t vb = (w.count + SPAWN_VPW * ppg - 1) / (SPAWN_VPW * ppg);
+                    spawnAffineKernel<EK, SPAWN_VPW><<<vb, 64 * ppg, SPAWN_VPW * lds, stream>>>(
+                        w.meshes, w.segs, w.nsegs, w.count, moveCoeff, cap);
+                }
                 return hipGetLastError();
             }
         spawnLaneKernel<T, EV, SA, EK><<<blocks, 64 * ppg, lds, stream>>>(w.meshes, w.segs, w.nsegs, w.count, c,
@@ -2071,7 +2271,8 @@ inline bool laneFits(const VertexWork& w, int smoother, LaneShape lane)
         return false;
     const bool aff = useAffine<T, EV, SA>(w);
     const size_t per = w.kind == EK_TET ? laneStageBytes<T, EK_TET>(cap, aff) : laneStageBytes<T, EK_HEX>(cap, aff);
-    const size_t lds = (smoother == SM_SPAWN_SEARCH ? (size_t)ppg : (size_t)8 * ppg) * per;
+    const size_t spawnTiles = (size_t)ppg * (aff ? SPAWN_VPW : 1);
+    const size_t lds = (smoother == SM_SPAWN_SEARCH ? spawnTiles : (size_t)8 * ppg) * per;
     return lds <= 64 * 1024;
 }
 
