@@ -20,7 +20,7 @@
 namespace gmd
 {
 
-const int EVAL_MAX_BLOCKS = 4096;
+const int EVAL_MAX_BLOCKS = 2048;
 const int EVAL_THREADS = 256;
 const int ELEM_BLOCK = 256;        // elements per block of the GETMe element kernels
 

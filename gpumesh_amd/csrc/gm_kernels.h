@@ -2100,7 +2100,20 @@ __global__ void __launch_bounds__(EVAL_THREADS) evalKernel(const DevMesh* __rest
         }
         else
             bad = 1;
-        atomicAdd(&shist[histBucket(q)], 1);
+        // One LDS increment per distinct bucket of the wavefront (most
+        // elements share a bucket; per-lane atomics on one address
+        // serialise).
+        const int bucket = histBucket(q);
+        unsigned long long pending = __ballot(true);
+        while(pending)
+        {
+            const int leader = (int)__builtin_ctzll(pending);
+            const int lb = __builtin_amdgcn_readlane(bucket, leader);
+            const unsigned long long same = __ballot(bucket == lb);
+            if((int)(threadIdx.x & 63) == leader)
+                atomicAdd(&shist[lb], (int)__builtin_popcountll(same));
+            pending &= ~same;
+        }
     }
     smin[threadIdx.x] = mn;
     ssum[threadIdx.x] = sum;
