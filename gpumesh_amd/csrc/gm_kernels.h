@@ -255,7 +255,10 @@ __global__ void __launch_bounds__(256) smoothPlainKernel(const DevMesh* __restri
 // optimiser in lock-step (FP32 reduces over a shuffle butterfly instead).
 // 64 / G vertices share a wavefront; a block is one wavefront.
 // ---------------------------------------------------------------------------
-template<typename T, int EV, int SA, int G>
+// EKIND: the patches' element kind (EK_TET or EK_HEX when the launch holds
+// one kind only, EK_MIXED otherwise); a tetrahedron-only launch keeps four
+// vertices per lane.
+template<typename T, int EV, int SA, int G, int EKIND = EK_MIXED>
 struct WavePatch
 {
     const gmq::Crew* c;
@@ -264,7 +267,7 @@ struct WavePatch
     int nElem;        // patch size (<= G)
     int isHex;
     int k;            // local index of the centre vertex in this lane's element
-    v3<T> vp[8];      // this lane's element (centre slot overwritten per call)
+    v3<T> vp[EKIND == EK_TET ? 4 : 8];   // this lane's element (centre slot overwritten per call)
     int count;
 
     __device__ T operator()(v3<T> p)
@@ -280,14 +283,15 @@ struct WavePatch
         T q = T(1);
         if(lane < nElem)
         {
-            if(!isHex)
+            const bool hex = EKIND == EK_HEX || (EKIND == EK_MIXED && isHex);
+            if(!hex)
             {
                 v3<T> e[4];
 #pragma unroll
                 for(int i = 0; i < 4; ++i) e[i] = (i == k) ? p : vp[i];
                 q = gmq::tetQuality<T, EV, SA>(*c, e);
             }
-            else
+            else if constexpr(EKIND != EK_TET)
             {
                 v3<T> e[8];
 #pragma unroll
@@ -1855,8 +1859,9 @@ __global__ void __launch_bounds__(512) getmeVertexKernel(const DevMesh* __restri
 }
 
 // GETMe vertex step, one vertex per group of G lanes (G >= patch size): lane j
-// pulls element j's contribution and holds its vertices; the weighted sum is
-// formed in patch order through LDS (bit-identical to getmeVertexKernel).
+// pulls element j's contribution and holds its vertices; in FP64 the
+// weighted sum is formed in patch order through LDS (bit-identical to
+// getmeVertexKernel), in FP32 by a DPP tree.
 // The patch quality before the move is reduced from getmeQual: every element
 // quality there was computed from the element's current vertex positions
 // (by the element step, or by the accepted move that last changed one of
@@ -1866,14 +1871,14 @@ __global__ void __launch_bounds__(512) getmeVertexKernel(const DevMesh* __restri
 // resident wavefronts to cover the dependent gathers.
 constexpr int GETME_WAVES = 4;
 
-template<typename T, int EV, int SA, int G>
+template<typename T, int EV, int SA, int G, int EKIND>
 __global__ void __launch_bounds__(64 * GETME_WAVES) getmeWaveKernel(const DevMesh* __restrict__ meshes,
                                                                     const uint4* __restrict__ segs, uint32_t nsegs,
                                                                     uint32_t count, gmq::Crew c)
 {
     typedef typename PosOf<T>::type P;
     __shared__ T red[2 * 64 * GETME_WAVES];
-    __shared__ T pull[4 * 64 * GETME_WAVES];
+    __shared__ T pull[sizeof(T) == 8 ? 4 * 64 * GETME_WAVES : 1];
     const int lane = threadIdx.x % G;
     const int grp = threadIdx.x / G;
     const uint32_t i = blockIdx.x * (64 * GETME_WAVES / G) + grp;
@@ -1887,7 +1892,7 @@ __global__ void __launch_bounds__(64 * GETME_WAVES) getmeWaveKernel(const DevMes
     P* pos = PosOf<T>::get(m);
     const P* tetOut = (const P*)m.getmeTet;
     const P* hexOut = (const P*)m.getmeHex;
-    WavePatch<T, EV, SA, G> f;
+    WavePatch<T, EV, SA, G, EKIND> f;
     f.c = &c;
     f.red = red + 2 * G * grp;
     f.lane = lane;
@@ -1900,6 +1905,8 @@ __global__ void __launch_bounds__(64 * GETME_WAVES) getmeWaveKernel(const DevMes
     T* qual = (T*)m.getmeQual;
     uint32_t qslot = 0;
     T qOld = T(1);
+    v3<T> pw = mk<T>(T(0), T(0), T(0));
+    T w = T(0);
     if(lane < f.nElem)
     {
         const uint32_t ref = m.eref[eb + lane];
@@ -1909,14 +1916,14 @@ __global__ void __launch_bounds__(64 * GETME_WAVES) getmeWaveKernel(const DevMes
         qslot = f.isHex ? m.ntets + id : id;
         qOld = qual[qslot];
         P o;
-        if(!f.isHex)
+        if(EKIND == EK_TET || (EKIND == EK_MIXED && !f.isHex))
         {
             uint4 t = m.tets[id];
             f.vp[0] = ld<T>(pos, t.x); f.vp[1] = ld<T>(pos, t.y);
             f.vp[2] = ld<T>(pos, t.z); f.vp[3] = ld<T>(pos, t.w);
             o = tetOut[4 * (size_t)id + f.k];
         }
-        else
+        else if constexpr(EKIND != EK_TET)
         {
             uint4 a = m.hexs[2 * id], b = m.hexs[2 * id + 1];
             f.vp[0] = ld<T>(pos, a.x); f.vp[1] = ld<T>(pos, a.y);
@@ -1925,24 +1932,37 @@ __global__ void __launch_bounds__(64 * GETME_WAVES) getmeWaveKernel(const DevMes
             f.vp[6] = ld<T>(pos, b.z); f.vp[7] = ld<T>(pos, b.w);
             o = hexOut[8 * (size_t)id + f.k];
         }
-        const T w = (T)o.w;
-        const v3<T> pw = mk<T>((T)o.x, (T)o.y, (T)o.z) * w;
-        pl[4 * lane] = pw.x;
-        pl[4 * lane + 1] = pw.y;
-        pl[4 * lane + 2] = pw.z;
-        pl[4 * lane + 3] = w;
+        w = (T)o.w;
+        pw = mk<T>((T)o.x, (T)o.y, (T)o.z) * w;
     }
-    __atomic_signal_fence(__ATOMIC_SEQ_CST);
-    __builtin_amdgcn_wave_barrier();
     v3<T> acc = mk<T>(T(0), T(0), T(0));
     T wacc = T(0);
-    for(int j = 0; j < f.nElem; ++j)
+    if constexpr(sizeof(T) == 4)
     {
-        acc = acc + mk<T>(pl[4 * j], pl[4 * j + 1], pl[4 * j + 2]);
-        wacc += pl[4 * j + 3];
+        // FP32: a DPP tree over the group (lanes past the patch add zeros).
+        acc = mk<float>(groupSum<G>(pw.x), groupSum<G>(pw.y), groupSum<G>(pw.z));
+        wacc = groupSum<G>(w);
     }
-    __atomic_signal_fence(__ATOMIC_SEQ_CST);
-    __builtin_amdgcn_wave_barrier();
+    else
+    {
+        // FP64: the patch order of the reference's accumulation.
+        if(lane < f.nElem)
+        {
+            pl[4 * lane] = pw.x;
+            pl[4 * lane + 1] = pw.y;
+            pl[4 * lane + 2] = pw.z;
+            pl[4 * lane + 3] = w;
+        }
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        __builtin_amdgcn_wave_barrier();
+        for(int j = 0; j < f.nElem; ++j)
+        {
+            acc = acc + mk<T>(pl[4 * j], pl[4 * j + 1], pl[4 * j + 2]);
+            wacc += pl[4 * j + 3];
+        }
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        __builtin_amdgcn_wave_barrier();
+    }
     if(wacc != T(0))
     {
         const v3<T> p1 = acc / wacc;
@@ -2349,6 +2369,24 @@ hipError_t Launchers<T, EV, SA>::getmeElements(const DevMesh* meshes, BlockWork 
     return hipGetLastError();
 }
 
+template<typename T, int EV, int SA, int EKIND>
+static void launchGetmeWave(const VertexWork& w, const gmq::Crew& c, hipStream_t stream)
+{
+    const int threads = 64 * GETME_WAVES;
+    if(w.maxPatch <= 8)
+        getmeWaveKernel<T, EV, SA, 8, EKIND><<<(w.count + threads / 8 - 1) / (threads / 8), threads, 0, stream>>>(
+            w.meshes, w.segs, w.nsegs, w.count, c);
+    else if(w.maxPatch <= 16)
+        getmeWaveKernel<T, EV, SA, 16, EKIND><<<(w.count + threads / 16 - 1) / (threads / 16), threads, 0, stream>>>(
+            w.meshes, w.segs, w.nsegs, w.count, c);
+    else if(w.maxPatch <= 32)
+        getmeWaveKernel<T, EV, SA, 32, EKIND><<<(w.count + threads / 32 - 1) / (threads / 32), threads, 0, stream>>>(
+            w.meshes, w.segs, w.nsegs, w.count, c);
+    else
+        getmeWaveKernel<T, EV, SA, 64, EKIND><<<(w.count + threads / 64 - 1) / (threads / 64), threads, 0, stream>>>(
+            w.meshes, w.segs, w.nsegs, w.count, c);
+}
+
 template<typename T, int EV, int SA>
 hipError_t Launchers<T, EV, SA>::getmeVertices(const VertexWork& w, const gmq::Crew& c, int shape, LaneShape lane,
                                                hipStream_t stream)
@@ -2357,19 +2395,9 @@ hipError_t Launchers<T, EV, SA>::getmeVertices(const VertexWork& w, const gmq::C
         return hipSuccess;
     if(shape == KS_WAVE && w.maxPatch <= 64)
     {
-        const int threads = 64 * GETME_WAVES;
-        if(w.maxPatch <= 8)
-            getmeWaveKernel<T, EV, SA, 8><<<(w.count + threads / 8 - 1) / (threads / 8), threads, 0, stream>>>(
-                w.meshes, w.segs, w.nsegs, w.count, c);
-        else if(w.maxPatch <= 16)
-            getmeWaveKernel<T, EV, SA, 16><<<(w.count + threads / 16 - 1) / (threads / 16), threads, 0, stream>>>(
-                w.meshes, w.segs, w.nsegs, w.count, c);
-        else if(w.maxPatch <= 32)
-            getmeWaveKernel<T, EV, SA, 32><<<(w.count + threads / 32 - 1) / (threads / 32), threads, 0, stream>>>(
-                w.meshes, w.segs, w.nsegs, w.count, c);
-        else
-            getmeWaveKernel<T, EV, SA, 64><<<(w.count + threads / 64 - 1) / (threads / 64), threads, 0, stream>>>(
-                w.meshes, w.segs, w.nsegs, w.count, c);
+        if(w.kind == EK_TET) launchGetmeWave<T, EV, SA, EK_TET>(w, c, stream);
+        else if(w.kind == EK_HEX) launchGetmeWave<T, EV, SA, EK_HEX>(w, c, stream);
+        else launchGetmeWave<T, EV, SA, EK_MIXED>(w, c, stream);
         return hipGetLastError();
     }
     const int ppg = lane.patchesPerGroup >= 1 && lane.patchesPerGroup <= 8 ? lane.patchesPerGroup : 4;
