@@ -249,7 +249,7 @@ int gm_mesh_get_positions(gm_mesh* m, double* pos)
     if(V) GM_HIP(hipMemcpyAsync(p64.data(), m->dm.pos64, V * sizeof(double4), hipMemcpyDeviceToHost, m->stream));
     GM_HIP(hipStreamSynchronize(m->stream));
     for(size_t v = 0; v < V; ++v)
-        for(int k = 0; k < 3; ++k) pos[3 * v + k] = p64[4 * v + k];
+        for(int k = 0; k < 3; ++k) pos[3 * v + k] = p64[4 * (size_t)m->slotOf[v] + k];
     return GM_OK;
 }
 
@@ -263,13 +263,14 @@ int gm_mesh_set_positions(gm_mesh* m, const double* pos)
     std::vector<float> p32(4 * V);
     for(size_t v = 0; v < V; ++v)
     {
+        const size_t s = m->slotOf[v];
         for(int k = 0; k < 3; ++k)
         {
-            p64[4 * v + k] = pos[3 * v + k];
-            p32[4 * v + k] = (float)pos[3 * v + k];
+            p64[4 * s + k] = pos[3 * v + k];
+            p32[4 * s + k] = (float)pos[3 * v + k];
         }
-        p64[4 * v + 3] = 0.0;
-        p32[4 * v + 3] = 0.0f;
+        p64[4 * s + 3] = 0.0;
+        p32[4 * s + 3] = 0.0f;
     }
     if(V)
     {
@@ -408,8 +409,9 @@ int gm_mesh_smooth_groups(gm_mesh* m, int smoother, int64_t groupBegin, int64_t 
     if((rc = gmr::sweepColours(m->plan, m->crew, m->evaluator, m->sampler, r, groupBegin, groupEnd))) return rc;
     if(evalCounts && m->topo->nverts)
     {
-        GM_HIP(hipMemcpyAsync(evalCounts, m->dm.evalCounts, (size_t)m->topo->nverts * 4, hipMemcpyDeviceToHost,
-                              m->stream));
+        const uint32_t V = (uint32_t)m->topo->nverts;
+        GM_HIP(gmd::launchGatherCounts(m->dm.evalCounts, m->dSlotOf, m->dCountsRef, V, m->stream));
+        GM_HIP(hipMemcpyAsync(evalCounts, m->dCountsRef, (size_t)V * 4, hipMemcpyDeviceToHost, m->stream));
         GM_HIP(hipStreamSynchronize(m->stream));
     }
     return GM_OK;

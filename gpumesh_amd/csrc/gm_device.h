@@ -188,6 +188,10 @@ hipError_t launchEvaluate(const DevMesh* meshes, BlockWork blocks, const uint32_
 // Position precision conversions over blocks of VERT_BLOCK vertices.
 const int VERT_BLOCK = 256;
 hipError_t launchToF32(const DevMesh* meshes, BlockWork vertexBlocks, hipStream_t stream);
+// out[v] = counts[slotOf[v]], v < n (per-vertex values from the device layout
+// to the reference numbering).
+hipError_t launchGatherCounts(const int32_t* counts, const uint32_t* slotOf, int32_t* out, uint32_t n,
+                              hipStream_t stream);
 hipError_t launchToF64(const DevMesh* meshes, BlockWork vertexBlocks, hipStream_t stream);
 // Zeroes evalCounts and sets passSlot = 0 for the meshes of a batch (fill
 // kernels, so that the reset can be captured in a graph).

@@ -61,6 +61,14 @@ struct gm_mesh
     hipStream_t stream = nullptr;
     std::shared_ptr<const gmt::Topology> topo;   // shared by meshes of equal connectivity
     std::vector<double> hostPos;  // renumbered input positions, until uploaded
+    // Device slot of each vertex (reference numbering -> slot): its input id.
+    // Per-vertex device arrays are laid out by slot, so vertices that are
+    // close in the input numbering (mesh generators and files number them
+    // with spatial coherence) are close in memory; the reference numbering
+    // scatters each colour group over the whole mesh.
+    std::vector<uint32_t> slotOf;
+    uint32_t* dSlotOf = nullptr;     // device copy of slotOf
+    int32_t* dCountsRef = nullptr;   // evaluation counts in the reference numbering
     gmd::DevMesh dm{};         // host copy of the device descriptor (owns its pointers)
     int kind = gmd::EK_TET;    // element kind of the whole mesh
     int maxPatch = 0;

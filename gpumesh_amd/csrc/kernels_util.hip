@@ -47,6 +47,22 @@ __global__ void resetSlotsKernel(const DevMesh* __restrict__ meshes, const uint3
     for(int b = 0; b < 20; ++b) m.evalHist[b] = 0;
 }
 
+__global__ void __launch_bounds__(VERT_BLOCK) gatherCountsKernel(const int32_t* __restrict__ counts,
+                                                                 const uint32_t* __restrict__ slotOf,
+                                                                 int32_t* __restrict__ out, uint32_t n)
+{
+    const uint32_t v = blockIdx.x * VERT_BLOCK + threadIdx.x;
+    if(v < n) out[v] = counts[slotOf[v]];
+}
+
+hipError_t launchGatherCounts(const int32_t* counts, const uint32_t* slotOf, int32_t* out, uint32_t n,
+                              hipStream_t stream)
+{
+    if(n == 0) return hipSuccess;
+    gatherCountsKernel<<<(n + VERT_BLOCK - 1) / VERT_BLOCK, VERT_BLOCK, 0, stream>>>(counts, slotOf, out, n);
+    return hipGetLastError();
+}
+
 hipError_t launchToF32(const DevMesh* meshes, BlockWork vb, hipStream_t stream)
 {
     if(vb.count == 0) return hipSuccess;

@@ -195,7 +195,7 @@ size_t residentBytes(const gm_mesh* m, int records, bool getme)
     size_t b = allocBytes<double4>(V) + allocBytes<float4>(V) + allocBytes<uint4>((size_t)t.ntets) +
                allocBytes<uint4>(2 * (size_t)t.nhexs) + 2 * allocBytes<uint32_t>(V + 1) +
                allocBytes<uint32_t>(t.eref.size()) + allocBytes<uint32_t>(t.nbr.size()) + allocBytes<uint8_t>(V) +
-               allocBytes<uint32_t>(t.groupNodes.size()) + allocBytes<int32_t>(V) +
+               allocBytes<uint32_t>(t.groupNodes.size()) + 2 * allocBytes<int32_t>(V) + allocBytes<uint32_t>(V) +
                allocBytes<double>(3 * (size_t)eb) + allocBytes<int32_t>(eb) + allocBytes<int32_t>(20) +
                allocBytes<int32_t>(2);
     if(records > 0) b += allocBytes<double>(3 * (size_t)records) + allocBytes<int32_t>(20 * (size_t)records);
@@ -212,12 +212,14 @@ static void freeMeshBuffers(gm_mesh* m)
     void* ptrs[] = {d.pos64, d.pos32, (void*)d.tets, (void*)d.hexs, (void*)d.eoff, (void*)d.eref,
                     (void*)d.noff, (void*)d.nbr, (void*)d.fixed, (void*)d.groupNodes, d.evalCounts, d.getmeTet,
                     d.getmeHex, d.getmeQual, d.evalPartials, d.evalFlags, d.evalHist, d.evalResult, d.evalHistOut, d.passSlot,
-                    m->elemQualBuf, m->texels};
+                    m->elemQualBuf, m->texels, m->dSlotOf, m->dCountsRef};
     for(void* p : ptrs)
         poolFree(p);
     d = gmd::DevMesh{};
     m->elemQualBuf = nullptr;
     m->texels = nullptr;
+    m->dSlotOf = nullptr;
+    m->dCountsRef = nullptr;
     m->resultCap = 0;
     m->getmeReady = false;
 }
@@ -277,17 +279,40 @@ int uploadMesh(gm_mesh* m)
     d.ntets = (uint32_t)t.ntets;
     d.nhexs = (uint32_t)t.nhexs;
     d.evalBlocks = gmd::evalBlocksFor((uint64_t)t.ntets + (uint64_t)t.nhexs);
+    // Slot s holds the vertex numbered t.permutation[s] (input id s).
+    m->slotOf.assign(V, 0);
+    for(size_t s_ = 0; s_ < V; ++s_) m->slotOf[(size_t)t.permutation[s_]] = (uint32_t)s_;
+    const std::vector<uint32_t>& slot = m->slotOf;
     std::vector<double> p64(4 * V);
     std::vector<float> p32(4 * V);
-    for(size_t v = 0; v < V; ++v)
+    std::vector<uint32_t> eoD(V + 1, 0), noD(V + 1, 0), erD(t.eref.size()), nbD(t.nbr.size());
+    std::vector<uint8_t> fxD(V);
+    for(size_t s_ = 0; s_ < V; ++s_)
     {
+        const size_t v = (size_t)t.permutation[s_];
         for(int k = 0; k < 3; ++k)
         {
-            p64[4 * v + k] = m->hostPos[3 * v + k];
-            p32[4 * v + k] = (float)m->hostPos[3 * v + k];
+            p64[4 * s_ + k] = m->hostPos[3 * v + k];
+            p32[4 * s_ + k] = (float)m->hostPos[3 * v + k];
         }
-        p64[4 * v + 3] = 0.0;
-        p32[4 * v + 3] = 0.0f;
+        p64[4 * s_ + 3] = 0.0;
+        p32[4 * s_ + 3] = 0.0f;
+        fxD[s_] = t.fixed[v];
+        // The patch's element order and the neighbour order (the order of
+        // the reference's sums) are kept.
+        eoD[s_ + 1] = eoD[s_] + (t.eoff[v + 1] - t.eoff[v]);
+        std::copy(t.eref.begin() + t.eoff[v], t.eref.begin() + t.eoff[v + 1], erD.begin() + eoD[s_]);
+        noD[s_ + 1] = noD[s_] + (t.noff[v + 1] - t.noff[v]);
+        for(uint32_t r = t.noff[v]; r < t.noff[v + 1]; ++r) nbD[noD[s_] + (r - t.noff[v])] = slot[t.nbr[r]];
+    }
+    std::vector<uint32_t> tD(t.tets.size()), hD(t.hexs.size()), gnD(t.groupNodes.size());
+    for(size_t k = 0; k < t.tets.size(); ++k) tD[k] = slot[t.tets[k]];
+    for(size_t k = 0; k < t.hexs.size(); ++k) hD[k] = slot[t.hexs[k]];
+    // Each colour group in slot order: consecutive work items are close.
+    for(int64_t g = 0; g + 1 < (int64_t)t.groupOff.size(); ++g)
+    {
+        for(int64_t r = t.groupOff[g]; r < t.groupOff[g + 1]; ++r) gnD[r] = slot[t.groupNodes[r]];
+        std::sort(gnD.begin() + t.groupOff[g], gnD.begin() + t.groupOff[g + 1]);
     }
     uint4 *tb, *hb;
     uint32_t *eo, *er, *no, *nb, *gn;
@@ -303,6 +328,8 @@ int uploadMesh(gm_mesh* m)
     GM_TRY(devAlloc(&fx, V));                      d.fixed = fx;
     GM_TRY(devAlloc(&gn, t.groupNodes.size()));    d.groupNodes = gn;
     GM_TRY(devAlloc(&d.evalCounts, V));
+    GM_TRY(devAlloc(&m->dSlotOf, V));
+    GM_TRY(devAlloc(&m->dCountsRef, V));
     GM_TRY(devAlloc(&d.evalPartials, 3 * (size_t)d.evalBlocks));
     GM_TRY(devAlloc(&d.evalFlags, (size_t)d.evalBlocks));
     GM_TRY(devAlloc(&d.evalHist, 20));
@@ -313,16 +340,17 @@ int uploadMesh(gm_mesh* m)
     {
         GM_TRY(hipMemcpyAsync(d.pos64, p64.data(), V * sizeof(double4), hipMemcpyHostToDevice, s));
         GM_TRY(hipMemcpyAsync(d.pos32, p32.data(), V * sizeof(float4), hipMemcpyHostToDevice, s));
-        GM_TRY(hipMemcpyAsync(eo, t.eoff.data(), (V + 1) * 4, hipMemcpyHostToDevice, s));
-        GM_TRY(hipMemcpyAsync(no, t.noff.data(), (V + 1) * 4, hipMemcpyHostToDevice, s));
-        GM_TRY(hipMemcpyAsync(fx, t.fixed.data(), V, hipMemcpyHostToDevice, s));
+        GM_TRY(hipMemcpyAsync(eo, eoD.data(), (V + 1) * 4, hipMemcpyHostToDevice, s));
+        GM_TRY(hipMemcpyAsync(no, noD.data(), (V + 1) * 4, hipMemcpyHostToDevice, s));
+        GM_TRY(hipMemcpyAsync(fx, fxD.data(), V, hipMemcpyHostToDevice, s));
+        GM_TRY(hipMemcpyAsync(m->dSlotOf, slot.data(), V * 4, hipMemcpyHostToDevice, s));
     }
-    if(t.ntets) GM_TRY(hipMemcpyAsync(tb, t.tets.data(), (size_t)t.ntets * 16, hipMemcpyHostToDevice, s));
-    if(t.nhexs) GM_TRY(hipMemcpyAsync(hb, t.hexs.data(), (size_t)t.nhexs * 32, hipMemcpyHostToDevice, s));
-    if(!t.eref.empty()) GM_TRY(hipMemcpyAsync(er, t.eref.data(), t.eref.size() * 4, hipMemcpyHostToDevice, s));
-    if(!t.nbr.empty()) GM_TRY(hipMemcpyAsync(nb, t.nbr.data(), t.nbr.size() * 4, hipMemcpyHostToDevice, s));
-    if(!t.groupNodes.empty())
-        GM_TRY(hipMemcpyAsync(gn, t.groupNodes.data(), t.groupNodes.size() * 4, hipMemcpyHostToDevice, s));
+    if(t.ntets) GM_TRY(hipMemcpyAsync(tb, tD.data(), (size_t)t.ntets * 16, hipMemcpyHostToDevice, s));
+    if(t.nhexs) GM_TRY(hipMemcpyAsync(hb, hD.data(), (size_t)t.nhexs * 32, hipMemcpyHostToDevice, s));
+    if(!erD.empty()) GM_TRY(hipMemcpyAsync(er, erD.data(), erD.size() * 4, hipMemcpyHostToDevice, s));
+    if(!nbD.empty()) GM_TRY(hipMemcpyAsync(nb, nbD.data(), nbD.size() * 4, hipMemcpyHostToDevice, s));
+    if(!gnD.empty())
+        GM_TRY(hipMemcpyAsync(gn, gnD.data(), gnD.size() * 4, hipMemcpyHostToDevice, s));
     GM_TRY(hipStreamSynchronize(s));
 #undef GM_TRY
     std::vector<gm_mesh*> one{m};
@@ -789,8 +817,12 @@ int fetchRecords(Plan* p, size_t i, int records, double* stats, int32_t* hist, i
     GMR_HIP(hipMemcpyAsync(h.data(), m->dm.evalHistOut, h.size() * sizeof(int32_t), hipMemcpyDeviceToHost,
                            p->stream));
     if(evalCounts && m->topo->nverts)
-        GMR_HIP(hipMemcpyAsync(evalCounts, m->dm.evalCounts, (size_t)m->topo->nverts * sizeof(int32_t),
-                               hipMemcpyDeviceToHost, p->stream));
+    {
+        const uint32_t V = (uint32_t)m->topo->nverts;
+        GMR_HIP(gmd::launchGatherCounts(m->dm.evalCounts, m->dSlotOf, m->dCountsRef, V, p->stream));
+        GMR_HIP(hipMemcpyAsync(evalCounts, m->dCountsRef, (size_t)V * sizeof(int32_t), hipMemcpyDeviceToHost,
+                               p->stream));
+    }
     GMR_HIP(hipStreamSynchronize(p->stream));
     for(int k = 0; k < records; ++k)
     {

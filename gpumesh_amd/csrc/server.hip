@@ -259,7 +259,7 @@ void retireJob(gm_server* s, Job* j)
         j->outPos.resize(3 * V);
         for(size_t old = 0; old < V && !rc; ++old)
         {
-            const double4 q = p[t.permutation[old]];
+            const double4 q = p[m->slotOf[t.permutation[old]]];
             j->outPos[3 * old] = q.x;
             j->outPos[3 * old + 1] = q.y;
             j->outPos[3 * old + 2] = q.z;
