@@ -56,6 +56,18 @@ __device__ __forceinline__ uint32_t spanVertex(const DevMesh& m, const uint4& s,
     return r < s.w ? m.groupNodes[s.y + r] : NO_ITEM;
 }
 
+// The hardware hands workgroup b of a launch to XCD b mod 8, and each XCD
+// has its own L2.  Vertex launches take their work in this order instead:
+// XCD x runs a contiguous run of about 1/8 of the launch's work items, so
+// neighbouring vertices, whose patches share vertices and cache lines, meet
+// in one L2.  A bijection of [0, gridDim.x).
+__device__ __forceinline__ uint32_t xcdBlock()
+{
+    const uint32_t b = blockIdx.x, n = gridDim.x;
+    const uint32_t q = n >> 3, r = n & 7, x = b & 7, k = b >> 3;
+    return x * q + (x < r ? x : r) + k;
+}
+
 // Block b of a block launch: (mesh index, block index within the mesh).
 __device__ __forceinline__ uint2 blockItem(const uint4* spans, uint32_t nspans, uint32_t b)
 {
@@ -540,7 +552,7 @@ __global__ void __launch_bounds__(512) spawnLaneKernel(const DevMesh* __restrict
     unsigned char* base = smem + (size_t)wave * stageBytes<T, EK>(cap);
     T* sp = (T*)base;
     uint8_t* sk = base + (size_t)cap * Slots<EK>::N * 3 * sizeof(T);
-    const uint32_t i = blockIdx.x * ppg + wave;
+    const uint32_t i = xcdBlock() * ppg + wave;
     const uint4 span = waveSpan(segs, nsegs, i);
     const DevMesh& m = meshes[span.x];
     const uint32_t item = i < count && !runStopped(m) ? spanVertex(m, span, i) : NO_ITEM;
@@ -588,7 +600,7 @@ __global__ void __launch_bounds__(512) gdLaneKernel(const DevMesh* __restrict__ 
     unsigned char* mine = smem + (size_t)g * stageBytes<T, EK>(cap);
     T* mySp = (T*)mine;
     uint8_t* mySk = mine + (size_t)cap * SL * sizeof(T);
-    const uint32_t i = blockIdx.x * (blockDim.x >> 3) + g;
+    const uint32_t i = xcdBlock() * (blockDim.x >> 3) + g;
     const uint4 span = waveSpan(segs, nsegs, i);
     const DevMesh& m = meshes[span.x];
     const uint32_t item = i < count && !runStopped(m) ? spanVertex(m, span, i) : NO_ITEM;
@@ -1063,7 +1075,7 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(EK == 
     const size_t tileBytes = (size_t)cap * affineBytes<float, EK>();
     unsigned char* waveSp = smem + (size_t)wave * VPW * tileBytes;
     unsigned char* sp = waveSp + (size_t)g * tileBytes;
-    const uint32_t i = (blockIdx.x * ppg + wave) * VPW + g;
+    const uint32_t i = (xcdBlock() * ppg + wave) * VPW + g;
     const uint4 span = waveSpan(segs, nsegs, i);
     const DevMesh& m = meshes[span.x];
     const uint32_t item = i < count && !runStopped(m) ? spanVertex(m, span, i) : NO_ITEM;
@@ -1172,7 +1184,7 @@ __global__ void __launch_bounds__(512) spawnAffineKernel(const DevMesh* __restri
     const size_t tileBytes = (size_t)cap * affineBytes<double, EK>();
     unsigned char* waveSp = smem + (size_t)wave * VPW * tileBytes;
     const unsigned char* sp = waveSp + (size_t)g * tileBytes;
-    const uint32_t i = (blockIdx.x * ppg + wave) * VPW + g;
+    const uint32_t i = (xcdBlock() * ppg + wave) * VPW + g;
     const uint4 span = waveSpan(segs, nsegs, i);
     const DevMesh& m = meshes[span.x];
     const uint32_t item = i < count && !runStopped(m) ? spanVertex(m, span, i) : NO_ITEM;
@@ -1242,7 +1254,7 @@ __global__ void __launch_bounds__(512) gdAffineKernel(const DevMesh* __restrict_
     const int lane = threadIdx.x;
     const int g = lane >> 3, s = lane & 7, base = lane & ~7;
     unsigned char* mySp = smem + (size_t)g * cap * affineBytes<T, EK>();
-    const uint32_t i = blockIdx.x * (blockDim.x >> 3) + g;
+    const uint32_t i = xcdBlock() * (blockDim.x >> 3) + g;
     const uint4 span = waveSpan(segs, nsegs, i);
     const DevMesh& m = meshes[span.x];
     const uint32_t item = i < count && !runStopped(m) ? spanVertex(m, span, i) : NO_ITEM;
@@ -1328,7 +1340,7 @@ __global__ void __launch_bounds__(512) gdMfmaKernel(const DevMesh* __restrict__ 
     const int lane = threadIdx.x;
     const int g = lane >> 3, s = lane & 7, base = lane & ~7;
     unsigned char* mySp = smem + (size_t)g * cap * affineBytes<float, EK>();
-    const uint32_t i = blockIdx.x * (blockDim.x >> 3) + g;
+    const uint32_t i = xcdBlock() * (blockDim.x >> 3) + g;
     const uint4 span = waveSpan(segs, nsegs, i);
     const DevMesh& m = meshes[span.x];
     const uint32_t item = i < count && !runStopped(m) ? spanVertex(m, span, i) : NO_ITEM;
@@ -1539,7 +1551,7 @@ __global__ void __launch_bounds__(256) nmQuadKernel(const DevMesh* __restrict__ 
                                                     const uint4* __restrict__ segs, uint32_t nsegs, uint32_t count)
 {
     const int sub = threadIdx.x & 3;
-    const uint32_t i = blockIdx.x * (blockDim.x >> 2) + (threadIdx.x >> 2);
+    const uint32_t i = xcdBlock() * (blockDim.x >> 2) + (threadIdx.x >> 2);
     if(i >= count)
         return;
     const uint4 sp = waveSpan(segs, nsegs, i);
@@ -1881,7 +1893,7 @@ __global__ void __launch_bounds__(64 * GETME_WAVES) getmeWaveKernel(const DevMes
     __shared__ T pull[sizeof(T) == 8 ? 4 * 64 * GETME_WAVES : 1];
     const int lane = threadIdx.x % G;
     const int grp = threadIdx.x / G;
-    const uint32_t i = blockIdx.x * (64 * GETME_WAVES / G) + grp;
+    const uint32_t i = xcdBlock() * (64 * GETME_WAVES / G) + grp;
     if(i >= count)
         return;
     const uint4 sp = waveSpan(segs, nsegs, i);
