@@ -51,7 +51,6 @@ struct DevMesh
     const uint32_t* noff;     // V+1
     const uint32_t* nbr;
     const uint8_t* fixed;     // 1 = fixed vertex (never moved; snaps to itself)
-    const uint32_t* groupNodes; // movable vertices, colour group by colour group
     int32_t* evalCounts;      // patch evaluations per vertex (accumulated)
     void* getmeTet;           // GETMe (vpp, weight) per tet corner, sized for FP64
     void* getmeHex;           // same per hex corner

@@ -30,13 +30,15 @@ template<> struct PosOf<float>
     static float4* host(const DevMesh& m) { return m.pos32; }
 };
 
-// Work item i of a vertex launch: (mesh index, vertex id), vertex id NO_ITEM
-// for a hole.
+// Work item i of a vertex launch: (mesh index, vertex slot), slot NO_ITEM for
+// a hole.  A colour group occupies consecutive slots, and its span holds the
+// first one.
 __device__ __forceinline__ uint2 workItem(const DevMesh* meshes, const uint4* segs, uint32_t nsegs, uint32_t i)
 {
+    (void)meshes;
     const uint4 s = segs[spanOf(segs, nsegs, i)];
     const uint32_t r = i - s.z;
-    return make_uint2(s.x, r < s.w ? meshes[s.x].groupNodes[s.y + r] : NO_ITEM);
+    return make_uint2(s.x, r < s.w ? s.y + r : NO_ITEM);
 }
 
 // The same for a kernel whose wavefront's items lie in one ITEM_ALIGN window
@@ -52,8 +54,9 @@ __device__ __forceinline__ uint4 waveSpan(const uint4* segs, uint32_t nsegs, uin
 
 __device__ __forceinline__ uint32_t spanVertex(const DevMesh& m, const uint4& s, uint32_t i)
 {
+    (void)m;
     const uint32_t r = i - s.z;
-    return r < s.w ? m.groupNodes[s.y + r] : NO_ITEM;
+    return r < s.w ? s.y + r : NO_ITEM;
 }
 
 // The hardware hands workgroup b of a launch to XCD b mod 8, and each XCD

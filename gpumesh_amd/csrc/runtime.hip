@@ -195,7 +195,7 @@ size_t residentBytes(const gm_mesh* m, int records, bool getme)
     size_t b = allocBytes<double4>(V) + allocBytes<float4>(V) + allocBytes<uint4>((size_t)t.ntets) +
                allocBytes<uint4>(2 * (size_t)t.nhexs) + 2 * allocBytes<uint32_t>(V + 1) +
                allocBytes<uint32_t>(t.eref.size()) + allocBytes<uint32_t>(t.nbr.size()) + allocBytes<uint8_t>(V) +
-               allocBytes<uint32_t>(t.groupNodes.size()) + 2 * allocBytes<int32_t>(V) + allocBytes<uint32_t>(V) +
+               2 * allocBytes<int32_t>(V) + allocBytes<uint32_t>(V) +
                allocBytes<double>(3 * (size_t)eb) + allocBytes<int32_t>(eb) + allocBytes<int32_t>(20) +
                allocBytes<int32_t>(2);
     if(records > 0) b += allocBytes<double>(3 * (size_t)records) + allocBytes<int32_t>(20 * (size_t)records);
@@ -210,7 +210,7 @@ static void freeMeshBuffers(gm_mesh* m)
 {
     gmd::DevMesh& d = m->dm;
     void* ptrs[] = {d.pos64, d.pos32, (void*)d.tets, (void*)d.hexs, (void*)d.eoff, (void*)d.eref,
-                    (void*)d.noff, (void*)d.nbr, (void*)d.fixed, (void*)d.groupNodes, d.evalCounts, d.getmeTet,
+                    (void*)d.noff, (void*)d.nbr, (void*)d.fixed, d.evalCounts, d.getmeTet,
                     d.getmeHex, d.getmeQual, d.evalPartials, d.evalFlags, d.evalHist, d.evalResult, d.evalHistOut, d.passSlot,
                     m->elemQualBuf, m->texels, m->dSlotOf, m->dCountsRef};
     for(void* p : ptrs)
@@ -279,9 +279,28 @@ int uploadMesh(gm_mesh* m)
     d.ntets = (uint32_t)t.ntets;
     d.nhexs = (uint32_t)t.nhexs;
     d.evalBlocks = gmd::evalBlocksFor((uint64_t)t.ntets + (uint64_t)t.nhexs);
-    // Slot s holds the vertex numbered t.permutation[s] (input id s).
+    // Slots: the fixed vertices in input order, then colour group by colour
+    // group, each group in input order.  slotOf: reference number -> slot;
+    // vertexAt: slot -> reference number.
+    std::vector<uint32_t> vertexAt(V);
+    {
+        std::vector<uint32_t> inputOf(V);   // reference number -> input id
+        for(size_t o = 0; o < V; ++o) inputOf[(size_t)t.permutation[o]] = (uint32_t)o;
+        size_t n = 0;
+        for(size_t o = 0; o < V; ++o)
+            if(t.fixed[(size_t)t.permutation[o]]) vertexAt[n++] = (uint32_t)t.permutation[o];
+        if(n != (size_t)t.nfixed) return bail(fail(GM_ERR_ARG, "fixed vertex count mismatch"));
+        for(int64_t g = 0; g + 1 < (int64_t)t.groupOff.size(); ++g)
+        {
+            const size_t b0 = n;
+            for(int64_t r = t.groupOff[g]; r < t.groupOff[g + 1]; ++r) vertexAt[n++] = t.groupNodes[r];
+            std::sort(vertexAt.begin() + b0, vertexAt.begin() + n,
+                      [&](uint32_t a, uint32_t b) { return inputOf[a] < inputOf[b]; });
+        }
+        if(n != V) return bail(fail(GM_ERR_ARG, "colour groups do not cover the movable vertices"));
+    }
     m->slotOf.assign(V, 0);
-    for(size_t s_ = 0; s_ < V; ++s_) m->slotOf[(size_t)t.permutation[s_]] = (uint32_t)s_;
+    for(size_t s_ = 0; s_ < V; ++s_) m->slotOf[vertexAt[s_]] = (uint32_t)s_;
     const std::vector<uint32_t>& slot = m->slotOf;
     std::vector<double> p64(4 * V);
     std::vector<float> p32(4 * V);
@@ -289,7 +308,7 @@ int uploadMesh(gm_mesh* m)
     std::vector<uint8_t> fxD(V);
     for(size_t s_ = 0; s_ < V; ++s_)
     {
-        const size_t v = (size_t)t.permutation[s_];
+        const size_t v = vertexAt[s_];
         for(int k = 0; k < 3; ++k)
         {
             p64[4 * s_ + k] = m->hostPos[3 * v + k];
@@ -305,17 +324,11 @@ int uploadMesh(gm_mesh* m)
         noD[s_ + 1] = noD[s_] + (t.noff[v + 1] - t.noff[v]);
         for(uint32_t r = t.noff[v]; r < t.noff[v + 1]; ++r) nbD[noD[s_] + (r - t.noff[v])] = slot[t.nbr[r]];
     }
-    std::vector<uint32_t> tD(t.tets.size()), hD(t.hexs.size()), gnD(t.groupNodes.size());
+    std::vector<uint32_t> tD(t.tets.size()), hD(t.hexs.size());
     for(size_t k = 0; k < t.tets.size(); ++k) tD[k] = slot[t.tets[k]];
     for(size_t k = 0; k < t.hexs.size(); ++k) hD[k] = slot[t.hexs[k]];
-    // Each colour group in slot order: consecutive work items are close.
-    for(int64_t g = 0; g + 1 < (int64_t)t.groupOff.size(); ++g)
-    {
-        for(int64_t r = t.groupOff[g]; r < t.groupOff[g + 1]; ++r) gnD[r] = slot[t.groupNodes[r]];
-        std::sort(gnD.begin() + t.groupOff[g], gnD.begin() + t.groupOff[g + 1]);
-    }
     uint4 *tb, *hb;
-    uint32_t *eo, *er, *no, *nb, *gn;
+    uint32_t *eo, *er, *no, *nb;
     uint8_t* fx;
     GM_TRY(devAlloc(&d.pos64, V));
     GM_TRY(devAlloc(&d.pos32, V));
@@ -326,7 +339,6 @@ int uploadMesh(gm_mesh* m)
     GM_TRY(devAlloc(&no, V + 1));                  d.noff = no;
     GM_TRY(devAlloc(&nb, t.nbr.size()));           d.nbr = nb;
     GM_TRY(devAlloc(&fx, V));                      d.fixed = fx;
-    GM_TRY(devAlloc(&gn, t.groupNodes.size()));    d.groupNodes = gn;
     GM_TRY(devAlloc(&d.evalCounts, V));
     GM_TRY(devAlloc(&m->dSlotOf, V));
     GM_TRY(devAlloc(&m->dCountsRef, V));
@@ -349,8 +361,6 @@ int uploadMesh(gm_mesh* m)
     if(t.nhexs) GM_TRY(hipMemcpyAsync(hb, hD.data(), (size_t)t.nhexs * 32, hipMemcpyHostToDevice, s));
     if(!erD.empty()) GM_TRY(hipMemcpyAsync(er, erD.data(), erD.size() * 4, hipMemcpyHostToDevice, s));
     if(!nbD.empty()) GM_TRY(hipMemcpyAsync(nb, nbD.data(), nbD.size() * 4, hipMemcpyHostToDevice, s));
-    if(!gnD.empty())
-        GM_TRY(hipMemcpyAsync(gn, gnD.data(), gnD.size() * 4, hipMemcpyHostToDevice, s));
     GM_TRY(hipStreamSynchronize(s));
 #undef GM_TRY
     std::vector<gm_mesh*> one{m};
@@ -427,7 +437,10 @@ static int buildPlanInto(Plan* p, const std::vector<gm_mesh*>& meshes)
                     // items of a wavefront never span two members).
                     const uint64_t z = ((uint64_t)l.count + gmd::ITEM_ALIGN - 1) / gmd::ITEM_ALIGN * gmd::ITEM_ALIGN;
                     if(z + n >= (1ull << 32)) return fail(GM_ERR_ARG, "colour step too large");
-                    spans.push_back(make_uint4((uint32_t)i, (uint32_t)t.groupOff[g], (uint32_t)z, (uint32_t)n));
+                    // The group's first slot (fixed vertices first, then the
+                    // colour groups in order, uploadMesh).
+                    spans.push_back(make_uint4((uint32_t)i, (uint32_t)(t.nfixed + t.groupOff[g]), (uint32_t)z,
+                                               (uint32_t)n));
                     l.count = (uint32_t)(z + n);
                     l.maxPatch = std::max(l.maxPatch, (int)t.groupMaxPatch[g]);
                 }
