@@ -185,7 +185,7 @@ def serving_leg(args):
     out = {"workload": f"{len(jobs)} jobs per rate: tet/hex 50/50 cubes, n log-uniform [8,100], jitter U[0.2,0.4]h, "
                        f"5-20 passes, NM/GD/GETMe uniform, fp32 + fp64 polish, Poisson arrivals",
            "jobs": len(jobs), "vertex_passes_per_run": work, "rates": {}}
-    with GpuServer(budget_bytes=288 * 10**9, worker_threads=8) as srv:
+    with GpuServer(budget_bytes=288 * 10**9, worker_threads=args.serve_workers) as srv:
         # Untimed warm-up jobs (first launches of each kernel).
         warm = [srv.submit_mesh(meshes[0], sm, 1) for sm in ("nm", "gd", "getme")]
         for w in warm:
@@ -253,6 +253,7 @@ def main():
     ap.add_argument("--serve-jobs", type=int, default=48)
     ap.add_argument("--serve-rates", default="50,200,1000,2000")
     ap.add_argument("--serve-plain-sample", type=int, default=8)
+    ap.add_argument("--serve-workers", type=int, default=8, help="host worker threads of the server")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-only", action="store_true", help="run only the serving leg and print its JSON")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",

@@ -92,7 +92,21 @@ bool build(int64_t V, const double* pos, const int32_t* fixed,
         {
             uint32_t vId = nextNodes[i];
             uint64_t token = (uint64_t)vId + 1;
-            for(uint32_t r = eoff[vId]; r < eoff[vId + 1]; ++r)
+            if(H == 0)
+            {
+                // Tetrahedra only: the element-mates are the edge neighbours,
+                // and the neighbour list holds them in the order the element
+                // walk below meets them (elements in patch order, a tet's
+                // other vertices in local order), without repeats.
+                const uint32_t* ln = &ntmp[(size_t)3 * eoff[vId]];
+                for(uint32_t n = 0; n < ncnt[vId]; ++n)
+                {
+                    int& g = groups[ln[n]];
+                    if(g == NO_GROUP) { g = UNSET_GROUP; nextNodes.push_back(ln[n]); }
+                    else if(g != UNSET_GROUP) usedBy[g] = token;
+                }
+            }
+            else for(uint32_t r = eoff[vId]; r < eoff[vId + 1]; ++r)
             {
                 uint32_t ref = eref[r];
                 uint32_t id = ref >> 4;
@@ -115,18 +129,23 @@ bool build(int64_t V, const double* pos, const int32_t* fixed,
 
     // ---- Ordering (determinePositions, NodeGroups.cpp:365-450): the same
     // comparator through std::sort, so ties resolve identically.
+    // The comparator's outcomes as one key per vertex (all fixed vertices
+    // tie; movable ones order by group, then patch size); std::sort's moves
+    // depend only on the outcomes, so sorting (key, id) pairs gives the same
+    // order as sorting ids with the comparator.
+    std::vector<std::pair<uint64_t, int>> keyed(V);
+    for(int64_t v = 0; v < V; ++v)
+    {
+        const uint64_t key = types[v] == FIXED_TYPE
+                                 ? 0
+                                 : ((uint64_t)1 << 62) | ((uint64_t)(uint32_t)groups[v] << 32) |
+                                       (uint64_t)(eoff[v + 1] - eoff[v]);
+        keyed[v] = std::make_pair(key, (int)v);
+    }
+    std::sort(keyed.begin(), keyed.end(),
+              [](const std::pair<uint64_t, int>& a, const std::pair<uint64_t, int>& b) { return a.first < b.first; });
     std::vector<int> indices(V);
-    std::iota(indices.begin(), indices.end(), 0);
-    std::sort(indices.begin(), indices.end(), [&](int a, int b) {
-        if(types[a] == types[b])
-        {
-            if(types[a] == FIXED_TYPE) return false;
-            if(groups[a] == groups[b])
-                return (eoff[a + 1] - eoff[a]) < (eoff[b + 1] - eoff[b]);
-            return groups[a] < groups[b];
-        }
-        return types[a] < types[b];
-    });
+    for(int64_t p = 0; p < V; ++p) indices[p] = keyed[p].second;
     out.permutation.assign(V, 0);
     for(int64_t p = 0; p < V; ++p) out.permutation[indices[p]] = p;
     const std::vector<int64_t>& P = out.permutation;

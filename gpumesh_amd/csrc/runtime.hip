@@ -284,20 +284,23 @@ int uploadMesh(gm_mesh* m)
     // vertexAt: slot -> reference number.
     std::vector<uint32_t> vertexAt(V);
     {
-        std::vector<uint32_t> inputOf(V);   // reference number -> input id
-        for(size_t o = 0; o < V; ++o) inputOf[(size_t)t.permutation[o]] = (uint32_t)o;
-        size_t n = 0;
+        // The reference numbering is already fixed-then-group-major (group g
+        // holds ids nfixed + groupOff[g] ..); one pass over the input ids
+        // fills each segment in input order.
+        const size_t G = t.groupOff.empty() ? 0 : t.groupOff.size() - 1;
+        if((size_t)t.nfixed + (G ? (size_t)t.groupOff[G] : 0) != V)
+            return bail(fail(GM_ERR_ARG, "colour groups do not cover the movable vertices"));
+        std::vector<size_t> starts(G), next(G + 1);
+        next[0] = 0;
+        for(size_t g = 0; g < G; ++g) next[g + 1] = starts[g] = (size_t)t.nfixed + (size_t)t.groupOff[g];
         for(size_t o = 0; o < V; ++o)
-            if(t.fixed[(size_t)t.permutation[o]]) vertexAt[n++] = (uint32_t)t.permutation[o];
-        if(n != (size_t)t.nfixed) return bail(fail(GM_ERR_ARG, "fixed vertex count mismatch"));
-        for(int64_t g = 0; g + 1 < (int64_t)t.groupOff.size(); ++g)
         {
-            const size_t b0 = n;
-            for(int64_t r = t.groupOff[g]; r < t.groupOff[g + 1]; ++r) vertexAt[n++] = t.groupNodes[r];
-            std::sort(vertexAt.begin() + b0, vertexAt.begin() + n,
-                      [&](uint32_t a, uint32_t b) { return inputOf[a] < inputOf[b]; });
+            const size_t v = (size_t)t.permutation[o];
+            const size_t seg = v < (size_t)t.nfixed
+                                   ? 0
+                                   : (size_t)(std::upper_bound(starts.begin(), starts.end(), v) - starts.begin());
+            vertexAt[next[seg]++] = (uint32_t)v;
         }
-        if(n != V) return bail(fail(GM_ERR_ARG, "colour groups do not cover the movable vertices"));
     }
     m->slotOf.assign(V, 0);
     for(size_t s_ = 0; s_ < V; ++s_) m->slotOf[vertexAt[s_]] = (uint32_t)s_;

@@ -63,6 +63,19 @@ def test_shuffled_numbering_and_mixed_elements(oracle):
     _compare(oracle, pos, fixed, tets, hexs)
 
 
+def test_shuffled_tet_only(oracle):
+    """Tetrahedra only (the colouring walks neighbour lists): random vertex
+    ids, random element order, random fixed vertices."""
+    rng = np.random.default_rng(5)
+    t = cube_mesh(7, "tet", 0.2, seed=4)
+    perm = rng.permutation(t.nverts)
+    inv = np.empty_like(perm)
+    inv[perm] = np.arange(t.nverts)
+    fixed = (rng.uniform(size=t.nverts) < 0.15).astype(np.int32)
+    tets = inv[t.tets][rng.permutation(len(t.tets))].astype(np.uint32)
+    _compare(oracle, t.pos[perm], fixed, tets, np.zeros((0, 8), np.uint32))
+
+
 def test_disconnected_components(oracle):
     a = cube_mesh(3, "tet", 0.1, seed=2)
     b = cube_mesh(2, "hex", 0.1, seed=3)
