@@ -52,8 +52,8 @@ struct DevMesh
     const uint32_t* nbr;
     const uint8_t* fixed;     // 1 = fixed vertex (never moved; snaps to itself)
     int32_t* evalCounts;      // patch evaluations per vertex (accumulated)
-    void* getmeTet;           // GETMe (vpp, weight) per tet corner, sized for FP64
-    void* getmeHex;           // same per hex corner
+    void* getmePull;          // GETMe (vpp, weight) per patch entry (eref order), sized for FP64
+    const uint32_t* cornerSlot; // patch entry of each element corner: tets 4 e + k, then hexes 4 T + 8 h + k
     void* getmeQual;          // GETMe: current quality per element (tets, then hexes), sized for FP64
     // Quality reduction of this mesh.
     double* evalPartials;     // 3 per block: min, sum, compensation

@@ -1737,7 +1737,7 @@ __global__ void __launch_bounds__(ELEM_BLOCK) getmeTetKernel(const DevMesh* __re
         return;
     typedef typename PosOf<T>::type P;
     const P* pos = PosOf<T>::get(m);
-    P* out = (P*)m.getmeTet;
+    P* out = (P*)m.getmePull;
     const T lambda = T(0.78);
     uint4 t = m.tets[e];
     uint32_t vi[4] = {t.x, t.y, t.z, t.w};
@@ -1766,7 +1766,7 @@ __global__ void __launch_bounds__(ELEM_BLOCK) getmeTetKernel(const DevMesh* __re
     {
         P o;
         o.x = vpp[k].x; o.y = vpp[k].y; o.z = vpp[k].z; o.w = weight;
-        out[4 * (size_t)e + k] = o;
+        out[m.cornerSlot[4 * (size_t)e + k]] = o;
     }
 }
 
@@ -1782,7 +1782,7 @@ __global__ void __launch_bounds__(ELEM_BLOCK) getmeHexKernel(const DevMesh* __re
         return;
     typedef typename PosOf<T>::type P;
     const P* pos = PosOf<T>::get(m);
-    P* out = (P*)m.getmeHex;
+    P* out = (P*)m.getmePull;
     const T lambda = T(0.78);
     uint4 ha = m.hexs[2 * e], hb = m.hexs[2 * e + 1];
     uint32_t vi[8] = {ha.x, ha.y, ha.z, ha.w, hb.x, hb.y, hb.z, hb.w};
@@ -1821,7 +1821,7 @@ __global__ void __launch_bounds__(ELEM_BLOCK) getmeHexKernel(const DevMesh* __re
     {
         P o;
         o.x = vpp[k].x; o.y = vpp[k].y; o.z = vpp[k].z; o.w = weight;
-        out[8 * (size_t)e + k] = o;
+        out[m.cornerSlot[4 * (size_t)m.ntets + 8 * (size_t)e + k]] = o;
     }
 }
 
@@ -1844,18 +1844,14 @@ __global__ void __launch_bounds__(512) getmeVertexKernel(const DevMesh* __restri
         return;
     const DevMesh& m = meshes[it.x];
     P* pos = PosOf<T>::get(m);
-    const P* tetOut = (const P*)m.getmeTet;
-    const P* hexOut = (const P*)m.getmeHex;
+    const P* pull = (const P*)m.getmePull;
     const uint32_t v = it.y;
     v3<T> acc = mk<T>(T(0), T(0), T(0));
     T wacc = T(0);
     uint32_t b = m.eoff[v], e = m.eoff[v + 1];
     for(uint32_t r = b; r < e; ++r)
     {
-        uint32_t ref = m.eref[r];
-        uint32_t id = ref >> 4;
-        uint32_t k = ref & 7;
-        P o = (ref & 8) ? hexOut[8 * (size_t)id + k] : tetOut[4 * (size_t)id + k];
+        P o = pull[r];
         T w = (T)o.w;
         acc = acc + mk<T>((T)o.x, (T)o.y, (T)o.z) * w;
         wacc += w;
@@ -1905,8 +1901,7 @@ __global__ void __launch_bounds__(64 * GETME_WAVES) getmeWaveKernel(const DevMes
     if(v == NO_ITEM || runStopped(m))
         return;
     P* pos = PosOf<T>::get(m);
-    const P* tetOut = (const P*)m.getmeTet;
-    const P* hexOut = (const P*)m.getmeHex;
+    const P* pullBuf = (const P*)m.getmePull;
     WavePatch<T, EV, SA, G, EKIND> f;
     f.c = &c;
     f.red = red + 2 * G * grp;
@@ -1936,7 +1931,7 @@ __global__ void __launch_bounds__(64 * GETME_WAVES) getmeWaveKernel(const DevMes
             uint4 t = m.tets[id];
             f.vp[0] = ld<T>(pos, t.x); f.vp[1] = ld<T>(pos, t.y);
             f.vp[2] = ld<T>(pos, t.z); f.vp[3] = ld<T>(pos, t.w);
-            o = tetOut[4 * (size_t)id + f.k];
+            o = pullBuf[eb + lane];
         }
         else if constexpr(EKIND != EK_TET)
         {
@@ -1945,7 +1940,7 @@ __global__ void __launch_bounds__(64 * GETME_WAVES) getmeWaveKernel(const DevMes
             f.vp[2] = ld<T>(pos, a.z); f.vp[3] = ld<T>(pos, a.w);
             f.vp[4] = ld<T>(pos, b.x); f.vp[5] = ld<T>(pos, b.y);
             f.vp[6] = ld<T>(pos, b.z); f.vp[7] = ld<T>(pos, b.w);
-            o = hexOut[8 * (size_t)id + f.k];
+            o = pullBuf[eb + lane];
         }
         w = (T)o.w;
         pw = mk<T>((T)o.x, (T)o.y, (T)o.z) * w;

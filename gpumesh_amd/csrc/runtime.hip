@@ -200,8 +200,8 @@ size_t residentBytes(const gm_mesh* m, int records, bool getme)
                allocBytes<int32_t>(2);
     if(records > 0) b += allocBytes<double>(3 * (size_t)records) + allocBytes<int32_t>(20 * (size_t)records);
     if(getme)
-        b += sizeClass(std::max<size_t>(32, 4 * (size_t)t.ntets * sizeof(double4))) +
-             sizeClass(std::max<size_t>(32, 8 * (size_t)t.nhexs * sizeof(double4))) +
+        b += sizeClass(std::max<size_t>(32, t.eref.size() * sizeof(double4))) +
+             sizeClass(std::max<size_t>(32, std::max<size_t>(t.eref.size(), 1) * sizeof(uint32_t))) +
              sizeClass(std::max<size_t>(32, ((size_t)t.ntets + (size_t)t.nhexs) * sizeof(double)));
     return b;
 }
@@ -210,8 +210,8 @@ static void freeMeshBuffers(gm_mesh* m)
 {
     gmd::DevMesh& d = m->dm;
     void* ptrs[] = {d.pos64, d.pos32, (void*)d.tets, (void*)d.hexs, (void*)d.eoff, (void*)d.eref,
-                    (void*)d.noff, (void*)d.nbr, (void*)d.fixed, d.evalCounts, d.getmeTet,
-                    d.getmeHex, d.getmeQual, d.evalPartials, d.evalFlags, d.evalHist, d.evalResult, d.evalHistOut, d.passSlot,
+                    (void*)d.noff, (void*)d.nbr, (void*)d.fixed, d.evalCounts, d.getmePull,
+                    (void*)d.cornerSlot, d.getmeQual, d.evalPartials, d.evalFlags, d.evalHist, d.evalResult, d.evalHistOut, d.passSlot,
                     m->elemQualBuf, m->texels, m->dSlotOf, m->dCountsRef};
     for(void* p : ptrs)
         poolFree(p);
@@ -541,14 +541,35 @@ static int ensureRecords(gm_mesh* m, int records)
 static int ensureGetme(gm_mesh* m)
 {
     if(m->getmeReady) return GM_OK;
-    // (vpp, weight) per element corner, sized for FP64; FP32 runs use the
-    // front half of each buffer.
-    const size_t tetBytes = 4 * (size_t)m->topo->ntets * sizeof(double4);
-    const size_t hexBytes = 8 * (size_t)m->topo->nhexs * sizeof(double4);
-    const size_t qualBytes = ((size_t)m->topo->ntets + (size_t)m->topo->nhexs) * sizeof(double);
-    GMR_HIP(poolAlloc(&m->dm.getmeTet, std::max<size_t>(32, tetBytes)));
-    GMR_HIP(poolAlloc(&m->dm.getmeHex, std::max<size_t>(32, hexBytes)));
+    // (vpp, weight) per patch entry, sized for FP64 (FP32 runs use the front
+    // half), and the patch entry of every element corner: the element step
+    // writes each corner's result where the vertex step reads it, so a
+    // vertex pulls one contiguous run.
+    const gmt::Topology& t = *m->topo;
+    const size_t V = (size_t)t.nverts, N = t.eref.size();
+    const size_t pullBytes = N * sizeof(double4);
+    const size_t qualBytes = ((size_t)t.ntets + (size_t)t.nhexs) * sizeof(double);
+    std::vector<uint32_t> vertexAt(V);
+    for(size_t v = 0; v < V; ++v) vertexAt[m->slotOf[v]] = (uint32_t)v;
+    std::vector<uint32_t> cs(std::max<size_t>(N, 1), 0);
+    uint32_t at = 0;
+    for(size_t s = 0; s < V; ++s)
+    {
+        const size_t v = vertexAt[s];
+        for(uint32_t r = t.eoff[v]; r < t.eoff[v + 1]; ++r, ++at)
+        {
+            const uint32_t ref = t.eref[r];
+            const size_t id = ref >> 4, k = ref & 7;
+            cs[(ref & 8) ? 4 * (size_t)t.ntets + 8 * id + k : 4 * id + k] = at;
+        }
+    }
+    uint32_t* dcs = nullptr;
+    GMR_HIP(poolAlloc(&m->dm.getmePull, std::max<size_t>(32, pullBytes)));
+    GMR_HIP(poolAlloc((void**)&dcs, std::max<size_t>(32, cs.size() * 4)));
+    m->dm.cornerSlot = dcs;
     GMR_HIP(poolAlloc(&m->dm.getmeQual, std::max<size_t>(32, qualBytes)));
+    GMR_HIP(hipMemcpyAsync(dcs, cs.data(), cs.size() * 4, hipMemcpyHostToDevice, m->stream));
+    GMR_HIP(hipStreamSynchronize(m->stream));
     m->getmeReady = true;
     return GM_OK;
 }
