@@ -382,7 +382,7 @@ def main():
                 kname = "getmeTetKernel<float, 0, 0>"
             else:
                 flop_per_launch = (2 * FLOP_MR_TET_PATCH + FLOP_GETME_PULL) * gm.sum_patch / best["launches"] * f32_passes
-                kname = "getmeWaveKernel<float, 0, 0, 32>"
+                kname = "getmeWaveKernel<float, 0, 0, 32, 0>"
             achieved = flop_per_launch / avg_s / 1e12
             roofline = {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                         "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None, "kernel": kname,
