@@ -1432,6 +1432,177 @@ __global__ void __launch_bounds__(512) gdMfmaKernel(const DevMesh* __restrict__ 
     }
 }
 
+// The nine terms of a tetrahedron corner row that the candidate loop reads
+// (36 bytes in LDS instead of a whole 48-byte Corner).
+struct RowCoef
+{
+    float d0, gx, gy, gz, c0, hx, hy, hz, s;
+};
+
+// mfmaRowSum over RowCoef rows.
+__device__ __forceinline__ float coefRowSum(const RowCoef* rows, int nRows, v3<float> x, float xx)
+{
+    const int sub = threadIdx.x & 3;
+    float sum = 0.0f;
+    for(int r0 = 0; r0 < nRows; r0 += 4)
+    {
+        const RowCoef& k = rows[r0 + sub];
+        mfv4 d = {0.0f, 0.0f, 0.0f, 0.0f}, f = {0.0f, 0.0f, 0.0f, 0.0f};
+        d = mfma4x4(k.d0, 1.0f, d);
+        f = mfma4x4(k.c0, 1.0f, f);
+        d = mfma4x4(k.gx, x.x, d);
+        f = mfma4x4(k.hx, x.x, f);
+        d = mfma4x4(k.gy, x.y, d);
+        f = mfma4x4(k.hy, x.y, f);
+        d = mfma4x4(k.gz, x.z, d);
+        f = mfma4x4(k.hz, x.z, f);
+        f = mfma4x4(k.s, xx, f);
+#pragma unroll
+        for(int r = 0; r < 4; ++r) sum += f[r] * gma::powm23(d[r]);
+    }
+    return sum;
+}
+
+__device__ __forceinline__ gma::Corner<float> coefCorner(const RowCoef& r)
+{
+    gma::Corner<float> c{};
+    c.d0 = r.d0; c.gx = r.gx; c.gy = r.gy; c.gz = r.gz;
+    c.c0 = r.c0; c.hx = r.hx; c.hy = r.hy; c.hz = r.hz; c.s = r.s;
+    return c;
+}
+
+// Gradient descent on tetrahedral patches of at most 8 RPL8 elements: the
+// kernel of gdMfmaKernel with 36-byte LDS rows.  The rows' near-degenerate
+// thresholds and element references, which only the final inversion test
+// reads, stay in the registers of the lane that staged the row (lane s of a
+// vertex stages rows s, s + 8, ... and tests the same rows), so more
+// wavefronts fit in LDS.
+template<int RPL8>
+__global__ void __launch_bounds__(512) gdMfmaTetKernel(const DevMesh* __restrict__ meshes,
+                                                       const uint4* __restrict__ segs, uint32_t nsegs, uint32_t count,
+                                                       int cap)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int lane = threadIdx.x;
+    const int g = lane >> 3, s = lane & 7, base = lane & ~7;
+    RowCoef* rows = (RowCoef*)smem + (size_t)g * cap;
+    const uint32_t i = xcdBlock() * (blockDim.x >> 3) + g;
+    const uint4 span = waveSpan(segs, nsegs, i);
+    const DevMesh& m = meshes[span.x];
+    const uint32_t item = i < count && !runStopped(m) ? spanVertex(m, span, i) : NO_ITEM;
+    const bool present = item != NO_ITEM;
+    const uint32_t v = present ? item : 0;
+    float4* pos = m.pos32;
+    const uint32_t eb = m.eoff[v];
+    const int nE = present ? (int)(m.eoff[v + 1] - eb) : 0;
+    const v3<float> o = present ? ld<float>(pos, v) : mk<float>(0.0f, 0.0f, 0.0f);
+    float thrR[RPL8];
+    uint32_t refR[RPL8];
+#pragma unroll
+    for(int k = 0; k < RPL8; ++k)
+    {
+        const int e = s + 8 * k;
+        thrR[k] = 0.0f;
+        refR[k] = 0;
+        if(e < nE)
+        {
+            const uint32_t ref = m.eref[eb + e];
+            const uint4 t = m.tets[ref >> 4];
+            v3<float> u[4] = {ld<float>(pos, t.x) - o, ld<float>(pos, t.y) - o, ld<float>(pos, t.z) - o,
+                              ld<float>(pos, t.w) - o};
+            const gma::Corner<float> c = gma::tetCorner<float>(u, (int)(ref & 7), ref);
+            rows[e] = RowCoef{c.d0, c.gx, c.gy, c.gz, c.c0, c.hx, c.hy, c.hz, c.s};
+            thrR[k] = c.thr;
+            refR[k] = ref;
+        }
+    }
+    // Neutral rows (d = 1, f2 = 0) up to the wavefront's largest patch,
+    // rounded to 4 rows; the tile holds them.
+    int nEmax = nE;
+#pragma unroll
+    for(int off = 8; off < 64; off <<= 1)
+        nEmax = max(nEmax, __shfl_xor(nEmax, off, 64));
+    const int nPad = (nEmax + 3) & ~3;
+    for(int e = nE + s; e < nPad; e += 8)
+        rows[e] = RowCoef{1.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    __syncthreads();
+    // Patch quality at offset x: 3 n / sum, or the exact rule when a corner
+    // is not positive.
+    auto quality = [&](v3<float> x) -> float {
+        const float xx = gmq::dot(x, x);
+        const float sum = coefRowSum(rows, nPad, x, xx);
+        if(!gma::fastBad<float>(sum))
+            return 3.0f * (float)nE * __builtin_amdgcn_rcpf(sum);
+        gma::Acc<float> a;
+        for(int e = 0; e < nE; ++e) gma::addTet<float>(a, coefCorner(rows[e]), x, xx);
+        return a.finish();
+    };
+
+    v3<float> x = mk<float>(0.0f, 0.0f, 0.0f);
+    float nodeShift = localSizeGroup<8>(m, pos, v, o, s) * (1.0f / 75.0f);
+    const float originalNodeShift = nodeShift;
+    int evals = 0;
+    bool active = present;
+    const float off = s == 0 ? -0.25f : s == 1 ? 0.00f : s == 2 ? 0.10f : s == 3 ? 0.20f
+                    : s == 4 ? 0.40f : s == 5 ? 0.80f : s == 6 ? 1.20f : 1.60f;
+    for(int cyc = 0; cyc < 5; ++cyc)
+    {
+        if(__ballot(active) == 0)
+            break;
+        const float sx = s == 0 ? -nodeShift : (s == 1 ? nodeShift : 0.0f);
+        const float sy = s == 2 ? -nodeShift : (s == 3 ? nodeShift : 0.0f);
+        const float sz = s == 4 ? -nodeShift : (s == 5 ? nodeShift : 0.0f);
+        const float q = quality(x + mk<float>(sx, sy, sz));
+        if(active) evals += 6;
+        const float q0 = __shfl(q, base + 0, 64), q1 = __shfl(q, base + 1, 64), q2 = __shfl(q, base + 2, 64);
+        const float q3 = __shfl(q, base + 3, 64), q4 = __shfl(q, base + 4, 64), q5 = __shfl(q, base + 5, 64);
+        const v3<float> gq = mk<float>(q1 - q0, q3 - q2, q5 - q4);
+        const float gn = gmq::tsqrt(gmq::dot(gq, gq));
+        if(gn == 0.0f)
+            active = false;
+        const v3<float> shift = gq * (nodeShift / gn);
+        const v3<float> prop = x + shift * off;
+        const float qp = quality(prop);
+        const int best = groupArgmaxFast<8>(qp);
+        const v3<float> nx = shflv<float>(prop, (base & 63) + best);
+        const float bestOff = __shfl(off, (base & 63) + best, 64);
+        if(active)
+        {
+            evals += 8;
+            x = nx;
+            nodeShift *= gmq::tabs(bestOff);
+            if(nodeShift < originalNodeShift / 10.0f)
+                active = false;
+        }
+    }
+    if(!present)
+        return;
+    // FP64 inversion test of the move over this lane's rows (testedMove).
+    const v3<float> pf = o + x;
+    const DetFix<0> fix{&m, m.pos32};
+    int flags = 0;
+#pragma unroll
+    for(int k = 0; k < RPL8; ++k)
+    {
+        const int e = s + 8 * k;
+        if(e < nE)
+        {
+            gma::Corner<float> c = coefCorner(rows[e]);
+            c.thr = thrR[k];
+            c.ref = refR[k];
+            flags |= cornerInversion<0>(c, x, pf, fix);
+        }
+    }
+    const int first = (threadIdx.x & 63) & ~7;
+    const bool reject = ((__ballot((flags & INV_FLIP) != 0) >> first) & 0xFFull) != 0 &&
+                        ((__ballot((flags & INV_NONPOS32) != 0) >> first) & 0xFFull) == 0;
+    if(s == 0)
+    {
+        st(pos, v, reject ? o : pf);
+        m.evalCounts[v] += evals;
+    }
+}
+
 // Nelder-Mead on affine terms: one vertex per group of G lanes, lane j holding
 // element j's staged corner terms in registers; a patch evaluation is one
 // affine element evaluation per lane and a butterfly over the group.
@@ -2291,7 +2462,21 @@ static hipError_t launchLane(const VertexWork& w, const gmq::Crew& c, LaneShape 
             if(aff)
             {
                 if constexpr(sizeof(T) == 4)
+                {
+                    if constexpr(EK == EK_TET)
+                        if(cap <= 32)
+                        {
+                            // 36-byte rows; the threshold and reference of
+                            // lane s's rows s + 8 k, k < RPL8, in registers.
+                            const size_t tl = (size_t)8 * ppg * cap * sizeof(RowCoef);
+                            if(cap <= 8) gdMfmaTetKernel<1><<<blocks, 64 * ppg, tl, stream>>>(w.meshes, w.segs, w.nsegs, w.count, cap);
+                            else if(cap <= 16) gdMfmaTetKernel<2><<<blocks, 64 * ppg, tl, stream>>>(w.meshes, w.segs, w.nsegs, w.count, cap);
+                            else if(cap <= 24) gdMfmaTetKernel<3><<<blocks, 64 * ppg, tl, stream>>>(w.meshes, w.segs, w.nsegs, w.count, cap);
+                            else gdMfmaTetKernel<4><<<blocks, 64 * ppg, tl, stream>>>(w.meshes, w.segs, w.nsegs, w.count, cap);
+                            return hipGetLastError();
+                        }
                     gdMfmaKernel<EK><<<blocks, 64 * ppg, lds, stream>>>(w.meshes, w.segs, w.nsegs, w.count, cap);
+                }
                 else
                     gdAffineKernel<T, EK><<<blocks, 64 * ppg, lds, stream>>>(w.meshes, w.segs, w.nsegs, w.count,
                                                                             cap);
