@@ -62,10 +62,12 @@ struct alignas(16) Corner
 GMA float flog2(float x) { return __builtin_amdgcn_logf(x); }
 GMA float fexp2(float x) { return __builtin_amdgcn_exp2f(x); }
 GMA float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+GMA float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 #else
 GMA float flog2(float x) { return __builtin_log2f(x); }
 GMA float fexp2(float x) { return __builtin_exp2f(x); }
 GMA float frcp(float x) { return 1.0f / x; }
+GMA float fsqrt(float x) { return __builtin_sqrtf(x); }
 #endif
 
 // a^(-2/3) for a > 0.
@@ -113,8 +115,10 @@ GMA Corner<T> makeCorner(v3<T> c0, v3<T> c1, v3<T> c2, T r0, T r1, T r2)
     k.hy = T(2) * (r0 * c0.y + r1 * c1.y + r2 * c2.y);
     k.hz = T(2) * (r0 * c0.z + r1 * c1.z + r2 * c2.z);
     k.s = r0 * r0 + r1 * r1 + r2 * r2;
+    // The band edge is a heuristic: the hardware square root (within an ulp)
+    // is precise enough.
     const float c0f = (float)k.c0;
-    k.thr = NEAR_DEGENERATE * c0f * __builtin_sqrtf(c0f);
+    k.thr = NEAR_DEGENERATE * c0f * fsqrt(c0f);
     k.ref = 0;
     k.corner = 0;
     return k;
