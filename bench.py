@@ -63,7 +63,7 @@ FLOP_AFFINE_TET = 20.0
 # HBM bytes per launch of the dominant kernel, measured with rocprofv3 --pmc
 # FETCH_SIZE and WRITE_SIZE (each scaled by the same counter's reading for a
 # 1 GiB device copy: x2.0 and x1.0) on the 100^3 Kuhn cube.
-MEASURED_TRAFFIC = {("spawnMfmaKernel<0, 4>", 100): 149686492.0}
+MEASURED_TRAFFIC = {("spawnMfmaKernel<0, 4>", 100): 109167442.0}
 
 
 def log(msg):
@@ -358,7 +358,7 @@ def main():
             # counted FLOP to the FP32 share of the passes.
             flop_per_launch = best["f_smooth"] * (f32_passes / steps) / best["launches"]
             achieved = flop_per_launch / avg_s / 1e12
-            kname = {"spawn": "spawnMfmaKernel<0, 4>", "gd": "gdMfmaKernel<0>",
+            kname = {"spawn": "spawnMfmaKernel<0, 4>", "gd": "gdMfmaTetKernel<3>",
                      "nm": "nmQuadKernel<6>"}.get(best["smoother"], "smoothWaveKernel<float>")
             executed = flop_per_launch * FLOP_AFFINE_TET / FLOP_MR_TET_PATCH / avg_s / 1e12
             # The spawn-search and gradient-descent kernels put the corner
