@@ -1919,10 +1919,10 @@ __global__ void __launch_bounds__(ELEM_BLOCK) getmeTetKernel(const DevMesh* __re
                   gmq::cross(vp[1] - vp[3], vp[0] - vp[3]), gmq::cross(vp[1] - vp[0], vp[2] - vp[0])};
     v3<T> vpp[4];
 #pragma unroll
-    for(int k = 0; k < 4; ++k) vpp[k] = vp[k] + n[k] * (lambda / gmq::tsqrt(gmq::tsqrt(gmq::dot(n[k], n[k]))));
+    for(int k = 0; k < 4; ++k) vpp[k] = vp[k] + n[k] * gmq::tdiv(lambda, gmq::tsqrt(gmq::tsqrt(gmq::dot(n[k], n[k]))));
     T volume = gmq::tetVolume<T, SA>(c, vp);
     T volumePrime = gmq::tetVolume<T, SA>(c, vpp);
-    T volumeVar = gmq::tcbrt(gmq::tabs(volume / volumePrime));
+    T volumeVar = gmq::tcbrt(gmq::tabs(gmq::tdiv(volume, volumePrime)));
     v3<T> center = T(1.0 / 4.0) * (vp[0] + vp[1] + vp[2] + vp[3]);
 #pragma unroll
     for(int k = 0; k < 4; ++k) vpp[k] = center + volumeVar * (vpp[k] - center);
@@ -1930,7 +1930,7 @@ __global__ void __launch_bounds__(ELEM_BLOCK) getmeTetKernel(const DevMesh* __re
     for(int k = 0; k < 4; ++k) if(m.fixed[vi[k]]) vpp[k] = vp[k];
     T quality = gmq::tetQuality<T, EV, SA>(c, vp);
     T qualityPrime = gmq::tetQuality<T, EV, SA>(c, vpp);
-    T weight = qualityPrime / (T(1) + quality);
+    T weight = gmq::tdiv(qualityPrime, T(1) + quality);
     ((T*)m.getmeQual)[e] = quality;
 #pragma unroll
     for(int k = 0; k < 4; ++k)
@@ -1974,10 +1974,10 @@ __global__ void __launch_bounds__(ELEM_BLOCK) getmeHexKernel(const DevMesh* __re
                       (aux[5] + aux[2] + aux[3]) / t3, (aux[5] + aux[3] + aux[4]) / t3};
     v3<T> vpp[8];
 #pragma unroll
-    for(int k = 0; k < 8; ++k) vpp[k] = bases[k] + n[k] * (lambda / gmq::tsqrt(gmq::tsqrt(gmq::dot(n[k], n[k]))));
+    for(int k = 0; k < 8; ++k) vpp[k] = bases[k] + n[k] * gmq::tdiv(lambda, gmq::tsqrt(gmq::tsqrt(gmq::dot(n[k], n[k]))));
     T volume = gmq::hexVolume<T, SA>(c, vp);
     T volumePrime = gmq::hexVolume<T, SA>(c, vpp);
-    T volumeVar = gmq::tcbrt(gmq::tabs(volume / volumePrime));
+    T volumeVar = gmq::tcbrt(gmq::tabs(gmq::tdiv(volume, volumePrime)));
     v3<T> center = T(1.0 / 8.0) * (vp[0] + vp[1] + vp[2] + vp[3] + vp[4] + vp[5] + vp[6] + vp[7]);
 #pragma unroll
     for(int k = 0; k < 8; ++k) vpp[k] = center + volumeVar * (vpp[k] - center);
@@ -1985,7 +1985,7 @@ __global__ void __launch_bounds__(ELEM_BLOCK) getmeHexKernel(const DevMesh* __re
     for(int k = 0; k < 8; ++k) if(m.fixed[vi[k]]) vpp[k] = vp[k];
     T quality = gmq::hexQuality<T, EV, SA>(c, vp);
     T qualityPrime = gmq::hexQuality<T, EV, SA>(c, vpp);
-    T weight = qualityPrime / (T(1) + quality);
+    T weight = gmq::tdiv(qualityPrime, T(1) + quality);
     ((T*)m.getmeQual)[m.ntets + e] = quality;
 #pragma unroll
     for(int k = 0; k < 8; ++k)

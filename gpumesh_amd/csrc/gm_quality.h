@@ -45,7 +45,16 @@ GMQ double tabs(double x) { return __builtin_fabs(x); }
 GMQ float tabs(float x) { return __builtin_fabsf(x); }
 
 GMQ double tsqrt(double x) { return __builtin_sqrt(x); }
+GMQ double tdiv(double a, double b) { return a / b; }
+#if defined(__HIP_DEVICE_COMPILE__)
+// FP32 on the device: the hardware square root and reciprocal (about 1 ulp;
+// the FP32 modes are not bit-exact, FP64 keeps correctly rounded operations).
+GMQ float tsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+GMQ float tdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+#else
 GMQ float tsqrt(float x) { return __builtin_sqrtf(x); }
+GMQ float tdiv(float a, float b) { return a / b; }
+#endif
 GMQ double tpow23(double x) { return gm_pow23(x); }
 GMQ double tcbrt(double x) { return gm_cbrt_nonneg(x); }
 GMQ double texp(double x) { return gm_exp(x); }
@@ -311,7 +320,7 @@ template<typename T> GMQ T cornerQuality(const m3<T>& F)
             d = (T)det(Fd);
         }
     }
-    return tsign(d) * T(3) * tpow23(tabs(d)) / f2;
+    return tdiv(tsign(d) * T(3) * tpow23(tabs(d)), f2);
 }
 
 // MetricConformityEvaluator::specifiedMetric (:25-45): 5-point Keast rule.
