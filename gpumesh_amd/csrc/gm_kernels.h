@@ -1313,6 +1313,110 @@ __global__ void __launch_bounds__(512) gdAffineKernel(const DevMesh* __restrict_
     }
 }
 
+// Gradient descent in FP64 on tetrahedral patches (the polish pass): the
+// schedule of gdAffineKernel with 72-byte LDS rows (the nine terms a
+// candidate reads; FP64 moves need no inversion test, so nothing else).
+struct RowCoefD
+{
+    double d0, gx, gy, gz, c0, hx, hy, hz, s;
+};
+
+static __device__ __forceinline__ double coefPatchQuality(const RowCoefD* rows, int nE, v3<double> x)
+{
+    const double xx = gmq::dot(x, x);
+    double inv3 = 0.0;
+    for(int e = 0; e < nE; ++e)
+    {
+        const RowCoefD& r = rows[e];
+        gma::Corner<double> c{};
+        c.d0 = r.d0; c.gx = r.gx; c.gy = r.gy; c.gz = r.gz;
+        c.c0 = r.c0; c.hx = r.hx; c.hy = r.hy; c.hz = r.hz; c.s = r.s;
+        inv3 += gma::fastTet<double>(c, x, xx);
+    }
+    if(!gma::fastBad<double>(inv3))
+        return 3.0 * (double)nE / inv3;
+    gma::Acc<double> a;
+    for(int e = 0; e < nE; ++e)
+    {
+        const RowCoefD& r = rows[e];
+        gma::Corner<double> c{};
+        c.d0 = r.d0; c.gx = r.gx; c.gy = r.gy; c.gz = r.gz;
+        c.c0 = r.c0; c.hx = r.hx; c.hy = r.hy; c.hz = r.hz; c.s = r.s;
+        gma::addTet<double>(a, c, x, xx);
+    }
+    return a.finish();
+}
+
+static __global__ void __launch_bounds__(512) gdAffineTetKernel(const DevMesh* __restrict__ meshes,
+                                                         const uint4* __restrict__ segs, uint32_t nsegs,
+                                                         uint32_t count, int cap)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int lane = threadIdx.x;
+    const int g = lane >> 3, s = lane & 7, base = lane & ~7;
+    RowCoefD* rows = (RowCoefD*)smem + (size_t)g * cap;
+    const uint32_t i = xcdBlock() * (blockDim.x >> 3) + g;
+    const uint4 span = waveSpan(segs, nsegs, i);
+    const DevMesh& m = meshes[span.x];
+    const uint32_t item = i < count && !runStopped(m) ? spanVertex(m, span, i) : NO_ITEM;
+    const bool present = item != NO_ITEM;
+    const uint32_t v = present ? item : 0;
+    double4* pos = m.pos64;
+    const uint32_t eb = m.eoff[v];
+    const int nE = present ? (int)(m.eoff[v + 1] - eb) : 0;
+    const v3<double> o = present ? ld<double>(pos, v) : mk<double>(0.0, 0.0, 0.0);
+    for(int e = s; e < nE; e += 8)
+    {
+        const uint32_t ref = m.eref[eb + e];
+        const uint4 t = m.tets[ref >> 4];
+        v3<double> u[4] = {ld<double>(pos, t.x) - o, ld<double>(pos, t.y) - o, ld<double>(pos, t.z) - o,
+                           ld<double>(pos, t.w) - o};
+        const gma::Corner<double> c = gma::tetCorner<double>(u, (int)(ref & 7), ref);
+        rows[e] = RowCoefD{c.d0, c.gx, c.gy, c.gz, c.c0, c.hx, c.hy, c.hz, c.s};
+    }
+    __syncthreads();
+    if(!present)
+        return;
+
+    v3<double> x = mk<double>(0.0, 0.0, 0.0);
+    double nodeShift = localSizeGroup<8>(m, pos, v, o, s) * (1.0 / 75.0);
+    const double originalNodeShift = nodeShift;
+    int evals = 0;
+    for(int cyc = 0; cyc < 5; ++cyc)
+    {
+        const double sx = s == 0 ? -nodeShift : (s == 1 ? nodeShift : 0.0);
+        const double sy = s == 2 ? -nodeShift : (s == 3 ? nodeShift : 0.0);
+        const double sz = s == 4 ? -nodeShift : (s == 5 ? nodeShift : 0.0);
+        double q = 0.0;
+        if(s < 6)
+            q = coefPatchQuality(rows, nE, x + mk<double>(sx, sy, sz));
+        evals += 6;
+        const double q0 = __shfl(q, base + 0, 64), q1 = __shfl(q, base + 1, 64), q2 = __shfl(q, base + 2, 64);
+        const double q3 = __shfl(q, base + 3, 64), q4 = __shfl(q, base + 4, 64), q5 = __shfl(q, base + 5, 64);
+        const v3<double> gq = mk<double>(q1 - q0, q3 - q2, q5 - q4);
+        const double gn = gmq::tsqrt(gmq::dot(gq, gq));
+        if(gn == 0.0)
+            break;
+        const v3<double> shift = gq * (nodeShift / gn);
+        const double off = s == 0 ? -0.25 : s == 1 ? 0.00 : s == 2 ? 0.10 : s == 3 ? 0.20
+                         : s == 4 ? 0.40 : s == 5 ? 0.80 : s == 6 ? 1.20 : 1.60;
+        const v3<double> prop = x + shift * off;
+        const double qp = coefPatchQuality(rows, nE, prop);
+        evals += 8;
+        const int best = groupArgmax<double>(qp, s, 8);
+        x = shflv<double>(prop, (base & 63) + best);
+        const double bestOff = __shfl(off, (base & 63) + best, 64);
+        nodeShift *= gmq::tabs(bestOff);
+        if(nodeShift < originalNodeShift / 10.0)
+            break;
+    }
+    if(s == 0)
+    {
+        st(pos, v, o + x);
+        m.evalCounts[v] += evals;
+    }
+}
+
 // Patch quality of this lane's candidate from the FP32 affine rows on the
 // matrix cores (every lane of the wavefront takes part); the exact rule, on
 // the vector ALUs, for a lane whose patch has a non-positive element.
@@ -2478,8 +2582,17 @@ static hipError_t launchLane(const VertexWork& w, const gmq::Crew& c, LaneShape 
                     gdMfmaKernel<EK><<<blocks, 64 * ppg, lds, stream>>>(w.meshes, w.segs, w.nsegs, w.count, cap);
                 }
                 else
-                    gdAffineKernel<T, EK><<<blocks, 64 * ppg, lds, stream>>>(w.meshes, w.segs, w.nsegs, w.count,
-                                                                            cap);
+                {
+                    if constexpr(EK == EK_TET)
+                    {
+                        // 72-byte rows (the nine terms a candidate reads).
+                        const size_t tl = (size_t)8 * ppg * cap * sizeof(RowCoefD);
+                        gdAffineTetKernel<<<blocks, 64 * ppg, tl, stream>>>(w.meshes, w.segs, w.nsegs, w.count, cap);
+                    }
+                    else
+                        gdAffineKernel<T, EK><<<blocks, 64 * ppg, lds, stream>>>(w.meshes, w.segs, w.nsegs, w.count,
+                                                                                cap);
+                }
                 return hipGetLastError();
             }
         gdLaneKernel<T, EV, SA, EK><<<blocks, 64 * ppg, lds, stream>>>(w.meshes, w.segs, w.nsegs, w.count, c, cap);
