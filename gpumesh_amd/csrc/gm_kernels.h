@@ -841,6 +841,30 @@ __device__ __forceinline__ T affineSumFast(const unsigned char* sp, int nE, v3<T
     return inv3;
 }
 
+// affineSumFast at P offsets, each staged row read once.
+template<typename T, int EK, int P>
+__device__ __forceinline__ void affineSumsFast(const unsigned char* sp, int nE, const v3<T> (&x)[P], const T (&xx)[P],
+                                               T (&inv3)[P])
+{
+#pragma unroll
+    for(int p = 0; p < P; ++p) inv3[p] = T(0);
+    for(int e = 0; e < nE; ++e)
+    {
+        if(EK == 0)
+        {
+            const gma::Corner<T> k = ((const gma::Corner<T>*)sp)[e];
+#pragma unroll
+            for(int p = 0; p < P; ++p) inv3[p] += gma::fastTet<T>(k, x[p], xx[p]);
+        }
+        else
+        {
+            const gma::Hex<T>& h = ((const gma::Hex<T>*)sp)[e];
+#pragma unroll
+            for(int p = 0; p < P; ++p) inv3[p] += gma::fastHex<T>(h, x[p], xx[p]);
+        }
+    }
+}
+
 // Patch quality assuming every element is positive (branch-free); `bad`
 // reports that the assumption failed.
 template<typename T, int EK>
@@ -1218,9 +1242,10 @@ __global__ void __launch_bounds__(512) spawnAffineKernel(const DevMesh* __restri
         {
             x[p] = xb + gms::spawnOffset<double>(j + L * p) * scale;
             xx[p] = gmq::dot(x[p], x[p]);
-            sum[p] = affineSumFast<double, EK>(sp, nE, x[p], xx[p]);
-            anyGood |= !gma::fastBad<double>(sum[p]);
         }
+        affineSumsFast<double, EK, P>(sp, nE, x, xx, sum);
+#pragma unroll
+        for(int p = 0; p < P; ++p) anyGood |= !gma::fastBad<double>(sum[p]);
         const bool exact = (__ballot(anyGood) & groupMask) == 0;
         double key[P];
         double best = -__builtin_huge_val();
