@@ -11,7 +11,9 @@ in FP64).  A step is one smoothing pass of one smoother over the whole mesh.
 
 value = movable vertices x passes x smoothers / wall time of the timed passes
 (inputs resident in HBM; the per-pass FP64 quality reductions are inside the
-timed region).  Also reported: the plain FP64 single-mesh kernel (one lane
+timed region).  The timed passes replay each pass body from a HIP graph (the
+mesh as a batch of one); an untimed eager run of the same passes, which must
+give identical results, supplies the per-launch kernel times.  Also reported: the plain FP64 single-mesh kernel (one lane
 per vertex) on the same workload, the roofline of the dominant kernel and the
 CPU oracle on a bounded sample of the same work.
 
@@ -275,7 +277,7 @@ def main():
         sys.exit(2)
 
     import gpumesh_amd
-    from gpumesh_amd import GpuMesh, cube_mesh
+    from gpumesh_amd import GpuBatch, GpuMesh, cube_mesh
 
     if gpumesh_amd.device_count() < 1:
         print("no HIP device", file=sys.stderr)
@@ -299,16 +301,32 @@ def main():
     start = gm.positions()
 
     per = {}
+    # The timed passes replay captured pass bodies (hipGraph) on the mesh as a
+    # batch of one; the warm-up runs at least two passes so that the FP32 and
+    # the FP64 (polish) bodies are captured before the timed region.  The
+    # same passes run once more eagerly, untimed, with per-launch HIP events
+    # (kernel times for the roofline) and per-vertex evaluation counts; their
+    # results must be identical bit for bit.
+    one = GpuBatch([gm])
     for sm in smoothers:
-        if args.warmup > 0:
-            gm.set_positions(start)
-            gm.smooth(sm, args.warmup, precision=args.precision, kernel=args.kernel, spawn_coeff=spawn_coeff)
+        gm.set_positions(start)
+        one.smooth(sm, max(args.warmup, 2), precision=args.precision, kernel=args.kernel, spawn_coeff=spawn_coeff,
+                   graph=True)
         gm.set_positions(start)
         t0 = time.perf_counter()
-        res = gm.smooth(sm, args.steps, precision=args.precision, kernel=args.kernel, spawn_coeff=spawn_coeff)
+        rg = one.smooth(sm, args.steps, precision=args.precision, kernel=args.kernel, spawn_coeff=spawn_coeff,
+                        graph=True)
         wall = time.perf_counter() - t0
+        end_pos = gm.positions()
+        gm.set_positions(start)
+        res = gm.smooth(sm, args.steps, precision=args.precision, kernel=args.kernel, spawn_coeff=spawn_coeff)
+        if not (np.array_equal(end_pos, gm.positions()) and np.array_equal(rg["inv_sum"][0], res["inv_sum"])
+                and np.array_equal(rg["min"][0], res["min"])):
+            raise RuntimeError(f"{sm}: graph replay and eager run differ")
+        res["graph_total_ms"] = rg["total_ms"]
         per[sm] = {"wall_s": wall, "res": res}
-        log(f"{sm}: {args.steps} passes {wall * 1e3:.1f} ms wall ({res['total_ms']:.1f} ms device), "
+        log(f"{sm}: {args.steps} passes {wall * 1e3:.1f} ms wall ({res['graph_total_ms']:.1f} ms device; eager "
+            f"{res['total_ms']:.1f} ms), "
             f"min {res['min'][0]:.4f} -> {res['min'][-1]:.4f}, mean {res['mean'][0]:.5f} -> {res['mean'][-1]:.5f}")
 
     total_wall = sum(p["wall_s"] for p in per.values())
@@ -434,7 +452,8 @@ def main():
                    "smoothers": smoothers, "passes": args.steps, "precision": args.precision,
                    "kernel": args.kernel, "spawn_coeff": spawn_coeff},
         "per_smoother": {sm: {"vertices_per_s": gm.nmovable * args.steps / p["wall_s"],
-                              "device_ms": p["res"]["total_ms"],
+                              "device_ms": p["res"]["graph_total_ms"],
+                              "eager_device_ms": p["res"]["total_ms"],
                               "final_min": float(p["res"]["min"][-1]), "final_mean": float(p["res"]["mean"][-1]),
                               "elements_evaluated_per_s": (float(np.sum(p["res"]["eval_counts"])) *
                                                            gm.sum_patch / gm.nmovable +
