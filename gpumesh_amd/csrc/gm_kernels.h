@@ -183,6 +183,64 @@ __device__ __forceinline__ float localSizeGroup(const DevMesh& m, const float4* 
     return groupSum<W>(total) * __builtin_amdgcn_rcpf((float)(e - b));
 }
 
+// Records of the FP32 passes (mean ratio, metric-free): each element's
+// quality in FP64 from the FP32 positions, with det(F)^(-2/3) from the
+// hardware seed and two Newton steps (gma::powm23) and the reciprocal from
+// the hardware estimate and two Newton steps, in place of the correctly
+// rounded cube root and divisions of gmq::cornerQuality (a few ulp apart).
+// Elements with a non-positive corner take the exact routine.  Returns q
+// and sets inv = 1 / q for q > 0.
+__device__ __forceinline__ double rcpNewton(double x)
+{
+    double r = __builtin_amdgcn_rcp(x);
+    double e = __builtin_fma(-x, r, 1.0);
+    r = __builtin_fma(r, e, r);
+    e = __builtin_fma(-x, r, 1.0);
+    return __builtin_fma(r, e, r);
+}
+
+__device__ __forceinline__ double evalCornerInv3(const gmq::m3<double>& F)
+{
+    const double d = gmq::det(F);
+    const double f2 = gmq::dot(F.c[0], F.c[0]) + gmq::dot(F.c[1], F.c[1]) + gmq::dot(F.c[2], F.c[2]);
+    return d > 0.0 ? f2 * gma::powm23(d) : __builtin_nan("");
+}
+
+__device__ __forceinline__ double fastTetEval(const gmq::Crew& c, const v3<double> vp[4], double& inv)
+{
+    const double s = evalCornerInv3(
+        gmq::mmul<double>(gmq::mcols<double>(vp[3] - vp[0], vp[3] - vp[1], vp[3] - vp[2]), gmq::frTetInv<double>()));
+    if(s < __builtin_huge_val())
+    {
+        inv = s * (1.0 / 3.0);
+        return 3.0 * rcpNewton(s);
+    }
+    const double q = gmq::tetQuality<double, 0, 0>(c, vp);
+    inv = 1.0 / q;
+    return q;
+}
+
+__device__ __forceinline__ double fastHexEval(const gmq::Crew& c, const v3<double> vp[8], double& inv)
+{
+    const v3<double> e01 = vp[1] - vp[0], e03 = vp[3] - vp[0], e04 = vp[4] - vp[0], e12 = vp[2] - vp[1];
+    const v3<double> e15 = vp[5] - vp[1], e23 = vp[3] - vp[2], e26 = vp[6] - vp[2], e37 = vp[7] - vp[3];
+    const v3<double> e45 = vp[5] - vp[4], e47 = vp[7] - vp[4], e56 = vp[6] - vp[5], e67 = vp[7] - vp[6];
+    using gmq::mcols;
+    const double s = evalCornerInv3(mcols<double>(e01, e04, -e03)) + evalCornerInv3(mcols<double>(e01, e12, e15)) +
+                     evalCornerInv3(mcols<double>(e12, e26, -e23)) + evalCornerInv3(mcols<double>(e03, e23, e37)) +
+                     evalCornerInv3(mcols<double>(e04, e45, e47)) + evalCornerInv3(mcols<double>(e15, -e56, e45)) +
+                     evalCornerInv3(mcols<double>(e26, e56, e67)) + evalCornerInv3(mcols<double>(e37, e67, -e47));
+    if(s < __builtin_huge_val())
+    {
+        // q = 8 / sum(1 / q_k) = 24 / sum(3 / q_k)
+        inv = s * (1.0 / 24.0);
+        return 24.0 * rcpNewton(s);
+    }
+    const double q = gmq::hexQuality<double, 0, 0>(c, vp);
+    inv = 1.0 / q;
+    return q;
+}
+
 // Quality of the patch element `ref` with the patch's centre vertex at p.
 template<typename T, int EV, int SA, typename P>
 __device__ __forceinline__ T elementQualityAt(const DevMesh& m, const gmq::Crew& c, const P* pos, uint32_t ref, v3<T> p)
@@ -273,7 +331,10 @@ __global__ void __launch_bounds__(256) smoothPlainKernel(const DevMesh* __restri
 // EKIND: the patches' element kind (EK_TET or EK_HEX when the launch holds
 // one kind only, EK_MIXED otherwise); a tetrahedron-only launch keeps four
 // vertices per lane.
-template<typename T, int EV, int SA, int G, int EKIND = EK_MIXED>
+// FASTQ (FP64 mean ratio, metric-free; the polish pass): tetrahedron
+// qualities from the hardware-seeded FP64 routines of the evaluation records
+// instead of the correctly rounded ones.
+template<typename T, int EV, int SA, int G, int EKIND = EK_MIXED, bool FASTQ = false>
 struct WavePatch
 {
     const gmq::Crew* c;
@@ -304,7 +365,13 @@ struct WavePatch
                 v3<T> e[4];
 #pragma unroll
                 for(int i = 0; i < 4; ++i) e[i] = (i == k) ? p : vp[i];
-                q = gmq::tetQuality<T, EV, SA>(*c, e);
+                if constexpr(FASTQ && sizeof(T) == 8 && EV == 0 && SA == 0)
+                {
+                    double inv;
+                    q = fastTetEval(*c, e, inv);
+                }
+                else
+                    q = gmq::tetQuality<T, EV, SA>(*c, e);
             }
             else if constexpr(EKIND != EK_TET)
             {
@@ -2182,7 +2249,7 @@ __global__ void __launch_bounds__(512) getmeVertexKernel(const DevMesh* __restri
 // resident wavefronts to cover the dependent gathers.
 constexpr int GETME_WAVES = 4;
 
-template<typename T, int EV, int SA, int G, int EKIND>
+template<typename T, int EV, int SA, int G, int EKIND, bool FASTQ>
 __global__ void __launch_bounds__(64 * GETME_WAVES) getmeWaveKernel(const DevMesh* __restrict__ meshes,
                                                                     const uint4* __restrict__ segs, uint32_t nsegs,
                                                                     uint32_t count, gmq::Crew c)
@@ -2202,7 +2269,7 @@ __global__ void __launch_bounds__(64 * GETME_WAVES) getmeWaveKernel(const DevMes
         return;
     P* pos = PosOf<T>::get(m);
     const P* pullBuf = (const P*)m.getmePull;
-    WavePatch<T, EV, SA, G, EKIND> f;
+    WavePatch<T, EV, SA, G, EKIND, FASTQ> f;
     f.c = &c;
     f.red = red + 2 * G * grp;
     f.lane = lane;
@@ -2312,64 +2379,6 @@ __device__ __forceinline__ void twoSum(double a, double b, double& s, double& e)
     s = a + b;
     double bb = s - a;
     e = (a - (s - bb)) + (b - bb);
-}
-
-// Records of the FP32 passes (mean ratio, metric-free): each element's
-// quality in FP64 from the FP32 positions, with det(F)^(-2/3) from the
-// hardware seed and two Newton steps (gma::powm23) and the reciprocal from
-// the hardware estimate and two Newton steps, in place of the correctly
-// rounded cube root and divisions of gmq::cornerQuality (a few ulp apart).
-// Elements with a non-positive corner take the exact routine.  Returns q
-// and sets inv = 1 / q for q > 0.
-__device__ __forceinline__ double rcpNewton(double x)
-{
-    double r = __builtin_amdgcn_rcp(x);
-    double e = __builtin_fma(-x, r, 1.0);
-    r = __builtin_fma(r, e, r);
-    e = __builtin_fma(-x, r, 1.0);
-    return __builtin_fma(r, e, r);
-}
-
-__device__ __forceinline__ double evalCornerInv3(const gmq::m3<double>& F)
-{
-    const double d = gmq::det(F);
-    const double f2 = gmq::dot(F.c[0], F.c[0]) + gmq::dot(F.c[1], F.c[1]) + gmq::dot(F.c[2], F.c[2]);
-    return d > 0.0 ? f2 * gma::powm23(d) : __builtin_nan("");
-}
-
-__device__ __forceinline__ double fastTetEval(const gmq::Crew& c, const v3<double> vp[4], double& inv)
-{
-    const double s = evalCornerInv3(
-        gmq::mmul<double>(gmq::mcols<double>(vp[3] - vp[0], vp[3] - vp[1], vp[3] - vp[2]), gmq::frTetInv<double>()));
-    if(s < __builtin_huge_val())
-    {
-        inv = s * (1.0 / 3.0);
-        return 3.0 * rcpNewton(s);
-    }
-    const double q = gmq::tetQuality<double, 0, 0>(c, vp);
-    inv = 1.0 / q;
-    return q;
-}
-
-__device__ __forceinline__ double fastHexEval(const gmq::Crew& c, const v3<double> vp[8], double& inv)
-{
-    const v3<double> e01 = vp[1] - vp[0], e03 = vp[3] - vp[0], e04 = vp[4] - vp[0], e12 = vp[2] - vp[1];
-    const v3<double> e15 = vp[5] - vp[1], e23 = vp[3] - vp[2], e26 = vp[6] - vp[2], e37 = vp[7] - vp[3];
-    const v3<double> e45 = vp[5] - vp[4], e47 = vp[7] - vp[4], e56 = vp[6] - vp[5], e67 = vp[7] - vp[6];
-    using gmq::mcols;
-    const double s = evalCornerInv3(mcols<double>(e01, e04, -e03)) + evalCornerInv3(mcols<double>(e01, e12, e15)) +
-                     evalCornerInv3(mcols<double>(e12, e26, -e23)) + evalCornerInv3(mcols<double>(e03, e23, e37)) +
-                     evalCornerInv3(mcols<double>(e04, e45, e47)) + evalCornerInv3(mcols<double>(e15, -e56, e45)) +
-                     evalCornerInv3(mcols<double>(e26, e56, e67)) + evalCornerInv3(mcols<double>(e37, e67, -e47));
-    if(s < __builtin_huge_val())
-    {
-        // q = 8 / sum(1 / q_k) = 24 / sum(3 / q_k)
-        inv = s * (1.0 / 24.0);
-        return 24.0 * rcpNewton(s);
-    }
-    const double q = gmq::hexQuality<double, 0, 0>(c, vp);
-    inv = 1.0 / q;
-    return q;
 }
 
 template<int EV, int SA, typename P, bool HEX>
@@ -2702,21 +2711,21 @@ hipError_t Launchers<T, EV, SA>::getmeElements(const DevMesh* meshes, BlockWork 
     return hipGetLastError();
 }
 
-template<typename T, int EV, int SA, int EKIND>
+template<typename T, int EV, int SA, int EKIND, bool FASTQ = false>
 static void launchGetmeWave(const VertexWork& w, const gmq::Crew& c, hipStream_t stream)
 {
     const int threads = 64 * GETME_WAVES;
     if(w.maxPatch <= 8)
-        getmeWaveKernel<T, EV, SA, 8, EKIND><<<(w.count + threads / 8 - 1) / (threads / 8), threads, 0, stream>>>(
+        getmeWaveKernel<T, EV, SA, 8, EKIND, FASTQ><<<(w.count + threads / 8 - 1) / (threads / 8), threads, 0, stream>>>(
             w.meshes, w.segs, w.nsegs, w.count, c);
     else if(w.maxPatch <= 16)
-        getmeWaveKernel<T, EV, SA, 16, EKIND><<<(w.count + threads / 16 - 1) / (threads / 16), threads, 0, stream>>>(
+        getmeWaveKernel<T, EV, SA, 16, EKIND, FASTQ><<<(w.count + threads / 16 - 1) / (threads / 16), threads, 0, stream>>>(
             w.meshes, w.segs, w.nsegs, w.count, c);
     else if(w.maxPatch <= 32)
-        getmeWaveKernel<T, EV, SA, 32, EKIND><<<(w.count + threads / 32 - 1) / (threads / 32), threads, 0, stream>>>(
+        getmeWaveKernel<T, EV, SA, 32, EKIND, FASTQ><<<(w.count + threads / 32 - 1) / (threads / 32), threads, 0, stream>>>(
             w.meshes, w.segs, w.nsegs, w.count, c);
     else
-        getmeWaveKernel<T, EV, SA, 64, EKIND><<<(w.count + threads / 64 - 1) / (threads / 64), threads, 0, stream>>>(
+        getmeWaveKernel<T, EV, SA, 64, EKIND, FASTQ><<<(w.count + threads / 64 - 1) / (threads / 64), threads, 0, stream>>>(
             w.meshes, w.segs, w.nsegs, w.count, c);
 }
 
@@ -2728,6 +2737,12 @@ hipError_t Launchers<T, EV, SA>::getmeVertices(const VertexWork& w, const gmq::C
         return hipSuccess;
     if(shape == KS_WAVE && w.maxPatch <= 64)
     {
+        if constexpr(sizeof(T) == 8 && EV == 0 && SA == 0)
+            if(w.kind == EK_TET && w.polish)
+            {
+                launchGetmeWave<T, EV, SA, EK_TET, true>(w, c, stream);
+                return hipGetLastError();
+            }
         if(w.kind == EK_TET) launchGetmeWave<T, EV, SA, EK_TET>(w, c, stream);
         else if(w.kind == EK_HEX) launchGetmeWave<T, EV, SA, EK_HEX>(w, c, stream);
         else launchGetmeWave<T, EV, SA, EK_MIXED>(w, c, stream);
