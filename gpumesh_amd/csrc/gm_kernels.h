@@ -581,18 +581,6 @@ template<int G> __device__ __forceinline__ int groupArgmaxFast(float key)
     return (int)__builtin_ctzll(mine);
 }
 
-__device__ __forceinline__ int waveArgmax(float key) { return groupArgmaxFast<64>(key); }
-
-__device__ __forceinline__ int waveArgmax(double key)
-{
-    key = (key == key) ? key : -__builtin_huge_val();
-    double mx = key;
-#pragma unroll
-    for(int off = 32; off > 0; off >>= 1)
-        mx = __builtin_fmax(mx, __shfl_xor(mx, off, 64));
-    return (int)__builtin_ctzll(__ballot(key == mx));
-}
-
 template<typename T>
 __device__ __forceinline__ v3<T> shflv(v3<T> a, int src)
 {
