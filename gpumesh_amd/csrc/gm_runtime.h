@@ -54,6 +54,17 @@ struct Plan;
 
 } // namespace gmr
 
+// Device slots of a topology's vertices and the per-slot arrays uploaded for
+// it (runtime.hip, buildLayout).
+struct DeviceLayout
+{
+    std::vector<uint32_t> slotOf;     // reference number -> slot
+    std::vector<uint32_t> vertexAt;   // slot -> reference number
+    std::vector<uint32_t> eoff, eref, noff, nbr, tets, hexs;
+    std::vector<uint8_t> fixed;
+    std::vector<uint32_t> cornerSlot; // patch entry of each element corner
+};
+
 struct gm_mesh
 {
     bool onDevice = false;     // false: topology only (gm_mesh_create_host)
@@ -67,6 +78,7 @@ struct gm_mesh
     // with spatial coherence) are close in memory; the reference numbering
     // scatters each colour group over the whole mesh.
     std::vector<uint32_t> slotOf;
+    std::shared_ptr<const DeviceLayout> layout;   // the topology's device layout (slotOf is its copy)
     uint32_t* dSlotOf = nullptr;     // device copy of slotOf
     int32_t* dCountsRef = nullptr;   // evaluation counts in the reference numbering
     gmd::DevMesh dm{};         // host copy of the device descriptor (owns its pointers)

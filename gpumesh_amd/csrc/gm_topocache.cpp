@@ -60,9 +60,14 @@ int64_t g_hits = 0, g_misses = 0;
 
 size_t topologyBytes(const Topology& t)
 {
+    // The device layout built from it on the first upload (slots twice,
+    // patch, neighbour and element arrays, fixed flags, corner slots) is
+    // counted with it.
+    const size_t layout = 2 * t.fixed.size() + 4 * (2 * t.fixed.size() + t.eoff.size() + 2 * t.eref.size() +
+                                                   t.noff.size() + t.nbr.size() + t.tets.size() + t.hexs.size());
     return t.fixed.size() + 4 * (t.tets.size() + t.hexs.size() + t.eoff.size() + t.eref.size() + t.noff.size() +
                                  t.nbr.size() + t.groupNodes.size() + t.serialGroup.size()) +
-           8 * (t.permutation.size() + t.groupOff.size()) + 4 * t.groupMaxPatch.size();
+           8 * (t.permutation.size() + t.groupOff.size()) + 4 * t.groupMaxPatch.size() + layout;
 }
 
 void evictTo(size_t cap)

@@ -40,6 +40,11 @@ struct Topology
     std::vector<int32_t> groupMaxPatch;    // largest patch per group
     std::vector<uint32_t> serialGroup;     // movable vertices in serial order
 
+    // The runtime's device-side layout of this topology (its own type),
+    // built on the first upload and shared by later meshes of a cached
+    // topology.
+    mutable std::shared_ptr<const void> deviceLayout;
+
     int64_t ngroups() const { return (int64_t)groupOff.size() - 1; }
 };
 

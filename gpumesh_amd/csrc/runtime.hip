@@ -241,6 +241,84 @@ void destroyMesh(gm_mesh* m)
     delete m;
 }
 
+// Device slots and the arrays laid out by them (see gm_runtime.h): the fixed
+// vertices, then colour group by colour group, each segment in input order
+// (the reference numbering is already fixed-then-group-major: group g holds
+// ids nfixed + groupOff[g] ..).  The patch's element order and the neighbour
+// order (the order of the reference's sums) are kept.  Computed once per
+// topology and kept with it, so meshes of a cached topology upload without
+// it.  nullptr if the groups do not cover the movable vertices.
+static std::shared_ptr<const DeviceLayout> buildLayout(const gmt::Topology& t)
+{
+    const size_t V = (size_t)t.nverts;
+    auto L = std::make_shared<DeviceLayout>();
+    std::vector<uint32_t>& vertexAt = L->vertexAt;
+    vertexAt.assign(V, 0);
+    const size_t G = t.groupOff.empty() ? 0 : t.groupOff.size() - 1;
+    if((size_t)t.nfixed + (G ? (size_t)t.groupOff[G] : 0) != V)
+        return nullptr;
+    std::vector<size_t> starts(G), next(G + 1);
+    next[0] = 0;
+    for(size_t g = 0; g < G; ++g) next[g + 1] = starts[g] = (size_t)t.nfixed + (size_t)t.groupOff[g];
+    for(size_t o = 0; o < V; ++o)
+    {
+        const size_t v = (size_t)t.permutation[o];
+        const size_t seg = v < (size_t)t.nfixed
+                               ? 0
+                               : (size_t)(std::upper_bound(starts.begin(), starts.end(), v) - starts.begin());
+        vertexAt[next[seg]++] = (uint32_t)v;
+    }
+    std::vector<uint32_t>& slot = L->slotOf;
+    slot.assign(V, 0);
+    for(size_t s = 0; s < V; ++s) slot[vertexAt[s]] = (uint32_t)s;
+    L->eoff.assign(V + 1, 0);
+    L->noff.assign(V + 1, 0);
+    L->eref.resize(t.eref.size());
+    L->nbr.resize(t.nbr.size());
+    L->fixed.resize(V);
+    L->cornerSlot.assign(std::max<size_t>(t.eref.size(), 1), 0);
+    for(size_t s = 0; s < V; ++s)
+    {
+        const size_t v = vertexAt[s];
+        L->fixed[s] = t.fixed[v];
+        const uint32_t e0 = t.eoff[v], e1 = t.eoff[v + 1];
+        L->eoff[s + 1] = L->eoff[s] + (e1 - e0);
+        for(uint32_t r = e0; r < e1; ++r)
+        {
+            const uint32_t at = L->eoff[s] + (r - e0);
+            const uint32_t ref = t.eref[r];
+            L->eref[at] = ref;
+            const size_t id = ref >> 4, k = ref & 7;
+            // The patch entry of each element corner (GETMe's element step
+            // writes there): tets 4 e + k, then hexes 4 T + 8 h + k.
+            L->cornerSlot[(ref & 8) ? 4 * (size_t)t.ntets + 8 * id + k : 4 * id + k] = at;
+        }
+        L->noff[s + 1] = L->noff[s] + (t.noff[v + 1] - t.noff[v]);
+        for(uint32_t r = t.noff[v]; r < t.noff[v + 1]; ++r) L->nbr[L->noff[s] + (r - t.noff[v])] = slot[t.nbr[r]];
+    }
+    L->tets.resize(t.tets.size());
+    L->hexs.resize(t.hexs.size());
+    for(size_t k = 0; k < t.tets.size(); ++k) L->tets[k] = slot[t.tets[k]];
+    for(size_t k = 0; k < t.hexs.size(); ++k) L->hexs[k] = slot[t.hexs[k]];
+    return L;
+}
+
+static std::mutex g_layoutMu;
+
+static std::shared_ptr<const DeviceLayout> layoutOf(const gmt::Topology& t)
+{
+    {
+        std::lock_guard<std::mutex> lock(g_layoutMu);
+        if(t.deviceLayout) return std::static_pointer_cast<const DeviceLayout>(t.deviceLayout);
+    }
+    std::shared_ptr<const DeviceLayout> L = buildLayout(t);
+    if(!L) return nullptr;
+    std::lock_guard<std::mutex> lock(g_layoutMu);
+    if(t.deviceLayout) return std::static_pointer_cast<const DeviceLayout>(t.deviceLayout);
+    t.deviceLayout = L;
+    return L;
+}
+
 int uploadMesh(gm_mesh* m)
 {
     const gmt::Topology& t = *m->topo;
@@ -279,39 +357,16 @@ int uploadMesh(gm_mesh* m)
     d.ntets = (uint32_t)t.ntets;
     d.nhexs = (uint32_t)t.nhexs;
     d.evalBlocks = gmd::evalBlocksFor((uint64_t)t.ntets + (uint64_t)t.nhexs);
-    // Slots: the fixed vertices in input order, then colour group by colour
-    // group, each group in input order.  slotOf: reference number -> slot;
-    // vertexAt: slot -> reference number.
-    std::vector<uint32_t> vertexAt(V);
-    {
-        // The reference numbering is already fixed-then-group-major (group g
-        // holds ids nfixed + groupOff[g] ..); one pass over the input ids
-        // fills each segment in input order.
-        const size_t G = t.groupOff.empty() ? 0 : t.groupOff.size() - 1;
-        if((size_t)t.nfixed + (G ? (size_t)t.groupOff[G] : 0) != V)
-            return bail(fail(GM_ERR_ARG, "colour groups do not cover the movable vertices"));
-        std::vector<size_t> starts(G), next(G + 1);
-        next[0] = 0;
-        for(size_t g = 0; g < G; ++g) next[g + 1] = starts[g] = (size_t)t.nfixed + (size_t)t.groupOff[g];
-        for(size_t o = 0; o < V; ++o)
-        {
-            const size_t v = (size_t)t.permutation[o];
-            const size_t seg = v < (size_t)t.nfixed
-                                   ? 0
-                                   : (size_t)(std::upper_bound(starts.begin(), starts.end(), v) - starts.begin());
-            vertexAt[next[seg]++] = (uint32_t)v;
-        }
-    }
-    m->slotOf.assign(V, 0);
-    for(size_t s_ = 0; s_ < V; ++s_) m->slotOf[vertexAt[s_]] = (uint32_t)s_;
-    const std::vector<uint32_t>& slot = m->slotOf;
+    std::shared_ptr<const DeviceLayout> L = layoutOf(t);
+    if(!L) return bail(fail(GM_ERR_ARG, "colour groups do not cover the movable vertices"));
+    m->layout = L;
+    m->slotOf = L->slotOf;
+    const std::vector<uint32_t>& slot = L->slotOf;
     std::vector<double> p64(4 * V);
     std::vector<float> p32(4 * V);
-    std::vector<uint32_t> eoD(V + 1, 0), noD(V + 1, 0), erD(t.eref.size()), nbD(t.nbr.size());
-    std::vector<uint8_t> fxD(V);
     for(size_t s_ = 0; s_ < V; ++s_)
     {
-        const size_t v = vertexAt[s_];
+        const size_t v = L->vertexAt[s_];
         for(int k = 0; k < 3; ++k)
         {
             p64[4 * s_ + k] = m->hostPos[3 * v + k];
@@ -319,17 +374,10 @@ int uploadMesh(gm_mesh* m)
         }
         p64[4 * s_ + 3] = 0.0;
         p32[4 * s_ + 3] = 0.0f;
-        fxD[s_] = t.fixed[v];
-        // The patch's element order and the neighbour order (the order of
-        // the reference's sums) are kept.
-        eoD[s_ + 1] = eoD[s_] + (t.eoff[v + 1] - t.eoff[v]);
-        std::copy(t.eref.begin() + t.eoff[v], t.eref.begin() + t.eoff[v + 1], erD.begin() + eoD[s_]);
-        noD[s_ + 1] = noD[s_] + (t.noff[v + 1] - t.noff[v]);
-        for(uint32_t r = t.noff[v]; r < t.noff[v + 1]; ++r) nbD[noD[s_] + (r - t.noff[v])] = slot[t.nbr[r]];
     }
-    std::vector<uint32_t> tD(t.tets.size()), hD(t.hexs.size());
-    for(size_t k = 0; k < t.tets.size(); ++k) tD[k] = slot[t.tets[k]];
-    for(size_t k = 0; k < t.hexs.size(); ++k) hD[k] = slot[t.hexs[k]];
+    const std::vector<uint32_t>&eoD = L->eoff, &noD = L->noff, &erD = L->eref, &nbD = L->nbr, &tD = L->tets,
+                                &hD = L->hexs;
+    const std::vector<uint8_t>& fxD = L->fixed;
     uint4 *tb, *hb;
     uint32_t *eo, *er, *no, *nb;
     uint8_t* fx;
@@ -546,23 +594,10 @@ static int ensureGetme(gm_mesh* m)
     // writes each corner's result where the vertex step reads it, so a
     // vertex pulls one contiguous run.
     const gmt::Topology& t = *m->topo;
-    const size_t V = (size_t)t.nverts, N = t.eref.size();
+    const size_t N = t.eref.size();
     const size_t pullBytes = N * sizeof(double4);
     const size_t qualBytes = ((size_t)t.ntets + (size_t)t.nhexs) * sizeof(double);
-    std::vector<uint32_t> vertexAt(V);
-    for(size_t v = 0; v < V; ++v) vertexAt[m->slotOf[v]] = (uint32_t)v;
-    std::vector<uint32_t> cs(std::max<size_t>(N, 1), 0);
-    uint32_t at = 0;
-    for(size_t s = 0; s < V; ++s)
-    {
-        const size_t v = vertexAt[s];
-        for(uint32_t r = t.eoff[v]; r < t.eoff[v + 1]; ++r, ++at)
-        {
-            const uint32_t ref = t.eref[r];
-            const size_t id = ref >> 4, k = ref & 7;
-            cs[(ref & 8) ? 4 * (size_t)t.ntets + 8 * id + k : 4 * id + k] = at;
-        }
-    }
+    const std::vector<uint32_t>& cs = m->layout->cornerSlot;
     uint32_t* dcs = nullptr;
     GMR_HIP(poolAlloc(&m->dm.getmePull, std::max<size_t>(32, pullBytes)));
     GMR_HIP(poolAlloc((void**)&dcs, std::max<size_t>(32, cs.size() * 4)));
