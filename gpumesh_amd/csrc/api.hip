@@ -655,7 +655,7 @@ int gm_batch_autotune(gm_batch* b, int smoother, int precision, int passes, int3
 
 int gm_math_probe(int op, const double* a, const double* b, double* out, int64_t n)
 {
-    if(op < 0 || op > 4 || n < 0 || (n && (!a || !out || (op == 1 && !b)))) return fail(GM_ERR_ARG, "bad probe arguments");
+    if(op < 0 || op > 5 || n < 0 || (n && (!a || !out || (op == 1 && !b)))) return fail(GM_ERR_ARG, "bad probe arguments");
     if(n == 0) return GM_OK;
     int ndev = 0;
     if(hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(GM_ERR_HIP, "no HIP device visible");

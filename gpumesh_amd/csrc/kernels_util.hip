@@ -105,6 +105,7 @@ __global__ void mathProbeKernel(int op, const double* a, const double* b, double
     case 1: r = x / b[i]; break;
     case 2: r = gm_pow23(x); break;
     case 3: r = gm_cbrt_nonneg(x); break;
+    case 5: r = gma::powm23(x); break;
     default: r = gm_exp(x); break;
     }
     out[i] = r;

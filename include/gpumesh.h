@@ -342,7 +342,9 @@ int gm_cube_mesh(int n, int kind, double jitter, uint64_t seed, double min_det_f
                  int32_t* fixed, uint32_t* elems);
 
 /* Device math probe (for tests): op 0 sqrt(a), 1 a/b, 2 a^(2/3), 3 a^(1/3),
- * 4 exp(a), all in FP64 on the device; n values. */
+ * 4 exp(a), 5 a^(-2/3) as the FP32-polish and evaluation-record kernels
+ * compute it (hardware seed, one Halley step), all in FP64 on the device;
+ * n values. */
 int gm_math_probe(int op, const double* a, const double* b, double* out, int64_t n);
 
 #ifdef __cplusplus

@@ -34,6 +34,26 @@ def test_math_probe_bitexact(oracle):
     assert np.array_equal(math_probe(4, e).view(np.uint64), oracle.gm_libm(4, e).view(np.uint64))
 
 
+def test_math_probe_fast_powm23_bound():
+    """a^(-2/3) of the mixed-precision kernels (FP32 hardware seed, one
+    Halley step): within 4e-15 relative of an 80-bit reference over the float
+    range, 1e-15 in the unit range; the exact routine beyond the float range
+    and at 0 (inf), inf (0) and negative input (NaN)."""
+    rng = np.random.default_rng(3)
+    x = np.concatenate([2.0 ** rng.uniform(-126, 127, 20000), rng.uniform(1e-9, 2.0, 20000)])
+    got = math_probe(5, x)
+    ref = np.power(x.astype(np.longdouble), np.longdouble(-2) / np.longdouble(3))
+    rel = np.abs((got.astype(np.longdouble) - ref) / ref).astype(np.float64)
+    assert rel.max() < 4e-15, rel.max()
+    unit = (x > 1e-9) & (x < 2.0)
+    assert rel[unit].max() < 1e-15, rel[unit].max()
+    edge = np.array([0.0, np.inf, -1.0, 1e-300, 1e300])
+    g = math_probe(5, edge)
+    assert g[0] == np.inf and g[1] == 0.0 and np.isnan(g[2])
+    ref = np.power(edge[3:].astype(np.longdouble), np.longdouble(-2) / np.longdouble(3))
+    assert np.all(np.abs((g[3:].astype(np.longdouble) - ref) / ref) < 1e-15)
+
+
 @pytest.mark.parametrize("kind", ["tet", "hex"])
 def test_element_qualities_bitexact(oracle, kind):
     m, om, gm = _pair(oracle, 8, kind, 0.3, 11)
