@@ -1,0 +1,72 @@
+"""Degenerate inputs on the GPU: meshes without a movable vertex, zero
+passes, and such meshes inside a batch.
+
+A one-cell cube has every vertex on the boundary, so it has no colour group:
+the passes launch only the element steps (GETMe) and the evaluation, the
+positions must not move, and every record equals the oracle's.  Zero passes
+give the one record of the starting mesh.
+"""
+import numpy as np
+import pytest
+
+from gpumesh_amd import GpuBatch, GpuMesh, cube_mesh
+
+pytestmark = pytest.mark.gpu
+
+COEFF = {"nm": 1.0 / 20.0, "gd": 1.0 / 20.0, "spawn": 0.25, "getme": 1.0 / 20.0}
+
+
+@pytest.mark.parametrize("smoother", ["nm", "gd", "spawn", "getme"])
+@pytest.mark.parametrize("kind", ["tet", "hex"])
+@pytest.mark.parametrize("precision", ["fp64", "fp32_polish"])
+def test_no_movable_vertex(oracle, smoother, kind, precision):
+    m = cube_mesh(1, kind, 0.0, seed=1)
+    assert m.fixed.all()
+    om = oracle.OracleMesh.from_cube(m)
+    gm = GpuMesh.from_cube(m)
+    assert gm.ngroups == 0 and gm.nmovable == 0
+    before = gm.positions()
+    ref = om.smooth(smoother, 2, threads=1, spawn_coeff=COEFF[smoother])
+    out = gm.smooth(smoother, 2, precision=precision, kernel="wave", spawn_coeff=COEFF[smoother])
+    assert np.array_equal(gm.positions(), before)
+    assert not out["eval_counts"].any()
+    if precision == "fp64":
+        assert np.array_equal(out["min"], ref["min"])
+        assert np.array_equal(out["hist"], ref["hist"])
+    else:
+        assert np.max(np.abs(out["min"] - ref["min"])) < 1e-6
+
+
+@pytest.mark.parametrize("smoother", ["nm", "gd", "spawn", "getme"])
+def test_zero_passes_is_the_start_record(oracle, smoother):
+    m = cube_mesh(4, "tet", 0.3, seed=2)
+    om = oracle.OracleMesh.from_cube(m)
+    gm = GpuMesh.from_cube(m)
+    before = gm.positions()
+    ref = om.smooth(smoother, 0, threads=1, spawn_coeff=COEFF[smoother])
+    out = gm.smooth(smoother, 0, precision="fp64", kernel="wave", spawn_coeff=COEFF[smoother])
+    assert out["min"].shape == (1,) and np.array_equal(out["min"], ref["min"])
+    assert np.array_equal(out["hist"], ref["hist"])
+    assert np.array_equal(gm.positions(), before)
+
+
+@pytest.mark.parametrize("graph", [False, True])
+@pytest.mark.parametrize("precision", ["fp64", "fp32_polish"])
+@pytest.mark.parametrize("smoother", ["gd", "getme"])
+def test_batch_with_meshes_without_movable_vertices(smoother, precision, graph):
+    """Members without a colour group next to ordinary ones: every member
+    comes out bit-identical to the same mesh smoothed alone."""
+    specs = [(1, "tet", 1), (5, "tet", 3), (1, "hex", 1), (4, "hex", 4)]
+    meshes = [cube_mesh(n, kind, 0.3 if n > 1 else 0.0, seed=s) for n, kind, s in specs]
+    alone = []
+    for m in meshes:
+        g = GpuMesh.from_cube(m)
+        r = g.smooth(smoother, 3, precision=precision, kernel="wave", spawn_coeff=COEFF[smoother])
+        alone.append((g.positions(), r))
+    members = [GpuMesh.from_cube(m) for m in meshes]
+    b = GpuBatch(members)
+    rb = b.smooth(smoother, 3, precision=precision, kernel="wave", spawn_coeff=COEFF[smoother], graph=graph)
+    for i, (pos, r) in enumerate(alone):
+        assert np.array_equal(members[i].positions().view(np.uint64), pos.view(np.uint64)), i
+        assert np.array_equal(rb["min"][i].view(np.uint64), r["min"].view(np.uint64)), i
+        assert np.array_equal(rb["hist"][i], r["hist"]), i
