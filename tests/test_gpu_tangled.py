@@ -65,3 +65,31 @@ def test_tangled_mixed_precision_untangles(smoother, kind):
     assert abs(r32["min"][0] - r64["min"][0]) < 1e-6 and r32["min"][0] < 0.0
     assert r32["min"][-1] > 0.0, r32["min"]
     assert abs(r32["mean"][-1] - r64["mean"][-1]) < 0.02, (r32["mean"][-1], r64["mean"][-1])
+
+
+@pytest.mark.parametrize("kind,evaluator,smoother,kernel", [
+    ("hex", "metric_conformity", "nm", "wave"), ("hex", "metric_conformity", "nm", "plain"),
+    ("hex", "metric_conformity", "gd", "wave"), ("hex", "metric_conformity", "spawn", "wave"),
+    ("hex", "metric_conformity", "getme", "wave"),
+    ("tet", "mean_ratio", "gd", "wave"), ("tet", "mean_ratio", "getme", "wave")])
+def test_tangled_anisotropic_fp64_bitexact(oracle, kind, evaluator, smoother, kernel):
+    """The same tangles under the shell field (analytic sampler, metric-wise
+    measurer): metric conformity signs inverted elements by det(F), the
+    metric-wise mean ratio measures adaptive Riemannian segments (wave
+    kernels only: the one-lane-per-vertex kernels walk those segments for a
+    whole patch in one lane)."""
+    m, pos = tangled_mesh(kind)
+    om = oracle.OracleMesh(pos, m.fixed, m.tets, m.hexs)
+    gm = GpuMesh(pos, m.fixed, m.tets, m.hexs)
+    om.set_crew(evaluator, "analytic", scaling=4.0)
+    gm.set_crew(evaluator, "analytic", scaling=4.0)
+    g0 = gm.evaluate(element_qualities=True)
+    assert np.array_equal(g0["elem_qual"], om.element_qualities())
+    assert g0["min"] < 0.0
+    coeff = 0.25 if smoother == "spawn" else 1.0 / 20.0
+    ref = om.smooth(smoother, 2, threads=2, spawn_coeff=coeff)
+    out = gm.smooth(smoother, 2, precision="fp64", kernel=kernel, spawn_coeff=coeff)
+    p_ref, p_gpu = om.positions(), gm.positions()
+    assert np.array_equal(p_gpu, p_ref), f"max |d| = {np.max(np.abs(p_gpu - p_ref))}"
+    assert np.array_equal(out["eval_counts"], ref["eval_counts"].astype(np.int32))
+    assert np.array_equal(out["min"], ref["min"])
