@@ -65,3 +65,18 @@ def test_server_needs_a_device():
     with pytest.raises(_lib.GpuMeshError) as e:
         GpuServer()
     assert e.value.status == 2
+
+
+def test_missing_library_fails_loudly(monkeypatch, tmp_path):
+    """Without the HIP library the product refuses to run (no CPU fallback):
+    every entry point raises ImportError naming the missing file."""
+    m = cube_mesh(2, "tet", 0.0, seed=1)
+    missing = str(tmp_path / "libgpumesh_hip.so")
+    monkeypatch.setattr(_lib, "LIB_PATH", missing)
+    monkeypatch.setattr(_lib, "_LIB", None)
+    with pytest.raises(ImportError, match="is not built"):
+        GpuMesh.from_cube(m, device=False)
+    with pytest.raises(ImportError, match="is not built"):
+        _lib.math_probe(0, np.ones(4))
+    with pytest.raises(ImportError, match="is not built"):
+        cube_mesh(2, "tet", 0.0, seed=1)
