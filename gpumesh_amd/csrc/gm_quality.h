@@ -236,6 +236,11 @@ template<typename T, int SA> GMQ T riemannianDistanceWise(const Crew& c, v3<T> a
         T l1 = tsqrt(dot(bv, mvec(M1, bv)));
         T sum = (l0 + l1);
         T diff = tabs(l0 - l1) / (sum / T(2));
+        // A NaN difference (a zero-length pair, or a non-finite position or
+        // metric) never falls below the threshold, so the subdivision would
+        // run until the int indices overflow; the walk ends here instead.
+        if(!(diff == diff))
+            return len + sum;
         if(diff < T(0.01))
         {
             len += sum;

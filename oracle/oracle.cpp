@@ -518,6 +518,11 @@ static double riemannianDistanceWise(const Crew& c, const dvec3& a, const dvec3&
         double l1 = std::sqrt(dot(bv, M1 * bv));
         double sum = (l0 + l1);
         double diff = std::abs(l0 - l1) / (sum / 2.0);
+        // Not in the reference: a NaN difference (zero-length pair or
+        // non-finite input) would subdivide until the int indices overflow;
+        // the product ends the walk here, and so does the oracle.
+        if(!(diff == diff))
+            return len + sum;
         if(diff < DIFF_THRESHOLD)
         {
             len += sum;

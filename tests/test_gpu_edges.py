@@ -70,3 +70,25 @@ def test_batch_with_meshes_without_movable_vertices(smoother, precision, graph):
         assert np.array_equal(members[i].positions().view(np.uint64), pos.view(np.uint64)), i
         assert np.array_equal(rb["min"][i].view(np.uint64), r["min"].view(np.uint64)), i
         assert np.array_equal(rb["hist"][i], r["hist"]), i
+
+
+def test_metric_wise_zero_length_edge(oracle):
+    """Two coincident vertices under the metric-wise measurer: the adaptive
+    walk ends on the NaN pair difference; the elements holding the edge get
+    NaN, the others their qualities, bit for bit with the oracle."""
+    m = cube_mesh(2, "tet", 0.0, seed=1)
+    inner = int(np.where(m.fixed == 0)[0][0])
+    t = m.tets[np.any(m.tets == inner, axis=1)][0]
+    pos = m.pos.copy()
+    pos[inner] = pos[int(t[t != inner][0])]
+    om = oracle.OracleMesh(pos, m.fixed, m.tets, m.hexs)
+    gm = GpuMesh(pos, m.fixed, m.tets, m.hexs)
+    om.set_crew("mean_ratio", "analytic", scaling=2.0)
+    gm.set_crew("mean_ratio", "analytic", scaling=2.0)
+    g = gm.evaluate(element_qualities=True)
+    q = om.element_qualities()
+    assert np.isnan(q).sum() == 6
+    assert np.array_equal(g["elem_qual"], q, equal_nan=True)
+    r = om.evaluate()
+    assert g["min"] == r["min"]
+    assert np.array_equal(g["hist"], r["hist"])

@@ -3,6 +3,7 @@ brute force (the reference ships no golden vectors; its only known-answer
 check is AbstractEvaluator::assessMeasureValidy, Evaluators/AbstractEvaluator.cpp:281-338).
 Each check is chosen so that a plausible slip in the restatement (a dropped
 term, a transposed operand, a wrong index or sign) fails it."""
+import os
 import numpy as np
 import pytest
 
@@ -367,3 +368,32 @@ def test_autopilot_limits(oracle):
     assert o.smooth_autopilot("gd", 3, min_qual_threshold=-1.0)["passes"] == 3   # never stops: the cap
     o = oracle.OracleMesh.from_cube(m)
     assert o.smooth_autopilot("gd", 5, min_qual_threshold=1.0, qual_mean_threshold=1.0)["passes"] == 1
+
+
+def test_metric_wise_zero_length_edge_ends():
+    """The metric-wise measurer's adaptive walk on a zero-length edge (two
+    coincident vertices): the pair difference is NaN, which never passes the
+    threshold; the walk must end (NaN quality for the elements holding the
+    edge) instead of subdividing without bound.  Runs in a child process
+    under a time limit so that a regression cannot stall the suite."""
+    import subprocess
+    import sys
+    import textwrap
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    code = textwrap.dedent("""
+        import numpy as np
+        from oracle import pyoracle as po
+        from gpumesh_amd import cube_mesh
+        m = cube_mesh(2, "tet", 0.0, seed=1)
+        inner = int(np.where(m.fixed == 0)[0][0])
+        t = m.tets[np.any(m.tets == inner, axis=1)][0]
+        pos = m.pos.copy()
+        pos[inner] = pos[int(t[t != inner][0])]
+        om = po.OracleMesh(pos, m.fixed, m.tets, m.hexs)
+        om.set_crew("mean_ratio", "analytic", scaling=2.0)
+        q = om.element_qualities()
+        print(int(np.isnan(q).sum()), int(np.isfinite(q).sum()))
+    """)
+    r = subprocess.run([sys.executable, "-c", code], cwd=root, timeout=120, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    assert r.stdout.split() == ["6", "42"]
