@@ -116,3 +116,23 @@ def test_job_files_match_alone():
         bad[-1] ^= 1
         with pytest.raises(GpuMeshError):
             srv.submit_job(bytes(bad))
+
+
+def test_jobs_without_movable_vertices():
+    """One-cell cubes have no movable vertex (no colour group).  Served
+    first on their own (a round with no colour step), then next to ordinary
+    jobs, each result equals the same mesh smoothed alone."""
+    specs = [(1, "tet", 0.0, 1, "getme", 3, "fp32_polish"), (1, "hex", 0.0, 1, "gd", 2, "fp64"),
+             (1, "tet", 0.0, 2, "nm", 2, "fp32"), (7, "tet", 0.3, 301, "getme", 3, "fp32_polish"),
+             (6, "hex", 0.3, 302, "gd", 2, "fp64")]
+    refs = [_alone(s) for s in specs]
+    with GpuServer(worker_threads=2) as srv:
+        first = [srv.submit_mesh(cube_mesh(*s[:4]), s[4], s[5], s[6]) for s in specs[:3]]
+        for k, j in enumerate(first):
+            assert srv.wait(j, 60000) == "done", k
+            _check(srv.result(j), refs[k], k)
+        ids = [srv.submit_mesh(cube_mesh(*s[:4]), s[4], s[5], s[6]) for s in specs]
+        for k, j in enumerate(ids):
+            assert srv.wait(j, 60000) == "done", k
+            _check(srv.result(j), refs[k], k)
+        assert srv.stats()["failed"] == 0
