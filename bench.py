@@ -47,10 +47,10 @@ PEAK_FP32_TFLOPS = 157.3
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
 
-# Algorithmic FLOP, counting +, -, *, /, sqrt and pow as one each (the
-# derivation is in DESIGN.md).  One mean-ratio tetrahedron evaluated inside a
-# patch: 3 metric-free edge vectors 18, F = E Fr^-1 45, det 14, |F|^2 17,
-# corner quality 4 (incl. pow), patch accumulation 3 (1/q and two adds).
+# Algorithmic FLOP, counting +, -, *, /, sqrt and pow as one each.  One
+# mean-ratio tetrahedron evaluated inside a patch: 3 metric-free edge vectors
+# 18, F = E Fr^-1 45, det 14, |F|^2 17, corner quality 4 (incl. pow), patch
+# accumulation 3 (1/q and two adds).
 FLOP_MR_TET_PATCH = 101.0
 # One GETMe tetrahedron transform: 4 face normals 60, 4 scaled moves 56,
 # 2 volumes 50, volume ratio and cube root 3, centre 12, rescale 36,

@@ -5,8 +5,8 @@ pass in FP32 and the last in FP64; both reduce min/mean quality in FP64.
 The optimisers branch on quality comparisons, so FP32 trajectories leave the
 FP64 ones vertex by vertex; the bounds below are on the outcome that matters
 (final positions relative to the cell size h, final min and mean quality).
-The measured values are written to gpurun_out/precision_errors.json when a
-GPU run has that folder.
+The measured values are written to precision_errors.json in the folder that
+the GPUMESH_RESULTS_DIR environment variable names, when it is set.
 """
 import json
 import os
@@ -42,8 +42,8 @@ def test_fp32_error_against_fp64_oracle(oracle, kind, n, smoother, passes, bx, b
     dmin = float(abs(out["min"][-1] - ref["min"][-1]))
     _RESULTS[f"{kind}/{smoother}/{precision}"] = {"max_dx_over_h": dx, "mean_err": dmean, "min_err": dmin,
                                            "fp64_mean": float(ref["mean"][-1]), "fp64_min": float(ref["min"][-1])}
-    out_dir = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "gpurun_out")
-    if os.path.isdir(out_dir):
+    out_dir = os.environ.get("GPUMESH_RESULTS_DIR")
+    if out_dir and os.path.isdir(out_dir):
         with open(os.path.join(out_dir, "precision_errors.json"), "w") as f:
             json.dump(_RESULTS, f, indent=1)
     assert dx < bx
