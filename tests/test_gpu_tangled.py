@@ -72,17 +72,19 @@ def test_tangled_mixed_precision_untangles(smoother, kind):
     ("hex", "metric_conformity", "gd", "wave"), ("hex", "metric_conformity", "spawn", "wave"),
     ("hex", "metric_conformity", "getme", "wave"),
     ("tet", "mean_ratio", "gd", "wave"), ("tet", "mean_ratio", "getme", "wave")])
-def test_tangled_anisotropic_fp64_bitexact(oracle, kind, evaluator, smoother, kernel):
+def test_tangled_anisotropic_fp64_bitexact(oracle, kind, evaluator, smoother, kernel, sampler="analytic"):
     """The same tangles under the shell field (analytic sampler, metric-wise
     measurer): metric conformity signs inverted elements by det(F), the
     metric-wise mean ratio measures adaptive Riemannian segments (wave
     kernels only: the one-lane-per-vertex kernels walk those segments for a
     whole patch in one lane)."""
+    from gpumesh_amd.samplers import shell_texture
     m, pos = tangled_mesh(kind)
     om = oracle.OracleMesh(pos, m.fixed, m.tets, m.hexs)
     gm = GpuMesh(pos, m.fixed, m.tets, m.hexs)
-    om.set_crew(evaluator, "analytic", scaling=4.0)
-    gm.set_crew(evaluator, "analytic", scaling=4.0)
+    tex = shell_texture(pos, 16, 4.0) if sampler == "texture" else None
+    om.set_crew(evaluator, sampler, scaling=4.0, texture=tex)
+    gm.set_crew(evaluator, sampler, scaling=4.0, texture=tex)
     g0 = gm.evaluate(element_qualities=True)
     assert np.array_equal(g0["elem_qual"], om.element_qualities())
     assert g0["min"] < 0.0
@@ -93,3 +95,11 @@ def test_tangled_anisotropic_fp64_bitexact(oracle, kind, evaluator, smoother, ke
     assert np.array_equal(p_gpu, p_ref), f"max |d| = {np.max(np.abs(p_gpu - p_ref))}"
     assert np.array_equal(out["eval_counts"], ref["eval_counts"].astype(np.int32))
     assert np.array_equal(out["min"], ref["min"])
+
+
+@pytest.mark.parametrize("smoother", ["nm", "gd", "spawn"])
+def test_tangled_texture_fp64_bitexact(oracle, smoother):
+    """The tangled hex cube under metric conformity with the texture sampler
+    (the shell field sampled at 16^3 cell centres in FP32), in the wave
+    kernels, which are where the library takes that sampler."""
+    test_tangled_anisotropic_fp64_bitexact(oracle, "hex", "metric_conformity", smoother, "wave", sampler="texture")
