@@ -105,7 +105,7 @@ def lib():
     L.gm_cube_mesh.argtypes = [C.c_int, C.c_int, dbl, C.c_uint64, dbl, vp, vp, vp]
     L.gm_math_probe.argtypes = [C.c_int, vp, vp, vp, i64]
     for name in EXPORTS:
-        if name not in ("gm_last_error", "gm_version"):
+        if name not in ("gm_last_error", "gm_version", "gm_job_size"):
             getattr(L, name).restype = C.c_int
     _LIB = L
     return L

@@ -304,13 +304,21 @@ int gm_mesh_set_crew(gm_mesh* m, int evaluator, int sampler, double scaling,
             c.tmin[k] = tmin[k];
             c.text[k] = text[k];
         }
+        // The new grid is allocated and filled before the old one is let go,
+        // so a failure leaves the previous crew (and its texels) intact.
         const size_t n = 6 * (size_t)tsize[0] * tsize[1] * tsize[2];
+        float* fresh = nullptr;
+        GM_HIP(devAlloc(&fresh, n));
+        hipError_t e = hipMemcpyAsync(fresh, texels, n * sizeof(float), hipMemcpyHostToDevice, m->stream);
+        if(e == hipSuccess) e = hipStreamSynchronize(m->stream);
+        if(e != hipSuccess)
+        {
+            gmr::poolFree(fresh);
+            return fail(GM_ERR_HIP, std::string("texture upload: ") + hipGetErrorString(e));
+        }
+        c.texels = fresh;
         gmr::poolFree(m->texels);
-        m->texels = nullptr;
-        GM_HIP(devAlloc(&m->texels, n));
-        GM_HIP(hipMemcpyAsync(m->texels, texels, n * sizeof(float), hipMemcpyHostToDevice, m->stream));
-        GM_HIP(hipStreamSynchronize(m->stream));
-        c.texels = m->texels;
+        m->texels = fresh;
     }
     m->crew = c;
     m->evaluator = evaluator;

@@ -80,3 +80,15 @@ def test_missing_library_fails_loudly(monkeypatch, tmp_path):
         _lib.math_probe(0, np.ones(4))
     with pytest.raises(ImportError, match="is not built"):
         cube_mesh(2, "tet", 0.0, seed=1)
+
+
+def test_job_size_is_exact_beyond_two_gib():
+    L = _lib.lib()
+    small = int(L.gm_job_size(10, 4, 2))
+    big = int(L.gm_job_size(2 ** 28, 2 ** 27, 0))
+    # Sections grow linearly in the counts: positions 24 B, fixed flags 1 B
+    # (8-byte aligned), tetrahedra 16 B per element.
+    assert big > 2 ** 32
+    grown = int(L.gm_job_size(2 ** 28 + 8, 2 ** 27 + 1, 0))
+    assert grown - big == 8 * 24 + 8 + 16
+    assert small > 0
