@@ -18,12 +18,16 @@
 // which the GPU kernels also use, so FP64 GPU results can be compared
 // bit-for-bit.  The distance between the two modes is measured by the tests.
 //
-// Parts whose parity is pinned only by this restatement's fidelity to the
-// cited lines (no golden vectors exist in the reference): the multi-pass
-// optimiser trajectories of Nelder-Mead, gradient descent, spawn search and
-// GETMe ("parity unpinned" against the reference binary, which cannot be
-// built here).  Element qualities, colouring and renumbering are pinned by
-// closed forms, invariants and brute-force checks in tests/.
+// What pins it (the reference ships no golden vectors): tests/test_oracle.py
+// and tests/test_oracle_pins.py check element qualities against closed forms;
+// every trial point and value of the Nelder-Mead, gradient-descent and
+// spawn-search vertex steps, the GETMe element transform and vertex step, and
+// two whole passes of each, against versions re-derived from the reference
+// sources and driven by those closed forms; colourings against cases derived
+// by hand; the Keast rule against exact integrals of a trilinear metric; the
+// metric-wise walk against the error band of its 1 % rule.  Parity unpinned:
+// the GLM boundary (formula order of its matrix product, determinant, inverse
+// and mix; version unknown), which moves results only in the last bits.
 
 #include <algorithm>
 #include <array>
@@ -727,10 +731,27 @@ static inline double finalizePatchQuality(double pq, double pw)
     return (pw > 0.0) ? pw / pq : pq;
 }
 
+// Test hook: while set, every patch evaluation appends (x, y, z, quality)
+// of the evaluated vertex, so tests can follow an optimiser's trial points.
+static std::vector<double>* g_trace = nullptr;
+
+static double patchQualityUntraced(const Mesh& m, uint32_t vId);
+
 // AbstractEvaluator::patchQuality (:236-279).
 static double patchQuality(const Mesh& m, uint32_t vId, int64_t* counter)
 {
     if(counter) ++*counter;
+    double q = patchQualityUntraced(m, vId);
+    if(g_trace)
+    {
+        const dvec3& p = m.verts[vId];
+        g_trace->insert(g_trace->end(), {p.x, p.y, p.z, q});
+    }
+    return q;
+}
+
+static double patchQualityUntraced(const Mesh& m, uint32_t vId)
+{
     const Topo& topo = m.topos[vId];
     double pw = 0.0, pq = 0.0;
     for(const NeigElem& ne : topo.neighborElems)
@@ -1026,15 +1047,13 @@ static void parallelFor(size_t n, int threads, F fn)
 // ------------------------------------------------------------------------
 struct ElemOut { dvec3 vpp[8]; double weight; };
 
-static void getmeTet(const Mesh& m, uint32_t e, ElemOut& o)
+// The dual transform and volume rescale of GetmeSmoother::smoothTets (:55-100),
+// before snapping.
+static void getmeTetTransform(const Crew& c, const dvec3 vp[4], dvec3 vpp[4])
 {
     const double lambda = 0.78;
-    const Crew& c = m.crew;
-    uint32_t vi[4]; dvec3 vp[4];
-    for(int k = 0; k < 4; ++k) { vi[k] = m.tets[e][k]; vp[k] = m.verts[vi[k]]; }
     dvec3 n[4] = {cross(vp[3] - vp[1], vp[2] - vp[1]), cross(vp[3] - vp[2], vp[0] - vp[2]),
                   cross(vp[1] - vp[3], vp[0] - vp[3]), cross(vp[1] - vp[0], vp[2] - vp[0])};
-    dvec3 vpp[4];
     for(int k = 0; k < 4; ++k) vpp[k] = vp[k] + n[k] * (lambda / std::sqrt(length(n[k])));
     double volume = tetVolume(c, vp);
     double volumePrime = tetVolume(c, vpp);
@@ -1042,6 +1061,14 @@ static void getmeTet(const Mesh& m, uint32_t e, ElemOut& o)
     double volumeVar = pow13(absVolumeRation);
     dvec3 center = (1.0 / 4.0) * (vp[0] + vp[1] + vp[2] + vp[3]);
     for(int k = 0; k < 4; ++k) vpp[k] = center + volumeVar * (vpp[k] - center);
+}
+
+static void getmeTet(const Mesh& m, uint32_t e, ElemOut& o)
+{
+    const Crew& c = m.crew;
+    uint32_t vi[4]; dvec3 vp[4], vpp[4];
+    for(int k = 0; k < 4; ++k) { vi[k] = m.tets[e][k]; vp[k] = m.verts[vi[k]]; }
+    getmeTetTransform(c, vp, vpp);
     for(int k = 0; k < 4; ++k) if(isConstrained(m.topos[vi[k]])) vpp[k] = snap(m.topos[vi[k]], vpp[k]);
     double quality = tetQuality(c, vp);
     double qualityPrime = tetQuality(c, vpp);
@@ -1049,12 +1076,10 @@ static void getmeTet(const Mesh& m, uint32_t e, ElemOut& o)
     for(int k = 0; k < 4; ++k) o.vpp[k] = vpp[k];
 }
 
-static void getmeHex(const Mesh& m, uint32_t e, ElemOut& o)
+// GetmeSmoother::smoothHexs (:245-352), before snapping.
+static void getmeHexTransform(const Crew& c, const dvec3 vp[8], dvec3 vpp[8])
 {
     const double lambda = 0.78;
-    const Crew& c = m.crew;
-    uint32_t vi[8]; dvec3 vp[8];
-    for(int k = 0; k < 8; ++k) { vi[k] = m.hexs[e][k]; vp[k] = m.verts[vi[k]]; }
     dvec3 aux[6] = {
         (vp[0] + vp[1] + vp[2] + vp[3]) / 4.0, (vp[0] + vp[4] + vp[5] + vp[1]) / 4.0,
         (vp[1] + vp[5] + vp[6] + vp[2]) / 4.0, (vp[2] + vp[6] + vp[7] + vp[3]) / 4.0,
@@ -1069,7 +1094,6 @@ static void getmeHex(const Mesh& m, uint32_t e, ElemOut& o)
         (aux[0] + aux[3] + aux[2]) / 3.0, (aux[0] + aux[4] + aux[3]) / 3.0,
         (aux[5] + aux[4] + aux[1]) / 3.0, (aux[5] + aux[1] + aux[2]) / 3.0,
         (aux[5] + aux[2] + aux[3]) / 3.0, (aux[5] + aux[3] + aux[4]) / 3.0};
-    dvec3 vpp[8];
     for(int k = 0; k < 8; ++k) vpp[k] = bases[k] + n[k] * (lambda / std::sqrt(length(n[k])));
     double volume = hexVolume(c, vp);
     double volumePrime = hexVolume(c, vpp);
@@ -1077,6 +1101,14 @@ static void getmeHex(const Mesh& m, uint32_t e, ElemOut& o)
     double volumeVar = pow13(absVolumeRation);
     dvec3 center = (1.0 / 8.0) * (vp[0] + vp[1] + vp[2] + vp[3] + vp[4] + vp[5] + vp[6] + vp[7]);
     for(int k = 0; k < 8; ++k) vpp[k] = center + volumeVar * (vpp[k] - center);
+}
+
+static void getmeHex(const Mesh& m, uint32_t e, ElemOut& o)
+{
+    const Crew& c = m.crew;
+    uint32_t vi[8]; dvec3 vp[8], vpp[8];
+    for(int k = 0; k < 8; ++k) { vi[k] = m.hexs[e][k]; vp[k] = m.verts[vi[k]]; }
+    getmeHexTransform(c, vp, vpp);
     for(int k = 0; k < 8; ++k) if(isConstrained(m.topos[vi[k]])) vpp[k] = snap(m.topos[vi[k]], vpp[k]);
     double quality = hexQuality(c, vp);
     double qualityPrime = hexQuality(c, vpp);
@@ -1307,6 +1339,50 @@ double orc_hex_volume(void* h, const double* p)
     dvec3 vp[8];
     for(int k = 0; k < 8; ++k) vp[k] = dvec3(p[3*k], p[3*k+1], p[3*k+2]);
     return hexVolume(((Mesh*)h)->crew, vp);
+}
+
+// Runs one vertex-wise smoother on vertex vId and records every patch
+// evaluation: out[4 i .. 4 i + 3] = (x, y, z, quality) of evaluation i, at
+// most cap entries; returns the number of evaluations.
+int64_t orc_trace_vertex(void* h, int smoother, double spawnCoeff, uint32_t vId, double* out, int64_t cap)
+{
+    Mesh* m = (Mesh*)h;
+    std::vector<double> trace;
+    g_trace = &trace;
+    switch(smoother)
+    {
+    case 0: nelderMeadVertex(*m, vId, nullptr); break;
+    case 1: gradientDescentVertex(*m, vId, nullptr); break;
+    case 2: spawnSearchVertex(*m, vId, spawnCoeff, nullptr); break;
+    }
+    g_trace = nullptr;
+    const int64_t n = (int64_t)trace.size() / 4;
+    std::memcpy(out, trace.data(), (size_t)std::min(n, cap) * 4 * sizeof(double));
+    return n;
+}
+
+// GETMe element step on free-standing vertices (no snapping): kind 0 = tet
+// (4 vertices), 1 = hex (8); vpp = transformed positions, *weight = q'/(1+q).
+void orc_getme_element(void* h, int kind, const double* p, double* vppOut, double* weight)
+{
+    const Crew& c = ((Mesh*)h)->crew;
+    const int n = kind == 0 ? 4 : 8;
+    dvec3 vp[8], vpp[8];
+    for(int k = 0; k < n; ++k) vp[k] = dvec3(p[3*k], p[3*k+1], p[3*k+2]);
+    if(kind == 0) getmeTetTransform(c, vp, vpp); else getmeHexTransform(c, vp, vpp);
+    const double q = kind == 0 ? tetQuality(c, vp) : hexQuality(c, vp);
+    const double qp = kind == 0 ? tetQuality(c, vpp) : hexQuality(c, vpp);
+    *weight = qp / (1.0 + q);
+    for(int k = 0; k < n; ++k) { vppOut[3*k] = vpp[k].x; vppOut[3*k+1] = vpp[k].y; vppOut[3*k+2] = vpp[k].z; }
+}
+
+// MetricConformityEvaluator::specifiedMetric of a tetrahedron, column-major.
+void orc_specified_metric(void* h, const double* p, double* out9)
+{
+    dvec3 v[4];
+    for(int k = 0; k < 4; ++k) v[k] = dvec3(p[3*k], p[3*k+1], p[3*k+2]);
+    dmat3 M = specifiedMetric(((Mesh*)h)->crew, v[0], v[1], v[2], v[3]);
+    for(int j = 0; j < 3; ++j) for(int i = 0; i < 3; ++i) out9[3*j+i] = M[j][i];
 }
 
 // Smooths one vertex-wise group list (tests of single-vertex behaviour).

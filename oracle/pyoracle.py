@@ -67,6 +67,10 @@ def lib():
     L.orc_metric_at.argtypes = [vp, _P(np.float64), _P(np.float64)]
     L.orc_riemannian_distance.argtypes = [vp, _P(np.float64), _P(np.float64)]
     L.orc_riemannian_distance.restype = C.c_double
+    L.orc_trace_vertex.argtypes = [vp, C.c_int, C.c_double, C.c_uint32, _P(np.float64), i64]
+    L.orc_trace_vertex.restype = i64
+    L.orc_getme_element.argtypes = [vp, C.c_int, _P(np.float64), _P(np.float64), _P(np.float64)]
+    L.orc_specified_metric.argtypes = [vp, _P(np.float64), _P(np.float64)]
     L.orc_smooth_vertices.argtypes = [vp, C.c_int, C.c_double, _P(np.uint32), i64, vp]
     L.orc_smooth.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, vp, vp, vp]
     L.orc_smooth.restype = C.c_int
@@ -212,6 +216,28 @@ class OracleMesh:
     def riemannian_distance(self, a, b) -> float:
         return lib().orc_riemannian_distance(self._h, np.ascontiguousarray(a, np.float64),
                                              np.ascontiguousarray(b, np.float64))
+
+    def trace_vertex(self, smoother: str, v: int, spawn_coeff: float = 1.0 / 20.0) -> np.ndarray:
+        """Smooths vertex v alone and returns every patch evaluation it made,
+        in order, as rows (x, y, z, quality)."""
+        cap = 4096
+        out = np.empty(4 * cap, np.float64)
+        n = int(lib().orc_trace_vertex(self._h, SMOOTHERS[smoother], float(spawn_coeff), int(v), out, cap))
+        return out[: 4 * min(n, cap)].reshape(-1, 4)
+
+    def getme_element(self, vp):
+        """GETMe element transform of free-standing vertices: (vpp, weight)."""
+        vp = np.ascontiguousarray(vp, np.float64).reshape(-1, 3)
+        out = np.empty(vp.size, np.float64)
+        w = np.empty(1, np.float64)
+        lib().orc_getme_element(self._h, 0 if vp.shape[0] == 4 else 1, vp.reshape(-1), out, w)
+        return out.reshape(-1, 3), float(w[0])
+
+    def specified_metric(self, vp) -> np.ndarray:
+        """Keast-rule metric of a tetrahedron (MetricConformityEvaluator::specifiedMetric)."""
+        out = np.empty(9, np.float64)
+        lib().orc_specified_metric(self._h, np.ascontiguousarray(vp, np.float64).reshape(-1), out)
+        return out.reshape(3, 3).T
 
     def smooth_vertices(self, smoother: str, vids, spawn_coeff: float = 1.0 / 20.0):
         vids = np.ascontiguousarray(vids, np.uint32)

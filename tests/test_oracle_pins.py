@@ -1,0 +1,682 @@
+"""Pins the oracle's optimisers, colouring, Keast rule and adaptive walk.
+
+Every check here is computed by code written from the reference's text and
+from mathematics, not from the oracle:
+
+* element qualities come from closed forms (tetrahedron: 12 (3|V|)^(2/3) /
+  sum of squared edges; hexahedron: corner Jacobians read off the reference
+  cube's geometry), patch qualities from the harmonic / min-sticky rule;
+* the four optimisers are re-derived from their reference sources
+  (NelderMeadSmoother.cpp:100-261, GradientDescentSmoother.cpp:73-202,
+  SpawnSearchSmoother.cpp:37-55 and :161-226, GetmeSmoother.cpp:55-127 and
+  :245-374 with AbstractElementWiseSmoother.cpp:406-444) and driven by those
+  closed forms; the oracle's trace of every patch evaluation (position and
+  value) must follow the same trial points, in the same order, to rounding;
+* colourings of tiny meshes are derived by hand from NodeGroups.cpp:243-363;
+* the Keast rule must integrate a trilinear metric exactly (it is exact to
+  degree 3), and the metric-wise walk must land within the error band that
+  its 1 % pair rule implies for a smooth field.
+
+A plausible slip in the oracle (swapped Nelder-Mead coefficients, `<` for
+`<=`, a wrong node shift, lattice order or offset table, a dropped Keast
+term, left-point instead of mid-point sampling, another threshold) changes a
+trial point or a value and fails one of these.
+"""
+import itertools
+
+import numpy as np
+import pytest
+
+from gpumesh_amd import cube_mesh
+
+# ---------------------------------------------------------------------------
+# Closed-form qualities.
+# ---------------------------------------------------------------------------
+
+# Reference corners of the hexahedron: v0..v3 counter-clockwise at z = 0,
+# v4..v7 above them (DebugMesher.cpp:285-293).
+HEX_REF = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0], [0, 0, 1], [1, 0, 1], [1, 1, 1], [0, 1, 1]])
+
+
+def tet_q(p):
+    """Mean ratio of a tetrahedron from its volume and edge lengths, signed by
+    the orientation det(v3-v0, v3-v1, v3-v2)."""
+    d = np.linalg.det(np.stack([p[3] - p[0], p[3] - p[1], p[3] - p[2]], axis=1))
+    l2 = sum(np.dot(p[i] - p[j], p[i] - p[j]) for i, j in itertools.combinations(range(4), 2))
+    return np.sign(d) * 12.0 * (3.0 * abs(d) / 6.0) ** (2.0 / 3.0) / l2
+
+
+def _hex_corner_frames():
+    frames = []
+    for k in range(8):
+        cols = []
+        for a in range(3):
+            r = HEX_REF[k].copy()
+            r[a] = 1 - r[a]
+            j = int(np.where((HEX_REF == r).all(axis=1))[0][0])
+            cols.append((j, 1.0 if r[a] > HEX_REF[k][a] else -1.0))
+        frames.append(cols)
+    return frames
+
+
+HEX_FRAMES = _hex_corner_frames()
+
+
+def hex_q(p):
+    """Mean ratio of a hexahedron: the corner Jacobian F_k maps the reference
+    corner's axis frame onto the physical edges; q_k = 3 |det F|^(2/3) /
+    |F|^2 signed by det F; harmonic mean of the corners, or the smallest
+    corner when one is not positive."""
+    q = []
+    for k, cols in enumerate(HEX_FRAMES):
+        F = np.stack([(p[j] - p[k]) * s for j, s in cols], axis=1)
+        d = np.linalg.det(F)
+        q.append(np.sign(d) * 3.0 * abs(d) ** (2.0 / 3.0) / np.sum(F * F))
+    q = np.array(q)
+    return q.min() if (q <= 0).any() else 8.0 / np.sum(1.0 / q)
+
+
+def patch_q(qs):
+    """AbstractEvaluator::accumulatePatchQuality / finalizePatchQuality."""
+    w, acc = 0.0, 0.0
+    for q in qs:
+        if w != -1.0 and q > 0:
+            w += 1.0
+            acc += 1.0 / q
+        else:
+            w = -1.0
+            acc = min(acc, q)
+    return w / acc if w > 0 else acc
+
+
+class Patch:
+    """One vertex of a cube mesh with its patch, in the mesh's own numbering."""
+
+    def __init__(self, mesh, v):
+        self.pos = mesh.pos.copy()
+        self.v = v
+        self.kind = "tet" if mesh.tets.shape[0] else "hex"
+        elems = mesh.tets if self.kind == "tet" else mesh.hexs
+        self.elems = elems[np.any(elems == v, axis=1)]
+        nbr = set()
+        for e in self.elems:
+            k = int(np.where(e == v)[0][0])
+            if self.kind == "tet":
+                nbr.update(int(x) for x in e if x != v)
+            else:
+                nbr.update(int(e[j]) for j, _ in HEX_FRAMES[k])
+        self.nbr = sorted(nbr)
+
+    def quality(self, x):
+        p = self.pos.copy()
+        p[self.v] = x
+        f = tet_q if self.kind == "tet" else hex_q
+        return patch_q([f(p[e]) for e in self.elems])
+
+    def local_size(self):
+        x = self.pos[self.v]
+        return sum(np.linalg.norm(x - self.pos[n]) for n in self.nbr) / len(self.nbr)
+
+
+class Recorder:
+    """Evaluates the patch quality at trial points and keeps them in order,
+    with the smallest gap seen by any comparison that decided a branch."""
+
+    def __init__(self, patch):
+        self.patch = patch
+        self.points = []
+        self.margin = np.inf
+
+    def __call__(self, x):
+        q = self.patch.quality(x)
+        self.points.append(np.r_[x, q])
+        return q
+
+    def cmp(self, a, b):
+        self.margin = min(self.margin, abs(a - b))
+
+
+# ---------------------------------------------------------------------------
+# Optimisers re-derived from the reference sources.
+# ---------------------------------------------------------------------------
+
+def nelder_mead(patch, rec):
+    """NelderMeadSmoother::smoothVertices (NelderMeadSmoother.cpp:100-261),
+    constants :17-23 (alpha 1, beta 0.5, gamma 2, 12 cycles, 1e-4, L/24)."""
+    alpha, beta, gamma = 1.0, 0.5, 2.0
+    shift = patch.local_size() / 24.0
+    x0 = patch.pos[patch.v].copy()
+    vo = (x0, rec(x0))
+    simplex = [(x0 + shift * np.eye(3)[i], 0.0) for i in range(3)] + [vo]
+    cycle, reset = 0, False
+    while True:
+        for i in range(3):
+            simplex[i] = (simplex[i][0], rec(simplex[i][0]))
+        for a, b in [(0, 1), (1, 2), (2, 3), (0, 1), (1, 2), (0, 1)]:
+            rec.cmp(simplex[a][1], simplex[b][1])
+            if simplex[a][1] > simplex[b][1]:
+                simplex[a], simplex[b] = simplex[b], simplex[a]
+        terminated = False
+        while cycle < 12:
+            c = (simplex[1][0] + simplex[2][0] + simplex[3][0]) / 3.0
+            xr = c + alpha * (c - simplex[0][0])
+            fr = rec(xr)
+            x, f = xr, fr
+            rec.cmp(simplex[3][1], fr)
+            if simplex[3][1] < fr:
+                xe = c + gamma * (xr - c)
+                fe = rec(xe)
+                rec.cmp(fe, fr)
+                x, f = (xr, fr) if fe <= fr else (xe, fe)
+            else:
+                rec.cmp(simplex[1][1], fr)
+                if simplex[1][1] >= fr:
+                    rec.cmp(fr, simplex[0][1])
+                    if fr > simplex[0][1]:
+                        x = c + beta * (xr - c)
+                    else:
+                        x = c + beta * (simplex[0][0] - c)
+                    f = rec(x)
+            vertex = (x, f)
+            for k in (3, 2, 1, 0):
+                rec.cmp(vertex[1], simplex[k][1])
+                if vertex[1] > simplex[k][1]:
+                    simplex[k], vertex = vertex, simplex[k]
+            rec.cmp(simplex[3][1] - simplex[1][1], 1e-4)
+            if simplex[3][1] - simplex[1][1] < 1e-4:
+                terminated = True
+                break
+            cycle += 1
+        if terminated or (cycle >= 12 and reset):
+            break
+        simplex = [(vo[0] - shift * np.eye(3)[i], vo[1]) for i in range(3)] + [vo]
+        reset, cycle = True, 0
+    return simplex[3][0]
+
+
+def gradient_descent(patch, rec):
+    """GradientDescentSmoother::smoothVertices (GradientDescentSmoother.cpp:73-202):
+    5 cycles, node shift L/75, central differences, 8 line points."""
+    offsets = [-0.25, 0.0, 0.1, 0.2, 0.4, 0.8, 1.2, 1.6]
+    shift = patch.local_size() / 75.0
+    original = shift
+    x = patch.pos[patch.v].copy()
+    for _ in range(5):
+        s = [rec(x + sgn * shift * np.eye(3)[a]) for a in range(3) for sgn in (-1.0, 1.0)]
+        g = np.array([s[1] - s[0], s[3] - s[2], s[5] - s[4]])
+        gn = np.linalg.norm(g)
+        if gn == 0:
+            break
+        step = g * (shift / gn)
+        best, bq = 0, -np.inf
+        for i, o in enumerate(offsets):
+            q = rec(x + step * o)
+            rec.cmp(q, bq)
+            if q > bq:
+                best, bq = i, q
+        x = x + step * offsets[best]
+        shift *= abs(offsets[best])
+        if shift < original / 10.0:
+            break
+    return x
+
+
+def spawn_lattice():
+    """SpawnSearchSmoother.cpp:37-55: the 4^3 lattice {-1.5,-0.5,0.5,1.5}^3,
+    i fastest, with entry 0 replaced by the origin."""
+    off = [np.array([i, j, k], float) - 1.5 for k in range(4) for j in range(4) for i in range(4)]
+    off[0] = np.zeros(3)
+    return off
+
+
+def spawn_search(patch, rec, coeff):
+    """SpawnSearchSmoother::smoothVertices (SpawnSearchSmoother.cpp:161-226):
+    two sweeps of the lattice, the second at a third of the scale."""
+    scale = coeff * patch.local_size()
+    x = patch.pos[patch.v].copy()
+    lattice = spawn_lattice()
+    for _ in range(2):
+        props = [x + o * scale for o in lattice]
+        best, bq = 0, -np.inf
+        for i, p in enumerate(props):
+            q = rec(p)
+            rec.cmp(q, bq)
+            if q > bq:
+                best, bq = i, q
+        x = props[best]
+        scale /= 3.0
+    return x
+
+
+def _traced_cases(kind, seeds=(3, 4)):
+    for seed in seeds:
+        m = cube_mesh(3, kind, 0.3, seed=seed)
+        for v in np.where(m.fixed == 0)[0]:
+            yield m, int(v)
+
+
+def _check_trace(oracle, m, v, smoother, run, coeff=1.0 / 20.0):
+    om = oracle.OracleMesh.from_cube(m)
+    perm = om.permutation()
+    tr = om.trace_vertex(smoother, int(perm[v]), spawn_coeff=coeff)
+    patch = Patch(m, v)
+    rec = Recorder(patch)
+    final = run(patch, rec)
+    mine = np.array(rec.points)
+    # The comparisons that chose the trial points are far from ties, so the
+    # rounding differences between the closed forms and the oracle's
+    # formulas cannot flip a branch.
+    assert rec.margin > 1e-9, rec.margin
+    assert tr.shape == mine.shape, (tr.shape, mine.shape)
+    np.testing.assert_allclose(tr[:, :3], mine[:, :3], rtol=0, atol=1e-13)
+    np.testing.assert_allclose(tr[:, 3], mine[:, 3], rtol=1e-12, atol=1e-14)
+    np.testing.assert_allclose(om.positions()[perm[v]], final, rtol=0, atol=1e-13)
+    return tr
+
+
+@pytest.mark.parametrize("kind", ["tet", "hex"])
+def test_nelder_mead_trace(oracle, kind):
+    for m, v in _traced_cases(kind):
+        tr = _check_trace(oracle, m, v, "nm", nelder_mead)
+        # The first simplex: the start, then pos + (L/24) e_i.
+        L = Patch(m, v).local_size()
+        for i in range(3):
+            np.testing.assert_allclose(tr[1 + i, :3] - tr[0, :3], (L / 24.0) * np.eye(3)[i], atol=1e-15)
+
+
+@pytest.mark.parametrize("kind", ["tet", "hex"])
+def test_gradient_descent_trace(oracle, kind):
+    for m, v in _traced_cases(kind):
+        tr = _check_trace(oracle, m, v, "gd", gradient_descent)
+        # Cycle 1: six samples at +-L/75 along the axes, then eight points on
+        # the line along the central-difference gradient.
+        L = Patch(m, v).local_size()
+        x0 = m.pos[v]
+        s = tr[:6]
+        for a in range(3):
+            np.testing.assert_allclose(s[2 * a, :3] - x0, -(L / 75.0) * np.eye(3)[a], atol=1e-15)
+            np.testing.assert_allclose(s[2 * a + 1, :3] - x0, (L / 75.0) * np.eye(3)[a], atol=1e-15)
+        g = np.array([s[1, 3] - s[0, 3], s[3, 3] - s[2, 3], s[5, 3] - s[4, 3]])
+        line = tr[6:14, :3] - x0
+        expect = np.outer([-0.25, 0.0, 0.1, 0.2, 0.4, 0.8, 1.2, 1.6], g / np.linalg.norm(g) * (L / 75.0))
+        np.testing.assert_allclose(line, expect, atol=1e-15)
+
+
+@pytest.mark.parametrize("kind", ["tet", "hex"])
+@pytest.mark.parametrize("coeff", [1.0 / 20.0, 0.25])
+def test_spawn_search_brute_force(oracle, kind, coeff):
+    for m, v in _traced_cases(kind):
+        tr = _check_trace(oracle, m, v, "spawn", lambda p, r: spawn_search(p, r, coeff), coeff)
+        assert tr.shape[0] == 128
+        # The chosen point of each sweep is the first maximum of its 64.
+        for sweep in range(2):
+            q = tr[64 * sweep: 64 * sweep + 64, 3]
+            nxt = tr[64, :3] if sweep == 0 else None
+            best = int(np.argmax(q))
+            if nxt is not None:
+                lattice = spawn_lattice()
+                scale = coeff * Patch(m, v).local_size()
+                # The second sweep is centred on the first sweep's choice.
+                np.testing.assert_allclose(nxt - tr[best, :3], lattice[0] * scale / 3, atol=1e-15)
+
+
+def test_nelder_mead_keeps_a_stationary_vertex(oracle):
+    """The centre of an unjittered hexahedral cube is a maximum of its patch
+    quality (every corner is a right angle there): Nelder-Mead's trials all
+    score lower, the simplex never replaces the start, and the vertex is
+    returned unchanged."""
+    m = cube_mesh(2, "hex", 0.0, seed=1)
+    om = oracle.OracleMesh.from_cube(m)
+    v = om.nverts - 1
+    before = om.positions()[v].copy()
+    tr = om.trace_vertex("nm", v)
+    assert tr[0, 3] == pytest.approx(1.0, abs=1e-15)
+    assert np.all(tr[1:, 3] < tr[0, 3])
+    assert np.array_equal(om.positions()[v], before)
+
+
+# ---------------------------------------------------------------------------
+# GETMe.
+# ---------------------------------------------------------------------------
+REG_TET = np.array([[0, 0, 0], [1, 0, 0], [0.5, np.sqrt(3) / 2, 0], [0.5, np.sqrt(3) / 6, np.sqrt(2.0 / 3)]])
+
+
+def getme_tet(vp, lam=0.78):
+    """GetmeSmoother::smoothTets (GetmeSmoother.cpp:55-127) without snapping:
+    vertex k moves along the normal of its opposite face scaled by
+    lambda / sqrt|n|, then the element is scaled about its centroid back to
+    its volume."""
+    n = [np.cross(vp[3] - vp[1], vp[2] - vp[1]), np.cross(vp[3] - vp[2], vp[0] - vp[2]),
+         np.cross(vp[1] - vp[3], vp[0] - vp[3]), np.cross(vp[1] - vp[0], vp[2] - vp[0])]
+    vpp = np.array([vp[k] + n[k] * (lam / np.sqrt(np.linalg.norm(n[k]))) for k in range(4)])
+    vol = lambda p: np.linalg.det(np.stack([p[3] - p[0], p[3] - p[1], p[3] - p[2]], axis=1)) / 6.0
+    s = abs(vol(vp) / vol(vpp)) ** (1.0 / 3.0)
+    c = vp.mean(axis=0)
+    return c + s * (vpp - c)
+
+
+def getme_hex(vp, lam=0.78):
+    """GetmeSmoother::smoothHexs (GetmeSmoother.cpp:245-374) without snapping:
+    the six face centres span an octahedron; vertex k moves from the centre
+    of its octahedron face along that face's normal, then the element is
+    scaled about its centroid back to its volume."""
+    f = [vp[[0, 1, 2, 3]], vp[[0, 4, 5, 1]], vp[[1, 5, 6, 2]], vp[[2, 6, 7, 3]], vp[[0, 3, 7, 4]], vp[[4, 7, 6, 5]]]
+    a = [x.mean(axis=0) for x in f]
+    tri = [(0, 1, 4), (0, 2, 1), (0, 3, 2), (0, 4, 3), (5, 4, 1), (5, 1, 2), (5, 2, 3), (5, 3, 4)]
+    vpp = []
+    for i, j, k in tri:
+        n = np.cross(a[j] - a[i], a[k] - a[i])
+        vpp.append((a[i] + a[j] + a[k]) / 3.0 + n * (lam / np.sqrt(np.linalg.norm(n))))
+    vpp = np.array(vpp)
+
+    def vol(p):   # five-tetrahedron split
+        t = [(0, 1, 3, 4), (1, 2, 3, 6), (1, 4, 5, 6), (3, 4, 6, 7), (1, 3, 4, 6)]
+        return sum(abs(np.linalg.det(np.stack([p[b] - p[a], p[c] - p[a], p[d] - p[a]], axis=1))) for a, b, c, d in t) / 6.0
+
+    s = abs(vol(vp) / vol(vpp)) ** (1.0 / 3.0)
+    c = vp.mean(axis=0)
+    return c + s * (vpp - c)
+
+
+@pytest.mark.parametrize("kind", ["tet", "hex"])
+def test_getme_regular_elements_are_fixed_points(oracle, kind):
+    """A regular tetrahedron and a cube map onto themselves, with weight
+    q'/(1+q) = 1/2."""
+    om = oracle.OracleMesh.from_cube(cube_mesh(1, "tet", 0.0, seed=1))
+    vp = REG_TET if kind == "tet" else HEX_REF.astype(float)
+    rng = np.random.default_rng(0)
+    R, _ = np.linalg.qr(rng.normal(size=(3, 3)))
+    if np.linalg.det(R) < 0:
+        R[:, 0] = -R[:, 0]
+    for p in (vp, vp @ R.T * 2.5 + 0.3):
+        vpp, w = om.getme_element(p)
+        np.testing.assert_allclose(vpp, p, atol=1e-14)
+        assert w == pytest.approx(0.5, abs=1e-14)
+
+
+@pytest.mark.parametrize("kind", ["tet", "hex"])
+def test_getme_element_transform(oracle, kind):
+    """Distorted elements: the transform equals the one derived from the
+    reference's text, keeps the volume, and improves the quality."""
+    om = oracle.OracleMesh.from_cube(cube_mesh(1, "tet", 0.0, seed=1))
+    rng = np.random.default_rng(12)
+    base = REG_TET if kind == "tet" else HEX_REF.astype(float)
+    f = tet_q if kind == "tet" else hex_q
+    mine = getme_tet if kind == "tet" else getme_hex
+    for _ in range(40):
+        p = base + rng.uniform(-0.2, 0.2, size=base.shape)
+        vpp, w = om.getme_element(p)
+        np.testing.assert_allclose(vpp, mine(p), rtol=0, atol=1e-13)
+        assert w == pytest.approx(f(vpp) / (1.0 + f(p)), rel=1e-12)
+        if kind == "tet":
+            vol = lambda x: np.linalg.det(np.stack([x[3] - x[0], x[3] - x[1], x[3] - x[2]], axis=1))
+            assert vol(vpp) == pytest.approx(vol(p), rel=1e-12)
+        assert f(vpp) > f(p)
+
+
+@pytest.mark.parametrize("kind", ["tet", "hex"])
+def test_getme_single_vertex_passes(oracle, kind):
+    """One free vertex: each pass moves it to the weight-averaged transformed
+    positions of its patch elements (fixed vertices snapped back), unless
+    the patch quality drops (AbstractElementWiseSmoother.cpp:406-444)."""
+    m = cube_mesh(2, kind, 0.0, seed=1)
+    pos = m.pos.copy()
+    v = int(np.where(m.fixed == 0)[0][0])
+    pos[v] += [0.11, -0.07, 0.05]
+    m.pos[:] = pos
+    om = oracle.OracleMesh(pos, m.fixed, m.tets, m.hexs)
+    pv = int(om.permutation()[v])
+    patch = Patch(m, v)
+    mine = getme_tet if kind == "tet" else getme_hex
+    x = pos[v].copy()
+    for passes in range(1, 4):
+        om.smooth("getme", 1, threads=1)
+        cur = patch.pos.copy()
+        cur[v] = x
+        acc, wsum = np.zeros(3), 0.0
+        f = tet_q if kind == "tet" else hex_q
+        for e in patch.elems:
+            vp = cur[e]
+            vpp = mine(vp)
+            for k, idx in enumerate(e):
+                if idx != v:
+                    vpp[k] = cur[idx]           # fixed vertices snap back
+            w = f(vpp) / (1.0 + f(vp))
+            k = int(np.where(e == v)[0][0])
+            acc += vpp[k] * w
+            wsum += w
+        cand = acc / wsum
+        if patch.quality(cand) >= patch.quality(x):
+            x = cand
+        np.testing.assert_allclose(om.positions()[pv], x, rtol=0, atol=1e-13)
+    assert patch.quality(x) > patch.quality(pos[v])
+
+
+# ---------------------------------------------------------------------------
+# Colouring of tiny meshes, derived by hand from NodeGroups.cpp:243-363:
+# breadth-first from the lowest unvisited id, each vertex taking the smallest
+# colour that no already coloured element-mate holds (a new colour if none).
+# ---------------------------------------------------------------------------
+TINY = {
+    "one tet": ([(0, 1, 2, 3)], [], 4, [0, 1, 2, 3]),
+    "two tets": ([(0, 1, 2, 3), (1, 2, 3, 4)], [], 5, [0, 1, 2, 3, 0]),
+    "tet chain": ([(0, 1, 2, 3), (2, 3, 4, 5), (4, 5, 6, 7)], [], 8, [0, 1, 2, 3, 0, 1, 2, 3]),
+    # Breadth-first order differs from id order (5 is coloured before 1),
+    # and vertex 4 belongs to no element (a second search start).
+    "bfs order": ([(0, 5, 6, 7), (5, 1, 2, 3)], [], 8, [0, 0, 2, 3, 0, 1, 2, 3]),
+    "two hexes": ([], [(0, 1, 2, 3, 4, 5, 6, 7), (1, 8, 9, 2, 5, 10, 11, 6)], 12,
+                  [0, 1, 2, 3, 4, 5, 6, 7, 0, 3, 4, 7]),
+}
+
+
+def _colours(groups, perm, nverts):
+    colour_new = -np.ones(nverts, int)
+    for g, r in enumerate(groups):
+        colour_new[r[4]:r[5]] = g
+    return colour_new[perm]
+
+
+@pytest.mark.parametrize("name", sorted(TINY))
+def test_colouring_by_hand(oracle, name):
+    from gpumesh_amd.mesh import GpuMesh
+    tets, hexs, nv, expect = TINY[name]
+    rng = np.random.default_rng(1)
+    pos = rng.uniform(size=(nv, 3))
+    fixed = np.zeros(nv, np.int32)
+    t = np.array(tets, np.uint32).reshape(-1, 4)
+    h = np.array(hexs, np.uint32).reshape(-1, 8)
+    om = oracle.OracleMesh(pos, fixed, t, h)
+    groups, _ = om.groups()
+    perm = om.permutation()
+    assert _colours(groups, perm, nv).tolist() == expect
+    # The product's host topology gives the same colours.
+    gm = GpuMesh(pos, fixed, t, h, device=False)
+    off, nodes = gm.groups()
+    colour_new = -np.ones(nv, int)
+    for g in range(gm.ngroups):
+        colour_new[off[g]:off[g + 1]] = g
+    assert colour_new[gm.permutation()].tolist() == expect
+    # Within a colour, vertices are ordered by patch size
+    # (NodeGroups.cpp:365-450): in "bfs order" the isolated vertex 4 comes
+    # first among colour 0.
+    if name == "bfs order":
+        assert perm[4] == 0 and sorted([perm[0], perm[1]]) == [1, 2]
+
+
+# ---------------------------------------------------------------------------
+# Keast rule and the metric-wise walk under varying metrics.
+# ---------------------------------------------------------------------------
+
+def _texture(n, field):
+    """Texels of `field` (6 components at points (..., 3)) at the cell
+    centres of an n^3 grid on the unit cube, FP32, in the sampler's layout."""
+    idx = np.stack(np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij"), -1)
+    centres = (idx + 0.5) / n
+    vals = field(centres)                                  # (6, i, j, k)
+    tex = np.transpose(vals, (3, 2, 1, 0)).astype(np.float32)
+    assert np.array_equal(tex.astype(np.float64), np.transpose(vals, (3, 2, 1, 0)))
+    return (np.array([n, n, n], np.int32), np.zeros(3), np.ones(3), tex.reshape(-1))
+
+
+# Six trilinear components (m00 m01 m02 m11 m12 m22) with dyadic coefficients
+# of the monomials 1, x, y, z, xy, xz, yz, xyz; positive definite on the cube.
+TRILINEAR = np.array([
+    [4.0, 1.0, 0.5, 0.25, 1.0, 0.5, 0.0, 2.0],
+    [0.5, 0.25, 0.0, 0.5, 0.0, 0.25, 0.5, 0.0],
+    [0.25, 0.0, 0.5, 0.0, 0.25, 0.0, 0.0, 0.5],
+    [3.0, 0.5, 1.0, 0.0, 0.5, 1.0, 0.25, 1.0],
+    [0.0, 0.25, 0.0, 0.25, 0.0, 0.5, 0.0, 0.25],
+    [5.0, 0.0, 0.25, 1.0, 0.0, 0.0, 1.0, 0.5],
+])
+
+
+def _trilinear(p):
+    x, y, z = p[..., 0], p[..., 1], p[..., 2]
+    mono = [np.ones_like(x), x, y, z, x * y, x * z, y * z, x * y * z]
+    return np.array([sum(c * m for c, m in zip(row, mono)) for row in TRILINEAR])
+
+
+def _poly_mul(a, b):
+    out = {}
+    for ea, ca in a.items():
+        for eb, cb in b.items():
+            e = tuple(i + j for i, j in zip(ea, eb))
+            out[e] = out.get(e, 0.0) + ca * cb
+    return out
+
+
+def _tet_mean_exact(vp, coeffs):
+    """Exact mean over a tetrahedron of sum c_m * monomial_m (the trilinear
+    basis above): each coordinate is affine in the barycentric coordinates
+    (l1, l2, l3), products are expanded, and the monomials integrate over the
+    unit simplex to i! j! k! / (i + j + k + 3)!."""
+    from math import factorial
+    axes = []
+    for a in range(3):
+        axes.append({(0, 0, 0): vp[0][a], (1, 0, 0): vp[1][a] - vp[0][a],
+                     (0, 1, 0): vp[2][a] - vp[0][a], (0, 0, 1): vp[3][a] - vp[0][a]})
+    one = {(0, 0, 0): 1.0}
+    X, Y, Z = axes
+    basis = [one, X, Y, Z, _poly_mul(X, Y), _poly_mul(X, Z), _poly_mul(Y, Z), _poly_mul(_poly_mul(X, Y), Z)]
+    total = 0.0
+    for c, poly in zip(coeffs, basis):
+        for (i, j, k), v in poly.items():
+            total += c * v * factorial(i) * factorial(j) * factorial(k) / factorial(i + j + k + 3)
+    return total * 6.0      # divided by the unit simplex volume 1/6
+
+
+def test_keast_rule_integrates_a_trilinear_metric(oracle):
+    """MetricConformityEvaluator::specifiedMetric (:25-45) is the 5-point
+    Keast rule, exact for cubic polynomials: under a trilinear texture field
+    it returns the exact mean of the metric over the tetrahedron."""
+    om = oracle.OracleMesh.from_cube(cube_mesh(1, "tet", 0.0, seed=1))
+    om.set_crew("metric_conformity", "texture", texture=_texture(8, _trilinear))
+    rng = np.random.default_rng(21)
+    lo, hi = 1.0 / 16, 1.0 - 1.0 / 16
+    for _ in range(30):
+        c = rng.uniform(lo + 0.15, hi - 0.15, 3)
+        vp = c + rng.uniform(-0.15, 0.15, size=(4, 3))
+        Ms = om.specified_metric(vp)
+        comp = [_tet_mean_exact(vp, row) for row in TRILINEAR]
+        expect = np.array([[comp[0], comp[1], comp[2]], [comp[1], comp[3], comp[4]], [comp[2], comp[4], comp[5]]])
+        np.testing.assert_allclose(Ms, expect, rtol=0, atol=1e-13 * np.abs(expect).max())
+        # The mean differs from the value at the centroid: the quadratic
+        # and cubic terms matter, so a rule exact only for affine fields fails.
+        centroid = _trilinear(vp.mean(axis=0))
+        assert np.abs(comp - centroid).max() > 1e-6
+
+
+def test_metric_wise_walk_error_band(oracle):
+    """MetricWiseMeasurer::riemannianDistance (:65-116) under M = c(x) I with
+    c = 1 + 3x: the length is |d| (2/3)(c_b^(3/2) - c_a^(3/2)) / (c_b - c_a).
+    Mid-point samples accepted when adjacent halves agree within 1 % leave a
+    relative error between 1e-7 and 1e-5 on these segments; end-point
+    samples miss by about 3e-3, a 10 % threshold by about 2e-4 and a 0.1 %
+    one by about 2e-8."""
+    om = oracle.OracleMesh.from_cube(cube_mesh(1, "tet", 0.0, seed=1))
+
+    def field(p):
+        c = 1.0 + 3.0 * p[..., 0]
+        z = np.zeros_like(c)
+        return np.array([c, z, z, c, z, c])
+
+    om.set_crew("mean_ratio", "texture", texture=_texture(16, field))
+    for a, b in [((0.05, .2, .3), (0.95, .4, .1)), ((0.9, .5, .5), (0.1, .5, .5)), ((0.3, .1, .2), (0.5, .9, .7)),
+                 ((0.06, .5, .5), (0.94, .52, .48))]:
+        a, b = np.array(a), np.array(b)
+        ca, cb = 1 + 3 * a[0], 1 + 3 * b[0]
+        exact = np.linalg.norm(b - a) * (2.0 / 3.0) * (cb ** 1.5 - ca ** 1.5) / (cb - ca)
+        err = (om.riemannian_distance(a, b) - exact) / exact
+        assert 1e-7 < err < 1e-5, err
+    # A segment across which the metric is constant (x fixed) is exact.
+    a, b = np.array([0.4, 0.1, 0.1]), np.array([0.4, 0.9, 0.8])
+    assert om.riemannian_distance(a, b) == pytest.approx(np.sqrt(1 + 1.2) * np.linalg.norm(b - a), rel=1e-14)
+
+
+# ---------------------------------------------------------------------------
+# Whole passes: the vertex steps above composed in colour order, and the
+# pass records (AbstractSmoother.cpp:103-211, QualityHistogram.cpp:63-92).
+# ---------------------------------------------------------------------------
+
+def _records(pos, elems, kind):
+    f = tet_q if kind == "tet" else hex_q
+    q = np.array([f(pos[e]) for e in elems])
+    hist = np.bincount(np.clip(np.floor(q * 20).astype(int), 0, 19), minlength=20)
+    return q.min(), len(q) / np.sum(1.0 / q), hist
+
+
+@pytest.mark.parametrize("smoother", ["nm", "gd", "spawn", "getme"])
+@pytest.mark.parametrize("kind", ["tet", "hex"])
+def test_passes_compose_vertex_steps(oracle, smoother, kind):
+    """Two passes over a 3^3 cube (8 free vertices): each pass visits the
+    colour groups in order and moves each vertex in place (GETMe: element
+    step from the pass-start positions, then the vertex step), and each
+    pass record holds the minimum, harmonic mean and 20-bucket histogram of
+    the element qualities.  The colour order is the oracle's; colouring
+    itself is pinned by the hand-derived cases above."""
+    m = cube_mesh(3, kind, 0.3, seed=5)
+    om = oracle.OracleMesh.from_cube(m)
+    perm = om.permutation()
+    inv = np.empty_like(perm)
+    inv[perm] = np.arange(m.nverts)
+    groups, _ = om.groups()
+    order = [int(inv[i]) for g in groups for i in range(g[4], g[5])]
+    coeff = 0.25
+    ref = om.smooth(smoother, 2, threads=1, spawn_coeff=coeff)
+    elems = m.tets if kind == "tet" else m.hexs
+    pos = m.pos.copy()
+    f = tet_q if kind == "tet" else hex_q
+    for p in range(3):
+        mn, mean, hist = _records(pos, elems, kind)
+        assert ref["min"][p] == pytest.approx(mn, rel=1e-12)
+        assert ref["mean"][p] == pytest.approx(mean, rel=1e-12)
+        assert np.array_equal(ref["hist"][p], hist)
+        if p == 2:
+            break
+        if smoother == "getme":
+            start = pos.copy()
+            xform = getme_tet if kind == "tet" else getme_hex
+            out = []
+            for e in elems:
+                vpp = xform(start[e])
+                vpp[m.fixed[e] != 0] = start[e][m.fixed[e] != 0]
+                out.append((vpp, f(vpp) / (1.0 + f(start[e]))))
+        for v in order:
+            mesh = cube_mesh(3, kind, 0.3, seed=5)
+            mesh.pos[:] = pos
+            patch = Patch(mesh, v)
+            if smoother == "getme":
+                acc, ws = np.zeros(3), 0.0
+                for i in np.where(np.any(elems == v, axis=1))[0]:
+                    vpp, w = out[i]
+                    acc += vpp[int(np.where(elems[i] == v)[0][0])] * w
+                    ws += w
+                cand = acc / ws
+                if not patch.quality(cand) < patch.quality(pos[v]):
+                    pos[v] = cand
+                continue
+            rec = Recorder(patch)
+            run = {"nm": nelder_mead, "gd": gradient_descent,
+                   "spawn": lambda a, b: spawn_search(a, b, coeff)}[smoother]
+            pos[v] = run(patch, rec)
+    np.testing.assert_allclose(om.positions()[perm], pos, rtol=0, atol=1e-13)
