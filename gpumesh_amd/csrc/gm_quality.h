@@ -218,6 +218,12 @@ template<typename T, int SA> GMQ m3<T> metricAt(const Crew& c, v3<T> p)
     return mdiag<T>(s2);
 }
 
+// A host build of these headers may define GMQ_WALK_STEP() to count the
+// adaptive walk's iterations; the library leaves it empty.
+#ifndef GMQ_WALK_STEP
+#define GMQ_WALK_STEP()
+#endif
+
 // MetricWiseMeasurer::riemannianDistance, localized segment division
 // (Measurers/MetricWiseMeasurer.cpp:65-116, DIFF_THRESHOLD :9).
 template<typename T, int SA> GMQ T riemannianDistanceWise(const Crew& c, v3<T> a, v3<T> b)
@@ -228,6 +234,7 @@ template<typename T, int SA> GMQ T riemannianDistanceWise(const Crew& c, v3<T> a
     v3<T> bv = d / (T)base;
     while(curr < base)
     {
+        GMQ_WALK_STEP();
         T p0 = ((T)curr + T(0.5)) / (T)base;
         T p1 = ((T)curr + T(1.5)) / (T)base;
         m3<T> M0 = metricAt<T, SA>(c, a + p0 * d);
