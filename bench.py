@@ -13,9 +13,18 @@ value = movable vertices x passes x smoothers / wall time of the timed passes
 (inputs resident in HBM; the per-pass FP64 quality reductions are inside the
 timed region).  The timed passes replay each pass body from a HIP graph (the
 mesh as a batch of one); an untimed eager run of the same passes, which must
-give identical results, supplies the per-launch kernel times.  Also reported: the plain FP64 single-mesh kernel (one lane
-per vertex) on the same workload, the roofline of the dominant kernel and the
-CPU oracle on a bounded sample of the same work.
+give identical results, supplies the per-launch kernel times.  Also reported:
+  * the plain FP64 single-mesh kernel (one lane per vertex) on the same
+    workload, the denominator of the speed-up;
+  * the exact FP64 kernels (the reference's operation order) and the
+    mixed-precision FP64 kernels for every pass (fp64_fast), with each run's
+    final min/mean against the exact FP64 run of the same passes; fp64_fast
+    is the mode whose quality agrees within 1e-6 ("equal_quality");
+  * the roofline of the dominant kernel: the work its formulation performs
+    per launch over its average launch time (HIP events), with the HBM bytes
+    of its committed rocprofv3 profile beside it;
+  * the CPU oracle (the reference's Thread mode on the workload mesh, Serial
+    mode on one pinned core on a smaller mesh).
 
 Two more legs run after it and are reported beside the headline value:
   * batched: 256 meshes (Kuhn tet or, with probability 0.3, hex cubes of n^3
@@ -59,45 +68,84 @@ FLOP_GETME_TET = 415.0
 # GETMe vertex update, per patch element pulled: 3 mul + 4 add.
 FLOP_GETME_PULL = 7.0
 # FLOP the FP32 kernels execute per tetrahedron evaluation on the affine
-# corner terms: determinant 6, squared norm 8, |d|^(-2/3) and 3/q 4,
-# accumulation 2.
+# corner terms (the formulation's own work, the unit of the roofline below):
+# determinant d0 + g.x 6 and squared norm c0 + h.x + s|x|^2 8 (these 14 on the
+# matrix cores in the spawn-search and gradient-descent kernels), |d|^(-2/3)
+# and 3/q 4, accumulation 2.
 FLOP_AFFINE_TET = 20.0
-# HBM bytes per launch of the dominant kernel, measured with rocprofv3 --pmc
-# FETCH_SIZE and WRITE_SIZE (each scaled by the same counter's reading for a
-# 1 GiB device copy: x2.0 and x1.0) on the 100^3 Kuhn cube.
-MEASURED_TRAFFIC = {("spawnMfmaKernel<0, 4>", 100): 109167442.0}
+# HBM bytes per launch of a kernel come from its committed rocprofv3 --pmc
+# summary (FETCH_SIZE and WRITE_SIZE, scaled by the same counters' reading for
+# a 1 GiB device copy); the summary is regenerated whenever the kernel changes.
+TRAFFIC_PROFILES = {("spawnMfmaKernel<0, 4>", 100): "profiles/spawn_mfma_tet100_hbm.md"}
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def algorithmic_flop(res, smoother, mesh):
-    """FLOP of the smoothing kernels of one run, from counted patch evaluations."""
-    patch = mesh.sum_patch / max(mesh.nmovable, 1)
-    evals = float(np.sum(res["eval_counts"]))
-    flop = evals * patch * FLOP_MR_TET_PATCH
-    if smoother == "getme":
-        passes = len(res["min"]) - 1
-        return flop, passes * mesh.ntets * FLOP_GETME_TET
-    return flop, 0.0
+def profiled_traffic(kernel, n):
+    """(bytes per launch, source) from the kernel's committed HBM profile, or (None, reason)."""
+    import re
+    rel = TRAFFIC_PROFILES.get((kernel, n))
+    if rel is None:
+        return None, "no HBM profile for this kernel and size"
+    try:
+        text = open(os.path.join(ROOT, rel)).read()
+    except OSError:
+        return None, f"{rel} not present"
+    m = re.search(r"HBM traffic per launch: [0-9.]+ MB \((\d+) bytes\)", text)
+    if not m:
+        return None, f"{rel} holds no traffic line"
+    return float(m.group(1)), f"{rel} (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE; profiled separately, not this run)"
 
 
-def cpu_baseline(n, passes, threads, smoothers, spawn_coeff):
-    """The FP64 CPU oracle (Thread mode, `threads` workers) on a bounded sample."""
+def patch_sizes(mesh):
+    """Elements in each vertex's patch (new numbering, the order of eval_counts)."""
+    eoff = mesh.topology()[0].astype(np.int64)
+    return np.diff(eoff)
+
+
+def evaluated_elements(res, sizes):
+    """Element evaluations of a run: each vertex's patch evaluations times its patch size."""
+    return float(np.dot(res["eval_counts"].astype(np.float64), sizes))
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(n, passes, threads, smoothers, spawn_coeff, cores=None):
+    """The FP64 CPU oracle (the reference's Thread mode with `threads` workers,
+    glibc libm) on a bounded sample: `passes` sweeps of each smoother on the
+    n^3 workload construction.  `cores` pins the process to those cores for
+    the timed runs (the reference's Serial mode runs on one)."""
     from gpumesh_amd import cube_mesh
     from oracle import pyoracle
     pyoracle.set_libm(0)
     m = cube_mesh(n, "tet", 0.35, seed=2)
     moved = 0
     t_total = 0.0
-    for sm in smoothers:
-        om = pyoracle.OracleMesh.from_cube(m)
-        t0 = time.perf_counter()
-        om.smooth(sm, passes, threads=threads, order=1, eval_chunks=threads, spawn_coeff=spawn_coeff)
-        t_total += time.perf_counter() - t0
-        moved += m.nmovable * passes
-    pyoracle.set_libm(1)
+    saved = os.sched_getaffinity(0) if hasattr(os, "sched_getaffinity") else None
+    try:
+        if cores is not None and saved is not None:
+            os.sched_setaffinity(0, cores)
+        for sm in smoothers:
+            om = pyoracle.OracleMesh.from_cube(m)
+            t0 = time.perf_counter()
+            om.smooth(sm, passes, threads=threads, order=1 if threads > 1 else 0, eval_chunks=threads,
+                      spawn_coeff=spawn_coeff)
+            t_total += time.perf_counter() - t0
+            moved += m.nmovable * passes
+    finally:
+        if cores is not None and saved is not None:
+            os.sched_setaffinity(0, saved)
+        pyoracle.set_libm(1)
     return moved / t_total, t_total, m
 
 
@@ -243,11 +291,13 @@ def main():
     ap.add_argument("--warmup", type=int, default=2, help="untimed passes per smoother before timing")
     ap.add_argument("--n", type=int, default=100, help="cells per cube edge")
     ap.add_argument("--smoothers", default="gd,getme,spawn")
-    ap.add_argument("--precision", default="fp32_polish", choices=["fp64", "fp32", "fp32_polish"])
+    ap.add_argument("--precision", default="fp32_polish", choices=["fp64", "fp32", "fp32_polish", "fp64_fast"])
+    ap.add_argument("--polish-passes", type=int, default=1, help="FP64 passes ending an fp32_polish run")
     ap.add_argument("--kernel", default="wave", choices=["plain", "wave"])
     ap.add_argument("--plain-steps", type=int, default=2, help="passes of the plain FP64 kernel per smoother")
-    ap.add_argument("--cpu-n", type=int, default=80, help="cube size of the CPU sample")
-    ap.add_argument("--cpu-passes", type=int, default=2, help="passes per smoother of the CPU sample")
+    ap.add_argument("--cpu-n", type=int, default=100, help="cube size of the CPU Thread-mode sample")
+    ap.add_argument("--cpu-passes", type=int, default=1, help="passes per smoother of the CPU sample")
+    ap.add_argument("--cpu-serial-n", type=int, default=40, help="cube size of the CPU Serial-mode sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batch-meshes", type=int, default=256)
     ap.add_argument("--batch-passes", type=int, default=10)
@@ -286,6 +336,7 @@ def main():
     spawn_coeff = 0.25
     from gpumesh_amd._lib import check, lib
     check(lib().gm_set_topology_cache(8 << 30), "gm_set_topology_cache")
+    check(lib().gm_set_polish_passes(args.polish_passes), "gm_set_polish_passes")
     if args.serve_only:
         print(json.dumps(serving_leg(args)))
         return
@@ -348,46 +399,47 @@ def main():
     if plain:
         plain_value = sum(gm.nmovable * p["passes"] for p in plain.values()) / sum(p["wall_s"] for p in plain.values())
 
-    # Dominant kernel class of the timed runs, and its roofline.
+    # Dominant kernel class of the timed runs, and its roofline.  `achieved`
+    # counts the work the kernel's formulation performs: for the affine FP32
+    # smoothing kernels FLOP_AFFINE_TET per tetrahedron evaluation (element
+    # evaluations counted exactly: each vertex's patch evaluations times its
+    # patch size), for the evaluation pass its compulsory bytes.  The rate of
+    # the reference's direct formula (FLOP_MR_TET_PATCH per evaluation) is
+    # reported beside it as a work-reduction ratio, not as the fraction.
+    sizes = patch_sizes(gm)
     best = None
     for sm, p in per.items():
         res = p["res"]
-        f_smooth, f_getme = algorithmic_flop(res, sm, gm)
         km, kn = res["kernel_ms"], res["kernel_launches"]
-        classes = {
-            "smooth_f32": (km["smooth_f32"], kn["smooth_f32"], None),
-            "getme_elem_f32": (km["getme_elem_f32"], kn["getme_elem_f32"], None),
-            "getme_vert_f32": (km["getme_vert_f32"], kn["getme_vert_f32"], None),
-            "evaluate": (km["evaluate"], kn["evaluate"], None),
-        }
-        for cls, (ms, launches, _) in classes.items():
-            if launches == 0:
-                continue
-            if best is None or ms > best["ms"]:
-                best = {"smoother": sm, "class": cls, "ms": ms, "launches": launches, "res": res,
-                        "f_smooth": f_smooth, "f_getme": f_getme}
+        for cls in ("smooth_f32", "getme_elem_f32", "getme_vert_f32", "evaluate"):
+            if kn[cls] and (best is None or km[cls] > best["ms"]):
+                best = {"smoother": sm, "class": cls, "ms": km[cls], "launches": kn[cls], "res": res}
     roofline = None
     if best is not None:
         avg_s = best["ms"] / best["launches"] / 1e3
         steps = args.steps
-        f32_passes = steps - 1 if args.precision == "fp32_polish" else steps
+        f32_passes = steps - min(args.polish_passes, steps) if args.precision == "fp32_polish" else steps
         if best["class"] == "smooth_f32":
-            # FP32 passes only (the last pass is the FP64 polish): scale the
-            # counted FLOP to the FP32 share of the passes.
-            flop_per_launch = best["f_smooth"] * (f32_passes / steps) / best["launches"]
-            achieved = flop_per_launch / avg_s / 1e12
+            # The FP32 share of the counted evaluations (spawn search makes the
+            # same 128 per vertex in every pass).
+            elems = evaluated_elements(best["res"], sizes) * (f32_passes / steps) / best["launches"]
             kname = {"spawn": "spawnMfmaKernel<0, 4>", "gd": "gdMfmaTetKernel<3>",
                      "nm": "nmQuadKernel<6>"}.get(best["smoother"], "smoothWaveKernel<float>")
-            executed = flop_per_launch * FLOP_AFFINE_TET / FLOP_MR_TET_PATCH / avg_s / 1e12
-            # The spawn-search and gradient-descent kernels put the corner
-            # terms' linear parts on the matrix cores; the FP32 peak is the
-            # same for the matrix cores and packed vector FP32.
-            roofline = {"bound": "mfma" if best["smoother"] in ("spawn", "gd") else "valu", "achieved": achieved,
-                        "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                        "frac": achieved / PEAK_FP32_TFLOPS,
-                        "traffic": MEASURED_TRAFFIC.get((kname, args.n)), "kernel": kname,
-                        "avg_launch_ms": avg_s * 1e3, "flop_per_launch": flop_per_launch,
-                        "executed_tflops": executed, "executed_frac_of_nonpacked_fp32": executed / (PEAK_FP32_TFLOPS / 2)}
+            flop = elems * FLOP_AFFINE_TET
+            achieved = flop / avg_s / 1e12
+            traffic, source = profiled_traffic(kname, args.n)
+            roofline = {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                        "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic, "traffic_source": source,
+                        "kernel": kname, "avg_launch_ms": avg_s * 1e3, "flop_per_launch": flop,
+                        "flop_unit": f"{FLOP_AFFINE_TET:.0f} FLOP per tetrahedron evaluation on affine corner terms "
+                                     f"({elems:.4g} element evaluations per launch)",
+                        "peak_note": "157.3 = FP32 matrix-core / packed-vector peak; the vector ALUs run "
+                                     "non-packed FP32 here (78.6 TFLOP/s)",
+                        "reference_formula_tflops": elems * FLOP_MR_TET_PATCH / avg_s / 1e12,
+                        "work_reduction_vs_reference_formula": FLOP_MR_TET_PATCH / FLOP_AFFINE_TET}
+            if traffic:
+                roofline["hbm_gbs"] = traffic / avg_s / 1e9
+                roofline["hbm_frac"] = roofline["hbm_gbs"] / PEAK_HBM_GBS
         elif best["class"] == "evaluate":
             bytes_per = 16.0 * gm.ntets + 32.0 * gm.nverts
             achieved = bytes_per / avg_s / 1e9
@@ -399,37 +451,76 @@ def main():
                 flop_per_launch = FLOP_GETME_TET * gm.ntets
                 kname = "getmeTetKernel<float, 0, 0>"
             else:
-                flop_per_launch = (2 * FLOP_MR_TET_PATCH + FLOP_GETME_PULL) * gm.sum_patch / best["launches"] * f32_passes
+                flop_per_launch = (2 * FLOP_MR_TET_PATCH + FLOP_GETME_PULL) * float(np.sum(sizes[gm.nverts - gm.nmovable:])) \
+                    / best["launches"] * f32_passes
                 kname = "getmeWaveKernel<float, 0, 0, 32, 0>"
             achieved = flop_per_launch / avg_s / 1e12
             roofline = {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                         "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None, "kernel": kname,
                         "avg_launch_ms": avg_s * 1e3, "flop_per_launch": flop_per_launch}
+        if roofline["frac"] > 1.0:
+            raise RuntimeError(f"roofline fraction {roofline['frac']:.3f} above 1: the work unit is wrong")
 
-    # Final quality of the mixed-precision runs against FP64 runs of the same
-    # passes (the FP64 wave kernels are bit-identical to the plain FP64 ones).
-    quality = {}
+    # FP64 on the same workload: the exact kernels (the reference's operation
+    # order, bit-identical to the plain FP64 kernel and the CPU oracle) and the
+    # mixed-precision FP64 kernels for every pass (GM_FP64_FAST).  The final
+    # quality of each run is compared with the exact FP64 run of the same
+    # passes; equal quality within 1e-6 is the north-star condition.
+    quality, fp64_exact, fp64_fast = {}, {}, {}
     for sm, p in per.items():
         gm.set_positions(start)
+        t0 = time.perf_counter()
         r64 = gm.smooth(sm, args.steps, precision="fp64", kernel="wave", spawn_coeff=spawn_coeff, counts=False)
+        fp64_exact[sm] = {"wall_s": time.perf_counter() - t0, "device_ms": r64["total_ms"]}
+        gm.set_positions(start)
+        one.smooth(sm, max(args.warmup, 1), precision="fp64_fast", spawn_coeff=spawn_coeff, graph=True)
+        gm.set_positions(start)
+        t0 = time.perf_counter()
+        rf = one.smooth(sm, args.steps, precision="fp64_fast", spawn_coeff=spawn_coeff, graph=True)
+        fp64_fast[sm] = {"wall_s": time.perf_counter() - t0, "device_ms": rf["total_ms"],
+                         "min_diff": float(rf["min"][0][-1] - r64["min"][-1]),
+                         "mean_diff": float(rf["mean"][0][-1] - r64["mean"][-1])}
         quality[sm] = {"fp64_final_min": float(r64["min"][-1]), "fp64_final_mean": float(r64["mean"][-1]),
                        "min_diff": float(p["res"]["min"][-1] - r64["min"][-1]),
                        "mean_diff": float(p["res"]["mean"][-1] - r64["mean"][-1])}
-        log(f"{sm}: final min/mean vs fp64: {quality[sm]['min_diff']:+.2e} / {quality[sm]['mean_diff']:+.2e}")
+        log(f"{sm}: final min/mean vs fp64: {args.precision} {quality[sm]['min_diff']:+.2e} / "
+            f"{quality[sm]['mean_diff']:+.2e}; fp64_fast {fp64_fast[sm]['min_diff']:+.2e} / "
+            f"{fp64_fast[sm]['mean_diff']:+.2e}; exact fp64 {fp64_exact[sm]['wall_s'] * 1e3:.1f} ms, "
+            f"fp64_fast {fp64_fast[sm]['wall_s'] * 1e3:.1f} ms wall")
+    fast_value = moved / sum(v["wall_s"] for v in fp64_fast.values())
+    exact_value = moved / sum(v["wall_s"] for v in fp64_exact.values())
+    worst_min = max(abs(v["min_diff"]) for v in fp64_fast.values())
+    worst_mean = max(abs(v["mean_diff"]) for v in fp64_fast.values())
+    equal_quality = {
+        "precision": "fp64_fast", "vertices_per_s": fast_value,
+        "speedup_vs_plain_fp64": (fast_value / plain_value) if plain_value else None,
+        "max_abs_final_min_diff": worst_min, "max_abs_final_mean_diff": worst_mean,
+        "within_1e-6": bool(worst_min <= 1e-6 and worst_mean <= 1e-6),
+        "per_smoother": fp64_fast}
+    headline_equal = max(max(abs(q["min_diff"]), abs(q["mean_diff"])) for q in quality.values()) <= 1e-6
 
     batched = None if args.no_batch else batched_leg(args)
     serving = None if args.no_serve else serving_leg(args)
 
     cpu = None
     if not args.no_cpu:
-        threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
-        threads = max(1, min(threads, 16))
-        cpu_value, cpu_s, cm = cpu_baseline(args.cpu_n, args.cpu_passes, threads, smoothers, spawn_coeff)
+        avail = sorted(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else list(range(os.cpu_count() or 1))
+        threads = max(1, min(len(avail), 16))
+        cpu_value, cpu_s, cm = cpu_baseline(args.cpu_n, args.cpu_passes, threads, smoothers, spawn_coeff,
+                                            cores=set(avail[:threads]))
+        serial_value, serial_s, sm_mesh = cpu_baseline(args.cpu_serial_n, 1, 1, smoothers, spawn_coeff,
+                                                       cores={avail[0]})
+        model = cpu_model()
         cpu = {"value": cpu_value, "unit": "smoothed vertices/s", "cores": threads, "kind": "port",
-               "sample": f"FP64 oracle, Thread mode, {args.cpu_passes} passes each of {','.join(smoothers)} on the same "
-                         f"construction at {args.cpu_n}^3 cells ({cm.nverts} vertices), glibc libm; "
-                         f"{cpu_s:.1f} s"}
-        log(f"cpu oracle: {cpu_value:.0f} vertices/s on {threads} threads ({cpu_s:.1f} s)")
+               "sample": f"FP64 oracle (glibc libm), Thread mode on {threads} pinned cores: {args.cpu_passes} "
+                         f"pass(es) each of {','.join(smoothers)} on the {args.cpu_n}^3 workload mesh "
+                         f"({cm.nverts} vertices), {cpu_s:.1f} s",
+               "cpu_model": model,
+               "serial": {"value": serial_value, "cores": 1,
+                          "sample": f"Serial mode on one pinned core: 1 pass each of {','.join(smoothers)} on the "
+                                    f"{args.cpu_serial_n}^3 construction ({sm_mesh.nverts} vertices), {serial_s:.1f} s"}}
+        log(f"cpu oracle ({model}): {cpu_value:.0f} vertices/s on {threads} threads ({cpu_s:.1f} s); serial "
+            f"{serial_value:.0f} vertices/s ({serial_s:.1f} s)")
 
     ms_per_step = total_wall / (args.steps * len(smoothers)) * 1e3
     out = {
@@ -443,7 +534,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32+f64" if args.precision == "fp32_polish" else ("f64" if args.precision == "fp64" else "f32"),
+        "dtype": {"fp32_polish": "f32+f64", "fp64": "f64", "fp64_fast": "f64", "fp32": "f32"}[args.precision],
         "data": "synthetic",
         "config": {"workload": f"Kuhn tet cube {args.n}^3 cells ({m.nverts} vertices, {gm.ntets} tets), "
                                f"jitter 0.35h seed 2, mean ratio, {'+'.join(smoothers)} x {args.steps} passes, "
@@ -462,6 +553,9 @@ def main():
         "plain_fp64_vertices_per_s": plain_value,
         "speedup_vs_plain_fp64": (value / plain_value) if plain_value else None,
         "quality_vs_fp64": quality,
+        "headline_within_1e-6_of_fp64": headline_equal,
+        "fp64_exact_vertices_per_s": exact_value,
+        "equal_quality": equal_quality,
         "batched": batched,
         "serving": serving,
         "roofline": roofline,

@@ -46,7 +46,7 @@ int checkRun(int smoother, int passes, int precision, int kernel, double spawnCo
 {
     if(smoother < 0 || smoother > 3) return fail(GM_ERR_ARG, "unknown smoother");
     if(passes < 0) return fail(GM_ERR_ARG, "negative pass count");
-    if(precision < 0 || precision > 2) return fail(GM_ERR_ARG, "unknown precision");
+    if(precision < 0 || precision > 3) return fail(GM_ERR_ARG, "unknown precision");
     if(kernel < 0 || kernel > 1) return fail(GM_ERR_ARG, "unknown kernel shape");
     if(!(spawnCoeff > 0.0)) return fail(GM_ERR_ARG, "spawn coefficient must be positive");
     return GM_OK;
@@ -181,6 +181,13 @@ int gm_set_topology_cache(int64_t max_bytes)
 {
     if(max_bytes < 0) return fail(GM_ERR_ARG, "negative cache size");
     gmt::setCacheCapacity((size_t)max_bytes);
+    return GM_OK;
+}
+
+int gm_set_polish_passes(int passes)
+{
+    if(passes < 1 || passes > 64) return fail(GM_ERR_ARG, "polish passes must be in [1, 64]");
+    gmr::setPolishPasses(passes);
     return GM_OK;
 }
 
@@ -366,6 +373,7 @@ int gm_mesh_smooth(gm_mesh* m, int smoother, int passes, int precision, int kern
     r.smoother = smoother;
     r.passes = passes;
     r.precision = precision;
+    r.polishPasses = gmr::polishPasses();
     r.kernel = kernel;
     r.spawnCoeff = spawnCoeff;
     r.lane = defaultLane(m->plan->meshes, smoother);
@@ -385,6 +393,7 @@ int gm_mesh_smooth_schedule(gm_mesh* m, int smoother, int maxPasses, const gm_sc
     r.smoother = smoother;
     r.passes = maxPasses;
     r.precision = precision;
+    r.polishPasses = gmr::polishPasses();
     r.kernel = kernel;
     r.spawnCoeff = spawnCoeff;
     r.lane = defaultLane(m->plan->meshes, smoother);
@@ -403,7 +412,8 @@ int gm_mesh_smooth_groups(gm_mesh* m, int smoother, int64_t groupBegin, int64_t 
     if(rc) return rc;
     if(smoother == gmd::SM_GETME)
         return fail(GM_ERR_UNSUPPORTED, "GETMe sweeps start with an element step over the whole mesh; use gm_mesh_smooth");
-    if(precision != GM_FP64 && precision != GM_FP32) return fail(GM_ERR_ARG, "precision must be GM_FP64 or GM_FP32");
+    if(precision == GM_FP32_POLISH)
+        return fail(GM_ERR_ARG, "precision must be GM_FP64, GM_FP32 or GM_FP64_FAST (a partial sweep has no last pass)");
     if((rc = checkSampler(m->sampler, smoother, kernel))) return rc;
     if(groupBegin < 0 || groupEnd > m->topo->ngroups() || groupBegin > groupEnd)
         return fail(GM_ERR_ARG, "bad group range");
@@ -411,6 +421,7 @@ int gm_mesh_smooth_groups(gm_mesh* m, int smoother, int64_t groupBegin, int64_t 
     gmr::RunParams r;
     r.smoother = smoother;
     r.precision = precision;
+    r.polishPasses = gmr::polishPasses();
     r.kernel = kernel;
     r.spawnCoeff = spawnCoeff;
     r.lane = defaultLane(m->plan->meshes, smoother);
@@ -531,6 +542,7 @@ int gm_batch_smooth(gm_batch* b, int smoother, int passes, int precision, int ke
     r.smoother = smoother;
     r.passes = passes;
     r.precision = precision;
+    r.polishPasses = gmr::polishPasses();
     r.kernel = kernel;
     r.spawnCoeff = spawnCoeff;
     r.lane = b->lane[smoother].patchesPerGroup ? b->lane[smoother] : defaultLane(b->plan->meshes, smoother);
@@ -551,6 +563,7 @@ int gm_batch_smooth_schedule(gm_batch* b, int smoother, int maxPasses, const gm_
     r.smoother = smoother;
     r.passes = maxPasses;
     r.precision = precision;
+    r.polishPasses = gmr::polishPasses();
     r.kernel = kernel;
     r.spawnCoeff = spawnCoeff;
     r.lane = b->lane[smoother].patchesPerGroup ? b->lane[smoother] : defaultLane(b->plan->meshes, smoother);
@@ -625,10 +638,11 @@ int gm_batch_autotune(gm_batch* b, int smoother, int precision, int passes, int3
             for(const auto& col : p->colours)
                 for(const gmr::VList& l : col)
                 {
-                    gmd::VertexWork w{p->dTable, l.segs, l.nsegs, l.count, l.kind, l.maxPatch, 0};
+                    const bool f64 = precision == GM_FP64 || precision == GM_FP64_FAST;
+                    gmd::VertexWork w{p->dTable, l.segs, l.nsegs, l.count, l.kind, l.maxPatch,
+                                      precision == GM_FP64_FAST ? 1 : 0};
                     if(!getme && l.kind != gmd::EK_MIXED &&
-                       !gmd::laneFits(w, smoother, shape, precision == GM_FP64, p->meshes[0]->evaluator,
-                                      p->meshes[0]->sampler))
+                       !gmd::laneFits(w, smoother, shape, f64, p->meshes[0]->evaluator, p->meshes[0]->sampler))
                         fits = false;
                 }
             double ms = -1.0;
@@ -638,6 +652,7 @@ int gm_batch_autotune(gm_batch* b, int smoother, int precision, int passes, int3
                 r.smoother = smoother;
                 r.passes = passes;
                 r.precision = precision;
+    r.polishPasses = gmr::polishPasses();
                 r.kernel = GM_KERNEL_WAVE;
                 r.spawnCoeff = 1.0 / 20.0;
                 r.lane = shape;

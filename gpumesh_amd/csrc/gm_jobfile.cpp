@@ -14,7 +14,7 @@
 //       24     8  nhexs (i64)
 //       32     4  smoother (i32, GM_NELDER_MEAD..GM_GETME)
 //       36     4  passes (i32)
-//       40     4  precision (i32, GM_FP64 / GM_FP32 / GM_FP32_POLISH)
+//       40     4  precision (i32, GM_FP64 / GM_FP32 / GM_FP32_POLISH / GM_FP64_FAST)
 //       44    20  reserved (zero)
 //       64        positions, 3 nverts f64 (x, y, z per vertex)
 //                 fixed flags, nverts u8, zero-padded to a multiple of 8
@@ -110,7 +110,7 @@ int gm_job_encode(int64_t nverts, const double* pos, const int32_t* fixed, int64
     if((nverts && (!pos || !fixed)) || (ntets && !tets) || (nhexs && !hexs) || !out)
         return fail(GM_ERR_ARG, "null array");
     if(outBytes < (int64_t)total) return fail(GM_ERR_ARG, "output buffer smaller than gm_job_size");
-    if(smoother < 0 || smoother > 3 || passes < 0 || precision < 0 || precision > 2)
+    if(smoother < 0 || smoother > 3 || passes < 0 || precision < 0 || precision > 3)
         return fail(GM_ERR_ARG, "bad smoothing request");
     unsigned char* b = (unsigned char*)out;
     std::memset(b, 0, total);

@@ -39,6 +39,7 @@ struct RunParams
     int smoother = 0;
     int passes = 0;
     int precision = GM_FP64;
+    int polishPasses = 1;        // FP64 passes ending a GM_FP32_POLISH run
     int kernel = GM_KERNEL_WAVE;
     double spawnCoeff = 1.0 / 20.0;
     gmd::LaneShape lane{1, 0};   // tileElems 0: chosen per work list
@@ -51,6 +52,22 @@ struct RunParams
 };
 
 struct Plan;
+
+// Pass `pass` of a run of `passes` sweeps runs in FP64 ...
+inline bool passF64(int precision, int pass, int passes, int polishPasses)
+{
+    return precision == GM_FP64 || precision == GM_FP64_FAST ||
+           (precision == GM_FP32_POLISH && pass >= passes - polishPasses);
+}
+// ... and, when it does, on the mixed-precision FP64 kernels unless the run
+// is exact FP64.
+inline bool passFast(int precision) { return precision == GM_FP32_POLISH || precision == GM_FP64_FAST; }
+// A run starts from the FP32 copy of the positions.
+inline bool startsF32(int precision) { return precision == GM_FP32 || precision == GM_FP32_POLISH; }
+
+// Process-wide number of FP64 passes ending a GM_FP32_POLISH run.
+int polishPasses();
+void setPolishPasses(int n);
 
 } // namespace gmr
 

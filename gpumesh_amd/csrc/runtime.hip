@@ -6,6 +6,8 @@
 // reduction trees are the same alone and batched.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
+
 #include <algorithm>
 #include <cstring>
 #include <map>
@@ -705,7 +707,7 @@ static hipError_t sweep(Plan* p, const gmq::Crew& c, int ev, int sa, const RunPa
         for(const VList& l : p->colours[g])
         {
             gmd::VertexWork w{p->dTable, l.segs, l.nsegs, l.count, l.kind, l.maxPatch,
-                              f64 && r.precision == GM_FP32_POLISH ? 1 : 0};
+                              f64 && gmr::passFast(r.precision) ? 1 : 0};
             gmd::LaneShape lane = r.lane;
             if(lane.tileElems == 0)
             {
@@ -784,6 +786,14 @@ static int passBody(Plan* p, const gmq::Crew& c, int ev, int sa, const RunParams
     return GM_OK;
 }
 
+namespace
+{
+std::atomic<int> g_polishPasses{1};
+}
+
+int polishPasses() { return g_polishPasses.load(); }
+void setPolishPasses(int n) { g_polishPasses.store(n); }
+
 int runPlan(Plan* p, const gmq::Crew& c, int ev, int sa, const RunParams& r)
 {
     for(gm_mesh* m : p->meshes)
@@ -800,7 +810,7 @@ int runPlan(Plan* p, const gmq::Crew& c, int ev, int sa, const RunParams& r)
     GMR_HIP(hipEventRecord(p->ev[0], s));
     GMR_HIP(gmd::launchResetRun(p->dTable, p->vertBlocks, p->meshList, p->nMeshes, s));
     bool inF32 = false;
-    if(r.precision != GM_FP64)
+    if(startsF32(r.precision))
     {
         GMR_HIP(gmd::launchToF32(p->dTable, p->vertBlocks, s));
         inF32 = true;
@@ -808,7 +818,7 @@ int runPlan(Plan* p, const gmq::Crew& c, int ev, int sa, const RunParams& r)
     GMR_HIP(evaluateAll(p, c, ev, sa, inF32, timed));
     for(int pass = 0; pass < r.passes; ++pass)
     {
-        const bool f64 = r.precision == GM_FP64 || (r.precision == GM_FP32_POLISH && pass == r.passes - 1);
+        const bool f64 = passF64(r.precision, pass, r.passes, r.polishPasses);
         if(f64 && inF32)
         {
             GMR_HIP(gmd::launchToF64(p->dTable, p->vertBlocks, s));
@@ -861,7 +871,7 @@ int sweepColours(Plan* p, const gmq::Crew& c, int ev, int sa, const RunParams& r
     hipStream_t s = p->stream;
     p->poolUsed = 0;
     GMR_HIP(gmd::launchResetRun(p->dTable, p->vertBlocks, p->meshList, p->nMeshes, s));
-    const bool f64 = r.precision == GM_FP64;
+    const bool f64 = !startsF32(r.precision);
     if(!f64) GMR_HIP(gmd::launchToF32(p->dTable, p->vertBlocks, s));
     hipError_t e = sweep(p, c, ev, sa, r, f64, std::min<int64_t>(g0, (int64_t)p->colours.size()),
                          std::min<int64_t>(g1, (int64_t)p->colours.size()), false);

@@ -59,6 +59,7 @@ struct Job
     const uint32_t* tets = nullptr;
     const uint32_t* hexs = nullptr;
     int smoother = 0, passes = 0, precision = 0;
+    int polishPasses = 1;   // gm_set_polish_passes at submission
     // State.
     int state = JOB_QUEUED;
     gm_mesh* mesh = nullptr;
@@ -210,7 +211,7 @@ void uploadJob(gm_server* s, Job* j)
         m->sampler = s->sampler;
         rc = gmr::preparePlan(m->plan, j->passes + 1, j->smoother == gmd::SM_GETME);
     }
-    j->inF32 = j->precision != GM_FP64;
+    j->inF32 = gmr::startsF32(j->precision);
     j->pass = 0;
     if(!rc) rc = gmr::launchBegin(m->plan, s->crew, s->evaluator, s->sampler, j->inF32);
     if(!rc && hipStreamSynchronize(m->stream) != hipSuccess) rc = fail(GM_ERR_HIP, "job start failed");
@@ -336,8 +337,8 @@ int launchRound(gm_server* s, const gmd::LaneShape* lanes)
     std::map<std::tuple<int, int, int, int>, std::vector<Job*>> members;
     for(Job* j : s->active)
     {
-        const bool polish = j->precision == GM_FP32_POLISH && j->pass == j->passes - 1;
-        const bool f64 = j->precision == GM_FP64 || polish;
+        const bool f64 = gmr::passF64(j->precision, j->pass, j->passes, j->polishPasses);
+        const bool polish = f64 && gmr::passFast(j->precision);
         const bool convert = f64 && j->inF32;
         members[std::make_tuple(j->smoother, (int)f64, (int)convert, (int)polish)].push_back(j);
     }
@@ -446,7 +447,7 @@ void schedulerLoop(gm_server* s)
         std::vector<Job*> still;
         for(Job* j : s->active)
         {
-            const bool f64 = j->precision == GM_FP64 || (j->precision == GM_FP32_POLISH && j->pass == j->passes - 1);
+            const bool f64 = gmr::passF64(j->precision, j->pass, j->passes, j->polishPasses);
             if(f64) j->inF32 = false;
             ++j->pass;
             if(j->pass < j->passes) still.push_back(j);
@@ -489,7 +490,7 @@ int checkRequest(int smoother, int passes, int precision)
 {
     if(smoother < 0 || smoother > 3) return fail(GM_ERR_ARG, "unknown smoother");
     if(passes < 0) return fail(GM_ERR_ARG, "negative pass count");
-    if(precision < 0 || precision > 2) return fail(GM_ERR_ARG, "unknown precision");
+    if(precision < 0 || precision > 3) return fail(GM_ERR_ARG, "unknown precision");
     return GM_OK;
 }
 
@@ -608,6 +609,7 @@ int gm_server_submit_job(gm_server* s, const void* blob, int64_t bytes, int64_t*
     j->smoother = request[0];
     j->passes = request[1];
     j->precision = request[2];
+    j->polishPasses = gmr::polishPasses();
     j->tSubmit = now;
     return enqueue(s, std::move(j), jobId);
 }
@@ -650,6 +652,7 @@ int gm_server_submit(gm_server* s, int64_t nverts, const double* pos, const int3
     j->smoother = smoother;
     j->passes = passes;
     j->precision = precision;
+    j->polishPasses = gmr::polishPasses();
     j->tSubmit = now;
     return enqueue(s, std::move(j), jobId);
 }

@@ -18,7 +18,7 @@ SMOOTHERS = {"nm": 0, "nelder_mead": 0, "gd": 1, "gradient_descent": 1, "spawn":
              "getme": 3}
 EVALUATORS = {"mean_ratio": 0, "metric_conformity": 1}
 SAMPLERS = {"uniform": 0, "analytic": 1, "texture": 2}
-PRECISIONS = {"fp64": 0, "fp32": 1, "fp32_polish": 2}
+PRECISIONS = {"fp64": 0, "fp32": 1, "fp32_polish": 2, "fp64_fast": 3}
 KERNELS = {"plain": 0, "wave": 1}
 # gm_kernel_class order (include/gpumesh.h).
 KERNEL_CLASSES = ["smooth_f32", "smooth_f64", "getme_elem_f32", "getme_elem_f64", "getme_vert_f32",

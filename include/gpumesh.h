@@ -39,9 +39,14 @@ enum gm_status
 enum gm_smoother { GM_NELDER_MEAD = 0, GM_GRADIENT_DESCENT = 1, GM_SPAWN_SEARCH = 2, GM_GETME = 3 };
 enum gm_evaluator { GM_MEAN_RATIO = 0, GM_METRIC_CONFORMITY = 1 };
 enum gm_sampler { GM_UNIFORM = 0, GM_ANALYTIC_SHELL = 1, GM_TEXTURE = 2 };
-/* GM_FP32_POLISH: every pass but the last in FP32, the last pass in FP64;
- * mesh-quality reductions are FP64 in every mode. */
-enum gm_precision { GM_FP64 = 0, GM_FP32 = 1, GM_FP32_POLISH = 2 };
+/* GM_FP64: every pass in FP64 with the reference's operation order (bit-exact
+ * with an FP64 CPU build).  GM_FP32: every pass in FP32.  GM_FP32_POLISH: FP32
+ * passes, then the last gm_set_polish_passes() passes (default 1) in FP64 on
+ * the mixed-precision kernels.  GM_FP64_FAST: every pass on those FP64
+ * kernels (FP64 arithmetic in another operation order, d^(-2/3) from a
+ * hardware seed refined to FP64).  Mesh-quality reductions are FP64 in every
+ * mode. */
+enum gm_precision { GM_FP64 = 0, GM_FP32 = 1, GM_FP32_POLISH = 2, GM_FP64_FAST = 3 };
 /* Kernel shape: one vertex per lane (plain), or one vertex per group of
  * lanes holding its patch elements (wave). */
 enum gm_kernel { GM_KERNEL_PLAIN = 0, GM_KERNEL_WAVE = 1 };
@@ -192,6 +197,11 @@ int gm_mesh_kernel_times(const gm_mesh* mesh, double* ms, int64_t* launches);
 int gm_set_topology_cache(int64_t max_bytes);
 /* out4 = {hits, misses, entries, bytes held}. */
 int gm_topology_cache_stats(int64_t* out4);
+
+/* Number of FP64 passes that end a GM_FP32_POLISH run (1 to 64, default 1;
+ * a run of fewer passes runs them all in FP64).  Applies to runs, batches
+ * and server jobs started afterwards, process-wide. */
+int gm_set_polish_passes(int passes);
 
 /* Device memory is allocated through a caching allocator: blocks freed by
  * destroyed meshes, batches and finished serving jobs are kept (per device,

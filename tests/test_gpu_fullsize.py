@@ -61,6 +61,36 @@ def test_group_replay_bitexact(oracle, c2, c2_oracle, smoother, coeff, group):
     assert np.array_equal(after[outside], before[outside])
 
 
+@pytest.mark.parametrize("smoother,coeff", [("gd", 0.05), ("spawn", 0.25), ("nm", 0.05)])
+def test_group_replay_against_glibc_oracle(oracle, c2, c2_oracle, smoother, coeff):
+    """The exact FP64 kernels (the project's correctly rounded pow/cbrt/exp)
+    against the oracle calling glibc, the reference's libm: group 3's sampled
+    vertices within 1e-9 h, their patch qualities within 1e-12."""
+    m, gm, start = c2
+    om = c2_oracle
+    group = 3
+    gm.set_positions(start)
+    gm.smooth_groups(smoother, 0, group, "fp64", "wave", coeff)
+    before = gm.positions()
+    gm.smooth_groups(smoother, group, group + 1, "fp64", "wave", coeff)
+    after = gm.positions()
+    off, nodes = gm.groups()
+    members = nodes[off[group]:off[group + 1]]
+    sample = np.sort(np.random.default_rng(7).choice(members, size=N_SAMPLE, replace=False)).astype(np.uint32)
+    om.set_positions(before)
+    oracle.set_libm(0)
+    try:
+        om.smooth_vertices(smoother, sample, spawn_coeff=coeff)
+        ref = om.positions()
+        q_ref = np.array([om.patch_quality(int(v)) for v in sample])
+        om.set_positions(after)
+        q_gpu = np.array([om.patch_quality(int(v)) for v in sample])
+    finally:
+        oracle.set_libm(1)
+    assert np.max(np.abs(after[sample] - ref[sample])) <= 1e-9 * 0.01
+    assert np.max(np.abs(q_gpu - q_ref)) <= 1e-12
+
+
 def test_kernel_shapes_agree_fullsize(c2):
     m, gm, start = c2
     res = {}
