@@ -7,6 +7,7 @@
 
 #include "gm_device.h"
 #include "gm_affine.h"
+#include "gm_reduce.h"
 #include "gm_smoothers.h"
 
 #include <type_traits>
@@ -2352,23 +2353,6 @@ __global__ void __launch_bounds__(64 * GETME_WAVES) getmeWaveKernel(const DevMes
 // Each block owns a contiguous element range; per-lane compensated sums are
 // combined in a fixed tree, so the result is deterministic.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ int histBucket(double value)
-{
-    // size_t(value * 20) clamped to [0, 19]; products that truncate to a
-    // negative integer and NaN land in the last bucket (x86-64 conversion).
-    double x = value * 20.0;
-    if(x != x || x <= -1.0 || x >= 9.2e18) return 19;
-    long long t = (long long)x;
-    return t < 0 ? 0 : (t > 19 ? 19 : (int)t);
-}
-
-__device__ __forceinline__ void twoSum(double a, double b, double& s, double& e)
-{
-    s = a + b;
-    double bb = s - a;
-    e = (a - (s - bb)) + (b - bb);
-}
-
 template<int EV, int SA, typename P, bool HEX>
 __global__ void __launch_bounds__(EVAL_THREADS) evalKernel(const DevMesh* __restrict__ meshes,
                                                            const uint4* __restrict__ spans, uint32_t nspans,

@@ -351,6 +351,30 @@ int gm_job_decode(const void* job_bytes, int64_t bytes, double* pos, int32_t* fi
 int gm_cube_mesh(int n, int kind, double jitter, uint64_t seed, double min_det_fraction, double* pos,
                  int32_t* fixed, uint32_t* elems);
 
+/* ---- Allocation-free evaluation.
+ *
+ * No counterpart in the reference: an entry point for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  It computes what gm_mesh_evaluate
+ * computes (the FP64 min, sum of 1/q and 20-bucket histogram of the element
+ * qualities; same bits) over arrays the caller already holds on the current
+ * device: d_pos[3 nverts] FP64 positions, d_tets[4 ntets], d_hexs[8 nhexs]
+ * (any vertex numbering), with the mean-ratio or metric-conformity evaluator
+ * and the uniform or analytic sampler at `scaling`.  Results go to device
+ * memory: d_out3 = {min, sum of 1/q (+inf if any q <= 0), element count},
+ * d_hist20[20].  The kernels are queued on `stream` (a hipStream_t; NULL =
+ * the default stream) and the call returns without waiting.  It calls no
+ * hipMalloc, hipFree or allocating routine: the reduction's partials live in
+ * a static device array (about 57 KB, held by each GPU the library runs on
+ * until the process ends), so calls that overlap on one device race (queue
+ * them on one stream).  Sizes beyond GM_NOALLOC_MAX_VERTS vertices or
+ * GM_NOALLOC_MAX_ELEMS elements are refused with GM_ERR_ARG. */
+#define GM_NOALLOC_MAX_VERTS 4294967295LL
+#define GM_NOALLOC_MAX_ELEMS 2147483648LL
+int gm_evaluate_noalloc(const double* d_pos, int64_t nverts, const uint32_t* d_tets, int64_t ntets,
+                        const uint32_t* d_hexs, int64_t nhexs, int evaluator, int sampler, double scaling,
+                        double* d_out3, int32_t* d_hist20, void* stream);
+
 /* Device math probe (for tests): op 0 sqrt(a), 1 a/b, 2 a^(2/3), 3 a^(1/3),
  * 4 exp(a), 5 a^(-2/3) as the FP32-polish and evaluation-record kernels
  * compute it (hardware seed, one Halley step), all in FP64 on the device;
