@@ -29,7 +29,7 @@ EXPORTS = [
     "gm_server_result", "gm_server_release", "gm_server_stats", "gm_cube_mesh", "gm_set_topology_cache", "gm_set_device_pool",
     "gm_topology_cache_stats", "gm_math_probe", "gm_mesh_smooth_schedule", "gm_batch_smooth_schedule",
     "gm_job_size", "gm_job_encode", "gm_job_info", "gm_job_decode", "gm_server_submit_job",
-    "gm_set_polish_passes", "gm_evaluate_noalloc",
+    "gm_set_polish_passes", "gm_evaluate_noalloc", "gm_mesh_create_grouped",
 ]
 
 
@@ -59,6 +59,7 @@ def lib():
     L.gm_device_count.argtypes = [C.POINTER(C.c_int)]
     for name in ("gm_mesh_create", "gm_mesh_create_host"):
         getattr(L, name).argtypes = [i64, vp, vp, i64, vp, i64, vp, C.POINTER(vp)]
+    L.gm_mesh_create_grouped.argtypes = [i64, vp, vp, i64, vp, i64, vp, i64, vp, vp, C.c_int, C.POINTER(vp)]
     L.gm_mesh_destroy.argtypes = [vp]
     L.gm_mesh_info.argtypes = [vp, vp]
     L.gm_mesh_permutation.argtypes = [vp, vp]

@@ -156,6 +156,29 @@ int gm_mesh_create(int64_t nverts, const double* pos, const int32_t* fixed,
     return GM_OK;
 }
 
+int gm_mesh_create_grouped(int64_t nverts, const double* pos, const int32_t* fixed, int64_t ntets,
+                           const uint32_t* tets, int64_t nhexs, const uint32_t* hexs, int64_t ngroups,
+                           const int64_t* group_off, const uint32_t* group_nodes, int device_copy, gm_mesh** out)
+{
+    if(!out) return fail(GM_ERR_ARG, "out is null");
+    *out = nullptr;
+    if(device_copy)
+    {
+        int ndev = 0;
+        if(hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(GM_ERR_HIP, "no HIP device visible");
+    }
+    gm_mesh* m = nullptr;
+    int rc = gmr::makeMeshGrouped(nverts, pos, fixed, ntets, tets, nhexs, hexs, ngroups, group_off, group_nodes, &m);
+    if(rc) return rc;
+    if(device_copy && (rc = gmr::uploadMesh(m)))
+    {
+        gmr::destroyMesh(m);
+        return rc;
+    }
+    *out = m;
+    return GM_OK;
+}
+
 int gm_mesh_destroy(gm_mesh* mesh)
 {
     gmr::destroyMesh(mesh);
@@ -410,8 +433,9 @@ int gm_mesh_smooth_groups(gm_mesh* m, int smoother, int64_t groupBegin, int64_t 
     GM_NEED_DEVICE(m);
     int rc = checkRun(smoother, 0, precision, kernel, spawnCoeff);
     if(rc) return rc;
-    if(smoother == gmd::SM_GETME)
-        return fail(GM_ERR_UNSUPPORTED, "GETMe sweeps start with an element step over the whole mesh; use gm_mesh_smooth");
+    if(smoother == gmd::SM_GETME && !(groupBegin == 0 && groupEnd == m->topo->ngroups()))
+        return fail(GM_ERR_UNSUPPORTED, "a GETMe sweep starts with an element step over the whole mesh: it runs "
+                                        "only over all colour groups [0, ngroups)");
     if(precision == GM_FP32_POLISH)
         return fail(GM_ERR_ARG, "precision must be GM_FP64, GM_FP32 or GM_FP64_FAST (a partial sweep has no last pass)");
     if((rc = checkSampler(m->sampler, smoother, kernel))) return rc;

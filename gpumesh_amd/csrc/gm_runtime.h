@@ -155,6 +155,10 @@ size_t poolCachedBytes();
 // Host topology of a new handle (through the topology cache).
 int makeMesh(int64_t nverts, const double* pos, const int32_t* fixed, int64_t ntets, const uint32_t* tets,
              int64_t nhexs, const uint32_t* hexs, gm_mesh** out);
+// makeMesh with the caller's numbering and colour groups (gmt::buildGrouped).
+int makeMeshGrouped(int64_t nverts, const double* pos, const int32_t* fixed, int64_t ntets, const uint32_t* tets,
+                    int64_t nhexs, const uint32_t* hexs, int64_t ngroups, const int64_t* groupOff,
+                    const uint32_t* groupNodes, gm_mesh** out);
 // Device memory a mesh needs when resident (connectivity, positions, reduction
 // buffers, `records` pass records, GETMe scratch when `getme`).
 size_t residentBytes(const gm_mesh* m, int records, bool getme);

@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <numeric>
+#include <string>
 
 namespace gmt
 {
@@ -16,9 +17,15 @@ static const int UNSET_GROUP = -1;
 static const int FIXED_TYPE = 0;
 static const int INTER_TYPE = 3;
 
-bool build(int64_t V, const double* pos, const int32_t* fixed,
-           int64_t T, const uint32_t* tets, int64_t H, const uint32_t* hexs,
-           Topology& out, std::string& err)
+// Patch element lists and neighbour lists in the input numbering
+// (Mesh::compileNeighborhoods and addEdge, Mesh.cpp:365-460).
+struct Adjacency
+{
+    std::vector<uint32_t> eoff, eref;   // CSR patch lists, eref = (id << 4) | (is_hex << 3) | local
+    std::vector<uint32_t> ncnt, ntmp;   // neighbour count, and 3 eoff[v] slots per vertex
+};
+
+static bool checkInputs(int64_t V, int64_t T, const uint32_t* tets, int64_t H, const uint32_t* hexs, std::string& err)
 {
     if(V < 0 || T < 0 || H < 0) { err = "negative size"; return false; }
     if(V > (int64_t)0xFFFFFFF0) { err = "too many vertices"; return false; }
@@ -26,13 +33,19 @@ bool build(int64_t V, const double* pos, const int32_t* fixed,
     for(int64_t i = 0; i < 4 * T; ++i) if((int64_t)tets[i] >= V) { err = "tet vertex id out of range"; return false; }
     for(int64_t i = 0; i < 8 * H; ++i) if((int64_t)hexs[i] >= V) { err = "hex vertex id out of range"; return false; }
     if(4 * T + 8 * H >= (int64_t)1 << 32) { err = "too many element references"; return false; }
+    return true;
+}
 
+static void buildAdjacency(int64_t V, int64_t T, const uint32_t* tets, int64_t H, const uint32_t* hexs, Adjacency& a)
+{
     // ---- Patch element lists (compileNeighborhoods: tets in id order, then hexes).
-    std::vector<uint32_t> eoff(V + 1, 0);
+    std::vector<uint32_t>& eoff = a.eoff;
+    eoff.assign(V + 1, 0);
     for(int64_t i = 0; i < 4 * T; ++i) ++eoff[tets[i] + 1];
     for(int64_t i = 0; i < 8 * H; ++i) ++eoff[hexs[i] + 1];
     for(int64_t v = 0; v < V; ++v) eoff[v + 1] += eoff[v];
-    std::vector<uint32_t> eref(eoff[V]);
+    std::vector<uint32_t>& eref = a.eref;
+    eref.assign(eoff[V], 0);
     {
         std::vector<uint32_t> cur(eoff.begin(), eoff.end() - 1);
         for(int64_t e = 0; e < T; ++e)
@@ -45,8 +58,10 @@ bool build(int64_t V, const double* pos, const int32_t* fixed,
 
     // ---- Neighbour vertices (addEdge: append when absent, both directions).
     // Every element adds at most 3 edges at each of its vertices.
-    std::vector<uint32_t> ncnt(V, 0);
-    std::vector<uint32_t> ntmp((size_t)3 * eoff[V]);
+    std::vector<uint32_t>& ncnt = a.ncnt;
+    std::vector<uint32_t>& ntmp = a.ntmp;
+    ncnt.assign(V, 0);
+    ntmp.assign((size_t)3 * eoff[V], 0);
     auto addEdge = [&](uint32_t a, uint32_t b) {
         uint32_t* la = &ntmp[(size_t)3 * eoff[a]];
         for(uint32_t i = 0; i < ncnt[a]; ++i)
@@ -61,6 +76,19 @@ bool build(int64_t V, const double* pos, const int32_t* fixed,
     for(int64_t e = 0; e < H; ++e)
         for(int k = 0; k < 12; ++k)
             addEdge(hexs[8 * e + HEX_EDGES[k][0]], hexs[8 * e + HEX_EDGES[k][1]]);
+}
+
+bool build(int64_t V, const double* pos, const int32_t* fixed,
+           int64_t T, const uint32_t* tets, int64_t H, const uint32_t* hexs,
+           Topology& out, std::string& err)
+{
+    if(!checkInputs(V, T, tets, H, hexs, err)) return false;
+    Adjacency adj;
+    buildAdjacency(V, T, tets, H, hexs, adj);
+    const std::vector<uint32_t>& eoff = adj.eoff;
+    const std::vector<uint32_t>& eref = adj.eref;
+    const std::vector<uint32_t>& ncnt = adj.ncnt;
+    const std::vector<uint32_t>& ntmp = adj.ntmp;
 
     // ---- Types: only fixed and free vertices exist, so every movable node is
     // interior (no movable boundary, hence no subsurface nodes).
@@ -201,6 +229,86 @@ bool build(int64_t V, const double* pos, const int32_t* fixed,
         out.groupNodes[p - out.nfixed] = (uint32_t)p;
         out.groupMaxPatch[g] = std::max<int32_t>(out.groupMaxPatch[g], (int32_t)(out.eoff[p + 1] - out.eoff[p]));
     }
+    return true;
+}
+
+bool buildGrouped(int64_t V, const double* pos, const int32_t* fixed, int64_t T, const uint32_t* tets, int64_t H,
+                  const uint32_t* hexs, int64_t G, const int64_t* groupOff, const uint32_t* groupNodes, Topology& out,
+                  std::string& err)
+{
+    if(!checkInputs(V, T, tets, H, hexs, err)) return false;
+    if(G < 0) { err = "negative group count"; return false; }
+    if(G > 0 && (!groupOff || (groupOff[G] > 0 && !groupNodes))) { err = "null group arrays"; return false; }
+    if(G > 0 && groupOff[0] != 0) { err = "group_off[0] must be 0"; return false; }
+    for(int64_t g = 0; g < G; ++g)
+        if(groupOff[g + 1] < groupOff[g]) { err = "group offsets must not decrease"; return false; }
+    const int64_t M = G > 0 ? groupOff[G] : 0;
+    int64_t nfixed = 0;
+    for(int64_t v = 0; v < V; ++v) if(fixed[v]) ++nfixed;
+    if(M != V - nfixed)
+    {
+        err = "the groups must hold every movable vertex exactly once (" + std::to_string(M) + " group entries, " +
+              std::to_string(V - nfixed) + " movable vertices)";
+        return false;
+    }
+    Adjacency adj;
+    buildAdjacency(V, T, tets, H, hexs, adj);
+    // Membership: each movable vertex in exactly one group, no fixed vertex.
+    std::vector<int32_t> groupOf(V, -1);
+    for(int64_t g = 0; g < G; ++g)
+        for(int64_t i = groupOff[g]; i < groupOff[g + 1]; ++i)
+        {
+            const uint32_t v = groupNodes[i];
+            if((int64_t)v >= V) { err = "group vertex id out of range"; return false; }
+            if(fixed[v]) { err = "fixed vertex " + std::to_string(v) + " is in group " + std::to_string(g); return false; }
+            if(groupOf[v] >= 0) { err = "vertex " + std::to_string(v) + " is in two groups"; return false; }
+            groupOf[v] = (int32_t)g;
+        }
+    // Independence (NodeGroups' invariant): no two vertices of one group share
+    // an element.
+    for(int64_t e = 0; e < T + H; ++e)
+    {
+        const bool hex = e >= T;
+        const uint32_t* ev = hex ? &hexs[8 * (size_t)(e - T)] : &tets[4 * (size_t)e];
+        const int nv = hex ? 8 : 4;
+        for(int a = 0; a < nv; ++a)
+            for(int b = a + 1; b < nv; ++b)
+                if(ev[a] != ev[b] && groupOf[ev[a]] >= 0 && groupOf[ev[a]] == groupOf[ev[b]])
+                {
+                    err = "vertices " + std::to_string(ev[a]) + " and " + std::to_string(ev[b]) + " of group " +
+                          std::to_string(groupOf[ev[a]]) + " share " + (hex ? "hexahedron " : "tetrahedron ") +
+                          std::to_string(hex ? e - T : e);
+                    return false;
+                }
+    }
+    out.nverts = V; out.ntets = T; out.nhexs = H;
+    out.pos.assign(pos, pos + 3 * V);
+    out.fixed.resize(V);
+    for(int64_t v = 0; v < V; ++v) out.fixed[v] = fixed[v] ? 1 : 0;
+    out.tets.assign(tets, tets + 4 * T);
+    out.hexs.assign(hexs, hexs + 8 * H);
+    out.eoff = adj.eoff;
+    out.eref = adj.eref;
+    out.noff.assign(V + 1, 0);
+    for(int64_t v = 0; v < V; ++v) out.noff[v + 1] = out.noff[v] + adj.ncnt[v];
+    out.nbr.resize(out.noff[V]);
+    for(int64_t v = 0; v < V; ++v)
+        std::copy(adj.ntmp.begin() + (size_t)3 * adj.eoff[v], adj.ntmp.begin() + (size_t)3 * adj.eoff[v] + adj.ncnt[v],
+                  out.nbr.begin() + out.noff[v]);
+    out.permutation.resize(V);
+    std::iota(out.permutation.begin(), out.permutation.end(), (int64_t)0);
+    out.nfixed = nfixed;
+    out.groupOff.assign(groupOff, groupOff + G + 1);
+    if(G == 0) out.groupOff.assign(1, 0);
+    out.groupNodes.assign(groupNodes, groupNodes + M);
+    out.serialGroup = out.groupNodes;
+    out.groupMaxPatch.assign(G, 0);
+    for(int64_t g = 0; g < G; ++g)
+        for(int64_t i = groupOff[g]; i < groupOff[g + 1]; ++i)
+        {
+            const uint32_t v = groupNodes[i];
+            out.groupMaxPatch[g] = std::max<int32_t>(out.groupMaxPatch[g], (int32_t)(adj.eoff[v + 1] - adj.eoff[v]));
+        }
     return true;
 }
 

@@ -34,9 +34,9 @@ struct Topology
 
     // Node groups.
     std::vector<int64_t> permutation;      // old id -> new id
-    int64_t nfixed = 0;                    // fixed vertices occupy [0, nfixed)
+    int64_t nfixed = 0;                    // number of fixed vertices ([0, nfixed) after renumbering)
     std::vector<int64_t> groupOff;         // ngroups + 1 offsets into groupNodes
-    std::vector<uint32_t> groupNodes;      // movable vertices, colour by colour
+    std::vector<uint32_t> groupNodes;      // movable vertices, colour by colour (any ids)
     std::vector<int32_t> groupMaxPatch;    // largest patch per group
     std::vector<uint32_t> serialGroup;     // movable vertices in serial order
 
@@ -53,6 +53,15 @@ struct Topology
 bool build(int64_t nverts, const double* pos, const int32_t* fixed,
            int64_t ntets, const uint32_t* tets, int64_t nhexs, const uint32_t* hexs,
            Topology& out, std::string& err);
+
+// The topology in the caller's numbering with the caller's colour groups
+// (the reference's NodeGroups::parallelGroups): group g holds
+// groupNodes[groupOff[g] .. groupOff[g + 1]).  Checks that every movable
+// vertex is in exactly one group, no fixed vertex is in one, and no two
+// vertices of a group share an element.  permutation is the identity.
+bool buildGrouped(int64_t nverts, const double* pos, const int32_t* fixed, int64_t ntets, const uint32_t* tets,
+                  int64_t nhexs, const uint32_t* hexs, int64_t ngroups, const int64_t* groupOff,
+                  const uint32_t* groupNodes, Topology& out, std::string& err);
 
 // build() through the shared topology cache (gm_topocache.cpp): returns the
 // topology with `pos` left empty and the renumbered positions in

@@ -189,6 +189,28 @@ int makeMesh(int64_t nverts, const double* pos, const int32_t* fixed, int64_t nt
     return GM_OK;
 }
 
+int makeMeshGrouped(int64_t nverts, const double* pos, const int32_t* fixed, int64_t ntets, const uint32_t* tets,
+                    int64_t nhexs, const uint32_t* hexs, int64_t ngroups, const int64_t* groupOff,
+                    const uint32_t* groupNodes, gm_mesh** out)
+{
+    if(!out) return fail(GM_ERR_ARG, "out is null");
+    *out = nullptr;
+    if(nverts < 0 || ntets < 0 || nhexs < 0) return fail(GM_ERR_ARG, "negative size");
+    if((nverts && (!pos || !fixed)) || (ntets && !tets) || (nhexs && !hexs)) return fail(GM_ERR_ARG, "null array");
+    auto t = std::make_shared<gmt::Topology>();
+    std::string err;
+    if(!gmt::buildGrouped(nverts, pos, fixed, ntets, tets, nhexs, hexs, ngroups, groupOff, groupNodes, *t, err))
+        return fail(GM_ERR_ARG, err);
+    gm_mesh* m = new gm_mesh();
+    m->hostPos.swap(t->pos);
+    m->topo = t;
+    for(int32_t p : m->topo->groupMaxPatch) m->maxPatch = std::max(m->maxPatch, (int)p);
+    m->kind = m->topo->nhexs == 0 ? gmd::EK_TET : (m->topo->ntets == 0 ? gmd::EK_HEX : gmd::EK_MIXED);
+    m->crew.scaling = m->crew.scaling2 = m->crew.scaling3 = 1.0;
+    *out = m;
+    return GM_OK;
+}
+
 size_t residentBytes(const gm_mesh* m, int records, bool getme)
 {
     const gmt::Topology& t = *m->topo;
@@ -244,12 +266,11 @@ void destroyMesh(gm_mesh* m)
 }
 
 // Device slots and the arrays laid out by them (see gm_runtime.h): the fixed
-// vertices, then colour group by colour group, each segment in input order
-// (the reference numbering is already fixed-then-group-major: group g holds
-// ids nfixed + groupOff[g] ..).  The patch's element order and the neighbour
-// order (the order of the reference's sums) are kept.  Computed once per
-// topology and kept with it, so meshes of a cached topology upload without
-// it.  nullptr if the groups do not cover the movable vertices.
+// vertices, then colour group by colour group, each segment in input order.
+// The patch's element order and the neighbour order (the order of the
+// reference's sums) are kept.  Computed once per topology and kept with it,
+// so meshes of a cached topology upload without it.  nullptr if the groups
+// do not cover the movable vertices.
 static std::shared_ptr<const DeviceLayout> buildLayout(const gmt::Topology& t)
 {
     const size_t V = (size_t)t.nverts;
@@ -259,16 +280,17 @@ static std::shared_ptr<const DeviceLayout> buildLayout(const gmt::Topology& t)
     const size_t G = t.groupOff.empty() ? 0 : t.groupOff.size() - 1;
     if((size_t)t.nfixed + (G ? (size_t)t.groupOff[G] : 0) != V)
         return nullptr;
-    std::vector<size_t> starts(G), next(G + 1);
+    // Segment of each vertex: 0 for the fixed ones, g + 1 for group g.
+    std::vector<uint32_t> segOf(V, 0);
+    for(size_t g = 0; g < G; ++g)
+        for(int64_t i = t.groupOff[g]; i < t.groupOff[g + 1]; ++i) segOf[t.groupNodes[i]] = (uint32_t)(g + 1);
+    std::vector<size_t> next(G + 1);
     next[0] = 0;
-    for(size_t g = 0; g < G; ++g) next[g + 1] = starts[g] = (size_t)t.nfixed + (size_t)t.groupOff[g];
+    for(size_t g = 0; g < G; ++g) next[g + 1] = (size_t)t.nfixed + (size_t)t.groupOff[g];
     for(size_t o = 0; o < V; ++o)
     {
         const size_t v = (size_t)t.permutation[o];
-        const size_t seg = v < (size_t)t.nfixed
-                               ? 0
-                               : (size_t)(std::upper_bound(starts.begin(), starts.end(), v) - starts.begin());
-        vertexAt[next[seg]++] = (uint32_t)v;
+        vertexAt[next[segOf[v]]++] = (uint32_t)v;
     }
     std::vector<uint32_t>& slot = L->slotOf;
     slot.assign(V, 0);
@@ -866,7 +888,7 @@ int launchToF64All(Plan* p)
 int sweepColours(Plan* p, const gmq::Crew& c, int ev, int sa, const RunParams& r, int64_t g0, int64_t g1)
 {
     for(gm_mesh* m : p->meshes) m->dm.autopilot = 0;
-    int rc = preparePlan(p, 1, false);
+    int rc = preparePlan(p, 1, r.smoother == gmd::SM_GETME);
     if(rc) return rc;
     hipStream_t s = p->stream;
     p->poolUsed = 0;

@@ -47,6 +47,30 @@ class GpuMesh:
         (self.nverts, self.ntets, self.nhexs, self.nmovable, self.ngroups, self.sum_patch,
          self.sum_nbr, self.max_patch) = (int(x) for x in info)
 
+    @classmethod
+    def grouped(cls, pos, fixed, tets, hexs, group_off, group_nodes, device: bool = True):
+        """A mesh in the caller's numbering with the caller's colour groups
+        (gm_mesh_create_grouped): no renumbering, groups checked on the host."""
+        L = lib()
+        self = cls.__new__(cls)
+        pos = np.ascontiguousarray(pos, np.float64).reshape(-1, 3)
+        fixed = np.ascontiguousarray(fixed, np.int32).reshape(-1)
+        tets = np.ascontiguousarray(tets, np.uint32).reshape(-1, 4)
+        hexs = np.ascontiguousarray(hexs, np.uint32).reshape(-1, 8)
+        off = np.ascontiguousarray(group_off, np.int64)
+        nodes = np.ascontiguousarray(group_nodes, np.uint32)
+        h = C.c_void_p()
+        check(L.gm_mesh_create_grouped(pos.shape[0], ptr(pos), ptr(fixed), tets.shape[0], ptr(tets), hexs.shape[0],
+                                       ptr(hexs), off.size - 1, ptr(off), ptr(nodes), int(device), C.byref(h)),
+              "gm_mesh_create_grouped")
+        self._h = h
+        self.device = device
+        info = np.zeros(8, np.int64)
+        check(L.gm_mesh_info(self._h, ptr(info)), "gm_mesh_info")
+        (self.nverts, self.ntets, self.nhexs, self.nmovable, self.ngroups, self.sum_patch,
+         self.sum_nbr, self.max_patch) = (int(x) for x in info)
+        return self
+
     def device_bytes(self) -> int:
         """Device memory held by the handle."""
         b = np.zeros(1, np.int64)

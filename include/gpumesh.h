@@ -74,6 +74,23 @@ int gm_mesh_create(int64_t nverts, const double* pos, const int32_t* fixed,
 int gm_mesh_create_host(int64_t nverts, const double* pos, const int32_t* fixed,
                         int64_t ntets, const uint32_t* tets, int64_t nhexs, const uint32_t* hexs,
                         gm_mesh** out);
+/* A mesh in the caller's own numbering with the caller's colour groups,
+ * the form in which the reference already holds them: group g is
+ * group_nodes[group_off[g] .. group_off[g+1]) (group_off[0] = 0, ngroups + 1
+ * entries), the concatenation of NodeGroups::parallelGroups()[g]
+ * .undispatchedNodes (DataStructures/NodeGroups.cpp:452-619; the GPU buffer
+ * of NodeGroups::gpuGroupsBuffer, :621-693, when no vertex slides on the
+ * boundary).  Fixed vertices belong to no group; every other vertex to
+ * exactly one; no two vertices of a group may share an element -- all
+ * checked on the host (GM_ERR_ARG naming the offending vertices).  Sweeps
+ * visit the groups in order.  No renumbering: positions, elements and counts
+ * stay in the caller's numbering and gm_mesh_permutation returns the
+ * identity.  device_copy = 0 builds the host topology only (as
+ * gm_mesh_create_host).  Replaces Mesh::updateCudaTopology and
+ * updateCudaGroupMembers (DataStructures/GpuMesh.cpp:14-25, 227-276). */
+int gm_mesh_create_grouped(int64_t nverts, const double* pos, const int32_t* fixed, int64_t ntets,
+                           const uint32_t* tets, int64_t nhexs, const uint32_t* hexs, int64_t ngroups,
+                           const int64_t* group_off, const uint32_t* group_nodes, int device_copy, gm_mesh** out);
 int gm_mesh_destroy(gm_mesh* mesh);
 
 /* info[8] = {nverts, ntets, nhexs, nmovable, ngroups, sum of patch sizes,
@@ -161,9 +178,10 @@ int gm_mesh_smooth_schedule(gm_mesh* mesh, int smoother, int max_passes, const g
  * colour groups [group_begin, group_end), with no quality evaluation: the
  * body of one pass of AbstractVertexWiseSmoother::smoothMeshThread
  * (Smoothers/VertexWise/AbstractVertexWiseSmoother.cpp:114-138) restricted to
- * those groups.  precision: GM_FP64 or GM_FP32 (FP32 rounds the positions
- * once before and widens them after).  GETMe returns GM_ERR_UNSUPPORTED (its
- * sweep starts with an element step over the whole mesh). */
+ * those groups.  precision: GM_FP64, GM_FP64_FAST or GM_FP32 (FP32 rounds
+ * the positions once before and widens them after).  GETMe runs only over
+ * every group, [0, ngroups) (its sweep starts with an element step over the
+ * whole mesh); other ranges return GM_ERR_UNSUPPORTED. */
 int gm_mesh_smooth_groups(gm_mesh* mesh, int smoother, int64_t group_begin, int64_t group_end, int precision,
                           int kernel, double spawn_coeff, int32_t* eval_counts);
 
