@@ -29,7 +29,7 @@ EXPORTS = [
     "gm_server_result", "gm_server_release", "gm_server_stats", "gm_cube_mesh", "gm_set_topology_cache", "gm_set_device_pool",
     "gm_topology_cache_stats", "gm_math_probe", "gm_mesh_smooth_schedule", "gm_batch_smooth_schedule",
     "gm_job_size", "gm_job_encode", "gm_job_info", "gm_job_decode", "gm_server_submit_job",
-    "gm_set_polish_passes", "gm_evaluate_noalloc", "gm_mesh_create_grouped",
+    "gm_set_polish_passes", "gm_evaluate_noalloc", "gm_mesh_create_grouped", "gm_mesh_set_aspect_ratio",
 ]
 
 
@@ -70,6 +70,7 @@ def lib():
     L.gm_mesh_set_positions.argtypes = [vp, vp]
     L.gm_mesh_set_crew.argtypes = [vp, C.c_int, C.c_int, dbl, vp, vp, vp, vp]
     L.gm_mesh_evaluate.argtypes = [vp, vp, vp, vp]
+    L.gm_mesh_set_aspect_ratio.argtypes = [vp, dbl]
     L.gm_mesh_smooth.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, dbl, vp, vp, vp]
     L.gm_mesh_smooth_groups.argtypes = [vp, C.c_int, i64, i64, C.c_int, C.c_int, dbl, vp]
     L.gm_mesh_last_timing.argtypes = [vp, vp]

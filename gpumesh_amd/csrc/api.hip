@@ -316,13 +316,14 @@ int gm_mesh_set_crew(gm_mesh* m, int evaluator, int sampler, double scaling,
 {
     GM_NEED_DEVICE(m);
     if(evaluator < 0 || evaluator > 1) return fail(GM_ERR_ARG, "unknown evaluator");
-    if(sampler < 0 || sampler > 2) return fail(GM_ERR_ARG, "unknown sampler");
+    if(sampler < 0 || sampler > 3) return fail(GM_ERR_ARG, "unknown sampler");
     if(!(scaling > 0.0)) return fail(GM_ERR_ARG, "scaling must be positive");
     GM_HIP(hipSetDevice(m->device));
     gmq::Crew c{};
     c.scaling = scaling;
     c.scaling2 = scaling * scaling;
     c.scaling3 = c.scaling2 * scaling;
+    gmr::setAnalyticField(c, m->aspect);
     if(sampler == GM_TEXTURE)
     {
         if(!tsize || !tmin || !text || !texels) return fail(GM_ERR_ARG, "texture sampler needs a grid");
@@ -353,6 +354,15 @@ int gm_mesh_set_crew(gm_mesh* m, int evaluator, int sampler, double scaling,
     m->crew = c;
     m->evaluator = evaluator;
     m->sampler = sampler;
+    return GM_OK;
+}
+
+int gm_mesh_set_aspect_ratio(gm_mesh* m, double aspect)
+{
+    if(!m) return fail(GM_ERR_ARG, "null mesh");
+    if(!(aspect > 0.0) || aspect == (double)__builtin_huge_val()) return fail(GM_ERR_ARG, "aspect ratio must be positive and finite");
+    m->aspect = aspect;
+    m->crew.aspect = aspect;
     return GM_OK;
 }
 
@@ -494,8 +504,9 @@ int gm_batch_create(gm_mesh* const* meshes, int64_t n, gm_batch** out)
         if(!m || !m->onDevice) return fail(GM_ERR_ARG, "batch member without a device copy");
         if(m->device != ms[0]->device) return fail(GM_ERR_ARG, "batch members live on different devices");
         if(m->sampler == GM_TEXTURE) return fail(GM_ERR_UNSUPPORTED, "batches do not take the texture sampler");
-        if(m->evaluator != ms[0]->evaluator || m->sampler != ms[0]->sampler || m->crew.scaling != ms[0]->crew.scaling)
-            return fail(GM_ERR_ARG, "batch members need the same evaluator, sampler and scaling");
+        if(m->evaluator != ms[0]->evaluator || m->sampler != ms[0]->sampler || m->crew.scaling != ms[0]->crew.scaling ||
+           m->crew.aspect != ms[0]->crew.aspect)
+            return fail(GM_ERR_ARG, "batch members need the same evaluator, sampler, scaling and aspect ratio");
     }
     std::vector<gm_mesh*> sorted = ms;
     std::sort(sorted.begin(), sorted.end());
@@ -549,7 +560,7 @@ static int checkBatchCrew(const gm_batch* b)
     const std::vector<gm_mesh*>& ms = b->plan->meshes;
     for(const gm_mesh* m : ms)
         if(m->sampler == GM_TEXTURE || m->evaluator != ms[0]->evaluator || m->sampler != ms[0]->sampler ||
-           m->crew.scaling != ms[0]->crew.scaling)
+           m->crew.scaling != ms[0]->crew.scaling || m->crew.aspect != ms[0]->crew.aspect)
             return fail(GM_ERR_ARG, "batch members' crews diverged after the batch was made");
     return GM_OK;
 }
@@ -702,7 +713,8 @@ int gm_batch_autotune(gm_batch* b, int smoother, int precision, int passes, int3
 
 int gm_math_probe(int op, const double* a, const double* b, double* out, int64_t n)
 {
-    if(op < 0 || op > 5 || n < 0 || (n && (!a || !out || (op == 1 && !b)))) return fail(GM_ERR_ARG, "bad probe arguments");
+    if(op < 0 || op > 8 || n < 0 || (n && (!a || !out || ((op == 1 || op == 8) && !b))))
+        return fail(GM_ERR_ARG, "bad probe arguments");
     if(n == 0) return GM_OK;
     int ndev = 0;
     if(hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(GM_ERR_HIP, "no HIP device visible");

@@ -5,26 +5,30 @@ namespace gmd
 {
 
 #define GMD_SELECT(EVV, SAV, T, CALL)                                               \
-    switch((EVV) * 3 + (SAV))                                                       \
+    switch((EVV) * 4 + (SAV))                                                       \
     {                                                                               \
     case 0: return Launchers<T, 0, 0>::CALL;                                        \
     case 1: return Launchers<T, 0, 1>::CALL;                                        \
     case 2: return Launchers<T, 0, 2>::CALL;                                        \
-    case 3: return Launchers<T, 1, 0>::CALL;                                        \
-    case 4: return Launchers<T, 1, 1>::CALL;                                        \
-    case 5: return Launchers<T, 1, 2>::CALL;                                        \
+    case 3: return Launchers<T, 0, 3>::CALL;                                        \
+    case 4: return Launchers<T, 1, 0>::CALL;                                        \
+    case 5: return Launchers<T, 1, 1>::CALL;                                        \
+    case 6: return Launchers<T, 1, 2>::CALL;                                        \
+    case 7: return Launchers<T, 1, 3>::CALL;                                        \
     default: return hipErrorInvalidValue;                                           \
     }
 
 #define GMD_SELECT_EVAL(EVV, SAV, CALL)                                             \
-    switch((EVV) * 3 + (SAV))                                                       \
+    switch((EVV) * 4 + (SAV))                                                       \
     {                                                                               \
     case 0: return Evaluators<0, 0>::CALL;                                          \
     case 1: return Evaluators<0, 1>::CALL;                                          \
     case 2: return Evaluators<0, 2>::CALL;                                          \
-    case 3: return Evaluators<1, 0>::CALL;                                          \
-    case 4: return Evaluators<1, 1>::CALL;                                          \
-    case 5: return Evaluators<1, 2>::CALL;                                          \
+    case 3: return Evaluators<0, 3>::CALL;                                          \
+    case 4: return Evaluators<1, 0>::CALL;                                          \
+    case 5: return Evaluators<1, 1>::CALL;                                          \
+    case 6: return Evaluators<1, 2>::CALL;                                          \
+    case 7: return Evaluators<1, 3>::CALL;                                          \
     default: return hipErrorInvalidValue;                                           \
     }
 

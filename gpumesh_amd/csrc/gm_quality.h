@@ -131,7 +131,7 @@ template<typename T> GMQ m3<T> inverse(const m3<T>& m)
 // Crew (sampler / measurer / evaluator selection and parameters).
 // ---------------------------------------------------------------------------
 enum { EV_MEAN_RATIO = 0, EV_METRIC_CONFORMITY = 1 };
-enum { SA_UNIFORM = 0, SA_SHELL = 1, SA_TEXTURE = 2 };
+enum { SA_UNIFORM = 0, SA_SHELL = 1, SA_TEXTURE = 2, SA_ANALYTIC = 3 };
 
 struct Crew
 {
@@ -139,6 +139,10 @@ struct Crew
     int tsize[3];
     double tmin[3], text[3];
     const float* texels;   // 6 per cell: m00 m01 m02 m11 m12 m22; cell (i,j,k) at ((k ny)+j) nx + i
+    // The reference's analytic field: aspect ratio and the sampler's
+    // rotation (column-major, rot[3 c + r]) and its transpose.
+    double aspect;
+    double rot[9], rotInv[9];
 };
 
 // UniformSampler::metricAt (Samplers/UniformSampler.cpp:45-51): K^2 I.
@@ -210,8 +214,38 @@ template<typename T> GMQ m3<T> textureMetric(const Crew& c, v3<T> position)
     return mixm<T>(mxy0, mxy1, a.z);
 }
 
+// AbstractSampler::vertMetric (Samplers/AbstractSampler.cpp:190-208), the
+// AnalyticSampler's field (AnalyticSampler.cpp:45-49): along the rotated x
+// axis the metric length scale is K a^(c^a) with c = (1 - cos(2.5 pi x'))/2,
+// K across it; M = R^T diag(Mx, K^2, K^2) R.  Evaluated in FP64 in both
+// precisions (the project's cos and pow, so the host and the device agree).
+GMQ m3<double> referenceMetric(const Crew& c, v3<double> p)
+{
+    m3<double> R, RI;
+    for(int j = 0; j < 3; ++j)
+    {
+        R.c[j] = mk<double>(c.rot[3 * j], c.rot[3 * j + 1], c.rot[3 * j + 2]);
+        RI.c[j] = mk<double>(c.rotInv[3 * j], c.rotInv[3 * j + 1], c.rotInv[3 * j + 2]);
+    }
+    const double x = mvec<double>(R, p).x * (2.5 * 3.14159265358979323846);
+    const double a = c.aspect;
+    const double cc = (1.0 - gm_cos(x)) / 2.0;
+    const double sizeX = c.scaling * gm_pow(a, gm_pow(cc, a));
+    const double Mx = sizeX * sizeX;
+    const double My = c.scaling2;
+    const m3<double> M = mcols<double>(mk<double>(Mx, 0.0, 0.0), mk<double>(0.0, My, 0.0), mk<double>(0.0, 0.0, My));
+    return mmul<double>(mmul<double>(RI, M), R);
+}
+
 template<typename T, int SA> GMQ m3<T> metricAt(const Crew& c, v3<T> p)
 {
+    if constexpr(SA == SA_ANALYTIC)
+    {
+        const m3<double> M = referenceMetric(c, mk<double>((double)p.x, (double)p.y, (double)p.z));
+        m3<T> r;
+        for(int j = 0; j < 3; ++j) r.c[j] = mk<T>((T)M.c[j].x, (T)M.c[j].y, (T)M.c[j].z);
+        return r;
+    }
     if(SA == SA_SHELL) return shellMetric<T>(c, p);
     if(SA == SA_TEXTURE) return textureMetric<T>(c, p);
     T s2 = (T)c.scaling * (T)c.scaling;

@@ -107,6 +107,7 @@ struct gm_mesh
     int evaluator = GM_MEAN_RATIO;
     int sampler = GM_UNIFORM;
     gmq::Crew crew{};
+    double aspect = 1.0;      // aspect ratio of the GM_ANALYTIC field
     float* texels = nullptr;
     gmr::Plan* plan = nullptr; // the mesh as a batch of one
 };
@@ -155,6 +156,11 @@ size_t poolCachedBytes();
 // Host topology of a new handle (through the topology cache).
 int makeMesh(int64_t nverts, const double* pos, const int32_t* fixed, int64_t ntets, const uint32_t* tets,
              int64_t nhexs, const uint32_t* hexs, gm_mesh** out);
+// The GM_ANALYTIC field's parameters of a crew: the aspect ratio and the
+// sampler's fixed rotation (AbstractSampler's constructor,
+// Samplers/AbstractSampler.cpp:30-41) and its transpose.
+void setAnalyticField(gmq::Crew& c, double aspect);
+
 // makeMesh with the caller's numbering and colour groups (gmt::buildGrouped).
 int makeMeshGrouped(int64_t nverts, const double* pos, const int32_t* fixed, int64_t ntets, const uint32_t* tets,
                     int64_t nhexs, const uint32_t* hexs, int64_t ngroups, const int64_t* groupOff,

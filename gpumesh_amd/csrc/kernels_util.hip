@@ -106,6 +106,9 @@ __global__ void mathProbeKernel(int op, const double* a, const double* b, double
     case 2: r = gm_pow23(x); break;
     case 3: r = gm_cbrt_nonneg(x); break;
     case 5: r = gma::powm23(x); break;
+    case 6: r = gm_cos(x); break;
+    case 7: r = gm_log(x); break;
+    case 8: r = gm_pow(x, b[i]); break;
     default: r = gm_exp(x); break;
     }
     out[i] = r;

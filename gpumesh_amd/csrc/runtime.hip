@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <atomic>
+#include <cmath>
 
 #include <algorithm>
 #include <cstring>
@@ -185,8 +186,51 @@ int makeMesh(int64_t nverts, const double* pos, const int32_t* fixed, int64_t nt
     for(int32_t p : m->topo->groupMaxPatch) m->maxPatch = std::max(m->maxPatch, (int)p);
     m->kind = m->topo->nhexs == 0 ? gmd::EK_TET : (m->topo->ntets == 0 ? gmd::EK_HEX : gmd::EK_MIXED);
     m->crew.scaling = m->crew.scaling2 = m->crew.scaling3 = 1.0;
+    setAnalyticField(m->crew, m->aspect);
     *out = m;
     return GM_OK;
+}
+
+// glm::rotate(m, angle, axis) on the 3x3 block (column-major, m[3 c + r]):
+// m times the axis-angle rotation, in the library's published formula.
+static void rotate3(double m[9], double angle, double ax, double ay, double az)
+{
+    const double c = std::cos(angle), s = std::sin(angle);
+    const double inv = 1.0 / std::sqrt(ax * ax + ay * ay + az * az);
+    const double axis[3] = {ax * inv, ay * inv, az * inv};
+    const double temp[3] = {(1.0 - c) * axis[0], (1.0 - c) * axis[1], (1.0 - c) * axis[2]};
+    double R[3][3];
+    R[0][0] = c + temp[0] * axis[0];
+    R[0][1] = temp[0] * axis[1] + s * axis[2];
+    R[0][2] = temp[0] * axis[2] - s * axis[1];
+    R[1][0] = temp[1] * axis[0] - s * axis[2];
+    R[1][1] = c + temp[1] * axis[1];
+    R[1][2] = temp[1] * axis[2] + s * axis[0];
+    R[2][0] = temp[2] * axis[0] + s * axis[1];
+    R[2][1] = temp[2] * axis[1] - s * axis[0];
+    R[2][2] = c + temp[2] * axis[2];
+    double out[9];
+    for(int i = 0; i < 3; ++i)
+        for(int r = 0; r < 3; ++r)
+            out[3 * i + r] = m[0 * 3 + r] * R[i][0] + m[1 * 3 + r] * R[i][1] + m[2 * 3 + r] * R[i][2];
+    std::memcpy(m, out, sizeof(out));
+}
+
+void setAnalyticField(gmq::Crew& c, double aspect)
+{
+    // The rotation starts from the identity, turns by pi/4 about z, then by
+    // pi/4 about -y; the inverse is its transpose.
+    const double pi = 3.14159265358979323846;
+    double r[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    rotate3(r, pi / 4.0, 0.0, 0.0, 1.0);
+    rotate3(r, pi / 4.0, 0.0, -1.0, 0.0);
+    c.aspect = aspect;
+    for(int j = 0; j < 3; ++j)
+        for(int i = 0; i < 3; ++i)
+        {
+            c.rot[3 * j + i] = r[3 * j + i];
+            c.rotInv[3 * j + i] = r[3 * i + j];
+        }
 }
 
 int makeMeshGrouped(int64_t nverts, const double* pos, const int32_t* fixed, int64_t ntets, const uint32_t* tets,
@@ -207,6 +251,7 @@ int makeMeshGrouped(int64_t nverts, const double* pos, const int32_t* fixed, int
     for(int32_t p : m->topo->groupMaxPatch) m->maxPatch = std::max(m->maxPatch, (int)p);
     m->kind = m->topo->nhexs == 0 ? gmd::EK_TET : (m->topo->ntets == 0 ? gmd::EK_HEX : gmd::EK_MIXED);
     m->crew.scaling = m->crew.scaling2 = m->crew.scaling3 = 1.0;
+    setAnalyticField(m->crew, m->aspect);
     *out = m;
     return GM_OK;
 }

@@ -520,9 +520,9 @@ int gm_server_create(int64_t budgetBytes, int maxActive, int workerThreads, int 
     if(!out) return fail(GM_ERR_ARG, "out is null");
     *out = nullptr;
     if(evaluator < 0 || evaluator > 1) return fail(GM_ERR_ARG, "unknown evaluator");
-    if(sampler < 0 || sampler > 1)
+    if(sampler < 0 || sampler > 3 || sampler == GM_TEXTURE)
         return fail(sampler == GM_TEXTURE ? GM_ERR_UNSUPPORTED : GM_ERR_ARG,
-                    "the server takes the uniform or analytic sampler");
+                    "the server takes the uniform or an analytic sampler");
     if(!(scaling > 0.0)) return fail(GM_ERR_ARG, "scaling must be positive");
     if(maxActive < 1 || workerThreads < 1 || workerThreads > 64) return fail(GM_ERR_ARG, "bad slot or thread count");
     int ndev = 0;
@@ -542,6 +542,7 @@ int gm_server_create(int64_t budgetBytes, int maxActive, int workerThreads, int 
     s->crew.scaling = scaling;
     s->crew.scaling2 = scaling * scaling;
     s->crew.scaling3 = s->crew.scaling2 * scaling;
+    gmr::setAnalyticField(s->crew, 1.0);
     gm_server* raw = s.get();
     for(int i = 0; i < workerThreads; ++i) raw->workers.emplace_back(workerLoop, raw);
     raw->scheduler = std::thread(schedulerLoop, raw);

@@ -17,7 +17,7 @@ from ._lib import Schedule, check, lib, ptr
 SMOOTHERS = {"nm": 0, "nelder_mead": 0, "gd": 1, "gradient_descent": 1, "spawn": 2, "spawn_search": 2,
              "getme": 3}
 EVALUATORS = {"mean_ratio": 0, "metric_conformity": 1}
-SAMPLERS = {"uniform": 0, "analytic": 1, "texture": 2}
+SAMPLERS = {"uniform": 0, "analytic": 1, "texture": 2, "analytic_reference": 3}
 PRECISIONS = {"fp64": 0, "fp32": 1, "fp32_polish": 2, "fp64_fast": 3}
 KERNELS = {"plain": 0, "wave": 1}
 # gm_kernel_class order (include/gpumesh.h).
@@ -129,6 +129,10 @@ class GpuMesh:
         check(lib().gm_mesh_set_positions(self._h, ptr(pos)), "gm_mesh_set_positions")
 
     # ----- crew
+    def set_aspect_ratio(self, aspect: float) -> None:
+        """Aspect ratio of the reference's analytic field ("analytic_reference")."""
+        check(lib().gm_mesh_set_aspect_ratio(self._h, float(aspect)), "gm_mesh_set_aspect_ratio")
+
     def set_crew(self, evaluator="mean_ratio", sampler="uniform", scaling=1.0, texture=None) -> None:
         args = [None, None, None, None]
         if sampler == "texture":

@@ -38,7 +38,11 @@ enum gm_status
 
 enum gm_smoother { GM_NELDER_MEAD = 0, GM_GRADIENT_DESCENT = 1, GM_SPAWN_SEARCH = 2, GM_GETME = 3 };
 enum gm_evaluator { GM_MEAN_RATIO = 0, GM_METRIC_CONFORMITY = 1 };
-enum gm_sampler { GM_UNIFORM = 0, GM_ANALYTIC_SHELL = 1, GM_TEXTURE = 2 };
+/* GM_ANALYTIC_SHELL: the spherical-shell field of the anisotropic hexahedral
+ * workload; GM_ANALYTIC: the reference's AnalyticSampler field (a rotated
+ * 1-D sinusoidal anisotropy, Samplers/AbstractSampler.cpp:190-208; aspect
+ * ratio from gm_mesh_set_aspect_ratio). */
+enum gm_sampler { GM_UNIFORM = 0, GM_ANALYTIC_SHELL = 1, GM_TEXTURE = 2, GM_ANALYTIC = 3 };
 /* GM_FP64: every pass in FP64 with the reference's operation order (bit-exact
  * with an FP64 CPU build).  GM_FP32: every pass in FP32.  GM_FP32_POLISH: FP32
  * passes, then the last gm_set_polish_passes() passes (default 1) in FP64 on
@@ -122,6 +126,12 @@ int gm_mesh_set_positions(gm_mesh* mesh, const double* pos);
  * updateCudaSamplerTextures (Samplers/TextureSampler.cpp:75-80). */
 int gm_mesh_set_crew(gm_mesh* mesh, int evaluator, int sampler, double scaling,
                      const int32_t* tsize, const double* tmin, const double* text, const float* texels);
+
+/* Aspect ratio of the GM_ANALYTIC field (AbstractSampler::setAspectRatio,
+ * Samplers/AbstractSampler.cpp:56-59; default 1), kept across
+ * gm_mesh_set_crew calls.  Replaces setCudaMetricAspectRatio
+ * (AbstractSampler.cpp:14, 103-111). */
+int gm_mesh_set_aspect_ratio(gm_mesh* mesh, double aspect_ratio);
 
 /* Mesh quality in FP64 (AbstractEvaluator::evaluateMeshQuality*,
  * Evaluators/AbstractEvaluator.cpp:384-453, 574-610; replaces
@@ -395,8 +405,8 @@ int gm_evaluate_noalloc(const double* d_pos, int64_t nverts, const uint32_t* d_t
 
 /* Device math probe (for tests): op 0 sqrt(a), 1 a/b, 2 a^(2/3), 3 a^(1/3),
  * 4 exp(a), 5 a^(-2/3) as the FP32-polish and evaluation-record kernels
- * compute it (hardware seed, one Halley step), all in FP64 on the device;
- * n values. */
+ * compute it (hardware seed, one Halley step), 6 cos(a), 7 ln(a), 8 a^b,
+ * all in FP64 on the device; n values. */
 int gm_math_probe(int op, const double* a, const double* b, double* out, int64_t n);
 
 #ifdef __cplusplus

@@ -150,6 +150,8 @@ static int g_libm = 1;
 static inline double pow23(double x) { return g_libm ? gm_pow23(x) : std::pow(x, 2.0 / 3.0); }
 static inline double pow13(double x) { return g_libm ? gm_cbrt_nonneg(x) : std::pow(x, 1.0 / 3.0); }
 static inline double dexp(double x) { return g_libm ? gm_exp(x) : std::exp(x); }
+static inline double dcos(double x) { return g_libm ? gm_cos(x) : std::cos(x); }
+static inline double dpow(double x, double y) { return g_libm ? gm_pow(x, y) : std::pow(x, y); }
 
 // ------------------------------------------------------------------------
 // Mesh model (DataStructures/Mesh.h:25-267).
@@ -181,7 +183,7 @@ struct ParallelGroup
 // Crew selections.
 enum { EVAL_MEAN_RATIO = 0, EVAL_METRIC_CONFORMITY = 1 };
 enum { MEAS_METRIC_FREE = 0, MEAS_METRIC_WISE = 1 };
-enum { SAMP_UNIFORM = 0, SAMP_ANALYTIC_SHELL = 1, SAMP_TEXTURE = 2 };
+enum { SAMP_UNIFORM = 0, SAMP_ANALYTIC_SHELL = 1, SAMP_TEXTURE = 2, SAMP_ANALYTIC = 3 };
 
 struct Crew
 {
@@ -194,7 +196,46 @@ struct Crew
     int tsize[3] = {0, 0, 0};
     dvec3 tmin, text;
     std::vector<float> texels;
+    // AbstractSampler's aspect ratio and rotation (AbstractSampler.cpp:17-41).
+    double aspect = 1.0;
+    dmat3 rotMat, rotInv;
 };
+
+// glm::rotate (GLM's matrix_transform) restricted to the 3x3 block:
+// m times the rotation by `angle` about the normalised axis.
+static dmat3 glmRotate(const dmat3& m, double angle, dvec3 v)
+{
+    const double a = angle;
+    const double c = std::cos(a);
+    const double s = std::sin(a);
+    dvec3 axis = v * (1.0 / std::sqrt(dot(v, v)));
+    dvec3 temp = axis * (1.0 - c);
+    dmat3 R;
+    R[0][0] = c + temp[0] * axis[0];
+    R[0][1] = temp[0] * axis[1] + s * axis[2];
+    R[0][2] = temp[0] * axis[2] - s * axis[1];
+    R[1][0] = temp[1] * axis[0] - s * axis[2];
+    R[1][1] = c + temp[1] * axis[1];
+    R[1][2] = temp[1] * axis[2] + s * axis[0];
+    R[2][0] = temp[2] * axis[0] + s * axis[1];
+    R[2][1] = temp[2] * axis[1] - s * axis[0];
+    R[2][2] = c + temp[2] * axis[2];
+    dmat3 out;
+    out[0] = m[0] * R[0][0] + m[1] * R[0][1] + m[2] * R[0][2];
+    out[1] = m[0] * R[1][0] + m[1] * R[1][1] + m[2] * R[1][2];
+    out[2] = m[0] * R[2][0] + m[1] * R[2][1] + m[2] * R[2][2];
+    return out;
+}
+
+// AbstractSampler's constructor (AbstractSampler.cpp:30-41).
+static void samplerRotation(Crew& c)
+{
+    const double pi = 3.14159265358979323846;
+    c.rotMat = dmat3(1.0);
+    c.rotMat = glmRotate(c.rotMat, pi / 4.0, dvec3(0, 0, 1));
+    c.rotMat = glmRotate(c.rotMat, pi / 4.0, dvec3(0, -1, 0));
+    c.rotInv = transpose(c.rotMat);
+}
 
 struct Mesh
 {
@@ -442,6 +483,21 @@ static dmat3 shellMetric(const Crew& c, const dvec3& p)
     return dmat3(dvec3(m00, m01, m02), dvec3(m01, m11, m12), dvec3(m02, m12, m22));
 }
 
+// AbstractSampler::vertMetric (AbstractSampler.cpp:190-208), the field of
+// AnalyticSampler::metricAt (AnalyticSampler.cpp:45-49).
+static dmat3 analyticMetric(const Crew& c, const dvec3& position)
+{
+    double x = (c.rotMat * position).x * (2.5 * 3.14159265358979323846);
+    double a = c.aspect;
+    double cc = (1.0 - dcos(x)) / 2.0;
+    double sizeX = c.scaling * dpow(a, dpow(cc, a));
+    double Mx = sizeX * sizeX;
+    double My = c.scaling2;
+    double Mz = My;
+    dmat3 M(dvec3(Mx, 0, 0), dvec3(0, My, 0), dvec3(0, 0, Mz));
+    return c.rotInv * M * c.rotMat;
+}
+
 static dmat3 texel(const Crew& c, int i, int j, int k)
 {
     const float* t = &c.texels[6 * (((size_t)k * c.tsize[1] + j) * c.tsize[0] + i)];
@@ -495,6 +551,7 @@ static dmat3 metricAt(const Crew& c, const dvec3& p)
     switch(c.sampler)
     {
     case SAMP_ANALYTIC_SHELL: return shellMetric(c, p);
+    case SAMP_ANALYTIC: return analyticMetric(c, p);
     case SAMP_TEXTURE: return textureMetric(c, p);
     default: return dmat3(c.scaling * c.scaling);
     }
@@ -1170,6 +1227,7 @@ void* orc_mesh_create(int64_t nv, const double* pos, const int32_t* fixed,
     for(int64_t e = 0; e < nhex; ++e) for(int k = 0; k < 8; ++k) m->hexs[e][k] = hexs[8 * e + k];
     compileNeighborhoods(*m);
     buildNodeGroups(*m);
+    samplerRotation(m->crew);
     return m;
 }
 
@@ -1260,6 +1318,7 @@ int orc_set_crew(void* h, int evaluator, int sampler, int measurer, double scali
     c.sampler = sampler;
     c.measurer = measurer >= 0 ? measurer : (sampler == SAMP_UNIFORM ? MEAS_METRIC_FREE : MEAS_METRIC_WISE);
     c.scaling = scaling; c.scaling2 = c.scaling * scaling; c.scaling3 = c.scaling2 * scaling;
+    samplerRotation(c);
     if(sampler == SAMP_TEXTURE)
     {
         for(int d = 0; d < 3; ++d) c.tsize[d] = tsize[d];
@@ -1270,6 +1329,9 @@ int orc_set_crew(void* h, int evaluator, int sampler, int measurer, double scali
     }
     return 0;
 }
+
+// AbstractSampler::setAspectRatio (AbstractSampler.cpp:56-59).
+void orc_set_aspect(void* h, double aspect) { ((Mesh*)h)->crew.aspect = aspect; }
 
 // out: [min, invSum, n], hist[20]
 void orc_evaluate(void* h, int chunks, double* out, int32_t* hist)
@@ -1302,12 +1364,16 @@ void orc_element_qualities(void* h, double* out)
     for(size_t e = 0; e < m->hexs.size(); ++e) out[i++] = hexQualityMesh(*m, (uint32_t)e);
 }
 
-// The project libm on the host (op 2 pow23, 3 cbrt, 4 exp), for the device probe test.
-void orc_gm_libm(int op, const double* in, double* out, int64_t n)
+// The project libm on the host (op 2 pow23, 3 cbrt, 4 exp, 6 cos, 7 log,
+// 8 pow(a, b)), for the device probe test.
+void orc_gm_libm2(int op, const double* a, const double* b, double* out, int64_t n)
 {
     for(int64_t i = 0; i < n; ++i)
-        out[i] = op == 2 ? gm_pow23(in[i]) : op == 3 ? gm_cbrt_nonneg(in[i]) : gm_exp(in[i]);
+        out[i] = op == 2 ? gm_pow23(a[i]) : op == 3 ? gm_cbrt_nonneg(a[i]) : op == 4 ? gm_exp(a[i])
+               : op == 6 ? gm_cos(a[i]) : op == 7 ? gm_log(a[i]) : gm_pow(a[i], b[i]);
 }
+
+void orc_gm_libm(int op, const double* in, double* out, int64_t n) { orc_gm_libm2(op, in, in, out, n); }
 
 double orc_patch_quality(void* h, uint32_t vId) { return patchQuality(*(Mesh*)h, vId, nullptr); }
 

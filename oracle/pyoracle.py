@@ -15,7 +15,7 @@ _LIB = None
 
 SMOOTHERS = {"nm": 0, "gd": 1, "spawn": 2, "getme": 3}
 EVALUATORS = {"mean_ratio": 0, "metric_conformity": 1}
-SAMPLERS = {"uniform": 0, "analytic": 1, "texture": 2}
+SAMPLERS = {"uniform": 0, "analytic": 1, "texture": 2, "analytic_reference": 3}
 
 _P = np.ctypeslib.ndpointer
 
@@ -54,12 +54,14 @@ def lib():
     L.orc_mesh_ngroups.restype = i64
     L.orc_mesh_groups.argtypes = [vp, _P(np.int64), _P(np.int64)]
     L.orc_set_crew.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_double, vp, vp, vp, vp]
+    L.orc_set_aspect.argtypes = [vp, C.c_double]
     L.orc_evaluate.argtypes = [vp, C.c_int, _P(np.float64), _P(np.int32)]
     for name in ("orc_tet_quality", "orc_hex_quality", "orc_tet_volume", "orc_hex_volume"):
         getattr(L, name).argtypes = [vp, _P(np.float64)]
         getattr(L, name).restype = C.c_double
     L.orc_element_qualities.argtypes = [vp, _P(np.float64)]
     L.orc_gm_libm.argtypes = [C.c_int, _P(np.float64), _P(np.float64), i64]
+    L.orc_gm_libm2.argtypes = [C.c_int, _P(np.float64), _P(np.float64), _P(np.float64), i64]
     L.orc_patch_quality.argtypes = [vp, C.c_uint32]
     L.orc_patch_quality.restype = C.c_double
     L.orc_local_size.argtypes = [vp, C.c_uint32]
@@ -81,11 +83,13 @@ def lib():
     return L
 
 
-def gm_libm(op: int, x) -> np.ndarray:
-    """The project's libm evaluated on the host (op 2 pow23, 3 cbrt, 4 exp)."""
+def gm_libm(op: int, x, y=None) -> np.ndarray:
+    """The project's libm evaluated on the host (op 2 pow23, 3 cbrt, 4 exp,
+    6 cos, 7 log, 8 pow(x, y))."""
     x = np.ascontiguousarray(x, np.float64)
+    y = x if y is None else np.ascontiguousarray(y, np.float64)
     out = np.empty_like(x)
-    lib().orc_gm_libm(int(op), x, out, x.size)
+    lib().orc_gm_libm2(int(op), x, y, out, x.size)
     return out
 
 
@@ -176,6 +180,9 @@ class OracleMesh:
             args = [a.ctypes.data for a in keep]
         L.orc_set_crew(self._h, EVALUATORS[evaluator], SAMPLERS[sampler], meas, float(scaling), *args)
         self.evaluator = evaluator
+
+    def set_aspect_ratio(self, aspect: float) -> None:
+        lib().orc_set_aspect(self._h, float(aspect))
 
     # --- queries
     def evaluate(self, chunks: int = 1):

@@ -32,6 +32,11 @@ def test_math_probe_bitexact(oracle):
     assert np.array_equal(math_probe(3, x).view(np.uint64), oracle.gm_libm(3, x).view(np.uint64))
     e = rng.uniform(-700, 700, 4000)
     assert np.array_equal(math_probe(4, e).view(np.uint64), oracle.gm_libm(4, e).view(np.uint64))
+    c = np.concatenate([rng.uniform(-40, 40, 4000), rng.uniform(-1e5, 1e5, 500)])
+    assert np.array_equal(math_probe(6, c).view(np.uint64), oracle.gm_libm(6, c).view(np.uint64))
+    assert np.array_equal(math_probe(7, x).view(np.uint64), oracle.gm_libm(7, x).view(np.uint64))
+    assert np.array_equal(math_probe(8, y, x[:y.size] % 7).view(np.uint64),
+                          oracle.gm_libm(8, y, x[:y.size] % 7).view(np.uint64))
 
 
 def test_math_probe_fast_powm23_bound():
@@ -89,10 +94,13 @@ def _crew_pair(oracle, kind, n, evaluator, sampler, K, seed=3, texture_n=16):
     tex = shell_texture(m.pos, texture_n, K) if sampler == "texture" else None
     om.set_crew(evaluator, sampler, scaling=K, texture=tex)
     gm.set_crew(evaluator, sampler, scaling=K, texture=tex)
+    if sampler == "analytic_reference":
+        om.set_aspect_ratio(4.0)
+        gm.set_aspect_ratio(4.0)
     return m, om, gm
 
 
-@pytest.mark.parametrize("sampler", ["analytic", "texture"])
+@pytest.mark.parametrize("sampler", ["analytic", "texture", "analytic_reference"])
 @pytest.mark.parametrize("evaluator", ["metric_conformity", "mean_ratio"])
 def test_anisotropic_element_qualities_bitexact(oracle, evaluator, sampler):
     """Metric-conformity and metric-wise mean-ratio qualities under the shell
@@ -107,7 +115,7 @@ def test_anisotropic_element_qualities_bitexact(oracle, evaluator, sampler):
 
 @pytest.mark.parametrize("smoother,kernel", [("nm", "wave"), ("nm", "plain"), ("gd", "wave"), ("spawn", "wave"),
                                              ("getme", "plain"), ("getme", "wave")])
-@pytest.mark.parametrize("sampler", ["analytic", "texture"])
+@pytest.mark.parametrize("sampler", ["analytic", "texture", "analytic_reference"])
 def test_anisotropic_smoothing_fp64_bitexact(oracle, smoother, kernel, sampler):
     """The anisotropic hex workload's crew (metric conformity, metric-wise
     measurer) on a small jittered hex cube."""
@@ -126,11 +134,12 @@ def test_anisotropic_smoothing_fp64_bitexact(oracle, smoother, kernel, sampler):
     assert np.array_equal(out["min"], ref["min"])
 
 
-def test_metric_wise_mean_ratio_smoothing_bitexact(oracle):
+@pytest.mark.parametrize("sampler", ["analytic", "analytic_reference"])
+def test_metric_wise_mean_ratio_smoothing_bitexact(oracle, sampler):
     """Mean ratio through the metric-wise measurer (adaptive Riemannian
-    segments) under the analytic field, GD and GETMe on a Kuhn cube."""
+    segments) under an analytic field, GD and GETMe on a Kuhn cube."""
     for smoother in ("gd", "getme"):
-        m, om, gm = _crew_pair(oracle, "tet", 4, "mean_ratio", "analytic", K=4.0)
+        m, om, gm = _crew_pair(oracle, "tet", 4, "mean_ratio", sampler, K=4.0)
         om.smooth(smoother, 2, threads=4)
         gm.smooth(smoother, 2, precision="fp64", kernel="wave")
         assert np.array_equal(gm.positions(), om.positions())

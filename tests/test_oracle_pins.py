@@ -680,3 +680,68 @@ def test_passes_compose_vertex_steps(oracle, smoother, kind):
                    "spawn": lambda a, b: spawn_search(a, b, coeff)}[smoother]
             pos[v] = run(patch, rec)
     np.testing.assert_allclose(om.positions()[perm], pos, rtol=0, atol=1e-13)
+
+
+# ---------------------------------------------------------------------------
+# The reference's analytic field (AbstractSampler::vertMetric) and the libm
+# routines it needs.
+# ---------------------------------------------------------------------------
+
+def test_project_cos_log_pow_accuracy(oracle):
+    rng = np.random.default_rng(31)
+    x = np.concatenate([rng.uniform(-40, 40, 20000), rng.uniform(-1e5, 1e5, 2000),
+                        np.array([0.0, np.pi / 2, np.pi, 1e-300, -7.5])])
+    ref = np.cos(x.astype(np.longdouble))
+    got = oracle.gm_libm(6, x)
+    ulp = np.spacing(np.abs(ref.astype(np.float64)))
+    assert np.max(np.abs(got - ref).astype(np.float64) / np.maximum(ulp, 5e-324)) <= 2.0
+    y = np.concatenate([10.0 ** rng.uniform(-300, 300, 20000), rng.uniform(0.5, 2.0, 20000), [1.0, 2.0, 5e-324]])
+    ref = np.log(y.astype(np.longdouble))
+    got = oracle.gm_libm(7, y)
+    nz = ref != 0
+    assert np.max((np.abs(got[nz] - ref[nz]) / np.spacing(np.abs(ref[nz]).astype(np.float64))).astype(np.float64)) <= 2.0
+    assert oracle.gm_libm(7, np.array([1.0]))[0] == 0.0
+    a, b = rng.uniform(0, 10, 20000), rng.uniform(0, 6, 20000)
+    ref = np.power(a.astype(np.longdouble), b.astype(np.longdouble))
+    got = oracle.gm_libm(8, a, b)
+    assert np.max((np.abs(got - ref) / ref).astype(np.float64)) < 1e-14
+    assert oracle.gm_libm(8, np.array([0.0, 0.0, 3.0]), np.array([2.0, 0.0, 0.0])).tolist() == [0.0, 1.0, 1.0]
+
+
+def _rodrigues(axis, angle):
+    axis = np.asarray(axis, float) / np.linalg.norm(axis)
+    K = np.array([[0, -axis[2], axis[1]], [axis[2], 0, -axis[0]], [-axis[1], axis[0], 0]])
+    return np.eye(3) + np.sin(angle) * K + (1 - np.cos(angle)) * K @ K
+
+
+def test_reference_analytic_field(oracle):
+    """The sampler turns the frame by pi/4 about z, then by pi/4 about -y
+    (AbstractSampler.cpp:30-41), so its rotation is the product of the two
+    textbook rotations; the field stretches the rotated x axis: where
+    cos(2.5 pi x') = 1 the metric is K^2 I, where it is -1 the metric has
+    the eigenvalue K^2 a^2 along R^T e_x and K^2 across it, and everywhere
+    R^T e_x is an eigenvector with eigenvalue (K a^(c^a))^2, c = (1 - cos)/2."""
+    om = oracle.OracleMesh.from_cube(cube_mesh(1, "tet", 0.0, seed=1))
+    K, a = 3.0, 4.0
+    om.set_crew("metric_conformity", "analytic_reference", scaling=K)
+    om.set_aspect_ratio(a)
+    # glm::rotate(m, angle, axis) returns m * Rot(axis, angle): the matrix
+    # maps a position to the sampler's frame.
+    R = (np.eye(3) @ _rodrigues([0, 0, 1], np.pi / 4)) @ _rodrigues([0, -1, 0], np.pi / 4)
+    ex = R.T @ np.array([1.0, 0, 0])
+    rng = np.random.default_rng(5)
+    for _ in range(40):
+        p = rng.uniform(-1, 1, 3)
+        M = om.metric_at(p)
+        xr = (R @ p)[0] * 2.5 * np.pi
+        c = (1 - np.cos(xr)) / 2
+        sx = K * a ** (c ** a)
+        assert np.allclose(M @ ex, sx * sx * ex, rtol=1e-12, atol=1e-12)
+        t = np.cross(ex, [0.3, 0.5, 0.7])
+        t /= np.linalg.norm(t)
+        assert np.allclose(M @ t, K * K * t, rtol=1e-12, atol=1e-12)
+    # Special points: x' = 0 (isotropic) and 2.5 pi x' = pi (full stretch).
+    for xprime, expect in [(0.0, K * K), (0.4, (K * a) ** 2), (0.8, K * K)]:
+        p = R.T @ np.array([xprime, 0.2, -0.1])
+        M = om.metric_at(p)
+        assert np.allclose(M @ ex, expect * ex, rtol=1e-12)
