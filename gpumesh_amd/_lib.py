@@ -30,7 +30,21 @@ EXPORTS = [
     "gm_topology_cache_stats", "gm_math_probe", "gm_mesh_smooth_schedule", "gm_batch_smooth_schedule",
     "gm_job_size", "gm_job_encode", "gm_job_info", "gm_job_decode", "gm_server_submit_job",
     "gm_set_polish_passes", "gm_evaluate_noalloc", "gm_mesh_create_grouped", "gm_mesh_set_aspect_ratio",
+    "gm_mesh_create_desc",
 ]
+
+
+class Constraint(C.Structure):
+    """gm_constraint: a plane (a = point, b = normal) or a segment (a, b ends)."""
+    _fields_ = [("kind", C.c_int), ("a", C.c_double * 3), ("b", C.c_double * 3)]
+
+
+class MeshDesc(C.Structure):
+    """gm_mesh_desc."""
+    _fields_ = [("nverts", C.c_int64), ("pos", C.c_void_p), ("fixed", C.c_void_p), ("ntets", C.c_int64),
+                ("tets", C.c_void_p), ("nhexs", C.c_int64), ("hexs", C.c_void_p), ("constraint", C.c_void_p),
+                ("nconstraints", C.c_int64), ("constraints", C.c_void_p), ("ngroups", C.c_int64),
+                ("group_off", C.c_void_p), ("group_nodes", C.c_void_p), ("device_copy", C.c_int)]
 
 
 class Schedule(C.Structure):
@@ -60,6 +74,7 @@ def lib():
     for name in ("gm_mesh_create", "gm_mesh_create_host"):
         getattr(L, name).argtypes = [i64, vp, vp, i64, vp, i64, vp, C.POINTER(vp)]
     L.gm_mesh_create_grouped.argtypes = [i64, vp, vp, i64, vp, i64, vp, i64, vp, vp, C.c_int, C.POINTER(vp)]
+    L.gm_mesh_create_desc.argtypes = [C.POINTER(MeshDesc), C.POINTER(vp)]
     L.gm_mesh_destroy.argtypes = [vp]
     L.gm_mesh_info.argtypes = [vp, vp]
     L.gm_mesh_permutation.argtypes = [vp, vp]

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -174,6 +175,67 @@ int gm_mesh_create_grouped(int64_t nverts, const double* pos, const int32_t* fix
     {
         gmr::destroyMesh(m);
         return rc;
+    }
+    *out = m;
+    return GM_OK;
+}
+
+int gm_mesh_create_desc(const gm_mesh_desc* d, gm_mesh** out)
+{
+    if(!out) return fail(GM_ERR_ARG, "out is null");
+    *out = nullptr;
+    if(!d) return fail(GM_ERR_ARG, "null description");
+    if(d->nconstraints < 0 || (d->nconstraints > 0 && !d->constraints))
+        return fail(GM_ERR_ARG, "bad constraint table");
+    // The table in the device layout: [kind, a, unit normal or direction, b,
+    // length, 0]; the normalisations are the constraints' constructors'
+    // (FaceConstraint.cpp:173-178, EdgeConstraint.cpp:177-183).
+    std::vector<double> table((size_t)d->nconstraints * gmd::CONS_STRIDE, 0.0);
+    for(int64_t k = 0; k < d->nconstraints; ++k)
+    {
+        const gm_constraint& c = d->constraints[k];
+        double* e = &table[(size_t)k * gmd::CONS_STRIDE];
+        double n[3];
+        if(c.kind == GM_CONSTRAINT_PLANE)
+            for(int i = 0; i < 3; ++i) n[i] = c.b[i];
+        else if(c.kind == GM_CONSTRAINT_SEGMENT)
+            for(int i = 0; i < 3; ++i) n[i] = c.b[i] - c.a[i];
+        else
+            return fail(GM_ERR_ARG, "constraint " + std::to_string(k) + " has an unknown kind");
+        const double len = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+        if(!(len > 0.0) || len == (double)__builtin_huge_val())
+            return fail(GM_ERR_ARG, "constraint " + std::to_string(k) + " has a zero or non-finite normal or length");
+        const double inv = 1.0 / len;
+        e[0] = (double)c.kind;
+        for(int i = 0; i < 3; ++i)
+        {
+            e[1 + i] = c.a[i];
+            e[4 + i] = n[i] * inv;
+            e[7 + i] = c.b[i];
+        }
+        e[10] = len;
+    }
+    gm_mesh* m = nullptr;
+    const std::vector<double>* tp = d->nconstraints > 0 ? &table : nullptr;
+    int rc = d->ngroups < 0
+                 ? gmr::makeMesh(d->nverts, d->pos, d->fixed, d->ntets, d->tets, d->nhexs, d->hexs, &m,
+                                 d->constraint, tp)
+                 : gmr::makeMeshGrouped(d->nverts, d->pos, d->fixed, d->ntets, d->tets, d->nhexs, d->hexs, d->ngroups,
+                                        d->group_off, d->group_nodes, &m, d->constraint, tp);
+    if(rc) return rc;
+    if(d->device_copy)
+    {
+        int ndev = 0;
+        if(hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+        {
+            gmr::destroyMesh(m);
+            return fail(GM_ERR_HIP, "no HIP device visible");
+        }
+        if((rc = gmr::uploadMesh(m)))
+        {
+            gmr::destroyMesh(m);
+            return rc;
+        }
     }
     *out = m;
     return GM_OK;
@@ -676,7 +738,7 @@ int gm_batch_autotune(gm_batch* b, int smoother, int precision, int passes, int3
                     const bool f64 = precision == GM_FP64 || precision == GM_FP64_FAST;
                     gmd::VertexWork w{p->dTable, l.segs, l.nsegs, l.count, l.kind, l.maxPatch,
                                       precision == GM_FP64_FAST ? 1 : 0};
-                    if(!getme && l.kind != gmd::EK_MIXED &&
+                    if(!getme && l.kind != gmd::EK_MIXED && !l.constrained &&
                        !gmd::laneFits(w, smoother, shape, f64, p->meshes[0]->evaluator, p->meshes[0]->sampler))
                         fits = false;
                 }

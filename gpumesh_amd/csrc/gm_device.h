@@ -75,7 +75,17 @@ struct DevMesh
     int32_t autopilot;
     double apMeanGain;
     double apMinGain;
+    // Sliding constraints (nullptr when no vertex slides): per slot an index
+    // into consTable or -1; CONS_STRIDE doubles per constraint.
+    const int32_t* cons;
+    const double* consTable;
 };
+
+// A constraint table entry: [kind, a.xyz, n.xyz, b.xyz, length, 0] -- a plane
+// through a with unit normal n, or a segment from a to b with unit direction
+// n and length `length`.
+constexpr int CONS_STRIDE = 12;
+enum { CONS_PLANE = 1, CONS_SEGMENT = 2 };
 
 // True once the autopilot stopped this mesh's run: the pass kernels leave.
 __device__ __forceinline__ bool runStopped(const DevMesh& m)
@@ -128,6 +138,7 @@ struct VertexWork
     int kind;                  // EK_*
     int maxPatch;              // largest patch in the list
     int polish;                // FP64 polish pass of the mixed-precision mode (not the exact FP64 mode)
+    int constrained;           // the list's vertices slide on constraints (generic kernels only)
 };
 
 // Block work: `count` blocks spread over `nspans` members.

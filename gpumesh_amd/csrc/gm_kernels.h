@@ -242,6 +242,36 @@ __device__ __forceinline__ double fastHexEval(const gmq::Crew& c, const v3<doubl
     return q;
 }
 
+// Constraint snapping of a sliding vertex (PlaneConstraint::operator(),
+// Boundaries/Constraints/FaceConstraint.cpp:180-185; SegmentConstraint::
+// operator(), EdgeConstraint.cpp:185-199), from its constraint table entry;
+// the identity for a vertex without one (c = nullptr).
+template<typename T> struct DevSnap
+{
+    const double* c;
+    __device__ v3<T> operator()(v3<T> p) const
+    {
+        if(!c) return p;
+        const v3<T> a = mk<T>((T)c[1], (T)c[2], (T)c[3]);
+        const v3<T> n = mk<T>((T)c[4], (T)c[5], (T)c[6]);
+        const v3<T> dist = p - a;
+        const T l = gmq::dot(dist, n);
+        if((int)c[0] == CONS_PLANE) return p - n * l;
+        if(l < T(0)) return a;
+        if(l > (T)c[10]) return mk<T>((T)c[7], (T)c[8], (T)c[9]);
+        const v3<T> t = dist - n * l;
+        return p - t;
+    }
+};
+
+// Table entry of slot v's sliding constraint, nullptr if it has none.
+__device__ __forceinline__ const double* consOf(const DevMesh& m, uint32_t v)
+{
+    if(!m.cons) return nullptr;
+    const int32_t k = m.cons[v];
+    return k >= 0 ? m.consTable + (size_t)CONS_STRIDE * k : nullptr;
+}
+
 // Quality of the patch element `ref` with the patch's centre vertex at p.
 template<typename T, int EV, int SA, typename P>
 __device__ __forceinline__ T elementQualityAt(const DevMesh& m, const gmq::Crew& c, const P* pos, uint32_t ref, v3<T> p)
@@ -312,10 +342,11 @@ __global__ void __launch_bounds__(256) smoothPlainKernel(const DevMesh* __restri
     v3<T> p0 = ld<T>(pos, v);
     T ls = localSize<T>(m, pos, v, p0);
     PlainPatch<T, EV, SA> f{m, c, pos, v, 0};
+    const DevSnap<T> snap{consOf(m, v)};
     v3<T> np;
-    if(SM == SM_NELDER_MEAD) np = gms::nelderMead<T>(p0, ls, f);
-    else if(SM == SM_GRADIENT_DESCENT) np = gms::gradientDescent<T>(p0, ls, f);
-    else np = gms::spawnSearch<T>(p0, ls, moveCoeff, f);
+    if(SM == SM_NELDER_MEAD) np = gms::nelderMead<T>(p0, ls, f, snap);
+    else if(SM == SM_GRADIENT_DESCENT) np = gms::gradientDescent<T>(p0, ls, f, snap);
+    else np = gms::spawnSearch<T>(p0, ls, moveCoeff, f, snap);
     st(pos, v, np);
     m.evalCounts[v] += f.count;
 }
@@ -474,10 +505,11 @@ __global__ void __launch_bounds__(64) smoothWaveKernel(const DevMesh* __restrict
     }
     v3<T> p0 = ld<T>(pos, v);
     T ls = localSize<T>(m, pos, v, p0);
+    const DevSnap<T> snap{consOf(m, v)};
     v3<T> np;
-    if(SM == SM_NELDER_MEAD) np = gms::nelderMead<T>(p0, ls, f);
-    else if(SM == SM_GRADIENT_DESCENT) np = gms::gradientDescent<T>(p0, ls, f);
-    else np = gms::spawnSearch<T>(p0, ls, moveCoeff, f);
+    if(SM == SM_NELDER_MEAD) np = gms::nelderMead<T>(p0, ls, f, snap);
+    else if(SM == SM_GRADIENT_DESCENT) np = gms::gradientDescent<T>(p0, ls, f, snap);
+    else np = gms::spawnSearch<T>(p0, ls, moveCoeff, f, snap);
     if(lane == 0)
     {
         st(pos, v, np);
@@ -2112,7 +2144,11 @@ __global__ void __launch_bounds__(ELEM_BLOCK) getmeTetKernel(const DevMesh* __re
 #pragma unroll
     for(int k = 0; k < 4; ++k) vpp[k] = center + volumeVar * (vpp[k] - center);
 #pragma unroll
-    for(int k = 0; k < 4; ++k) if(m.fixed[vi[k]]) vpp[k] = vp[k];
+    for(int k = 0; k < 4; ++k)
+    {
+        if(m.fixed[vi[k]]) vpp[k] = vp[k];
+        else if(const double* cs = consOf(m, vi[k])) vpp[k] = DevSnap<T>{cs}(vpp[k]);
+    }
     T quality = gmq::tetQuality<T, EV, SA>(c, vp);
     T qualityPrime = gmq::tetQuality<T, EV, SA>(c, vpp);
     T weight = gmq::tdiv(qualityPrime, T(1) + quality);
@@ -2167,7 +2203,11 @@ __global__ void __launch_bounds__(ELEM_BLOCK) getmeHexKernel(const DevMesh* __re
 #pragma unroll
     for(int k = 0; k < 8; ++k) vpp[k] = center + volumeVar * (vpp[k] - center);
 #pragma unroll
-    for(int k = 0; k < 8; ++k) if(m.fixed[vi[k]]) vpp[k] = vp[k];
+    for(int k = 0; k < 8; ++k)
+    {
+        if(m.fixed[vi[k]]) vpp[k] = vp[k];
+        else if(const double* cs = consOf(m, vi[k])) vpp[k] = DevSnap<T>{cs}(vpp[k]);
+    }
     T quality = gmq::hexQuality<T, EV, SA>(c, vp);
     T qualityPrime = gmq::hexQuality<T, EV, SA>(c, vpp);
     T weight = gmq::tdiv(qualityPrime, T(1) + quality);
@@ -2215,7 +2255,7 @@ __global__ void __launch_bounds__(512) getmeVertexKernel(const DevMesh* __restri
     if(wacc != T(0))
     {
         v3<T> p0 = ld<T>(pos, v);
-        v3<T> p1 = acc / wacc;
+        v3<T> p1 = DevSnap<T>{consOf(m, v)}(acc / wacc);
         PlainPatch<T, EV, SA> f{m, c, pos, v, 0};
         T q0 = f(p0);
         T q1 = f(p1);
@@ -2331,7 +2371,7 @@ __global__ void __launch_bounds__(64 * GETME_WAVES) getmeWaveKernel(const DevMes
     }
     if(wacc != T(0))
     {
-        const v3<T> p1 = acc / wacc;
+        const v3<T> p1 = DevSnap<T>{consOf(m, v)}(acc / wacc);
         const T q0 = f.reduce(qOld);
         const T qNew = f.elemQuality(p1);
         const T q1 = f.reduce(qNew);
@@ -2629,16 +2669,18 @@ static hipError_t launchSmoothTyped(const VertexWork& w, const gmq::Crew& c, int
 {
     if(w.count == 0)
         return hipSuccess;
+    // Sliding vertices run on the generic kernels, which snap every trial
+    // position; the affine and candidate-parallel kernels take free ones.
     if constexpr(AffineCrew<EV, SA>::on && SM == SM_NELDER_MEAD)
     {
-        if(shape == KS_WAVE && useAffine<T, EV, SA>(w) && w.maxPatch <= 64 && w.kind != EK_MIXED)
+        if(!w.constrained && shape == KS_WAVE && useAffine<T, EV, SA>(w) && w.maxPatch <= 64 && w.kind != EK_MIXED)
         {
             if(w.kind == EK_TET) launchNmAffine<T, EK_TET>(w, stream);
             else launchNmAffine<T, EK_HEX>(w, stream);
             return hipGetLastError();
         }
     }
-    if(shape == KS_WAVE && laneFits<T, EV, SA>(w, SM, lane))
+    if(!w.constrained && shape == KS_WAVE && laneFits<T, EV, SA>(w, SM, lane))
     {
         if(w.kind == EK_TET) return launchLane<T, EV, SA, SM, EK_TET>(w, c, lane, moveCoeff, stream);
         return launchLane<T, EV, SA, SM, EK_HEX>(w, c, lane, moveCoeff, stream);

@@ -14,8 +14,10 @@
 
 #if defined(__HIPCC__) || defined(__HIP__)
 #define GMQ __host__ __device__ __forceinline__
+#define GMQ_MEMBER __host__ __device__ __forceinline__
 #else
 #define GMQ static inline
+#define GMQ_MEMBER inline
 #endif
 
 namespace gmq

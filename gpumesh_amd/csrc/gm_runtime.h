@@ -31,6 +31,7 @@ struct VList
     uint32_t count = 0;
     int kind = gmd::EK_TET;
     int maxPatch = 0;
+    int constrained = 0;           // vertices sliding on constraints
     size_t firstSpan = 0;          // offset of segs in the plan's span buffer
 };
 
@@ -79,6 +80,7 @@ struct DeviceLayout
     std::vector<uint32_t> vertexAt;   // slot -> reference number
     std::vector<uint32_t> eoff, eref, noff, nbr, tets, hexs;
     std::vector<uint8_t> fixed;
+    std::vector<int32_t> cons;        // sliding constraint per slot (empty when none slides)
     std::vector<uint32_t> cornerSlot; // patch entry of each element corner
 };
 
@@ -108,6 +110,9 @@ struct gm_mesh
     int sampler = GM_UNIFORM;
     gmq::Crew crew{};
     double aspect = 1.0;      // aspect ratio of the GM_ANALYTIC field
+    // Sliding constraints: gmd::CONS_STRIDE doubles per entry (host copy;
+    // the device copy is dm.consTable).
+    std::vector<double> consTable;
     float* texels = nullptr;
     gmr::Plan* plan = nullptr; // the mesh as a batch of one
 };
@@ -121,7 +126,11 @@ struct Plan
     hipStream_t stream = nullptr;
     std::vector<gm_mesh*> meshes;
     gmd::DevMesh* dTable = nullptr;
-    std::vector<std::vector<VList>> colours;      // [colour][list]
+    std::vector<std::vector<VList>> colours;      // [colour][list]: vertex-wise sweeps, colour by colour
+    // GETMe's vertex steps in NodeGroups' serial order (boundary runs of every
+    // colour, then subsurface, then interior); empty when no member has
+    // sliding vertices (then they follow `colours`).
+    std::vector<std::vector<VList>> getmeSteps;
     gmd::BlockWork tetBlocks{}, hexBlocks{}, evalBlocks{}, vertBlocks{};
     uint32_t* meshList = nullptr; uint32_t nMeshes = 0;
     uint4* spans = nullptr;                       // every span table of the plan, one buffer
@@ -155,7 +164,8 @@ size_t poolCachedBytes();
 
 // Host topology of a new handle (through the topology cache).
 int makeMesh(int64_t nverts, const double* pos, const int32_t* fixed, int64_t ntets, const uint32_t* tets,
-             int64_t nhexs, const uint32_t* hexs, gm_mesh** out);
+             int64_t nhexs, const uint32_t* hexs, gm_mesh** out, const int32_t* cons = nullptr,
+             const std::vector<double>* consTable = nullptr);
 // The GM_ANALYTIC field's parameters of a crew: the aspect ratio and the
 // sampler's fixed rotation (AbstractSampler's constructor,
 // Samplers/AbstractSampler.cpp:30-41) and its transpose.
@@ -164,7 +174,8 @@ void setAnalyticField(gmq::Crew& c, double aspect);
 // makeMesh with the caller's numbering and colour groups (gmt::buildGrouped).
 int makeMeshGrouped(int64_t nverts, const double* pos, const int32_t* fixed, int64_t ntets, const uint32_t* tets,
                     int64_t nhexs, const uint32_t* hexs, int64_t ngroups, const int64_t* groupOff,
-                    const uint32_t* groupNodes, gm_mesh** out);
+                    const uint32_t* groupNodes, gm_mesh** out, const int32_t* cons = nullptr,
+                    const std::vector<double>* consTable = nullptr);
 // Device memory a mesh needs when resident (connectivity, positions, reduction
 // buffers, `records` pass records, GETMe scratch when `getme`).
 size_t residentBytes(const gm_mesh* m, int records, bool getme);

@@ -48,6 +48,7 @@ struct Entry
     uint64_t key;
     int64_t nverts, ntets, nhexs;
     std::vector<uint8_t> fixed;
+    std::vector<int32_t> cons;      // empty without sliding constraints
     std::vector<uint32_t> tets, hexs;
     std::shared_ptr<const Topology> topo;
     size_t bytes;
@@ -65,9 +66,10 @@ size_t topologyBytes(const Topology& t)
     // counted with it.
     const size_t layout = 2 * t.fixed.size() + 4 * (2 * t.fixed.size() + t.eoff.size() + 2 * t.eref.size() +
                                                    t.noff.size() + t.nbr.size() + t.tets.size() + t.hexs.size());
-    return t.fixed.size() + 4 * (t.tets.size() + t.hexs.size() + t.eoff.size() + t.eref.size() + t.noff.size() +
-                                 t.nbr.size() + t.groupNodes.size() + t.serialGroup.size()) +
-           8 * (t.permutation.size() + t.groupOff.size()) + 4 * t.groupMaxPatch.size() + layout;
+    return 2 * t.fixed.size() + 4 * (t.tets.size() + t.hexs.size() + t.eoff.size() + t.eref.size() + t.noff.size() +
+                                     t.nbr.size() + t.groupNodes.size() + t.serialGroup.size() + t.cons.size()) +
+           8 * (t.permutation.size() + t.groupOff.size() + t.runOff.size()) +
+           4 * (t.groupMaxPatch.size() + t.runMaxPatch.size()) + layout;
 }
 
 void evictTo(size_t cap)
@@ -97,12 +99,22 @@ void cacheStats(int64_t* out4)
     out4[3] = (int64_t)g_bytes;
 }
 
-std::shared_ptr<const Topology> buildShared(int64_t nverts, const double* pos, const int32_t* fixed, int64_t ntets,
-                                            const uint32_t* tets, int64_t nhexs, const uint32_t* hexs,
-                                            std::vector<double>& renumbered, std::string& err)
+std::shared_ptr<const Topology> buildShared(int64_t nverts, const double* pos, const int32_t* fixed,
+                                            const int32_t* cons, int64_t ntets, const uint32_t* tets, int64_t nhexs,
+                                            const uint32_t* hexs, std::vector<double>& renumbered, std::string& err)
 {
     std::vector<uint8_t> fx((size_t)std::max<int64_t>(nverts, 0));
     for(int64_t v = 0; v < nverts; ++v) fx[v] = fixed[v] != 0;
+    // The constraint ids take part in the key (they decide vertex types and
+    // hence the numbering); a mesh without sliding vertices keys on none.
+    std::vector<int32_t> cs;
+    if(cons)
+        for(int64_t v = 0; v < nverts; ++v)
+            if(cons[v] >= 0 && !fixed[v])
+            {
+                cs.assign(cons, cons + nverts);
+                break;
+            }
     size_t cap;
     {
         std::lock_guard<std::mutex> lock(g_mutex);
@@ -113,6 +125,7 @@ std::shared_ptr<const Topology> buildShared(int64_t nverts, const double* pos, c
     {
         key = hashBytes(0x243F6A8885A308D3ull, &nverts, sizeof(nverts));
         key = hashBytes(key, fx.data(), fx.size());
+        key = hashBytes(key, cs.data(), 4 * cs.size());
         key = hashBytes(key, tets, 16 * (size_t)ntets);
         key = hashBytes(key, hexs, 32 * (size_t)nhexs);
         std::shared_ptr<const Topology> hit;
@@ -121,7 +134,7 @@ std::shared_ptr<const Topology> buildShared(int64_t nverts, const double* pos, c
             for(auto it = g_entries.begin(); it != g_entries.end(); ++it)
             {
                 if(it->key != key || it->nverts != nverts || it->ntets != ntets || it->nhexs != nhexs) continue;
-                if(it->fixed != fx) continue;
+                if(it->fixed != fx || it->cons != cs) continue;
                 if(ntets && std::memcmp(it->tets.data(), tets, 16 * (size_t)ntets) != 0) continue;
                 if(nhexs && std::memcmp(it->hexs.data(), hexs, 32 * (size_t)nhexs) != 0) continue;
                 hit = it->topo;
@@ -144,7 +157,7 @@ std::shared_ptr<const Topology> buildShared(int64_t nverts, const double* pos, c
         }
     }
     auto t = std::make_shared<Topology>();
-    if(!build(nverts, pos, fixed, ntets, tets, nhexs, hexs, *t, err)) return nullptr;
+    if(!build(nverts, pos, fixed, cs.empty() ? nullptr : cs.data(), ntets, tets, nhexs, hexs, *t, err)) return nullptr;
     renumbered = std::move(t->pos);
     t->pos.clear();
     t->pos.shrink_to_fit();
@@ -157,10 +170,11 @@ std::shared_ptr<const Topology> buildShared(int64_t nverts, const double* pos, c
         e.ntets = ntets;
         e.nhexs = nhexs;
         e.fixed = std::move(fx);
+        e.cons = std::move(cs);
         e.tets.assign(tets, tets + 4 * (size_t)ntets);
         e.hexs.assign(hexs, hexs + 8 * (size_t)nhexs);
         e.topo = ct;
-        e.bytes = topologyBytes(*ct) + e.fixed.size() + 4 * (e.tets.size() + e.hexs.size());
+        e.bytes = topologyBytes(*ct) + e.fixed.size() + 4 * (e.cons.size() + e.tets.size() + e.hexs.size());
         std::lock_guard<std::mutex> lock(g_mutex);
         ++g_misses;
         if(e.bytes <= g_capacity)

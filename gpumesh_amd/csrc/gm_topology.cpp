@@ -14,8 +14,6 @@ static const int HEX_EDGES[12][2] = {{0,1},{0,3},{1,2},{2,3},{0,4},{1,5},{2,6},{
 
 static const int NO_GROUP = -2;
 static const int UNSET_GROUP = -1;
-static const int FIXED_TYPE = 0;
-static const int INTER_TYPE = 3;
 
 // Patch element lists and neighbour lists in the input numbering
 // (Mesh::compileNeighborhoods and addEdge, Mesh.cpp:365-460).
@@ -78,7 +76,79 @@ static void buildAdjacency(int64_t V, int64_t T, const uint32_t* tets, int64_t H
             addEdge(hexs[8 * e + HEX_EDGES[k][0]], hexs[8 * e + HEX_EDGES[k][1]]);
 }
 
-bool build(int64_t V, const double* pos, const int32_t* fixed,
+// NodeGroups::determineTypes (NodeGroups.cpp:124-241): a vertex on a
+// sliding constraint is a boundary vertex, a free vertex of an element that
+// holds one is a subsurface vertex, any other free vertex is interior.
+static bool computeTypes(int64_t V, const int32_t* fixed, const int32_t* cons, int64_t T, const uint32_t* tets,
+                         int64_t H, const uint32_t* hexs, const Adjacency& a, std::vector<uint8_t>& types,
+                         std::string& err)
+{
+    types.assign(V, TYPE_INTERIOR);
+    bool sliding = false;
+    for(int64_t v = 0; v < V; ++v)
+    {
+        if(fixed[v])
+        {
+            if(cons && cons[v] >= 0) { err = "vertex " + std::to_string(v) + " is both fixed and sliding"; return false; }
+            types[v] = TYPE_FIXED;
+        }
+        else if(cons && cons[v] >= 0)
+        {
+            types[v] = TYPE_BOUNDARY;
+            sliding = true;
+        }
+    }
+    if(!sliding) return true;
+    std::vector<uint8_t> boundElem((size_t)(T + H), 0);
+    for(int64_t e = 0; e < T; ++e)
+        for(int k = 0; k < 4; ++k)
+            if(types[tets[4 * e + k]] == TYPE_BOUNDARY) { boundElem[e] = 1; break; }
+    for(int64_t e = 0; e < H; ++e)
+        for(int k = 0; k < 8; ++k)
+            if(types[hexs[8 * e + k]] == TYPE_BOUNDARY) { boundElem[T + e] = 1; break; }
+    for(int64_t v = 0; v < V; ++v)
+    {
+        if(types[v] != TYPE_INTERIOR) continue;
+        for(uint32_t r = a.eoff[v]; r < a.eoff[v + 1]; ++r)
+        {
+            const uint32_t ref = a.eref[r], id = ref >> 4;
+            if(boundElem[(ref & 8) ? T + id : id]) { types[v] = TYPE_SUBSURFACE; break; }
+        }
+    }
+    return true;
+}
+
+// Orders each group's members by type (stable) and fills groupNodes,
+// groupOff, runOff, runMaxPatch, groupMaxPatch and serialGroup from
+// per-group member lists (ids of `out`'s numbering).
+static void finishGroups(const std::vector<std::vector<uint32_t>>& members, Topology& out)
+{
+    const size_t G = members.size();
+    out.groupOff.assign(G + 1, 0);
+    out.runOff.assign(3 * G + 1, 0);
+    out.runMaxPatch.assign(3 * G, 0);
+    out.groupMaxPatch.assign(G, 0);
+    out.groupNodes.clear();
+    for(size_t g = 0; g < G; ++g)
+    {
+        for(int t = 0; t < 3; ++t)
+        {
+            out.runOff[3 * g + t] = (int64_t)out.groupNodes.size();
+            for(uint32_t v : members[g])
+                if(out.vtype[v] == TYPE_BOUNDARY + t)
+                {
+                    out.groupNodes.push_back(v);
+                    const int32_t patch = (int32_t)(out.eoff[v + 1] - out.eoff[v]);
+                    out.runMaxPatch[3 * g + t] = std::max(out.runMaxPatch[3 * g + t], patch);
+                    out.groupMaxPatch[g] = std::max(out.groupMaxPatch[g], patch);
+                }
+        }
+        out.groupOff[g + 1] = (int64_t)out.groupNodes.size();
+    }
+    out.runOff[3 * G] = (int64_t)out.groupNodes.size();
+}
+
+bool build(int64_t V, const double* pos, const int32_t* fixed, const int32_t* cons,
            int64_t T, const uint32_t* tets, int64_t H, const uint32_t* hexs,
            Topology& out, std::string& err)
 {
@@ -90,10 +160,9 @@ bool build(int64_t V, const double* pos, const int32_t* fixed,
     const std::vector<uint32_t>& ncnt = adj.ncnt;
     const std::vector<uint32_t>& ntmp = adj.ntmp;
 
-    // ---- Types: only fixed and free vertices exist, so every movable node is
-    // interior (no movable boundary, hence no subsurface nodes).
-    std::vector<int> types(V);
-    for(int64_t v = 0; v < V; ++v) types[v] = fixed[v] ? FIXED_TYPE : INTER_TYPE;
+    // ---- Types (determineTypes).
+    std::vector<uint8_t> types;
+    if(!computeTypes(V, fixed, cons, T, tets, H, hexs, adj, types, err)) return false;
 
     // ---- Greedy colouring (determineGroups, NodeGroups.cpp:243-363): the
     // smallest existing colour unused by already-coloured element-mates,
@@ -164,9 +233,9 @@ bool build(int64_t V, const double* pos, const int32_t* fixed,
     std::vector<std::pair<uint64_t, int>> keyed(V);
     for(int64_t v = 0; v < V; ++v)
     {
-        const uint64_t key = types[v] == FIXED_TYPE
+        const uint64_t key = types[v] == TYPE_FIXED
                                  ? 0
-                                 : ((uint64_t)1 << 62) | ((uint64_t)(uint32_t)groups[v] << 32) |
+                                 : ((uint64_t)types[v] << 60) | ((uint64_t)(uint32_t)groups[v] << 32) |
                                        (uint64_t)(eoff[v + 1] - eoff[v]);
         keyed[v] = std::make_pair(key, (int)v);
     }
@@ -207,34 +276,43 @@ bool build(int64_t V, const double* pos, const int32_t* fixed,
     out.hexs.resize(8 * H);
     for(int64_t i = 0; i < 8 * H; ++i) out.hexs[i] = (uint32_t)P[hexs[i]];
 
-    // ---- Groups: movable nodes are sorted by colour, so each colour's nodes
-    // are one contiguous run of new ids; colours that only fixed vertices use
-    // below the largest movable colour stay as empty groups.
+    // ---- Groups (clusterNodes): movable nodes are sorted by type, then
+    // colour, so a group's nodes in ascending new id are its boundary, then
+    // subsurface, then interior nodes (the order of its undispatchedNodes);
+    // without sliding vertices each colour is one contiguous run of new ids.
+    // Colours that only fixed vertices use below the largest movable colour
+    // stay as empty groups.
     out.nfixed = 0;
     for(int64_t v = 0; v < V; ++v) if(fixed[v]) ++out.nfixed;
+    out.vtype.resize(V);
+    for(int64_t p = 0; p < V; ++p) out.vtype[p] = types[indices[p]];
+    if(cons)
+    {
+        bool any = false;
+        for(int64_t v = 0; v < V && !any; ++v) any = types[v] == TYPE_BOUNDARY;
+        if(any)
+        {
+            out.cons.resize(V);
+            for(int64_t p = 0; p < V; ++p) out.cons[p] = types[indices[p]] == TYPE_BOUNDARY ? cons[indices[p]] : -1;
+        }
+    }
     int maxGroup = -1;
     for(int64_t p = out.nfixed; p < V; ++p) maxGroup = std::max(maxGroup, groups[indices[p]]);
-    int G = maxGroup + 1;
-    out.groupOff.assign(G + 1, 0);
-    for(int64_t p = out.nfixed; p < V; ++p) ++out.groupOff[groups[indices[p]] + 1];
-    for(int g = 0; g < G; ++g) out.groupOff[g + 1] += out.groupOff[g];
-    out.groupNodes.resize(V - out.nfixed);
+    const int G = maxGroup + 1;
+    std::vector<std::vector<uint32_t>> members(G);
     out.serialGroup.resize(V - out.nfixed);
-    out.groupMaxPatch.assign(G, 0);
     for(int64_t p = out.nfixed; p < V; ++p)
     {
-        int g = groups[indices[p]];
-        // Sorted order: position within the group is p - (first new id of g).
+        members[groups[indices[p]]].push_back((uint32_t)p);
         out.serialGroup[p - out.nfixed] = (uint32_t)p;
-        out.groupNodes[p - out.nfixed] = (uint32_t)p;
-        out.groupMaxPatch[g] = std::max<int32_t>(out.groupMaxPatch[g], (int32_t)(out.eoff[p + 1] - out.eoff[p]));
     }
+    finishGroups(members, out);
     return true;
 }
 
-bool buildGrouped(int64_t V, const double* pos, const int32_t* fixed, int64_t T, const uint32_t* tets, int64_t H,
-                  const uint32_t* hexs, int64_t G, const int64_t* groupOff, const uint32_t* groupNodes, Topology& out,
-                  std::string& err)
+bool buildGrouped(int64_t V, const double* pos, const int32_t* fixed, const int32_t* cons, int64_t T,
+                  const uint32_t* tets, int64_t H, const uint32_t* hexs, int64_t G, const int64_t* groupOff,
+                  const uint32_t* groupNodes, Topology& out, std::string& err)
 {
     if(!checkInputs(V, T, tets, H, hexs, err)) return false;
     if(G < 0) { err = "negative group count"; return false; }
@@ -253,6 +331,8 @@ bool buildGrouped(int64_t V, const double* pos, const int32_t* fixed, int64_t T,
     }
     Adjacency adj;
     buildAdjacency(V, T, tets, H, hexs, adj);
+    std::vector<uint8_t> types;
+    if(!computeTypes(V, fixed, cons, T, tets, H, hexs, adj, types, err)) return false;
     // Membership: each movable vertex in exactly one group, no fixed vertex.
     std::vector<int32_t> groupOf(V, -1);
     for(int64_t g = 0; g < G; ++g)
@@ -298,17 +378,24 @@ bool buildGrouped(int64_t V, const double* pos, const int32_t* fixed, int64_t T,
     out.permutation.resize(V);
     std::iota(out.permutation.begin(), out.permutation.end(), (int64_t)0);
     out.nfixed = nfixed;
-    out.groupOff.assign(groupOff, groupOff + G + 1);
-    if(G == 0) out.groupOff.assign(1, 0);
-    out.groupNodes.assign(groupNodes, groupNodes + M);
-    out.serialGroup = out.groupNodes;
-    out.groupMaxPatch.assign(G, 0);
-    for(int64_t g = 0; g < G; ++g)
-        for(int64_t i = groupOff[g]; i < groupOff[g + 1]; ++i)
-        {
-            const uint32_t v = groupNodes[i];
-            out.groupMaxPatch[g] = std::max<int32_t>(out.groupMaxPatch[g], (int32_t)(adj.eoff[v + 1] - adj.eoff[v]));
-        }
+    out.vtype = types;
+    if(cons)
+        for(int64_t v = 0; v < V; ++v)
+            if(types[v] == TYPE_BOUNDARY)
+            {
+                out.cons.resize(V);
+                for(int64_t u = 0; u < V; ++u) out.cons[u] = types[u] == TYPE_BOUNDARY ? cons[u] : -1;
+                break;
+            }
+    std::vector<std::vector<uint32_t>> members((size_t)G);
+    for(int64_t g = 0; g < G; ++g) members[g].assign(groupNodes + groupOff[g], groupNodes + groupOff[g + 1]);
+    finishGroups(members, out);
+    // Serial order (NodeGroups::serialGroup): type-major, then colour.
+    out.serialGroup.clear();
+    for(int t = 0; t < 3; ++t)
+        for(int64_t g = 0; g < G; ++g)
+            out.serialGroup.insert(out.serialGroup.end(), out.groupNodes.begin() + out.runOff[3 * g + t],
+                                   out.groupNodes.begin() + out.runOff[3 * g + t + 1]);
     return true;
 }
 

@@ -18,6 +18,8 @@
 namespace gmt
 {
 
+enum { TYPE_FIXED = 0, TYPE_BOUNDARY = 1, TYPE_SUBSURFACE = 2, TYPE_INTERIOR = 3 };
+
 struct Topology
 {
     // Inputs after renumbering (new vertex ids).
@@ -40,6 +42,21 @@ struct Topology
     std::vector<int32_t> groupMaxPatch;    // largest patch per group
     std::vector<uint32_t> serialGroup;     // movable vertices in serial order
 
+    // NodeGroups' vertex types (NodeGroups.cpp:124-241), per vertex:
+    // TYPE_FIXED, TYPE_BOUNDARY (slides on a face or edge constraint),
+    // TYPE_SUBSURFACE (free, in an element with a boundary vertex) or
+    // TYPE_INTERIOR.
+    std::vector<uint8_t> vtype;
+    // Sliding constraint of each vertex: an index into the mesh's constraint
+    // table, -1 for none; empty when no vertex slides.
+    std::vector<int32_t> cons;
+    // Each group's members ordered by type: group g's boundary, subsurface
+    // and interior vertices are groupNodes[runOff[3g + t] .. runOff[3g + t + 1])
+    // for t = 0, 1, 2 (runOff[3g] = groupOff[g]); runMaxPatch[3g + t] is the
+    // largest patch of run (g, t).
+    std::vector<int64_t> runOff;
+    std::vector<int32_t> runMaxPatch;
+
     // The runtime's device-side layout of this topology (its own type),
     // built on the first upload and shared by later meshes of a cached
     // topology.
@@ -48,9 +65,10 @@ struct Topology
     int64_t ngroups() const { return (int64_t)groupOff.size() - 1; }
 };
 
-// Builds the topology.  `fixed[v]` != 0 marks a fixed vertex; every other
-// vertex is free.  Returns false and sets `err` on invalid input.
-bool build(int64_t nverts, const double* pos, const int32_t* fixed,
+// Builds the topology.  `fixed[v]` != 0 marks a fixed vertex; `cons` (may
+// be null) gives each vertex's sliding constraint (an index >= 0, or -1);
+// every other vertex is free.  Returns false and sets `err` on invalid input.
+bool build(int64_t nverts, const double* pos, const int32_t* fixed, const int32_t* cons,
            int64_t ntets, const uint32_t* tets, int64_t nhexs, const uint32_t* hexs,
            Topology& out, std::string& err);
 
@@ -59,17 +77,19 @@ bool build(int64_t nverts, const double* pos, const int32_t* fixed,
 // groupNodes[groupOff[g] .. groupOff[g + 1]).  Checks that every movable
 // vertex is in exactly one group, no fixed vertex is in one, and no two
 // vertices of a group share an element.  permutation is the identity.
-bool buildGrouped(int64_t nverts, const double* pos, const int32_t* fixed, int64_t ntets, const uint32_t* tets,
-                  int64_t nhexs, const uint32_t* hexs, int64_t ngroups, const int64_t* groupOff,
-                  const uint32_t* groupNodes, Topology& out, std::string& err);
+// Each group's members are reordered by type (boundary, subsurface,
+// interior), keeping their order within a type.
+bool buildGrouped(int64_t nverts, const double* pos, const int32_t* fixed, const int32_t* cons, int64_t ntets,
+                  const uint32_t* tets, int64_t nhexs, const uint32_t* hexs, int64_t ngroups,
+                  const int64_t* groupOff, const uint32_t* groupNodes, Topology& out, std::string& err);
 
 // build() through the shared topology cache (gm_topocache.cpp): returns the
 // topology with `pos` left empty and the renumbered positions in
 // `renumbered`; nullptr (and `err`) on invalid input.  With a zero capacity
 // (the default) every call builds.
-std::shared_ptr<const Topology> buildShared(int64_t nverts, const double* pos, const int32_t* fixed, int64_t ntets,
-                                            const uint32_t* tets, int64_t nhexs, const uint32_t* hexs,
-                                            std::vector<double>& renumbered, std::string& err);
+std::shared_ptr<const Topology> buildShared(int64_t nverts, const double* pos, const int32_t* fixed,
+                                            const int32_t* cons, int64_t ntets, const uint32_t* tets, int64_t nhexs,
+                                            const uint32_t* hexs, std::vector<double>& renumbered, std::string& err);
 void setCacheCapacity(size_t bytes);
 // {hits, misses, entries, bytes}
 void cacheStats(int64_t* out4);

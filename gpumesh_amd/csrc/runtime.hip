@@ -168,16 +168,31 @@ template<typename X> static size_t allocBytes(size_t count)
 // ---------------------------------------------------------------------------
 // Mesh handles.
 // ---------------------------------------------------------------------------
+// Constraint ids must index the table (or be -1).
+static int checkConstraintIds(int64_t nverts, const int32_t* cons, const std::vector<double>* table)
+{
+    if(!cons) return GM_OK;
+    const int64_t n = table ? (int64_t)(table->size() / gmd::CONS_STRIDE) : 0;
+    for(int64_t v = 0; v < nverts; ++v)
+        if(cons[v] < -1 || cons[v] >= n)
+            return fail(GM_ERR_ARG, "vertex " + std::to_string(v) + " names constraint " + std::to_string(cons[v]) +
+                                        " of a table of " + std::to_string(n));
+    return GM_OK;
+}
+
 int makeMesh(int64_t nverts, const double* pos, const int32_t* fixed, int64_t ntets, const uint32_t* tets,
-             int64_t nhexs, const uint32_t* hexs, gm_mesh** out)
+             int64_t nhexs, const uint32_t* hexs, gm_mesh** out, const int32_t* cons,
+             const std::vector<double>* consTable)
 {
     if(!out) return fail(GM_ERR_ARG, "out is null");
     *out = nullptr;
     if(nverts < 0 || ntets < 0 || nhexs < 0) return fail(GM_ERR_ARG, "negative size");
     if((nverts && (!pos || !fixed)) || (ntets && !tets) || (nhexs && !hexs)) return fail(GM_ERR_ARG, "null array");
+    if(int rc = checkConstraintIds(nverts, cons, consTable)) return rc;
     gm_mesh* m = new gm_mesh();
+    if(consTable) m->consTable = *consTable;
     std::string err;
-    m->topo = gmt::buildShared(nverts, pos, fixed, ntets, tets, nhexs, hexs, m->hostPos, err);
+    m->topo = gmt::buildShared(nverts, pos, fixed, cons, ntets, tets, nhexs, hexs, m->hostPos, err);
     if(!m->topo)
     {
         delete m;
@@ -235,17 +250,20 @@ void setAnalyticField(gmq::Crew& c, double aspect)
 
 int makeMeshGrouped(int64_t nverts, const double* pos, const int32_t* fixed, int64_t ntets, const uint32_t* tets,
                     int64_t nhexs, const uint32_t* hexs, int64_t ngroups, const int64_t* groupOff,
-                    const uint32_t* groupNodes, gm_mesh** out)
+                    const uint32_t* groupNodes, gm_mesh** out, const int32_t* cons,
+                    const std::vector<double>* consTable)
 {
     if(!out) return fail(GM_ERR_ARG, "out is null");
     *out = nullptr;
     if(nverts < 0 || ntets < 0 || nhexs < 0) return fail(GM_ERR_ARG, "negative size");
     if((nverts && (!pos || !fixed)) || (ntets && !tets) || (nhexs && !hexs)) return fail(GM_ERR_ARG, "null array");
+    if(int rc = checkConstraintIds(nverts, cons, consTable)) return rc;
     auto t = std::make_shared<gmt::Topology>();
     std::string err;
-    if(!gmt::buildGrouped(nverts, pos, fixed, ntets, tets, nhexs, hexs, ngroups, groupOff, groupNodes, *t, err))
+    if(!gmt::buildGrouped(nverts, pos, fixed, cons, ntets, tets, nhexs, hexs, ngroups, groupOff, groupNodes, *t, err))
         return fail(GM_ERR_ARG, err);
     gm_mesh* m = new gm_mesh();
+    if(consTable) m->consTable = *consTable;
     m->hostPos.swap(t->pos);
     m->topo = t;
     for(int32_t p : m->topo->groupMaxPatch) m->maxPatch = std::max(m->maxPatch, (int)p);
@@ -268,6 +286,7 @@ size_t residentBytes(const gm_mesh* m, int records, bool getme)
                allocBytes<double>(3 * (size_t)eb) + allocBytes<int32_t>(eb) + allocBytes<int32_t>(20) +
                allocBytes<int32_t>(2);
     if(records > 0) b += allocBytes<double>(3 * (size_t)records) + allocBytes<int32_t>(20 * (size_t)records);
+    if(!t.cons.empty()) b += allocBytes<int32_t>(V) + allocBytes<double>(m->consTable.size());
     if(getme)
         b += sizeClass(std::max<size_t>(32, t.eref.size() * sizeof(double4))) +
              sizeClass(std::max<size_t>(32, std::max<size_t>(t.eref.size(), 1) * sizeof(uint32_t))) +
@@ -281,6 +300,7 @@ static void freeMeshBuffers(gm_mesh* m)
     void* ptrs[] = {d.pos64, d.pos32, (void*)d.tets, (void*)d.hexs, (void*)d.eoff, (void*)d.eref,
                     (void*)d.noff, (void*)d.nbr, (void*)d.fixed, d.evalCounts, d.getmePull,
                     (void*)d.cornerSlot, d.getmeQual, d.evalPartials, d.evalFlags, d.evalHist, d.evalResult, d.evalHistOut, d.passSlot,
+                    (void*)d.cons, (void*)d.consTable,
                     m->elemQualBuf, m->texels, m->dSlotOf, m->dCountsRef};
     for(void* p : ptrs)
         poolFree(p);
@@ -325,13 +345,17 @@ static std::shared_ptr<const DeviceLayout> buildLayout(const gmt::Topology& t)
     const size_t G = t.groupOff.empty() ? 0 : t.groupOff.size() - 1;
     if((size_t)t.nfixed + (G ? (size_t)t.groupOff[G] : 0) != V)
         return nullptr;
-    // Segment of each vertex: 0 for the fixed ones, g + 1 for group g.
+    // Segment of each vertex: 0 for the fixed ones, 1 + 3 g + r for run r
+    // (boundary, subsurface, interior) of group g, so a group's slots hold
+    // its boundary vertices first and every run is contiguous.
     std::vector<uint32_t> segOf(V, 0);
     for(size_t g = 0; g < G; ++g)
-        for(int64_t i = t.groupOff[g]; i < t.groupOff[g + 1]; ++i) segOf[t.groupNodes[i]] = (uint32_t)(g + 1);
-    std::vector<size_t> next(G + 1);
+        for(int r = 0; r < 3; ++r)
+            for(int64_t i = t.runOff[3 * g + r]; i < t.runOff[3 * g + r + 1]; ++i)
+                segOf[t.groupNodes[i]] = (uint32_t)(1 + 3 * g + r);
+    std::vector<size_t> next(3 * G + 1);
     next[0] = 0;
-    for(size_t g = 0; g < G; ++g) next[g + 1] = (size_t)t.nfixed + (size_t)t.groupOff[g];
+    for(size_t k = 0; k < 3 * G; ++k) next[k + 1] = (size_t)t.nfixed + (size_t)t.runOff[k];
     for(size_t o = 0; o < V; ++o)
     {
         const size_t v = (size_t)t.permutation[o];
@@ -350,6 +374,11 @@ static std::shared_ptr<const DeviceLayout> buildLayout(const gmt::Topology& t)
     {
         const size_t v = vertexAt[s];
         L->fixed[s] = t.fixed[v];
+        if(!t.cons.empty())
+        {
+            if(L->cons.empty()) L->cons.assign(V, -1);
+            L->cons[s] = t.cons[v];
+        }
         const uint32_t e0 = t.eoff[v], e1 = t.eoff[v + 1];
         L->eoff[s + 1] = L->eoff[s] + (e1 - e0);
         for(uint32_t r = e0; r < e1; ++r)
@@ -459,6 +488,13 @@ int uploadMesh(gm_mesh* m)
     GM_TRY(devAlloc(&no, V + 1));                  d.noff = no;
     GM_TRY(devAlloc(&nb, t.nbr.size()));           d.nbr = nb;
     GM_TRY(devAlloc(&fx, V));                      d.fixed = fx;
+    int32_t* cs = nullptr;
+    double* ct = nullptr;
+    if(!L->cons.empty())
+    {
+        GM_TRY(devAlloc(&cs, V));                  d.cons = cs;
+        GM_TRY(devAlloc(&ct, m->consTable.size())); d.consTable = ct;
+    }
     GM_TRY(devAlloc(&d.evalCounts, V));
     GM_TRY(devAlloc(&m->dSlotOf, V));
     GM_TRY(devAlloc(&m->dCountsRef, V));
@@ -476,6 +512,11 @@ int uploadMesh(gm_mesh* m)
         GM_TRY(hipMemcpyAsync(no, noD.data(), (V + 1) * 4, hipMemcpyHostToDevice, s));
         GM_TRY(hipMemcpyAsync(fx, fxD.data(), V, hipMemcpyHostToDevice, s));
         GM_TRY(hipMemcpyAsync(m->dSlotOf, slot.data(), V * 4, hipMemcpyHostToDevice, s));
+    }
+    if(cs)
+    {
+        GM_TRY(hipMemcpyAsync(cs, L->cons.data(), V * 4, hipMemcpyHostToDevice, s));
+        GM_TRY(hipMemcpyAsync(ct, m->consTable.data(), m->consTable.size() * sizeof(double), hipMemcpyHostToDevice, s));
     }
     if(t.ntets) GM_TRY(hipMemcpyAsync(tb, tD.data(), (size_t)t.ntets * 16, hipMemcpyHostToDevice, s));
     if(t.nhexs) GM_TRY(hipMemcpyAsync(hb, hD.data(), (size_t)t.nhexs * 32, hipMemcpyHostToDevice, s));
@@ -526,48 +567,108 @@ static gmd::BlockWork blockSpans(const std::vector<gm_mesh*>& meshes, std::vecto
     return w;
 }
 
-static int buildPlanInto(Plan* p, const std::vector<gm_mesh*>& meshes)
+// A run of consecutive device slots of one member: a colour group's
+// boundary vertices, or its other movable vertices, or (GETMe with sliding
+// vertices) one type of one group.
+struct SlotRun
 {
-    int64_t maxColours = 0;
-    for(gm_mesh* m : meshes) maxColours = std::max(maxColours, m->topo->ngroups());
-    p->colours.resize((size_t)maxColours);
-    std::vector<uint4> spans;
-    for(int64_t g = 0; g < maxColours; ++g)
-    {
-        // One list per (element kind, patch class).  The kind keeps the
-        // candidate-parallel kernels on patches of a single kind; the patch
-        // class (largest patch of the member's colour group, rounded up to
-        // the kernels' tile boundaries) makes each list pick, for every
-        // member, the kernel that member picks when it runs alone.  Within a
-        // list, vertices keep the member order and each member's group order.
+    uint32_t first;
+    uint64_t count;
+    int maxPatch;
+    int constrained;
+};
+
+// Appends the work lists of one step (an independent set of every member)
+// to `out`: one list per (sliding or not, element kind, patch class).  The
+// kind keeps the candidate-parallel kernels on patches of a single kind; the
+// patch class (largest patch of the member's run, rounded up to the kernels'
+// tile boundaries) makes each list pick, for every member, the kernel that
+// member picks when it runs alone.  Within a list, vertices keep the member
+// order and each run's slot order.
+template<class F>
+static int stepLists(const std::vector<gm_mesh*>& meshes, std::vector<uint4>& spans, std::vector<VList>& out,
+                     F runsOf)
+{
+    std::vector<std::vector<SlotRun>> runs(meshes.size());
+    for(size_t i = 0; i < meshes.size(); ++i) runs[i] = runsOf(meshes[i]);
+    for(int constrained = 1; constrained >= 0; --constrained)
         for(int kind = gmd::EK_TET; kind <= gmd::EK_MIXED; ++kind)
             for(int cls = 0; cls < PATCH_CLASSES; ++cls)
             {
                 VList l;
                 l.kind = kind;
+                l.constrained = constrained;
                 l.firstSpan = spans.size();
                 for(size_t i = 0; i < meshes.size(); ++i)
                 {
-                    const gm_mesh* m = meshes[i];
-                    const gmt::Topology& t = *m->topo;
-                    if(m->kind != kind || g >= t.ngroups() || patchClass(t.groupMaxPatch[g]) != cls) continue;
-                    const uint64_t n = (uint64_t)(t.groupOff[g + 1] - t.groupOff[g]);
-                    if(n == 0) continue;
-                    // Each member starts at a multiple of ITEM_ALIGN (the
-                    // items of a wavefront never span two members).
-                    const uint64_t z = ((uint64_t)l.count + gmd::ITEM_ALIGN - 1) / gmd::ITEM_ALIGN * gmd::ITEM_ALIGN;
-                    if(z + n >= (1ull << 32)) return fail(GM_ERR_ARG, "colour step too large");
-                    // The group's first slot (fixed vertices first, then the
-                    // colour groups in order, uploadMesh).
-                    spans.push_back(make_uint4((uint32_t)i, (uint32_t)(t.nfixed + t.groupOff[g]), (uint32_t)z,
-                                               (uint32_t)n));
-                    l.count = (uint32_t)(z + n);
-                    l.maxPatch = std::max(l.maxPatch, (int)t.groupMaxPatch[g]);
+                    if(meshes[i]->kind != kind) continue;
+                    for(const SlotRun& r : runs[i])
+                    {
+                        if(r.count == 0 || r.constrained != constrained || patchClass(r.maxPatch) != cls) continue;
+                        // Each run starts at a multiple of ITEM_ALIGN (the
+                        // items of a wavefront never span two runs).
+                        const uint64_t z = ((uint64_t)l.count + gmd::ITEM_ALIGN - 1) / gmd::ITEM_ALIGN * gmd::ITEM_ALIGN;
+                        if(z + r.count >= (1ull << 32)) return fail(GM_ERR_ARG, "colour step too large");
+                        spans.push_back(make_uint4((uint32_t)i, r.first, (uint32_t)z, (uint32_t)r.count));
+                        l.count = (uint32_t)(z + r.count);
+                        l.maxPatch = std::max(l.maxPatch, r.maxPatch);
+                    }
                 }
                 l.nsegs = (uint32_t)(spans.size() - l.firstSpan);
-                if(l.count) p->colours[g].push_back(l);
+                if(l.count) out.push_back(l);
             }
+    return GM_OK;
+}
+
+static int buildPlanInto(Plan* p, const std::vector<gm_mesh*>& meshes)
+{
+    int64_t maxColours = 0;
+    bool sliding = false;
+    for(gm_mesh* m : meshes)
+    {
+        maxColours = std::max(maxColours, m->topo->ngroups());
+        sliding = sliding || !m->topo->cons.empty();
     }
+    p->colours.resize((size_t)maxColours);
+    std::vector<uint4> spans;
+    // Vertex-wise sweeps: colour by colour, each group's boundary vertices in
+    // lists of their own (the generic kernels, which snap every trial
+    // position onto the vertex's constraint), the rest in the others.
+    for(int64_t g = 0; g < maxColours; ++g)
+    {
+        int rc = stepLists(meshes, spans, p->colours[g], [g](const gm_mesh* m) {
+            const gmt::Topology& t = *m->topo;
+            std::vector<SlotRun> r;
+            if(g >= t.ngroups()) return r;
+            const int64_t* o = &t.runOff[3 * g];
+            r.push_back(SlotRun{(uint32_t)(t.nfixed + o[0]), (uint64_t)(o[1] - o[0]), t.runMaxPatch[3 * g], 1});
+            r.push_back(SlotRun{(uint32_t)(t.nfixed + o[1]), (uint64_t)(o[3] - o[1]),
+                                std::max(t.runMaxPatch[3 * g + 1], t.runMaxPatch[3 * g + 2]), 0});
+            return r;
+        });
+        if(rc) return rc;
+    }
+    // GETMe's vertex steps follow NodeGroups' serial order (its reference
+    // implementation is serial): every group's boundary run, then every
+    // subsurface run, then every interior run.  Without sliding vertices
+    // every movable vertex is interior and that order is the colour order.
+    if(sliding)
+        for(int type = 0; type < 3; ++type)
+            for(int64_t g = 0; g < maxColours; ++g)
+            {
+                std::vector<VList> step;
+                int rc = stepLists(meshes, spans, step, [g, type](const gm_mesh* m) {
+                    const gmt::Topology& t = *m->topo;
+                    std::vector<SlotRun> r;
+                    if(g >= t.ngroups()) return r;
+                    const int64_t k = 3 * g + type;
+                    r.push_back(SlotRun{(uint32_t)(t.nfixed + t.runOff[k]), (uint64_t)(t.runOff[k + 1] - t.runOff[k]),
+                                        t.runMaxPatch[k], type == 0 ? 1 : 0});
+                    return r;
+                });
+                if(rc) return rc;
+                if(!step.empty()) p->getmeSteps.push_back(std::move(step));
+            }
     size_t fTet, fHex, fEval, fVert;
     p->tetBlocks = blockSpans(meshes, spans, fTet, [](const gm_mesh* m) { return ceilDiv(m->topo->ntets, gmd::ELEM_BLOCK); });
     p->hexBlocks = blockSpans(meshes, spans, fHex, [](const gm_mesh* m) { return ceilDiv(m->topo->nhexs, gmd::ELEM_BLOCK); });
@@ -584,6 +685,8 @@ static int buildPlanInto(Plan* p, const std::vector<gm_mesh*>& meshes)
     GMR_HIP(hipMemcpyAsync(p->meshList, ml.data(), ml.size() * sizeof(uint32_t), hipMemcpyHostToDevice, p->stream));
     for(auto& col : p->colours)
         for(VList& l : col) l.segs = p->spans + l.firstSpan;
+    for(auto& step : p->getmeSteps)
+        for(VList& l : step) l.segs = p->spans + l.firstSpan;
     p->tetBlocks.spans = p->spans + fTet;
     p->hexBlocks.spans = p->spans + fHex;
     p->evalBlocks.spans = p->spans + fEval;
@@ -769,12 +872,19 @@ static hipError_t sweep(Plan* p, const gmq::Crew& c, int ev, int sa, const RunPa
                 : gmd::launchGetmeElements<float>(p->dTable, p->tetBlocks, p->hexBlocks, c, ev, sa, p->stream);
         timedEnd(p, slot);
     }
+    // GETMe always sweeps every group, in serial order when members slide.
+    const std::vector<std::vector<VList>>& steps = getme && !p->getmeSteps.empty() ? p->getmeSteps : p->colours;
+    if(getme)
+    {
+        g0 = 0;
+        g1 = (int64_t)steps.size();
+    }
     for(int64_t g = g0; e == hipSuccess && g < g1; ++g)
     {
-        for(const VList& l : p->colours[g])
+        for(const VList& l : steps[g])
         {
             gmd::VertexWork w{p->dTable, l.segs, l.nsegs, l.count, l.kind, l.maxPatch,
-                              f64 && gmr::passFast(r.precision) ? 1 : 0};
+                              f64 && gmr::passFast(r.precision) ? 1 : 0, l.constrained};
             gmd::LaneShape lane = r.lane;
             if(lane.tileElems == 0)
             {

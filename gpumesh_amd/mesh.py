@@ -48,6 +48,49 @@ class GpuMesh:
          self.sum_nbr, self.max_patch) = (int(x) for x in info)
 
     @classmethod
+    def constrained(cls, pos, fixed, tets, hexs, constraint, constraints, groups=None, device: bool = True):
+        """A mesh whose vertices may slide on planes or segments
+        (gm_mesh_create_desc): constraint[v] indexes `constraints` (rows
+        (kind, a, b), kind 1 plane through a with normal b, 2 segment a-b) or
+        is -1; groups = (group_off, group_nodes) keeps the caller's numbering
+        and colouring, None colours and renumbers."""
+        L = lib()
+        self = cls.__new__(cls)
+        pos = np.ascontiguousarray(pos, np.float64).reshape(-1, 3)
+        fixed = np.ascontiguousarray(fixed, np.int32).reshape(-1)
+        tets = np.ascontiguousarray(tets, np.uint32).reshape(-1, 4)
+        hexs = np.ascontiguousarray(hexs, np.uint32).reshape(-1, 8)
+        cons = np.ascontiguousarray(constraint, np.int32).reshape(-1)
+        table = (_lib.Constraint * max(len(constraints), 1))()
+        for k, (kind, a, b) in enumerate(constraints):
+            table[k].kind = int(kind)
+            for i in range(3):
+                table[k].a[i] = float(a[i])
+                table[k].b[i] = float(b[i])
+        keep = [pos, fixed, tets, hexs, cons]
+        d = _lib.MeshDesc()
+        d.nverts, d.pos, d.fixed = pos.shape[0], ptr(pos), ptr(fixed)
+        d.ntets, d.tets, d.nhexs, d.hexs = tets.shape[0], ptr(tets), hexs.shape[0], ptr(hexs)
+        d.constraint, d.nconstraints, d.constraints = ptr(cons), len(constraints), C.addressof(table)
+        d.ngroups = -1
+        if groups is not None:
+            off = np.ascontiguousarray(groups[0], np.int64)
+            nodes = np.ascontiguousarray(groups[1], np.uint32)
+            keep += [off, nodes]
+            d.ngroups, d.group_off, d.group_nodes = off.size - 1, ptr(off), ptr(nodes)
+        d.device_copy = int(device)
+        h = C.c_void_p()
+        check(L.gm_mesh_create_desc(C.byref(d), C.byref(h)), "gm_mesh_create_desc")
+        del keep
+        self._h = h
+        self.device = device
+        info = np.zeros(8, np.int64)
+        check(L.gm_mesh_info(self._h, ptr(info)), "gm_mesh_info")
+        (self.nverts, self.ntets, self.nhexs, self.nmovable, self.ngroups, self.sum_patch,
+         self.sum_nbr, self.max_patch) = (int(x) for x in info)
+        return self
+
+    @classmethod
     def grouped(cls, pos, fixed, tets, hexs, group_off, group_nodes, device: bool = True):
         """A mesh in the caller's numbering with the caller's colour groups
         (gm_mesh_create_grouped): no renumbering, groups checked on the host."""

@@ -223,3 +223,56 @@ def cube_mesh_numpy(n: int, kind: str = "tet", jitter: float = 0.3, seed: int = 
     hexs = elems.astype(np.uint32) if kind == "hex" else np.zeros((0, 8), np.uint32)
     return CubeMesh(pos=np.ascontiguousarray(pos), fixed=fixed, tets=np.ascontiguousarray(tets),
                     hexs=np.ascontiguousarray(hexs), n=n, kind=kind, seed=seed, jitter=jitter)
+
+
+def box_boundary(m: CubeMesh, slide_jitter: float = 0.2, seed: int = 7):
+    """The cube mesh on the reference's BoxBoundary [-1, 1]^3
+    (Boundaries/BoxBoundary.cpp:7-36): corner vertices fixed, edge vertices
+    sliding on the box's 12 edges (segments), face vertices on its 6 faces
+    (planes), interior vertices free.  Edge and face vertices get a seeded
+    jitter of +-slide_jitter h along their constraint.  Returns (pos, fixed,
+    constraint, constraints) with constraints as rows (kind, a, b): kind 1 a
+    plane through a with normal b, kind 2 the segment from a to b."""
+    n = m.n
+    pos = m.pos * 2.0 - 1.0
+    h = 2.0 / n
+    V = pos.shape[0]
+    on = np.isclose(np.abs(pos), 1.0)                 # (V, 3): coordinate on a box face
+    count = on.sum(axis=1)
+    constraints = []
+    index = {}
+    # Faces: xNeg, xPos, yNeg, yPos, zNeg, zPos (BoxBoundary.cpp:29-34).
+    for axis in range(3):
+        for sgn in (-1.0, 1.0):
+            p = [0.0, 0.0, 0.0]
+            p[axis] = sgn
+            index[("face", axis, sgn)] = len(constraints)
+            constraints.append((1, tuple(p), tuple(p)))
+    # Edges: along axis `axis`, at the other two coordinates' signs.
+    for axis in range(3):
+        o1, o2 = [a for a in range(3) if a != axis]
+        for s1 in (-1.0, 1.0):
+            for s2 in (-1.0, 1.0):
+                a = [0.0, 0.0, 0.0]
+                b = [0.0, 0.0, 0.0]
+                a[axis], b[axis] = -1.0, 1.0
+                a[o1] = b[o1] = s1
+                a[o2] = b[o2] = s2
+                index[("edge", axis, s1, s2)] = len(constraints)
+                constraints.append((2, tuple(a), tuple(b)))
+    fixed = (count == 3).astype(np.int32)
+    constraint = -np.ones(V, np.int32)
+    rng = np.random.default_rng(seed)
+    jit = rng.uniform(-slide_jitter * h, slide_jitter * h, size=(V, 3))
+    for v in np.where(count == 1)[0]:
+        axis = int(np.where(on[v])[0][0])
+        constraint[v] = index[("face", axis, float(np.sign(pos[v, axis])))]
+        j = jit[v].copy()
+        j[axis] = 0.0
+        pos[v] += j
+    for v in np.where(count == 2)[0]:
+        free_axis = int(np.where(~on[v])[0][0])
+        o1, o2 = [a for a in range(3) if a != free_axis]
+        constraint[v] = index[("edge", free_axis, float(np.sign(pos[v, o1])), float(np.sign(pos[v, o2])))]
+        pos[v, free_axis] += jit[v, free_axis]
+    return pos, fixed, constraint, constraints

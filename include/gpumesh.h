@@ -95,6 +95,55 @@ int gm_mesh_create_host(int64_t nverts, const double* pos, const int32_t* fixed,
 int gm_mesh_create_grouped(int64_t nverts, const double* pos, const int32_t* fixed, int64_t ntets,
                            const uint32_t* tets, int64_t nhexs, const uint32_t* hexs, int64_t ngroups,
                            const int64_t* group_off, const uint32_t* group_nodes, int device_copy, gm_mesh** out);
+/* ---- Meshes with sliding boundary vertices.
+ *
+ * The reference's boundaries (Boundaries/, e.g. BoxBoundary on [-1,1]^3,
+ * Boundaries/BoxBoundary.cpp:7-36) let vertices slide on faces and edges:
+ * every trial position of such a vertex is projected onto its constraint
+ * (MeshTopo::snapToBoundary, DataStructures/Mesh.h:256-267).  A constraint
+ * is a plane through a with normal b (PlaneConstraint::operator(),
+ * Boundaries/Constraints/FaceConstraint.cpp:180-185) or a segment from a to b
+ * (SegmentConstraint::operator(), EdgeConstraint.cpp:185-199); a fixed
+ * vertex is a point constraint (the `fixed` flag).  Vertex types and the
+ * numbering follow NodeGroups (NodeGroups.cpp:124-450): a sliding vertex is
+ * a boundary vertex, a free vertex of one of its elements a subsurface
+ * vertex; vertex-wise sweeps run colour by colour, GETMe's vertex steps in
+ * NodeGroups' serial order (boundary, subsurface, interior). */
+enum gm_constraint_kind { GM_CONSTRAINT_PLANE = 1, GM_CONSTRAINT_SEGMENT = 2 };
+typedef struct gm_constraint
+{
+    int kind;        /* gm_constraint_kind */
+    double a[3];     /* plane: a point on it; segment: first end */
+    double b[3];     /* plane: its normal (any length); segment: second end */
+} gm_constraint;
+
+typedef struct gm_mesh_desc
+{
+    int64_t nverts;
+    const double* pos;            /* 3 nverts */
+    const int32_t* fixed;         /* nverts flags */
+    int64_t ntets;
+    const uint32_t* tets;         /* 4 ntets */
+    int64_t nhexs;
+    const uint32_t* hexs;         /* 8 nhexs */
+    /* Sliding constraints (optional): constraint[v] = index into
+     * constraints[0 .. nconstraints), or -1; a fixed vertex takes -1. */
+    const int32_t* constraint;
+    int64_t nconstraints;
+    const gm_constraint* constraints;
+    /* Colour groups (optional): ngroups < 0 colours as gm_mesh_create
+     * does (renumbering); ngroups >= 0 takes the caller's numbering and
+     * groups as gm_mesh_create_grouped does. */
+    int64_t ngroups;
+    const int64_t* group_off;
+    const uint32_t* group_nodes;
+    int device_copy;              /* 0: host topology only */
+} gm_mesh_desc;
+
+/* gm_mesh_create / gm_mesh_create_grouped from a description, with sliding
+ * constraints.  A zero-length normal or segment is refused (GM_ERR_ARG). */
+int gm_mesh_create_desc(const gm_mesh_desc* desc, gm_mesh** out);
+
 int gm_mesh_destroy(gm_mesh* mesh);
 
 /* info[8] = {nverts, ntets, nhexs, nmovable, ngroups, sum of patch sizes,

@@ -6,8 +6,8 @@
  * into the reference without a build dependency and into the tests against
  * stand-ins with the same members:
  *   Mesh:       verts[v].p.{x,y,z} (DataStructures/Mesh.h:25-37),
- *               topos[v].snapToBoundary->isFixed() / ->isConstrained()
- *               (Mesh.h:256-267, Boundaries/Constraints/AbstractConstraint.h:65-73),
+ *               topos[v].snapToBoundary->isFixed() / ->isConstrained() / ->id()
+ *               (Mesh.h:256-267, Boundaries/Constraints/AbstractConstraint.h:24,65-73),
  *               tets[e].v[4], hexs[e].v[8] (Mesh.h:66-97, 199-234),
  *               nodeGroups().parallelGroups()[g].undispatchedNodes
  *               (DataStructures/NodeGroups.h:71-93);
@@ -15,15 +15,23 @@
  *               setBucket (DataStructures/QualityHistogram.h:20-34).
  *
  * The device copy keeps the Mesh's own numbering and its NodeGroups colour
- * groups (gm_mesh_create_grouped), so positions go back to the vertices they
- * came from without any permutation.  How AbstractVertexWiseSmoother::
- * smoothMeshCuda and AbstractEvaluator::evaluateMeshQualityCuda use it is
- * shown in INTEGRATION.md.
+ * groups (gm_mesh_create_desc with groups), so positions go back to the
+ * vertices they came from without any permutation.
+ *
+ * Vertices that slide on a face or an edge need their constraint's
+ * geometry, which PlaneConstraint and SegmentConstraint keep private
+ * (FaceConstraint.h:37-45, EdgeConstraint.h:31-40).  The caller passes a
+ * `describe(const Constraint&, gm_constraint&) -> bool` function that knows
+ * it (the Boundary that made the constraints does: BoxBoundary.cpp:7-36);
+ * constraints are shared by id(), and one that describe() declines (a curved
+ * face, say) makes the constructor throw, as does any sliding vertex when no
+ * describe() is given.
  */
 #ifndef GPUMESH_BINDING_HPP
 #define GPUMESH_BINDING_HPP
 
 #include <cstdint>
+#include <map>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -38,30 +46,60 @@ inline void check(int rc, const char* where)
     if(rc != GM_OK) throw std::runtime_error(std::string(where) + ": " + gm_last_error());
 }
 
+// Positions of a reference Mesh as 3 doubles per vertex.
+template<class MeshT> std::vector<double> positionsOf(const MeshT& mesh)
+{
+    const size_t V = mesh.verts.size();
+    std::vector<double> pos(3 * V);
+    for(size_t v = 0; v < V; ++v)
+    {
+        pos[3 * v] = mesh.verts[v].p.x;
+        pos[3 * v + 1] = mesh.verts[v].p.y;
+        pos[3 * v + 2] = mesh.verts[v].p.z;
+    }
+    return pos;
+}
+
+// A describe() that knows no constraint: every sliding vertex is refused.
+struct NoSlidingConstraints
+{
+    template<class C> bool operator()(const C&, gm_constraint&) const { return false; }
+};
+
 // The flat arrays of a reference Mesh, in its own numbering.
 template<class MeshT> struct MeshArrays
 {
     std::vector<double> pos;
     std::vector<int32_t> fixed;
+    std::vector<int32_t> cons;                // constraint table index per vertex, -1 for none
+    std::vector<gm_constraint> table;
     std::vector<uint32_t> tets, hexs;
     std::vector<int64_t> groupOff;
     std::vector<uint32_t> groupNodes;
 
-    explicit MeshArrays(const MeshT& mesh)
+    template<class Describe> MeshArrays(const MeshT& mesh, const Describe& describe) : pos(positionsOf(mesh))
     {
         const size_t V = mesh.verts.size();
-        pos.resize(3 * V);
         fixed.resize(V);
+        cons.assign(V, -1);
+        std::map<int, int32_t> byId;
         for(size_t v = 0; v < V; ++v)
         {
-            pos[3 * v] = mesh.verts[v].p.x;
-            pos[3 * v + 1] = mesh.verts[v].p.y;
-            pos[3 * v + 2] = mesh.verts[v].p.z;
             const auto& c = *mesh.topos[v].snapToBoundary;
-            if(c.isConstrained() && !c.isFixed())
-                throw std::runtime_error("vertex " + std::to_string(v) + " slides on a boundary constraint: "
-                                         "smooth this mesh with the Thread implementation");
             fixed[v] = c.isFixed() ? 1 : 0;
+            if(!c.isConstrained() || c.isFixed()) continue;
+            auto it = byId.find(c.id());
+            if(it == byId.end())
+            {
+                gm_constraint g{};
+                if(!describe(c, g))
+                    throw std::runtime_error("vertex " + std::to_string(v) + " slides on constraint " +
+                                             std::to_string(c.id()) + ", which has no plane or segment "
+                                             "description: smooth this mesh with the Thread implementation");
+                it = byId.emplace(c.id(), (int32_t)table.size()).first;
+                table.push_back(g);
+            }
+            cons[v] = it->second;
         }
         tets.reserve(4 * mesh.tets.size());
         for(const auto& t : mesh.tets) tets.insert(tets.end(), t.v, t.v + 4);
@@ -82,14 +120,27 @@ template<class MeshT> struct MeshArrays
 template<class MeshT> class Session
 {
 public:
-    explicit Session(const MeshT& mesh, bool device = true) : _mesh(&mesh)
+    explicit Session(const MeshT& mesh, bool device = true) : Session(mesh, NoSlidingConstraints(), device) {}
+
+    template<class Describe> Session(const MeshT& mesh, const Describe& describe, bool device = true) : _mesh(&mesh)
     {
-        MeshArrays<MeshT> a(mesh);
-        check(gm_mesh_create_grouped((int64_t)mesh.verts.size(), a.pos.data(), a.fixed.data(),
-                                     (int64_t)mesh.tets.size(), a.tets.data(), (int64_t)mesh.hexs.size(),
-                                     a.hexs.data(), (int64_t)a.groupOff.size() - 1, a.groupOff.data(),
-                                     a.groupNodes.data(), device ? 1 : 0, &_h),
-              "gm_mesh_create_grouped");
+        MeshArrays<MeshT> a(mesh, describe);
+        gm_mesh_desc d{};
+        d.nverts = (int64_t)mesh.verts.size();
+        d.pos = a.pos.data();
+        d.fixed = a.fixed.data();
+        d.ntets = (int64_t)mesh.tets.size();
+        d.tets = a.tets.data();
+        d.nhexs = (int64_t)mesh.hexs.size();
+        d.hexs = a.hexs.data();
+        d.constraint = a.table.empty() ? nullptr : a.cons.data();
+        d.nconstraints = (int64_t)a.table.size();
+        d.constraints = a.table.empty() ? nullptr : a.table.data();
+        d.ngroups = (int64_t)a.groupOff.size() - 1;
+        d.group_off = a.groupOff.data();
+        d.group_nodes = a.groupNodes.data();
+        d.device_copy = device ? 1 : 0;
+        check(gm_mesh_create_desc(&d, &_h), "gm_mesh_create_desc");
         int64_t info[8];
         check(gm_mesh_info(_h, info), "gm_mesh_info");
         _groups = info[4];
@@ -122,8 +173,8 @@ public:
     // mesh.verts -> device (updateCudaVerts).
     void upload(const MeshT& mesh)
     {
-        MeshArrays<MeshT> a(mesh);
-        check(gm_mesh_set_positions(_h, a.pos.data()), "gm_mesh_set_positions");
+        const std::vector<double> pos = positionsOf(mesh);
+        check(gm_mesh_set_positions(_h, pos.data()), "gm_mesh_set_positions");
     }
 
     // Device -> mesh.verts (fetchCudaVerts), every position to its own vertex.

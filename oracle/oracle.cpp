@@ -164,12 +164,23 @@ static const int HEX_EDGES[12][2] = {{0,1},{0,3},{1,2},{2,3},{0,4},{1,5},{2,6},{
 
 struct NeigElem { uint32_t id; int type; int vId; };
 
+// PlaneConstraint (FaceConstraint.cpp:173-185) and SegmentConstraint
+// (EdgeConstraint.cpp:177-199) with their constructors' normalisations.
+struct SlideConstraint
+{
+    int kind = 0;          // 1 plane, 2 segment
+    dvec3 p, n;            // plane: point, unit normal
+    dvec3 a, b, u;         // segment: ends, unit direction
+    double length = 0.0;
+};
+
 struct Topo
 {
     std::vector<uint32_t> neighborVerts;
     std::vector<NeigElem> neighborElems;
-    int constraint;       // 0: free (VolumeConstraint), -1: fixed (VertexConstraint)
+    int constraint;       // 0: free (VolumeConstraint), -1: fixed (VertexConstraint), k + 1: slides on constraint k
     dvec3 fixedPos;       // VertexConstraint::_pos (VertexConstraint.cpp:60-63)
+    const SlideConstraint* slide = nullptr;
 };
 
 struct Range { size_t begin = 0, end = 0; };
@@ -251,6 +262,7 @@ struct Mesh
     std::vector<int64_t> permutation;   // positions[] of determinePositions: old id -> new id
 
     Crew crew;
+    std::vector<SlideConstraint> slides;
 };
 
 // ------------------------------------------------------------------------
@@ -888,9 +900,25 @@ static Histogram evaluateMesh(const Mesh& m, int chunks)
 // ------------------------------------------------------------------------
 // Constraint snapping (VertexConstraint.cpp:60-63; VolumeConstraint is the identity).
 // ------------------------------------------------------------------------
-static inline dvec3 snap(const Topo& t, const dvec3& p)
+static inline dvec3 snap(const Topo& t, const dvec3& pos)
 {
-    return isFixed(t) ? t.fixedPos : p;
+    if(isFixed(t)) return t.fixedPos;
+    const SlideConstraint* c = t.slide;
+    if(!c) return pos;
+    if(c->kind == 1)
+    {
+        // PlaneConstraint::operator() (FaceConstraint.cpp:180-185)
+        dvec3 dist = pos - c->p;
+        double l = dot(dist, c->n);
+        return pos - c->n * l;
+    }
+    // SegmentConstraint::operator() (EdgeConstraint.cpp:185-199)
+    dvec3 dist = pos - c->a;
+    double l = dot(dist, c->u);
+    if(l < 0) return c->a;
+    else if(l > c->length) return c->b;
+    dvec3 t2 = dist - c->u * l;
+    return pos - t2;
 }
 
 // ------------------------------------------------------------------------
@@ -1220,6 +1248,81 @@ void* orc_mesh_create(int64_t nv, const double* pos, const int32_t* fixed,
         m->verts[v] = dvec3(pos[3 * v], pos[3 * v + 1], pos[3 * v + 2]);
         m->topos[v].constraint = fixed[v] ? -1 : 0;
         m->topos[v].fixedPos = m->verts[v];
+    }
+    m->tets.resize(ntet);
+    for(int64_t e = 0; e < ntet; ++e) for(int k = 0; k < 4; ++k) m->tets[e][k] = tets[4 * e + k];
+    m->hexs.resize(nhex);
+    for(int64_t e = 0; e < nhex; ++e) for(int k = 0; k < 8; ++k) m->hexs[e][k] = hexs[8 * e + k];
+    compileNeighborhoods(*m);
+    buildNodeGroups(*m);
+    samplerRotation(m->crew);
+    return m;
+}
+
+// A mesh with sliding constraints: cons[v] = index into the table or -1;
+// kinds[k] 1 plane (params a = point, b = normal), 2 segment (a, b ends).
+// One snap of `pos` onto a constraint built as orc_mesh_create_constrained
+// builds it (kind 1 plane through a with normal b, 2 segment a-b).
+void orc_constraint_snap(int kind, const double* a, const double* b, const double* pos, double* out)
+{
+    SlideConstraint c;
+    c.kind = kind;
+    if(kind == 1)
+    {
+        c.p = dvec3(a[0], a[1], a[2]);
+        c.n = normalize(dvec3(b[0], b[1], b[2]));
+    }
+    else
+    {
+        c.a = dvec3(a[0], a[1], a[2]);
+        c.b = dvec3(b[0], b[1], b[2]);
+        c.u = normalize(c.b - c.a);
+        c.length = length(c.b - c.a);
+    }
+    Topo t;
+    t.constraint = 1;
+    t.slide = &c;
+    const dvec3 r = snap(t, dvec3(pos[0], pos[1], pos[2]));
+    out[0] = r.x; out[1] = r.y; out[2] = r.z;
+}
+
+void* orc_mesh_create_constrained(int64_t nv, const double* pos, const int32_t* fixed, const int32_t* cons,
+                                  int64_t ncons, const int32_t* kinds, const double* params, int64_t ntet,
+                                  const uint32_t* tets, int64_t nhex, const uint32_t* hexs)
+{
+    Mesh* m = new Mesh();
+    m->slides.resize(ncons);
+    for(int64_t k = 0; k < ncons; ++k)
+    {
+        SlideConstraint& c = m->slides[k];
+        const double* q = params + 6 * k;
+        c.kind = kinds[k];
+        if(c.kind == 1)
+        {
+            c.p = dvec3(q[0], q[1], q[2]);
+            c.n = normalize(dvec3(q[3], q[4], q[5]));
+        }
+        else
+        {
+            c.a = dvec3(q[0], q[1], q[2]);
+            c.b = dvec3(q[3], q[4], q[5]);
+            c.u = normalize(c.b - c.a);
+            c.length = length(c.b - c.a);
+        }
+    }
+    m->verts.resize(nv);
+    m->topos.resize(nv);
+    for(int64_t v = 0; v < nv; ++v)
+    {
+        m->verts[v] = dvec3(pos[3 * v], pos[3 * v + 1], pos[3 * v + 2]);
+        m->topos[v].fixedPos = m->verts[v];
+        if(fixed[v]) m->topos[v].constraint = -1;
+        else if(cons && cons[v] >= 0)
+        {
+            m->topos[v].constraint = cons[v] + 1;
+            m->topos[v].slide = &m->slides[cons[v]];
+        }
+        else m->topos[v].constraint = 0;
     }
     m->tets.resize(ntet);
     for(int64_t e = 0; e < ntet; ++e) for(int k = 0; k < 4; ++k) m->tets[e][k] = tets[4 * e + k];

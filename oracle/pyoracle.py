@@ -40,6 +40,10 @@ def lib():
     L.orc_mesh_create.restype = vp
     L.orc_mesh_create.argtypes = [i64, _P(np.float64), _P(np.int32), i64, _P(np.uint32), i64, _P(np.uint32)]
     L.orc_mesh_destroy.argtypes = [vp]
+    L.orc_mesh_create_constrained.restype = vp
+    L.orc_mesh_create_constrained.argtypes = [i64, _P(np.float64), _P(np.int32), _P(np.int32), i64, _P(np.int32),
+                                              _P(np.float64), i64, _P(np.uint32), i64, _P(np.uint32)]
+    L.orc_constraint_snap.argtypes = [C.c_int, _P(np.float64), _P(np.float64), _P(np.float64), _P(np.float64)]
     L.orc_set_libm.argtypes = [C.c_int]
     L.orc_set_libm.restype = C.c_int
     L.orc_mesh_nverts.argtypes = [vp]
@@ -93,6 +97,15 @@ def gm_libm(op: int, x, y=None) -> np.ndarray:
     return out
 
 
+def constraint_snap(kind: int, a, b, pos) -> np.ndarray:
+    """One snap of `pos` onto a plane (kind 1: through a, normal b) or a
+    segment (kind 2: from a to b)."""
+    out = np.empty(3, np.float64)
+    lib().orc_constraint_snap(int(kind), np.ascontiguousarray(a, np.float64), np.ascontiguousarray(b, np.float64),
+                              np.ascontiguousarray(pos, np.float64), out)
+    return out
+
+
 def set_libm(mode: int) -> int:
     """0: glibc std::pow/std::exp (the reference's calls); 1: the project's gm_libm."""
     return lib().orc_set_libm(int(mode))
@@ -101,7 +114,9 @@ def set_libm(mode: int) -> int:
 class OracleMesh:
     """An oracle mesh: topology + node groups built (and renumbered) at creation."""
 
-    def __init__(self, pos, fixed, tets, hexs):
+    def __init__(self, pos, fixed, tets, hexs, constraint=None, constraints=None):
+        """constraint[v]: index into `constraints` or -1; constraints: rows
+        (kind, a, b) with kind 1 = plane (a point, b normal), 2 = segment."""
         L = lib()
         pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
         fixed = np.ascontiguousarray(fixed, dtype=np.int32).reshape(-1)
@@ -110,8 +125,17 @@ class OracleMesh:
         self.ntet = tets.shape[0]
         self.nhex = hexs.shape[0]
         self.nverts = pos.shape[0]
-        self._h = L.orc_mesh_create(self.nverts, pos.reshape(-1), fixed, self.ntet, tets.reshape(-1),
-                                    self.nhex, hexs.reshape(-1))
+        if constraint is None:
+            self._h = L.orc_mesh_create(self.nverts, pos.reshape(-1), fixed, self.ntet, tets.reshape(-1),
+                                        self.nhex, hexs.reshape(-1))
+        else:
+            cons = np.ascontiguousarray(constraint, np.int32).reshape(-1)
+            kinds = np.ascontiguousarray([int(c[0]) for c in constraints] or [0], np.int32)
+            params = np.ascontiguousarray([list(c[1]) + list(c[2]) for c in constraints] or [[0.0] * 6],
+                                          np.float64).reshape(-1)
+            self._h = L.orc_mesh_create_constrained(self.nverts, pos.reshape(-1), fixed, cons, len(constraints),
+                                                    kinds, params, self.ntet, tets.reshape(-1), self.nhex,
+                                                    hexs.reshape(-1))
         self.evaluator = "mean_ratio"
 
     @classmethod

@@ -11,7 +11,11 @@
 //                               before the passes, a record per pass,
 //                               positions fetched into the mesh at the end
 // Inputs (raw arrays written by tests/test_binding.py): pos.bin (f64),
-// fixed.bin (i32), tets.bin, hexs.bin (u32), goff.bin (i64), gnodes.bin (u32).
+// fixed.bin (i32), tets.bin, hexs.bin (u32), goff.bin (i64), gnodes.bin (u32);
+// for sliding vertices also cons.bin (i32, a row of ctab.bin or -1) and
+// ctab.bin (7 f64 a row: kind 1 plane through a with normal b, 2 segment
+// a-b), which become stand-in PlaneConstraint / SegmentConstraint objects
+// described to the binding by their geometry.
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
@@ -27,9 +31,22 @@ struct dvec3 { double x, y, z; };
 struct MeshVert { dvec3 p; };
 struct AbstractConstraint
 {
-    int id;   // < 0 fixed, 0 free (AbstractConstraint.h:65-73)
-    bool isFixed() const { return id < 0; }
-    bool isConstrained() const { return id != 0; }
+    explicit AbstractConstraint(int id) : _id(id) {}
+    virtual ~AbstractConstraint() = default;
+    int id() const { return _id; }
+    bool isFixed() const { return _id < 0; }          // < 0 fixed, 0 free (AbstractConstraint.h:65-73)
+    bool isConstrained() const { return _id != 0; }
+    int _id;
+};
+struct PlaneConstraint : AbstractConstraint
+{
+    PlaneConstraint(int id, dvec3 p, dvec3 n) : AbstractConstraint(id), p(p), n(n) {}
+    dvec3 p, n;
+};
+struct SegmentConstraint : AbstractConstraint
+{
+    SegmentConstraint(int id, dvec3 a, dvec3 b) : AbstractConstraint(id), a(a), b(b) {}
+    dvec3 a, b;
 };
 struct MeshTopo { const AbstractConstraint* snapToBoundary; };
 struct MeshTet { uint32_t v[4]; };
@@ -80,6 +97,38 @@ template<typename X> static void save(const std::string& dir, const char* name, 
 }
 
 static const ref::AbstractConstraint FIXED{-1}, FREE{0};
+static std::vector<std::unique_ptr<ref::AbstractConstraint>> g_sliding;
+
+// What an integrator writes for its boundary: the geometry of the plane and
+// segment constraints, nothing for any other kind.
+struct Describe
+{
+    bool operator()(const ref::AbstractConstraint& c, gm_constraint& out) const
+    {
+        if(const auto* p = dynamic_cast<const ref::PlaneConstraint*>(&c))
+        {
+            out.kind = GM_CONSTRAINT_PLANE;
+            out.a[0] = p->p.x; out.a[1] = p->p.y; out.a[2] = p->p.z;
+            out.b[0] = p->n.x; out.b[1] = p->n.y; out.b[2] = p->n.z;
+            return true;
+        }
+        if(const auto* e = dynamic_cast<const ref::SegmentConstraint*>(&c))
+        {
+            out.kind = GM_CONSTRAINT_SEGMENT;
+            out.a[0] = e->a.x; out.a[1] = e->a.y; out.a[2] = e->a.z;
+            out.b[0] = e->b.x; out.b[1] = e->b.y; out.b[2] = e->b.z;
+            return true;
+        }
+        return false;
+    }
+};
+
+static bool exists(const std::string& dir, const char* name)
+{
+    FILE* f = std::fopen((dir + "/" + name).c_str(), "rb");
+    if(f) std::fclose(f);
+    return f != nullptr;
+}
 
 static ref::Mesh loadMesh(const std::string& dir)
 {
@@ -90,12 +139,25 @@ static ref::Mesh loadMesh(const std::string& dir)
     const std::vector<uint32_t> hexs = load<uint32_t>(dir, "hexs.bin");
     const std::vector<int64_t> goff = load<int64_t>(dir, "goff.bin");
     const std::vector<uint32_t> gnodes = load<uint32_t>(dir, "gnodes.bin");
+    std::vector<int32_t> cons;
+    if(exists(dir, "cons.bin"))
+    {
+        cons = load<int32_t>(dir, "cons.bin");
+        const std::vector<double> t = load<double>(dir, "ctab.bin");
+        for(size_t k = 0; 7 * k < t.size(); ++k)
+        {
+            const double* r = &t[7 * k];
+            const ref::dvec3 a{r[1], r[2], r[3]}, b{r[4], r[5], r[6]};
+            if((int)r[0] == 1) g_sliding.emplace_back(new ref::PlaneConstraint((int)k + 1, a, b));
+            else g_sliding.emplace_back(new ref::SegmentConstraint((int)k + 1, a, b));
+        }
+    }
     m.verts.resize(fixed.size());
     m.topos.resize(fixed.size());
     for(size_t v = 0; v < fixed.size(); ++v)
     {
         m.verts[v].p = ref::dvec3{pos[3 * v], pos[3 * v + 1], pos[3 * v + 2]};
-        m.topos[v].snapToBoundary = fixed[v] ? &FIXED : &FREE;
+        m.topos[v].snapToBoundary = fixed[v] ? &FIXED : !cons.empty() && cons[v] >= 0 ? g_sliding[cons[v]].get() : &FREE;
     }
     m.tets.resize(tets.size() / 4);
     for(size_t e = 0; e < m.tets.size(); ++e) for(int k = 0; k < 4; ++k) m.tets[e].v[k] = tets[4 * e + k];
@@ -113,9 +175,17 @@ int main(int argc, char** argv)
     ref::Mesh mesh = loadMesh(dir);
     try
     {
+        const bool sliding = !g_sliding.empty();
         if(mode == "host")
         {
-            gpumesh_binding::Session<ref::Mesh> s(mesh, false);
+            if(sliding)
+            {
+                bool refused = false;
+                try { gpumesh_binding::Session<ref::Mesh> t(mesh, false); }
+                catch(const std::exception& e) { refused = true; std::printf("undescribed refused: %s\n", e.what()); }
+                if(!refused) { std::fprintf(stderr, "sliding vertices accepted without a description\n"); return 3; }
+            }
+            gpumesh_binding::Session<ref::Mesh> s(mesh, Describe(), false);
             const size_t V = mesh.verts.size();
             std::vector<int64_t> perm(V);
             gpumesh_binding::check(gm_mesh_permutation(s.handle(), perm.data()), "gm_mesh_permutation");
@@ -157,7 +227,7 @@ int main(int argc, char** argv)
                         if(*it == b) { g.undispatchedNodes.erase(it); break; }
                 ga->undispatchedNodes.push_back(b);
                 bool refused = false;
-                try { gpumesh_binding::Session<ref::Mesh> t(bad, false); }
+                try { gpumesh_binding::Session<ref::Mesh> t(bad, Describe(), false); }
                 catch(const std::exception& e) { refused = true; std::printf("refused: %s\n", e.what()); }
                 if(!refused) { std::fprintf(stderr, "shared-element group accepted\n"); return 3; }
             }
@@ -170,7 +240,7 @@ int main(int argc, char** argv)
             std::vector<double> stats;
             std::vector<int32_t> hist;
             {
-                gpumesh_binding::Session<ref::Mesh> s(mesh, true);
+                gpumesh_binding::Session<ref::Mesh> s(mesh, Describe(), true);
                 s.setCrew(GM_MEAN_RATIO, GM_UNIFORM, 1.0);
                 s.upload(mesh);
                 // The reference's loop: an initial evaluation, then a record

@@ -9,7 +9,7 @@ import subprocess
 import numpy as np
 import pytest
 
-from gpumesh_amd import GpuMeshError, cube_mesh
+from gpumesh_amd import GpuMeshError, box_boundary, cube_mesh
 from gpumesh_amd.mesh import GpuMesh
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -34,9 +34,33 @@ def clustered(oracle, kind, n, jitter=0.3, seed=3):
     return m, om, pos, fixed, tets, hexs, goff, gnodes
 
 
-def write_inputs(d, pos, fixed, tets, hexs, goff, gnodes):
-    for name, a, dt in [("pos", pos, np.float64), ("fixed", fixed, np.int32), ("tets", tets, np.uint32),
-                        ("hexs", hexs, np.uint32), ("goff", goff, np.int64), ("gnodes", gnodes, np.uint32)]:
+def clustered_box(oracle, kind, n, seed=5):
+    """A BoxBoundary mesh (faces and edges sliding) as the reference holds it
+    after NodeGroups::build: renumbered type-major, each parallel group's
+    undispatchedNodes its boundary, subsurface and interior ranges."""
+    m = cube_mesh(n, kind, 0.25, seed=seed)
+    pos0, fixed0, cons0, constraints = box_boundary(m, 0.2, seed)
+    om = oracle.OracleMesh(pos0, fixed0, m.tets, m.hexs, constraints=constraints, constraint=cons0)
+    groups, _ = om.groups()
+    perm = om.permutation()
+    fixed = np.empty(m.nverts, np.int32)
+    fixed[perm] = fixed0
+    cons = np.empty(m.nverts, np.int32)
+    cons[perm] = cons0
+    tets, hexs = om.elements()
+    runs = [np.concatenate([np.arange(g[2 * t], g[2 * t + 1], dtype=np.uint32) for t in range(3)]) for g in groups]
+    gnodes = np.concatenate(runs).astype(np.uint32)
+    goff = np.concatenate([[0], np.cumsum([len(r) for r in runs])]).astype(np.int64)
+    ctab = np.array([[k] + list(a) + list(b) for k, a, b in constraints], np.float64)
+    return m, om, om.positions(), fixed, tets, hexs, goff, gnodes, cons, ctab
+
+
+def write_inputs(d, pos, fixed, tets, hexs, goff, gnodes, cons=None, ctab=None):
+    arrays = [("pos", pos, np.float64), ("fixed", fixed, np.int32), ("tets", tets, np.uint32),
+              ("hexs", hexs, np.uint32), ("goff", goff, np.int64), ("gnodes", gnodes, np.uint32)]
+    if cons is not None:
+        arrays += [("cons", cons, np.int32), ("ctab", ctab, np.float64)]
+    for name, a, dt in arrays:
         np.ascontiguousarray(a, dt).tofile(os.path.join(d, name + ".bin"))
 
 
@@ -60,6 +84,22 @@ def test_binding_keeps_the_reference_numbering(oracle, exe, tmp_path, kind, n):
     assert np.array_equal(perm, np.arange(m.nverts))
     assert np.array_equal(np.fromfile(os.path.join(d, "out_tets.bin"), np.uint32), tets.reshape(-1))
     assert np.array_equal(np.fromfile(os.path.join(d, "out_hexs.bin"), np.uint32), hexs.reshape(-1))
+    assert np.array_equal(np.fromfile(os.path.join(d, "out_goff.bin"), np.int64), goff)
+    assert np.array_equal(np.fromfile(os.path.join(d, "out_gnodes.bin"), np.uint32), gnodes)
+
+
+@pytest.mark.parametrize("kind,n", [("tet", 5), ("hex", 5)])
+def test_binding_describes_sliding_vertices(oracle, exe, tmp_path, kind, n):
+    """Stand-in PlaneConstraint and SegmentConstraint objects described to
+    the binding by their geometry: the handle keeps the numbering and the
+    groups, and the same mesh without a description is refused."""
+    m, om, pos, fixed, tets, hexs, goff, gnodes, cons, ctab = clustered_box(oracle, kind, n)
+    d = str(tmp_path)
+    write_inputs(d, pos, fixed, tets, hexs, goff, gnodes, cons, ctab)
+    r = subprocess.run([exe, "host", d], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert "host ok" in r.stdout and "undescribed refused" in r.stdout and "no plane or segment" in r.stdout
+    assert np.array_equal(np.fromfile(os.path.join(d, "out_perm.bin"), np.int64), np.arange(m.nverts))
     assert np.array_equal(np.fromfile(os.path.join(d, "out_goff.bin"), np.int64), goff)
     assert np.array_equal(np.fromfile(os.path.join(d, "out_gnodes.bin"), np.uint32), gnodes)
 

@@ -14,7 +14,7 @@ import pytest
 from gpumesh_amd import cube_mesh
 from gpumesh_amd.mesh import GpuMesh
 
-from test_binding import EXE, clustered, write_inputs
+from test_binding import EXE, clustered, clustered_box, write_inputs
 
 pytestmark = pytest.mark.gpu
 
@@ -41,6 +41,27 @@ def test_binding_run_matches_oracle(oracle, tmp_path, kind, n, smoother):
     assert np.array_equal(stats[:, 0], ref["min"])
     assert np.allclose(stats[:, 1], ref["inv_sum"], rtol=1e-12, atol=0)
     assert np.array_equal(hist, ref["hist"])
+
+
+@pytest.mark.parametrize("smoother", ["nm", "gd", "spawn", "getme"])
+@pytest.mark.parametrize("kind,n", [("tet", 7), ("hex", 6)])
+def test_binding_sliding_box_matches_oracle(oracle, tmp_path, kind, n, smoother):
+    """The binding on a BoxBoundary mesh whose faces and edges slide, with
+    the constraints described by their geometry: the oracle's positions and
+    pass records."""
+    assert os.path.exists(EXE), "tests/host/_build/binding_test is built by __graft_entry__.build()"
+    m, om, pos, fixed, tets, hexs, goff, gnodes, cons, ctab = clustered_box(oracle, kind, n)
+    d = str(tmp_path)
+    write_inputs(d, pos, fixed, tets, hexs, goff, gnodes, cons, ctab)
+    r = subprocess.run([EXE, "device", d, str(SMOOTHERS[smoother]), "2", "0"], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    ref = om.smooth(smoother, 2, threads=4)
+    got = np.fromfile(os.path.join(d, "out_pos.bin"), np.float64).reshape(-1, 3)
+    assert np.array_equal(got, om.positions())
+    stats = np.fromfile(os.path.join(d, "out_stats.bin"), np.float64).reshape(-1, 3)
+    assert np.array_equal(stats[:, 0], ref["min"])
+    assert np.array_equal(np.fromfile(os.path.join(d, "out_hist.bin"), np.int32).reshape(-1, 20), ref["hist"])
 
 
 @pytest.mark.parametrize("kind,n", [("tet", 9), ("hex", 7)])
