@@ -215,8 +215,14 @@ def batched_leg(args):
 
 
 def serving_leg(args):
-    """Poisson job stream through the serving loop at several rates."""
-    from gpumesh_amd import GpuMesh, GpuServer, cube_mesh
+    """Poisson job stream through the serving loop at several rates.  Every
+    (kind, n) shape of the job list is built into the topology cache, with
+    its device arrays, before the stream (untimed, reported), as a server
+    that knows its shape catalogue does; jobs of one shape share the element
+    arrays."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    from gpumesh_amd import GpuMesh, GpuServer, cube_mesh, prepare_topology, set_topology_cache, topology_cache_stats
     rng = np.random.default_rng(5)
     jobs = []
     for k in range(args.serve_jobs):
@@ -226,15 +232,39 @@ def serving_leg(args):
         passes = int(rng.integers(5, 21))
         smoother = ["nm", "gd", "getme"][int(rng.integers(0, 3))]
         jobs.append((n, kind, jitter, 5000 + k, smoother, passes))
+    shapes = sorted({(kind, n) for n, kind, _, _, _, _ in jobs})
+    threads = max(1, min(16, os.cpu_count() or 1))
     t0 = time.perf_counter()
-    meshes = [cube_mesh(n, kind, jit, seed=sd) for n, kind, jit, sd, _, _ in jobs]
-    log(f"serving: {len(jobs)} jobs, {sum(m.nverts for m in meshes)} vertices in total, generated in "
-        f"{time.perf_counter() - t0:.2f}s")
+    base = {}
+    with ThreadPoolExecutor(threads) as ex:
+        for (kind, n), m in zip(shapes, ex.map(lambda s: cube_mesh(s[1], s[0], 0.0, seed=1), shapes)):
+            base[(kind, n)] = m
+
+        def make(j):
+            n, kind, jit, sd, _, _ = j
+            m = cube_mesh(n, kind, jit, seed=sd)
+            b = base[(kind, n)]
+            m.tets, m.hexs = b.tets, b.hexs            # one element array per shape
+            return m
+        meshes = list(ex.map(make, jobs))
+    t_gen = time.perf_counter() - t0
+    log(f"serving: {len(jobs)} jobs, {len(shapes)} shapes, {sum(m.nverts for m in meshes)} vertices in total, "
+        f"generated in {t_gen:.2f}s")
+    set_topology_cache(args.serve_cache_gb << 30)
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(threads) as ex:
+        list(ex.map(lambda k: prepare_topology(base[k].pos, base[k].fixed, base[k].tets, base[k].hexs, device=True),
+                    shapes))
+    t_prep = time.perf_counter() - t0
+    cache = topology_cache_stats()
+    log(f"serving: {cache['entries']} topologies prepared (host and device) in {t_prep:.2f}s on {threads} threads, "
+        f"{cache['bytes'] / 1e9:.1f} GB host")
     work = sum(m.nmovable * j[5] for m, j in zip(meshes, jobs))
     rates = [float(x) for x in args.serve_rates.split(",") if x]
     out = {"workload": f"{len(jobs)} jobs per rate: tet/hex 50/50 cubes, n log-uniform [8,100], jitter U[0.2,0.4]h, "
                        f"5-20 passes, NM/GD/GETMe uniform, fp32 + fp64 polish, Poisson arrivals",
-           "jobs": len(jobs), "vertex_passes_per_run": work, "rates": {}}
+           "jobs": len(jobs), "vertex_passes_per_run": work, "shapes": len(shapes),
+           "prepare_s": t_prep, "prepare_threads": threads, "cache_host_gb": cache["bytes"] / 1e9, "rates": {}}
     with GpuServer(budget_bytes=288 * 10**9, worker_threads=args.serve_workers) as srv:
         # Untimed warm-up jobs (first launches of each kernel).
         warm = [srv.submit_mesh(meshes[0], sm, 1) for sm in ("nm", "gd", "getme")]
@@ -242,6 +272,7 @@ def serving_leg(args):
             srv.wait(w)
             srv.release(w)
         for rate in rates:
+            st0 = srv.stats()
             arr = np.cumsum(np.random.default_rng(int(rate) + 5).exponential(1.0 / rate, len(jobs)))
             ids = []
             t0 = time.perf_counter()
@@ -250,20 +281,34 @@ def serving_leg(args):
                 if delay > 0:
                     time.sleep(delay)
                 ids.append(srv.submit_mesh(m, sm, p, "fp32_polish"))
-            lat = []
+            t_sub = time.perf_counter() - t0
+            lat, parts = [], []
             for j in ids:
                 st = srv.wait(j, 120000)
                 if st != "done":
                     raise RuntimeError(f"serving job ended {st}")
-                lat.append(srv.result(j)["latency_ms"])
+                r = srv.result(j)
+                lat.append(r["latency_ms"])
+                parts.append((r["build_ms"], r["admit_wait_ms"], r["device_ms"]))
             span = time.perf_counter() - t0
+            st1 = srv.stats()
             lat = np.array(lat)
+            parts = np.array(parts)
             out["rates"][str(int(rate))] = {
                 "offered_jobs_per_s": rate, "jobs_per_s": len(jobs) / span, "vertices_per_s": work / span,
+                "arrival_span_s": float(arr[-1]), "submit_span_s": t_sub, "span_s": span,
                 "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
-                "max_ms": float(lat.max()), "span_s": span}
-            log(f"serving rate {rate:.0f}/s: {len(jobs) / span:.1f} jobs/s, {work / span / 1e6:.1f} M vertices/s, "
-                f"p50 {np.percentile(lat, 50):.1f} ms, p99 {np.percentile(lat, 99):.1f} ms")
+                "max_ms": float(lat.max()),
+                "p50_build_ms": float(np.percentile(parts[:, 0], 50)),
+                "p99_build_ms": float(np.percentile(parts[:, 0], 99)),
+                "p50_admit_wait_ms": float(np.percentile(parts[:, 1], 50)),
+                "p50_device_ms": float(np.percentile(parts[:, 2], 50)),
+                "rounds": st1["rounds"] - st0["rounds"],
+                "round_busy_frac": (st1["round_ms"] - st0["round_ms"]) / 1e3 / span}
+            log(f"serving rate {rate:.0f}/s: {len(jobs) / span:.1f} jobs/s over {span:.2f}s (arrivals span "
+                f"{arr[-1]:.2f}s), {work / span / 1e6:.1f} M vertices/s, p50 {np.percentile(lat, 50):.1f} ms, "
+                f"p99 {np.percentile(lat, 99):.1f} ms; build p50 {np.percentile(parts[:, 0], 50):.2f} ms; rounds "
+                f"{st1['rounds'] - st0['rounds']:.0f}, busy {(st1['round_ms'] - st0['round_ms']) / 1e3 / span:.2f}")
         st = srv.stats()
         out["server"] = {"rounds": st["rounds"], "round_wall_s": st["round_ms"] / 1e3, "peak_active": st["peak_active"],
                          "peak_reserved_gb": st["peak_reserved_bytes"] / 1e9, "budget_gb": st["budget_bytes"] / 1e9}
@@ -281,6 +326,8 @@ def serving_leg(args):
     out["plain_fp64_one_by_one_vertices_per_s"] = w_plain / t_plain if t_plain else None
     out["plain_sample_jobs"] = len(sample)
     log(f"serving plain fp64 sample: {len(sample)} jobs, {w_plain / max(t_plain, 1e-9) / 1e6:.1f} M vertices/s")
+    set_topology_cache(0)            # drops the serving shapes and their device copies
+    set_topology_cache(8 << 30)
     return out
 
 
@@ -302,7 +349,8 @@ def main():
     ap.add_argument("--batch-meshes", type=int, default=256)
     ap.add_argument("--batch-passes", type=int, default=10)
     ap.add_argument("--no-batch", action="store_true")
-    ap.add_argument("--serve-jobs", type=int, default=48)
+    ap.add_argument("--serve-jobs", type=int, default=1000)
+    ap.add_argument("--serve-cache-gb", type=int, default=96, help="topology cache of the serving leg (host GB)")
     ap.add_argument("--serve-rates", default="50,200,1000,2000")
     ap.add_argument("--serve-plain-sample", type=int, default=8)
     ap.add_argument("--serve-workers", type=int, default=8, help="host worker threads of the server")

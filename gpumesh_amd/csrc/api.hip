@@ -283,6 +283,25 @@ int gm_set_device_pool(int64_t max_cached_bytes)
     return GM_OK;
 }
 
+int gm_topology_prepare(int64_t nverts, const double* pos, const int32_t* fixed, int64_t ntets,
+                        const uint32_t* tets, int64_t nhexs, const uint32_t* hexs, int device_copy)
+{
+    if(gmt::cacheCapacity() == 0) return fail(GM_ERR_ARG, "the topology cache is off (gm_set_topology_cache)");
+    gm_mesh* m = nullptr;
+    int rc = gmr::makeMesh(nverts, pos, fixed, ntets, tets, nhexs, hexs, &m);
+    if(rc) return rc;
+    if(!gmt::isCached(m->topo.get()))
+        rc = fail(GM_ERR_NOMEM, "the topology is larger than the topology cache's capacity");
+    if(!rc && device_copy)
+    {
+        int ndev = 0;
+        if(hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) rc = fail(GM_ERR_HIP, "no HIP device visible");
+        else rc = gmr::prepareDeviceTopology(m);
+    }
+    gmr::destroyMesh(m);
+    return rc;
+}
+
 int gm_topology_cache_stats(int64_t* out4)
 {
     if(!out4) return fail(GM_ERR_ARG, "null argument");

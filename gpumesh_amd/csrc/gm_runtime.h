@@ -7,6 +7,7 @@
 
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <tuple>
 #include <vector>
@@ -72,10 +73,34 @@ void setPolishPasses(int n);
 
 } // namespace gmr
 
+// The read-only device arrays of one topology on one device (elements,
+// patch and neighbour lists, fixed flags, sliding constraint ids, slot map,
+// GETMe corner slots), shared by every mesh of that topology there
+// (runtime.hip, deviceTopoOf).  The destructor returns them to the pool.
+struct DeviceTopo
+{
+    uint4* tets = nullptr;
+    uint4* hexs = nullptr;
+    uint32_t *eoff = nullptr, *eref = nullptr, *noff = nullptr, *nbr = nullptr, *slotOf = nullptr;
+    uint8_t* fixed = nullptr;
+    int32_t* cons = nullptr;
+    uint32_t* cornerSlot = nullptr;   // made by the first GETMe run on this device
+    DeviceTopo() = default;
+    DeviceTopo(const DeviceTopo&) = delete;
+    DeviceTopo& operator=(const DeviceTopo&) = delete;
+    ~DeviceTopo();
+};
+
 // Device slots of a topology's vertices and the per-slot arrays uploaded for
 // it (runtime.hip, buildLayout).
 struct DeviceLayout
 {
+    // Device copies, one per device, made by the first upload there and kept
+    // as long as the layout lives: with its topology while the topology cache
+    // holds it, else with the meshes that use it.  devMu guards them.
+    mutable std::mutex devMu;
+    mutable std::map<int, std::shared_ptr<DeviceTopo>> onDevice;
+
     std::vector<uint32_t> slotOf;     // reference number -> slot
     std::vector<uint32_t> vertexAt;   // slot -> reference number
     std::vector<uint32_t> eoff, eref, noff, nbr, tets, hexs;
@@ -98,6 +123,7 @@ struct gm_mesh
     // scatters each colour group over the whole mesh.
     std::vector<uint32_t> slotOf;
     std::shared_ptr<const DeviceLayout> layout;   // the topology's device layout (slotOf is its copy)
+    std::shared_ptr<DeviceTopo> devTopo;          // its device arrays, shared (dm points into them)
     uint32_t* dSlotOf = nullptr;     // device copy of slotOf
     int32_t* dCountsRef = nullptr;   // evaluation counts in the reference numbering
     gmd::DevMesh dm{};         // host copy of the device descriptor (owns its pointers)
@@ -182,6 +208,9 @@ size_t residentBytes(const gm_mesh* m, int records, bool getme);
 // Allocates and uploads the device copy on the current device, with its own
 // stream and a plan of one.  On failure everything allocated is released.
 int uploadMesh(gm_mesh* m);
+// Makes (or finds) the device copy of m's topology arrays on the current
+// device without uploading the mesh itself.
+int prepareDeviceTopology(const gm_mesh* m);
 void destroyMesh(gm_mesh* m);
 
 // Builds the fused plan of `meshes` (all resident on one device) on `stream`.

@@ -28,22 +28,34 @@ uint64_t mix(uint64_t h, uint64_t w)
     return h * 0xC2B2AE3D27D4EB4Full + 0x165667B19E3779F9ull;
 }
 
+// Four independent lanes over 32-byte strides, so the multiply chains
+// overlap (a mesh of 6M tets hashes 96 MB per lookup).
 uint64_t hashBytes(uint64_t h, const void* data, size_t n)
 {
     const unsigned char* p = (const unsigned char*)data;
+    uint64_t l[4] = {h, h ^ 0x9E3779B97F4A7C15ull, h ^ 0xC2B2AE3D27D4EB4Full, h ^ 0x165667B19E3779F9ull};
     size_t i = 0;
+    for(; i + 32 <= n; i += 32)
+        for(int k = 0; k < 4; ++k)
+        {
+            uint64_t w;
+            std::memcpy(&w, p + i + 8 * k, 8);
+            l[k] = mix(l[k], w);
+        }
     for(; i + 8 <= n; i += 8)
     {
         uint64_t w;
         std::memcpy(&w, p + i, 8);
-        h = mix(h, w);
+        l[0] = mix(l[0], w);
     }
     uint64_t tail = 0;
     std::memcpy(&tail, p + i, n - i);
-    return mix(h, tail ^ (uint64_t)n);
+    return mix(mix(mix(mix(l[0], l[1]), l[2]), l[3]), tail ^ (uint64_t)n);
 }
 
-struct Entry
+// The inputs a cached topology was built from (compared in full on lookup,
+// outside the cache's lock), and the topology.
+struct Key
 {
     uint64_t key;
     int64_t nverts, ntets, nhexs;
@@ -51,6 +63,11 @@ struct Entry
     std::vector<int32_t> cons;      // empty without sliding constraints
     std::vector<uint32_t> tets, hexs;
     std::shared_ptr<const Topology> topo;
+};
+
+struct Entry
+{
+    std::shared_ptr<const Key> k;
     size_t bytes;
 };
 
@@ -88,6 +105,20 @@ void setCacheCapacity(size_t bytes)
     std::lock_guard<std::mutex> lock(g_mutex);
     g_capacity = bytes;
     evictTo(bytes);
+}
+
+size_t cacheCapacity()
+{
+    std::lock_guard<std::mutex> lock(g_mutex);
+    return g_capacity;
+}
+
+bool isCached(const Topology* t)
+{
+    std::lock_guard<std::mutex> lock(g_mutex);
+    for(const Entry& e : g_entries)
+        if(e.k->topo.get() == t) return true;
+    return false;
 }
 
 void cacheStats(int64_t* out4)
@@ -128,20 +159,37 @@ std::shared_ptr<const Topology> buildShared(int64_t nverts, const double* pos, c
         key = hashBytes(key, cs.data(), 4 * cs.size());
         key = hashBytes(key, tets, 16 * (size_t)ntets);
         key = hashBytes(key, hexs, 32 * (size_t)nhexs);
-        std::shared_ptr<const Topology> hit;
+        // Candidates with this hash, taken under the lock; their inputs are
+        // immutable and kept alive by the shared pointers, so the full
+        // comparison runs without it.
+        std::vector<std::shared_ptr<const Key>> cand;
         {
             std::lock_guard<std::mutex> lock(g_mutex);
+            for(const Entry& e : g_entries)
+                if(e.k->key == key && e.k->nverts == nverts && e.k->ntets == ntets && e.k->nhexs == nhexs)
+                    cand.push_back(e.k);
+        }
+        std::shared_ptr<const Key> match;
+        for(const auto& k : cand)
+        {
+            if(k->fixed != fx || k->cons != cs) continue;
+            if(ntets && std::memcmp(k->tets.data(), tets, 16 * (size_t)ntets) != 0) continue;
+            if(nhexs && std::memcmp(k->hexs.data(), hexs, 32 * (size_t)nhexs) != 0) continue;
+            match = k;
+            break;
+        }
+        std::shared_ptr<const Topology> hit;
+        if(match)
+        {
+            hit = match->topo;
+            std::lock_guard<std::mutex> lock(g_mutex);
+            ++g_hits;
             for(auto it = g_entries.begin(); it != g_entries.end(); ++it)
-            {
-                if(it->key != key || it->nverts != nverts || it->ntets != ntets || it->nhexs != nhexs) continue;
-                if(it->fixed != fx || it->cons != cs) continue;
-                if(ntets && std::memcmp(it->tets.data(), tets, 16 * (size_t)ntets) != 0) continue;
-                if(nhexs && std::memcmp(it->hexs.data(), hexs, 32 * (size_t)nhexs) != 0) continue;
-                hit = it->topo;
-                g_entries.splice(g_entries.begin(), g_entries, it);
-                ++g_hits;
-                break;
-            }
+                if(it->k == match)
+                {
+                    g_entries.splice(g_entries.begin(), g_entries, it);   // most recently used
+                    break;
+                }
         }
         if(hit)
         {
@@ -164,17 +212,19 @@ std::shared_ptr<const Topology> buildShared(int64_t nverts, const double* pos, c
     std::shared_ptr<const Topology> ct = t;
     if(cap > 0 && nverts > 0)
     {
+        auto k = std::make_shared<Key>();
+        k->key = key;
+        k->nverts = nverts;
+        k->ntets = ntets;
+        k->nhexs = nhexs;
+        k->fixed = std::move(fx);
+        k->cons = std::move(cs);
+        k->tets.assign(tets, tets + 4 * (size_t)ntets);
+        k->hexs.assign(hexs, hexs + 8 * (size_t)nhexs);
+        k->topo = ct;
         Entry e;
-        e.key = key;
-        e.nverts = nverts;
-        e.ntets = ntets;
-        e.nhexs = nhexs;
-        e.fixed = std::move(fx);
-        e.cons = std::move(cs);
-        e.tets.assign(tets, tets + 4 * (size_t)ntets);
-        e.hexs.assign(hexs, hexs + 8 * (size_t)nhexs);
-        e.topo = ct;
-        e.bytes = topologyBytes(*ct) + e.fixed.size() + 4 * (e.cons.size() + e.tets.size() + e.hexs.size());
+        e.bytes = topologyBytes(*ct) + k->fixed.size() + 4 * (k->cons.size() + k->tets.size() + k->hexs.size());
+        e.k = std::move(k);
         std::lock_guard<std::mutex> lock(g_mutex);
         ++g_misses;
         if(e.bytes <= g_capacity)

@@ -91,6 +91,9 @@ std::shared_ptr<const Topology> buildShared(int64_t nverts, const double* pos, c
                                             const int32_t* cons, int64_t ntets, const uint32_t* tets, int64_t nhexs,
                                             const uint32_t* hexs, std::vector<double>& renumbered, std::string& err);
 void setCacheCapacity(size_t bytes);
+size_t cacheCapacity();
+// Whether the cache holds topology t.
+bool isCached(const Topology* t);
 // {hits, misses, entries, bytes}
 void cacheStats(int64_t* out4);
 

@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <cstring>
 #include <map>
+#include <cstdlib>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -294,16 +295,18 @@ size_t residentBytes(const gm_mesh* m, int records, bool getme)
     return b;
 }
 
+// Frees the mesh's own device buffers and drops its share of the topology's
+// device arrays (dm's tets, hexs, eoff, eref, noff, nbr, fixed, cons,
+// cornerSlot and dSlotOf point into them).
 static void freeMeshBuffers(gm_mesh* m)
 {
     gmd::DevMesh& d = m->dm;
-    void* ptrs[] = {d.pos64, d.pos32, (void*)d.tets, (void*)d.hexs, (void*)d.eoff, (void*)d.eref,
-                    (void*)d.noff, (void*)d.nbr, (void*)d.fixed, d.evalCounts, d.getmePull,
-                    (void*)d.cornerSlot, d.getmeQual, d.evalPartials, d.evalFlags, d.evalHist, d.evalResult, d.evalHistOut, d.passSlot,
-                    (void*)d.cons, (void*)d.consTable,
-                    m->elemQualBuf, m->texels, m->dSlotOf, m->dCountsRef};
+    void* ptrs[] = {d.pos64, d.pos32, d.evalCounts, d.getmePull, d.getmeQual, d.evalPartials, d.evalFlags,
+                    d.evalHist, d.evalResult, d.evalHistOut, d.passSlot, (void*)d.consTable,
+                    m->elemQualBuf, m->texels, m->dCountsRef};
     for(void* p : ptrs)
         poolFree(p);
+    m->devTopo.reset();
     d = gmd::DevMesh{};
     m->elemQualBuf = nullptr;
     m->texels = nullptr;
@@ -403,6 +406,80 @@ static std::shared_ptr<const DeviceLayout> buildLayout(const gmt::Topology& t)
 
 static std::mutex g_layoutMu;
 
+// Device copies held by cached topologies are released at exit by a handler
+// registered after the HIP runtime has initialised, so it runs before the
+// runtime's own teardown; the cache's static destructor then frees no device
+// memory.
+static void releaseCachedDeviceCopies()
+{
+    gmt::setCacheCapacity(0);
+}
+
+// The device arrays of layout L (of topology t) on the current device:
+// found, or uploaded on stream s and published once complete.  Uploads of one
+// topology are serialised by L.devMu, so concurrent first uploads make one
+// copy.
+static hipError_t deviceTopoOf(const DeviceLayout& L, const gmt::Topology& t, int device, hipStream_t s,
+                               std::shared_ptr<DeviceTopo>& out)
+{
+    static std::once_flag atExit;
+    std::call_once(atExit, [] { std::atexit(releaseCachedDeviceCopies); });
+    std::lock_guard<std::mutex> lock(L.devMu);
+    auto it = L.onDevice.find(device);
+    if(it != L.onDevice.end())
+    {
+        out = it->second;
+        return hipSuccess;
+    }
+    auto D = std::make_shared<DeviceTopo>();
+    const size_t V = (size_t)t.nverts;
+    hipError_t e = hipSuccess;
+    auto put = [&](auto** dst, const auto& src, size_t count) {
+        if(e != hipSuccess) return;
+        e = devAlloc(dst, count);
+        if(e == hipSuccess && count)
+            e = hipMemcpyAsync(*dst, src.data(), count * sizeof(**dst), hipMemcpyHostToDevice, s);
+    };
+    static_assert(sizeof(uint4) == 16, "uint4 is 4 ids");
+    e = devAlloc(&D->tets, (size_t)t.ntets);
+    if(e == hipSuccess && t.ntets)
+        e = hipMemcpyAsync(D->tets, L.tets.data(), (size_t)t.ntets * 16, hipMemcpyHostToDevice, s);
+    if(e == hipSuccess) e = devAlloc(&D->hexs, 2 * (size_t)t.nhexs);
+    if(e == hipSuccess && t.nhexs)
+        e = hipMemcpyAsync(D->hexs, L.hexs.data(), (size_t)t.nhexs * 32, hipMemcpyHostToDevice, s);
+    put(&D->eoff, L.eoff, V + 1);
+    put(&D->eref, L.eref, L.eref.size());
+    put(&D->noff, L.noff, V + 1);
+    put(&D->nbr, L.nbr, L.nbr.size());
+    put(&D->fixed, L.fixed, V);
+    put(&D->slotOf, L.slotOf, V);
+    if(!L.cons.empty()) put(&D->cons, L.cons, V);
+    if(e == hipSuccess) e = hipStreamSynchronize(s);
+    if(e != hipSuccess) return e;
+    L.onDevice[device] = D;
+    out = D;
+    return hipSuccess;
+}
+
+// The GETMe corner slots of a topology's device copy (uploaded once).
+static hipError_t cornerSlotsOf(const DeviceLayout& L, DeviceTopo& D, hipStream_t s)
+{
+    std::lock_guard<std::mutex> lock(L.devMu);
+    if(D.cornerSlot) return hipSuccess;
+    uint32_t* p = nullptr;
+    hipError_t e = devAlloc(&p, L.cornerSlot.size());
+    if(e == hipSuccess)
+        e = hipMemcpyAsync(p, L.cornerSlot.data(), L.cornerSlot.size() * 4, hipMemcpyHostToDevice, s);
+    if(e == hipSuccess) e = hipStreamSynchronize(s);
+    if(e != hipSuccess)
+    {
+        poolFree(p);
+        return e;
+    }
+    D.cornerSlot = p;
+    return hipSuccess;
+}
+
 static std::shared_ptr<const DeviceLayout> layoutOf(const gmt::Topology& t)
 {
     {
@@ -415,6 +492,24 @@ static std::shared_ptr<const DeviceLayout> layoutOf(const gmt::Topology& t)
     if(t.deviceLayout) return std::static_pointer_cast<const DeviceLayout>(t.deviceLayout);
     t.deviceLayout = L;
     return L;
+}
+
+int prepareDeviceTopology(const gm_mesh* m)
+{
+    const gmt::Topology& t = *m->topo;
+    std::shared_ptr<const DeviceLayout> L = layoutOf(t);
+    if(!L) return fail(GM_ERR_ARG, "colour groups do not cover the movable vertices");
+    int dev = 0;
+    hipStream_t s = nullptr;
+    hipError_t e = hipGetDevice(&dev);
+    if(e == hipSuccess) e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+    std::shared_ptr<DeviceTopo> D;
+    if(e == hipSuccess) e = deviceTopoOf(*L, t, dev, s, D);
+    if(s) (void)hipStreamDestroy(s);
+    if(e != hipSuccess)
+        return fail(e == hipErrorOutOfMemory ? GM_ERR_NOMEM : GM_ERR_HIP,
+                    std::string("topology upload: ") + hipGetErrorString(e));
+    return GM_OK;
 }
 
 int uploadMesh(gm_mesh* m)
@@ -459,7 +554,6 @@ int uploadMesh(gm_mesh* m)
     if(!L) return bail(fail(GM_ERR_ARG, "colour groups do not cover the movable vertices"));
     m->layout = L;
     m->slotOf = L->slotOf;
-    const std::vector<uint32_t>& slot = L->slotOf;
     std::vector<double> p64(4 * V);
     std::vector<float> p32(4 * V);
     for(size_t s_ = 0; s_ < V; ++s_)
@@ -473,30 +567,28 @@ int uploadMesh(gm_mesh* m)
         p64[4 * s_ + 3] = 0.0;
         p32[4 * s_ + 3] = 0.0f;
     }
-    const std::vector<uint32_t>&eoD = L->eoff, &noD = L->noff, &erD = L->eref, &nbD = L->nbr, &tD = L->tets,
-                                &hD = L->hexs;
-    const std::vector<uint8_t>& fxD = L->fixed;
-    uint4 *tb, *hb;
-    uint32_t *eo, *er, *no, *nb;
-    uint8_t* fx;
+    // The topology's arrays: shared with every mesh of this topology on this
+    // device, uploaded by the first of them.
+    GM_TRY(deviceTopoOf(*L, t, m->device, m->stream, m->devTopo));
+    const DeviceTopo& D = *m->devTopo;
+    d.tets = D.tets;
+    d.hexs = D.hexs;
+    d.eoff = D.eoff;
+    d.eref = D.eref;
+    d.noff = D.noff;
+    d.nbr = D.nbr;
+    d.fixed = D.fixed;
+    d.cons = D.cons;
+    m->dSlotOf = D.slotOf;
     GM_TRY(devAlloc(&d.pos64, V));
     GM_TRY(devAlloc(&d.pos32, V));
-    GM_TRY(devAlloc(&tb, (size_t)t.ntets));        d.tets = tb;
-    GM_TRY(devAlloc(&hb, 2 * (size_t)t.nhexs));    d.hexs = hb;
-    GM_TRY(devAlloc(&eo, V + 1));                  d.eoff = eo;
-    GM_TRY(devAlloc(&er, t.eref.size()));          d.eref = er;
-    GM_TRY(devAlloc(&no, V + 1));                  d.noff = no;
-    GM_TRY(devAlloc(&nb, t.nbr.size()));           d.nbr = nb;
-    GM_TRY(devAlloc(&fx, V));                      d.fixed = fx;
-    int32_t* cs = nullptr;
     double* ct = nullptr;
     if(!L->cons.empty())
     {
-        GM_TRY(devAlloc(&cs, V));                  d.cons = cs;
-        GM_TRY(devAlloc(&ct, m->consTable.size())); d.consTable = ct;
+        GM_TRY(devAlloc(&ct, m->consTable.size()));
+        d.consTable = ct;
     }
     GM_TRY(devAlloc(&d.evalCounts, V));
-    GM_TRY(devAlloc(&m->dSlotOf, V));
     GM_TRY(devAlloc(&m->dCountsRef, V));
     GM_TRY(devAlloc(&d.evalPartials, 3 * (size_t)d.evalBlocks));
     GM_TRY(devAlloc(&d.evalFlags, (size_t)d.evalBlocks));
@@ -508,20 +600,9 @@ int uploadMesh(gm_mesh* m)
     {
         GM_TRY(hipMemcpyAsync(d.pos64, p64.data(), V * sizeof(double4), hipMemcpyHostToDevice, s));
         GM_TRY(hipMemcpyAsync(d.pos32, p32.data(), V * sizeof(float4), hipMemcpyHostToDevice, s));
-        GM_TRY(hipMemcpyAsync(eo, eoD.data(), (V + 1) * 4, hipMemcpyHostToDevice, s));
-        GM_TRY(hipMemcpyAsync(no, noD.data(), (V + 1) * 4, hipMemcpyHostToDevice, s));
-        GM_TRY(hipMemcpyAsync(fx, fxD.data(), V, hipMemcpyHostToDevice, s));
-        GM_TRY(hipMemcpyAsync(m->dSlotOf, slot.data(), V * 4, hipMemcpyHostToDevice, s));
     }
-    if(cs)
-    {
-        GM_TRY(hipMemcpyAsync(cs, L->cons.data(), V * 4, hipMemcpyHostToDevice, s));
+    if(ct)
         GM_TRY(hipMemcpyAsync(ct, m->consTable.data(), m->consTable.size() * sizeof(double), hipMemcpyHostToDevice, s));
-    }
-    if(t.ntets) GM_TRY(hipMemcpyAsync(tb, tD.data(), (size_t)t.ntets * 16, hipMemcpyHostToDevice, s));
-    if(t.nhexs) GM_TRY(hipMemcpyAsync(hb, hD.data(), (size_t)t.nhexs * 32, hipMemcpyHostToDevice, s));
-    if(!erD.empty()) GM_TRY(hipMemcpyAsync(er, erD.data(), erD.size() * 4, hipMemcpyHostToDevice, s));
-    if(!nbD.empty()) GM_TRY(hipMemcpyAsync(nb, nbD.data(), nbD.size() * 4, hipMemcpyHostToDevice, s));
     GM_TRY(hipStreamSynchronize(s));
 #undef GM_TRY
     std::vector<gm_mesh*> one{m};
@@ -769,14 +850,10 @@ static int ensureGetme(gm_mesh* m)
     const size_t N = t.eref.size();
     const size_t pullBytes = N * sizeof(double4);
     const size_t qualBytes = ((size_t)t.ntets + (size_t)t.nhexs) * sizeof(double);
-    const std::vector<uint32_t>& cs = m->layout->cornerSlot;
-    uint32_t* dcs = nullptr;
+    GMR_HIP(cornerSlotsOf(*m->layout, *m->devTopo, m->stream));
+    m->dm.cornerSlot = m->devTopo->cornerSlot;
     GMR_HIP(poolAlloc(&m->dm.getmePull, std::max<size_t>(32, pullBytes)));
-    GMR_HIP(poolAlloc((void**)&dcs, std::max<size_t>(32, cs.size() * 4)));
-    m->dm.cornerSlot = dcs;
     GMR_HIP(poolAlloc(&m->dm.getmeQual, std::max<size_t>(32, qualBytes)));
-    GMR_HIP(hipMemcpyAsync(dcs, cs.data(), cs.size() * 4, hipMemcpyHostToDevice, m->stream));
-    GMR_HIP(hipStreamSynchronize(m->stream));
     m->getmeReady = true;
     return GM_OK;
 }
@@ -1098,3 +1175,9 @@ int fetchRecords(Plan* p, size_t i, int records, double* stats, int32_t* hist, i
 }
 
 } // namespace gmr
+
+DeviceTopo::~DeviceTopo()
+{
+    void* ptrs[] = {tets, hexs, eoff, eref, noff, nbr, slotOf, fixed, cons, cornerSlot};
+    for(void* p : ptrs) gmr::poolFree(p);
+}

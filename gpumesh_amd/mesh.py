@@ -252,6 +252,29 @@ class GpuMesh:
         return r
 
 
+def set_topology_cache(max_bytes: int) -> None:
+    """gm_set_topology_cache: meshes of identical connectivity share one host
+    topology (and, while it is cached, its device arrays); 0 turns it off."""
+    check(lib().gm_set_topology_cache(int(max_bytes)), "gm_set_topology_cache")
+
+
+def topology_cache_stats() -> dict:
+    out = np.zeros(4, np.int64)
+    check(lib().gm_topology_cache_stats(ptr(out)), "gm_topology_cache_stats")
+    return {"hits": int(out[0]), "misses": int(out[1]), "entries": int(out[2]), "bytes": int(out[3])}
+
+
+def prepare_topology(pos, fixed, tets, hexs, device: bool = True) -> None:
+    """gm_topology_prepare: builds a mesh shape's topology into the cache
+    (and its device arrays when `device`) without creating a mesh."""
+    pos = np.ascontiguousarray(pos, np.float64).reshape(-1, 3)
+    fixed = np.ascontiguousarray(fixed, np.int32).reshape(-1)
+    tets = np.ascontiguousarray(tets, np.uint32).reshape(-1, 4)
+    hexs = np.ascontiguousarray(hexs, np.uint32).reshape(-1, 8)
+    check(lib().gm_topology_prepare(pos.shape[0], ptr(pos), ptr(fixed), tets.shape[0], ptr(tets), hexs.shape[0],
+                                    ptr(hexs), int(bool(device))), "gm_topology_prepare")
+
+
 def _timings(L, fn_t, fn_k, h):
     t = np.zeros(2, np.float64)
     check(fn_t(h, ptr(t)), "last_timing")

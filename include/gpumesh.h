@@ -272,8 +272,22 @@ int gm_mesh_kernel_times(const gm_mesh* mesh, double* ms, int64_t* launches);
  * exact comparison of the inputs.  max_bytes bounds the host memory it holds
  * (least recently used entries go first); 0, the default, disables it. */
 int gm_set_topology_cache(int64_t max_bytes);
-/* out4 = {hits, misses, entries, bytes held}. */
+/* out4 = {hits, misses, entries, bytes held}.  The bytes are the host
+ * topologies'; while a topology is cached, the device copy of its arrays
+ * (elements, patch and neighbour lists, fixed flags, constraint ids, slot
+ * map: about the same size), made by the first upload of one of its meshes
+ * on each device, stays resident with it, and later meshes of that
+ * topology upload only their positions. */
 int gm_topology_cache_stats(int64_t* out4);
+
+/* Builds a mesh's topology into the topology cache without creating a mesh
+ * (a server pre-building the shapes it will see, so that no cold build sits
+ * on a job's latency path); with device_copy, also its device arrays on the
+ * current device.  The positions only serve the build.  GM_ERR_ARG when the
+ * cache is off, GM_ERR_NOMEM when the topology exceeds its capacity.
+ * Thread-safe.  No counterpart in the reference. */
+int gm_topology_prepare(int64_t nverts, const double* pos, const int32_t* fixed, int64_t ntets,
+                        const uint32_t* tets, int64_t nhexs, const uint32_t* hexs, int device_copy);
 
 /* Number of FP64 passes that end a GM_FP32_POLISH run (1 to 64, default 1;
  * a run of fewer passes runs them all in FP64).  Applies to runs, batches

@@ -136,3 +136,38 @@ def test_jobs_without_movable_vertices():
             assert srv.wait(j, 60000) == "done", k
             _check(srv.result(j), refs[k], k)
         assert srv.stats()["failed"] == 0
+
+
+def test_shared_device_topology_and_prepare():
+    """Meshes of one cached topology share its device arrays: results are
+    bit-identical to uncached meshes, a mesh keeps working after another of
+    its topology is destroyed, and a topology prepared ahead of time
+    (gm_topology_prepare) serves later meshes the same way."""
+    from gpumesh_amd import prepare_topology, set_topology_cache, topology_cache_stats
+    ma, mb = cube_mesh(10, "tet", 0.3, seed=1), cube_mesh(10, "tet", 0.3, seed=2)
+    mh = cube_mesh(9, "hex", 0.3, seed=3)
+    ref = {}
+    for key, m, sm in (("a", ma, "nm"), ("b", mb, "getme"), ("h", mh, "gd")):
+        g = GpuMesh.from_cube(m)
+        r = g.smooth(sm, 2, precision="fp64", kernel="wave")
+        ref[key] = (g.positions(), r["min"])
+        g.close()
+    set_topology_cache(1 << 30)
+    try:
+        prepare_topology(mh.pos, mh.fixed, mh.tets, mh.hexs, device=True)
+        a, b = GpuMesh.from_cube(ma), GpuMesh.from_cube(mb)
+        st = topology_cache_stats()
+        assert st["entries"] == 2 and st["hits"] >= 1
+        ra = a.smooth("nm", 2, precision="fp64", kernel="wave")
+        a.close()
+        rb = b.smooth("getme", 2, precision="fp64", kernel="wave")
+        assert _same(ra["min"], ref["a"][1]) and _same(rb["min"], ref["b"][1])
+        assert _same(b.positions(), ref["b"][0])
+        h = GpuMesh.from_cube(mh)
+        rh = h.smooth("gd", 2, precision="fp64", kernel="wave")
+        assert _same(h.positions(), ref["h"][0]) and _same(rh["min"], ref["h"][1])
+        assert topology_cache_stats()["hits"] >= 2
+        b.close()
+        h.close()
+    finally:
+        set_topology_cache(0)

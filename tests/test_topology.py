@@ -126,3 +126,28 @@ def test_topology_cache_hits_are_exact(oracle):
     finally:
         check(lib().gm_set_topology_cache(0), "cache off")
     assert stats()[2] == 0
+
+
+def test_topology_prepare_host_only():
+    """gm_topology_prepare: refused with the cache off, refused for a
+    topology larger than the capacity, and a cache hit for the next mesh of
+    that shape otherwise."""
+    from gpumesh_amd import GpuMeshError, prepare_topology, set_topology_cache, topology_cache_stats
+    m = cube_mesh(6, "hex", 0.2, seed=4)
+    with pytest.raises(GpuMeshError, match="cache is off"):
+        prepare_topology(m.pos, m.fixed, m.tets, m.hexs, device=False)
+    set_topology_cache(4096)
+    try:
+        with pytest.raises(GpuMeshError, match="capacity"):
+            prepare_topology(m.pos, m.fixed, m.tets, m.hexs, device=False)
+        set_topology_cache(1 << 28)
+        prepare_topology(m.pos, m.fixed, m.tets, m.hexs, device=False)
+        before = topology_cache_stats()
+        assert before["entries"] == 1
+        other = cube_mesh(6, "hex", 0.3, seed=9)
+        g = GpuMesh.from_cube(other, device=False)
+        assert topology_cache_stats()["hits"] == before["hits"] + 1
+        ref = GpuMesh(other.pos, other.fixed, other.tets, other.hexs, device=False)
+        assert np.array_equal(g.permutation(), ref.permutation())
+    finally:
+        set_topology_cache(0)
