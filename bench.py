@@ -79,6 +79,17 @@ FLOP_AFFINE_TET = 20.0
 TRAFFIC_PROFILES = {("spawnMfmaKernel<0, 4>", 100): "profiles/spawn_mfma_tet100_hbm.md"}
 
 
+def issue_rates():
+    """FP32 vector issue rates of the device (gm_throughput_probe): chains of
+    the log2/exp2 pair, chains of FMAs, and both in one loop."""
+    from gpumesh_amd._lib import check, lib
+    r = [np.zeros(3) for _ in range(3)]
+    for op in range(3):
+        check(lib().gm_throughput_probe(op, r[op].ctypes.data), "gm_throughput_probe")
+    return {"trans_lane_ops_per_s": float(r[0][0]), "fma_lane_ops_per_s": float(r[1][1]),
+            "trans_ms": float(r[0][2]), "fma_ms": float(r[1][2]), "mixed_ms": float(r[2][2])}
+
+
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
@@ -382,6 +393,10 @@ def main():
         sys.exit(1)
     smoothers = [s for s in args.smoothers.split(",") if s]
     spawn_coeff = 0.25
+    issue = issue_rates()
+    log(f"issue rates: transcendental {issue['trans_lane_ops_per_s'] / 1e12:.2f} T lane-ops/s, FMA "
+        f"{issue['fma_lane_ops_per_s'] / 1e12:.2f} T lane-ops/s, both {issue['mixed_ms']:.3f} ms vs "
+        f"{issue['trans_ms'] + issue['fma_ms']:.3f} ms apart")
     from gpumesh_amd._lib import check, lib
     check(lib().gm_set_topology_cache(8 << 30), "gm_set_topology_cache")
     check(lib().gm_set_polish_passes(args.polish_passes), "gm_set_polish_passes")
@@ -488,6 +503,16 @@ def main():
             if traffic:
                 roofline["hbm_gbs"] = traffic / avg_s / 1e9
                 roofline["hbm_frac"] = roofline["hbm_gbs"] / PEAK_HBM_GBS
+            # The vector-issue ceiling measured on this device: every element
+            # evaluation issues a log2 and an exp2 (transcendental rate) and a
+            # scale and an accumulate (FMA rate); the affine terms run on the
+            # matrix cores.  The two kinds share issue slots (the probe's mixed
+            # run takes the sum of the two times).
+            trans, fma = issue["trans_lane_ops_per_s"], issue["fma_lane_ops_per_s"]
+            core_s = elems * (2.0 / trans + 2.0 / fma)
+            roofline["issue_ceiling"] = dict(issue, core_instructions_per_evaluation="2 transcendental + 2 FMA-rate",
+                                             evaluations_per_launch=elems, core_ms_per_launch=core_s * 1e3,
+                                             frac=core_s / avg_s)
         elif best["class"] == "evaluate":
             bytes_per = 16.0 * gm.ntets + 32.0 * gm.nverts
             achieved = bytes_per / avg_s / 1e9

@@ -30,7 +30,7 @@ EXPORTS = [
     "gm_topology_cache_stats", "gm_math_probe", "gm_mesh_smooth_schedule", "gm_batch_smooth_schedule",
     "gm_job_size", "gm_job_encode", "gm_job_info", "gm_job_decode", "gm_server_submit_job",
     "gm_set_polish_passes", "gm_evaluate_noalloc", "gm_mesh_create_grouped", "gm_mesh_set_aspect_ratio",
-    "gm_mesh_create_desc", "gm_topology_prepare",
+    "gm_mesh_create_desc", "gm_topology_prepare", "gm_throughput_probe",
 ]
 
 
@@ -123,6 +123,7 @@ def lib():
     L.gm_evaluate_noalloc.argtypes = [vp, i64, vp, i64, vp, i64, C.c_int, C.c_int, dbl, vp, vp, vp]
     L.gm_topology_cache_stats.argtypes = [vp]
     L.gm_topology_prepare.argtypes = [i64, vp, vp, i64, vp, i64, vp, C.c_int]
+    L.gm_throughput_probe.argtypes = [C.c_int, vp]
     L.gm_cube_mesh.argtypes = [C.c_int, C.c_int, dbl, C.c_uint64, dbl, vp, vp, vp]
     L.gm_math_probe.argtypes = [C.c_int, vp, vp, vp, i64]
     for name in EXPORTS:

@@ -792,6 +792,44 @@ int gm_batch_autotune(gm_batch* b, int smoother, int precision, int passes, int3
     return GM_OK;
 }
 
+int gm_throughput_probe(int op, double* rates)
+{
+    if(op < 0 || op > 2 || !rates) return fail(GM_ERR_ARG, "bad probe arguments");
+    int ndev = 0, dev = 0;
+    if(hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(GM_ERR_HIP, "no HIP device visible");
+    GM_HIP(hipGetDevice(&dev));
+    int cus = 0;
+    GM_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    const int blocks = 16 * cus, threads = 256, iters = 2048;
+    float* out = nullptr;
+    GM_HIP(devAlloc(&out, (size_t)blocks * threads));
+    hipStream_t s = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+    if(e == hipSuccess) e = hipEventCreate(&e0);
+    if(e == hipSuccess) e = hipEventCreate(&e1);
+    float ms = 0.0f, best = 0.0f;
+    for(int rep = 0; rep < 4 && e == hipSuccess; ++rep)   // the first run warms up
+    {
+        e = hipEventRecord(e0, s);
+        if(e == hipSuccess) e = gmd::launchThroughput(op, blocks, iters, out, s);
+        if(e == hipSuccess) e = hipEventRecord(e1, s);
+        if(e == hipSuccess) e = hipEventSynchronize(e1);
+        if(e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+        if(e == hipSuccess && rep > 0 && (best == 0.0f || ms < best)) best = ms;
+    }
+    if(e1) (void)hipEventDestroy(e1);
+    if(e0) (void)hipEventDestroy(e0);
+    if(s) (void)hipStreamDestroy(s);
+    gmr::poolFree(out);
+    if(e != hipSuccess) return fail(GM_ERR_HIP, std::string("throughput probe: ") + hipGetErrorString(e));
+    const double steps = (double)blocks * threads * iters * gmd::throughputChains() / (best * 1e-3);
+    rates[0] = op == 1 ? 0.0 : 2.0 * steps;   // transcendental instructions per second (lane ops)
+    rates[1] = op == 0 ? 0.0 : steps;         // FMA instructions per second (lane ops)
+    rates[2] = best;
+    return GM_OK;
+}
+
 int gm_math_probe(int op, const double* a, const double* b, double* out, int64_t n)
 {
     if(op < 0 || op > 8 || n < 0 || (n && (!a || !out || ((op == 1 || op == 8) && !b))))

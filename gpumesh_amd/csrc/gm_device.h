@@ -214,6 +214,8 @@ bool laneFits(const VertexWork& w, int smoother, LaneShape lane, bool fp64, int 
 
 // Math self-test: op 0 sqrt, 1 div (a/b), 2 pow23, 3 cbrt, 4 exp; n values.
 hipError_t launchMathProbe(int op, const double* a, const double* b, double* out, int n, hipStream_t stream);
+hipError_t launchThroughput(int op, int blocks, int iters, float* out, hipStream_t stream);
+int throughputChains();
 
 } // namespace gmd
 

@@ -114,6 +114,51 @@ __global__ void mathProbeKernel(int op, const double* a, const double* b, double
     out[i] = r;
 }
 
+// Issue-rate probe of the FP32 vector instructions the affine kernels spend
+// their time in.  Every lane runs THR_ACC independent chains for `iters`
+// steps: op 0 the transcendental pair of a^(-2/3) (a = exp2(-|log2 a|), a
+// fixed point for a in (0, 1], so the chain stays bounded with no other
+// instruction: the negation and the absolute value are operand modifiers),
+// op 1 one FMA per step, op 2 both on separate chains (whether the two kinds
+// overlap).  Each lane writes the sum of its chains.
+constexpr int THR_ACC = 8;
+
+__global__ void __launch_bounds__(256) throughputKernel(int op, int iters, float* out)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    float a[THR_ACC], f[THR_ACC];
+#pragma unroll
+    for(int k = 0; k < THR_ACC; ++k)
+    {
+        a[k] = 0.25f + 0.0625f * (float)k + 1e-7f * (float)(t & 1023);
+        f[k] = a[k];
+    }
+    const float b = 0.999f, c = 1e-3f;
+    for(int i = 0; i < iters; ++i)
+    {
+        if(op != 1)
+#pragma unroll
+            for(int k = 0; k < THR_ACC; ++k)
+                a[k] = __builtin_amdgcn_exp2f(-__builtin_fabsf(__builtin_amdgcn_logf(a[k])));
+        if(op != 0)
+#pragma unroll
+            for(int k = 0; k < THR_ACC; ++k)
+                f[k] = __builtin_fmaf(f[k], b, c);
+    }
+    float s = 0.0f;
+#pragma unroll
+    for(int k = 0; k < THR_ACC; ++k) s += a[k] + f[k];
+    out[t] = s;
+}
+
+hipError_t launchThroughput(int op, int blocks, int iters, float* out, hipStream_t stream)
+{
+    throughputKernel<<<blocks, 256, 0, stream>>>(op, iters, out);
+    return hipGetLastError();
+}
+
+int throughputChains() { return THR_ACC; }
+
 hipError_t launchMathProbe(int op, const double* a, const double* b, double* out, int n, hipStream_t stream)
 {
     mathProbeKernel<<<(n + 255) / 256, 256, 0, stream>>>(op, a, b, out, n);

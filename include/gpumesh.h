@@ -472,6 +472,14 @@ int gm_evaluate_noalloc(const double* d_pos, int64_t nverts, const uint32_t* d_t
  * all in FP64 on the device; n values. */
 int gm_math_probe(int op, const double* a, const double* b, double* out, int64_t n);
 
+/* Issue-rate probe of the current device's FP32 vector instructions (the
+ * ceiling the affine smoothing kernels are measured against): op 0 chains of
+ * the transcendental pair exp2/log2 of a^(-2/3), op 1 chains of FMAs, op 2
+ * both on independent chains in one loop.  rates = {transcendental lane ops
+ * per second, FMA lane ops per second, best kernel ms}.  No counterpart in
+ * the reference. */
+int gm_throughput_probe(int op, double* rates);
+
 #ifdef __cplusplus
 }
 #endif

@@ -92,3 +92,16 @@ def test_metric_wise_zero_length_edge(oracle):
     r = om.evaluate()
     assert g["min"] == r["min"]
     assert np.array_equal(g["hist"], r["hist"])
+
+
+def test_throughput_probe_is_sane():
+    """gm_throughput_probe: positive rates, transcendentals no faster than
+    FMAs, and the mixed loop no faster than its slower part."""
+    from gpumesh_amd._lib import check, lib
+    r = [np.zeros(3) for _ in range(3)]
+    for op in range(3):
+        check(lib().gm_throughput_probe(op, r[op].ctypes.data), "gm_throughput_probe")
+    trans, fma = r[0][0], r[1][1]
+    assert 1e12 < trans <= 1.1 * fma < 1e15
+    assert r[0][1] == 0.0 and r[1][0] == 0.0
+    assert r[2][2] >= 0.9 * max(r[0][2], r[1][2])
