@@ -30,7 +30,7 @@ EXPORTS = [
     "gm_topology_cache_stats", "gm_math_probe", "gm_mesh_smooth_schedule", "gm_batch_smooth_schedule",
     "gm_job_size", "gm_job_encode", "gm_job_info", "gm_job_decode", "gm_server_submit_job",
     "gm_set_polish_passes", "gm_evaluate_noalloc", "gm_mesh_create_grouped", "gm_mesh_set_aspect_ratio",
-    "gm_mesh_create_desc", "gm_topology_prepare", "gm_throughput_probe",
+    "gm_mesh_create_desc", "gm_topology_prepare", "gm_throughput_probe", "gm_server_set_texture",
 ]
 
 
@@ -111,6 +111,7 @@ def lib():
     L.gm_server_create.argtypes = [i64, C.c_int, C.c_int, C.c_int, C.c_int, dbl, C.POINTER(vp)]
     L.gm_server_destroy.argtypes = [vp]
     L.gm_server_set_shape.argtypes = [vp, C.c_int, C.c_int, C.c_int]
+    L.gm_server_set_texture.argtypes = [vp, vp, vp, vp, vp]
     L.gm_server_submit.argtypes = [vp, i64, vp, vp, i64, vp, i64, vp, C.c_int, C.c_int, C.c_int, C.c_int,
                                    C.POINTER(i64)]
     L.gm_server_wait.argtypes = [vp, i64, dbl, C.POINTER(C.c_int)]

@@ -53,12 +53,18 @@ int checkRun(int smoother, int passes, int precision, int kernel, double spawnCo
     return GM_OK;
 }
 
-int checkSampler(int sampler, int smoother, int kernel)
+// Batch members share one crew: the same evaluator, sampler, scaling and
+// aspect ratio, and under the texture sampler the same grid and texels.
+bool sameCrew(const gm_mesh* a, const gm_mesh* b)
 {
-    if(sampler == GM_TEXTURE && (kernel == GM_KERNEL_PLAIN || smoother == gmd::SM_GETME))
-        return fail(GM_ERR_UNSUPPORTED, "the texture sampler runs only in the wavefront kernels of the vertex-wise "
-                                        "smoothers (GM_KERNEL_WAVE with Nelder-Mead, gradient descent or spawn search)");
-    return GM_OK;
+    if(a->evaluator != b->evaluator || a->sampler != b->sampler || a->crew.scaling != b->crew.scaling ||
+       a->crew.aspect != b->crew.aspect)
+        return false;
+    if(a->sampler != GM_TEXTURE) return true;
+    for(int k = 0; k < 3; ++k)
+        if(a->crew.tsize[k] != b->crew.tsize[k] || a->crew.tmin[k] != b->crew.tmin[k] || a->crew.text[k] != b->crew.text[k])
+            return false;
+    return a->texHash == b->texHash;
 }
 
 // Default launch shape: one wavefront per workgroup (GETMe vertex update: 4)
@@ -431,6 +437,7 @@ int gm_mesh_set_crew(gm_mesh* m, int evaluator, int sampler, double scaling,
         c.texels = fresh;
         gmr::poolFree(m->texels);
         m->texels = fresh;
+        m->texHash = gmr::texelHash(texels, n);
     }
     m->crew = c;
     m->evaluator = evaluator;
@@ -481,7 +488,6 @@ int gm_mesh_smooth(gm_mesh* m, int smoother, int passes, int precision, int kern
     GM_NEED_DEVICE(m);
     int rc = checkRun(smoother, passes, precision, kernel, spawnCoeff);
     if(rc) return rc;
-    if((rc = checkSampler(m->sampler, smoother, kernel))) return rc;
     GM_HIP(hipSetDevice(m->device));
     gmr::RunParams r;
     r.smoother = smoother;
@@ -501,7 +507,6 @@ int gm_mesh_smooth_schedule(gm_mesh* m, int smoother, int maxPasses, const gm_sc
     GM_NEED_DEVICE(m);
     int rc = checkRun(smoother, maxPasses, precision, kernel, spawnCoeff);
     if(rc) return rc;
-    if((rc = checkSampler(m->sampler, smoother, kernel))) return rc;
     GM_HIP(hipSetDevice(m->device));
     gmr::RunParams r;
     r.smoother = smoother;
@@ -529,7 +534,6 @@ int gm_mesh_smooth_groups(gm_mesh* m, int smoother, int64_t groupBegin, int64_t 
                                         "only over all colour groups [0, ngroups)");
     if(precision == GM_FP32_POLISH)
         return fail(GM_ERR_ARG, "precision must be GM_FP64, GM_FP32 or GM_FP64_FAST (a partial sweep has no last pass)");
-    if((rc = checkSampler(m->sampler, smoother, kernel))) return rc;
     if(groupBegin < 0 || groupEnd > m->topo->ngroups() || groupBegin > groupEnd)
         return fail(GM_ERR_ARG, "bad group range");
     GM_HIP(hipSetDevice(m->device));
@@ -584,10 +588,9 @@ int gm_batch_create(gm_mesh* const* meshes, int64_t n, gm_batch** out)
     {
         if(!m || !m->onDevice) return fail(GM_ERR_ARG, "batch member without a device copy");
         if(m->device != ms[0]->device) return fail(GM_ERR_ARG, "batch members live on different devices");
-        if(m->sampler == GM_TEXTURE) return fail(GM_ERR_UNSUPPORTED, "batches do not take the texture sampler");
-        if(m->evaluator != ms[0]->evaluator || m->sampler != ms[0]->sampler || m->crew.scaling != ms[0]->crew.scaling ||
-           m->crew.aspect != ms[0]->crew.aspect)
-            return fail(GM_ERR_ARG, "batch members need the same evaluator, sampler, scaling and aspect ratio");
+        if(!sameCrew(m, ms[0]))
+            return fail(GM_ERR_ARG, "batch members need the same evaluator, sampler, scaling and aspect ratio "
+                                    "(and the same texture grid and texels under the texture sampler)");
     }
     std::vector<gm_mesh*> sorted = ms;
     std::sort(sorted.begin(), sorted.end());
@@ -640,8 +643,7 @@ static int checkBatchCrew(const gm_batch* b)
 {
     const std::vector<gm_mesh*>& ms = b->plan->meshes;
     for(const gm_mesh* m : ms)
-        if(m->sampler == GM_TEXTURE || m->evaluator != ms[0]->evaluator || m->sampler != ms[0]->sampler ||
-           m->crew.scaling != ms[0]->crew.scaling || m->crew.aspect != ms[0]->crew.aspect)
+        if(!sameCrew(m, ms[0]))
             return fail(GM_ERR_ARG, "batch members' crews diverged after the batch was made");
     return GM_OK;
 }

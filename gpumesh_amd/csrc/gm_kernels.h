@@ -304,7 +304,7 @@ __device__ __forceinline__ T elementQualityAt(const DevMesh& m, const gmq::Crew&
 // AbstractEvaluator::patchQuality (Evaluators/AbstractEvaluator.cpp:236-279),
 // one lane walking the whole patch.
 template<typename T, int EV, int SA>
-struct PlainPatch
+struct PlainPatchWalk
 {
     const DevMesh& m;
     const gmq::Crew& c;
@@ -312,7 +312,7 @@ struct PlainPatch
     uint32_t v;
     int count;
 
-    __device__ T operator()(v3<T> p)
+    __device__ __forceinline__ T at(v3<T> p)
     {
         ++count;
         T pq = T(0), pw = T(0);
@@ -321,6 +321,28 @@ struct PlainPatch
             gmq::accumulatePatch<T>(pq, pw, elementQualityAt<T, EV, SA>(m, c, pos, m.eref[r], p));
         return gmq::finalizePatch<T>(pq, pw);
     }
+};
+
+// The patch functor of the one-lane-per-vertex kernels; the compiler may
+// keep it out of line.
+template<typename T, int EV, int SA>
+struct PlainPatch : PlainPatchWalk<T, EV, SA>
+{
+    __device__ T operator()(v3<T> p) { return this->at(p); }
+};
+
+// Under the texture sampler the walk is always inlined into its kernel.  Its
+// out-of-line copy (metric conformity, FP64) is so long that the compiler
+// expands its branches into s_getpc/s_setpc sequences through s[30:31], the
+// register pair that holds the function's return address, without saving
+// it: the function's final s_setpc_b64 s[30:31] then jumps back into its own
+// body and the kernel never ends.  A kernel has no return address, so long
+// branches inside it are harmless.  tests/test_isa_hazards.py checks every
+// function of the built library for such an expansion.
+template<typename T, int EV>
+struct PlainPatch<T, EV, gmq::SA_TEXTURE> : PlainPatchWalk<T, EV, gmq::SA_TEXTURE>
+{
+    __device__ __forceinline__ T operator()(v3<T> p) { return this->at(p); }
 };
 
 // Plain kernel: one vertex of the work list per lane.

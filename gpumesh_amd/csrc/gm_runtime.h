@@ -140,6 +140,7 @@ struct gm_mesh
     // the device copy is dm.consTable).
     std::vector<double> consTable;
     float* texels = nullptr;
+    uint64_t texHash = 0;      // FNV-1a of the texels (batch members must share the grid)
     gmr::Plan* plan = nullptr; // the mesh as a batch of one
 };
 
@@ -196,6 +197,9 @@ int makeMesh(int64_t nverts, const double* pos, const int32_t* fixed, int64_t nt
 // sampler's fixed rotation (AbstractSampler's constructor,
 // Samplers/AbstractSampler.cpp:30-41) and its transpose.
 void setAnalyticField(gmq::Crew& c, double aspect);
+// FNV-1a (64-bit) over the bytes of n texels: the identity of a texture grid
+// for crews that must share one.
+uint64_t texelHash(const float* texels, size_t n);
 
 // makeMesh with the caller's numbering and colour groups (gmt::buildGrouped).
 int makeMeshGrouped(int64_t nverts, const double* pos, const int32_t* fixed, int64_t ntets, const uint32_t* tets,

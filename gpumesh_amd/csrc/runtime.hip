@@ -249,6 +249,18 @@ void setAnalyticField(gmq::Crew& c, double aspect)
         }
 }
 
+uint64_t texelHash(const float* texels, size_t n)
+{
+    uint64_t h = 0xcbf29ce484222325ull;
+    const unsigned char* b = (const unsigned char*)texels;
+    for(size_t i = 0; i < n * sizeof(float); ++i)
+    {
+        h ^= b[i];
+        h *= 0x100000001b3ull;
+    }
+    return h;
+}
+
 int makeMeshGrouped(int64_t nverts, const double* pos, const int32_t* fixed, int64_t ntets, const uint32_t* tets,
                     int64_t nhexs, const uint32_t* hexs, int64_t ngroups, const int64_t* groupOff,
                     const uint32_t* groupNodes, gm_mesh** out, const int32_t* cons,

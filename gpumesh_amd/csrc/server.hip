@@ -100,6 +100,7 @@ struct gm_server
     int maxActive = 0;
     int evaluator = 0, sampler = 0;
     gmq::Crew crew{};
+    float* texels = nullptr;         // the texture grid every job samples (GM_TEXTURE)
     gmd::LaneShape lane[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
 
     std::mutex mu;
@@ -499,6 +500,8 @@ int enqueue(gm_server* s, std::unique_ptr<Job> j, int64_t* jobId)
 {
     std::lock_guard<std::mutex> lock(s->mu);
     if(s->stop) return fail(GM_ERR_ARG, "server is shutting down");
+    if(s->sampler == GM_TEXTURE && !s->crew.texels)
+        return fail(GM_ERR_ARG, "texture sampler: gm_server_set_texture must come before the first job");
     j->id = s->nextId++;
     *jobId = j->id;
     Job* raw = j.get();
@@ -520,9 +523,7 @@ int gm_server_create(int64_t budgetBytes, int maxActive, int workerThreads, int 
     if(!out) return fail(GM_ERR_ARG, "out is null");
     *out = nullptr;
     if(evaluator < 0 || evaluator > 1) return fail(GM_ERR_ARG, "unknown evaluator");
-    if(sampler < 0 || sampler > 3 || sampler == GM_TEXTURE)
-        return fail(sampler == GM_TEXTURE ? GM_ERR_UNSUPPORTED : GM_ERR_ARG,
-                    "the server takes the uniform or an analytic sampler");
+    if(sampler < 0 || sampler > 3) return fail(GM_ERR_ARG, "unknown sampler");
     if(!(scaling > 0.0)) return fail(GM_ERR_ARG, "scaling must be positive");
     if(maxActive < 1 || workerThreads < 1 || workerThreads > 64) return fail(GM_ERR_ARG, "bad slot or thread count");
     int ndev = 0;
@@ -565,7 +566,44 @@ int gm_server_destroy(gm_server* s)
     // still hold a host topology or device memory.
     for(auto& kv : s->jobs)
         if(kv.second->mesh) gmr::destroyMesh(kv.second->mesh);
+    gmr::poolFree(s->texels);
     delete s;
+    return GM_OK;
+}
+
+int gm_server_set_texture(gm_server* s, const int32_t* tsize, const double* tmin, const double* text,
+                          const float* texels)
+{
+    if(!s) return fail(GM_ERR_ARG, "null server");
+    if(s->sampler != GM_TEXTURE) return fail(GM_ERR_ARG, "the server was not created with the texture sampler");
+    if(!tsize || !tmin || !text || !texels) return fail(GM_ERR_ARG, "texture sampler needs a grid");
+    gmq::Crew c = s->crew;
+    for(int k = 0; k < 3; ++k)
+    {
+        if(tsize[k] < 1) return fail(GM_ERR_ARG, "texture size must be >= 1");
+        if(!(text[k] > 0.0)) return fail(GM_ERR_ARG, "texture extents must be positive");
+        c.tsize[k] = tsize[k];
+        c.tmin[k] = tmin[k];
+        c.text[k] = text[k];
+    }
+    std::lock_guard<std::mutex> lock(s->mu);
+    // Jobs read the crew without the lock, so it is fixed once one is queued.
+    if(s->submitted) return fail(GM_ERR_ARG, "the texture is set before the first job is submitted");
+    if(hipSetDevice(s->device) != hipSuccess) return fail(GM_ERR_HIP, "hipSetDevice failed");
+    const size_t n = 6 * (size_t)tsize[0] * tsize[1] * tsize[2];
+    float* fresh = nullptr;
+    hipError_t e = gmr::poolAlloc((void**)&fresh, n * sizeof(float));
+    if(e != hipSuccess) return fail(e == hipErrorOutOfMemory ? GM_ERR_NOMEM : GM_ERR_HIP, "texture allocation failed");
+    e = hipMemcpy(fresh, texels, n * sizeof(float), hipMemcpyHostToDevice);
+    if(e != hipSuccess)
+    {
+        gmr::poolFree(fresh);
+        return fail(GM_ERR_HIP, std::string("texture upload: ") + hipGetErrorString(e));
+    }
+    gmr::poolFree(s->texels);
+    s->texels = fresh;
+    c.texels = fresh;
+    s->crew = c;
     return GM_OK;
 }
 

@@ -18,11 +18,17 @@ class GpuServer:
     (no copy) and held here until the job is built."""
 
     def __init__(self, budget_bytes: int = 0, max_active: int = 4096, worker_threads: int = 8,
-                 evaluator: str = "mean_ratio", sampler: str = "uniform", scaling: float = 1.0):
+                 evaluator: str = "mean_ratio", sampler: str = "uniform", scaling: float = 1.0, texture=None):
+        """texture = (size[3], tmin[3], extent[3], texels) for sampler "texture"."""
         h = C.c_void_p()
         check(lib().gm_server_create(int(budget_bytes), int(max_active), int(worker_threads), EVALUATORS[evaluator],
                                      SAMPLERS[sampler], float(scaling), C.byref(h)), "gm_server_create")
         self._h = h
+        if sampler == "texture" and texture is not None:
+            size, tmin, text, texels = texture
+            keep = [np.ascontiguousarray(size, np.int32), np.ascontiguousarray(tmin, np.float64),
+                    np.ascontiguousarray(text, np.float64), np.ascontiguousarray(texels, np.float32)]
+            check(lib().gm_server_set_texture(self._h, *[ptr(a) for a in keep]), "gm_server_set_texture")
         self._passes = {}
         self._nverts = {}
         self._lent = {}

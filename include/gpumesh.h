@@ -315,8 +315,9 @@ int gm_mesh_device_bytes(const gm_mesh* mesh, int64_t* bytes);
  * one launch each over all members.  Members keep their own positions,
  * statistics and pass records; each member's results are bit-identical to
  * gm_mesh_smooth on that mesh alone (same kernels, same per-mesh reduction
- * tree).  Members need the same evaluator, sampler and scaling (the texture
- * sampler is not batched) and must live on one device.  A mesh may belong to
+ * tree).  Members need the same evaluator, sampler, scaling and aspect ratio
+ * (under the texture sampler also the same grid and texels, compared by
+ * content) and must live on one device.  A mesh may belong to
  * several batches, but runs that touch one mesh must not overlap. */
 typedef struct gm_batch gm_batch;
 
@@ -377,13 +378,20 @@ int gm_batch_autotune(gm_batch* batch, int smoother, int precision, int passes, 
  *
  * budget_bytes <= 0: all device memory free at creation less 1 GiB (a larger
  * request is clamped to that).  A job whose footprint exceeds the budget fails
- * with GM_ERR_NOMEM.  Crew: evaluator, sampler (uniform or analytic) and
- * scaling for every job.  Destroying the server cancels unfinished jobs. */
+ * with GM_ERR_NOMEM.  Crew: evaluator, sampler and scaling for every job; the
+ * texture sampler's grid comes from gm_server_set_texture before the first
+ * job.  Destroying the server cancels unfinished jobs. */
 typedef struct gm_server gm_server;
 
 int gm_server_create(int64_t budget_bytes, int max_active, int worker_threads, int evaluator, int sampler,
                      double scaling, gm_server** out);
 int gm_server_destroy(gm_server* server);
+/* The texture grid every job of a GM_TEXTURE server samples (arguments as
+ * gm_mesh_set_crew); only before the first job is submitted.  Replaces
+ * updateCudaSamplerTextures (Samplers/TextureSampler.cpp:75-80) for the
+ * served meshes. */
+int gm_server_set_texture(gm_server* server, const int32_t* tsize, const double* tmin, const double* text,
+                          const float* texels);
 /* Launch shape for `smoother` (as gm_batch_set_shape). */
 int gm_server_set_shape(gm_server* server, int smoother, int patches_per_group, int tile_elems);
 /* Queues a job for the mesh (caller numbering); *job = its id.  borrow = 0

@@ -89,6 +89,39 @@ def test_batch_anisotropic_crew_matches_alone(smoother):
         assert _same(r["inv_sum"][i], ref[i][1]["inv_sum"]), i
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32_polish"])
+@pytest.mark.parametrize("smoother", ["nm", "gd", "spawn", "getme"])
+def test_batch_texture_crew_matches_alone(smoother, precision):
+    """Members sharing one texture grid (metric conformity, the shell field
+    sampled on 12^3 cells) come out as they do alone, from graph replay."""
+    from gpumesh_amd.samplers import shell_texture
+    members = _members()[:4]
+    tex = shell_texture(members[0][0], 12, 3.0)
+    crew = {"evaluator": "metric_conformity", "sampler": "texture", "scaling": 3.0, "texture": tex}
+    ref = _alone(members, smoother, 2, precision, crew)
+    ms = _make(members, crew)
+    b = GpuBatch(ms)
+    r = b.smooth(smoother, 2, precision=precision, kernel="wave", spawn_coeff=COEFF[smoother], graph=True)
+    for i, m in enumerate(ms):
+        assert _same(m.positions(), ref[i][0]), i
+        assert _same(r["min"][i], ref[i][1]["min"]), i
+        assert _same(r["inv_sum"][i], ref[i][1]["inv_sum"]), i
+    b.close()
+
+
+def test_batch_refuses_members_with_different_textures():
+    from gpumesh_amd import GpuMeshError
+    from gpumesh_amd.samplers import shell_texture
+    members = _members()[:2]
+    ms = _make(members)
+    ms[0].set_crew("metric_conformity", "texture", 3.0, shell_texture(members[0][0], 12, 3.0))
+    ms[1].set_crew("metric_conformity", "texture", 3.0, shell_texture(members[0][0], 12, 2.0))
+    with pytest.raises(GpuMeshError):
+        GpuBatch(ms)
+    ms[1].set_crew("metric_conformity", "texture", 3.0, shell_texture(members[0][0], 12, 3.0))
+    GpuBatch(ms).close()
+
+
 def test_graph_replays_match_eager_with_work_between():
     """The captured pass bodies are replayed in several runs, with eager work
     of other handles in between; every replay must reproduce the eager run."""

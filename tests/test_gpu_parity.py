@@ -118,12 +118,7 @@ def test_anisotropic_element_qualities_bitexact(oracle, evaluator, sampler):
 @pytest.mark.parametrize("sampler", ["analytic", "texture", "analytic_reference"])
 def test_anisotropic_smoothing_fp64_bitexact(oracle, smoother, kernel, sampler):
     """The anisotropic hex workload's crew (metric conformity, metric-wise
-    measurer) on a small jittered hex cube."""
-    if sampler == "texture" and (kernel == "plain" or smoother == "getme"):
-        pytest.skip("texture sampler in the one-lane-per-vertex kernels (plain shape, GETMe): runs of NM-plain on "
-                    "a 7^3 and GETMe on a 5^3 hex cube exceeded the GPU runner's time limits while the oracle "
-                    "finishes the same work in 0.1 s; no cause found yet, so the library refuses these "
-                    "combinations (GM_ERR_UNSUPPORTED)")
+    measurer) on a small jittered hex cube; every kernel shape, every sampler."""
     m, om, gm = _crew_pair(oracle, "hex", 4, "metric_conformity", sampler, K=4.0)
     coeff = 0.25 if smoother == "spawn" else 1.0 / 20.0
     ref = om.smooth(smoother, 2, threads=4, spawn_coeff=coeff)
@@ -145,13 +140,19 @@ def test_metric_wise_mean_ratio_smoothing_bitexact(oracle, sampler):
         assert np.array_equal(gm.positions(), om.positions())
 
 
-def test_texture_sampler_refused_outside_wave_kernels(oracle):
-    from gpumesh_amd import GpuMeshError
-    m, om, gm = _crew_pair(oracle, "hex", 3, "metric_conformity", "texture", K=3.0)
-    with pytest.raises(GpuMeshError):
-        gm.smooth("nm", 1, precision="fp64", kernel="plain")
-    with pytest.raises(GpuMeshError):
-        gm.smooth("getme", 1, precision="fp64", kernel="wave")
+@pytest.mark.parametrize("smoother,kernel", [("nm", "plain"), ("getme", "plain"), ("getme", "wave")])
+def test_texture_sampler_larger_hex_cubes(oracle, smoother, kernel):
+    """The 7^3 (NM, plain kernel) and 5^3 (GETMe) hex cubes under metric
+    conformity with the texture sampler, the runs whose kernels never ended
+    while the plain patch functor was kept out of line (its long branches
+    went through its return-address registers)."""
+    n = 7 if smoother == "nm" else 5
+    m, om, gm = _crew_pair(oracle, "hex", n, "metric_conformity", "texture", K=float(n))
+    ref = om.smooth(smoother, 2, threads=4)
+    out = gm.smooth(smoother, 2, precision="fp64", kernel=kernel)
+    assert np.array_equal(gm.positions(), om.positions())
+    assert np.array_equal(out["eval_counts"], ref["eval_counts"].astype(np.int32))
+    assert np.array_equal(out["min"], ref["min"])
 
 
 def test_mixed_element_mesh_bitexact(oracle):

@@ -62,6 +62,36 @@ def test_stream_results_match_alone():
         assert st["peak_active"] >= 2            # jobs overlapped on the device
 
 
+def test_texture_server_matches_alone():
+    """A server created with the texture sampler (metric conformity, the
+    shell field on 12^3 cells): jobs of every smoother, tets and hexes, come
+    out as the same mesh smoothed alone under that crew; jobs before the
+    texture is set are refused."""
+    from gpumesh_amd.samplers import shell_texture
+    specs = [(6, "hex", 0.3, 201, "nm", 2, "fp64"), (7, "tet", 0.3, 202, "gd", 3, "fp32_polish"),
+             (5, "hex", 0.3, 203, "getme", 2, "fp64"), (6, "tet", 0.25, 204, "spawn", 2, "fp32"),
+             (8, "hex", 0.3, 205, "gd", 2, "fp32_polish")]
+    base = cube_mesh(4, "tet", 0.0, 1)
+    tex = shell_texture(base.pos, 12, 3.0)
+    refs = []
+    for n, kind, jitter, seed, smoother, passes, precision in specs:
+        m = cube_mesh(n, kind, jitter, seed)
+        gm = GpuMesh.from_cube(m)
+        gm.set_crew("metric_conformity", "texture", 3.0, tex)
+        r = gm.smooth(smoother, passes, precision=precision, kernel="wave", spawn_coeff=1.0 / 20.0)
+        refs.append((gm.positions()[gm.permutation()], r))
+    with GpuServer(worker_threads=2, evaluator="metric_conformity", sampler="texture", scaling=3.0,
+                   texture=tex) as srv:
+        ids = [srv.submit_mesh(cube_mesh(n, kind, jitter, seed), smoother, passes, precision)
+               for n, kind, jitter, seed, smoother, passes, precision in specs]
+        for k, j in enumerate(ids):
+            assert srv.wait(j, 60000) == "done", k
+            _check(srv.result(j), refs[k], k)
+    with GpuServer(worker_threads=1, evaluator="metric_conformity", sampler="texture", scaling=3.0) as srv:
+        with pytest.raises(GpuMeshError):
+            srv.submit_mesh(cube_mesh(4, "tet", 0.3, 1), "nm", 1, "fp64")
+
+
 def test_admission_by_memory_budget():
     """A budget that holds about two of the jobs: jobs wait for memory, never
     exceed the budget, and still match their solo results; a job larger than

@@ -97,9 +97,10 @@ def test_tangled_anisotropic_fp64_bitexact(oracle, kind, evaluator, smoother, ke
     assert np.array_equal(out["min"], ref["min"])
 
 
-@pytest.mark.parametrize("smoother", ["nm", "gd", "spawn"])
-def test_tangled_texture_fp64_bitexact(oracle, smoother):
+@pytest.mark.parametrize("smoother,kernel", [("nm", "wave"), ("gd", "wave"), ("spawn", "wave"), ("getme", "wave"),
+                                             ("nm", "plain"), ("getme", "plain")])
+def test_tangled_texture_fp64_bitexact(oracle, smoother, kernel):
     """The tangled hex cube under metric conformity with the texture sampler
-    (the shell field sampled at 16^3 cell centres in FP32), in the wave
-    kernels, which are where the library takes that sampler."""
-    test_tangled_anisotropic_fp64_bitexact(oracle, "hex", "metric_conformity", smoother, "wave", sampler="texture")
+    (the shell field sampled at 16^3 cell centres in FP32), every kernel
+    shape."""
+    test_tangled_anisotropic_fp64_bitexact(oracle, "hex", "metric_conformity", smoother, kernel, sampler="texture")
