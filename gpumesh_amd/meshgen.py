@@ -20,7 +20,7 @@ with a portable counter-based generator (SplitMix64 of seed and counter,
 """
 from __future__ import annotations
 
-from dataclasses import dataclass
+from dataclasses import dataclass, field
 
 import numpy as np
 
@@ -47,7 +47,8 @@ def uniform01(seed: int, counters: np.ndarray) -> np.ndarray:
 @dataclass
 class CubeMesh:
     """A generated mesh: positions (V,3) float64, fixed flags (V,) int32,
-    tets (T,4) uint32, hexs (H,8) uint32, and its grid parameters."""
+    tets (T,4) uint32, hexs (H,8) uint32, prisms (P,6) uint32 and its grid
+    parameters."""
     pos: np.ndarray
     fixed: np.ndarray
     tets: np.ndarray
@@ -56,6 +57,7 @@ class CubeMesh:
     kind: str
     seed: int
     jitter: float
+    pris: np.ndarray = field(default_factory=lambda: np.zeros((0, 6), np.uint32))
 
     @property
     def nverts(self) -> int:
@@ -63,7 +65,7 @@ class CubeMesh:
 
     @property
     def nelems(self) -> int:
-        return int(self.tets.shape[0] + self.hexs.shape[0])
+        return int(self.tets.shape[0] + self.pris.shape[0] + self.hexs.shape[0])
 
     @property
     def nmovable(self) -> int:
@@ -145,6 +147,80 @@ def _hex_min_corner_det(pos: np.ndarray, hexs: np.ndarray) -> np.ndarray:
               (e(2, 6), e(5, 6), e(6, 7)), (e(3, 7), e(6, 7), -e(4, 7))]
     dets = [_dot3(a, _cross3(b, c)) for a, b, c in frames]
     return np.min(np.stack(dets, axis=1), axis=1)
+
+
+# The two prisms of a cell: the bottom square split along its (0,0)-(1,1)
+# diagonal into counter-clockwise triangles (viewed from +z), extruded to the
+# top square; cell-corner codes bx + 2 by + 4 bz.  Vertices 0-2 form the
+# bottom triangle and 3-5 the top one above them (MeshPri's order).
+_PRISM_CELL = [(0, 1, 3, 4, 5, 7), (0, 3, 2, 4, 7, 6)]
+
+
+def _pri_min_corner_det(pos: np.ndarray, pris: np.ndarray) -> np.ndarray:
+    """Minimum over the 6 corner frames of MeanRatioEvaluator::priQuality
+    (Evaluators/MeanRatioEvaluator.cpp:70-75) before the right factor
+    Fr_PRI_INV, whose determinant is positive."""
+    p = pos[pris]
+
+    def e(i, j):
+        return p[:, j] - p[:, i]
+
+    frames = [(-e(0, 1), e(2, 0), e(0, 3)), (-e(1, 2), e(0, 1), e(1, 4)), (-e(2, 0), e(1, 2), e(2, 5)),
+              (-e(3, 4), e(5, 3), e(0, 3)), (-e(4, 5), e(3, 4), e(1, 4)), (-e(5, 3), e(4, 5), e(2, 5))]
+    dets = [_dot3(a, _cross3(b, c)) for a, b, c in frames]
+    return np.min(np.stack(dets, axis=1), axis=1)
+
+
+def element_cube_mesh(n: int, layout: str = "prism", jitter: float = 0.3, seed: int = 1,
+                      min_det_fraction: float = 0.05) -> CubeMesh:
+    """An n^3-cell unit-cube mesh with prisms: layout "prism" splits every
+    cell into two prisms; layout "mixed" cuts the cube into three slabs
+    along x, hexahedra for i < n/3, prisms for n/3 <= i < 2n/3 and Kuhn
+    tetrahedra beyond (the slabs meet without matching faces, which the
+    smoothers do not need).  Jitter and its repair as cube_mesh_numpy."""
+    if n < 1:
+        raise ValueError("n must be >= 1")
+    if layout not in ("prism", "mixed"):
+        raise ValueError("layout must be 'prism' or 'mixed'")
+    h = 1.0 / n
+    base, boundary = _grid(n)
+    V = base.shape[0]
+    vid = np.arange(V, dtype=np.uint64)
+    u = np.stack([uniform01(seed, vid * np.uint64(3) + np.uint64(d)) for d in range(3)], axis=1)
+    disp = (2.0 * u - 1.0) * (jitter * h)
+    disp[boundary] = 0.0
+    cells = _cell_base(n)
+    ci = np.arange(n ** 3) % n
+    off = _corner_offsets(n)
+    if layout == "prism":
+        hex_cells = np.zeros(0, np.int64)
+        pri_cells = cells
+        tet_cells = np.zeros(0, np.int64)
+    else:
+        hex_cells = cells[ci < n // 3]
+        pri_cells = cells[(ci >= n // 3) & (ci < (2 * n) // 3)]
+        tet_cells = cells[ci >= (2 * n) // 3]
+    tets = (tet_cells[:, None, None] + off[_kuhn_local()][None, :, :]).reshape(-1, 4)
+    pris = (pri_cells[:, None, None] + off[np.array(_PRISM_CELL)][None, :, :]).reshape(-1, 6)
+    hexs = hex_cells[:, None] + off[[0, 1, 3, 2, 4, 5, 7, 6]][None, :]
+    ref_tet = h ** 3
+    checks = [(tets, _tet_dets, ref_tet), (pris, _pri_min_corner_det, h ** 3), (hexs, _hex_min_corner_det, h ** 3)]
+    scale = np.ones(V)
+    for _ in range(64):
+        pos = base + disp * scale[:, None]
+        bad_verts = [np.unique(el[fn(pos, el) < min_det_fraction * ref].reshape(-1)) for el, fn, ref in checks
+                     if el.shape[0]]
+        bad = np.unique(np.concatenate(bad_verts)) if bad_verts else np.zeros(0, np.int64)
+        if bad.size == 0:
+            break
+        scale[bad] *= 0.5
+    else:  # pragma: no cover - scale -> 0 gives the regular grid
+        raise RuntimeError("jitter repair did not converge")
+    pos = base + disp * scale[:, None]
+    return CubeMesh(pos=np.ascontiguousarray(pos), fixed=boundary.astype(np.int32),
+                    tets=np.ascontiguousarray(tets.astype(np.uint32).reshape(-1, 4)),
+                    hexs=np.ascontiguousarray(hexs.astype(np.uint32).reshape(-1, 8)), n=n, kind=layout,
+                    seed=seed, jitter=jitter, pris=np.ascontiguousarray(pris.astype(np.uint32).reshape(-1, 6)))
 
 
 def cube_mesh(n: int, kind: str = "tet", jitter: float = 0.3, seed: int = 1,

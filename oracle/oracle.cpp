@@ -158,8 +158,9 @@ static inline double dpow(double x, double y) { return g_libm ? gm_pow(x, y) : s
 // ------------------------------------------------------------------------
 enum { TET = 0, PYR = 1, PRI = 2, HEX = 3 };
 
-// MeshTet::edges / MeshHex::edges (Mesh.cpp:21-28, 98-111).
+// MeshTet::edges / MeshPri::edges / MeshHex::edges (Mesh.cpp:21-28, 70-80, 98-111).
 static const int TET_EDGES[6][2] = {{0,1},{0,2},{0,3},{1,2},{1,3},{2,3}};
+static const int PRI_EDGES[9][2] = {{0,1},{0,2},{1,2},{0,3},{1,4},{2,5},{3,4},{3,5},{4,5}};
 static const int HEX_EDGES[12][2] = {{0,1},{0,3},{1,2},{2,3},{0,4},{1,5},{2,6},{3,7},{4,5},{4,7},{5,6},{6,7}};
 
 struct NeigElem { uint32_t id; int type; int vId; };
@@ -253,6 +254,7 @@ struct Mesh
     std::vector<dvec3> verts;
     std::vector<Topo> topos;
     std::vector<std::array<uint32_t, 4>> tets;
+    std::vector<std::array<uint32_t, 6>> pris;
     std::vector<std::array<uint32_t, 8>> hexs;
 
     // NodeGroups (NodeGroups.h:160-192)
@@ -288,6 +290,13 @@ static void compileNeighborhoods(Mesh& m)
         for(int e = 0; e < 6; ++e)
             addEdge(m, m.tets[i][TET_EDGES[e][0]], m.tets[i][TET_EDGES[e][1]]);
     }
+    for(size_t i = 0; i < m.pris.size(); ++i)
+    {
+        for(int v = 0; v < 6; ++v)
+            m.topos[m.pris[i][v]].neighborElems.push_back(NeigElem{(uint32_t)i, PRI, v});
+        for(int e = 0; e < 9; ++e)
+            addEdge(m, m.pris[i][PRI_EDGES[e][0]], m.pris[i][PRI_EDGES[e][1]]);
+    }
     for(size_t i = 0; i < m.hexs.size(); ++i)
     {
         for(int v = 0; v < 8; ++v)
@@ -312,10 +321,13 @@ static void determineTypes(const Mesh& m, std::vector<int>& types)
 {
     size_t V = m.verts.size();
     types.assign(V, -1);
-    std::vector<bool> bTets(m.tets.size(), false), bHexs(m.hexs.size(), false);
+    std::vector<bool> bTets(m.tets.size(), false), bPris(m.pris.size(), false), bHexs(m.hexs.size(), false);
     for(size_t e = 0; e < m.tets.size(); ++e)
         for(int k = 0; k < 4; ++k)
             if(isMovableBound(m.topos[m.tets[e][k]])) { bTets[e] = true; break; }
+    for(size_t e = 0; e < m.pris.size(); ++e)
+        for(int k = 0; k < 6; ++k)
+            if(isMovableBound(m.topos[m.pris[e][k]])) { bPris[e] = true; break; }
     for(size_t e = 0; e < m.hexs.size(); ++e)
         for(int k = 0; k < 8; ++k)
             if(isMovableBound(m.topos[m.hexs[e][k]])) { bHexs[e] = true; break; }
@@ -328,7 +340,8 @@ static void determineTypes(const Mesh& m, std::vector<int>& types)
         {
             bool sub = false;
             for(const NeigElem& ne : t.neighborElems)
-                if((ne.type == TET && bTets[ne.id]) || (ne.type == HEX && bHexs[ne.id])) { sub = true; break; }
+                if((ne.type == TET && bTets[ne.id]) || (ne.type == PRI && bPris[ne.id]) ||
+                   (ne.type == HEX && bHexs[ne.id])) { sub = true; break; }
             types[v] = sub ? SUBSU_TYPE : INTER_TYPE;
         }
     }
@@ -363,8 +376,9 @@ static void determineGroups(const Mesh& m, std::vector<int>& groups)
             for(size_t e = 0; e < topo.neighborElems.size(); ++e)
             {
                 const NeigElem& ne = topo.neighborElems[e];
-                const uint32_t* ev = ne.type == TET ? m.tets[ne.id].data() : m.hexs[ne.id].data();
-                int nv = ne.type == TET ? 4 : 8;
+                const uint32_t* ev = ne.type == TET ? m.tets[ne.id].data()
+                                   : ne.type == PRI ? m.pris[ne.id].data() : m.hexs[ne.id].data();
+                int nv = ne.type == TET ? 4 : ne.type == PRI ? 6 : 8;
                 for(int n = 0; n < nv; ++n)
                 {
                     int& g = groups[ev[n]];
@@ -420,6 +434,7 @@ static void clusterNodes(Mesh& m, std::vector<int>& types, std::vector<int>& gro
 {
     size_t V = m.verts.size();
     for(auto& t : m.tets) for(int k = 0; k < 4; ++k) t[k] = (uint32_t)positions[t[k]];
+    for(auto& p : m.pris) for(int k = 0; k < 6; ++k) p[k] = (uint32_t)positions[p[k]];
     for(auto& h : m.hexs) for(int k = 0; k < 8; ++k) h[k] = (uint32_t)positions[h[k]];
     for(Topo& t : m.topos) for(uint32_t& n : t.neighborVerts) n = (uint32_t)positions[n];
 
@@ -635,6 +650,33 @@ static double tetVolume(const Crew& c, const dvec3 vp[])
     return c.scaling3 * det / 6.0;
 }
 
+// MetricFreeMeasurer::priVolume (:58-75) / MetricWise (:141-158).
+static double priVolume(const Crew& c, const dvec3 vp[])
+{
+    dvec3 e02, e12, e32, e42, e52;
+    if(c.measurer == MEAS_METRIC_WISE)
+    {
+        e02 = riemannianSegment(c, vp[0], vp[2]);
+        e12 = riemannianSegment(c, vp[1], vp[2]);
+        e32 = riemannianSegment(c, vp[3], vp[2]);
+        e42 = riemannianSegment(c, vp[4], vp[2]);
+        e52 = riemannianSegment(c, vp[5], vp[2]);
+    }
+    else
+    {
+        e02 = vp[2] - vp[0];
+        e12 = vp[2] - vp[1];
+        e32 = vp[2] - vp[3];
+        e42 = vp[2] - vp[4];
+        e52 = vp[2] - vp[5];
+    }
+    double detSum = 0.0;
+    detSum += determinant(dmat3(e32, e52, e42));
+    detSum += determinant(dmat3(e02, e32, e42));
+    detSum += determinant(dmat3(e12, e02, e42));
+    return c.measurer == MEAS_METRIC_WISE ? detSum / 6.0 : c.scaling3 * detSum / 6.0;
+}
+
 // MetricFreeMeasurer::hexVolume (:77-105) / MetricWise (:160-188).
 static double hexVolume(const Crew& c, const dvec3 vp[])
 {
@@ -675,6 +717,11 @@ static const dmat3 Fr_TET_INV(
     dvec3(1, 0, 0),
     dvec3(-0.5773502691896257645091, 1.154700538379251529018, 0),
     dvec3(-0.4082482904638630163662, -0.4082482904638630163662, 1.224744871391589049099));
+// AbstractEvaluator.cpp:32-35
+static const dmat3 Fr_PRI_INV(
+    dvec3(1.0, 0.0, 0.0),
+    dvec3(-0.5773502691896257645091, 1.154700538379251529018, 0.0),
+    dvec3(0.0, 0.0, 1.0));
 
 // MeanRatioEvaluator::cornerQuality (MeanRatioEvaluator.cpp:22-32).
 static double cornerQuality(const dmat3& Fk)
@@ -740,6 +787,55 @@ static double hexMeanOrMin(const double q[8])
     return 8.0 / (1 / q[0] + 1 / q[1] + 1 / q[2] + 1 / q[3] + 1 / q[4] + 1 / q[5] + 1 / q[6] + 1 / q[7]);
 }
 
+// MeanRatioEvaluator.cpp:85-103 / MetricConformityEvaluator.cpp:133-151
+static double priMeanOrMin(const double q[6])
+{
+    if(q[0] <= 0.0 || q[1] <= 0.0 || q[2] <= 0.0 || q[3] <= 0.0 || q[4] <= 0.0 || q[5] <= 0.0)
+    {
+        double mn = q[0];
+        for(int k = 1; k < 6; ++k) if(q[k] < mn) mn = q[k];
+        return mn;
+    }
+    return 6.0 / (1 / q[0] + 1 / q[1] + 1 / q[2] + 1 / q[3] + 1 / q[4] + 1 / q[5]);
+}
+
+// MeanRatioEvaluator::priQuality (:52-104), MetricConformityEvaluator::priQuality (:93-152).
+static double priQuality(const Crew& c, const dvec3 vp[])
+{
+    double q[6];
+    dvec3 e03, e14, e25, e01, e12, e20, e34, e45, e53;
+    if(c.evaluator == EVAL_METRIC_CONFORMITY)
+    {
+        e03 = vp[3] - vp[0]; e14 = vp[4] - vp[1]; e25 = vp[5] - vp[2];
+        e01 = vp[1] - vp[0]; e12 = vp[2] - vp[1]; e20 = vp[0] - vp[2];
+        e34 = vp[4] - vp[3]; e45 = vp[5] - vp[4]; e53 = vp[3] - vp[5];
+    }
+    else
+    {
+        e03 = riemannianSegment(c, vp[0], vp[3]);
+        e14 = riemannianSegment(c, vp[1], vp[4]);
+        e25 = riemannianSegment(c, vp[2], vp[5]);
+        e01 = riemannianSegment(c, vp[0], vp[1]);
+        e12 = riemannianSegment(c, vp[1], vp[2]);
+        e20 = riemannianSegment(c, vp[2], vp[0]);
+        e34 = riemannianSegment(c, vp[3], vp[4]);
+        e45 = riemannianSegment(c, vp[4], vp[5]);
+        e53 = riemannianSegment(c, vp[5], vp[3]);
+    }
+    dmat3 Fk[6] = {dmat3(-e01, e20, e03) * Fr_PRI_INV, dmat3(-e12, e01, e14) * Fr_PRI_INV,
+                   dmat3(-e20, e12, e25) * Fr_PRI_INV, dmat3(-e34, e53, e03) * Fr_PRI_INV,
+                   dmat3(-e45, e34, e14) * Fr_PRI_INV, dmat3(-e53, e45, e25) * Fr_PRI_INV};
+    if(c.evaluator == EVAL_METRIC_CONFORMITY)
+    {
+        static const int T[6][4] = {{0,1,2,3},{0,1,2,4},{0,1,2,5},{0,3,4,5},{1,3,4,5},{2,3,4,5}};
+        for(int k = 0; k < 6; ++k)
+            q[k] = metricConformity(Fk[k], specifiedMetric(c, vp[T[k][0]], vp[T[k][1]], vp[T[k][2]], vp[T[k][3]]));
+    }
+    else
+        for(int k = 0; k < 6; ++k) q[k] = cornerQuality(Fk[k]);
+    return priMeanOrMin(q);
+}
+
 // MeanRatioEvaluator::hexQuality (:106-167), MetricConformityEvaluator::hexQuality (:154-224).
 static double hexQuality(const Crew& c, const dvec3 vp[])
 {
@@ -780,6 +876,13 @@ static double tetQualityMesh(const Mesh& m, uint32_t e)
     dvec3 vp[4];
     for(int k = 0; k < 4; ++k) vp[k] = m.verts[m.tets[e][k]];
     return tetQuality(m.crew, vp);
+}
+
+static double priQualityMesh(const Mesh& m, uint32_t e)
+{
+    dvec3 vp[6];
+    for(int k = 0; k < 6; ++k) vp[k] = m.verts[m.pris[e][k]];
+    return priQuality(m.crew, vp);
 }
 
 static double hexQualityMesh(const Mesh& m, uint32_t e)
@@ -826,6 +929,7 @@ static double patchQualityUntraced(const Mesh& m, uint32_t vId)
     for(const NeigElem& ne : topo.neighborElems)
     {
         if(ne.type == TET) accumulatePatchQuality(pq, pw, tetQualityMesh(m, ne.id));
+        else if(ne.type == PRI) accumulatePatchQuality(pq, pw, priQualityMesh(m, ne.id));
         else if(ne.type == HEX) accumulatePatchQuality(pq, pw, hexQualityMesh(m, ne.id));
     }
     return finalizePatchQuality(pq, pw);
@@ -875,21 +979,24 @@ struct Histogram
 static Histogram evaluateMesh(const Mesh& m, int chunks)
 {
     Histogram out;
-    int tetCount = (int)m.tets.size(), hexCount = (int)m.hexs.size();
+    int tetCount = (int)m.tets.size(), priCount = (int)m.pris.size(), hexCount = (int)m.hexs.size();
     if(chunks <= 1)
     {
         for(int i = 0; i < tetCount; ++i) out.add(tetQualityMesh(m, i));
+        for(int i = 0; i < priCount; ++i) out.add(priQualityMesh(m, i));
         for(int i = 0; i < hexCount; ++i) out.add(hexQualityMesh(m, i));
         return out;
     }
-    if(tetCount + hexCount == 0) return out;
+    if(tetCount + priCount + hexCount == 0) return out;
     std::vector<Histogram> parts(chunks);
     std::vector<std::thread> th;
     for(int t = 0; t < chunks; ++t)
         th.emplace_back([&, t]() {
             int tb = (int)(((int64_t)tetCount * t) / chunks), te = (int)(((int64_t)tetCount * (t + 1)) / chunks);
+            int pb = (int)(((int64_t)priCount * t) / chunks), pe = (int)(((int64_t)priCount * (t + 1)) / chunks);
             int hb = (int)(((int64_t)hexCount * t) / chunks), he = (int)(((int64_t)hexCount * (t + 1)) / chunks);
             for(int i = tb; i < te; ++i) parts[t].add(tetQualityMesh(m, i));
+            for(int i = pb; i < pe; ++i) parts[t].add(priQualityMesh(m, i));
             for(int i = hb; i < he; ++i) parts[t].add(hexQualityMesh(m, i));
         });
     for(auto& t : th) t.join();
@@ -1201,16 +1308,58 @@ static void getmeHex(const Mesh& m, uint32_t e, ElemOut& o)
     for(int k = 0; k < 8; ++k) o.vpp[k] = vpp[k];
 }
 
+// GetmeSmoother::smoothPris (:131-243), before snapping.
+static void getmePriTransform(const Crew& c, const dvec3 vp[6], dvec3 vpp[6])
+{
+    const double lambda = 0.78;
+    dvec3 aux[5] = {
+        (vp[0] + vp[1] + vp[2]) / 3.0, (vp[0] + vp[1] + vp[4] + vp[3]) / 4.0,
+        (vp[1] + vp[2] + vp[5] + vp[4]) / 4.0, (vp[2] + vp[0] + vp[3] + vp[5]) / 4.0,
+        (vp[3] + vp[4] + vp[5]) / 3.0};
+    dvec3 n[6] = {
+        cross(aux[1] - aux[0], aux[3] - aux[0]), cross(aux[2] - aux[0], aux[1] - aux[0]),
+        cross(aux[3] - aux[0], aux[2] - aux[0]), cross(aux[3] - aux[4], aux[1] - aux[4]),
+        cross(aux[1] - aux[4], aux[2] - aux[4]), cross(aux[2] - aux[4], aux[3] - aux[4])};
+    const double t = (4.0 / 5.0) * (1.0 - dpow(4.0 / 39.0, 0.25) * lambda);
+    const double it = 1.0 - t;
+    dvec3 bases[6] = {
+        it * aux[0] + t * (aux[3] + aux[1]) / 2.0, it * aux[0] + t * (aux[1] + aux[2]) / 2.0,
+        it * aux[0] + t * (aux[2] + aux[3]) / 2.0, it * aux[4] + t * (aux[3] + aux[1]) / 2.0,
+        it * aux[4] + t * (aux[1] + aux[2]) / 2.0, it * aux[4] + t * (aux[2] + aux[3]) / 2.0};
+    for(int k = 0; k < 6; ++k) vpp[k] = bases[k] + n[k] * (lambda / std::sqrt(length(n[k])));
+    double volume = priVolume(c, vp);
+    double volumePrime = priVolume(c, vpp);
+    double absVolumeRation = std::abs(volume / volumePrime);
+    double volumeVar = pow13(absVolumeRation);
+    dvec3 center = (1.0 / 6.0) * (vp[0] + vp[1] + vp[2] + vp[3] + vp[4] + vp[5]);
+    for(int k = 0; k < 6; ++k) vpp[k] = center + volumeVar * (vpp[k] - center);
+}
+
+static void getmePri(const Mesh& m, uint32_t e, ElemOut& o)
+{
+    const Crew& c = m.crew;
+    uint32_t vi[6]; dvec3 vp[6], vpp[6];
+    for(int k = 0; k < 6; ++k) { vi[k] = m.pris[e][k]; vp[k] = m.verts[vi[k]]; }
+    getmePriTransform(c, vp, vpp);
+    for(int k = 0; k < 6; ++k) if(isConstrained(m.topos[vi[k]])) vpp[k] = snap(m.topos[vi[k]], vpp[k]);
+    double quality = priQuality(c, vp);
+    double qualityPrime = priQuality(c, vpp);
+    o.weight = qualityPrime / (1.0 + quality);
+    for(int k = 0; k < 6; ++k) o.vpp[k] = vpp[k];
+}
+
+// Element-step outputs per element kind (indexed by TET, PYR, PRI, HEX).
+struct GetmeOut { std::vector<ElemOut> kind[4]; };
+
 // AbstractElementWiseSmoother::updateVertexPositions for one vertex (:406-444),
 // with NotThreadSafeVertexAccum (VertexAccum.cpp:28-49) summed by pull.
-static void getmeUpdateVertex(Mesh& m, uint32_t vId, const std::vector<ElemOut>& tetOut,
-                              const std::vector<ElemOut>& hexOut, int64_t* cnt)
+static void getmeUpdateVertex(Mesh& m, uint32_t vId, const GetmeOut& out, int64_t* cnt)
 {
     dvec3 posAccum(0.0);
     double weightAccum = 0.0;
     for(const NeigElem& ne : m.topos[vId].neighborElems)
     {
-        const ElemOut& o = (ne.type == TET) ? tetOut[ne.id] : hexOut[ne.id];
+        const ElemOut& o = out.kind[ne.type][ne.id];
         posAccum = posAccum + o.vpp[ne.vId] * o.weight;
         weightAccum += o.weight;
     }
@@ -1237,26 +1386,15 @@ using namespace orc;
 // ------------------------------------------------------------------------
 extern "C" {
 
+void* orc_mesh_create_full(int64_t nv, const double* pos, const int32_t* fixed, const int32_t* cons,
+                           int64_t ncons, const int32_t* kinds, const double* params, int64_t ntet,
+                           const uint32_t* tets, int64_t npri, const uint32_t* pris, int64_t nhex,
+                           const uint32_t* hexs);
+
 void* orc_mesh_create(int64_t nv, const double* pos, const int32_t* fixed,
                       int64_t ntet, const uint32_t* tets, int64_t nhex, const uint32_t* hexs)
 {
-    Mesh* m = new Mesh();
-    m->verts.resize(nv);
-    m->topos.resize(nv);
-    for(int64_t v = 0; v < nv; ++v)
-    {
-        m->verts[v] = dvec3(pos[3 * v], pos[3 * v + 1], pos[3 * v + 2]);
-        m->topos[v].constraint = fixed[v] ? -1 : 0;
-        m->topos[v].fixedPos = m->verts[v];
-    }
-    m->tets.resize(ntet);
-    for(int64_t e = 0; e < ntet; ++e) for(int k = 0; k < 4; ++k) m->tets[e][k] = tets[4 * e + k];
-    m->hexs.resize(nhex);
-    for(int64_t e = 0; e < nhex; ++e) for(int k = 0; k < 8; ++k) m->hexs[e][k] = hexs[8 * e + k];
-    compileNeighborhoods(*m);
-    buildNodeGroups(*m);
-    samplerRotation(m->crew);
-    return m;
+    return orc_mesh_create_full(nv, pos, fixed, nullptr, 0, nullptr, nullptr, ntet, tets, 0, nullptr, nhex, hexs);
 }
 
 // A mesh with sliding constraints: cons[v] = index into the table or -1;
@@ -1289,6 +1427,16 @@ void orc_constraint_snap(int kind, const double* a, const double* b, const doubl
 void* orc_mesh_create_constrained(int64_t nv, const double* pos, const int32_t* fixed, const int32_t* cons,
                                   int64_t ncons, const int32_t* kinds, const double* params, int64_t ntet,
                                   const uint32_t* tets, int64_t nhex, const uint32_t* hexs)
+{
+    return orc_mesh_create_full(nv, pos, fixed, cons, ncons, kinds, params, ntet, tets, 0, nullptr, nhex, hexs);
+}
+
+// A mesh of tetrahedra, prisms (6 vertices each, MeshPri) and hexahedra,
+// with optional sliding constraints as orc_mesh_create_constrained.
+void* orc_mesh_create_full(int64_t nv, const double* pos, const int32_t* fixed, const int32_t* cons,
+                           int64_t ncons, const int32_t* kinds, const double* params, int64_t ntet,
+                           const uint32_t* tets, int64_t npri, const uint32_t* pris, int64_t nhex,
+                           const uint32_t* hexs)
 {
     Mesh* m = new Mesh();
     m->slides.resize(ncons);
@@ -1326,6 +1474,8 @@ void* orc_mesh_create_constrained(int64_t nv, const double* pos, const int32_t* 
     }
     m->tets.resize(ntet);
     for(int64_t e = 0; e < ntet; ++e) for(int k = 0; k < 4; ++k) m->tets[e][k] = tets[4 * e + k];
+    m->pris.resize(npri);
+    for(int64_t e = 0; e < npri; ++e) for(int k = 0; k < 6; ++k) m->pris[e][k] = pris[6 * e + k];
     m->hexs.resize(nhex);
     for(int64_t e = 0; e < nhex; ++e) for(int k = 0; k < 8; ++k) m->hexs[e][k] = hexs[8 * e + k];
     compileNeighborhoods(*m);
@@ -1363,6 +1513,14 @@ void orc_mesh_elements(void* h, uint32_t* tets, uint32_t* hexs)
     Mesh* m = (Mesh*)h;
     for(size_t e = 0; e < m->tets.size(); ++e) for(int k = 0; k < 4; ++k) tets[4*e+k] = m->tets[e][k];
     for(size_t e = 0; e < m->hexs.size(); ++e) for(int k = 0; k < 8; ++k) hexs[8*e+k] = m->hexs[e][k];
+}
+
+int64_t orc_mesh_npris(void* h) { return (int64_t)((Mesh*)h)->pris.size(); }
+
+void orc_mesh_prisms(void* h, uint32_t* pris)
+{
+    Mesh* m = (Mesh*)h;
+    for(size_t e = 0; e < m->pris.size(); ++e) for(int k = 0; k < 6; ++k) pris[6*e+k] = m->pris[e][k];
 }
 
 // Sizes of the topology arrays: [sum neighborElems, sum neighborVerts].
@@ -1458,12 +1616,28 @@ double orc_hex_quality(void* h, const double* p)
     return hexQuality(((Mesh*)h)->crew, vp);
 }
 
-// Per-element qualities, tets then hexes.
+double orc_pri_quality(void* h, const double* p)
+{
+    dvec3 vp[6];
+    for(int k = 0; k < 6; ++k) vp[k] = dvec3(p[3*k], p[3*k+1], p[3*k+2]);
+    return priQuality(((Mesh*)h)->crew, vp);
+}
+
+double orc_pri_volume(void* h, const double* p)
+{
+    dvec3 vp[6];
+    for(int k = 0; k < 6; ++k) vp[k] = dvec3(p[3*k], p[3*k+1], p[3*k+2]);
+    return priVolume(((Mesh*)h)->crew, vp);
+}
+
+// Per-element qualities, tets, then prisms, then hexes (the order of
+// evaluateMeshQualitySerial).
 void orc_element_qualities(void* h, double* out)
 {
     Mesh* m = (Mesh*)h;
     size_t i = 0;
     for(size_t e = 0; e < m->tets.size(); ++e) out[i++] = tetQualityMesh(*m, (uint32_t)e);
+    for(size_t e = 0; e < m->pris.size(); ++e) out[i++] = priQualityMesh(*m, (uint32_t)e);
     for(size_t e = 0; e < m->hexs.size(); ++e) out[i++] = hexQualityMesh(*m, (uint32_t)e);
 }
 
@@ -1531,16 +1705,19 @@ int64_t orc_trace_vertex(void* h, int smoother, double spawnCoeff, uint32_t vId,
 }
 
 // GETMe element step on free-standing vertices (no snapping): kind 0 = tet
-// (4 vertices), 1 = hex (8); vpp = transformed positions, *weight = q'/(1+q).
+// (4 vertices), 1 = hex (8), 2 = prism (6); vpp = transformed positions,
+// *weight = q'/(1+q).
 void orc_getme_element(void* h, int kind, const double* p, double* vppOut, double* weight)
 {
     const Crew& c = ((Mesh*)h)->crew;
-    const int n = kind == 0 ? 4 : 8;
+    const int n = kind == 0 ? 4 : kind == 2 ? 6 : 8;
     dvec3 vp[8], vpp[8];
     for(int k = 0; k < n; ++k) vp[k] = dvec3(p[3*k], p[3*k+1], p[3*k+2]);
-    if(kind == 0) getmeTetTransform(c, vp, vpp); else getmeHexTransform(c, vp, vpp);
-    const double q = kind == 0 ? tetQuality(c, vp) : hexQuality(c, vp);
-    const double qp = kind == 0 ? tetQuality(c, vpp) : hexQuality(c, vpp);
+    if(kind == 0) getmeTetTransform(c, vp, vpp);
+    else if(kind == 2) getmePriTransform(c, vp, vpp);
+    else getmeHexTransform(c, vp, vpp);
+    const double q = kind == 0 ? tetQuality(c, vp) : kind == 2 ? priQuality(c, vp) : hexQuality(c, vp);
+    const double qp = kind == 0 ? tetQuality(c, vpp) : kind == 2 ? priQuality(c, vpp) : hexQuality(c, vpp);
     *weight = qp / (1.0 + q);
     for(int k = 0; k < n; ++k) { vppOut[3*k] = vpp[k].x; vppOut[3*k+1] = vpp[k].y; vppOut[3*k+2] = vpp[k].z; }
 }
@@ -1623,7 +1800,7 @@ int orc_smooth_schedule(void* h, int smoother, int passes, int autopilot, double
         lastMean = hh.harmonicMean();
     };
 
-    std::vector<ElemOut> tetOut, hexOut;
+    GetmeOut gout;
     int swept = 0;
     for(int pass = 0; pass <= passes; ++pass)
     {
@@ -1632,20 +1809,22 @@ int orc_smooth_schedule(void* h, int smoother, int passes, int autopilot, double
         ++swept;
         if(smoother == 3)
         {
-            tetOut.resize(m.tets.size());
-            hexOut.resize(m.hexs.size());
-            parallelFor(m.tets.size(), threads, [&](size_t e) { getmeTet(m, (uint32_t)e, tetOut[e]); });
-            parallelFor(m.hexs.size(), threads, [&](size_t e) { getmeHex(m, (uint32_t)e, hexOut[e]); });
+            gout.kind[TET].resize(m.tets.size());
+            gout.kind[PRI].resize(m.pris.size());
+            gout.kind[HEX].resize(m.hexs.size());
+            parallelFor(m.tets.size(), threads, [&](size_t e) { getmeTet(m, (uint32_t)e, gout.kind[TET][e]); });
+            parallelFor(m.pris.size(), threads, [&](size_t e) { getmePri(m, (uint32_t)e, gout.kind[PRI][e]); });
+            parallelFor(m.hexs.size(), threads, [&](size_t e) { getmeHex(m, (uint32_t)e, gout.kind[HEX][e]); });
             if(order == 0 || hasBoundaryTypes)
             {
-                for(uint32_t v : m.serialGroup) getmeUpdateVertex(m, v, tetOut, hexOut, &counts[v]);
+                for(uint32_t v : m.serialGroup) getmeUpdateVertex(m, v, gout, &counts[v]);
             }
             else
             {
                 for(const ParallelGroup& g : m.parallelGroups)
                     parallelFor(g.undispatchedNodes.size(), threads, [&](size_t i) {
                         uint32_t v = g.undispatchedNodes[i];
-                        getmeUpdateVertex(m, v, tetOut, hexOut, &counts[v]);
+                        getmeUpdateVertex(m, v, gout, &counts[v]);
                     });
             }
         }

@@ -43,6 +43,12 @@ def lib():
     L.orc_mesh_create_constrained.restype = vp
     L.orc_mesh_create_constrained.argtypes = [i64, _P(np.float64), _P(np.int32), _P(np.int32), i64, _P(np.int32),
                                               _P(np.float64), i64, _P(np.uint32), i64, _P(np.uint32)]
+    L.orc_mesh_create_full.restype = vp
+    L.orc_mesh_create_full.argtypes = [i64, _P(np.float64), _P(np.int32), vp, i64, vp, vp, i64, _P(np.uint32), i64,
+                                       _P(np.uint32), i64, _P(np.uint32)]
+    L.orc_mesh_npris.argtypes = [vp]
+    L.orc_mesh_npris.restype = i64
+    L.orc_mesh_prisms.argtypes = [vp, _P(np.uint32)]
     L.orc_constraint_snap.argtypes = [C.c_int, _P(np.float64), _P(np.float64), _P(np.float64), _P(np.float64)]
     L.orc_set_libm.argtypes = [C.c_int]
     L.orc_set_libm.restype = C.c_int
@@ -60,7 +66,8 @@ def lib():
     L.orc_set_crew.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_double, vp, vp, vp, vp]
     L.orc_set_aspect.argtypes = [vp, C.c_double]
     L.orc_evaluate.argtypes = [vp, C.c_int, _P(np.float64), _P(np.int32)]
-    for name in ("orc_tet_quality", "orc_hex_quality", "orc_tet_volume", "orc_hex_volume"):
+    for name in ("orc_tet_quality", "orc_hex_quality", "orc_tet_volume", "orc_hex_volume", "orc_pri_quality",
+                 "orc_pri_volume"):
         getattr(L, name).argtypes = [vp, _P(np.float64)]
         getattr(L, name).restype = C.c_double
     L.orc_element_qualities.argtypes = [vp, _P(np.float64)]
@@ -114,18 +121,37 @@ def set_libm(mode: int) -> int:
 class OracleMesh:
     """An oracle mesh: topology + node groups built (and renumbered) at creation."""
 
-    def __init__(self, pos, fixed, tets, hexs, constraint=None, constraints=None):
+    def __init__(self, pos, fixed, tets, hexs, constraint=None, constraints=None, pris=None):
         """constraint[v]: index into `constraints` or -1; constraints: rows
-        (kind, a, b) with kind 1 = plane (a point, b normal), 2 = segment."""
+        (kind, a, b) with kind 1 = plane (a point, b normal), 2 = segment.
+        pris: prisms, 6 vertices each (MeshPri)."""
         L = lib()
         pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
         fixed = np.ascontiguousarray(fixed, dtype=np.int32).reshape(-1)
         tets = np.ascontiguousarray(tets, dtype=np.uint32).reshape(-1, 4)
         hexs = np.ascontiguousarray(hexs, dtype=np.uint32).reshape(-1, 8)
+        pris = np.ascontiguousarray(np.zeros((0, 6)) if pris is None else pris, dtype=np.uint32).reshape(-1, 6)
         self.ntet = tets.shape[0]
         self.nhex = hexs.shape[0]
+        self.npri = pris.shape[0]
         self.nverts = pos.shape[0]
-        if constraint is None:
+        if self.npri:
+            one = np.zeros(1, np.uint32)
+            keep = []
+            cptr = kptr = pptr = None
+            ncons = 0
+            if constraint is not None:
+                cons = np.ascontiguousarray(constraint, np.int32).reshape(-1)
+                kinds = np.ascontiguousarray([int(c[0]) for c in constraints] or [0], np.int32)
+                params = np.ascontiguousarray([list(c[1]) + list(c[2]) for c in constraints] or [[0.0] * 6],
+                                              np.float64).reshape(-1)
+                keep = [cons, kinds, params]
+                cptr, kptr, pptr = (a.ctypes.data for a in keep)
+                ncons = len(constraints)
+            self._h = L.orc_mesh_create_full(self.nverts, pos.reshape(-1), fixed, cptr, ncons, kptr, pptr, self.ntet,
+                                             tets.reshape(-1) if self.ntet else one, self.npri, pris.reshape(-1),
+                                             self.nhex, hexs.reshape(-1) if self.nhex else one)
+        elif constraint is None:
             self._h = L.orc_mesh_create(self.nverts, pos.reshape(-1), fixed, self.ntet, tets.reshape(-1),
                                         self.nhex, hexs.reshape(-1))
         else:
@@ -140,7 +166,7 @@ class OracleMesh:
 
     @classmethod
     def from_cube(cls, m):
-        return cls(m.pos, m.fixed, m.tets, m.hexs)
+        return cls(m.pos, m.fixed, m.tets, m.hexs, pris=getattr(m, "pris", None))
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -216,10 +242,17 @@ class OracleMesh:
         return {"min": float(out[0]), "inv_sum": float(out[1]), "n": int(out[2]),
                 "mean": float(out[2] / out[1]), "hist": hist}
 
+    def prisms(self) -> np.ndarray:
+        p = np.empty(max(6 * self.npri, 1), np.uint32)
+        lib().orc_mesh_prisms(self._h, p)
+        return p[: 6 * self.npri].reshape(-1, 6)
+
     def element_qualities(self) -> np.ndarray:
-        out = np.empty(max(self.ntet + self.nhex, 1), np.float64)
+        """Tets, then prisms, then hexes."""
+        n = self.ntet + self.npri + self.nhex
+        out = np.empty(max(n, 1), np.float64)
         lib().orc_element_qualities(self._h, out)
-        return out[: self.ntet + self.nhex]
+        return out[:n]
 
     def tet_quality(self, vp) -> float:
         return lib().orc_tet_quality(self._h, np.ascontiguousarray(vp, np.float64).reshape(-1))
@@ -232,6 +265,12 @@ class OracleMesh:
 
     def hex_volume(self, vp) -> float:
         return lib().orc_hex_volume(self._h, np.ascontiguousarray(vp, np.float64).reshape(-1))
+
+    def pri_quality(self, vp) -> float:
+        return lib().orc_pri_quality(self._h, np.ascontiguousarray(vp, np.float64).reshape(-1))
+
+    def pri_volume(self, vp) -> float:
+        return lib().orc_pri_volume(self._h, np.ascontiguousarray(vp, np.float64).reshape(-1))
 
     def patch_quality(self, v: int) -> float:
         return lib().orc_patch_quality(self._h, int(v))
@@ -261,7 +300,8 @@ class OracleMesh:
         vp = np.ascontiguousarray(vp, np.float64).reshape(-1, 3)
         out = np.empty(vp.size, np.float64)
         w = np.empty(1, np.float64)
-        lib().orc_getme_element(self._h, 0 if vp.shape[0] == 4 else 1, vp.reshape(-1), out, w)
+        kind = {4: 0, 6: 2, 8: 1}[vp.shape[0]]
+        lib().orc_getme_element(self._h, kind, vp.reshape(-1), out, w)
         return out.reshape(-1, 3), float(w[0])
 
     def specified_metric(self, vp) -> np.ndarray:
