@@ -30,7 +30,7 @@ EXPORTS = [
     "gm_topology_cache_stats", "gm_math_probe", "gm_mesh_smooth_schedule", "gm_batch_smooth_schedule",
     "gm_job_size", "gm_job_encode", "gm_job_info", "gm_job_decode", "gm_server_submit_job",
     "gm_set_polish_passes", "gm_evaluate_noalloc", "gm_mesh_create_grouped", "gm_mesh_set_aspect_ratio",
-    "gm_mesh_create_desc", "gm_topology_prepare", "gm_throughput_probe", "gm_server_set_texture",
+    "gm_mesh_create_desc", "gm_topology_prepare", "gm_throughput_probe", "gm_server_set_texture", "gm_mesh_prisms",
 ]
 
 
@@ -44,7 +44,8 @@ class MeshDesc(C.Structure):
     _fields_ = [("nverts", C.c_int64), ("pos", C.c_void_p), ("fixed", C.c_void_p), ("ntets", C.c_int64),
                 ("tets", C.c_void_p), ("nhexs", C.c_int64), ("hexs", C.c_void_p), ("constraint", C.c_void_p),
                 ("nconstraints", C.c_int64), ("constraints", C.c_void_p), ("ngroups", C.c_int64),
-                ("group_off", C.c_void_p), ("group_nodes", C.c_void_p), ("device_copy", C.c_int)]
+                ("group_off", C.c_void_p), ("group_nodes", C.c_void_p), ("device_copy", C.c_int),
+                ("npris", C.c_int64), ("pris", C.c_void_p)]
 
 
 class Schedule(C.Structure):
@@ -112,6 +113,7 @@ def lib():
     L.gm_server_destroy.argtypes = [vp]
     L.gm_server_set_shape.argtypes = [vp, C.c_int, C.c_int, C.c_int]
     L.gm_server_set_texture.argtypes = [vp, vp, vp, vp, vp]
+    L.gm_mesh_prisms.argtypes = [vp, vp, vp]
     L.gm_server_submit.argtypes = [vp, i64, vp, vp, i64, vp, i64, vp, C.c_int, C.c_int, C.c_int, C.c_int,
                                    C.POINTER(i64)]
     L.gm_server_wait.argtypes = [vp, i64, dbl, C.POINTER(C.c_int)]

@@ -193,6 +193,7 @@ int gm_mesh_create_desc(const gm_mesh_desc* d, gm_mesh** out)
     if(!d) return fail(GM_ERR_ARG, "null description");
     if(d->nconstraints < 0 || (d->nconstraints > 0 && !d->constraints))
         return fail(GM_ERR_ARG, "bad constraint table");
+    if(d->npris < 0 || (d->npris > 0 && !d->pris)) return fail(GM_ERR_ARG, "bad prism array");
     // The table in the device layout: [kind, a, unit normal or direction, b,
     // length, 0]; the normalisations are the constraints' constructors'
     // (FaceConstraint.cpp:173-178, EdgeConstraint.cpp:177-183).
@@ -225,9 +226,9 @@ int gm_mesh_create_desc(const gm_mesh_desc* d, gm_mesh** out)
     const std::vector<double>* tp = d->nconstraints > 0 ? &table : nullptr;
     int rc = d->ngroups < 0
                  ? gmr::makeMesh(d->nverts, d->pos, d->fixed, d->ntets, d->tets, d->nhexs, d->hexs, &m,
-                                 d->constraint, tp)
+                                 d->constraint, tp, d->npris, d->pris)
                  : gmr::makeMeshGrouped(d->nverts, d->pos, d->fixed, d->ntets, d->tets, d->nhexs, d->hexs, d->ngroups,
-                                        d->group_off, d->group_nodes, &m, d->constraint, tp);
+                                        d->group_off, d->group_nodes, &m, d->constraint, tp, d->npris, d->pris);
     if(rc) return rc;
     if(d->device_copy)
     {
@@ -341,7 +342,18 @@ int gm_mesh_elements(const gm_mesh* m, uint32_t* tets, uint32_t* hexs)
 {
     if(!m) return fail(GM_ERR_ARG, "null mesh");
     if(tets) std::memcpy(tets, m->topo->tets.data(), m->topo->tets.size() * 4);
-    if(hexs) std::memcpy(hexs, m->topo->hexs.data(), m->topo->hexs.size() * 4);
+    if(hexs) std::memcpy(hexs, m->topo->hexs.data(), 8 * (size_t)m->topo->nhexs * 4);
+    return GM_OK;
+}
+
+int gm_mesh_prisms(const gm_mesh* m, int64_t* npris, uint32_t* pris)
+{
+    if(!m) return fail(GM_ERR_ARG, "null mesh");
+    const gmt::Topology& t = *m->topo;
+    if(npris) *npris = t.npris;
+    if(pris)
+        for(int64_t p = 0; p < t.npris; ++p)
+            for(int k = 0; k < 6; ++k) pris[6 * p + k] = t.hexs[8 * (size_t)(t.nhexs + p) + k];
     return GM_OK;
 }
 
@@ -458,7 +470,7 @@ int gm_mesh_evaluate(gm_mesh* m, double* out3, int32_t* hist20, double* elemQual
 {
     GM_NEED_DEVICE(m);
     GM_HIP(hipSetDevice(m->device));
-    const size_t E = (size_t)(m->topo->ntets + m->topo->nhexs);
+    const size_t E = (size_t)(m->topo->ntets + m->topo->nhexs + m->topo->npris);
     if(elemQual && !m->elemQualBuf) GM_HIP(devAlloc(&m->elemQualBuf, E));
     gmr::Plan* p = m->plan;
     // Per-element output is switched on in the descriptor for this call only.
@@ -468,7 +480,7 @@ int gm_mesh_evaluate(gm_mesh* m, double* out3, int32_t* hist20, double* elemQual
     if(rc) return rc;
     GM_HIP(gmd::launchResetRun(p->dTable, p->vertBlocks, p->meshList, p->nMeshes, p->stream));
     hipError_t e = gmd::launchEvaluate<double>(p->dTable, p->evalBlocks, p->meshList, p->nMeshes, m->crew,
-                                               m->evaluator, m->sampler, p->hexBlocks.count != 0, p->stream);
+                                               m->evaluator, m->sampler, gmr::planHasRecords(p), p->stream);
     if(e != hipSuccess) return fail(GM_ERR_HIP, std::string("evaluate: ") + hipGetErrorString(e));
     double r[3];
     int32_t h[20];

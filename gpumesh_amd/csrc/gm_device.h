@@ -45,7 +45,8 @@ struct DevMesh
     double4* pos64;
     float4* pos32;
     const uint4* tets;        // 1 per tet
-    const uint4* hexs;        // 2 per hex (v0..v3, v4..v7)
+    const uint4* hexs;        // 2 per record: nhexs hexahedra (v0..v3, v4..v7), then npris prisms
+                              // (v0..v3, v4 v5 v5 v5)
     const uint32_t* eoff;     // V+1
     const uint32_t* eref;
     const uint32_t* noff;     // V+1
@@ -64,7 +65,7 @@ struct DevMesh
     double* elemQual;         // optional per-element qualities (nullptr = off)
     int32_t* passSlot;        // [0] next pass record to write (advanced by the final kernel),
                               // [1] 1 once the autopilot stopped the run (pass kernels then do nothing)
-    uint32_t nverts, ntets, nhexs;
+    uint32_t nverts, ntets, nhexs, npris;
     uint32_t evalBlocks;
     uint32_t resultCap;       // pass records evalResult / evalHistOut hold
     // Pass schedule (AbstractSmoother::evaluateMeshQuality,
@@ -156,8 +157,8 @@ struct Launchers
 {
     static hipError_t smooth(const VertexWork& w, const gmq::Crew& c, int smoother, int shape,
                              LaneShape lane, double moveCoeff, hipStream_t stream);
-    static hipError_t getmeElements(const DevMesh* meshes, BlockWork tets, BlockWork hexs, const gmq::Crew& c,
-                                    hipStream_t stream);
+    static hipError_t getmeElements(const DevMesh* meshes, BlockWork tets, BlockWork hexs, BlockWork pris,
+                                    const gmq::Crew& c, hipStream_t stream);
     // GETMe vertex updates: KS_WAVE, one vertex per group of lanes (patches
     // of up to 64 elements); otherwise one vertex per thread, 64 x
     // lane.patchesPerGroup threads per block.
@@ -183,14 +184,14 @@ template<typename T>
 hipError_t launchSmooth(const VertexWork& w, const gmq::Crew& c, int evaluator, int sampler, int smoother,
                         int shape, LaneShape lane, double moveCoeff, hipStream_t stream);
 template<typename T>
-hipError_t launchGetmeElements(const DevMesh* meshes, BlockWork tets, BlockWork hexs, const gmq::Crew& c,
-                               int evaluator, int sampler, hipStream_t stream);
+hipError_t launchGetmeElements(const DevMesh* meshes, BlockWork tets, BlockWork hexs, BlockWork pris,
+                               const gmq::Crew& c, int evaluator, int sampler, hipStream_t stream);
 template<typename T>
 hipError_t launchGetmeVertices(const VertexWork& w, const gmq::Crew& c, int evaluator, int sampler, int shape,
                                LaneShape lane, hipStream_t stream);
 // P = position precision read (double: pos64, float: pos32); always FP64 math.
-// `hexes`: some mesh of the batch has hexahedra (a tet-only batch runs a
-// kernel without the hexahedron branch, which needs fewer registers).
+// `hexes`: some mesh of the batch has hexahedra or prisms (a tet-only batch
+// runs a kernel without the record branch, which needs fewer registers).
 template<typename P>
 hipError_t launchEvaluate(const DevMesh* meshes, BlockWork blocks, const uint32_t* meshList, uint32_t nmeshes,
                           const gmq::Crew& c, int evaluator, int sampler, bool hexes, hipStream_t stream);

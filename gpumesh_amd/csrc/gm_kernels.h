@@ -296,6 +296,8 @@ __device__ __forceinline__ T elementQualityAt(const DevMesh& m, const gmq::Crew&
     vp[3] = k == 3 ? p : ld<T>(pos, a.w);
     vp[4] = k == 4 ? p : ld<T>(pos, b.x);
     vp[5] = k == 5 ? p : ld<T>(pos, b.y);
+    if(id >= m.nhexs)
+        return gmq::priQuality<T, EV, SA>(c, vp);
     vp[6] = k == 6 ? p : ld<T>(pos, b.z);
     vp[7] = k == 7 ? p : ld<T>(pos, b.w);
     return gmq::hexQuality<T, EV, SA>(c, vp);
@@ -395,7 +397,8 @@ struct WavePatch
     T* red;           // LDS: 2 values per lane of this group (quality, 1/quality)
     int lane;         // lane within the group
     int nElem;        // patch size (<= G)
-    int isHex;
+    int isHex;        // this lane's element is a record: a hexahedron, or a prism when isPri is set
+    int isPri;
     int k;            // local index of the centre vertex in this lane's element
     v3<T> vp[EKIND == EK_TET ? 4 : 8];   // this lane's element (centre slot overwritten per call)
     int count;
@@ -432,7 +435,10 @@ struct WavePatch
                 v3<T> e[8];
 #pragma unroll
                 for(int i = 0; i < 8; ++i) e[i] = (i == k) ? p : vp[i];
-                q = gmq::hexQuality<T, EV, SA>(*c, e);
+                if(EKIND == EK_MIXED && isPri)
+                    q = gmq::priQuality<T, EV, SA>(*c, e);
+                else
+                    q = gmq::hexQuality<T, EV, SA>(*c, e);
             }
         }
         return q;
@@ -503,6 +509,7 @@ __global__ void __launch_bounds__(64) smoothWaveKernel(const DevMesh* __restrict
     uint32_t eb = m.eoff[v], ee = m.eoff[v + 1];
     f.nElem = (int)(ee - eb);
     f.isHex = 0;
+    f.isPri = 0;
     f.k = 0;
     if(lane < f.nElem)
     {
@@ -510,6 +517,7 @@ __global__ void __launch_bounds__(64) smoothWaveKernel(const DevMesh* __restrict
         uint32_t id = ref >> 4;
         f.k = (int)(ref & 7);
         f.isHex = (ref >> 3) & 1;
+        f.isPri = f.isHex && id >= m.nhexs;
         if(!f.isHex)
         {
             uint4 t = m.tets[id];
@@ -2243,6 +2251,70 @@ __global__ void __launch_bounds__(ELEM_BLOCK) getmeHexKernel(const DevMesh* __re
     }
 }
 
+// GetmeSmoother::smoothPris (GetmeSmoother.cpp:131-243): prism e is record
+// nhexs + e.  Base points sit between the triangle centres and the side-face
+// centres at t = (4/5)(1 - (4/39)^(1/4) lambda), which maps the regular prism
+// onto itself.
+template<typename T, int EV, int SA>
+__global__ void __launch_bounds__(ELEM_BLOCK) getmePriKernel(const DevMesh* __restrict__ meshes,
+                                                             const uint4* __restrict__ spans, uint32_t nspans,
+                                                             gmq::Crew c)
+{
+    const uint2 bl = blockItem(spans, nspans, blockIdx.x);
+    const DevMesh& m = meshes[bl.x];
+    const uint32_t e = bl.y * ELEM_BLOCK + threadIdx.x;
+    if(e >= m.npris || runStopped(m))
+        return;
+    typedef typename PosOf<T>::type P;
+    const P* pos = PosOf<T>::get(m);
+    P* out = (P*)m.getmePull;
+    const T lambda = T(0.78);
+    const size_t r = (size_t)m.nhexs + e;
+    uint4 ha = m.hexs[2 * r], hb = m.hexs[2 * r + 1];
+    uint32_t vi[6] = {ha.x, ha.y, ha.z, ha.w, hb.x, hb.y};
+    v3<T> vp[6];
+#pragma unroll
+    for(int k = 0; k < 6; ++k) vp[k] = ld<T>(pos, vi[k]);
+    const T q4 = T(4), t3 = T(3), two = T(2);
+    v3<T> aux[5] = {(vp[0] + vp[1] + vp[2]) / t3, (vp[0] + vp[1] + vp[4] + vp[3]) / q4,
+                    (vp[1] + vp[2] + vp[5] + vp[4]) / q4, (vp[2] + vp[0] + vp[3] + vp[5]) / q4,
+                    (vp[3] + vp[4] + vp[5]) / t3};
+    v3<T> n[6] = {gmq::cross(aux[1] - aux[0], aux[3] - aux[0]), gmq::cross(aux[2] - aux[0], aux[1] - aux[0]),
+                  gmq::cross(aux[3] - aux[0], aux[2] - aux[0]), gmq::cross(aux[3] - aux[4], aux[1] - aux[4]),
+                  gmq::cross(aux[1] - aux[4], aux[2] - aux[4]), gmq::cross(aux[2] - aux[4], aux[3] - aux[4])};
+    const T t = (T)((4.0 / 5.0) * (1.0 - gm_pow(4.0 / 39.0, 0.25) * 0.78));
+    const T it = T(1) - t;
+    v3<T> bases[6] = {it * aux[0] + t * (aux[3] + aux[1]) / two, it * aux[0] + t * (aux[1] + aux[2]) / two,
+                      it * aux[0] + t * (aux[2] + aux[3]) / two, it * aux[4] + t * (aux[3] + aux[1]) / two,
+                      it * aux[4] + t * (aux[1] + aux[2]) / two, it * aux[4] + t * (aux[2] + aux[3]) / two};
+    v3<T> vpp[6];
+#pragma unroll
+    for(int k = 0; k < 6; ++k) vpp[k] = bases[k] + n[k] * gmq::tdiv(lambda, gmq::tsqrt(gmq::tsqrt(gmq::dot(n[k], n[k]))));
+    T volume = gmq::priVolume<T, SA>(c, vp);
+    T volumePrime = gmq::priVolume<T, SA>(c, vpp);
+    T volumeVar = gmq::tcbrt(gmq::tabs(gmq::tdiv(volume, volumePrime)));
+    v3<T> center = T(1.0 / 6.0) * (vp[0] + vp[1] + vp[2] + vp[3] + vp[4] + vp[5]);
+#pragma unroll
+    for(int k = 0; k < 6; ++k) vpp[k] = center + volumeVar * (vpp[k] - center);
+#pragma unroll
+    for(int k = 0; k < 6; ++k)
+    {
+        if(m.fixed[vi[k]]) vpp[k] = vp[k];
+        else if(const double* cs = consOf(m, vi[k])) vpp[k] = DevSnap<T>{cs}(vpp[k]);
+    }
+    T quality = gmq::priQuality<T, EV, SA>(c, vp);
+    T qualityPrime = gmq::priQuality<T, EV, SA>(c, vpp);
+    T weight = gmq::tdiv(qualityPrime, T(1) + quality);
+    ((T*)m.getmeQual)[m.ntets + r] = quality;
+#pragma unroll
+    for(int k = 0; k < 6; ++k)
+    {
+        P o;
+        o.x = vpp[k].x; o.y = vpp[k].y; o.z = vpp[k].z; o.w = weight;
+        out[m.cornerSlot[4 * (size_t)m.ntets + 8 * r + k]] = o;
+    }
+}
+
 // GETMe vertex step (AbstractElementWiseSmoother::updateVertexPositions,
 // AbstractElementWiseSmoother.cpp:406-444; accumulation VertexAccum.cpp:28-49)
 // for one colour group: pull the contributions in patch order (the push order
@@ -2328,6 +2400,7 @@ __global__ void __launch_bounds__(64 * GETME_WAVES) getmeWaveKernel(const DevMes
     const uint32_t eb = m.eoff[v], ee = m.eoff[v + 1];
     f.nElem = (int)(ee - eb);
     f.isHex = 0;
+    f.isPri = 0;
     f.k = 0;
     T* pl = pull + 4 * G * grp;
     T* qual = (T*)m.getmeQual;
@@ -2341,6 +2414,7 @@ __global__ void __launch_bounds__(64 * GETME_WAVES) getmeWaveKernel(const DevMes
         const uint32_t id = ref >> 4;
         f.k = (int)(ref & 7);
         f.isHex = (ref >> 3) & 1;
+        f.isPri = f.isHex && id >= m.nhexs;
         qslot = f.isHex ? m.ntets + id : id;
         qOld = qual[qslot];
         P o;
@@ -2431,7 +2505,9 @@ __global__ void __launch_bounds__(EVAL_THREADS) evalKernel(const DevMesh* __rest
     if(threadIdx.x == 0) sbad = 0;
     __syncthreads();
     const P* pos = PosOf<typename std::conditional<std::is_same<P, double4>::value, double, float>::type>::get(m);
-    const uint64_t E = (uint64_t)m.ntets + m.nhexs;
+    // The tet-only variant never reads past the tetrahedra, whatever it is
+    // launched on.
+    const uint64_t E = HEX ? (uint64_t)m.ntets + m.nhexs + m.npris : (uint64_t)m.ntets;
     const uint64_t chunk = (E + m.evalBlocks - 1) / m.evalBlocks;
     const uint64_t b0 = (uint64_t)bl.y * chunk;
     const uint64_t b1 = (b0 + chunk < E) ? b0 + chunk : E;
@@ -2442,6 +2518,9 @@ __global__ void __launch_bounds__(EVAL_THREADS) evalKernel(const DevMesh* __rest
     for(uint64_t e = b0 + threadIdx.x; e < b1; e += blockDim.x)
     {
         double q, inv = 0.0;
+        // Per-element qualities go out in the reference's order: tets, then
+        // prisms, then hexahedra (records hold the hexahedra first).
+        uint64_t out = e;
         if(!HEX || e < m.ntets)
         {
             uint4 t = m.tets[e];
@@ -2457,12 +2536,22 @@ __global__ void __launch_bounds__(EVAL_THREADS) evalKernel(const DevMesh* __rest
             uint4 a = m.hexs[2 * h], bq = m.hexs[2 * h + 1];
             v3<double> vp[8] = {ld<double>(pos, a.x), ld<double>(pos, a.y), ld<double>(pos, a.z), ld<double>(pos, a.w),
                                 ld<double>(pos, bq.x), ld<double>(pos, bq.y), ld<double>(pos, bq.z), ld<double>(pos, bq.w)};
-            if constexpr(FAST)
-                q = fastHexEval(c, vp, inv);
+            if(h >= m.nhexs)
+            {
+                q = gmq::priQuality<double, EV, SA>(c, vp);
+                inv = 1.0 / q;
+                out = m.ntets + (h - m.nhexs);
+            }
             else
-                q = gmq::hexQuality<double, EV, SA>(c, vp);
+            {
+                if constexpr(FAST)
+                    q = fastHexEval(c, vp, inv);
+                else
+                    q = gmq::hexQuality<double, EV, SA>(c, vp);
+                out = (uint64_t)m.ntets + m.npris + h;
+            }
         }
-        if(m.elemQual) m.elemQual[e] = q;
+        if(m.elemQual) m.elemQual[out] = q;
         mn = gmq::tmin(mn, q);
         if(q > 0.0)
         {
@@ -2562,7 +2651,7 @@ static __global__ void __launch_bounds__(EVAL_THREADS) evalFinalKernel(const Dev
     {
         const double mn0 = smin[0];
         const double inv = sbad ? (double)__builtin_huge_val() : ssum[0] + scmp[0];
-        const double n = (double)m.ntets + (double)m.nhexs;
+        const double n = (double)m.ntets + (double)m.nhexs + (double)m.npris;
         m.evalResult[3 * slot] = mn0;
         m.evalResult[3 * slot + 1] = inv;
         m.evalResult[3 * slot + 2] = n;
@@ -2738,12 +2827,14 @@ hipError_t Launchers<T, EV, SA>::smooth(const VertexWork& w, const gmq::Crew& c,
 
 template<typename T, int EV, int SA>
 hipError_t Launchers<T, EV, SA>::getmeElements(const DevMesh* meshes, BlockWork tets, BlockWork hexs,
-                                               const gmq::Crew& c, hipStream_t stream)
+                                               BlockWork pris, const gmq::Crew& c, hipStream_t stream)
 {
     if(tets.count)
         getmeTetKernel<T, EV, SA><<<tets.count, ELEM_BLOCK, 0, stream>>>(meshes, tets.spans, tets.nspans, c);
     if(hexs.count)
         getmeHexKernel<T, EV, SA><<<hexs.count, ELEM_BLOCK, 0, stream>>>(meshes, hexs.spans, hexs.nspans, c);
+    if(pris.count)
+        getmePriKernel<T, EV, SA><<<pris.count, ELEM_BLOCK, 0, stream>>>(meshes, pris.spans, pris.nspans, c);
     return hipGetLastError();
 }
 

@@ -15,9 +15,15 @@
 #if defined(__HIPCC__) || defined(__HIP__)
 #define GMQ __host__ __device__ __forceinline__
 #define GMQ_MEMBER __host__ __device__ __forceinline__
+// A routine kept out of line: it is inlined at so many call sites (every
+// Keast point of every corner) that the functions holding them grow long
+// enough for the compiler to expand their branches through the register
+// pair that holds a function's return address.
+#define GMQ_OUTLINE static __host__ __device__ __attribute__((noinline))
 #else
 #define GMQ static inline
 #define GMQ_MEMBER inline
+#define GMQ_OUTLINE static __attribute__((noinline))
 #endif
 
 namespace gmq
@@ -221,7 +227,7 @@ template<typename T> GMQ m3<T> textureMetric(const Crew& c, v3<T> position)
 // axis the metric length scale is K a^(c^a) with c = (1 - cos(2.5 pi x'))/2,
 // K across it; M = R^T diag(Mx, K^2, K^2) R.  Evaluated in FP64 in both
 // precisions (the project's cos and pow, so the host and the device agree).
-GMQ m3<double> referenceMetric(const Crew& c, v3<double> p)
+GMQ_OUTLINE m3<double> referenceMetric(const Crew& c, v3<double> p)
 {
     m3<double> R, RI;
     for(int j = 0; j < 3; ++j)
@@ -324,6 +330,29 @@ template<typename T, int SA> GMQ T tetVolume(const Crew& c, const v3<T> vp[4])
     return d / T(6);
 }
 
+// MetricFreeMeasurer::priVolume (:58-75) / MetricWiseMeasurer::priVolume (:141-158).
+template<typename T, int SA> GMQ T priVolume(const Crew& c, const v3<T> vp[6])
+{
+    v3<T> e02, e12, e32, e42, e52;
+    if(SA == SA_UNIFORM)
+    {
+        e02 = vp[2] - vp[0]; e12 = vp[2] - vp[1]; e32 = vp[2] - vp[3]; e42 = vp[2] - vp[4]; e52 = vp[2] - vp[5];
+    }
+    else
+    {
+        e02 = segment<T, SA>(c, vp[0], vp[2]); e12 = segment<T, SA>(c, vp[1], vp[2]);
+        e32 = segment<T, SA>(c, vp[3], vp[2]); e42 = segment<T, SA>(c, vp[4], vp[2]);
+        e52 = segment<T, SA>(c, vp[5], vp[2]);
+    }
+    T s = T(0);
+    s += det(mcols<T>(e32, e52, e42));
+    s += det(mcols<T>(e02, e32, e42));
+    s += det(mcols<T>(e12, e02, e42));
+    if(SA == SA_UNIFORM)
+        return (T)c.scaling3 * s / T(6);
+    return s / T(6);
+}
+
 template<typename T, int SA> GMQ T hexVolume(const Crew& c, const v3<T> vp[8])
 {
     T s = T(0);
@@ -350,6 +379,14 @@ template<typename T> GMQ m3<T> frTetInv()
     return mcols<T>(mk<T>(T(1), T(0), T(0)),
                     mk<T>(T(-0.5773502691896257645091), T(1.154700538379251529018), T(0)),
                     mk<T>(T(-0.4082482904638630163662), T(-0.4082482904638630163662), T(1.224744871391589049099)));
+}
+
+// Fr_PRI_INV (Evaluators/AbstractEvaluator.cpp:32-35).
+template<typename T> GMQ m3<T> frPriInv()
+{
+    return mcols<T>(mk<T>(T(1), T(0), T(0)),
+                    mk<T>(T(-0.5773502691896257645091), T(1.154700538379251529018), T(0)),
+                    mk<T>(T(0), T(0), T(1)));
 }
 
 // MeanRatioEvaluator::cornerQuality (MeanRatioEvaluator.cpp:22-32).
@@ -424,6 +461,61 @@ template<typename T> GMQ T hexMeanOrMin(const T q[8])
     }
     return T(8) / (T(1) / q[0] + T(1) / q[1] + T(1) / q[2] + T(1) / q[3] +
                    T(1) / q[4] + T(1) / q[5] + T(1) / q[6] + T(1) / q[7]);
+}
+
+// The prism's mean-or-min rule (MeanRatioEvaluator.cpp:85-103,
+// MetricConformityEvaluator.cpp:133-151).
+template<typename T> GMQ T priMeanOrMin(const T q[6])
+{
+    if(q[0] <= T(0) || q[1] <= T(0) || q[2] <= T(0) || q[3] <= T(0) || q[4] <= T(0) || q[5] <= T(0))
+    {
+        T mn = q[0];
+        for(int k = 1; k < 6; ++k) if(q[k] < mn) mn = q[k];
+        return mn;
+    }
+    return T(6) / (T(1) / q[0] + T(1) / q[1] + T(1) / q[2] + T(1) / q[3] + T(1) / q[4] + T(1) / q[5]);
+}
+
+// Prism quality: MeanRatioEvaluator::priQuality (:52-104) or
+// MetricConformityEvaluator::priQuality (:93-152).  The corner frames
+// (-e01, e20, e03) ... times Fr_PRI_INV are not invariant under edge swaps.
+template<typename T, int EV, int SA> GMQ T priQuality(const Crew& c, const v3<T> vp[6])
+{
+    v3<T> e03, e14, e25, e01, e12, e20, e34, e45, e53;
+    if(EV == EV_METRIC_CONFORMITY)
+    {
+        e03 = vp[3] - vp[0]; e14 = vp[4] - vp[1]; e25 = vp[5] - vp[2];
+        e01 = vp[1] - vp[0]; e12 = vp[2] - vp[1]; e20 = vp[0] - vp[2];
+        e34 = vp[4] - vp[3]; e45 = vp[5] - vp[4]; e53 = vp[3] - vp[5];
+    }
+    else
+    {
+        e03 = segment<T, SA>(c, vp[0], vp[3]); e14 = segment<T, SA>(c, vp[1], vp[4]);
+        e25 = segment<T, SA>(c, vp[2], vp[5]); e01 = segment<T, SA>(c, vp[0], vp[1]);
+        e12 = segment<T, SA>(c, vp[1], vp[2]); e20 = segment<T, SA>(c, vp[2], vp[0]);
+        e34 = segment<T, SA>(c, vp[3], vp[4]); e45 = segment<T, SA>(c, vp[4], vp[5]);
+        e53 = segment<T, SA>(c, vp[5], vp[3]);
+    }
+    const m3<T> fr = frPriInv<T>();
+    const m3<T> F0 = mmul<T>(mcols<T>(-e01, e20, e03), fr), F1 = mmul<T>(mcols<T>(-e12, e01, e14), fr);
+    const m3<T> F2 = mmul<T>(mcols<T>(-e20, e12, e25), fr), F3 = mmul<T>(mcols<T>(-e34, e53, e03), fr);
+    const m3<T> F4 = mmul<T>(mcols<T>(-e45, e34, e14), fr), F5 = mmul<T>(mcols<T>(-e53, e45, e25), fr);
+    T q[6];
+    if(EV == EV_METRIC_CONFORMITY)
+    {
+        q[0] = metricConformity<T>(F0, specifiedMetric<T, SA>(c, vp[0], vp[1], vp[2], vp[3]));
+        q[1] = metricConformity<T>(F1, specifiedMetric<T, SA>(c, vp[0], vp[1], vp[2], vp[4]));
+        q[2] = metricConformity<T>(F2, specifiedMetric<T, SA>(c, vp[0], vp[1], vp[2], vp[5]));
+        q[3] = metricConformity<T>(F3, specifiedMetric<T, SA>(c, vp[0], vp[3], vp[4], vp[5]));
+        q[4] = metricConformity<T>(F4, specifiedMetric<T, SA>(c, vp[1], vp[3], vp[4], vp[5]));
+        q[5] = metricConformity<T>(F5, specifiedMetric<T, SA>(c, vp[2], vp[3], vp[4], vp[5]));
+    }
+    else
+    {
+        q[0] = cornerQuality<T>(F0); q[1] = cornerQuality<T>(F1); q[2] = cornerQuality<T>(F2);
+        q[3] = cornerQuality<T>(F3); q[4] = cornerQuality<T>(F4); q[5] = cornerQuality<T>(F5);
+    }
+    return priMeanOrMin<T>(q);
 }
 
 // Hex quality: MeanRatioEvaluator::hexQuality (:106-167) or

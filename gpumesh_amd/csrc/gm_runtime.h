@@ -158,7 +158,7 @@ struct Plan
     // colour, then subsurface, then interior); empty when no member has
     // sliding vertices (then they follow `colours`).
     std::vector<std::vector<VList>> getmeSteps;
-    gmd::BlockWork tetBlocks{}, hexBlocks{}, evalBlocks{}, vertBlocks{};
+    gmd::BlockWork tetBlocks{}, hexBlocks{}, priBlocks{}, evalBlocks{}, vertBlocks{};
     uint32_t* meshList = nullptr; uint32_t nMeshes = 0;
     uint4* spans = nullptr;                       // every span table of the plan, one buffer
     std::vector<void*> owned;                     // device allocations of this plan
@@ -190,9 +190,10 @@ void setPoolCap(size_t bytes);
 size_t poolCachedBytes();
 
 // Host topology of a new handle (through the topology cache).
+// npris prisms (6 vertices each) may follow.
 int makeMesh(int64_t nverts, const double* pos, const int32_t* fixed, int64_t ntets, const uint32_t* tets,
              int64_t nhexs, const uint32_t* hexs, gm_mesh** out, const int32_t* cons = nullptr,
-             const std::vector<double>* consTable = nullptr);
+             const std::vector<double>* consTable = nullptr, int64_t npris = 0, const uint32_t* pris = nullptr);
 // The GM_ANALYTIC field's parameters of a crew: the aspect ratio and the
 // sampler's fixed rotation (AbstractSampler's constructor,
 // Samplers/AbstractSampler.cpp:30-41) and its transpose.
@@ -205,7 +206,8 @@ uint64_t texelHash(const float* texels, size_t n);
 int makeMeshGrouped(int64_t nverts, const double* pos, const int32_t* fixed, int64_t ntets, const uint32_t* tets,
                     int64_t nhexs, const uint32_t* hexs, int64_t ngroups, const int64_t* groupOff,
                     const uint32_t* groupNodes, gm_mesh** out, const int32_t* cons = nullptr,
-                    const std::vector<double>* consTable = nullptr);
+                    const std::vector<double>* consTable = nullptr, int64_t npris = 0,
+                    const uint32_t* pris = nullptr);
 // Device memory a mesh needs when resident (connectivity, positions, reduction
 // buffers, `records` pass records, GETMe scratch when `getme`).
 size_t residentBytes(const gm_mesh* m, int records, bool getme);
@@ -219,6 +221,10 @@ void destroyMesh(gm_mesh* m);
 
 // Builds the fused plan of `meshes` (all resident on one device) on `stream`.
 int buildPlan(const std::vector<gm_mesh*>& meshes, hipStream_t stream, Plan** out);
+// Whether some member has hexahedra or prisms: the evaluation then runs the
+// kernel with the record branch (the tet-only one reads every element as a
+// tetrahedron).
+bool planHasRecords(const Plan* p);
 void freePlan(Plan* p);
 // Makes sure every mesh has room for `records` pass records and (for GETMe)
 // its element scratch, then uploads the descriptors.

@@ -58,10 +58,10 @@ uint64_t hashBytes(uint64_t h, const void* data, size_t n)
 struct Key
 {
     uint64_t key;
-    int64_t nverts, ntets, nhexs;
+    int64_t nverts, ntets, nhexs, npris;
     std::vector<uint8_t> fixed;
     std::vector<int32_t> cons;      // empty without sliding constraints
-    std::vector<uint32_t> tets, hexs;
+    std::vector<uint32_t> tets, hexs, pris;
     std::shared_ptr<const Topology> topo;
 };
 
@@ -132,7 +132,8 @@ void cacheStats(int64_t* out4)
 
 std::shared_ptr<const Topology> buildShared(int64_t nverts, const double* pos, const int32_t* fixed,
                                             const int32_t* cons, int64_t ntets, const uint32_t* tets, int64_t nhexs,
-                                            const uint32_t* hexs, std::vector<double>& renumbered, std::string& err)
+                                            const uint32_t* hexs, int64_t npris, const uint32_t* pris,
+                                            std::vector<double>& renumbered, std::string& err)
 {
     std::vector<uint8_t> fx((size_t)std::max<int64_t>(nverts, 0));
     for(int64_t v = 0; v < nverts; ++v) fx[v] = fixed[v] != 0;
@@ -159,6 +160,7 @@ std::shared_ptr<const Topology> buildShared(int64_t nverts, const double* pos, c
         key = hashBytes(key, cs.data(), 4 * cs.size());
         key = hashBytes(key, tets, 16 * (size_t)ntets);
         key = hashBytes(key, hexs, 32 * (size_t)nhexs);
+        key = hashBytes(key, pris, 24 * (size_t)npris);
         // Candidates with this hash, taken under the lock; their inputs are
         // immutable and kept alive by the shared pointers, so the full
         // comparison runs without it.
@@ -166,7 +168,8 @@ std::shared_ptr<const Topology> buildShared(int64_t nverts, const double* pos, c
         {
             std::lock_guard<std::mutex> lock(g_mutex);
             for(const Entry& e : g_entries)
-                if(e.k->key == key && e.k->nverts == nverts && e.k->ntets == ntets && e.k->nhexs == nhexs)
+                if(e.k->key == key && e.k->nverts == nverts && e.k->ntets == ntets && e.k->nhexs == nhexs &&
+                   e.k->npris == npris)
                     cand.push_back(e.k);
         }
         std::shared_ptr<const Key> match;
@@ -175,6 +178,7 @@ std::shared_ptr<const Topology> buildShared(int64_t nverts, const double* pos, c
             if(k->fixed != fx || k->cons != cs) continue;
             if(ntets && std::memcmp(k->tets.data(), tets, 16 * (size_t)ntets) != 0) continue;
             if(nhexs && std::memcmp(k->hexs.data(), hexs, 32 * (size_t)nhexs) != 0) continue;
+            if(npris && std::memcmp(k->pris.data(), pris, 24 * (size_t)npris) != 0) continue;
             match = k;
             break;
         }
@@ -205,7 +209,8 @@ std::shared_ptr<const Topology> buildShared(int64_t nverts, const double* pos, c
         }
     }
     auto t = std::make_shared<Topology>();
-    if(!build(nverts, pos, fixed, cs.empty() ? nullptr : cs.data(), ntets, tets, nhexs, hexs, *t, err)) return nullptr;
+    if(!build(nverts, pos, fixed, cs.empty() ? nullptr : cs.data(), ntets, tets, nhexs, hexs, npris, pris, *t, err))
+        return nullptr;
     renumbered = std::move(t->pos);
     t->pos.clear();
     t->pos.shrink_to_fit();
@@ -217,13 +222,16 @@ std::shared_ptr<const Topology> buildShared(int64_t nverts, const double* pos, c
         k->nverts = nverts;
         k->ntets = ntets;
         k->nhexs = nhexs;
+        k->npris = npris;
         k->fixed = std::move(fx);
         k->cons = std::move(cs);
         k->tets.assign(tets, tets + 4 * (size_t)ntets);
         k->hexs.assign(hexs, hexs + 8 * (size_t)nhexs);
+        k->pris.assign(pris, pris + 6 * (size_t)npris);
         k->topo = ct;
         Entry e;
-        e.bytes = topologyBytes(*ct) + k->fixed.size() + 4 * (k->cons.size() + k->tets.size() + k->hexs.size());
+        e.bytes = topologyBytes(*ct) + k->fixed.size() +
+                  4 * (k->cons.size() + k->tets.size() + k->hexs.size() + k->pris.size());
         e.k = std::move(k);
         std::lock_guard<std::mutex> lock(g_mutex);
         ++g_misses;

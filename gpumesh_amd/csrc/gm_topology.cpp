@@ -8,8 +8,10 @@
 namespace gmt
 {
 
-// Element edge tables: MeshTet::edges and MeshHex::edges (Mesh.cpp:21-28, 98-111).
+// Element edge tables: MeshTet::edges, MeshPri::edges and MeshHex::edges
+// (Mesh.cpp:21-28, 70-80, 98-111).
 static const int TET_EDGES[6][2] = {{0,1},{0,2},{0,3},{1,2},{1,3},{2,3}};
+static const int PRI_EDGES[9][2] = {{0,1},{0,2},{1,2},{0,3},{1,4},{2,5},{3,4},{3,5},{4,5}};
 static const int HEX_EDGES[12][2] = {{0,1},{0,3},{1,2},{2,3},{0,4},{1,5},{2,6},{3,7},{4,5},{4,7},{5,6},{6,7}};
 
 static const int NO_GROUP = -2;
@@ -19,28 +21,50 @@ static const int UNSET_GROUP = -1;
 // (Mesh::compileNeighborhoods and addEdge, Mesh.cpp:365-460).
 struct Adjacency
 {
-    std::vector<uint32_t> eoff, eref;   // CSR patch lists, eref = (id << 4) | (is_hex << 3) | local
+    std::vector<uint32_t> eoff, eref;   // CSR patch lists, eref = (id << 4) | (is_record << 3) | local
     std::vector<uint32_t> ncnt, ntmp;   // neighbour count, and 3 eoff[v] slots per vertex
 };
 
-static bool checkInputs(int64_t V, int64_t T, const uint32_t* tets, int64_t H, const uint32_t* hexs, std::string& err)
+// The element records of a mesh: 8 slots per hexahedron, then per prism
+// (its 6 vertices, then the last one twice).
+static std::vector<uint32_t> makeRecords(int64_t H, const uint32_t* hexs, int64_t P, const uint32_t* pris)
 {
-    if(V < 0 || T < 0 || H < 0) { err = "negative size"; return false; }
+    std::vector<uint32_t> rec(8 * (size_t)(H + P));
+    if(H) std::copy(hexs, hexs + 8 * H, rec.begin());
+    for(int64_t p = 0; p < P; ++p)
+        for(int k = 0; k < 8; ++k) rec[8 * (size_t)(H + p) + k] = pris[6 * p + (k < 6 ? k : 5)];
+    return rec;
+}
+
+// Vertices of record r (hexahedra below H, prisms from H on).
+static inline int recordVerts(int64_t H, uint32_t r) { return (int64_t)r < H ? 8 : 6; }
+
+static bool checkInputs(int64_t V, int64_t T, const uint32_t* tets, int64_t H, const uint32_t* hexs, int64_t P,
+                        const uint32_t* pris, std::string& err)
+{
+    if(V < 0 || T < 0 || H < 0 || P < 0) { err = "negative size"; return false; }
     if(V > (int64_t)0xFFFFFFF0) { err = "too many vertices"; return false; }
-    if(T >= (int64_t)1 << 28 || H >= (int64_t)1 << 28) { err = "too many elements (limit 2^28 per type)"; return false; }
+    if(T >= (int64_t)1 << 28 || H + P >= (int64_t)1 << 28)
+        { err = "too many elements (limit 2^28 tetrahedra, 2^28 hexahedra and prisms together)"; return false; }
     for(int64_t i = 0; i < 4 * T; ++i) if((int64_t)tets[i] >= V) { err = "tet vertex id out of range"; return false; }
     for(int64_t i = 0; i < 8 * H; ++i) if((int64_t)hexs[i] >= V) { err = "hex vertex id out of range"; return false; }
-    if(4 * T + 8 * H >= (int64_t)1 << 32) { err = "too many element references"; return false; }
+    for(int64_t i = 0; i < 6 * P; ++i) if((int64_t)pris[i] >= V) { err = "prism vertex id out of range"; return false; }
+    if(4 * T + 6 * P + 8 * H >= (int64_t)1 << 32) { err = "too many element references"; return false; }
     return true;
 }
 
-static void buildAdjacency(int64_t V, int64_t T, const uint32_t* tets, int64_t H, const uint32_t* hexs, Adjacency& a)
+// rec: H hexahedron records, then P prism records (makeRecords).
+static void buildAdjacency(int64_t V, int64_t T, const uint32_t* tets, int64_t H, int64_t P, const uint32_t* rec,
+                           Adjacency& a)
 {
-    // ---- Patch element lists (compileNeighborhoods: tets in id order, then hexes).
+    // ---- Patch element lists (compileNeighborhoods: tets in id order, then
+    // prisms, then hexes).
     std::vector<uint32_t>& eoff = a.eoff;
     eoff.assign(V + 1, 0);
     for(int64_t i = 0; i < 4 * T; ++i) ++eoff[tets[i] + 1];
-    for(int64_t i = 0; i < 8 * H; ++i) ++eoff[hexs[i] + 1];
+    for(int64_t p = 0; p < P; ++p)
+        for(int k = 0; k < 6; ++k) ++eoff[rec[8 * (H + p) + k] + 1];
+    for(int64_t i = 0; i < 8 * H; ++i) ++eoff[rec[i] + 1];
     for(int64_t v = 0; v < V; ++v) eoff[v + 1] += eoff[v];
     std::vector<uint32_t>& eref = a.eref;
     eref.assign(eoff[V], 0);
@@ -49,9 +73,12 @@ static void buildAdjacency(int64_t V, int64_t T, const uint32_t* tets, int64_t H
         for(int64_t e = 0; e < T; ++e)
             for(int k = 0; k < 4; ++k)
                 eref[cur[tets[4 * e + k]]++] = ((uint32_t)e << 4) | (uint32_t)k;
+        for(int64_t p = 0; p < P; ++p)
+            for(int k = 0; k < 6; ++k)
+                eref[cur[rec[8 * (H + p) + k]]++] = ((uint32_t)(H + p) << 4) | 8u | (uint32_t)k;
         for(int64_t e = 0; e < H; ++e)
             for(int k = 0; k < 8; ++k)
-                eref[cur[hexs[8 * e + k]]++] = ((uint32_t)e << 4) | 8u | (uint32_t)k;
+                eref[cur[rec[8 * e + k]]++] = ((uint32_t)e << 4) | 8u | (uint32_t)k;
     }
 
     // ---- Neighbour vertices (addEdge: append when absent, both directions).
@@ -71,16 +98,19 @@ static void buildAdjacency(int64_t V, int64_t T, const uint32_t* tets, int64_t H
     for(int64_t e = 0; e < T; ++e)
         for(int k = 0; k < 6; ++k)
             addEdge(tets[4 * e + TET_EDGES[k][0]], tets[4 * e + TET_EDGES[k][1]]);
+    for(int64_t p = 0; p < P; ++p)
+        for(int k = 0; k < 9; ++k)
+            addEdge(rec[8 * (H + p) + PRI_EDGES[k][0]], rec[8 * (H + p) + PRI_EDGES[k][1]]);
     for(int64_t e = 0; e < H; ++e)
         for(int k = 0; k < 12; ++k)
-            addEdge(hexs[8 * e + HEX_EDGES[k][0]], hexs[8 * e + HEX_EDGES[k][1]]);
+            addEdge(rec[8 * e + HEX_EDGES[k][0]], rec[8 * e + HEX_EDGES[k][1]]);
 }
 
 // NodeGroups::determineTypes (NodeGroups.cpp:124-241): a vertex on a
 // sliding constraint is a boundary vertex, a free vertex of an element that
 // holds one is a subsurface vertex, any other free vertex is interior.
 static bool computeTypes(int64_t V, const int32_t* fixed, const int32_t* cons, int64_t T, const uint32_t* tets,
-                         int64_t H, const uint32_t* hexs, const Adjacency& a, std::vector<uint8_t>& types,
+                         int64_t H, int64_t P, const uint32_t* rec, const Adjacency& a, std::vector<uint8_t>& types,
                          std::string& err)
 {
     types.assign(V, TYPE_INTERIOR);
@@ -99,13 +129,13 @@ static bool computeTypes(int64_t V, const int32_t* fixed, const int32_t* cons, i
         }
     }
     if(!sliding) return true;
-    std::vector<uint8_t> boundElem((size_t)(T + H), 0);
+    std::vector<uint8_t> boundElem((size_t)(T + H + P), 0);
     for(int64_t e = 0; e < T; ++e)
         for(int k = 0; k < 4; ++k)
             if(types[tets[4 * e + k]] == TYPE_BOUNDARY) { boundElem[e] = 1; break; }
-    for(int64_t e = 0; e < H; ++e)
-        for(int k = 0; k < 8; ++k)
-            if(types[hexs[8 * e + k]] == TYPE_BOUNDARY) { boundElem[T + e] = 1; break; }
+    for(int64_t e = 0; e < H + P; ++e)
+        for(int k = 0; k < recordVerts(H, (uint32_t)e); ++k)
+            if(types[rec[8 * e + k]] == TYPE_BOUNDARY) { boundElem[T + e] = 1; break; }
     for(int64_t v = 0; v < V; ++v)
     {
         if(types[v] != TYPE_INTERIOR) continue;
@@ -150,11 +180,12 @@ static void finishGroups(const std::vector<std::vector<uint32_t>>& members, Topo
 
 bool build(int64_t V, const double* pos, const int32_t* fixed, const int32_t* cons,
            int64_t T, const uint32_t* tets, int64_t H, const uint32_t* hexs,
-           Topology& out, std::string& err)
+           int64_t P, const uint32_t* pris, Topology& out, std::string& err)
 {
-    if(!checkInputs(V, T, tets, H, hexs, err)) return false;
+    if(!checkInputs(V, T, tets, H, hexs, P, pris, err)) return false;
+    const std::vector<uint32_t> rec = makeRecords(H, hexs, P, pris);
     Adjacency adj;
-    buildAdjacency(V, T, tets, H, hexs, adj);
+    buildAdjacency(V, T, tets, H, P, rec.data(), adj);
     const std::vector<uint32_t>& eoff = adj.eoff;
     const std::vector<uint32_t>& eref = adj.eref;
     const std::vector<uint32_t>& ncnt = adj.ncnt;
@@ -162,7 +193,7 @@ bool build(int64_t V, const double* pos, const int32_t* fixed, const int32_t* co
 
     // ---- Types (determineTypes).
     std::vector<uint8_t> types;
-    if(!computeTypes(V, fixed, cons, T, tets, H, hexs, adj, types, err)) return false;
+    if(!computeTypes(V, fixed, cons, T, tets, H, P, rec.data(), adj, types, err)) return false;
 
     // ---- Greedy colouring (determineGroups, NodeGroups.cpp:243-363): the
     // smallest existing colour unused by already-coloured element-mates,
@@ -189,7 +220,7 @@ bool build(int64_t V, const double* pos, const int32_t* fixed, const int32_t* co
         {
             uint32_t vId = nextNodes[i];
             uint64_t token = (uint64_t)vId + 1;
-            if(H == 0)
+            if(H == 0 && P == 0)
             {
                 // Tetrahedra only: the element-mates are the edge neighbours,
                 // and the neighbour list holds them in the order the element
@@ -207,8 +238,8 @@ bool build(int64_t V, const double* pos, const int32_t* fixed, const int32_t* co
             {
                 uint32_t ref = eref[r];
                 uint32_t id = ref >> 4;
-                const uint32_t* ev = (ref & 8) ? &hexs[8 * (size_t)id] : &tets[4 * (size_t)id];
-                int nv = (ref & 8) ? 8 : 4;
+                const uint32_t* ev = (ref & 8) ? &rec[8 * (size_t)id] : &tets[4 * (size_t)id];
+                int nv = (ref & 8) ? recordVerts(H, id) : 4;
                 for(int n = 0; n < nv; ++n)
                 {
                     int& g = groups[ev[n]];
@@ -245,10 +276,10 @@ bool build(int64_t V, const double* pos, const int32_t* fixed, const int32_t* co
     for(int64_t p = 0; p < V; ++p) indices[p] = keyed[p].second;
     out.permutation.assign(V, 0);
     for(int64_t p = 0; p < V; ++p) out.permutation[indices[p]] = p;
-    const std::vector<int64_t>& P = out.permutation;
+    const std::vector<int64_t>& Perm = out.permutation;
 
     // ---- Renumbering (clusterNodes, NodeGroups.cpp:452-619).
-    out.nverts = V; out.ntets = T; out.nhexs = H;
+    out.nverts = V; out.ntets = T; out.nhexs = H; out.npris = P;
     out.pos.resize(3 * V);
     out.fixed.resize(V);
     out.eoff.assign(V + 1, 0);
@@ -269,12 +300,12 @@ bool build(int64_t V, const double* pos, const int32_t* fixed, const int32_t* co
         std::copy(eref.begin() + eoff[old], eref.begin() + eoff[old + 1], out.eref.begin() + out.eoff[p]);
         const uint32_t* ln = &ntmp[(size_t)3 * eoff[old]];
         for(uint32_t i = 0; i < ncnt[old]; ++i)
-            out.nbr[out.noff[p] + i] = (uint32_t)P[ln[i]];
+            out.nbr[out.noff[p] + i] = (uint32_t)Perm[ln[i]];
     }
     out.tets.resize(4 * T);
-    for(int64_t i = 0; i < 4 * T; ++i) out.tets[i] = (uint32_t)P[tets[i]];
-    out.hexs.resize(8 * H);
-    for(int64_t i = 0; i < 8 * H; ++i) out.hexs[i] = (uint32_t)P[hexs[i]];
+    for(int64_t i = 0; i < 4 * T; ++i) out.tets[i] = (uint32_t)Perm[tets[i]];
+    out.hexs.resize(rec.size());
+    for(size_t i = 0; i < rec.size(); ++i) out.hexs[i] = (uint32_t)Perm[rec[i]];
 
     // ---- Groups (clusterNodes): movable nodes are sorted by type, then
     // colour, so a group's nodes in ascending new id are its boundary, then
@@ -311,10 +342,10 @@ bool build(int64_t V, const double* pos, const int32_t* fixed, const int32_t* co
 }
 
 bool buildGrouped(int64_t V, const double* pos, const int32_t* fixed, const int32_t* cons, int64_t T,
-                  const uint32_t* tets, int64_t H, const uint32_t* hexs, int64_t G, const int64_t* groupOff,
-                  const uint32_t* groupNodes, Topology& out, std::string& err)
+                  const uint32_t* tets, int64_t H, const uint32_t* hexs, int64_t P, const uint32_t* pris, int64_t G,
+                  const int64_t* groupOff, const uint32_t* groupNodes, Topology& out, std::string& err)
 {
-    if(!checkInputs(V, T, tets, H, hexs, err)) return false;
+    if(!checkInputs(V, T, tets, H, hexs, P, pris, err)) return false;
     if(G < 0) { err = "negative group count"; return false; }
     if(G > 0 && (!groupOff || (groupOff[G] > 0 && !groupNodes))) { err = "null group arrays"; return false; }
     if(G > 0 && groupOff[0] != 0) { err = "group_off[0] must be 0"; return false; }
@@ -329,10 +360,11 @@ bool buildGrouped(int64_t V, const double* pos, const int32_t* fixed, const int3
               std::to_string(V - nfixed) + " movable vertices)";
         return false;
     }
+    const std::vector<uint32_t> rec = makeRecords(H, hexs, P, pris);
     Adjacency adj;
-    buildAdjacency(V, T, tets, H, hexs, adj);
+    buildAdjacency(V, T, tets, H, P, rec.data(), adj);
     std::vector<uint8_t> types;
-    if(!computeTypes(V, fixed, cons, T, tets, H, hexs, adj, types, err)) return false;
+    if(!computeTypes(V, fixed, cons, T, tets, H, P, rec.data(), adj, types, err)) return false;
     // Membership: each movable vertex in exactly one group, no fixed vertex.
     std::vector<int32_t> groupOf(V, -1);
     for(int64_t g = 0; g < G; ++g)
@@ -346,27 +378,28 @@ bool buildGrouped(int64_t V, const double* pos, const int32_t* fixed, const int3
         }
     // Independence (NodeGroups' invariant): no two vertices of one group share
     // an element.
-    for(int64_t e = 0; e < T + H; ++e)
+    for(int64_t e = 0; e < T + H + P; ++e)
     {
-        const bool hex = e >= T;
-        const uint32_t* ev = hex ? &hexs[8 * (size_t)(e - T)] : &tets[4 * (size_t)e];
-        const int nv = hex ? 8 : 4;
+        const bool record = e >= T;
+        const uint32_t* ev = record ? &rec[8 * (size_t)(e - T)] : &tets[4 * (size_t)e];
+        const int nv = record ? recordVerts(H, (uint32_t)(e - T)) : 4;
         for(int a = 0; a < nv; ++a)
             for(int b = a + 1; b < nv; ++b)
                 if(ev[a] != ev[b] && groupOf[ev[a]] >= 0 && groupOf[ev[a]] == groupOf[ev[b]])
                 {
+                    const char* what = !record ? "tetrahedron " : e - T < H ? "hexahedron " : "prism ";
+                    const int64_t id = !record ? e : e - T < H ? e - T : e - T - H;
                     err = "vertices " + std::to_string(ev[a]) + " and " + std::to_string(ev[b]) + " of group " +
-                          std::to_string(groupOf[ev[a]]) + " share " + (hex ? "hexahedron " : "tetrahedron ") +
-                          std::to_string(hex ? e - T : e);
+                          std::to_string(groupOf[ev[a]]) + " share " + what + std::to_string(id);
                     return false;
                 }
     }
-    out.nverts = V; out.ntets = T; out.nhexs = H;
+    out.nverts = V; out.ntets = T; out.nhexs = H; out.npris = P;
     out.pos.assign(pos, pos + 3 * V);
     out.fixed.resize(V);
     for(int64_t v = 0; v < V; ++v) out.fixed[v] = fixed[v] ? 1 : 0;
     out.tets.assign(tets, tets + 4 * T);
-    out.hexs.assign(hexs, hexs + 8 * H);
+    out.hexs = rec;
     out.eoff = adj.eoff;
     out.eref = adj.eref;
     out.noff.assign(V + 1, 0);

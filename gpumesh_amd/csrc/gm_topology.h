@@ -23,13 +23,17 @@ enum { TYPE_FIXED = 0, TYPE_BOUNDARY = 1, TYPE_SUBSURFACE = 2, TYPE_INTERIOR = 3
 struct Topology
 {
     // Inputs after renumbering (new vertex ids).
-    int64_t nverts = 0, ntets = 0, nhexs = 0;
+    int64_t nverts = 0, ntets = 0, nhexs = 0, npris = 0;
     std::vector<double> pos;          // 3 * nverts
     std::vector<uint8_t> fixed;       // nverts
     std::vector<uint32_t> tets;       // 4 * ntets
-    std::vector<uint32_t> hexs;       // 8 * nhexs
+    // 8 * (nhexs + npris) "records": the hexahedra, then the prisms
+    // (v0 .. v5, then v5 twice), so both kinds share one 8-slot layout.
+    std::vector<uint32_t> hexs;
 
-    // Patch element lists: eref packs (element id << 4) | (is_hex << 3) | local index.
+    // Patch element lists: eref packs (element id << 4) | (is_record << 3) |
+    // local index; a record id below nhexs is a hexahedron, nhexs + p is
+    // prism p.
     std::vector<uint32_t> eoff, eref;
     // Neighbour vertex lists (edge neighbours, in discovery order).
     std::vector<uint32_t> noff, nbr;
@@ -68,9 +72,10 @@ struct Topology
 // Builds the topology.  `fixed[v]` != 0 marks a fixed vertex; `cons` (may
 // be null) gives each vertex's sliding constraint (an index >= 0, or -1);
 // every other vertex is free.  Returns false and sets `err` on invalid input.
+// npris prisms (6 vertices each, MeshPri order) may follow the hexahedra.
 bool build(int64_t nverts, const double* pos, const int32_t* fixed, const int32_t* cons,
            int64_t ntets, const uint32_t* tets, int64_t nhexs, const uint32_t* hexs,
-           Topology& out, std::string& err);
+           int64_t npris, const uint32_t* pris, Topology& out, std::string& err);
 
 // The topology in the caller's numbering with the caller's colour groups
 // (the reference's NodeGroups::parallelGroups): group g holds
@@ -80,8 +85,9 @@ bool build(int64_t nverts, const double* pos, const int32_t* fixed, const int32_
 // Each group's members are reordered by type (boundary, subsurface,
 // interior), keeping their order within a type.
 bool buildGrouped(int64_t nverts, const double* pos, const int32_t* fixed, const int32_t* cons, int64_t ntets,
-                  const uint32_t* tets, int64_t nhexs, const uint32_t* hexs, int64_t ngroups,
-                  const int64_t* groupOff, const uint32_t* groupNodes, Topology& out, std::string& err);
+                  const uint32_t* tets, int64_t nhexs, const uint32_t* hexs, int64_t npris, const uint32_t* pris,
+                  int64_t ngroups, const int64_t* groupOff, const uint32_t* groupNodes, Topology& out,
+                  std::string& err);
 
 // build() through the shared topology cache (gm_topocache.cpp): returns the
 // topology with `pos` left empty and the renumbered positions in
@@ -89,7 +95,8 @@ bool buildGrouped(int64_t nverts, const double* pos, const int32_t* fixed, const
 // (the default) every call builds.
 std::shared_ptr<const Topology> buildShared(int64_t nverts, const double* pos, const int32_t* fixed,
                                             const int32_t* cons, int64_t ntets, const uint32_t* tets, int64_t nhexs,
-                                            const uint32_t* hexs, std::vector<double>& renumbered, std::string& err);
+                                            const uint32_t* hexs, int64_t npris, const uint32_t* pris,
+                                            std::vector<double>& renumbered, std::string& err);
 void setCacheCapacity(size_t bytes);
 size_t cacheCapacity();
 // Whether the cache holds topology t.

@@ -181,26 +181,36 @@ static int checkConstraintIds(int64_t nverts, const int32_t* cons, const std::ve
     return GM_OK;
 }
 
+// Element kind of a whole mesh: a launch of one kind runs the kernels
+// specialised for it; prisms (and any mix) take the mixed kernels.
+static int meshKind(const gmt::Topology& t)
+{
+    if(t.nhexs == 0 && t.npris == 0) return gmd::EK_TET;
+    if(t.ntets == 0 && t.npris == 0) return gmd::EK_HEX;
+    return gmd::EK_MIXED;
+}
+
 int makeMesh(int64_t nverts, const double* pos, const int32_t* fixed, int64_t ntets, const uint32_t* tets,
              int64_t nhexs, const uint32_t* hexs, gm_mesh** out, const int32_t* cons,
-             const std::vector<double>* consTable)
+             const std::vector<double>* consTable, int64_t npris, const uint32_t* pris)
 {
     if(!out) return fail(GM_ERR_ARG, "out is null");
     *out = nullptr;
-    if(nverts < 0 || ntets < 0 || nhexs < 0) return fail(GM_ERR_ARG, "negative size");
-    if((nverts && (!pos || !fixed)) || (ntets && !tets) || (nhexs && !hexs)) return fail(GM_ERR_ARG, "null array");
+    if(nverts < 0 || ntets < 0 || nhexs < 0 || npris < 0) return fail(GM_ERR_ARG, "negative size");
+    if((nverts && (!pos || !fixed)) || (ntets && !tets) || (nhexs && !hexs) || (npris && !pris))
+        return fail(GM_ERR_ARG, "null array");
     if(int rc = checkConstraintIds(nverts, cons, consTable)) return rc;
     gm_mesh* m = new gm_mesh();
     if(consTable) m->consTable = *consTable;
     std::string err;
-    m->topo = gmt::buildShared(nverts, pos, fixed, cons, ntets, tets, nhexs, hexs, m->hostPos, err);
+    m->topo = gmt::buildShared(nverts, pos, fixed, cons, ntets, tets, nhexs, hexs, npris, pris, m->hostPos, err);
     if(!m->topo)
     {
         delete m;
         return fail(GM_ERR_ARG, err);
     }
     for(int32_t p : m->topo->groupMaxPatch) m->maxPatch = std::max(m->maxPatch, (int)p);
-    m->kind = m->topo->nhexs == 0 ? gmd::EK_TET : (m->topo->ntets == 0 ? gmd::EK_HEX : gmd::EK_MIXED);
+    m->kind = meshKind(*m->topo);
     m->crew.scaling = m->crew.scaling2 = m->crew.scaling3 = 1.0;
     setAnalyticField(m->crew, m->aspect);
     *out = m;
@@ -264,36 +274,45 @@ uint64_t texelHash(const float* texels, size_t n)
 int makeMeshGrouped(int64_t nverts, const double* pos, const int32_t* fixed, int64_t ntets, const uint32_t* tets,
                     int64_t nhexs, const uint32_t* hexs, int64_t ngroups, const int64_t* groupOff,
                     const uint32_t* groupNodes, gm_mesh** out, const int32_t* cons,
-                    const std::vector<double>* consTable)
+                    const std::vector<double>* consTable, int64_t npris, const uint32_t* pris)
 {
     if(!out) return fail(GM_ERR_ARG, "out is null");
     *out = nullptr;
-    if(nverts < 0 || ntets < 0 || nhexs < 0) return fail(GM_ERR_ARG, "negative size");
-    if((nverts && (!pos || !fixed)) || (ntets && !tets) || (nhexs && !hexs)) return fail(GM_ERR_ARG, "null array");
+    if(nverts < 0 || ntets < 0 || nhexs < 0 || npris < 0) return fail(GM_ERR_ARG, "negative size");
+    if((nverts && (!pos || !fixed)) || (ntets && !tets) || (nhexs && !hexs) || (npris && !pris))
+        return fail(GM_ERR_ARG, "null array");
     if(int rc = checkConstraintIds(nverts, cons, consTable)) return rc;
     auto t = std::make_shared<gmt::Topology>();
     std::string err;
-    if(!gmt::buildGrouped(nverts, pos, fixed, cons, ntets, tets, nhexs, hexs, ngroups, groupOff, groupNodes, *t, err))
+    if(!gmt::buildGrouped(nverts, pos, fixed, cons, ntets, tets, nhexs, hexs, npris, pris, ngroups, groupOff,
+                          groupNodes, *t, err))
         return fail(GM_ERR_ARG, err);
     gm_mesh* m = new gm_mesh();
     if(consTable) m->consTable = *consTable;
     m->hostPos.swap(t->pos);
     m->topo = t;
     for(int32_t p : m->topo->groupMaxPatch) m->maxPatch = std::max(m->maxPatch, (int)p);
-    m->kind = m->topo->nhexs == 0 ? gmd::EK_TET : (m->topo->ntets == 0 ? gmd::EK_HEX : gmd::EK_MIXED);
+    m->kind = meshKind(*m->topo);
     m->crew.scaling = m->crew.scaling2 = m->crew.scaling3 = 1.0;
     setAnalyticField(m->crew, m->aspect);
     *out = m;
     return GM_OK;
 }
 
+// Entries of the GETMe corner-slot table: 4 per tetrahedron, 8 per record.
+static size_t cornerSlotCount(const gmt::Topology& t)
+{
+    return std::max<size_t>(4 * (size_t)t.ntets + 8 * (size_t)(t.nhexs + t.npris), 1);
+}
+
 size_t residentBytes(const gm_mesh* m, int records, bool getme)
 {
     const gmt::Topology& t = *m->topo;
     const size_t V = (size_t)t.nverts;
-    const uint32_t eb = gmd::evalBlocksFor((uint64_t)t.ntets + (uint64_t)t.nhexs);
+    const uint64_t elemRecords = (uint64_t)t.nhexs + (uint64_t)t.npris;
+    const uint32_t eb = gmd::evalBlocksFor((uint64_t)t.ntets + elemRecords);
     size_t b = allocBytes<double4>(V) + allocBytes<float4>(V) + allocBytes<uint4>((size_t)t.ntets) +
-               allocBytes<uint4>(2 * (size_t)t.nhexs) + 2 * allocBytes<uint32_t>(V + 1) +
+               allocBytes<uint4>(2 * (size_t)elemRecords) + 2 * allocBytes<uint32_t>(V + 1) +
                allocBytes<uint32_t>(t.eref.size()) + allocBytes<uint32_t>(t.nbr.size()) + allocBytes<uint8_t>(V) +
                2 * allocBytes<int32_t>(V) + allocBytes<uint32_t>(V) +
                allocBytes<double>(3 * (size_t)eb) + allocBytes<int32_t>(eb) + allocBytes<int32_t>(20) +
@@ -302,8 +321,8 @@ size_t residentBytes(const gm_mesh* m, int records, bool getme)
     if(!t.cons.empty()) b += allocBytes<int32_t>(V) + allocBytes<double>(m->consTable.size());
     if(getme)
         b += sizeClass(std::max<size_t>(32, t.eref.size() * sizeof(double4))) +
-             sizeClass(std::max<size_t>(32, std::max<size_t>(t.eref.size(), 1) * sizeof(uint32_t))) +
-             sizeClass(std::max<size_t>(32, ((size_t)t.ntets + (size_t)t.nhexs) * sizeof(double)));
+             sizeClass(std::max<size_t>(32, cornerSlotCount(t) * sizeof(uint32_t))) +
+             sizeClass(std::max<size_t>(32, ((size_t)t.ntets + (size_t)elemRecords) * sizeof(double)));
     return b;
 }
 
@@ -384,7 +403,7 @@ static std::shared_ptr<const DeviceLayout> buildLayout(const gmt::Topology& t)
     L->eref.resize(t.eref.size());
     L->nbr.resize(t.nbr.size());
     L->fixed.resize(V);
-    L->cornerSlot.assign(std::max<size_t>(t.eref.size(), 1), 0);
+    L->cornerSlot.assign(cornerSlotCount(t), 0);
     for(size_t s = 0; s < V; ++s)
     {
         const size_t v = vertexAt[s];
@@ -403,7 +422,8 @@ static std::shared_ptr<const DeviceLayout> buildLayout(const gmt::Topology& t)
             L->eref[at] = ref;
             const size_t id = ref >> 4, k = ref & 7;
             // The patch entry of each element corner (GETMe's element step
-            // writes there): tets 4 e + k, then hexes 4 T + 8 h + k.
+            // writes there): tets 4 e + k, then records (hexahedra, then
+            // prisms) 4 T + 8 r + k.
             L->cornerSlot[(ref & 8) ? 4 * (size_t)t.ntets + 8 * id + k : 4 * id + k] = at;
         }
         L->noff[s + 1] = L->noff[s] + (t.noff[v + 1] - t.noff[v]);
@@ -456,9 +476,10 @@ static hipError_t deviceTopoOf(const DeviceLayout& L, const gmt::Topology& t, in
     e = devAlloc(&D->tets, (size_t)t.ntets);
     if(e == hipSuccess && t.ntets)
         e = hipMemcpyAsync(D->tets, L.tets.data(), (size_t)t.ntets * 16, hipMemcpyHostToDevice, s);
-    if(e == hipSuccess) e = devAlloc(&D->hexs, 2 * (size_t)t.nhexs);
-    if(e == hipSuccess && t.nhexs)
-        e = hipMemcpyAsync(D->hexs, L.hexs.data(), (size_t)t.nhexs * 32, hipMemcpyHostToDevice, s);
+    const size_t records = (size_t)(t.nhexs + t.npris);
+    if(e == hipSuccess) e = devAlloc(&D->hexs, 2 * records);
+    if(e == hipSuccess && records)
+        e = hipMemcpyAsync(D->hexs, L.hexs.data(), records * 32, hipMemcpyHostToDevice, s);
     put(&D->eoff, L.eoff, V + 1);
     put(&D->eref, L.eref, L.eref.size());
     put(&D->noff, L.noff, V + 1);
@@ -561,7 +582,8 @@ int uploadMesh(gm_mesh* m)
     d.nverts = (uint32_t)t.nverts;
     d.ntets = (uint32_t)t.ntets;
     d.nhexs = (uint32_t)t.nhexs;
-    d.evalBlocks = gmd::evalBlocksFor((uint64_t)t.ntets + (uint64_t)t.nhexs);
+    d.npris = (uint32_t)t.npris;
+    d.evalBlocks = gmd::evalBlocksFor((uint64_t)t.ntets + (uint64_t)t.nhexs + (uint64_t)t.npris);
     std::shared_ptr<const DeviceLayout> L = layoutOf(t);
     if(!L) return bail(fail(GM_ERR_ARG, "colour groups do not cover the movable vertices"));
     m->layout = L;
@@ -762,9 +784,10 @@ static int buildPlanInto(Plan* p, const std::vector<gm_mesh*>& meshes)
                 if(rc) return rc;
                 if(!step.empty()) p->getmeSteps.push_back(std::move(step));
             }
-    size_t fTet, fHex, fEval, fVert;
+    size_t fTet, fHex, fPri, fEval, fVert;
     p->tetBlocks = blockSpans(meshes, spans, fTet, [](const gm_mesh* m) { return ceilDiv(m->topo->ntets, gmd::ELEM_BLOCK); });
     p->hexBlocks = blockSpans(meshes, spans, fHex, [](const gm_mesh* m) { return ceilDiv(m->topo->nhexs, gmd::ELEM_BLOCK); });
+    p->priBlocks = blockSpans(meshes, spans, fPri, [](const gm_mesh* m) { return ceilDiv(m->topo->npris, gmd::ELEM_BLOCK); });
     p->evalBlocks = blockSpans(meshes, spans, fEval, [](const gm_mesh* m) { return m->dm.evalBlocks; });
     p->vertBlocks = blockSpans(meshes, spans, fVert, [](const gm_mesh* m) { return ceilDiv(m->topo->nverts, gmd::VERT_BLOCK); });
     std::vector<uint32_t> ml(meshes.size());
@@ -782,6 +805,7 @@ static int buildPlanInto(Plan* p, const std::vector<gm_mesh*>& meshes)
         for(VList& l : step) l.segs = p->spans + l.firstSpan;
     p->tetBlocks.spans = p->spans + fTet;
     p->hexBlocks.spans = p->spans + fHex;
+    p->priBlocks.spans = p->spans + fPri;
     p->evalBlocks.spans = p->spans + fEval;
     p->vertBlocks.spans = p->spans + fVert;
     GMR_HIP(devAlloc(&p->dTable, meshes.size()));
@@ -797,7 +821,7 @@ int buildPlan(const std::vector<gm_mesh*>& meshes, hipStream_t stream, Plan** ou
     if(meshes.empty()) return fail(GM_ERR_ARG, "a plan needs at least one mesh");
     for(const gm_mesh* m : meshes)
         if((uint64_t)m->topo->nverts >= (1ull << 32) || (uint64_t)m->topo->ntets >= (1ull << 32) ||
-           (uint64_t)m->topo->nhexs >= (1ull << 32))
+           (uint64_t)m->topo->nhexs + (uint64_t)m->topo->npris >= (1ull << 32))
             return fail(GM_ERR_ARG, "mesh too large for 32-bit work items");
     if(meshes.size() >= (1ull << 32)) return fail(GM_ERR_ARG, "too many meshes");
     Plan* p = new Plan();
@@ -861,7 +885,7 @@ static int ensureGetme(gm_mesh* m)
     const gmt::Topology& t = *m->topo;
     const size_t N = t.eref.size();
     const size_t pullBytes = N * sizeof(double4);
-    const size_t qualBytes = ((size_t)t.ntets + (size_t)t.nhexs) * sizeof(double);
+    const size_t qualBytes = ((size_t)t.ntets + (size_t)t.nhexs + (size_t)t.npris) * sizeof(double);
     GMR_HIP(cornerSlotsOf(*m->layout, *m->devTopo, m->stream));
     m->dm.cornerSlot = m->devTopo->cornerSlot;
     GMR_HIP(poolAlloc(&m->dm.getmePull, std::max<size_t>(32, pullBytes)));
@@ -935,14 +959,22 @@ static void collectTimings(Plan* p)
     p->poolUsed = 0;
 }
 
+bool planHasRecords(const Plan* p)
+{
+    for(const gm_mesh* m : p->meshes)
+        if(m->topo->nhexs + m->topo->npris > 0) return true;
+    return false;
+}
+
 static hipError_t evaluateAll(Plan* p, const gmq::Crew& c, int ev, int sa, bool fromF32, bool timed)
 {
     int slot = timedBegin(p, timed, GM_KT_EVALUATE);
+    const bool records = planHasRecords(p);
     hipError_t e = fromF32
-        ? gmd::launchEvaluate<float>(p->dTable, p->evalBlocks, p->meshList, p->nMeshes, c, ev, sa,
-                                     p->hexBlocks.count != 0, p->stream)
-        : gmd::launchEvaluate<double>(p->dTable, p->evalBlocks, p->meshList, p->nMeshes, c, ev, sa,
-                                      p->hexBlocks.count != 0, p->stream);
+        ? gmd::launchEvaluate<float>(p->dTable, p->evalBlocks, p->meshList, p->nMeshes, c, ev, sa, records,
+                                     p->stream)
+        : gmd::launchEvaluate<double>(p->dTable, p->evalBlocks, p->meshList, p->nMeshes, c, ev, sa, records,
+                                      p->stream);
     timedEnd(p, slot);
     return e;
 }
@@ -957,8 +989,10 @@ static hipError_t sweep(Plan* p, const gmq::Crew& c, int ev, int sa, const RunPa
     if(getme)
     {
         int slot = timedBegin(p, timed, f64 ? GM_KT_GETME_ELEM_F64 : GM_KT_GETME_ELEM_F32);
-        e = f64 ? gmd::launchGetmeElements<double>(p->dTable, p->tetBlocks, p->hexBlocks, c, ev, sa, p->stream)
-                : gmd::launchGetmeElements<float>(p->dTable, p->tetBlocks, p->hexBlocks, c, ev, sa, p->stream);
+        e = f64 ? gmd::launchGetmeElements<double>(p->dTable, p->tetBlocks, p->hexBlocks, p->priBlocks, c, ev, sa,
+                                                   p->stream)
+                : gmd::launchGetmeElements<float>(p->dTable, p->tetBlocks, p->hexBlocks, p->priBlocks, c, ev, sa,
+                                                  p->stream);
         timedEnd(p, slot);
     }
     // GETMe always sweeps every group, in serial order when members slide.

@@ -30,48 +30,56 @@ class GpuMesh:
     """A mesh uploaded to the current HIP device (or, with device=False, its
     host-side topology only).  Vertex ids are the renumbered ones."""
 
-    def __init__(self, pos, fixed, tets, hexs, device: bool = True):
+    def __init__(self, pos, fixed, tets, hexs, device: bool = True, pris=None):
+        """pris: prisms, 6 vertices each (MeshPri order), or None."""
         L = lib()
         pos = np.ascontiguousarray(pos, np.float64).reshape(-1, 3)
         fixed = np.ascontiguousarray(fixed, np.int32).reshape(-1)
         tets = np.ascontiguousarray(tets, np.uint32).reshape(-1, 4)
         hexs = np.ascontiguousarray(hexs, np.uint32).reshape(-1, 8)
+        if pris is not None and len(pris):
+            self._create_desc(pos, fixed, tets, hexs, None, [], None, device, pris)
+            return
         h = C.c_void_p()
         fn = L.gm_mesh_create if device else L.gm_mesh_create_host
         check(fn(pos.shape[0], ptr(pos), ptr(fixed), tets.shape[0], ptr(tets), hexs.shape[0], ptr(hexs),
                  C.byref(h)), "gm_mesh_create" if device else "gm_mesh_create_host")
         self._h = h
         self.device = device
+        self._read_info()
+
+    def _read_info(self):
+        L = lib()
         info = np.zeros(8, np.int64)
         check(L.gm_mesh_info(self._h, ptr(info)), "gm_mesh_info")
         (self.nverts, self.ntets, self.nhexs, self.nmovable, self.ngroups, self.sum_patch,
          self.sum_nbr, self.max_patch) = (int(x) for x in info)
+        n = C.c_int64(0)
+        check(L.gm_mesh_prisms(self._h, C.byref(n), None), "gm_mesh_prisms")
+        self.npris = int(n.value)
 
-    @classmethod
-    def constrained(cls, pos, fixed, tets, hexs, constraint, constraints, groups=None, device: bool = True):
-        """A mesh whose vertices may slide on planes or segments
-        (gm_mesh_create_desc): constraint[v] indexes `constraints` (rows
-        (kind, a, b), kind 1 plane through a with normal b, 2 segment a-b) or
-        is -1; groups = (group_off, group_nodes) keeps the caller's numbering
-        and colouring, None colours and renumbers."""
+    def _create_desc(self, pos, fixed, tets, hexs, constraint, constraints, groups, device, pris):
         L = lib()
-        self = cls.__new__(cls)
         pos = np.ascontiguousarray(pos, np.float64).reshape(-1, 3)
         fixed = np.ascontiguousarray(fixed, np.int32).reshape(-1)
         tets = np.ascontiguousarray(tets, np.uint32).reshape(-1, 4)
         hexs = np.ascontiguousarray(hexs, np.uint32).reshape(-1, 8)
-        cons = np.ascontiguousarray(constraint, np.int32).reshape(-1)
+        pris = np.ascontiguousarray(np.zeros((0, 6)) if pris is None else pris, np.uint32).reshape(-1, 6)
         table = (_lib.Constraint * max(len(constraints), 1))()
         for k, (kind, a, b) in enumerate(constraints):
             table[k].kind = int(kind)
             for i in range(3):
                 table[k].a[i] = float(a[i])
                 table[k].b[i] = float(b[i])
-        keep = [pos, fixed, tets, hexs, cons]
+        keep = [pos, fixed, tets, hexs, pris]
         d = _lib.MeshDesc()
         d.nverts, d.pos, d.fixed = pos.shape[0], ptr(pos), ptr(fixed)
         d.ntets, d.tets, d.nhexs, d.hexs = tets.shape[0], ptr(tets), hexs.shape[0], ptr(hexs)
-        d.constraint, d.nconstraints, d.constraints = ptr(cons), len(constraints), C.addressof(table)
+        d.npris, d.pris = pris.shape[0], ptr(pris)
+        if constraint is not None:
+            cons = np.ascontiguousarray(constraint, np.int32).reshape(-1)
+            keep.append(cons)
+            d.constraint, d.nconstraints, d.constraints = ptr(cons), len(constraints), C.addressof(table)
         d.ngroups = -1
         if groups is not None:
             off = np.ascontiguousarray(groups[0], np.int64)
@@ -84,10 +92,18 @@ class GpuMesh:
         del keep
         self._h = h
         self.device = device
-        info = np.zeros(8, np.int64)
-        check(L.gm_mesh_info(self._h, ptr(info)), "gm_mesh_info")
-        (self.nverts, self.ntets, self.nhexs, self.nmovable, self.ngroups, self.sum_patch,
-         self.sum_nbr, self.max_patch) = (int(x) for x in info)
+        self._read_info()
+
+    @classmethod
+    def constrained(cls, pos, fixed, tets, hexs, constraint, constraints, groups=None, device: bool = True,
+                    pris=None):
+        """A mesh whose vertices may slide on planes or segments
+        (gm_mesh_create_desc): constraint[v] indexes `constraints` (rows
+        (kind, a, b), kind 1 plane through a with normal b, 2 segment a-b) or
+        is -1; groups = (group_off, group_nodes) keeps the caller's numbering
+        and colouring, None colours and renumbers; pris as in the constructor."""
+        self = cls.__new__(cls)
+        self._create_desc(pos, fixed, tets, hexs, constraint, constraints, groups, device, pris)
         return self
 
     @classmethod
@@ -108,10 +124,7 @@ class GpuMesh:
               "gm_mesh_create_grouped")
         self._h = h
         self.device = device
-        info = np.zeros(8, np.int64)
-        check(L.gm_mesh_info(self._h, ptr(info)), "gm_mesh_info")
-        (self.nverts, self.ntets, self.nhexs, self.nmovable, self.ngroups, self.sum_patch,
-         self.sum_nbr, self.max_patch) = (int(x) for x in info)
+        self._read_info()
         return self
 
     def device_bytes(self) -> int:
@@ -122,7 +135,13 @@ class GpuMesh:
 
     @classmethod
     def from_cube(cls, m, device: bool = True):
-        return cls(m.pos, m.fixed, m.tets, m.hexs, device=device)
+        return cls(m.pos, m.fixed, m.tets, m.hexs, device=device, pris=getattr(m, "pris", None))
+
+    def prisms(self) -> np.ndarray:
+        """Renumbered prisms (P, 6)."""
+        out = np.zeros((max(self.npris, 1), 6), np.uint32)
+        check(lib().gm_mesh_prisms(self._h, None, ptr(out)), "gm_mesh_prisms")
+        return out[: self.npris]
 
     def close(self) -> None:
         if getattr(self, "_h", None):
@@ -190,12 +209,13 @@ class GpuMesh:
     def evaluate(self, element_qualities: bool = False):
         out = np.zeros(3, np.float64)
         hist = np.zeros(20, np.int32)
-        q = np.zeros(max(self.ntets + self.nhexs, 1), np.float64) if element_qualities else None
+        ne = self.ntets + self.npris + self.nhexs
+        q = np.zeros(max(ne, 1), np.float64) if element_qualities else None
         check(lib().gm_mesh_evaluate(self._h, ptr(out), ptr(hist), ptr(q)), "gm_mesh_evaluate")
         r = {"min": float(out[0]), "inv_sum": float(out[1]), "n": int(out[2]),
              "mean": float(out[2] / out[1]) if out[1] != 0 else float("nan"), "hist": hist}
         if element_qualities:
-            r["elem_qual"] = q[: self.ntets + self.nhexs]
+            r["elem_qual"] = q[:ne]
         return r
 
     def smooth_groups(self, smoother: str, group_begin: int, group_end: int, precision: str = "fp64",

@@ -138,6 +138,10 @@ typedef struct gm_mesh_desc
     const int64_t* group_off;
     const uint32_t* group_nodes;
     int device_copy;              /* 0: host topology only */
+    /* Prisms (MeshPri, DataStructures/Mesh.h): 6 vertices each, the first
+     * triangle then the one above it; npris = 0 for none. */
+    int64_t npris;
+    const uint32_t* pris;         /* 6 npris */
 } gm_mesh_desc;
 
 /* gm_mesh_create / gm_mesh_create_grouped from a description, with sliding
@@ -155,8 +159,13 @@ int gm_mesh_permutation(const gm_mesh* mesh, int64_t* perm);
 int gm_mesh_groups(const gm_mesh* mesh, int64_t* group_off, uint32_t* group_nodes);
 /* Renumbered connectivity: tets[4*ntets], hexs[8*nhexs] (either may be NULL). */
 int gm_mesh_elements(const gm_mesh* mesh, uint32_t* tets, uint32_t* hexs);
+/* Renumbered prisms: *npris (may be NULL) and pris[6*npris] (may be NULL).
+ * Replaces reading Mesh::pris after NodeGroups::clusterNodes
+ * (DataStructures/NodeGroups.cpp:452-619). */
+int gm_mesh_prisms(const gm_mesh* mesh, int64_t* npris, uint32_t* pris);
 /* Patch lists and neighbour lists (MeshTopo::neighborElems / neighborVerts):
- * eoff[nverts+1], eref[sum patch] = (elem id << 4) | (is_hex << 3) | local index,
+ * eoff[nverts+1], eref[sum patch] = (elem id << 4) | (is_record << 3) | local index,
+ * where a record id below nhexs is a hexahedron and nhexs + p is prism p,
  * noff[nverts+1], nbr[sum neighbours]. */
 int gm_mesh_topology(const gm_mesh* mesh, uint32_t* eoff, uint32_t* eref, uint32_t* noff, uint32_t* nbr);
 

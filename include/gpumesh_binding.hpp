@@ -73,7 +73,7 @@ template<class MeshT> struct MeshArrays
     std::vector<int32_t> fixed;
     std::vector<int32_t> cons;                // constraint table index per vertex, -1 for none
     std::vector<gm_constraint> table;
-    std::vector<uint32_t> tets, hexs;
+    std::vector<uint32_t> tets, hexs, pris;
     std::vector<int64_t> groupOff;
     std::vector<uint32_t> groupNodes;
 
@@ -105,6 +105,8 @@ template<class MeshT> struct MeshArrays
         for(const auto& t : mesh.tets) tets.insert(tets.end(), t.v, t.v + 4);
         hexs.reserve(8 * mesh.hexs.size());
         for(const auto& h : mesh.hexs) hexs.insert(hexs.end(), h.v, h.v + 8);
+        pris.reserve(6 * mesh.pris.size());
+        for(const auto& p : mesh.pris) pris.insert(pris.end(), p.v, p.v + 6);
         groupOff.push_back(0);
         for(const auto& g : mesh.nodeGroups().parallelGroups())
         {
@@ -133,6 +135,8 @@ public:
         d.tets = a.tets.data();
         d.nhexs = (int64_t)mesh.hexs.size();
         d.hexs = a.hexs.data();
+        d.npris = (int64_t)mesh.pris.size();
+        d.pris = a.pris.data();
         d.constraint = a.table.empty() ? nullptr : a.cons.data();
         d.nconstraints = (int64_t)a.table.size();
         d.constraints = a.table.empty() ? nullptr : a.table.data();

@@ -51,6 +51,7 @@ struct SegmentConstraint : AbstractConstraint
 struct MeshTopo { const AbstractConstraint* snapToBoundary; };
 struct MeshTet { uint32_t v[4]; };
 struct MeshHex { uint32_t v[8]; };
+struct MeshPri { uint32_t v[6]; };
 struct ParallelGroup { std::vector<uint32_t> undispatchedNodes; };
 struct NodeGroups
 {
@@ -62,6 +63,7 @@ struct Mesh
     std::vector<MeshVert> verts;
     std::vector<MeshTopo> topos;
     std::vector<MeshTet> tets;
+    std::vector<MeshPri> pris;
     std::vector<MeshHex> hexs;
     NodeGroups groups;
     const NodeGroups& nodeGroups() const { return groups; }

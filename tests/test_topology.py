@@ -151,3 +151,59 @@ def test_topology_prepare_host_only():
         assert np.array_equal(g.permutation(), ref.permutation())
     finally:
         set_topology_cache(0)
+
+
+def _compare_with_prisms(oracle, pos, fixed, tets, hexs, pris):
+    om = oracle.OracleMesh(pos, fixed, tets, hexs, pris=pris)
+    gm = GpuMesh(pos, fixed, tets, hexs, device=False, pris=pris)
+    assert gm.npris == len(pris)
+    assert np.array_equal(om.permutation(), gm.permutation())
+    eo1, er1, no1, nb1 = om.topology()
+    eo2, er2, no2, nb2 = gm.topology()
+    # oracle: (id << 5) | (type << 3) | local with type 0 tet, 2 prism, 3 hex;
+    # product: (record id << 4) | (is_record << 3) | local, prism p = record H + p.
+    typ = (er1 >> np.uint64(3)) & np.uint64(3)
+    oid = er1 >> np.uint64(5)
+    rid = np.where(typ == 2, oid + np.uint64(len(hexs)), oid)
+    er1c = (rid << np.uint64(4)) | ((typ >= 2).astype(np.uint64) << np.uint64(3)) | (er1 & np.uint64(7))
+    assert np.array_equal(eo1, eo2.astype(np.int64))
+    assert np.array_equal(er1c.astype(np.uint32), er2)
+    assert np.array_equal(no1, no2.astype(np.int64))
+    assert np.array_equal(nb1, nb2)
+    t1, h1 = om.elements()
+    t2, h2 = gm.elements()
+    assert np.array_equal(t1, t2) and np.array_equal(h1, h2)
+    assert np.array_equal(om.prisms(), gm.prisms())
+    groups, tr = om.groups()
+    off, nodes = gm.groups()
+    assert gm.ngroups == len(groups)
+    nfixed = tr[0, 1]
+    for g in range(gm.ngroups):
+        if groups[g, 5] > groups[g, 4]:
+            assert off[g] + nfixed == groups[g, 4] and off[g + 1] + nfixed == groups[g, 5]
+        else:
+            assert off[g] == off[g + 1]
+
+
+@pytest.mark.parametrize("layout,n,seed", [("prism", 1, 1), ("prism", 5, 2), ("mixed", 6, 3), ("mixed", 9, 4)])
+def test_prism_meshes(oracle, layout, n, seed):
+    from gpumesh_amd.meshgen import element_cube_mesh
+    m = element_cube_mesh(n, layout, 0.3, seed)
+    _compare_with_prisms(oracle, m.pos, m.fixed, m.tets, m.hexs, m.pris)
+
+
+def test_shuffled_prism_mesh(oracle):
+    """Random vertex ids, random element order and free boundary vertices
+    with prisms between the tetrahedra and hexahedra."""
+    from gpumesh_amd.meshgen import element_cube_mesh
+    rng = np.random.default_rng(5)
+    m = element_cube_mesh(6, "mixed", 0.2, 8)
+    perm = rng.permutation(m.nverts)
+    inv = np.empty_like(perm)
+    inv[perm] = np.arange(m.nverts)
+    pos = m.pos[perm]
+    fixed = (rng.uniform(size=m.nverts) < 0.2).astype(np.int32)
+    tets = inv[m.tets][rng.permutation(len(m.tets))].astype(np.uint32)
+    hexs = inv[m.hexs][rng.permutation(len(m.hexs))].astype(np.uint32)
+    pris = inv[m.pris][rng.permutation(len(m.pris))].astype(np.uint32)
+    _compare_with_prisms(oracle, pos, fixed, tets, hexs, pris)
