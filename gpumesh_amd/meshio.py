@@ -21,11 +21,13 @@ to a boundary constraint rather than to the stored position
 17 significant digits by default (exact for doubles) and
 `reference_digits=True` reproduces the reference's 6; `read_json` keeps the
 stored position of a fixed vertex.  Faces and edges of a sliding boundary
-(positive ids) and prisms are outside this library's smoothing path and are
-refused.
+(positive ids) are refused here (the library takes them through
+gm_mesh_create_desc with explicit planes and segments).  Prisms are read and
+written.
 
-Job files (`encode_job` / `decode_job`, the C ABI's gm_job_*) carry a mesh
-and its smoothing request bit for bit, for queues and the serving loop.
+Job files (`encode_job` / `decode_job`, the C ABI's gm_job_*) carry a
+tetrahedron/hexahedron mesh and its smoothing request bit for bit, for queues
+and the serving loop.
 """
 from __future__ import annotations
 
@@ -49,6 +51,7 @@ class MeshData:
     hexs: np.ndarray         # (H, 8) uint32
     model: str = ""
     boundary: str = ""
+    pris: np.ndarray = None  # (P, 6) uint32
 
     @property
     def nverts(self) -> int:
@@ -62,13 +65,16 @@ def _num(x: float, digits: int) -> str:
 
 
 def write_json(path: str, m, model: str = "", boundary: str = "", reference_digits: bool = False) -> None:
-    """Writes `m` (anything with pos, fixed, tets, hexs) in the reference's
-    JSON layout; 17 significant digits unless reference_digits."""
+    """Writes `m` (anything with pos, fixed, tets, hexs and optionally pris)
+    in the reference's JSON layout; 17 significant digits unless
+    reference_digits."""
     digits = 6 if reference_digits else 17
     pos = np.asarray(m.pos, np.float64).reshape(-1, 3)
     fixed = np.asarray(m.fixed).reshape(-1)
     tets = np.asarray(m.tets, np.int64).reshape(-1, 4)
     hexs = np.asarray(m.hexs, np.int64).reshape(-1, 8)
+    pris = getattr(m, "pris", None)
+    pris = np.zeros((0, 6), np.int64) if pris is None else np.asarray(pris, np.int64).reshape(-1, 6)
     ind = "    "
     out = ["{\n", f'{ind}"Model": "{model}",\n', f'{ind}"Boundary": "{boundary}",\n', f'{ind}"verts": [\n']
     n = pos.shape[0]
@@ -78,7 +84,7 @@ def write_json(path: str, m, model: str = "", boundary: str = "", reference_digi
     out.append(f"{ind}],\n{ind}\"topos\": [\n")
     for i, f in enumerate(fixed):
         out.append(f"{ind}{ind}{-1 if f else 0}" + ("," if i != n - 1 else "") + "\n")
-    for tag, arr in (("tets", tets), ("pris", np.zeros((0, 6), np.int64)), ("hexs", hexs)):
+    for tag, arr in (("tets", tets), ("pris", pris), ("hexs", hexs)):
         out.append(f"{ind}],\n{ind}\"{tag}\": [\n")
         k = arr.shape[0]
         for i, row in enumerate(arr):
@@ -102,15 +108,14 @@ def read_json(path: str) -> MeshData:
     if np.any(topos > 0):
         raise ValueError("sliding boundary constraints (positive topo ids) are not supported: only fixed "
                          "(negative) and free (0) vertices")
-    if len(doc.get("pris", [])):
-        raise ValueError("prisms are not supported")
     tets = np.asarray(doc.get("tets", []), np.int64).reshape(-1, 4)
+    pris = np.asarray(doc.get("pris", []), np.int64).reshape(-1, 6)
     hexs = np.asarray(doc.get("hexs", []), np.int64).reshape(-1, 8)
-    for name, arr in (("tets", tets), ("hexs", hexs)):
+    for name, arr in (("tets", tets), ("pris", pris), ("hexs", hexs)):
         if arr.size and (arr.min() < 0 or arr.max() >= pos.shape[0]):
             raise ValueError(f"{name} reference a vertex outside verts")
     return MeshData(pos, (topos < 0).astype(np.int32), tets.astype(np.uint32), hexs.astype(np.uint32),
-                    str(doc.get("Model", "")), str(doc.get("Boundary", "")))
+                    str(doc.get("Model", "")), str(doc.get("Boundary", "")), pris.astype(np.uint32))
 
 
 def encode_job(m, smoother: str, passes: int, precision: str = "fp32_polish") -> bytes:

@@ -94,12 +94,23 @@ def test_reference_digits_match_cxx_ostream(tmp_path):
 
 def test_refuses_what_the_smoother_cannot_take(tmp_path):
     p = tmp_path / "bad.json"
-    p.write_text(REFERENCE_LAYOUT.replace('"pris": [\n    ]', '"pris": [\n        [0, 1, 2, 3, 4, 0]\n    ]'))
+    p.write_text(REFERENCE_LAYOUT.replace('"pris": [\n    ]', '"pris": [\n        [0, 1, 2, 3, 4, 99]\n    ]'))
     with pytest.raises(ValueError):
-        read_json(str(p))
+        read_json(str(p))                                                  # a prism vertex outside verts
     p.write_text(REFERENCE_LAYOUT.replace("        0,\n", "        3,\n"))     # a face constraint id
     with pytest.raises(ValueError):
         read_json(str(p))
+
+
+def test_prism_mesh_round_trip_is_exact(tmp_path):
+    from gpumesh_amd.meshgen import element_cube_mesh
+    m = element_cube_mesh(3, "mixed", 0.3, 4)
+    p = tmp_path / "m.json"
+    write_json(str(p), m)
+    r = read_json(str(p))
+    assert np.array_equal(r.pos.view(np.uint64), m.pos.view(np.uint64))
+    assert np.array_equal(r.fixed, m.fixed)
+    assert np.array_equal(r.tets, m.tets) and np.array_equal(r.hexs, m.hexs) and np.array_equal(r.pris, m.pris)
 
 
 @pytest.mark.parametrize("kind", ["tet", "hex"])
