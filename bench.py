@@ -194,16 +194,24 @@ def batched_leg(args):
     t0 = time.perf_counter()
     r = b.smooth("getme", passes, precision="fp32", graph=True)
     wall = time.perf_counter() - t0
+    replayed = [g.positions() for g in gms]
     reset()
     t0 = time.perf_counter()
-    b.smooth("getme", passes, precision="fp32", graph=False)
+    re = b.smooth("getme", passes, precision="fp32", graph=False)
     wall_eager = time.perf_counter() - t0
+    # Graph replay, the eager run and every member alone must agree bit for bit.
+    same = lambda a, b: np.array_equal(np.asarray(a).view(np.uint64), np.asarray(b).view(np.uint64))
+    if not (all(same(g.positions(), p) for g, p in zip(gms, replayed)) and same(r["inv_sum"], re["inv_sum"])
+            and same(r["min"], re["min"])):
+        raise RuntimeError("batched leg: the graph-replayed passes differ from the eager ones")
     # The same meshes one by one (FP32, then the plain FP64 kernel).
     reset()
     t0 = time.perf_counter()
-    for g in gms:
-        g.smooth("getme", passes, precision="fp32", kernel="wave", counts=False)
+    solo = [g.smooth("getme", passes, precision="fp32", kernel="wave", counts=False) for g in gms]
     wall_single = time.perf_counter() - t0
+    if not all(same(g.positions(), p) and same(sr["inv_sum"], r["inv_sum"][i])
+               for i, (g, p, sr) in enumerate(zip(gms, replayed, solo))):
+        raise RuntimeError("batched leg: a member differs from the same mesh smoothed alone")
     reset()
     t0 = time.perf_counter()
     for g in gms:
@@ -222,6 +230,7 @@ def batched_leg(args):
             "one_by_one_plain_fp64_vertices_per_s": movable * passes / wall_plain,
             "speedup_vs_plain_fp64": wall_plain / wall,
             "autotune": {"best_patches_per_group": ppg, "sweep_ms_by_ppg": [float(x) for x in times[:, 0]]},
+            "checked": "graph replay == eager run == every member alone (positions and pass records, bit for bit)",
             "final_min_mean": [float(np.min(r["min"][:, -1])), float(np.mean(r["mean"][:, -1]))]}
 
 
