@@ -37,7 +37,7 @@ def _crew(obj, sampler, tex):
 @pytest.mark.parametrize("sampler", ["analytic", "texture"])
 @pytest.mark.parametrize("smoother", ["nm", "gd"])
 @pytest.mark.parametrize("group", [0, 3])
-def test_group_replay_bitexact_hex(oracle, c3, sampler, smoother, group):
+def test_group_replay_bitexact_hex(oracle, c3, sampler, smoother, group, kernel="wave"):
     m, tex = c3
     gm = GpuMesh.from_cube(m)
     _crew(gm, sampler, tex)
@@ -45,9 +45,9 @@ def test_group_replay_bitexact_hex(oracle, c3, sampler, smoother, group):
     _crew(om, sampler, tex)
     assert np.array_equal(om.permutation(), gm.permutation())
     if group > 0:
-        gm.smooth_groups(smoother, 0, group, "fp64", "wave")
+        gm.smooth_groups(smoother, 0, group, "fp64", kernel)
     before = gm.positions()
-    counts = gm.smooth_groups(smoother, group, group + 1, "fp64", "wave")
+    counts = gm.smooth_groups(smoother, group, group + 1, "fp64", kernel)
     after = gm.positions()
     off, nodes = gm.groups()
     members = nodes[off[group]:off[group + 1]]
@@ -61,6 +61,14 @@ def test_group_replay_bitexact_hex(oracle, c3, sampler, smoother, group):
     outside = np.ones(m.nverts, bool)
     outside[members] = False
     assert np.array_equal(after[outside], before[outside])
+
+
+@pytest.mark.parametrize("group", [0, 3])
+def test_group_replay_plain_kernel_texture(oracle, c3, group):
+    """The one-lane-per-vertex kernel under the 64^3 texture at full size
+    (its patch walk is inlined into the kernel; out of line, its long
+    branches once overwrote the function's return address)."""
+    test_group_replay_bitexact_hex(oracle, c3, "texture", "nm", group, kernel="plain")
 
 
 @pytest.mark.parametrize("sampler", ["analytic", "texture"])
