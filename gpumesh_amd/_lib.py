@@ -31,6 +31,7 @@ EXPORTS = [
     "gm_job_size", "gm_job_encode", "gm_job_info", "gm_job_decode", "gm_server_submit_job",
     "gm_set_polish_passes", "gm_evaluate_noalloc", "gm_mesh_create_grouped", "gm_mesh_set_aspect_ratio",
     "gm_mesh_create_desc", "gm_topology_prepare", "gm_throughput_probe", "gm_server_set_texture", "gm_mesh_prisms",
+    "gm_mesh_restructure",
 ]
 
 
@@ -114,6 +115,7 @@ def lib():
     L.gm_server_set_shape.argtypes = [vp, C.c_int, C.c_int, C.c_int]
     L.gm_server_set_texture.argtypes = [vp, vp, vp, vp, vp]
     L.gm_mesh_prisms.argtypes = [vp, vp, vp]
+    L.gm_mesh_restructure.argtypes = [vp, C.c_int, C.c_int, dbl, dbl, vp]
     L.gm_server_submit.argtypes = [vp, i64, vp, vp, i64, vp, i64, vp, C.c_int, C.c_int, C.c_int, C.c_int,
                                    C.POINTER(i64)]
     L.gm_server_wait.argtypes = [vp, i64, dbl, C.POINTER(C.c_int)]

@@ -8,6 +8,7 @@
 #include <string>
 #include <vector>
 
+#include "gm_batr.h"
 #include "gm_runtime.h"
 
 using gmr::fail;
@@ -370,8 +371,12 @@ int gm_mesh_topology(const gm_mesh* m, uint32_t* eoff, uint32_t* eref, uint32_t*
 
 int gm_mesh_get_positions(gm_mesh* m, double* pos)
 {
-    GM_NEED_DEVICE(m);
-    if(!pos) return fail(GM_ERR_ARG, "null argument");
+    if(!m || !pos) return fail(GM_ERR_ARG, "null argument");
+    if(!m->onDevice)
+    {
+        std::copy(m->hostPos.begin(), m->hostPos.end(), pos);
+        return GM_OK;
+    }
     GM_HIP(hipSetDevice(m->device));
     const size_t V = (size_t)m->topo->nverts;
     std::vector<double> p64(4 * V);
@@ -413,11 +418,13 @@ int gm_mesh_set_positions(gm_mesh* m, const double* pos)
 int gm_mesh_set_crew(gm_mesh* m, int evaluator, int sampler, double scaling,
                      const int32_t* tsize, const double* tmin, const double* text, const float* texels)
 {
-    GM_NEED_DEVICE(m);
+    if(!m) return fail(GM_ERR_ARG, "null mesh");
     if(evaluator < 0 || evaluator > 1) return fail(GM_ERR_ARG, "unknown evaluator");
     if(sampler < 0 || sampler > 3) return fail(GM_ERR_ARG, "unknown sampler");
     if(!(scaling > 0.0)) return fail(GM_ERR_ARG, "scaling must be positive");
-    GM_HIP(hipSetDevice(m->device));
+    if(!m->onDevice && sampler == GM_TEXTURE)
+        return fail(GM_ERR_ARG, "the texture sampler needs a device copy (mesh created with gm_mesh_create_host)");
+    if(m->onDevice) GM_HIP(hipSetDevice(m->device));
     gmq::Crew c{};
     c.scaling = scaling;
     c.scaling2 = scaling * scaling;
@@ -533,6 +540,92 @@ int gm_mesh_smooth_schedule(gm_mesh* m, int smoother, int maxPasses, const gm_sc
     rc = runAndFetch(m->plan, m, r, stats, hist, evalCounts, &run);
     if(!rc && passesRun) *passesRun = run;
     return rc;
+}
+
+int gm_mesh_restructure(gm_mesh* m, int topoPasses, int refinementSweeps, double minEdge, double maxEdge,
+                        int64_t* counts)
+{
+    if(!m) return fail(GM_ERR_ARG, "null mesh");
+    if(topoPasses < 1 || refinementSweeps < 0) return fail(GM_ERR_ARG, "topo_passes must be >= 1, refinement_sweeps >= 0");
+    if(!(minEdge > 0.0) || !(maxEdge > minEdge) || maxEdge == (double)__builtin_huge_val())
+        return fail(GM_ERR_ARG, "edge bounds must satisfy 0 < min_edge < max_edge < inf");
+    if(counts)
+        for(int k = 0; k < 5; ++k) counts[k] = 0;
+    const gmt::Topology& t = *m->topo;
+    if(t.ntets == 0 || t.nhexs || t.npris) return GM_OK;
+    if(!t.cons.empty())
+        return fail(GM_ERR_UNSUPPORTED, "topology operations need free or fixed vertices (this mesh has sliding ones)");
+    if(m->onDevice) GM_HIP(hipSetDevice(m->device));
+
+    gmb::TetMesh in;
+    in.pos.resize(3 * (size_t)t.nverts);
+    int rc = gm_mesh_get_positions(m, in.pos.data());
+    if(rc) return rc;
+    in.fixed = t.fixed;
+    in.tets = t.tets;
+    in.eoff = t.eoff;
+    in.eref = t.eref;
+    in.noff = t.noff;
+    in.nbr = t.nbr;
+    gmb::Settings s;
+    s.topoPasses = topoPasses;
+    s.refinementSweeps = refinementSweeps;
+    s.minEdge = minEdge;
+    s.maxEdge = maxEdge;
+    s.evaluator = m->evaluator;
+    s.sampler = m->sampler;
+    s.crew = m->crew;
+    std::vector<float> texels;
+    if(m->sampler == GM_TEXTURE && m->onDevice)
+    {
+        texels.resize(6 * (size_t)m->crew.tsize[0] * m->crew.tsize[1] * m->crew.tsize[2]);
+        GM_HIP(hipMemcpyAsync(texels.data(), m->texels, texels.size() * sizeof(float), hipMemcpyDeviceToHost,
+                              m->stream));
+        GM_HIP(hipStreamSynchronize(m->stream));
+        s.crew.texels = texels.data();
+    }
+    gmb::TetMesh out;
+    gmb::Counts c;
+    std::string err;
+    if(!gmb::restructure(in, s, out, c, err)) return fail(GM_ERR_ARG, "restructure: " + err);
+
+    // The new mesh: compiled (neighbourhoods, node groups, renumbering) and
+    // uploaded beside the old one, which it replaces only once complete.
+    const int64_t V = (int64_t)out.fixed.size();
+    std::vector<int32_t> fixed(out.fixed.begin(), out.fixed.end());
+    gm_mesh* fresh = nullptr;
+    rc = gmr::makeMesh(V, out.pos.data(), fixed.data(), (int64_t)(out.tets.size() / 4), out.tets.data(), 0, nullptr,
+                       &fresh);
+    if(rc) return rc;
+    fresh->evaluator = m->evaluator;
+    fresh->sampler = m->sampler;
+    fresh->aspect = m->aspect;
+    fresh->crew = m->crew;
+    fresh->texHash = m->texHash;
+    if(m->onDevice)
+    {
+        rc = gmr::uploadMesh(fresh);
+        if(rc)
+        {
+            gmr::destroyMesh(fresh);
+            return rc;
+        }
+        // The texture grid moves over to the new mesh.
+        fresh->texels = m->texels;
+        m->texels = nullptr;
+    }
+    std::swap(*m, *fresh);
+    if(m->plan) m->plan->meshes[0] = m;
+    gmr::destroyMesh(fresh);
+    if(counts)
+    {
+        counts[0] = c.faceSwaps;
+        counts[1] = c.edgeSwaps;
+        counts[2] = c.splits;
+        counts[3] = c.merges;
+        counts[4] = c.passes;
+    }
+    return GM_OK;
 }
 
 int gm_mesh_smooth_groups(gm_mesh* m, int smoother, int64_t groupBegin, int64_t groupEnd, int precision,

@@ -249,6 +249,32 @@ class GpuMesh:
             r["eval_counts"] = cnt
         return r
 
+    def restructure(self, topo_passes: int = 5, refinement_sweeps: int = 5, min_edge: float = 0.5,
+                    max_edge: float = 1.5) -> dict:
+        """BATR topology operations (gm_mesh_restructure): face swaps, edge
+        swaps and edge splits/merges in the crew's metric, then the mesh is
+        rebuilt (new vertex and tet counts, numbering and colour groups).
+        Returns the operation counts."""
+        counts = np.zeros(5, np.int64)
+        check(lib().gm_mesh_restructure(self._h, int(topo_passes), int(refinement_sweeps), float(min_edge),
+                                        float(max_edge), ptr(counts)), "gm_mesh_restructure")
+        self._read_info()
+        return dict(zip(("face_swaps", "edge_swaps", "splits", "merges", "passes"), (int(c) for c in counts)))
+
+    def smooth_global(self, smoother: str, global_passes: int, relocation_passes: int, topo_passes: int = 5,
+                      refinement_sweeps: int = 5, min_edge: float = 0.5, max_edge: float = 1.5,
+                      precision: str = "fp64", kernel: str = "wave", spawn_coeff: float = 1.0 / 20.0) -> list:
+        """The reference's global loop with topology operations on
+        (AbstractVertexWiseSmoother.cpp:53-80): per global pass, restructure,
+        then `relocation_passes` sweeps on the device; returns each global
+        pass's operation counts and pass records."""
+        out = []
+        for _ in range(int(global_passes)):
+            counts = self.restructure(topo_passes, refinement_sweeps, min_edge, max_edge)
+            r = self.smooth(smoother, relocation_passes, precision=precision, kernel=kernel, spawn_coeff=spawn_coeff)
+            out.append({"counts": counts, **r})
+        return out
+
     def smooth_autopilot(self, smoother: str, max_passes: int, min_qual_threshold: float = 0.001,
                          qual_mean_threshold: float = 0.0, precision: str = "fp64", kernel: str = "wave",
                          spawn_coeff: float = 1.0 / 20.0, counts: bool = True):

@@ -169,7 +169,8 @@ int gm_mesh_prisms(const gm_mesh* mesh, int64_t* npris, uint32_t* pris);
  * noff[nverts+1], nbr[sum neighbours]. */
 int gm_mesh_topology(const gm_mesh* mesh, uint32_t* eoff, uint32_t* eref, uint32_t* noff, uint32_t* nbr);
 
-/* FP64 positions, new numbering (replaces fetchCudaVerts, resources/cuda/Mesh.cu:56-72). */
+/* FP64 positions, new numbering (replaces fetchCudaVerts, resources/cuda/Mesh.cu:56-72);
+ * on a mesh from gm_mesh_create_host, its input positions. */
 int gm_mesh_get_positions(gm_mesh* mesh, double* pos);
 /* Replaces updateCudaVerts (resources/cuda/Mesh.cu:74-90). */
 int gm_mesh_set_positions(gm_mesh* mesh, const double* pos);
@@ -181,7 +182,9 @@ int gm_mesh_set_positions(gm_mesh* mesh, const double* pos);
  * texels[6*cells] FP32 (m00 m01 m02 m11 m12 m22, cell (i,j,k) at
  * (k*ny + j)*nx + i).  Replaces installCuda*Evaluator/Measurer/Sampler,
  * setCudaMetricScaling* (Samplers/AbstractSampler.cpp:11-15, 103-111) and
- * updateCudaSamplerTextures (Samplers/TextureSampler.cpp:75-80). */
+ * updateCudaSamplerTextures (Samplers/TextureSampler.cpp:75-80).  A mesh
+ * from gm_mesh_create_host takes every sampler but GM_TEXTURE (its crew
+ * serves gm_mesh_restructure). */
 int gm_mesh_set_crew(gm_mesh* mesh, int evaluator, int sampler, double scaling,
                      const int32_t* tsize, const double* tmin, const double* text, const float* texels);
 
@@ -241,6 +244,26 @@ typedef struct gm_schedule
 int gm_mesh_smooth_schedule(gm_mesh* mesh, int smoother, int max_passes, const gm_schedule* schedule,
                             int precision, int kernel, double spawn_coeff, double* stats, int32_t* hist,
                             int32_t* eval_counts, int32_t* passes_run);
+
+/* BATR topology operations on a tetrahedral mesh, the step before the
+ * relocation passes of each global pass of the reference's loop: replaces
+ * BatrTopologist::restructureMesh (Topologists/BatrTopologist.cpp:76-128),
+ * its Mesh::compileTopology included (AbstractVertexWiseSmoother.cpp:53-80
+ * calls it).  Face swaps, edge swaps and edge splits/merges run on the host
+ * over the mesh's current positions, in the reference's order, with
+ * topo_passes = Schedule::topoOperationPassCount, refinement_sweeps =
+ * Schedule::refinementSweepCount and edge lengths measured in the crew's
+ * metric against min_edge / max_edge (AbstractTopologist's bounds, 0.5 and
+ * 1.5 by default).  The mesh is then rebuilt in place: new vertex and tet
+ * counts (gm_mesh_info), numbering and colour groups, device copy and the
+ * crew kept.  counts (may be null): face swaps, edge swaps, splits, merges,
+ * passes.  A mesh with prisms or hexahedra, or without tets, is left as it
+ * is (needTopologicalModifications, :67-74); vertices that slide on
+ * constraints give GM_ERR_UNSUPPORTED.  A batch holding the mesh must be
+ * rebuilt afterwards.  On a mesh from gm_mesh_create_host the operations run
+ * on its input positions and the result stays host-only. */
+int gm_mesh_restructure(gm_mesh* mesh, int topo_passes, int refinement_sweeps, double min_edge, double max_edge,
+                        int64_t* counts);
 
 /* One partial sweep of a vertex-wise smoother (NM, GD, spawn search) over the
  * colour groups [group_begin, group_end), with no quality evaluation: the

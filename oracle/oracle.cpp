@@ -181,6 +181,7 @@ struct Topo
     std::vector<NeigElem> neighborElems;
     int constraint;       // 0: free (VolumeConstraint), -1: fixed (VertexConstraint), k + 1: slides on constraint k
     dvec3 fixedPos;       // VertexConstraint::_pos (VertexConstraint.cpp:60-63)
+    int fixedId = -1;     // identity of the VertexConstraint object (one per fixed vertex)
     const SlideConstraint* slide = nullptr;
 };
 
@@ -1377,6 +1378,847 @@ static void getmeUpdateVertex(Mesh& m, uint32_t vId, const GetmeOut& out, int64_
     }
 }
 
+// ------------------------------------------------------------------------
+// BATR topologist (Topologists/BatrTopologist.cpp) on meshes of tetrahedra
+// whose vertices are free or fixed.
+//
+// The boundary algebra of those two kinds (AbstractBoundary.cpp:39-66,
+// VertexConstraint.cpp:65-150, VolumeConstraint.cpp:52-60): a free vertex
+// holds a VolumeConstraint, a fixed one its own VertexConstraint
+// (Mesh.cpp:139-147), so
+//   merge(a, b): a == b -> a; a free -> b (VolumeConstraint::merge returns its
+//     argument); b free -> a (a point merged with a volume is the point); two
+//     different points -> invalid (PREVENT_MERGE);
+//   split(a, b): a == b -> a; otherwise a volume (point-point gives
+//     SPLIT_VOLUME, point-volume the volume, volume-anything itself).
+// Volume constraints are interchangeable (identity snap, same algebra).
+// cureBoundaries (:1486-1712) acts on vertices of a face constraint
+// (dimension 2) and on tets with three constrained vertices whose split is
+// constrained; with free and fixed vertices neither exists (three distinct
+// points split to the volume), so it changes nothing and is not restated.
+// Meshes with sliding vertices are refused.
+// ------------------------------------------------------------------------
+struct BCons { int constraint = 0; dvec3 fixedPos; int fixedId = -1; bool valid = true; };
+
+static BCons consOf(const Topo& t)
+{
+    BCons c;
+    c.constraint = t.constraint;
+    c.fixedPos = t.fixedPos;
+    c.fixedId = t.fixedId;
+    return c;
+}
+static bool sameCons(const BCons& a, const BCons& b)
+{
+    return a.constraint == b.constraint && (a.constraint == 0 || a.fixedId == b.fixedId);
+}
+static BCons mergeCons(const BCons& a, const BCons& b)
+{
+    if(sameCons(a, b)) return a;
+    if(a.constraint == 0) return b;
+    if(b.constraint == 0) return a;
+    BCons bad;
+    bad.valid = false;
+    return bad;
+}
+static BCons splitCons(const BCons& a, const BCons& b)
+{
+    if(sameCons(a, b)) return a;
+    return BCons{};
+}
+static void setCons(Topo& t, const BCons& c)
+{
+    t.constraint = c.constraint;
+    t.fixedPos = c.fixedPos;
+    t.fixedId = c.fixedId;
+}
+// (*constraint)(pos): VertexConstraint.cpp:60-63, the volume's identity.
+static dvec3 applyCons(const BCons& c, const dvec3& p) { return c.constraint < 0 ? c.fixedPos : p; }
+
+typedef std::array<uint32_t, 4> Tet4;
+
+// AbstractEvaluator::tetQuality(mesh, sampler, measurer, tet).
+static double batrQuality(const Mesh& m, const Tet4& t)
+{
+    const dvec3 vp[4] = {m.verts[t[0]], m.verts[t[1]], m.verts[t[2]], m.verts[t[3]]};
+    return tetQuality(m.crew, vp);
+}
+
+// MetricFreeMeasurer::riemannianDistance (MetricFreeMeasurer.cpp:27-34),
+// MetricWiseMeasurer::riemannianDistance (MetricWiseMeasurer.cpp:65-116).
+static double batrDistance(const Crew& c, const dvec3& a, const dvec3& b)
+{
+    if(c.measurer == MEAS_METRIC_WISE) return riemannianDistanceWise(c, a, b);
+    return length(a - b) * c.scaling;
+}
+
+// AbstractMeasurer::tetEuclideanVolume (AbstractMeasurer.cpp:61-84).
+static double tetEuclideanVolume(const Mesh& m, const Tet4& t)
+{
+    const dvec3 *p0 = &m.verts[t[0]], *p1 = &m.verts[t[1]], *p2 = &m.verts[t[2]], *p3 = &m.verts[t[3]];
+    return determinant(dmat3(*p3 - *p0, *p3 - *p1, *p3 - *p2)) / 6.0;
+}
+
+static NeigElem toTet(uint32_t id) { return NeigElem{id, TET, -1}; }
+static uint32_t idOf(uint32_t x) { return x; }
+static uint32_t idOf(const NeigElem& e) { return e.id; }
+
+// BatrTopologist::popOut (:1279-1324) and make_union (:1326-1340).
+template<typename T> static bool popOut(std::vector<T>& vec, uint32_t val)
+{
+    const size_t n = vec.size();
+    for(size_t i = 0; i < n; ++i)
+        if(idOf(vec[i]) == val)
+        {
+            std::swap(vec[i], vec.back());
+            vec.pop_back();
+            return true;
+        }
+    return false;
+}
+template<typename T> static bool popOutAll(std::vector<T>& vec, const std::vector<uint32_t>& vals)
+{
+    size_t popCount = 0, i = 0;
+    while(i < vec.size())
+    {
+        bool found = false;
+        for(uint32_t val : vals)
+            if(idOf(vec[i]) == val)
+            {
+                std::swap(vec[i], vec.back());
+                vec.pop_back();
+                found = true;
+                ++popCount;
+                break;
+            }
+        if(!found) ++i;
+    }
+    return popCount == vals.size();
+}
+static bool makeUnion(std::vector<uint32_t>& vec, uint32_t val)
+{
+    for(uint32_t x : vec)
+        if(x == val) return false;
+    vec.push_back(val);
+    return true;
+}
+
+// BatrTopologist::buildVertNeighborhood (:1342-1357).
+static void buildVertNeighborhood(Mesh& m, uint32_t vId)
+{
+    std::set<uint32_t> s;
+    for(const NeigElem& e : m.topos[vId].neighborElems)
+    {
+        const Tet4& t = m.tets[e.id];
+        for(int k = 0; k < 4; ++k)
+            if(t[k] != vId) s.insert(t[k]);
+    }
+    m.topos[vId].neighborVerts.assign(s.begin(), s.end());
+}
+
+// BatrTopologist::findRing (:1359-1393).
+static void findRing(const Mesh& m, uint32_t vId, uint32_t nId, std::vector<uint32_t>& ringVerts,
+                     std::vector<uint32_t>& ringElems)
+{
+    for(const NeigElem& ve : m.topos[vId].neighborElems)
+        for(const NeigElem& ne : m.topos[nId].neighborElems)
+            if(ve.id == ne.id)
+            {
+                ringElems.push_back(ve.id);
+                break;
+            }
+    std::set<uint32_t> s;
+    for(uint32_t e : ringElems)
+        for(int k = 0; k < 4; ++k)
+            if(m.tets[e][k] != vId && m.tets[e][k] != nId) s.insert(m.tets[e][k]);
+    ringVerts.assign(s.begin(), s.end());
+}
+
+// BatrTopologist::findExclusiveElems (:1395-1418).
+static void findExclusiveElems(const Mesh& m, uint32_t vId, const std::vector<uint32_t>& ringElems,
+                               std::vector<uint32_t>& exElems)
+{
+    for(const NeigElem& ve : m.topos[vId].neighborElems)
+    {
+        bool in = false;
+        for(uint32_t r : ringElems)
+            if(ve.id == r)
+            {
+                in = true;
+                break;
+            }
+        if(!in) exElems.push_back(ve.id);
+    }
+}
+
+// BatrTopologist::trimVerts (:1420-1455).
+static void trimVerts(Mesh& m, const std::vector<bool>& aliveVerts)
+{
+    size_t copy = 0;
+    for(size_t vId = 0; vId < m.verts.size(); ++vId)
+    {
+        if(!aliveVerts[vId]) continue;
+        if(copy != vId)
+        {
+            for(uint32_t nv : m.topos[vId].neighborVerts)
+                for(uint32_t& nnv : m.topos[nv].neighborVerts)
+                    if(nnv == vId)
+                    {
+                        nnv = (uint32_t)copy;
+                        break;
+                    }
+            for(const NeigElem& ne : m.topos[vId].neighborElems)
+            {
+                Tet4& t = m.tets[ne.id];
+                if(t[0] == vId) t[0] = (uint32_t)copy;
+                else if(t[1] == vId) t[1] = (uint32_t)copy;
+                else if(t[2] == vId) t[2] = (uint32_t)copy;
+                else if(t[3] == vId) t[3] = (uint32_t)copy;
+            }
+            m.verts[copy] = m.verts[vId];
+            m.topos[copy] = m.topos[vId];
+        }
+        ++copy;
+    }
+    m.verts.resize(copy);
+    m.topos.resize(copy);
+}
+
+// BatrTopologist::trimTets (:1457-1484).
+static void trimTets(Mesh& m, const std::vector<bool>& aliveTets)
+{
+    size_t copy = 0;
+    const size_t n = m.tets.size();
+    for(size_t tId = 0; tId < n; ++tId)
+    {
+        if(!aliveTets[tId]) continue;
+        if(copy != tId)
+        {
+            const Tet4 t = m.tets[tId];
+            for(int i = 0; i < 4; ++i)
+                for(NeigElem& ne : m.topos[t[i]].neighborElems)
+                    if(ne.id == tId)
+                    {
+                        ne.id = (uint32_t)copy;
+                        break;
+                    }
+            m.tets[copy] = t;
+        }
+        ++copy;
+    }
+    m.tets.resize(copy);
+}
+
+// MeshTet::tris (Mesh.cpp:30-35): the face opposite each corner.
+static const int TET_TRIS[4][3] = {{1, 2, 3}, {0, 3, 2}, {0, 1, 3}, {0, 2, 1}};
+
+// BatrTopologist::faceSwapping (:635-836).
+static size_t faceSwapping(Mesh& m)
+{
+    std::vector<Tet4>& tets = m.tets;
+    std::vector<Topo>& topos = m.topos;
+    size_t faceSwapCount = 0;
+    bool stillTetsToTry = true;
+    std::vector<bool> tetsToTry(tets.size(), true);
+    while(stillTetsToTry)
+    {
+        stillTetsToTry = false;
+        for(size_t t = 0; t < tets.size(); ++t)
+        {
+            if(!tetsToTry[t]) continue;
+            tetsToTry[t] = false;
+            const Tet4 tet = tets[t];
+            for(int f = 0; f < 4; ++f)
+            {
+                const uint32_t tri[3] = {tet[TET_TRIS[f][0]], tet[TET_TRIS[f][1]], tet[TET_TRIS[f][2]]};
+                if(!((tri[0] < tri[1] && tri[1] < tri[2]) || (tri[1] < tri[2] && tri[2] < tri[0]) ||
+                     (tri[2] < tri[0] && tri[0] < tri[1])))
+                    continue;
+                uint32_t nb = 0, common[2] = {0, 0};
+                std::vector<NeigElem>& n0 = topos[tri[0]].neighborElems;
+                std::vector<NeigElem>& n1 = topos[tri[1]].neighborElems;
+                std::vector<NeigElem>& n2 = topos[tri[2]].neighborElems;
+                for(const NeigElem& a : n0)
+                {
+                    bool in = false;
+                    for(const NeigElem& b : n1)
+                        if(a.id == b.id)
+                        {
+                            in = true;
+                            break;
+                        }
+                    if(!in) continue;
+                    in = false;
+                    for(const NeigElem& c : n2)
+                        if(a.id == c.id)
+                        {
+                            in = true;
+                            break;
+                        }
+                    if(in)
+                    {
+                        if(nb < 2) common[nb] = a.id;
+                        ++nb;
+                    }
+                }
+                if(nb != 2) continue;
+
+                const uint32_t tOp = tet[f];
+                const uint32_t nt = ((uint32_t)t != common[0]) ? common[0] : common[1];
+                const Tet4 ntet = tets[nt];
+                uint32_t nOp = ntet[0];
+                if(ntet[1] != tri[0] && ntet[1] != tri[1] && ntet[1] != tri[2]) nOp = ntet[1];
+                else if(ntet[2] != tri[0] && ntet[2] != tri[1] && ntet[2] != tri[2]) nOp = ntet[2];
+                else if(ntet[3] != tri[0] && ntet[3] != tri[1] && ntet[3] != tri[2]) nOp = ntet[3];
+
+                const dvec3 vPos = m.verts[tOp];
+                const dvec3 nPos = m.verts[nOp];
+                const dvec3 v0 = m.verts[tri[0]] - vPos;
+                const dvec3 v1 = m.verts[tri[1]] - vPos;
+                const dvec3 v2 = m.verts[tri[2]] - vPos;
+                const dvec3 vn = nPos - vPos;
+                if(dot(cross(v0, v1), vn) < 0.0 || dot(cross(v1, v2), vn) < 0.0 || dot(cross(v2, v0), vn) < 0.0)
+                    continue;
+
+                const double tQual = batrQuality(m, tet);
+                const double nQual = batrQuality(m, ntet);
+                const double hQual = 2.0 / (1.0 / tQual + 1.0 / nQual);
+                const Tet4 nt0 = {tOp, tri[0], tri[1], nOp};
+                const Tet4 nt1 = {tOp, tri[1], tri[2], nOp};
+                const Tet4 nt2 = {tOp, tri[2], tri[0], nOp};
+                const double q0 = batrQuality(m, nt0);
+                const double q1 = batrQuality(m, nt1);
+                const double q2 = batrQuality(m, nt2);
+                const double newHQual = 3.0 / (1 / q0 + 1 / q1 + 1 / q2);
+                if(newHQual <= hQual) continue;
+
+                tets[t] = nt0;
+                tets[nt] = nt1;
+                const uint32_t lt = (uint32_t)tets.size();
+                tets.push_back(nt2);
+                topos[tOp].neighborElems.push_back(toTet(nt));
+                topos[tOp].neighborElems.push_back(toTet(lt));
+                topos[tOp].neighborVerts.push_back(nOp);
+                topos[nOp].neighborElems.push_back(toTet((uint32_t)t));
+                topos[nOp].neighborElems.push_back(toTet(lt));
+                topos[nOp].neighborVerts.push_back(tOp);
+                for(NeigElem& e : n0)
+                    if(e.id == nt)
+                    {
+                        e.id = lt;
+                        break;
+                    }
+                for(NeigElem& e : n2)
+                    if(e.id == t)
+                    {
+                        e.id = lt;
+                        break;
+                    }
+                stillTetsToTry = true;
+                tetsToTry.push_back(true);
+                tetsToTry[nt] = true;
+                tetsToTry[t] = true;
+                ++faceSwapCount;
+                break;
+            }
+        }
+    }
+    return faceSwapCount;
+}
+
+// BatrTopologist's ring configurations (BatrTopologist.cpp:36-60): per ring
+// size, the triangulations of the ring polygon and how many rotations of
+// each are distinct.
+struct RingConfig { uint32_t rotCount; std::vector<std::array<uint32_t, 3>> tris; };
+static const std::vector<std::vector<RingConfig>>& ringConfigDictionary()
+{
+    static const std::vector<std::vector<RingConfig>> d = {
+        {}, {}, {},
+        {{1, {{0, 1, 2}}}},
+        {{2, {{0, 1, 2}, {0, 2, 3}}}},
+        {{5, {{0, 1, 3}, {0, 3, 4}, {1, 2, 3}}}},
+        {{6, {{0, 1, 2}, {0, 2, 3}, {0, 3, 4}, {0, 4, 5}}},
+         {3, {{0, 1, 2}, {0, 2, 3}, {0, 3, 5}, {3, 4, 5}}},
+         {3, {{0, 1, 3}, {1, 2, 3}, {0, 3, 4}, {0, 4, 5}}},
+         {2, {{0, 1, 2}, {0, 2, 4}, {2, 3, 4}, {0, 4, 5}}}},
+        {{7, {{0, 1, 2}, {0, 2, 3}, {0, 3, 4}, {0, 4, 5}, {0, 5, 6}}},
+         {7, {{0, 1, 2}, {0, 2, 3}, {0, 3, 6}, {3, 4, 6}, {4, 5, 6}}},
+         {7, {{0, 1, 2}, {0, 2, 3}, {0, 3, 4}, {0, 4, 6}, {4, 5, 6}}},
+         {7, {{0, 1, 3}, {1, 2, 3}, {0, 3, 4}, {0, 4, 5}, {0, 5, 6}}},
+         {7, {{0, 1, 2}, {0, 2, 4}, {2, 3, 4}, {0, 4, 5}, {0, 5, 6}}},
+         {7, {{0, 1, 2}, {0, 2, 3}, {0, 3, 5}, {3, 4, 5}, {0, 5, 6}}}}};
+    return d;
+}
+
+// BatrTopologist::edgeSwapping (:838-1278).
+static size_t edgeSwapping(Mesh& m)
+{
+    std::vector<Tet4>& tets = m.tets;
+    std::vector<Topo>& topos = m.topos;
+    const std::vector<std::vector<RingConfig>>& dict = ringConfigDictionary();
+    bool stillVertsToTry = true;
+    std::vector<bool> vertsToTry(m.verts.size(), true);
+    std::vector<bool> aliveVerts(m.verts.size(), true);
+    std::vector<bool> aliveTets(tets.size(), true);
+    std::vector<uint32_t> deadTets;
+    size_t total = 0;
+
+    while(stillVertsToTry)
+    {
+        stillVertsToTry = false;
+        for(size_t vId = 0; vId < m.verts.size(); ++vId)
+        {
+            if(!vertsToTry[vId]) continue;
+            vertsToTry[vId] = false;
+            Topo& vTopo = topos[vId];
+            std::vector<uint32_t>& vVerts = vTopo.neighborVerts;
+            std::vector<NeigElem>& vElems = vTopo.neighborElems;
+
+            for(size_t nAr = 0; nAr < vVerts.size(); ++nAr)
+            {
+                const uint32_t nId = vVerts[nAr];
+                if(nId < vId) continue;
+
+                std::vector<uint32_t> ringVerts, ringElems;
+                findRing(m, (uint32_t)vId, nId, ringVerts, ringElems);
+
+                // Order the ring by walking its tets (:889-962).
+                std::vector<uint32_t> elemPool = ringElems;
+                int notFoundCount = 0;
+                size_t r = 0, nx = 1;
+                while(nx != ringVerts.size())
+                {
+                    bool found = false;
+                    for(size_t i = 0; i < elemPool.size() && !found; ++i)
+                    {
+                        const Tet4 tet = tets[elemPool[i]];
+                        for(int j = 0; j < 4 && !found; ++j)
+                        {
+                            if(tet[j] != ringVerts[r]) continue;
+                            for(int k = 0; k < 4 && !found; ++k)
+                            {
+                                if(k == j || tet[k] == vId || tet[k] == nId) continue;
+                                for(size_t w = nx; w != ringVerts.size() && !found; ++w)
+                                    if(ringVerts[w] == tet[k])
+                                    {
+                                        std::swap(elemPool[i], elemPool.back());
+                                        elemPool.pop_back();
+                                        std::swap(ringVerts[w], ringVerts[nx]);
+                                        found = true;
+                                    }
+                            }
+                        }
+                    }
+                    if(!found)
+                    {
+                        if(++notFoundCount >= 2) break;
+                        size_t cb = 0, cf = r;
+                        while(cb < cf) std::swap(ringVerts[cb++], ringVerts[cf--]);
+                    }
+                    else
+                    {
+                        ++r;
+                        ++nx;
+                    }
+                }
+                if(nx != ringVerts.size())
+                {
+                    // A ring that cannot be closed ends the operation (:964-993).
+                    trimTets(m, aliveTets);
+                    return 0;
+                }
+
+                const size_t ringSize = ringVerts.size();
+                if(ringSize < 3 || ringSize >= dict.size()) continue;
+
+                // Counter-clockwise around v-n (:1009-1029); GLM's dvec3
+                // default constructor zero-fills.
+                dvec3 ringAir(0.0, 0.0, 0.0);
+                const dvec3 ring0 = m.verts[ringVerts[0]];
+                for(size_t i = 2; i < ringSize; ++i)
+                    ringAir = ringAir + cross(m.verts[ringVerts[i - 1]] - ring0, m.verts[ringVerts[i]] - ring0);
+                const dvec3 vPos = m.verts[vId];
+                const dvec3 nPos = m.verts[nId];
+                const dvec3 vn = nPos - vPos;
+                if(dot(ringAir, vn) < 0.0)
+                {
+                    size_t i = 0, j = ringVerts.size() - 1;
+                    while(i < j) std::swap(ringVerts[i++], ringVerts[j--]);
+                }
+
+                double minQual = INFINITY;
+                for(uint32_t e : ringElems) minQual = gmin(minQual, batrQuality(m, tets[e]));
+
+                // Outsider check (:1041-1061): every ring edge must turn the
+                // same way around v-n.
+                bool outsidersOk = true;
+                for(size_t i = 1; i <= ringSize; ++i)
+                {
+                    const uint32_t aId = ringVerts[i - 1];
+                    const uint32_t bId = ringVerts[i % ringSize];
+                    const dvec3 normal = cross(m.verts[aId] - vPos, m.verts[bId] - vPos);
+                    if(dot(vn, normal) < 0.0)
+                    {
+                        outsidersOk = false;
+                        break;
+                    }
+                }
+                if(!outsidersOk) continue;
+
+                int bestConf = -1, bestRot = -1;
+                const std::vector<RingConfig>& confs = dict[ringSize];
+                for(size_t conf = 0; conf < confs.size(); ++conf)
+                    for(uint32_t rot = 0; rot < confs[conf].rotCount; ++rot)
+                    {
+                        double rotMin = INFINITY;
+                        for(const auto& rt : confs[conf].tris)
+                        {
+                            const uint32_t a = ringVerts[(rt[0] + rot) % ringSize];
+                            const uint32_t b = ringVerts[(rt[1] + rot) % ringSize];
+                            const uint32_t c = ringVerts[(rt[2] + rot) % ringSize];
+                            const double qv = batrQuality(m, Tet4{b, a, c, (uint32_t)vId});
+                            rotMin = gmin(rotMin, qv);
+                            if(qv < minQual) break;
+                            const double qn = batrQuality(m, Tet4{a, b, c, nId});
+                            rotMin = gmin(rotMin, qn);
+                            if(qn < minQual) break;
+                        }
+                        if(rotMin > minQual)
+                        {
+                            bestConf = (int)conf;
+                            bestRot = (int)rot;
+                            minQual = rotMin;
+                        }
+                    }
+                if(bestConf < 0) continue;
+                ++total;
+
+                // Topology update (:1122-1212).
+                Topo& nTopo = topos[nId];
+                popOutAll(vElems, ringElems);
+                popOutAll(nTopo.neighborElems, ringElems);
+                popOut(vVerts, nId);
+                popOut(nTopo.neighborVerts, (uint32_t)vId);
+                for(uint32_t rv : ringVerts) popOutAll(topos[rv].neighborElems, ringElems);
+                for(uint32_t e : ringElems)
+                {
+                    aliveTets[e] = false;
+                    deadTets.push_back(e);
+                }
+                std::vector<Tet4> bestTets;
+                for(const auto& rt : confs[bestConf].tris)
+                {
+                    const uint32_t a = ringVerts[(rt[0] + bestRot) % ringSize];
+                    const uint32_t b = ringVerts[(rt[1] + bestRot) % ringSize];
+                    const uint32_t c = ringVerts[(rt[2] + bestRot) % ringSize];
+                    bestTets.push_back(Tet4{b, a, c, (uint32_t)vId});
+                    bestTets.push_back(Tet4{a, b, c, nId});
+                    makeUnion(topos[a].neighborVerts, b);
+                    makeUnion(topos[a].neighborVerts, c);
+                    makeUnion(topos[b].neighborVerts, a);
+                    makeUnion(topos[b].neighborVerts, c);
+                    makeUnion(topos[c].neighborVerts, a);
+                    makeUnion(topos[c].neighborVerts, b);
+                }
+                for(const Tet4& t : bestTets)
+                {
+                    uint32_t id;
+                    if(deadTets.empty())
+                    {
+                        id = (uint32_t)tets.size();
+                        tets.push_back(t);
+                        aliveTets.push_back(true);
+                    }
+                    else
+                    {
+                        id = deadTets.back();
+                        deadTets.pop_back();
+                        tets[id] = t;
+                        aliveTets[id] = true;
+                    }
+                    for(int k = 0; k < 4; ++k) topos[t[k]].neighborElems.push_back(toTet(id));
+                }
+                stillVertsToTry = true;
+                for(uint32_t x : vTopo.neighborVerts) vertsToTry[x] = true;
+                for(uint32_t x : nTopo.neighborVerts) vertsToTry[x] = true;
+            }
+        }
+    }
+    trimTets(m, aliveTets);
+    trimVerts(m, aliveVerts);
+    return total;
+}
+
+// BatrTopologist::edgeSplitMerge (:137-633); edge lengths in the crew's
+// metric (AbstractTopologist's minEdgeLength / maxEdgeLength).
+static void edgeSplitMerge(Mesh& m, int refinementSweeps, double minEdge, double maxEdge, int64_t* splits,
+                           int64_t* merges)
+{
+    std::vector<Tet4>& tets = m.tets;
+    std::vector<Topo>& topos = m.topos;
+    const double MIN_GEN_QUALITY = 1e-6;   // _minAcceptableGenQuality (:38)
+    bool stillVertsToTry = true;
+    std::vector<bool> aliveVerts(m.verts.size(), true);
+    std::vector<bool> vertsToTry(m.verts.size(), true);
+    std::vector<bool> aliveTets(tets.size(), true);
+    std::vector<uint32_t> deadVerts, deadTets;
+
+    for(int pass = 0; pass < refinementSweeps && stillVertsToTry; ++pass)
+    {
+        stillVertsToTry = false;
+        for(size_t vId = 0; vId < m.verts.size(); ++vId)
+        {
+            if(!aliveVerts[vId]) continue;
+            vertsToTry[vId] = false;
+
+            uint32_t nId = (uint32_t)-1;
+            double dist = INFINITY, priority = 0.0;
+            BCons constraint;
+            bool have = false;
+            for(size_t nAr = 0; nAr < topos[vId].neighborVerts.size(); ++nAr)
+            {
+                const uint32_t cand = topos[vId].neighborVerts[nAr];
+                if(vId > cand && !vertsToTry[cand]) continue;
+                const double cd = batrDistance(m.crew, m.verts[vId], m.verts[cand]);
+                if(cd < minEdge)
+                {
+                    const double cp = minEdge / cd;
+                    if(cp > priority)
+                    {
+                        const BCons mc = mergeCons(consOf(topos[vId]), consOf(topos[cand]));
+                        if(mc.valid)
+                        {
+                            constraint = mc;
+                            have = true;
+                            priority = cp;
+                            dist = cd;
+                            nId = cand;
+                        }
+                    }
+                }
+                else if(cd > maxEdge)
+                {
+                    const double cp = cd / maxEdge;
+                    if(cp > priority)
+                    {
+                        constraint = splitCons(consOf(topos[vId]), consOf(topos[cand]));
+                        have = true;
+                        priority = cp;
+                        dist = cd;
+                        nId = cand;
+                    }
+                }
+            }
+            if(!have) continue;
+
+            const dvec3 vert = m.verts[vId];
+            const dvec3 neig = m.verts[nId];
+            std::vector<uint32_t> ringVerts, ringElems, nExElems;
+            findRing(m, (uint32_t)vId, nId, ringVerts, ringElems);
+            findExclusiveElems(m, nId, ringElems, nExElems);
+
+            if(dist < minEdge)
+            {
+                // Merge n into v (:257-400).
+                std::vector<uint32_t> vExElems;
+                findExclusiveElems(m, (uint32_t)vId, ringElems, vExElems);
+                std::vector<uint32_t> allEx(vExElems);
+                allEx.insert(allEx.end(), nExElems.begin(), nExElems.end());
+                dvec3 middle = (vert + neig) / 2.0;
+                middle = applyCons(constraint, middle);
+                double minQuality = 1.0;
+                for(uint32_t e : allEx) minQuality = gmin(minQuality, batrQuality(m, tets[e]));
+                minQuality = gmax(minQuality / priority, MIN_GEN_QUALITY);
+                m.verts[vId] = m.verts[nId] = middle;
+                bool conformal = true;
+                for(uint32_t e : allEx)
+                    if(batrQuality(m, tets[e]) <= minQuality)
+                    {
+                        conformal = false;
+                        break;
+                    }
+                if(!conformal)
+                {
+                    m.verts[vId] = vert;
+                    m.verts[nId] = neig;
+                    continue;
+                }
+                deadVerts.push_back(nId);
+                aliveVerts[nId] = false;
+                for(uint32_t e : ringElems)
+                {
+                    deadTets.push_back(e);
+                    aliveTets[e] = false;
+                }
+                for(uint32_t e : nExElems)
+                {
+                    Tet4& t = tets[e];
+                    if(t[0] == nId) t[0] = (uint32_t)vId;
+                    else if(t[1] == nId) t[1] = (uint32_t)vId;
+                    else if(t[2] == nId) t[2] = (uint32_t)vId;
+                    else if(t[3] == nId) t[3] = (uint32_t)vId;
+                }
+                const std::vector<uint32_t> ringCopy = ringVerts;
+                for(uint32_t rv : ringCopy)
+                {
+                    popOutAll(topos[rv].neighborElems, ringElems);
+                    if(topos[rv].neighborElems.empty())
+                    {
+                        popOut(ringVerts, rv);
+                        aliveVerts[rv] = false;
+                        deadVerts.push_back(rv);
+                    }
+                }
+                Topo& vTopo = topos[vId];
+                setCons(vTopo, constraint);
+                vTopo.neighborElems.clear();
+                for(uint32_t e : vExElems) vTopo.neighborElems.push_back(toTet(e));
+                for(uint32_t e : nExElems) vTopo.neighborElems.push_back(toTet(e));
+                buildVertNeighborhood(m, (uint32_t)vId);
+                for(uint32_t x : std::vector<uint32_t>(vTopo.neighborVerts)) buildVertNeighborhood(m, x);
+                ++*merges;
+                stillVertsToTry = true;
+                vertsToTry[vId] = true;
+                vertsToTry[nId] = false;
+                for(uint32_t x : vTopo.neighborVerts) vertsToTry[x] = true;
+            }
+            else if(dist > maxEdge)
+            {
+                // Split v-n at w (:402-590).
+                dvec3 middle = (vert + neig) / 2.0;
+                middle = applyCons(constraint, middle);
+                Topo wTopo;
+                setCons(wTopo, constraint);
+                uint32_t wId;
+                if(deadVerts.empty())
+                {
+                    wId = (uint32_t)m.verts.size();
+                    topos.push_back(wTopo);
+                    m.verts.push_back(middle);
+                    aliveVerts.push_back(true);
+                    vertsToTry.push_back(true);
+                }
+                else
+                {
+                    wId = deadVerts.back();
+                    deadVerts.pop_back();
+                    topos[wId] = wTopo;
+                    m.verts[wId] = middle;
+                    aliveVerts[wId] = true;
+                    vertsToTry[wId] = true;
+                }
+                double minQuality = 1.0;
+                for(uint32_t e : ringElems) minQuality = gmin(minQuality, batrQuality(m, tets[e]));
+                minQuality = gmax(minQuality / priority, MIN_GEN_QUALITY);
+                auto halves = [&](const Tet4& t, Tet4& vT, Tet4& nT) {
+                    vT = t;
+                    nT = t;
+                    for(int k = 0; k < 4; ++k)
+                    {
+                        if(t[k] == vId) nT[k] = wId;
+                        else if(t[k] == nId) vT[k] = wId;
+                    }
+                };
+                bool conformal = true;
+                for(uint32_t e : ringElems)
+                {
+                    Tet4 vT, nT;
+                    halves(tets[e], vT, nT);
+                    if(batrQuality(m, vT) <= minQuality || batrQuality(m, nT) <= minQuality)
+                    {
+                        conformal = false;
+                        break;
+                    }
+                }
+                if(!conformal)
+                {
+                    vertsToTry[wId] = false;
+                    aliveVerts[wId] = false;
+                    deadVerts.push_back(wId);
+                    continue;
+                }
+                for(uint32_t e : ringElems)
+                {
+                    const Tet4 t = tets[e];
+                    for(int k = 0; k < 4; ++k) popOut(topos[t[k]].neighborElems, e);
+                    deadTets.push_back(e);
+                    aliveTets[e] = false;
+                    Tet4 vT, nT;
+                    halves(t, vT, nT);
+                    if(tetEuclideanVolume(m, vT) < 0.0) std::swap(vT[0], vT[1]);
+                    if(tetEuclideanVolume(m, nT) < 0.0) std::swap(nT[0], nT[1]);
+                    uint32_t ids[2];
+                    const Tet4* halvesOut[2] = {&vT, &nT};
+                    for(int h = 0; h < 2; ++h)
+                    {
+                        if(deadTets.empty())
+                        {
+                            ids[h] = (uint32_t)tets.size();
+                            aliveTets.push_back(true);
+                            tets.push_back(*halvesOut[h]);
+                        }
+                        else
+                        {
+                            ids[h] = deadTets.back();
+                            deadTets.pop_back();
+                            aliveTets[ids[h]] = true;
+                            tets[ids[h]] = *halvesOut[h];
+                        }
+                    }
+                    for(int k = 0; k < 4; ++k) topos[vT[k]].neighborElems.push_back(toTet(ids[0]));
+                    for(int k = 0; k < 4; ++k) topos[nT[k]].neighborElems.push_back(toTet(ids[1]));
+                }
+                buildVertNeighborhood(m, wId);
+                for(uint32_t x : std::vector<uint32_t>(topos[wId].neighborVerts))
+                {
+                    buildVertNeighborhood(m, x);
+                    vertsToTry[x] = true;
+                }
+                ++*splits;
+                stillVertsToTry = true;
+            }
+        }
+    }
+    trimTets(m, aliveTets);
+    trimVerts(m, aliveVerts);
+}
+
+// BatrTopologist::restructureMesh (:76-128) followed by its
+// Mesh::compileTopology (compileNeighborhoods + NodeGroups::build).
+// counts: face swaps, edge swaps, splits, merges, passes.  Returns 0, or -2
+// for a mesh with sliding vertices.  Meshes with prisms or hexahedra, or
+// without tetrahedra, are left as they are (needTopologicalModifications,
+// :67-74).
+static int restructureMesh(Mesh& m, int topoPasses, int refinementSweeps, double minEdge, double maxEdge,
+                           int64_t counts[5])
+{
+    for(int k = 0; k < 5; ++k) counts[k] = 0;
+    if(m.tets.empty() || !(m.pris.empty() && m.hexs.empty())) return 0;
+    for(const Topo& t : m.topos)
+        if(t.constraint > 0) return -2;
+    int64_t lastOps = 0;
+    int64_t passDone = 0;
+    while(true)
+    {
+        const int64_t fs = (int64_t)faceSwapping(m);
+        const int64_t es = (int64_t)edgeSwapping(m);
+        int64_t sp = 0, mg = 0;
+        edgeSplitMerge(m, refinementSweeps, minEdge, maxEdge, &sp, &mg);
+        counts[0] += fs;
+        counts[1] += es;
+        counts[2] += sp;
+        counts[3] += mg;
+        const int64_t ops = fs + es + sp + mg;
+        ++passDone;
+        if(ops == 0 || ops == lastOps || passDone >= topoPasses) break;
+        lastOps = ops;
+    }
+    counts[4] = passDone;
+    compileNeighborhoods(m);
+    buildNodeGroups(m);
+    return 0;
+}
+
 } // namespace orc
 
 using namespace orc;
@@ -1464,7 +2306,11 @@ void* orc_mesh_create_full(int64_t nv, const double* pos, const int32_t* fixed, 
     {
         m->verts[v] = dvec3(pos[3 * v], pos[3 * v + 1], pos[3 * v + 2]);
         m->topos[v].fixedPos = m->verts[v];
-        if(fixed[v]) m->topos[v].constraint = -1;
+        if(fixed[v])
+        {
+            m->topos[v].constraint = -1;
+            m->topos[v].fixedId = (int)v;
+        }
         else if(cons && cons[v] >= 0)
         {
             m->topos[v].constraint = cons[v] + 1;
@@ -1516,6 +2362,28 @@ void orc_mesh_elements(void* h, uint32_t* tets, uint32_t* hexs)
 }
 
 int64_t orc_mesh_npris(void* h) { return (int64_t)((Mesh*)h)->pris.size(); }
+
+int64_t orc_mesh_ntets(void* h) { return (int64_t)((Mesh*)h)->tets.size(); }
+
+// Fixed flags per vertex (1 fixed, 0 otherwise).
+void orc_mesh_fixed(void* h, int32_t* out)
+{
+    const Mesh& m = *(Mesh*)h;
+    for(size_t v = 0; v < m.topos.size(); ++v) out[v] = m.topos[v].constraint < 0 ? 1 : 0;
+}
+
+// BatrTopologist::restructureMesh with Schedule::topoOperationPassCount =
+// topoPasses and Schedule::refinementSweepCount = refinementSweeps, edge
+// lengths AbstractTopologist::minEdgeLength / maxEdgeLength; the mesh is then
+// recompiled (neighbourhoods and node groups, renumbered).  counts: face
+// swaps, edge swaps, splits, merges, passes.  Returns 0, or -2 when a vertex
+// slides on a constraint.
+int orc_mesh_restructure(void* h, int topoPasses, int refinementSweeps, double minEdge, double maxEdge,
+                         int64_t* counts)
+{
+    return restructureMesh(*(Mesh*)h, topoPasses, refinementSweeps, minEdge, maxEdge, counts);
+}
+
 
 void orc_mesh_prisms(void* h, uint32_t* pris)
 {

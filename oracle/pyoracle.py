@@ -90,6 +90,11 @@ def lib():
     L.orc_smooth_schedule.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int,
                                       C.c_int, C.c_double, vp, vp, vp, vp]
     L.orc_smooth_schedule.restype = C.c_int
+    L.orc_mesh_ntets.argtypes = [vp]
+    L.orc_mesh_ntets.restype = i64
+    L.orc_mesh_fixed.argtypes = [vp, _P(np.int32)]
+    L.orc_mesh_restructure.argtypes = [vp, C.c_int, C.c_int, C.c_double, C.c_double, _P(np.int64)]
+    L.orc_mesh_restructure.restype = C.c_int
     _LIB = L
     return L
 
@@ -349,3 +354,36 @@ class OracleMesh:
         k = run.value + 1
         return {"passes": run.value, "min": stats[:k, 0].copy(), "mean": stats[:k, 1].copy(),
                 "inv_sum": stats[:k, 2].copy(), "hist": hist[:k].copy(), "eval_counts": cnt}
+
+    # --- topology operations
+    def fixed(self) -> np.ndarray:
+        out = np.empty(self.nverts, np.int32)
+        lib().orc_mesh_fixed(self._h, out)
+        return out
+
+    def restructure(self, topo_passes: int = 5, refinement_sweeps: int = 5, min_edge: float = 0.5,
+                    max_edge: float = 1.5) -> dict:
+        """BATR topology operations (face swaps, edge swaps, edge splits and
+        merges) on a tetrahedral mesh, then the mesh is recompiled and
+        renumbered; returns the operation counts."""
+        counts = np.zeros(5, np.int64)
+        rc = lib().orc_mesh_restructure(self._h, int(topo_passes), int(refinement_sweeps), float(min_edge),
+                                        float(max_edge), counts)
+        if rc != 0:
+            raise RuntimeError(f"oracle restructure refused: {rc}")
+        self.nverts = int(lib().orc_mesh_nverts(self._h))
+        self.ntet = int(lib().orc_mesh_ntets(self._h))
+        return dict(zip(("face_swaps", "edge_swaps", "splits", "merges", "passes"), (int(c) for c in counts)))
+
+    def smooth_global(self, smoother: str, global_passes: int, relocation_passes: int, topo_passes: int = 5,
+                      refinement_sweeps: int = 5, min_edge: float = 0.5, max_edge: float = 1.5, threads: int = 1,
+                      spawn_coeff: float = 1.0 / 20.0) -> list:
+        """The reference's global loop with topology operations on
+        (AbstractVertexWiseSmoother.cpp:53-80): per global pass, restructure,
+        then `relocation_passes` sweeps; returns each pass's counts and records."""
+        out = []
+        for _ in range(global_passes):
+            counts = self.restructure(topo_passes, refinement_sweeps, min_edge, max_edge)
+            r = self.smooth(smoother, relocation_passes, threads=threads, spawn_coeff=spawn_coeff)
+            out.append({"counts": counts, **r})
+        return out

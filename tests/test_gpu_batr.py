@@ -1,0 +1,92 @@
+"""BATR topology operations on device-resident meshes (gm_mesh_restructure)
+and the reference's global loop (restructure, then relocation passes on the
+device) against the oracle: FP64 runs are bit-identical (positions, tets,
+evaluation counts, minimum; harmonic mean within 1e-12, reduction order)."""
+import numpy as np
+import pytest
+
+from gpumesh_amd import GpuMesh
+from gpumesh_amd.meshgen import cube_mesh
+
+pytestmark = pytest.mark.gpu
+
+MEAN_TOL = 1e-12
+
+
+def _pair(oracle, n, K, seed, evaluator="mean_ratio", sampler="uniform", texture_n=16):
+    from gpumesh_amd.samplers import shell_texture
+    m = cube_mesh(n, "tet", 0.35, seed)
+    om = oracle.OracleMesh.from_cube(m)
+    gm = GpuMesh.from_cube(m)
+    tex = shell_texture(m.pos, texture_n, K) if sampler == "texture" else None
+    om.set_crew(evaluator, sampler, scaling=K, texture=tex)
+    gm.set_crew(evaluator, sampler, scaling=K, texture=tex)
+    return om, gm
+
+
+def _same_mesh(om, gm):
+    assert (gm.nverts, gm.ntets) == (om.nverts, om.ntet)
+    assert np.array_equal(gm.positions(), om.positions())
+    assert np.array_equal(gm.elements()[0], om.elements()[0])
+    assert np.array_equal(gm.permutation(), om.permutation())
+
+
+@pytest.mark.parametrize("n,K,seed", [(6, 6.0, 3), (8, 8.0, 11), (6, 9.0, 5), (12, 12.0, 1)])
+def test_restructure_on_device_matches_oracle(oracle, n, K, seed):
+    om, gm = _pair(oracle, n, K, seed)
+    c_ref = om.restructure(5, 5, 0.5, 1.5)
+    c_gpu = gm.restructure(5, 5, 0.5, 1.5)
+    assert c_gpu == c_ref and c_gpu["edge_swaps"] > 0
+    _same_mesh(om, gm)
+    # The rebuilt device copy evaluates as the oracle's mesh does.
+    r, g = om.evaluate(), gm.evaluate()
+    assert g["min"] == r["min"] and abs(g["mean"] - r["mean"]) <= MEAN_TOL
+    assert np.array_equal(g["hist"], r["hist"])
+
+
+@pytest.mark.parametrize("evaluator,sampler,K", [("metric_conformity", "texture", 5.0),
+                                                 ("metric_conformity", "analytic", 1.0),
+                                                 ("mean_ratio", "analytic_reference", 5.0)])
+def test_restructure_under_metric_crews(oracle, evaluator, sampler, K):
+    """The texture grid is read back from the device for the host operations
+    and stays with the rebuilt mesh."""
+    om, gm = _pair(oracle, 5, K, 4, evaluator, sampler)
+    assert gm.restructure(3, 3, 0.5, 1.5) == om.restructure(3, 3, 0.5, 1.5)
+    _same_mesh(om, gm)
+    om.smooth("nm", 2, threads=4)
+    gm.smooth("nm", 2, precision="fp64", kernel="wave")
+    assert np.array_equal(gm.positions(), om.positions())
+
+
+@pytest.mark.parametrize("kernel", ["plain", "wave"])
+@pytest.mark.parametrize("smoother", ["nm", "gd", "spawn", "getme"])
+def test_global_loop_matches_oracle(oracle, smoother, kernel):
+    """Two global passes of (restructure, three relocation passes)."""
+    om, gm = _pair(oracle, 6, 6.0, 9)
+    coeff = 0.25 if smoother == "spawn" else 1.0 / 20.0
+    ref = om.smooth_global(smoother, 2, 3, topo_passes=3, refinement_sweeps=3, threads=4, spawn_coeff=coeff)
+    out = gm.smooth_global(smoother, 2, 3, topo_passes=3, refinement_sweeps=3, precision="fp64", kernel=kernel,
+                           spawn_coeff=coeff)
+    for r, g in zip(ref, out):
+        assert g["counts"] == r["counts"]
+        assert np.array_equal(g["min"], r["min"])
+        assert np.max(np.abs(g["mean"] - r["mean"])) <= MEAN_TOL
+        assert np.array_equal(g["eval_counts"], r["eval_counts"].astype(np.int32))
+    _same_mesh(om, gm)
+    assert out[-1]["mean"][-1] > ref[0]["mean"][0] - 1.0   # records are real numbers
+    assert out[-1]["min"][-1] > 0
+
+
+def test_global_loop_in_fp32_improves_quality():
+    """The mixed-precision passes between restructures: a valid mesh whose
+    quality rises over the initial one."""
+    m = cube_mesh(10, "tet", 0.35, 2)
+    gm = GpuMesh.from_cube(m)
+    gm.set_crew("mean_ratio", "uniform", scaling=10.0)
+    q0 = gm.evaluate()
+    out = gm.smooth_global("gd", 3, 4, precision="fp32_polish", kernel="wave")
+    assert out[-1]["min"][-1] > q0["min"] and out[-1]["mean"][-1] > q0["mean"]
+    t = gm.elements()[0].astype(np.int64)
+    p = gm.positions()
+    a, b, c, d = (p[t[:, k]] for k in range(4))
+    assert np.einsum("ij,ij->i", b - a, np.cross(c - a, d - a)).min() > 0
