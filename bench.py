@@ -234,6 +234,38 @@ def batched_leg(args):
             "final_min_mean": [float(np.min(r["min"][:, -1])), float(np.mean(r["mean"][:, -1]))]}
 
 
+def topology_leg(args):
+    """The reference's global loop with topology operations on: per global
+    pass, the BATR restructure on the host (gm_mesh_restructure) and
+    relocation passes on the device."""
+    from gpumesh_amd import GpuMesh, cube_mesh
+    n = args.topo_n
+    m = cube_mesh(n, "tet", 0.35, seed=2)
+    gm = GpuMesh.from_cube(m)
+    gm.set_crew("mean_ratio", "uniform", scaling=float(n))
+    q0 = gm.evaluate()
+    passes = []
+    for g in range(args.topo_global):
+        tets_in = gm.ntets
+        t0 = time.perf_counter()
+        counts = gm.restructure(5, 5, 0.5, 1.5)
+        t_restructure = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        r = gm.smooth("gd", args.topo_reloc, precision="fp32_polish", kernel="wave", counts=False)
+        t_reloc = time.perf_counter() - t0
+        passes.append({"tets_in": tets_in, "counts": counts, "restructure_s": t_restructure,
+                       "tets_per_s": tets_in / t_restructure, "relocation_wall_ms": t_reloc * 1e3,
+                       "relocation_device_ms": r["total_ms"], "vertices": gm.nverts, "tets": gm.ntets,
+                       "min": float(r["min"][-1]), "mean": float(r["mean"][-1])})
+        log(f"topology pass {g}: restructure {t_restructure:.2f} s ({counts}), {gm.nverts} vertices "
+            f"{gm.ntets} tets; {args.topo_reloc} GD passes {t_reloc * 1e3:.1f} ms wall; "
+            f"min {passes[-1]['min']:.4f} mean {passes[-1]['mean']:.4f}")
+    return {"workload": f"Kuhn tet cube {n}^3 cells, jitter 0.35h seed 2, mean ratio (K = {n}), "
+                        f"{args.topo_global} global passes of BATR restructure (5 topology passes, 5 refinement "
+                        f"sweeps, edges in [0.5, 1.5]) + {args.topo_reloc} GD passes fp32_polish",
+            "initial_min_mean": [q0["min"], q0["mean"]], "passes": passes}
+
+
 def serving_leg(args):
     """Poisson job stream through the serving loop at several rates.  Every
     (kind, n) shape of the job list is built into the topology cache, with
@@ -375,6 +407,10 @@ def main():
     ap.add_argument("--serve-plain-sample", type=int, default=8)
     ap.add_argument("--serve-workers", type=int, default=8, help="host worker threads of the server")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--topo-n", type=int, default=20, help="cube size of the topology leg")
+    ap.add_argument("--topo-global", type=int, default=2, help="global passes of the topology leg")
+    ap.add_argument("--topo-reloc", type=int, default=5, help="relocation passes per global pass")
+    ap.add_argument("--no-topo", action="store_true")
     ap.add_argument("--serve-only", action="store_true", help="run only the serving leg and print its JSON")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device to device 5 times (the reference kernel for HBM counters) and exit")
@@ -583,6 +619,7 @@ def main():
 
     batched = None if args.no_batch else batched_leg(args)
     serving = None if args.no_serve else serving_leg(args)
+    topology = None if args.no_topo else topology_leg(args)
 
     cpu = None
     if not args.no_cpu:
@@ -603,6 +640,22 @@ def main():
                                     f"{args.cpu_serial_n}^3 construction ({sm_mesh.nverts} vertices), {serial_s:.1f} s"}}
         log(f"cpu oracle ({model}): {cpu_value:.0f} vertices/s on {threads} threads ({cpu_s:.1f} s); serial "
             f"{serial_value:.0f} vertices/s ({serial_s:.1f} s)")
+        if topology is not None:
+            # The oracle's restructure of the topology leg's first mesh, one core.
+            from gpumesh_amd import cube_mesh
+            from oracle.pyoracle import OracleMesh
+            tm = cube_mesh(args.topo_n, "tet", 0.35, seed=2)
+            om = OracleMesh.from_cube(tm)
+            om.set_crew("mean_ratio", "uniform", scaling=float(args.topo_n))
+            t0 = time.perf_counter()
+            oc = om.restructure(5, 5, 0.5, 1.5)
+            t_or = time.perf_counter() - t0
+            if oc != topology["passes"][0]["counts"]:
+                raise RuntimeError("topology leg: the first restructure's operation counts differ from the oracle's")
+            cpu["topology"] = {"restructure_s": t_or, "tets_per_s": len(tm.tets) / t_or, "cores": 1,
+                               "sample": f"oracle BATR restructure of the {args.topo_n}^3 tet cube "
+                                         f"({len(tm.tets)} tets), one core"}
+            log(f"cpu oracle restructure: {t_or:.2f} s")
 
     ms_per_step = total_wall / (args.steps * len(smoothers)) * 1e3
     out = {
@@ -640,6 +693,7 @@ def main():
         "equal_quality": equal_quality,
         "batched": batched,
         "serving": serving,
+        "topology": topology,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -37,8 +37,18 @@ def test_host_mesh_rejects_device_calls():
     m = cube_mesh(3, "tet", 0.2, seed=1)
     gm = GpuMesh.from_cube(m, device=False)
     with pytest.raises(_lib.GpuMeshError) as e:
-        gm.positions()
+        gm.evaluate()
     assert e.value.status == 1
+    with pytest.raises(_lib.GpuMeshError) as e:
+        gm.smooth("nm", 1)
+    assert e.value.status == 1
+    with pytest.raises(_lib.GpuMeshError, match="texture"):
+        gm.set_crew("metric_conformity", "texture", 2.0, texture=([1, 1, 1], [0, 0, 0], [1, 1, 1], np.ones(6)))
+    # Positions of a host mesh are its input positions in the new numbering.
+    perm = gm.permutation()
+    want = np.empty_like(m.pos)
+    want[perm] = m.pos
+    assert np.array_equal(gm.positions(), want)
 
 
 def test_invalid_inputs_are_rejected():
