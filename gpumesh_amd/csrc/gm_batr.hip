@@ -156,6 +156,15 @@ struct Work
     std::vector<int32_t> anchor; // FREE, or an index into anchorAt
     std::vector<P3> anchorAt;
 
+    // Tet-membership stamps: a list's tets marked with the current
+    // generation answer "is e in that list" in O(1).
+    std::vector<uint32_t> stamp;
+    uint32_t gen = 0;
+    // Qualities of tet slots while no vertex moves (the swap operations);
+    // a slot's entry is dropped whenever the slot is rewritten.
+    std::vector<double> slotQ;
+    std::vector<uint8_t> slotQOk;
+
     explicit Work(const Settings& st) : s(st) {}
 
     double quality(const Tet& t) const
@@ -163,6 +172,42 @@ struct Work
         const P3 vp[4] = {x[t[0]], x[t[1]], x[t[2]], x[t[3]]};
         return qualityFn(s.crew, vp);
     }
+    double slotQuality(uint32_t e)
+    {
+        if(e >= slotQOk.size())
+        {
+            slotQOk.resize(tet.size(), 0);
+            slotQ.resize(tet.size());
+        }
+        if(!slotQOk[e])
+        {
+            slotQ[e] = quality(tet[e]);
+            slotQOk[e] = 1;
+        }
+        return slotQ[e];
+    }
+    void rewritten(uint32_t e)
+    {
+        if(e < slotQOk.size()) slotQOk[e] = 0;
+    }
+    void forgetQualities() { std::fill(slotQOk.begin(), slotQOk.end(), 0); }
+
+    uint32_t nextGeneration(uint32_t step = 1)
+    {
+        if(stamp.size() < tet.size()) stamp.resize(tet.size(), 0);
+        if(gen > 0xFFFFFFF0u)
+        {
+            std::fill(stamp.begin(), stamp.end(), 0);
+            gen = 0;
+        }
+        gen += step;
+        return gen;
+    }
+    void mark(const List& l, uint32_t g)
+    {
+        for(uint32_t e : l) stamp[e] = g;
+    }
+    bool marked(uint32_t e, uint32_t g) const { return e < stamp.size() && stamp[e] == g; }
     double length(uint32_t a, uint32_t b) const { return lengthFn(s.crew, x[a], x[b]); }
 
     // AbstractMeasurer::tetEuclideanVolume (AbstractMeasurer.cpp:75-84), times 6.
@@ -179,29 +224,54 @@ struct Work
         for(uint32_t e : tetsAt[v])
             for(uint32_t c : tet[e])
                 if(c != v) n.push_back(c);
-        std::sort(n.begin(), n.end());
-        n.erase(std::unique(n.begin(), n.end()), n.end());
+        sortUnique(n);
         nbrsAt[v] = n;
+    }
+
+    // Ascending, duplicates dropped (short lists: insertion sort).
+    static void sortUnique(List& l)
+    {
+        if(l.size() > 24)
+        {
+            std::sort(l.begin(), l.end());
+            l.erase(std::unique(l.begin(), l.end()), l.end());
+            return;
+        }
+        for(size_t i = 1; i < l.size(); ++i)
+        {
+            const uint32_t k = l[i];
+            size_t j = i;
+            while(j > 0 && l[j - 1] > k)
+            {
+                l[j] = l[j - 1];
+                --j;
+            }
+            l[j] = k;
+        }
+        l.erase(std::unique(l.begin(), l.end()), l.end());
     }
 
     // The tets around edge a-b, in a's patch order, and their other
     // vertices, ascending (BatrTopologist::findRing, :1359-1393).
-    void edgeRing(uint32_t a, uint32_t b, List& around, List& ring) const
+    void edgeRing(uint32_t a, uint32_t b, List& around, List& ring)
     {
+        const uint32_t g = nextGeneration();
+        mark(tetsAt[b], g);
         for(uint32_t e : tetsAt[a])
-            if(has(tetsAt[b], e)) around.push_back(e);
+            if(marked(e, g)) around.push_back(e);
         for(uint32_t e : around)
             for(uint32_t c : tet[e])
                 if(c != a && c != b) ring.push_back(c);
-        std::sort(ring.begin(), ring.end());
-        ring.erase(std::unique(ring.begin(), ring.end()), ring.end());
+        sortUnique(ring);
     }
 
     // v's tets outside `around`, in patch order.
-    void tetsOnlyAt(uint32_t v, const List& around, List& out) const
+    void tetsOnlyAt(uint32_t v, const List& around, List& out)
     {
+        const uint32_t g = nextGeneration();
+        mark(around, g);
         for(uint32_t e : tetsAt[v])
-            if(!has(around, e)) out.push_back(e);
+            if(!marked(e, g)) out.push_back(e);
     }
 
     // Drops dead tets, moving the survivors down and renaming them in the
@@ -222,6 +292,7 @@ struct Work
             ++to;
         }
         tet.resize(to);
+        forgetQualities();
     }
 
     // Drops dead vertices the same way (BatrTopologist::trimVerts, :1420-1455).
@@ -267,6 +338,7 @@ uint32_t placeTet(Work& w, const Tet& t, List& freeTets, std::vector<bool>& aliv
         w.tet[id] = t;
         alive[id] = true;
     }
+    w.rewritten(id);
     return id;
 }
 
@@ -294,8 +366,12 @@ int64_t swapFaces(Work& w)
                 if(!((a < b && b < c) || (b < c && c < a) || (c < a && a < b))) continue;
                 uint32_t pair[2] = {0, 0};
                 int shared = 0;
+                const uint32_t g = w.nextGeneration(2);
+                w.mark(w.tetsAt[b], g - 1);
+                for(uint32_t e : w.tetsAt[c])
+                    if(w.stamp[e] == g - 1) w.stamp[e] = g;
                 for(uint32_t e : w.tetsAt[a])
-                    if(has(w.tetsAt[b], e) && has(w.tetsAt[c], e))
+                    if(w.marked(e, g))
                     {
                         if(shared < 2) pair[shared] = e;
                         ++shared;
@@ -321,7 +397,7 @@ int64_t swapFaces(Work& w)
                    gmq::dot(gmq::cross(pc, pa), d) < 0.0)
                     continue;
 
-                const double before = 2.0 / (1.0 / w.quality(cur) + 1.0 / w.quality(other));
+                const double before = 2.0 / (1.0 / w.slotQuality((uint32_t)t) + 1.0 / w.slotQuality(nt));
                 const Tet t0 = {top, a, b, bottom}, t1 = {top, b, c, bottom}, t2 = {top, c, a, bottom};
                 const double q0 = w.quality(t0), q1 = w.quality(t1), q2 = w.quality(t2);
                 const double after = 3.0 / (1.0 / q0 + 1.0 / q1 + 1.0 / q2);
@@ -331,6 +407,9 @@ int64_t swapFaces(Work& w)
                 w.tet[nt] = t1;
                 const uint32_t lt = (uint32_t)w.tet.size();
                 w.tet.push_back(t2);
+                w.rewritten((uint32_t)t);
+                w.rewritten(nt);
+                w.rewritten(lt);
                 w.tetsAt[top].push_back(nt);
                 w.tetsAt[top].push_back(lt);
                 w.nbrsAt[top].push_back(bottom);
@@ -436,7 +515,7 @@ int64_t swapEdges(Work& w)
                 if(gmq::dot(area, axis) < 0.0) std::reverse(ring.begin(), ring.end());
 
                 double worst = INFINITY;
-                for(uint32_t e : around) worst = gmq::tmin(worst, w.quality(w.tet[e]));
+                for(uint32_t e : around) worst = gmq::tmin(worst, w.slotQuality(e));
 
                 // Every ring edge must turn the same way around the axis.
                 bool convex = true;
