@@ -8,7 +8,10 @@
 // The same results therefore need the same visiting order and the same list
 // edits, kept here on per-vertex arrays; element qualities and edge lengths
 // come from the routines the device kernels use (gm_quality.h, FP64), so a
-// restructured mesh matches the reference's bit for bit.
+// restructured mesh matches the reference's bit for bit.  The greedy choices
+// also hinge on last-bit differences between nearly equal qualities
+// (congruent configurations tie up to rounding), so the transcendentals are
+// the C library's, the reference's libm, not the project's (the device's).
 //
 // Vertices are free or fixed.  For those two kinds the reference's boundary
 // algebra (AbstractBoundary.cpp:39-66, VertexConstraint.cpp:65-150,
@@ -862,6 +865,16 @@ bool restructure(const TetMesh& in, const Settings& s, TetMesh& out, Counts& cou
         err = "texture sampler without texels";
         return false;
     }
+
+#if !defined(__HIP_DEVICE_COMPILE__)
+    // The C library's pow, exp and cos for this thread while it runs.
+    struct LibcScope
+    {
+        bool old = gmq::hostLibc;
+        LibcScope() { gmq::hostLibc = true; }
+        ~LibcScope() { gmq::hostLibc = old; }
+    } libc;
+#endif
 
     Work w(s);
     if(s.evaluator == gmq::EV_METRIC_CONFORMITY) pickSampler<gmq::EV_METRIC_CONFORMITY>(w, s.sampler);

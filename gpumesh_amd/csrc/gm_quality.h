@@ -12,6 +12,10 @@
 
 #include "gm_libm.h"
 
+#if !defined(__HIP_DEVICE_COMPILE__)
+#include <cmath>
+#endif
+
 #if defined(__HIPCC__) || defined(__HIP__)
 #define GMQ __host__ __device__ __forceinline__
 #define GMQ_MEMBER __host__ __device__ __forceinline__
@@ -63,9 +67,24 @@ GMQ float tdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 GMQ float tsqrt(float x) { return __builtin_sqrtf(x); }
 GMQ float tdiv(float a, float b) { return a / b; }
 #endif
+#if defined(__HIP_DEVICE_COMPILE__)
 GMQ double tpow23(double x) { return gm_pow23(x); }
 GMQ double tcbrt(double x) { return gm_cbrt_nonneg(x); }
 GMQ double texp(double x) { return gm_exp(x); }
+GMQ double tcos(double x) { return gm_cos(x); }
+GMQ double tpow(double x, double y) { return gm_pow(x, y); }
+#else
+// Host code may take the C library's transcendentals (the reference's libm)
+// instead of the project's, which the device always uses: the topologist
+// does, so its greedy decisions are the ones the reference makes.  Per
+// thread, off unless a caller turns it on.
+inline thread_local bool hostLibc = false;
+GMQ double tpow23(double x) { return hostLibc ? std::pow(x, 2.0 / 3.0) : gm_pow23(x); }
+GMQ double tcbrt(double x) { return hostLibc ? std::pow(x, 1.0 / 3.0) : gm_cbrt_nonneg(x); }
+GMQ double texp(double x) { return hostLibc ? std::exp(x) : gm_exp(x); }
+GMQ double tcos(double x) { return hostLibc ? std::cos(x) : gm_cos(x); }
+GMQ double tpow(double x, double y) { return hostLibc ? std::pow(x, y) : gm_pow(x, y); }
+#endif
 #if defined(__HIP_DEVICE_COMPILE__)
 GMQ float tpow23(float x) { return x > 0.0f ? __builtin_amdgcn_exp2f(0.6666666666666666f * __builtin_amdgcn_logf(x)) : (x == 0.0f ? 0.0f : __builtin_nanf("")); }
 GMQ float tcbrt(float x) { return x > 0.0f ? __builtin_amdgcn_exp2f(0.3333333333333333f * __builtin_amdgcn_logf(x)) : (x == 0.0f ? 0.0f : __builtin_nanf("")); }
@@ -226,7 +245,8 @@ template<typename T> GMQ m3<T> textureMetric(const Crew& c, v3<T> position)
 // AnalyticSampler's field (AnalyticSampler.cpp:45-49): along the rotated x
 // axis the metric length scale is K a^(c^a) with c = (1 - cos(2.5 pi x'))/2,
 // K across it; M = R^T diag(Mx, K^2, K^2) R.  Evaluated in FP64 in both
-// precisions (the project's cos and pow, so the host and the device agree).
+// precisions (the project's cos and pow, as on the device, unless a host
+// caller selects the C library's).
 GMQ_OUTLINE m3<double> referenceMetric(const Crew& c, v3<double> p)
 {
     m3<double> R, RI;
@@ -237,8 +257,8 @@ GMQ_OUTLINE m3<double> referenceMetric(const Crew& c, v3<double> p)
     }
     const double x = mvec<double>(R, p).x * (2.5 * 3.14159265358979323846);
     const double a = c.aspect;
-    const double cc = (1.0 - gm_cos(x)) / 2.0;
-    const double sizeX = c.scaling * gm_pow(a, gm_pow(cc, a));
+    const double cc = (1.0 - tcos(x)) / 2.0;
+    const double sizeX = c.scaling * tpow(a, tpow(cc, a));
     const double Mx = sizeX * sizeX;
     const double My = c.scaling2;
     const m3<double> M = mcols<double>(mk<double>(Mx, 0.0, 0.0), mk<double>(0.0, My, 0.0), mk<double>(0.0, 0.0, My));

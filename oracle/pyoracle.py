@@ -362,13 +362,18 @@ class OracleMesh:
         return out
 
     def restructure(self, topo_passes: int = 5, refinement_sweeps: int = 5, min_edge: float = 0.5,
-                    max_edge: float = 1.5) -> dict:
+                    max_edge: float = 1.5, libm: int = 0) -> dict:
         """BATR topology operations (face swaps, edge swaps, edge splits and
         merges) on a tetrahedral mesh, then the mesh is recompiled and
-        renumbered; returns the operation counts."""
+        renumbered; returns the operation counts.  libm: 0 = glibc (the
+        reference's, and the library's host topologist's), 1 = the project's."""
         counts = np.zeros(5, np.int64)
-        rc = lib().orc_mesh_restructure(self._h, int(topo_passes), int(refinement_sweeps), float(min_edge),
-                                        float(max_edge), counts)
+        old = set_libm(libm)
+        try:
+            rc = lib().orc_mesh_restructure(self._h, int(topo_passes), int(refinement_sweeps), float(min_edge),
+                                            float(max_edge), counts)
+        finally:
+            set_libm(old)
         if rc != 0:
             raise RuntimeError(f"oracle restructure refused: {rc}")
         self.nverts = int(lib().orc_mesh_nverts(self._h))
@@ -379,8 +384,9 @@ class OracleMesh:
                       refinement_sweeps: int = 5, min_edge: float = 0.5, max_edge: float = 1.5, threads: int = 1,
                       spawn_coeff: float = 1.0 / 20.0) -> list:
         """The reference's global loop with topology operations on
-        (AbstractVertexWiseSmoother.cpp:53-80): per global pass, restructure,
-        then `relocation_passes` sweeps; returns each pass's counts and records."""
+        (AbstractVertexWiseSmoother.cpp:53-80): per global pass, restructure
+        (glibc), then `relocation_passes` sweeps (the current libm mode);
+        returns each pass's counts and records."""
         out = []
         for _ in range(global_passes):
             counts = self.restructure(topo_passes, refinement_sweeps, min_edge, max_edge)

@@ -1,7 +1,8 @@
 """The BATR topologist (Topologists/BatrTopologist.cpp): the oracle pinned by
 cases the geometry decides and by invariants, and the library's host
 implementation (gm_mesh_restructure on a host-only mesh) against the oracle,
-bit for bit.
+bit for bit.  Both restructure with glibc's pow, exp and cos, the reference's
+libm (the oracle's libm mode 0).
 
 What pins the oracle:
 * Two tets sharing a face versus the three tets around the segment joining
@@ -321,3 +322,25 @@ def test_restructure_leaves_other_meshes_and_refuses_sliding(oracle):
     for bad in (dict(topo_passes=0), dict(min_edge=0.0), dict(min_edge=2.0, max_edge=1.0)):
         with pytest.raises(GpuMeshError):
             g.restructure(**bad)
+
+
+def test_topologist_follows_the_c_library(oracle):
+    """The greedy choices hinge on last-bit differences between nearly equal
+    qualities: on these cubes the oracle under the project's libm and under
+    glibc (the reference's) end on different meshes, and the library's host
+    topologist makes glibc's choices."""
+    differ = 0
+    for n, K, seed in [(8, 8.0, 11), (6, 9.0, 5), (10, 10.0, 1)]:
+        m = cube_mesh(n, "tet", 0.35, seed)
+        meshes = []
+        for libm in (0, 1):
+            om = oracle.OracleMesh.from_cube(m)
+            om.set_crew("mean_ratio", "uniform", scaling=K)
+            om.restructure(5, 5, 0.5, 1.5, libm=libm)
+            meshes.append((om.positions(), om.elements()[0]))
+        gm = GpuMesh.from_cube(m, device=False)
+        gm.set_crew("mean_ratio", "uniform", scaling=K)
+        gm.restructure(5, 5, 0.5, 1.5)
+        assert np.array_equal(gm.positions(), meshes[0][0]) and np.array_equal(gm.elements()[0], meshes[0][1])
+        differ += not (meshes[0][0].shape == meshes[1][0].shape and np.array_equal(meshes[0][1], meshes[1][1]))
+    assert differ >= 1
