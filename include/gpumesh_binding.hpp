@@ -202,6 +202,54 @@ public:
               "gm_mesh_smooth_groups");
     }
 
+    // BATR topology operations (BatrTopologist::restructureMesh,
+    // Topologists/BatrTopologist.cpp:76-128) on the handle, which is rebuilt
+    // in place (gm_mesh_restructure).  `mesh` receives the result as the
+    // reference's restructure leaves it just before its Mesh::compileTopology:
+    // positions and tets in that numbering (the handle's permutation maps it
+    // to the handle's), and each vertex's topology entry from
+    // makeTopo(fixed, x, y, z) -- a MeshTopo holding the vertex's own fixed
+    // point, or a free one.  The caller then runs the reference's
+    // compileTopology, which renumbers the mesh as the handle already is, so
+    // fetch/upload/sweep go on without a new session.  Returns the face swap,
+    // edge swap, split, merge and pass counts.
+    template<class MakeTopo>
+    std::vector<int64_t> restructure(MeshT& mesh, int topoPasses, int refinementSweeps, double minEdge,
+                                     double maxEdge, const MakeTopo& makeTopo)
+    {
+        std::vector<int64_t> counts(5, 0);
+        check(gm_mesh_restructure(_h, topoPasses, refinementSweeps, minEdge, maxEdge, counts.data()),
+              "gm_mesh_restructure");
+        if(counts[4] == 0) return counts;   // prisms or hexahedra, or no tets: left as it was
+        int64_t info[8];
+        check(gm_mesh_info(_h, info), "gm_mesh_info");
+        const size_t V = (size_t)info[0], T = (size_t)info[1];
+        const int64_t nfixed = info[0] - info[3];   // fixed vertices come first
+        _groups = info[4];
+        std::vector<int64_t> perm(V);
+        check(gm_mesh_permutation(_h, perm.data()), "gm_mesh_permutation");
+        std::vector<double> pos(3 * V);
+        check(gm_mesh_get_positions(_h, pos.data()), "gm_mesh_get_positions");
+        std::vector<uint32_t> tets(4 * T + 1), hexs(1);
+        check(gm_mesh_elements(_h, tets.data(), hexs.data()), "gm_mesh_elements");
+        std::vector<uint32_t> before(V);
+        for(size_t v = 0; v < V; ++v) before[(size_t)perm[v]] = (uint32_t)v;
+        mesh.verts.resize(V);
+        mesh.topos.resize(V);
+        for(size_t v = 0; v < V; ++v)
+        {
+            const double* p = &pos[3 * (size_t)perm[v]];
+            mesh.verts[v].p.x = p[0];
+            mesh.verts[v].p.y = p[1];
+            mesh.verts[v].p.z = p[2];
+            mesh.topos[v] = makeTopo(perm[v] < nfixed, p[0], p[1], p[2]);
+        }
+        mesh.tets.resize(T);
+        for(size_t e = 0; e < T; ++e)
+            for(int k = 0; k < 4; ++k) mesh.tets[e].v[k] = before[tets[4 * e + k]];
+        return counts;
+    }
+
     // The device positions' quality into a QualityHistogram-like object.
     template<class HistogramT> void evaluate(HistogramT& histogram) const
     {

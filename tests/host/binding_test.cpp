@@ -6,6 +6,9 @@
 //
 //   binding_test host <dir>     host topology only: writes the handle's
 //                               permutation, elements and groups back
+//   binding_test restructure <dir> <K>
+//                               host topology only: BATR on the handle, the
+//                               mesh handed back as before compileTopology
 //   binding_test device <dir> <smoother> <passes> <precision>
 //                               the whole run on the GPU: one evaluation
 //                               before the passes, a record per pass,
@@ -234,6 +237,32 @@ int main(int argc, char** argv)
                 if(!refused) { std::fprintf(stderr, "shared-element group accepted\n"); return 3; }
             }
             std::printf("host ok\n");
+            return 0;
+        }
+        if(mode == "restructure" && argc >= 4)
+        {
+            gpumesh_binding::Session<ref::Mesh> s(mesh, Describe(), false);
+            s.setCrew(GM_MEAN_RATIO, GM_UNIFORM, std::atof(argv[3]));
+            const std::vector<int64_t> counts = s.restructure(
+                mesh, 5, 5, 0.5, 1.5,
+                [](bool fixed, double, double, double) { return ref::MeshTopo{fixed ? &FIXED : &FREE}; });
+            const size_t V = mesh.verts.size();
+            std::vector<double> pos(3 * V);
+            std::vector<int32_t> fixed(V);
+            for(size_t v = 0; v < V; ++v)
+            {
+                pos[3 * v] = mesh.verts[v].p.x;
+                pos[3 * v + 1] = mesh.verts[v].p.y;
+                pos[3 * v + 2] = mesh.verts[v].p.z;
+                fixed[v] = mesh.topos[v].snapToBoundary->isFixed() ? 1 : 0;
+            }
+            std::vector<uint32_t> tets;
+            for(const auto& t : mesh.tets) tets.insert(tets.end(), t.v, t.v + 4);
+            save(dir, "out_counts.bin", counts.data(), counts.size());
+            save(dir, "out_pos.bin", pos.data(), pos.size());
+            save(dir, "out_fixed.bin", fixed.data(), fixed.size());
+            save(dir, "out_tets.bin", tets.data(), tets.size());
+            std::printf("restructure ok\n");
             return 0;
         }
         if(mode == "device" && argc >= 6)

@@ -104,6 +104,31 @@ def test_binding_describes_sliding_vertices(oracle, exe, tmp_path, kind, n):
     assert np.array_equal(np.fromfile(os.path.join(d, "out_gnodes.bin"), np.uint32), gnodes)
 
 
+@pytest.mark.parametrize("n,K", [(6, 6.0), (7, 8.0)])
+def test_binding_restructure_hands_back_the_mesh_before_compile(oracle, exe, tmp_path, n, K):
+    """Session::restructure on a mesh in the reference's numbering: the mesh it
+    writes back, compiled as the reference compiles it (the oracle's
+    compileNeighborhoods and NodeGroups build), is the mesh the oracle's
+    BATR restructure ends on, and the handle's numbering."""
+    m, om, pos, fixed, tets, hexs, goff, gnodes = clustered(oracle, "tet", n)
+    d = str(tmp_path)
+    write_inputs(d, pos, fixed, tets, hexs, goff, gnodes)
+    r = subprocess.run([exe, "restructure", d, str(K)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    counts = np.fromfile(os.path.join(d, "out_counts.bin"), np.int64)
+    back_pos = np.fromfile(os.path.join(d, "out_pos.bin"), np.float64).reshape(-1, 3)
+    back_fixed = np.fromfile(os.path.join(d, "out_fixed.bin"), np.int32)
+    back_tets = np.fromfile(os.path.join(d, "out_tets.bin"), np.uint32).reshape(-1, 4)
+    ref = om   # the mesh the reference holds (creating one from the arrays would cluster it again)
+    ref.set_crew("mean_ratio", "uniform", scaling=K)
+    want = ref.restructure(5, 5, 0.5, 1.5)
+    assert list(counts) == [want[k] for k in ("face_swaps", "edge_swaps", "splits", "merges", "passes")]
+    compiled = oracle.OracleMesh(back_pos, back_fixed, back_tets, np.zeros((0, 8), np.uint32))
+    assert np.array_equal(compiled.positions(), ref.positions())
+    assert np.array_equal(compiled.elements()[0], ref.elements()[0])
+    assert np.array_equal(compiled.fixed(), ref.fixed())
+
+
 def test_grouped_topology_equals_the_clustered_build(oracle):
     """A mesh in its original numbering with the reference's colouring mapped
     back to it: patch and neighbour lists are the renumbering build's, mapped."""
