@@ -9,6 +9,10 @@
 //   binding_test restructure <dir> <K>
 //                               host topology only: BATR on the handle, the
 //                               mesh handed back as before compileTopology
+//   binding_test global <dir> <K> <smoother> <passes>
+//                               on the GPU: BATR on the handle, the mesh
+//                               renumbered as the reference's compileTopology
+//                               would, `passes` FP64 sweeps, positions fetched
 //   binding_test device <dir> <smoother> <passes> <precision>
 //                               the whole run on the GPU: one evaluation
 //                               before the passes, a record per pass,
@@ -263,6 +267,47 @@ int main(int argc, char** argv)
             save(dir, "out_fixed.bin", fixed.data(), fixed.size());
             save(dir, "out_tets.bin", tets.data(), tets.size());
             std::printf("restructure ok\n");
+            return 0;
+        }
+        if(mode == "global" && argc >= 6)
+        {
+            const int smoother = std::atoi(argv[4]), passes = std::atoi(argv[5]);
+            {
+                gpumesh_binding::Session<ref::Mesh> s(mesh, Describe(), true);
+                s.setCrew(GM_MEAN_RATIO, GM_UNIFORM, std::atof(argv[3]));
+                s.upload(mesh);
+                const std::vector<int64_t> counts = s.restructure(
+                    mesh, 5, 5, 0.5, 1.5,
+                    [](bool fixed, double, double, double) { return ref::MeshTopo{fixed ? &FIXED : &FREE}; });
+                save(dir, "out_counts.bin", counts.data(), counts.size());
+                // Mesh::compileTopology's renumbering, which the handle has.
+                const size_t V = mesh.verts.size();
+                std::vector<int64_t> perm(V);
+                gpumesh_binding::check(gm_mesh_permutation(s.handle(), perm.data()), "gm_mesh_permutation");
+                ref::Mesh compiled = mesh;
+                for(size_t v = 0; v < V; ++v)
+                {
+                    compiled.verts[(size_t)perm[v]] = mesh.verts[v];
+                    compiled.topos[(size_t)perm[v]] = mesh.topos[v];
+                }
+                for(auto& t : compiled.tets)
+                    for(uint32_t& c : t.v) c = (uint32_t)perm[c];
+                mesh = compiled;
+                for(int p = 0; p < passes; ++p) s.sweep(smoother, GM_FP64, 1.0 / 20.0);
+                s.fetch(mesh);
+            }
+            std::vector<double> pos(3 * mesh.verts.size());
+            for(size_t v = 0; v < mesh.verts.size(); ++v)
+            {
+                pos[3 * v] = mesh.verts[v].p.x;
+                pos[3 * v + 1] = mesh.verts[v].p.y;
+                pos[3 * v + 2] = mesh.verts[v].p.z;
+            }
+            std::vector<uint32_t> tets;
+            for(const auto& t : mesh.tets) tets.insert(tets.end(), t.v, t.v + 4);
+            save(dir, "out_pos.bin", pos.data(), pos.size());
+            save(dir, "out_tets.bin", tets.data(), tets.size());
+            std::printf("global ok\n");
             return 0;
         }
         if(mode == "device" && argc >= 6)

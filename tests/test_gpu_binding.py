@@ -64,6 +64,27 @@ def test_binding_sliding_box_matches_oracle(oracle, tmp_path, kind, n, smoother)
     assert np.array_equal(np.fromfile(os.path.join(d, "out_hist.bin"), np.int32).reshape(-1, 20), ref["hist"])
 
 
+@pytest.mark.parametrize("smoother", ["nm", "gd", "spawn", "getme"])
+def test_binding_global_pass_matches_oracle(oracle, tmp_path, smoother):
+    """One global pass of the reference's loop through the binding: BATR on
+    the device handle (Session::restructure), the reference's renumbering,
+    then FP64 sweeps: the oracle's restructured mesh and positions."""
+    assert os.path.exists(EXE), "tests/host/_build/binding_test is built by __graft_entry__.build()"
+    m, om, pos, fixed, tets, hexs, goff, gnodes = clustered(oracle, "tet", 7)
+    d = str(tmp_path)
+    write_inputs(d, pos, fixed, tets, hexs, goff, gnodes)
+    r = subprocess.run([EXE, "global", d, "7.0", str(SMOOTHERS[smoother]), "2"], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    om.set_crew("mean_ratio", "uniform", scaling=7.0)
+    want = om.restructure(5, 5, 0.5, 1.5)
+    counts = np.fromfile(os.path.join(d, "out_counts.bin"), np.int64)
+    assert list(counts) == [want[k] for k in ("face_swaps", "edge_swaps", "splits", "merges", "passes")]
+    om.smooth(smoother, 2, threads=4)
+    assert np.array_equal(np.fromfile(os.path.join(d, "out_tets.bin"), np.uint32).reshape(-1, 4), om.elements()[0])
+    assert np.array_equal(np.fromfile(os.path.join(d, "out_pos.bin"), np.float64).reshape(-1, 3), om.positions())
+
+
 @pytest.mark.parametrize("kind,n", [("tet", 9), ("hex", 7)])
 def test_grouped_mesh_in_caller_numbering(oracle, kind, n):
     """The original numbering, the reference's groups mapped to it, each
