@@ -177,6 +177,29 @@ def test_mixed_element_mesh_bitexact(oracle):
         assert np.array_equal(out["eval_counts"], ref["eval_counts"].astype(np.int32))
 
 
+@pytest.mark.parametrize("kernel", ["plain", "wave"])
+@pytest.mark.parametrize("smoother", ["nm", "gd", "spawn", "getme"])
+def test_c1_every_smoother_against_fixture(smoother, kernel):
+    """Every smoother on the 20^3 plain-method mesh in FP64: the committed
+    fixture's positions after passes 1, 2 and 10, minima, histograms and
+    evaluation count (tests/golden/c1_smoothers.json)."""
+    import hashlib
+    import json
+    import os
+    g = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                                    "c1_smoothers.json")))["smoothers"][smoother]
+    gm = GpuMesh.from_cube(cube_mesh(20, "tet", 0.3, seed=1))
+    mins, evals = [], 0
+    for upto, step in ((1, 1), (2, 1), (10, 8)):
+        out = gm.smooth(smoother, step, precision="fp64", kernel=kernel)
+        mins += out["min"].tolist() if not mins else out["min"][1:].tolist()
+        evals += int(out["eval_counts"].sum())
+        pos = gm.positions()
+        assert hashlib.sha256(np.ascontiguousarray(pos).tobytes()).hexdigest() == g["positions_sha256"][str(upto)]
+        assert out["hist"][-1].tolist() == g["hist"][str(upto)]
+    assert mins == g["min"] and evals == g["eval_count_total"]
+
+
 def test_c1_plain_method_workload_against_fixture(oracle):
     """The plain-method workload (20^3 Kuhn cube, jitter 0.3 h, seed 1, NM x 10,
     FP64) on the GPU reproduces the committed oracle fixture bit for bit."""
