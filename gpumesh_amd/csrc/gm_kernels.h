@@ -153,6 +153,14 @@ template<int G> __device__ __forceinline__ double groupSum(double v)
     return v;
 }
 
+template<int G> __device__ __forceinline__ double groupMin(double v)
+{
+#pragma unroll
+    for(int off = G / 2; off > 0; off >>= 1)
+        v = __builtin_fmin(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
 // The local size with the neighbours spread over a group of W lanes (tree
 // sum; FP32 with the hardware square root); every lane of the group gets the
 // result.  Used by the FP32 kernels and the FP64 polish pass (not by the
@@ -460,6 +468,19 @@ struct WavePatch
             if(!bad)
                 return cnt / sum;
             return groupMin<G>((valid && !pos && q == q) ? q : __builtin_huge_valf());
+        }
+        if constexpr(FASTQ)
+        {
+            // The mixed-precision FP64 passes: the same rule over the tree
+            // (deterministic; the exact FP64 mode keeps the patch order).
+            const bool valid = lane < nElem;
+            const bool pos = valid && q > 0.0;
+            const double sum = groupSum<G>(pos ? 1.0 / q : 0.0);
+            const double cnt = groupSum<G>(pos ? 1.0 : 0.0);
+            const bool bad = groupMin<G>((valid && !pos) ? 0.0 : 1.0) == 0.0;
+            if(!bad)
+                return cnt / sum;
+            return groupMin<G>((valid && !pos && q == q) ? q : __builtin_huge_val());
         }
         T inv = T(1) / q;
         // LDS accesses of one wavefront complete in issue order, so the reads
@@ -2439,10 +2460,11 @@ __global__ void __launch_bounds__(64 * GETME_WAVES) getmeWaveKernel(const DevMes
     }
     v3<T> acc = mk<T>(T(0), T(0), T(0));
     T wacc = T(0);
-    if constexpr(sizeof(T) == 4)
+    if constexpr(sizeof(T) == 4 || FASTQ)
     {
-        // FP32: a DPP tree over the group (lanes past the patch add zeros).
-        acc = mk<float>(groupSum<G>(pw.x), groupSum<G>(pw.y), groupSum<G>(pw.z));
+        // FP32 and the mixed-precision FP64 passes: a tree over the group
+        // (lanes past the patch add zeros).
+        acc = mk<T>(groupSum<G>(pw.x), groupSum<G>(pw.y), groupSum<G>(pw.z));
         wacc = groupSum<G>(w);
     }
     else
