@@ -41,9 +41,9 @@ hipError_t launchSmooth(const VertexWork& w, const gmq::Crew& c, int evaluator, 
 
 template<typename T>
 hipError_t launchGetmeElements(const DevMesh* meshes, BlockWork tets, BlockWork hexs, BlockWork pris,
-                               const gmq::Crew& c, int evaluator, int sampler, hipStream_t stream)
+                               const gmq::Crew& c, int evaluator, int sampler, bool fast, hipStream_t stream)
 {
-    GMD_SELECT(evaluator, sampler, T, getmeElements(meshes, tets, hexs, pris, c, stream))
+    GMD_SELECT(evaluator, sampler, T, getmeElements(meshes, tets, hexs, pris, c, fast, stream))
 }
 
 template<typename T>
@@ -74,9 +74,9 @@ template hipError_t launchSmooth<double>(const VertexWork&, const gmq::Crew&, in
 template hipError_t launchSmooth<float>(const VertexWork&, const gmq::Crew&, int, int, int, int, LaneShape, double,
                                         hipStream_t);
 template hipError_t launchGetmeElements<double>(const DevMesh*, BlockWork, BlockWork, BlockWork, const gmq::Crew&,
-                                                int, int, hipStream_t);
+                                                int, int, bool, hipStream_t);
 template hipError_t launchGetmeElements<float>(const DevMesh*, BlockWork, BlockWork, BlockWork, const gmq::Crew&,
-                                               int, int, hipStream_t);
+                                               int, int, bool, hipStream_t);
 template hipError_t launchGetmeVertices<double>(const VertexWork&, const gmq::Crew&, int, int, int, LaneShape,
                                                 hipStream_t);
 template hipError_t launchGetmeVertices<float>(const VertexWork&, const gmq::Crew&, int, int, int, LaneShape,

@@ -157,8 +157,10 @@ struct Launchers
 {
     static hipError_t smooth(const VertexWork& w, const gmq::Crew& c, int smoother, int shape,
                              LaneShape lane, double moveCoeff, hipStream_t stream);
+    // fast: the mixed-precision FP64 passes (seeded routines for mean ratio
+    // with the uniform sampler on tets).
     static hipError_t getmeElements(const DevMesh* meshes, BlockWork tets, BlockWork hexs, BlockWork pris,
-                                    const gmq::Crew& c, hipStream_t stream);
+                                    const gmq::Crew& c, bool fast, hipStream_t stream);
     // GETMe vertex updates: KS_WAVE, one vertex per group of lanes (patches
     // of up to 64 elements); otherwise one vertex per thread, 64 x
     // lane.patchesPerGroup threads per block.
@@ -185,7 +187,7 @@ hipError_t launchSmooth(const VertexWork& w, const gmq::Crew& c, int evaluator, 
                         int shape, LaneShape lane, double moveCoeff, hipStream_t stream);
 template<typename T>
 hipError_t launchGetmeElements(const DevMesh* meshes, BlockWork tets, BlockWork hexs, BlockWork pris,
-                               const gmq::Crew& c, int evaluator, int sampler, hipStream_t stream);
+                               const gmq::Crew& c, int evaluator, int sampler, bool fast, hipStream_t stream);
 template<typename T>
 hipError_t launchGetmeVertices(const VertexWork& w, const gmq::Crew& c, int evaluator, int sampler, int shape,
                                LaneShape lane, hipStream_t stream);

@@ -215,6 +215,19 @@ __device__ __forceinline__ double evalCornerInv3(const gmq::m3<double>& F)
     return d > 0.0 ? f2 * gma::powm23(d) : __builtin_nan("");
 }
 
+// Cube root of x >= 0 for the mixed-precision FP64 passes: the hardware
+// FP32 log/exp seed (about 2^-22 relative) and one Halley step
+// y (y^3 + 2x) / (2 y^3 + x), whose cubic convergence leaves the last bits;
+// the project's correctly rounded routine outside the FP32 range.
+__device__ __forceinline__ double fastCbrt(double x)
+{
+    if(!(x >= 1.1754943508222875e-38 && x <= 3.4028234663852886e+38))
+        return gm_cbrt_nonneg(x);
+    const double y = (double)__builtin_amdgcn_exp2f(0.3333333333333333f * __builtin_amdgcn_logf((float)x));
+    const double y3 = y * y * y;
+    return y * (y3 + 2.0 * x) / (2.0 * y3 + x);
+}
+
 __device__ __forceinline__ double fastTetEval(const gmq::Crew& c, const v3<double> vp[4], double& inv)
 {
     const double s = evalCornerInv3(
@@ -2164,11 +2177,14 @@ static void launchNmAffine(const VertexWork& w, hipStream_t stream)
 // element's quality goes to getmeQual (tets, then hexes), which the vertex
 // steps of the pass keep current.
 // ---------------------------------------------------------------------------
-template<typename T, int EV, int SA>
+template<typename T, int EV, int SA, bool FAST = false>
 __global__ void __launch_bounds__(ELEM_BLOCK) getmeTetKernel(const DevMesh* __restrict__ meshes,
                                                              const uint4* __restrict__ spans, uint32_t nspans,
                                                              gmq::Crew c)
 {
+    // FAST (FP64, mean ratio, uniform sampler): the mixed-precision passes'
+    // routines -- seeded cube root and tet quality, as their vertex step.
+    constexpr bool FASTE = FAST && sizeof(T) == 8 && EV == 0 && SA == 0;
     const uint2 bl = blockItem(spans, nspans, blockIdx.x);
     const DevMesh& m = meshes[bl.x];
     const uint32_t e = bl.y * ELEM_BLOCK + threadIdx.x;
@@ -2190,7 +2206,11 @@ __global__ void __launch_bounds__(ELEM_BLOCK) getmeTetKernel(const DevMesh* __re
     for(int k = 0; k < 4; ++k) vpp[k] = vp[k] + n[k] * gmq::tdiv(lambda, gmq::tsqrt(gmq::tsqrt(gmq::dot(n[k], n[k]))));
     T volume = gmq::tetVolume<T, SA>(c, vp);
     T volumePrime = gmq::tetVolume<T, SA>(c, vpp);
-    T volumeVar = gmq::tcbrt(gmq::tabs(gmq::tdiv(volume, volumePrime)));
+    T volumeVar;
+    if constexpr(FASTE)
+        volumeVar = fastCbrt(gmq::tabs(gmq::tdiv(volume, volumePrime)));
+    else
+        volumeVar = gmq::tcbrt(gmq::tabs(gmq::tdiv(volume, volumePrime)));
     v3<T> center = T(1.0 / 4.0) * (vp[0] + vp[1] + vp[2] + vp[3]);
 #pragma unroll
     for(int k = 0; k < 4; ++k) vpp[k] = center + volumeVar * (vpp[k] - center);
@@ -2200,8 +2220,18 @@ __global__ void __launch_bounds__(ELEM_BLOCK) getmeTetKernel(const DevMesh* __re
         if(m.fixed[vi[k]]) vpp[k] = vp[k];
         else if(const double* cs = consOf(m, vi[k])) vpp[k] = DevSnap<T>{cs}(vpp[k]);
     }
-    T quality = gmq::tetQuality<T, EV, SA>(c, vp);
-    T qualityPrime = gmq::tetQuality<T, EV, SA>(c, vpp);
+    T quality, qualityPrime;
+    if constexpr(FASTE)
+    {
+        double inv;
+        quality = fastTetEval(c, vp, inv);
+        qualityPrime = fastTetEval(c, vpp, inv);
+    }
+    else
+    {
+        quality = gmq::tetQuality<T, EV, SA>(c, vp);
+        qualityPrime = gmq::tetQuality<T, EV, SA>(c, vpp);
+    }
     T weight = gmq::tdiv(qualityPrime, T(1) + quality);
     ((T*)m.getmeQual)[e] = quality;
 #pragma unroll
@@ -2849,10 +2879,15 @@ hipError_t Launchers<T, EV, SA>::smooth(const VertexWork& w, const gmq::Crew& c,
 
 template<typename T, int EV, int SA>
 hipError_t Launchers<T, EV, SA>::getmeElements(const DevMesh* meshes, BlockWork tets, BlockWork hexs,
-                                               BlockWork pris, const gmq::Crew& c, hipStream_t stream)
+                                               BlockWork pris, const gmq::Crew& c, bool fast, hipStream_t stream)
 {
     if(tets.count)
-        getmeTetKernel<T, EV, SA><<<tets.count, ELEM_BLOCK, 0, stream>>>(meshes, tets.spans, tets.nspans, c);
+    {
+        if(sizeof(T) == 8 && EV == 0 && SA == 0 && fast)
+            getmeTetKernel<T, EV, SA, true><<<tets.count, ELEM_BLOCK, 0, stream>>>(meshes, tets.spans, tets.nspans, c);
+        else
+            getmeTetKernel<T, EV, SA><<<tets.count, ELEM_BLOCK, 0, stream>>>(meshes, tets.spans, tets.nspans, c);
+    }
     if(hexs.count)
         getmeHexKernel<T, EV, SA><<<hexs.count, ELEM_BLOCK, 0, stream>>>(meshes, hexs.spans, hexs.nspans, c);
     if(pris.count)

@@ -990,9 +990,9 @@ static hipError_t sweep(Plan* p, const gmq::Crew& c, int ev, int sa, const RunPa
     {
         int slot = timedBegin(p, timed, f64 ? GM_KT_GETME_ELEM_F64 : GM_KT_GETME_ELEM_F32);
         e = f64 ? gmd::launchGetmeElements<double>(p->dTable, p->tetBlocks, p->hexBlocks, p->priBlocks, c, ev, sa,
-                                                   p->stream)
+                                                   gmr::passFast(r.precision), p->stream)
                 : gmd::launchGetmeElements<float>(p->dTable, p->tetBlocks, p->hexBlocks, p->priBlocks, c, ev, sa,
-                                                  p->stream);
+                                                  false, p->stream);
         timedEnd(p, slot);
     }
     // GETMe always sweeps every group, in serial order when members slide.
