@@ -583,7 +583,7 @@ def main():
     # order, bit-identical to the plain FP64 kernel and the CPU oracle) and the
     # mixed-precision FP64 kernels for every pass (GM_FP64_FAST).  The final
     # quality of each run is compared with the exact FP64 run of the same
-    # passes; equal quality within 1e-6 is the north-star condition.
+    # passes; "equal quality" means final min and mean within 1e-6 of it.
     quality, fp64_exact, fp64_fast = {}, {}, {}
     for sm, p in per.items():
         gm.set_positions(start)

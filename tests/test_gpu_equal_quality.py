@@ -1,5 +1,6 @@
-"""Equal quality: the FP64 modes against the FP64 oracle at the north star's
-tolerances (positions within 1e-9 h, final min and mean within 1e-12).
+"""Equal quality: the FP64 modes against the FP64 oracle at the parity
+tolerances of this path (positions within 1e-9 h, final min and mean within
+1e-12).
 
 * GM_FP64_FAST runs every pass on the mixed-precision FP64 kernels (FP64
   arithmetic in another operation order, d^(-2/3) from a hardware seed
