@@ -561,6 +561,10 @@ int gm_mesh_restructure(gm_mesh* m, int topoPasses, int refinementSweeps, double
     in.pos.resize(3 * (size_t)t.nverts);
     int rc = gm_mesh_get_positions(m, in.pos.data());
     if(rc) return rc;
+    // A non-finite coordinate makes every quality comparison false, and the
+    // swaps (taken unless the new configuration is no better) would not end.
+    for(double x : in.pos)
+        if(!(x - x == 0.0)) return fail(GM_ERR_ARG, "restructure: a vertex position is not finite");
     in.fixed = t.fixed;
     in.tets = t.tets;
     in.eoff = t.eoff;

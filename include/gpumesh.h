@@ -261,7 +261,8 @@ int gm_mesh_smooth_schedule(gm_mesh* mesh, int smoother, int max_passes, const g
  * is (needTopologicalModifications, :67-74); vertices that slide on
  * constraints give GM_ERR_UNSUPPORTED.  A batch holding the mesh must be
  * rebuilt afterwards.  On a mesh from gm_mesh_create_host the operations run
- * on its input positions and the result stays host-only. */
+ * on its input positions and the result stays host-only.  A non-finite
+ * position gives GM_ERR_ARG. */
 int gm_mesh_restructure(gm_mesh* mesh, int topo_passes, int refinement_sweeps, double min_edge, double max_edge,
                         int64_t* counts);
 

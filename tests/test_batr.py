@@ -322,6 +322,11 @@ def test_restructure_leaves_other_meshes_and_refuses_sliding(oracle):
     for bad in (dict(topo_passes=0), dict(min_edge=0.0), dict(min_edge=2.0, max_edge=1.0)):
         with pytest.raises(GpuMeshError):
             g.restructure(**bad)
+    pos = t.pos.copy()
+    pos[int(np.flatnonzero(t.fixed == 0)[0])] = np.nan
+    gn = GpuMesh(pos, t.fixed, t.tets, t.hexs, device=False)
+    with pytest.raises(GpuMeshError, match="not finite"):
+        gn.restructure()
 
 
 def test_topologist_follows_the_c_library(oracle):
