@@ -5,6 +5,8 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
+#include <set>
 #include <string>
 #include <vector>
 
@@ -26,6 +28,18 @@ using gmr::fail;
         if(!(m)) return fail(GM_ERR_ARG, "null mesh");                                                      \
         if(!(m)->onDevice) return fail(GM_ERR_ARG, "mesh has no device copy (created with gm_mesh_create_host)"); \
     } while(0)
+
+// Meshes held by live batches (by address, never dereferenced here): a
+// batch's descriptor table points into its members' device buffers, so a
+// member must not be rebuilt under it.
+static std::mutex g_memberMu;
+static std::multiset<const gm_mesh*> g_batchMembers;
+
+static bool inLiveBatch(const gm_mesh* m)
+{
+    std::lock_guard<std::mutex> lock(g_memberMu);
+    return g_batchMembers.count(m) != 0;
+}
 
 struct gm_batch
 {
@@ -555,6 +569,8 @@ int gm_mesh_restructure(gm_mesh* m, int topoPasses, int refinementSweeps, double
     if(t.ntets == 0 || t.nhexs || t.npris) return GM_OK;
     if(!t.cons.empty())
         return fail(GM_ERR_UNSUPPORTED, "topology operations need free or fixed vertices (this mesh has sliding ones)");
+    if(inLiveBatch(m))
+        return fail(GM_ERR_ARG, "the mesh is a member of a live batch: destroy the batch before restructuring");
     if(m->onDevice) GM_HIP(hipSetDevice(m->device));
 
     gmb::TetMesh in;
@@ -721,6 +737,10 @@ int gm_batch_create(gm_mesh* const* meshes, int64_t n, gm_batch** out)
         delete b;
         return rc;
     }
+    {
+        std::lock_guard<std::mutex> lock(g_memberMu);
+        for(const gm_mesh* m : ms) g_batchMembers.insert(m);
+    }
     *out = b;
     return GM_OK;
 }
@@ -728,6 +748,14 @@ int gm_batch_create(gm_mesh* const* meshes, int64_t n, gm_batch** out)
 int gm_batch_destroy(gm_batch* b)
 {
     if(!b) return GM_OK;
+    {
+        std::lock_guard<std::mutex> lock(g_memberMu);
+        for(const gm_mesh* m : b->plan->meshes)
+        {
+            auto it = g_batchMembers.find(m);
+            if(it != g_batchMembers.end()) g_batchMembers.erase(it);
+        }
+    }
     (void)hipSetDevice(b->device);
     (void)hipStreamSynchronize(b->stream);
     gmr::freePlan(b->plan);

@@ -259,8 +259,8 @@ int gm_mesh_smooth_schedule(gm_mesh* mesh, int smoother, int max_passes, const g
  * crew kept.  counts (may be null): face swaps, edge swaps, splits, merges,
  * passes.  A mesh with prisms or hexahedra, or without tets, is left as it
  * is (needTopologicalModifications, :67-74); vertices that slide on
- * constraints give GM_ERR_UNSUPPORTED.  A batch holding the mesh must be
- * rebuilt afterwards.  On a mesh from gm_mesh_create_host the operations run
+ * constraints give GM_ERR_UNSUPPORTED.  A member of a live batch gives
+ * GM_ERR_ARG (destroy the batch first, build a new one afterwards).  On a mesh from gm_mesh_create_host the operations run
  * on its input positions and the result stays host-only.  A non-finite
  * position gives GM_ERR_ARG. */
 int gm_mesh_restructure(gm_mesh* mesh, int topo_passes, int refinement_sweeps, double min_edge, double max_edge,

@@ -90,3 +90,23 @@ def test_global_loop_in_fp32_improves_quality():
     p = gm.positions()
     a, b, c, d = (p[t[:, k]] for k in range(4))
     assert np.einsum("ij,ij->i", b - a, np.cross(c - a, d - a)).min() > 0
+
+
+def test_restructure_refuses_a_member_of_a_live_batch():
+    """A batch's descriptor table points into its members' device buffers:
+    a member is not rebuilt under a live batch, and is once it is gone."""
+    from gpumesh_amd import GpuBatch
+    from gpumesh_amd._lib import GpuMeshError
+    ms = [GpuMesh.from_cube(cube_mesh(n, "tet", 0.3, s)) for n, s in ((5, 1), (6, 2))]
+    for g in ms:
+        g.set_crew("mean_ratio", "uniform", scaling=6.0)
+    b = GpuBatch(ms)
+    with pytest.raises(GpuMeshError, match="live batch"):
+        ms[0].restructure()
+    b.smooth("nm", 1, precision="fp64")
+    b.close()
+    assert ms[0].restructure()["passes"] >= 1
+    b2 = GpuBatch(ms)
+    r = b2.smooth("nm", 1, precision="fp64")
+    assert np.all(np.asarray(r["min"])[:, -1] > 0)
+    b2.close()
