@@ -455,6 +455,7 @@ int gm_mesh_set_crew(gm_mesh* m, int evaluator, int sampler, double scaling,
             c.tmin[k] = tmin[k];
             c.text[k] = text[k];
         }
+        gmq::textureFields(c);
         // The new grid is allocated and filled before the old one is let go,
         // so a failure leaves the previous crew (and its texels) intact.
         const size_t n = 6 * (size_t)tsize[0] * tsize[1] * tsize[2];

@@ -166,6 +166,9 @@ struct Crew
     int tsize[3];
     double tmin[3], text[3];
     const float* texels;   // 6 per cell: m00 m01 m02 m11 m12 m22; cell (i,j,k) at ((k ny)+j) nx + i
+    // The grid's cell size, its reciprocal, the box's low corner and the last
+    // clamp corner in FP32, for the FP32 sampling (textureFields fills them).
+    float tcellf[3], tinvf[3], tminf[3], tclampf[3];
     // The reference's analytic field: aspect ratio and the sampler's
     // rotation (column-major, rot[3 c + r]) and its transpose.
     double aspect;
@@ -209,8 +212,52 @@ template<typename T> GMQ m3<T> mixm(const m3<T>& x, const m3<T>& y, T a)
 GMQ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
 // TextureSampler::metricAt (Samplers/TextureSampler.cpp:450-488), cellId (:513-524).
+// Fills the FP32 fields of a texture crew from tsize, tmin and text.
+inline void textureFields(Crew& c)
+{
+    for(int k = 0; k < 3; ++k)
+    {
+        const double cell = c.text[k] / (double)c.tsize[k];
+        c.tcellf[k] = (float)cell;
+        c.tinvf[k] = (float)((double)c.tsize[k] / c.text[k]);
+        c.tminf[k] = (float)c.tmin[k];
+        c.tclampf[k] = (float)((c.tmin[k] + c.text[k]) - cell * 1.5);
+    }
+}
+
+// TextureSampler::metricAt in FP32 (the FP32 passes): the same cell, clamp
+// and weights, with the crew's FP32 cell size and reciprocal in place of the
+// divisions.
+GMQ m3<float> textureMetricF32(const Crew& c, v3<float> p)
+{
+    const float cx = c.tcellf[0], cy = c.tcellf[1], cz = c.tcellf[2];
+    const float ix = c.tinvf[0], iy = c.tinvf[1], iz = c.tinvf[2];
+    const v3<float> lo = mk<float>(c.tminf[0], c.tminf[1], c.tminf[2]);
+    const v3<float> q = mk<float>(p.x - 0.5f * cx, p.y - 0.5f * cy, p.z - 0.5f * cz);
+    const v3<float> cp0 = mk<float>(tmin(tmax(q.x, lo.x), c.tclampf[0]), tmin(tmax(q.y, lo.y), c.tclampf[1]),
+                                    tmin(tmax(q.z, lo.z), c.tclampf[2]));
+    const int i0 = clampi((int)((cp0.x - lo.x) * ix), 0, c.tsize[0] - 1);
+    const int j0 = clampi((int)((cp0.y - lo.y) * iy), 0, c.tsize[1] - 1);
+    const int k0 = clampi((int)((cp0.z - lo.z) * iz), 0, c.tsize[2] - 1);
+    const int i1 = (i0 + 1 < c.tsize[0] - 1) ? i0 + 1 : c.tsize[0] - 1;
+    const int j1 = (j0 + 1 < c.tsize[1] - 1) ? j0 + 1 : c.tsize[1] - 1;
+    const int k1 = (k0 + 1 < c.tsize[2] - 1) ? k0 + 1 : c.tsize[2] - 1;
+    const m3<float> m000 = texel<float>(c, i0, j0, k0), m100 = texel<float>(c, i1, j0, k0);
+    const m3<float> m010 = texel<float>(c, i0, j1, k0), m110 = texel<float>(c, i1, j1, k0);
+    const m3<float> m001 = texel<float>(c, i0, j0, k1), m101 = texel<float>(c, i1, j0, k1);
+    const m3<float> m011 = texel<float>(c, i0, j1, k1), m111 = texel<float>(c, i1, j1, k1);
+    v3<float> a = mk<float>((p.x - (lo.x + cx * ((float)i0 + 0.5f))) * ix, (p.y - (lo.y + cy * ((float)j0 + 0.5f))) * iy,
+                            (p.z - (lo.z + cz * ((float)k0 + 0.5f))) * iz);
+    a = mk<float>(tmin(tmax(a.x, 0.0f), 1.0f), tmin(tmax(a.y, 0.0f), 1.0f), tmin(tmax(a.z, 0.0f), 1.0f));
+    const m3<float> mxy0 = mixm<float>(mixm<float>(m000, m100, a.x), mixm<float>(m010, m110, a.x), a.y);
+    const m3<float> mxy1 = mixm<float>(mixm<float>(m001, m101, a.x), mixm<float>(m011, m111, a.x), a.y);
+    return mixm<float>(mxy0, mxy1, a.z);
+}
+
 template<typename T> GMQ m3<T> textureMetric(const Crew& c, v3<T> position)
 {
+    if constexpr(sizeof(T) == 4)
+        return textureMetricF32(c, position);
     v3<T> size = mk<T>((T)c.tsize[0], (T)c.tsize[1], (T)c.tsize[2]);
     v3<T> tminv = mk<T>((T)c.tmin[0], (T)c.tmin[1], (T)c.tmin[2]);
     v3<T> textv = mk<T>((T)c.text[0], (T)c.text[1], (T)c.text[2]);

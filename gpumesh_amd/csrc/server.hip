@@ -586,6 +586,7 @@ int gm_server_set_texture(gm_server* s, const int32_t* tsize, const double* tmin
         c.tmin[k] = tmin[k];
         c.text[k] = text[k];
     }
+    gmq::textureFields(c);
     std::lock_guard<std::mutex> lock(s->mu);
     // Jobs read the crew without the lock, so it is fixed once one is queued.
     if(s->submitted) return fail(GM_ERR_ARG, "the texture is set before the first job is submitted");
