@@ -199,7 +199,14 @@ template<typename T> GMQ m3<T> shellMetric(const Crew& c, v3<T> p)
 
 template<typename T> GMQ m3<T> texel(const Crew& c, int i, int j, int k)
 {
+#if defined(__HIP_DEVICE_COMPILE__)
+    // The grid is in device memory: global-address-space loads, not flat ones.
+    const __attribute__((address_space(1))) float* t =
+        (const __attribute__((address_space(1))) float*)(const void*)c.texels +
+        6 * (((long long)k * c.tsize[1] + j) * c.tsize[0] + i);
+#else
     const float* t = c.texels + 6 * (((long long)k * c.tsize[1] + j) * c.tsize[0] + i);
+#endif
     T m00 = (T)t[0], m01 = (T)t[1], m02 = (T)t[2], m11 = (T)t[3], m12 = (T)t[4], m22 = (T)t[5];
     return mcols<T>(mk<T>(m00, m01, m02), mk<T>(m01, m11, m12), mk<T>(m02, m12, m22));
 }
