@@ -79,15 +79,9 @@ __device__ __forceinline__ uint2 blockItem(const uint4* spans, uint32_t nspans, 
     return make_uint2(s.x, b - s.z);
 }
 
-// Positions live in device memory, but a pointer read from the mesh
-// descriptor is a generic one, for which the compiler emits flat loads (which
-// also wait on LDS traffic); the load goes through the global address space.
 template<typename T, typename P> __device__ __forceinline__ v3<T> ld(const P* pos, uint32_t v)
 {
-    typedef decltype(pos->x) E;
-    typedef E V __attribute__((ext_vector_type(4)));
-    const __attribute__((address_space(1))) V* g = (const __attribute__((address_space(1))) V*)(const void*)pos;
-    const V q = g[v];
+    P q = pos[v];
     return mk<T>((T)q.x, (T)q.y, (T)q.z);
 }
 
