@@ -266,6 +266,36 @@ def topology_leg(args):
             "initial_min_mean": [q0["min"], q0["mean"]], "passes": passes}
 
 
+def aniso_leg(args):
+    """The anisotropic hexahedral workload (64^3 hexes, jitter 0.3 h, seed 3,
+    metric conformity with the metric-wise measurer, K = 64, Nelder-Mead x 15)
+    under the analytic shell field and its 64^3 FP32 texture: FP32 with the
+    FP64 polish, exact FP64 (wave kernel), and the plain FP64 kernel (2
+    passes), wall clock around each run after a one-pass warm-up."""
+    from gpumesh_amd import GpuMesh, cube_mesh
+    from gpumesh_amd.samplers import shell_texture
+    m = cube_mesh(64, "hex", 0.3, seed=3)
+    out = {"workload": "hex cube 64^3 cells (274625 vertices), jitter 0.3h seed 3, metric conformity, K = 64, "
+                       "Nelder-Mead x 15", "runs": {}}
+    for sampler in ("analytic", "texture"):
+        tex = shell_texture(m.pos, 64, 64.0) if sampler == "texture" else None
+        for prec, kernel, passes in (("fp32_polish", "wave", 15), ("fp64", "wave", 15), ("fp64", "plain", 2)):
+            def fresh():
+                g = GpuMesh.from_cube(m)
+                g.set_crew("metric_conformity", sampler, scaling=64.0, texture=tex)
+                return g
+            fresh().smooth("nm", 1, precision=prec, kernel=kernel)
+            gm = fresh()
+            t0 = time.perf_counter()
+            r = gm.smooth("nm", passes, precision=prec, kernel=kernel)
+            wall = time.perf_counter() - t0
+            key = f"{sampler}/{prec}/{kernel}"
+            out["runs"][key] = {"passes": passes, "wall_ms": wall * 1e3, "vertices_per_s": gm.nmovable * passes / wall,
+                                "final_min": float(r["min"][-1]), "final_mean": float(r["mean"][-1])}
+            log(f"aniso {key}: {passes} passes {wall * 1e3:.1f} ms, {gm.nmovable * passes / wall:.3g} vertices/s")
+    return out
+
+
 def serving_leg(args):
     """Poisson job stream through the serving loop at several rates.  Every
     (kind, n) shape of the job list is built into the topology cache, with
@@ -411,6 +441,8 @@ def main():
     ap.add_argument("--topo-global", type=int, default=2, help="global passes of the topology leg")
     ap.add_argument("--topo-reloc", type=int, default=5, help="relocation passes per global pass")
     ap.add_argument("--no-topo", action="store_true")
+    ap.add_argument("--aniso-only", action="store_true",
+                    help="run only the anisotropic hexahedral workload's timings and print their JSON")
     ap.add_argument("--serve-only", action="store_true", help="run only the serving leg and print its JSON")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device to device 5 times (the reference kernel for HBM counters) and exit")
@@ -447,6 +479,9 @@ def main():
     check(lib().gm_set_polish_passes(args.polish_passes), "gm_set_polish_passes")
     if args.serve_only:
         print(json.dumps(serving_leg(args)))
+        return
+    if args.aniso_only:
+        print(json.dumps(aniso_leg(args)))
         return
 
     t0 = time.perf_counter()
