@@ -31,7 +31,7 @@ def _same_mesh(om, gm):
     assert np.array_equal(gm.permutation(), om.permutation())
 
 
-@pytest.mark.parametrize("n,K,seed", [(6, 6.0, 3), (8, 8.0, 11), (6, 9.0, 5), (12, 12.0, 1)])
+@pytest.mark.parametrize("n,K,seed", [(6, 6.0, 3), (8, 8.0, 11), (6, 9.0, 5), (12, 12.0, 1), (20, 20.0, 2)])
 def test_restructure_on_device_matches_oracle(oracle, n, K, seed):
     om, gm = _pair(oracle, n, K, seed)
     c_ref = om.restructure(5, 5, 0.5, 1.5)
